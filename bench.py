@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Benchmark of the SiMuse network step on one MI355X.
+
+Workload: the Izhikevich network of 1,000,000 neurons (800k regular spiking,
+200k fast spiking) with 100 random outgoing synapses each (100M synapses,
+CSR, int32 indices), Gaussian thalamic input (sd 5 RS / 2 FS), dt = 1 ms as
+two 0.5 ms half steps.  A "step" is one full network step: neuron update,
+spike compaction, delayed CSR scatter.  Inputs are resident in HBM before the
+timed region.
+
+Prints one JSON line.  `value` is neuron-steps/s of the whole job;
+synaptic events/s is reported beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(REPO, "dioma-core_amd"), os.path.join(REPO, "oracle")]
+
+import numpy as np  # noqa: E402
+
+METRIC = "synaptic events/s + neuron-steps/s (1 GPU); batched vs plain; p99 req latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def izh_algorithmic_bytes_per_neuron(spec) -> float:
+    """Bytes one Izhikevich neuron-step must move in this layout: v and u read
+    and written (16 B), the ring slot read and zeroed (8 B), and 4 B for every
+    parameter column that varies per neuron (uniform columns are scalars);
+    the refractory timer is elided when tau_ref == 0."""
+    total = 0.0
+    for p in spec.pops:
+        varying = 0
+        for k in (1, 2, 3, 4, 6):  # a, b, c, d, v_thresh
+            col = p.params[k, : p.n]
+            if not np.all(col == col[0]):
+                varying += 1
+        refr = 0 if np.all(p.params[5, : p.n] <= 0) else 8
+        total += p.n * (16 + 8 + 4 * varying + refr)
+    return total / sum(p.n for p in spec.pops)
+
+
+def cpu_baseline(spec, steps: int, threads: int):
+    import pyoracle as po
+    o = po.OracleNetwork(spec, 32, threads=threads)
+    o.step(3)  # warm-up
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    return spec.n_total * steps / dt, dt
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--neurons", type=int, default=1_000_000)
+    ap.add_argument("--synapses-per-neuron", type=int, default=100)
+    ap.add_argument("--graph-steps", type=int, default=16)
+    ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--scatter", choices=["auto", "tile", "wave"], default="auto")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(a single network fits one MI355X; independent networks shard by network)", file=sys.stderr)
+        return 2
+
+    import simuse
+    from simuse.workloads import build_device, izhikevich_network
+
+    if simuse.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        return 1
+
+    log(f"building Izhikevich network N={args.neurons} k={args.synapses_per_neuron}")
+    t0 = time.perf_counter()
+    spec = izhikevich_network(n=args.neurons, k=args.synapses_per_neuron, seed=1,
+                              steps=args.steps + args.warmup)
+    log(f"  generated in {time.perf_counter() - t0:.1f}s")
+    net = build_device(spec, graph_steps=args.graph_steps)
+    log(f"  device bytes {net.device_bytes() / 2**30:.2f} GiB")
+    if args.scatter == "auto":
+        mode = net.autotune(16)
+    else:
+        net.set_scatter_mode(simuse.SCATTER_TILE if args.scatter == "tile" else simuse.SCATTER_WAVE)
+        mode = net.scatter_mode()
+    log(f"  scatter schedule {'tile' if mode == simuse.SCATTER_TILE else 'wave'}")
+
+    net.run(args.warmup)
+    s0, e0 = net.counters()
+    net.sync()
+    t0 = time.perf_counter()
+    net.step_graph(args.steps)
+    net.sync()
+    wall = time.perf_counter() - t0
+    s1, e1 = net.counters()
+    spikes = s1 - s0
+    events = e1 - e0
+    ms_per_step = wall * 1e3 / args.steps
+    neuron_steps_per_s = spec.n_total * args.steps / wall
+    events_per_s = events / wall
+    log(f"  {ms_per_step * 1e3:.2f} us/step, {spikes / args.steps:.0f} spikes/step, "
+        f"{events / args.steps:.0f} events/step")
+
+    # Live per-kernel durations: HIP events around each kernel on the
+    # network's stream, over profile_steps further steps.
+    net.set_profiling(True)
+    ps0, pe0 = net.counters()
+    net.run(args.profile_steps)
+    k_ms = net.kernel_times()
+    ps1, pe1 = net.counters()
+    net.set_profiling(False)
+    neuron_ms, stdp_ms, insert_ms, fold_ms = k_ms
+    bpn = izh_algorithmic_bytes_per_neuron(spec)
+    neuron_bytes = bpn * spec.n_total
+    achieved = neuron_bytes / (neuron_ms * 1e-3) / 1e9
+    ev_step = (pe1 - pe0) / max(1, args.profile_steps)
+    act_step = (ps1 - ps0) / max(1, args.profile_steps)
+    scatter_bytes = ev_step * 17 + act_step * 8
+    scatter_gbs = scatter_bytes / ((insert_ms + fold_ms) * 1e-3) / 1e9 if insert_ms + fold_ms > 0 else 0.0
+    step_bytes = neuron_bytes + scatter_bytes + act_step * 8
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "traffic_izhikevich_1m.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get("neuron_kernel_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: oracle FP32 path, {args.cpu_steps} steps on {threads} threads")
+        v_all, t_all = cpu_baseline(spec, args.cpu_steps, threads)
+        v_one, t_one = cpu_baseline(spec, max(4, args.cpu_steps // 8), 1)
+        cpu = {"value": v_all, "unit": "neuron-steps/s", "cores": threads, "kind": "port",
+               "sample": f"same 1M/100M network, {args.cpu_steps} steps after 3 warm-up steps "
+                         f"({t_all:.1f}s); 1 thread: {v_one:.3e} neuron-steps/s over "
+                         f"{max(4, args.cpu_steps // 8)} steps ({t_one:.1f}s)",
+               "one_core_value": v_one}
+
+    out = {
+        "metric": METRIC,
+        "value": neuron_steps_per_s,
+        "unit": "neuron-steps/s",
+        "synaptic_events_per_s": events_per_s,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Izhikevich 2003 network, random-init parameters and weights)",
+        "config": {"workload": "izhikevich_1M_100M", "neurons": spec.n_total,
+                   "synapses": spec.n_syn, "dt_ms": 1.0, "graph_steps": args.graph_steps,
+                   "scatter": "tile" if mode == simuse.SCATTER_TILE else "wave",
+                   "spikes_per_step": spikes / args.steps, "events_per_step": events / args.steps},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "neuron_kernel<Izhikevich>",
+                     "algorithmic_bytes_per_neuron": bpn,
+                     "kernel_ms": neuron_ms},
+        "kernels_ms_per_step": {"neuron": neuron_ms, "stdp": stdp_ms, "insert": insert_ms, "fold": fold_ms},
+        "scatter_gbs_algorithmic": scatter_gbs,
+        "step_frac_of_peak": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    net.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,88 @@
+// net_kernels.hpp — device data layout shared by the network kernels and the
+// host runtime.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace simuse {
+
+constexpr uint32_t kTile = 256;          // neurons per tile = threads per block
+constexpr uint32_t kWavesPerTile = kTile / 64;
+constexpr uint32_t kSlots = 32;          // P/src/api/net/graph.rs:33
+constexpr uint32_t kEmpty = 0xFFFFFFFFu; // end of a scatter chain
+constexpr int kMaxParams = 16;
+
+enum : uint32_t {
+    kPopRefrElided = 1u,  // refractory timer provably stays 0: not loaded/stored
+    kPopNoise = 2u,       // Gaussian current noise enabled
+    kPopNoiseScalar = 4u, // every neuron has the same sigma (sigma_uni)
+};
+
+// One population as the kernels see it.  Parameter column k is either a
+// per-neuron array pr[k*dstride + i] or, when bit k of uni_mask is set, the
+// single value puni[k] (uniform columns cost no HBM traffic).  puni[0]
+// always holds dt of neuron 0, which the reference broadcasts to the whole
+// population (P/src/core/exec/population.rs:307,361).
+struct PopDev {
+    int32_t model;
+    uint32_t n;
+    uint32_t off;       // global index of neuron 0
+    uint32_t tile0;     // global index of the first tile
+    uint32_t word0;     // first 64-bit spike word
+    uint32_t dstride;   // device column stride (multiple of 64)
+    uint32_t uni_mask;
+    uint32_t flags;
+    float* st;          // [ns][dstride]
+    const float* pr;    // [np][dstride]
+    const float* sigma; // [n] noise amplitude
+    float sigma_uni;    // noise amplitude when kPopNoiseScalar
+    uint64_t seed;
+    float puni[kMaxParams];
+};
+
+// Everything one step of one network touches.
+struct NetArgs {
+    const PopDev* pops;
+    const uint32_t* tile_pop;   // tile -> population
+    uint32_t N;                 // neurons
+    uint32_t n_tiles;           // neuron tiles
+    uint32_t n_src_tiles;       // Poisson-source tiles (after the neuron tiles)
+    uint32_t n_src;
+    uint64_t n_syn;
+    float* ring;                // [32][N]
+    uint32_t* head;             // [32][N] chain heads, kEmpty when idle
+    uint32_t* next;             // [n_syn + n_src_edges] chain links
+    const uint32_t* row_ptr;    // [N+1]
+    const uint32_t* col;        // [n_syn]
+    const float* w;             // [n_syn]
+    const uint8_t* delay;       // [n_syn]
+    const uint32_t* src_row_ptr;
+    const uint32_t* src_post;
+    const float* src_w;
+    const uint8_t* src_delay;
+    const float* src_p;
+    const uint32_t* src_seed;
+    uint32_t* tile_count;       // [n_tiles] spikes of the last step per tile
+    uint32_t* active;           // [n_tiles*kTile] ascending ids per tile
+    uint64_t* bits;             // spike bit words of the last step
+    uint64_t* tile_spikes;      // [n_tiles] running totals
+    uint64_t* tile_events;      // [n_tiles + n_src_tiles] running totals
+    uint64_t* step;             // device step counter
+    uint32_t* done;             // block completion counter of the fold kernel
+    uint64_t* raster;           // optional [raster_cap][n_words]
+    uint32_t raster_cap;
+    uint32_t n_words;
+    uint64_t* raster_base;      // step at which recording started
+    // pair-based STDP
+    int32_t stdp;
+    double* w64;                // FP64 master weights
+    float* tr_pre;
+    float* tr_post;
+    const uint32_t* in_ptr;     // [N+1]
+    const uint32_t* in_edge;    // [n_syn] in-edges per post, ascending edge id
+    const uint32_t* edge_pre;   // [n_syn]
+    const uint8_t* plastic;     // [n_syn] or null (all)
+    float stdp_dpre, stdp_dpost, a_plus, a_minus, w_min, w_max;
+};
+
+}  // namespace simuse

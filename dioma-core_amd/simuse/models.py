@@ -1,0 +1,58 @@
+"""Population builders with the reference's SoA column order.
+
+Columns follow the ``#[state]``/``#[param]`` field order of each model:
+  LIF        P/src/zoo/neurons/lif.rs:22-40
+             state [v, refr]; params [dt, inv_tau_m, v_rest, v_thresh, v_reset, tau_ref]
+  Izhikevich P/src/zoo/neurons/izhikevich.rs:42-64
+             state [v, u, refr]; params [dt, a, b, c, d, tau_ref, v_thresh]
+  HH         field order of P/src/zoo/neurons/hh.rs:12-32 (the CPU stub) with
+             the equations of PC/kernels/hh.cu:107-216
+             state [v, m, h, n, refr, c]; params [dt, g_na, g_k, g_leak, e_na, e_k,
+             e_leak, inv_cm, q10, v_thresh, g_ca_t, e_ca_t, tau_ref]
+(crates/SiMuse/crates/ is the prefix of every path above.)
+"""
+from __future__ import annotations
+
+from typing import Sequence, Union
+
+import numpy as np
+
+from ._lib import MODEL_HH, MODEL_IZHIKEVICH, MODEL_LIF
+from .network import N_PARAMS, N_STATE, NetworkPop, stride_for
+
+Num = Union[float, np.ndarray]
+
+
+def _cols(n: int, values: Sequence[Num]) -> np.ndarray:
+    s = stride_for(n)
+    out = np.zeros((len(values), s), dtype=np.float32)
+    for k, v in enumerate(values):
+        out[k, :n] = np.asarray(v, dtype=np.float32) if np.ndim(v) else np.float32(v)
+    return out
+
+
+def make_pop(model: int, n: int, params: Sequence[Num], states: Sequence[Num]) -> NetworkPop:
+    if len(params) != N_PARAMS[model] or len(states) != N_STATE[model]:
+        raise ValueError("wrong number of columns for the model")
+    return NetworkPop(model, n, _cols(n, states), _cols(n, params))
+
+
+def lif_pop(n: int, dt: Num, inv_tau_m: Num, v_rest: Num, v_thresh: Num, v_reset: Num, tau_ref: Num,
+            v0: Num, refr0: Num = 0.0) -> NetworkPop:
+    return make_pop(MODEL_LIF, n, [dt, inv_tau_m, v_rest, v_thresh, v_reset, tau_ref], [v0, refr0])
+
+
+def izh_pop(n: int, dt: Num, a: Num, b: Num, c: Num, d: Num, tau_ref: Num, v_thresh: Num, v0: Num,
+            u0: Num, refr0: Num = 0.0) -> NetworkPop:
+    return make_pop(MODEL_IZHIKEVICH, n, [dt, a, b, c, d, tau_ref, v_thresh], [v0, u0, refr0])
+
+
+def hh_pop(n: int, dt: Num, g_na: Num = 120.0, g_k: Num = 36.0, g_leak: Num = 0.3, e_na: Num = 50.0,
+           e_k: Num = -77.0, e_leak: Num = -65.0, inv_cm: Num = 1.0, q10: Num = 1.0, v_thresh: Num = 0.0,
+           g_ca_t: Num = 0.0, e_ca_t: Num = 120.0, tau_ref: Num = 2.0, v0: Num = -65.0, m0: Num = 0.0529,
+           h0: Num = 0.596, n0: Num = 0.318, c0: Num = 0.0) -> NetworkPop:
+    """Standard HH (parameters and resting gates of
+    crates/SiMuse/crates/pillars-connectome/src/sim/hh.rs:21-38)."""
+    return make_pop(MODEL_HH, n,
+                    [dt, g_na, g_k, g_leak, e_na, e_k, e_leak, inv_cm, q10, v_thresh, g_ca_t, e_ca_t, tau_ref],
+                    [v0, m0, h0, n0, 0.0, c0])

@@ -1,0 +1,280 @@
+"""Host-side mirror of the reference's GPU network API.
+
+``CompiledNetwork`` has the method names and argument meaning of
+``pillars_cuda::CompiledNetwork`` (crates/SiMuse/crates/pillars-cuda/src/
+network.rs:182-882): ``new`` (the constructor), ``step``, ``step_graph``,
+``sync``, ``download_spikes``, ``download_states``, ``upload_weights``,
+``download_weights``, ``inject_current``, ``step_counter``, ``reset``,
+``n_total``, ``n_pops``, ``pop_n``.  Where the reference returns
+``Err(String)`` this raises :class:`SimuseError` with the library's message.
+Steps follow the CPU canon ``pillars::api::Network::step``
+(crates/SiMuse/crates/pillars/src/api/net/network/step.rs:18-189).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import PopDesc, StdpParams, check, lib
+
+N_STATE = {_lib.MODEL_LIF: 2, _lib.MODEL_IZHIKEVICH: 3, _lib.MODEL_HH: 6}
+N_PARAMS = {_lib.MODEL_LIF: 6, _lib.MODEL_IZHIKEVICH: 7, _lib.MODEL_HH: 13}
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def stride_for(n: int) -> int:
+    """Column stride of the reference's SoA layout: ceil8(n)."""
+    return (n + 7) // 8 * 8
+
+
+@dataclass
+class NetworkPop:
+    """One population (mirrors ``NetworkPop``, PC/src/network.rs:142-156).
+
+    ``states``/``params`` are column-major SoA arrays of shape
+    ``(ns, stride)`` / ``(np, stride)``.
+    """
+
+    model: int
+    n: int
+    states: np.ndarray
+    params: np.ndarray
+
+    @property
+    def ns(self) -> int:
+        return N_STATE[self.model]
+
+    @property
+    def np_(self) -> int:
+        return N_PARAMS[self.model]
+
+    @property
+    def stride(self) -> int:
+        return int(self.states.shape[1])
+
+    def desc(self) -> PopDesc:
+        st = np.ascontiguousarray(self.states, dtype=np.float32)
+        pr = np.ascontiguousarray(self.params, dtype=np.float32)
+        if st.shape != (self.ns, self.stride) or pr.shape != (self.np_, self.stride):
+            raise ValueError("state/param arrays must be (ns, stride) and (np, stride)")
+        self._keep = (st, pr)
+        return PopDesc(self.model, self.n, self.ns, self.np_, self.stride, _ptr(st, C.c_float), _ptr(pr, C.c_float))
+
+
+def _descs(pops: Sequence[NetworkPop]):
+    arr = (PopDesc * len(pops))()
+    for i, p in enumerate(pops):
+        arr[i] = p.desc()
+    return arr
+
+
+class CompiledNetwork:
+    """A spiking network resident on one MI355X (one HIP stream per handle)."""
+
+    def __init__(self, pops: Sequence[NetworkPop], row_ptr, col_idx, weights, delays, dt: float = 0.0):
+        self._h = None
+        self.pops = list(pops)
+        rp = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+        ci = np.ascontiguousarray(col_idx, dtype=np.uint32)
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        d = np.ascontiguousarray(delays, dtype=np.uint8)
+        if not (len(ci) == len(w) == len(d)):
+            raise ValueError("col_idx/weights/delays length mismatch")
+        self.n_syn = int(len(ci))
+        descs = _descs(self.pops)
+        h = C.c_void_p()
+        check(lib().simuse_net_create(descs, len(self.pops), _ptr(rp, C.c_uint32), _ptr(ci, C.c_uint32),
+                                      _ptr(w, C.c_float), _ptr(d, C.c_uint8), self.n_syn, float(dt), C.byref(h)))
+        self._h = h
+        self._n_total = sum(p.n for p in self.pops)
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self) -> None:
+        if self._h is not None:
+            lib().simuse_net_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- configuration (before the first step) --------------------------------
+    def add_poisson_source(self, rate_hz: float, dt_ms: float, seed: int, posts, weights, delays) -> None:
+        p = np.ascontiguousarray(posts, dtype=np.uint32)
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        d = np.ascontiguousarray(delays, dtype=np.uint8)
+        check(lib().simuse_net_add_poisson_source(self._h, float(rate_hz), float(dt_ms), int(seed) & 0xFFFFFFFF,
+                                                  _ptr(p, C.c_uint32), _ptr(w, C.c_float), _ptr(d, C.c_uint8), len(p)))
+
+    def set_noise(self, pop: int, sigma, seed: int) -> None:
+        s = np.ascontiguousarray(sigma, dtype=np.float32)
+        if len(s) != self.pops[pop].n:
+            raise ValueError("sigma must have one entry per neuron")
+        check(lib().simuse_net_set_noise(self._h, pop, _ptr(s, C.c_float), int(seed) & ((1 << 64) - 1)))
+
+    def set_plasticity(self, a_plus, a_minus, tau_plus, tau_minus, w_min, w_max, plastic=None) -> None:
+        sp = StdpParams(a_plus, a_minus, tau_plus, tau_minus, w_min, w_max)
+        if plastic is None:
+            check(lib().simuse_net_set_plasticity(self._h, C.byref(sp), None))
+        else:
+            m = np.ascontiguousarray(plastic, dtype=np.uint8)
+            check(lib().simuse_net_set_plasticity(self._h, C.byref(sp), _ptr(m, C.c_uint8)))
+
+    def set_scatter_mode(self, mode: int) -> None:
+        check(lib().simuse_net_set_scatter_mode(self._h, int(mode)))
+
+    def scatter_mode(self) -> int:
+        m = C.c_int32()
+        check(lib().simuse_net_get_scatter_mode(self._h, C.byref(m)))
+        return int(m.value)
+
+    def autotune(self, trial_steps: int = 16) -> int:
+        check(lib().simuse_net_autotune(self._h, int(trial_steps)))
+        return self.scatter_mode()
+
+    def set_graph_steps(self, steps: int) -> None:
+        check(lib().simuse_net_set_graph_steps(self._h, int(steps)))
+
+    # -- stepping ---------------------------------------------------------------
+    def step(self) -> None:
+        """One step, synchronous (CompiledNetwork::step, network.rs:411)."""
+        check(lib().simuse_net_step(self._h))
+
+    def step_graph(self, n_steps: int = 1) -> None:
+        """Enqueue n steps of graph replay and return (network.rs:431)."""
+        check(lib().simuse_net_step_async(self._h, int(n_steps)))
+
+    def run(self, n_steps: int) -> None:
+        self.step_graph(n_steps)
+        self.sync()
+
+    def sync(self) -> None:
+        check(lib().simuse_net_sync(self._h))
+
+    # -- transfers ----------------------------------------------------------------
+    def download_spikes(self) -> np.ndarray:
+        out = np.zeros(self._n_total, dtype=np.int32)
+        check(lib().simuse_net_download_spikes(self._h, _ptr(out, C.c_int32), out.size))
+        return out
+
+    def download_spike_ids(self) -> np.ndarray:
+        cnt = C.c_uint32()
+        check(lib().simuse_net_download_spike_ids(self._h, None, 0, C.byref(cnt)))
+        out = np.zeros(max(1, cnt.value), dtype=np.uint32)
+        check(lib().simuse_net_download_spike_ids(self._h, _ptr(out, C.c_uint32), out.size, C.byref(cnt)))
+        return out[: cnt.value]
+
+    def download_states(self, pop: int) -> np.ndarray:
+        p = self.pops[pop]
+        out = np.zeros((p.ns, p.stride), dtype=np.float32)
+        check(lib().simuse_net_download_state(self._h, pop, _ptr(out, C.c_float), out.size))
+        return out
+
+    def download_ring(self) -> np.ndarray:
+        out = np.zeros((32, self._n_total), dtype=np.float32)
+        check(lib().simuse_net_download_ring(self._h, _ptr(out, C.c_float), out.size))
+        return out
+
+    def upload_weights(self, weights) -> None:
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        check(lib().simuse_net_upload_weights(self._h, _ptr(w, C.c_float), w.size))
+
+    def download_weights(self) -> np.ndarray:
+        out = np.zeros(self.n_syn, dtype=np.float32)
+        check(lib().simuse_net_download_weights(self._h, _ptr(out, C.c_float), out.size))
+        return out
+
+    def inject_current(self, pop: int, indices, values) -> None:
+        i = np.ascontiguousarray(indices, dtype=np.uint32)
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        if i.size != v.size:
+            raise ValueError("inject_current: indices/values length mismatch")
+        check(lib().simuse_net_inject_current(self._h, pop, _ptr(i, C.c_uint32), _ptr(v, C.c_float), i.size))
+
+    def reset(self, pops: Sequence[NetworkPop]) -> None:
+        self.pops = list(pops)
+        descs = _descs(self.pops)
+        check(lib().simuse_net_reset(self._h, descs, len(self.pops)))
+
+    # -- introspection -------------------------------------------------------------
+    def step_counter(self) -> int:
+        s = C.c_uint64()
+        check(lib().simuse_net_step_counter(self._h, C.byref(s)))
+        return int(s.value)
+
+    def n_total(self) -> int:
+        return self._n_total
+
+    def n_pops(self) -> int:
+        return len(self.pops)
+
+    def pop_n(self, pop: int) -> int:
+        return self.pops[pop].n if 0 <= pop < len(self.pops) else 0
+
+    def counters(self):
+        s, e = C.c_uint64(), C.c_uint64()
+        check(lib().simuse_net_counters(self._h, C.byref(s), C.byref(e)))
+        return int(s.value), int(e.value)
+
+    def device_bytes(self) -> int:
+        b = C.c_uint64()
+        check(lib().simuse_net_device_bytes(self._h, C.byref(b)))
+        return int(b.value)
+
+    # -- recording and statistics ----------------------------------------------
+    def record(self, max_steps: int) -> None:
+        check(lib().simuse_net_record(self._h, int(max_steps)))
+
+    def download_raster(self) -> np.ndarray:
+        """Dense int32 raster [steps, n_total] from the device bit raster."""
+        words = C.c_uint64()
+        check(lib().simuse_net_raster_words(self._h, C.byref(words)))
+        steps = C.c_uint32()
+        check(lib().simuse_net_download_raster(self._h, None, 0, C.byref(steps)))
+        buf = np.zeros(max(1, steps.value * words.value), dtype=np.uint64)
+        check(lib().simuse_net_download_raster(self._h, _ptr(buf, C.c_uint64), buf.size, C.byref(steps)))
+        T, W = int(steps.value), int(words.value)
+        bits = buf[: T * W].reshape(T, W)
+        out = np.zeros((T, self._n_total), dtype=np.int32)
+        word0 = 0
+        off = 0
+        for p in self.pops:
+            nt = (p.n + 255) // 256
+            wp = bits[:, word0: word0 + nt * 4]
+            un = np.unpackbits(wp.view(np.uint8).reshape(T, -1), axis=1, bitorder="little")
+            out[:, off: off + p.n] = un[:, : p.n]
+            word0 += nt * 4
+            off += p.n
+        return out
+
+    def raster_stats(self, first: int, count: int, dt_s: float):
+        rates = np.zeros(count, dtype=np.float32)
+        cvs = np.zeros(count, dtype=np.float32)
+        check(lib().simuse_net_raster_stats(self._h, first, count, float(dt_s), _ptr(rates, C.c_float),
+                                            _ptr(cvs, C.c_float)))
+        return rates, cvs
+
+    # -- profiling -------------------------------------------------------------------
+    def set_profiling(self, on: bool) -> None:
+        check(lib().simuse_net_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_times(self):
+        """Mean ms per step of [neuron, stdp, insert, fold] over profiled steps."""
+        out = (C.c_float * 4)()
+        check(lib().simuse_net_kernel_times(self._h, out, 4))
+        return [float(x) for x in out]

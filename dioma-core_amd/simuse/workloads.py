@@ -1,0 +1,218 @@
+"""Seeded synthetic networks for the benchmark configurations and tests.
+
+A :class:`NetSpec` holds everything needed to build the same network on the
+device and in a CPU checker: populations, the CSR, Poisson sources and
+Gaussian current noise.  All randomness comes from numpy's seeded PCG64, so a
+spec is reproducible from its seed.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+from .models import hh_pop, izh_pop, lif_pop
+from .network import CompiledNetwork, NetworkPop
+
+
+@dataclass
+class Source:
+    rate_hz: float
+    dt_ms: float
+    seed: int
+    posts: np.ndarray
+    weights: np.ndarray
+    delays: np.ndarray
+
+
+@dataclass
+class NetSpec:
+    name: str
+    pops: List[NetworkPop]
+    row_ptr: np.ndarray
+    col: np.ndarray
+    w: np.ndarray
+    delay: np.ndarray
+    dt: float
+    steps: int
+    sources: List[Source] = field(default_factory=list)
+    noise: List[Tuple[int, np.ndarray, int]] = field(default_factory=list)
+    stdp: Optional[dict] = None
+
+    @property
+    def n_total(self) -> int:
+        return int(sum(p.n for p in self.pops))
+
+    @property
+    def n_syn(self) -> int:
+        return int(self.col.size)
+
+
+def csr_from_triplets(n: int, pre, post, w, d):
+    """CSR from (pre, post, w, delay) with a *stable* sort by pre.
+
+    The reference sorts with ``sort_unstable_by_key`` (P/src/api/net/graph.rs:147,
+    crates/SiMuse/crates/pillars/ prefix), which leaves the order of a row's
+    edges unspecified; a stable sort keeps insertion order, and both the device
+    and the checker then consume the same CSR arrays.
+    """
+    pre = np.asarray(pre, dtype=np.uint32)
+    order = np.argsort(pre, kind="stable")
+    counts = np.bincount(pre, minlength=n).astype(np.uint64)
+    rp = np.zeros(n + 1, dtype=np.uint64)
+    np.cumsum(counts, out=rp[1:])
+    return (rp.astype(np.uint32), np.asarray(post, dtype=np.uint32)[order],
+            np.asarray(w, dtype=np.float32)[order], np.asarray(d, dtype=np.uint8)[order])
+
+
+def _fixed_outdegree_targets(rng: np.random.Generator, n: int, k: int) -> np.ndarray:
+    """k distinct targets per neuron, uniform, no autapses; rows sorted."""
+    tgt = rng.integers(0, n - 1, size=(n, k), dtype=np.int64)
+    pre = np.arange(n, dtype=np.int64)[:, None]
+    tgt += tgt >= pre
+    tgt.sort(axis=1)
+    for _ in range(64):
+        dup = np.zeros(tgt.shape, dtype=bool)
+        dup[:, 1:] = tgt[:, 1:] == tgt[:, :-1]
+        rows = np.nonzero(dup.any(axis=1))[0]
+        if rows.size == 0:
+            break
+        sub = tgt[rows]
+        dsub = dup[rows]
+        new = rng.integers(0, n - 1, size=int(dsub.sum()), dtype=np.int64)
+        rr = np.broadcast_to(rows[:, None], dsub.shape)[dsub]
+        new += new >= rr
+        sub[dsub] = new
+        sub.sort(axis=1)
+        tgt[rows] = sub
+    return tgt.astype(np.uint32)
+
+
+def izhikevich_network(n: int = 1_000_000, k: int = 100, seed: int = 1, steps: int = 1000,
+                       noise: bool = True) -> NetSpec:
+    """Izhikevich (2003) network: 80% regular spiking, 20% fast spiking.
+
+    Two populations (RS then FS).  RS: a=0.02, b=0.2, c=-65+15r^2,
+    d=8-6r^2; FS: a=0.02+0.08r, b=0.25-0.05r, c=-65, d=2; r ~ U(0,1).  k
+    random distinct targets per neuron, weights exc U(0,0.5), inh U(-1,0),
+    delay 0 (next step).  dt = 1 ms integrated as two 0.5 ms half steps by the
+    model; no refractory period; threshold 30; v0=-65, u0=b*v0.  Thalamic
+    input: Gaussian current, sd 5 (RS) and 2 (FS) per step.
+    """
+    rng = np.random.default_rng(seed)
+    ne = n * 4 // 5
+    ni = n - ne
+    re = rng.random(ne, dtype=np.float32)
+    ri = rng.random(ni, dtype=np.float32)
+    v0e = np.full(ne, -65.0, np.float32)
+    v0i = np.full(ni, -65.0, np.float32)
+    ce = (-65.0 + 15.0 * re * re).astype(np.float32)
+    de = (8.0 - 6.0 * re * re).astype(np.float32)
+    ai = (0.02 + 0.08 * ri).astype(np.float32)
+    bi = (0.25 - 0.05 * ri).astype(np.float32)
+    rs = izh_pop(ne, 1.0, 0.02, 0.2, ce, de, 0.0, 30.0, v0e, np.float32(0.2) * v0e)
+    fs = izh_pop(ni, 1.0, ai, bi, -65.0, 2.0, 0.0, 30.0, v0i, (bi * v0i).astype(np.float32))
+    tgt = _fixed_outdegree_targets(rng, n, k)
+    w = np.empty((n, k), dtype=np.float32)
+    w[:ne] = rng.random((ne, k), dtype=np.float32) * np.float32(0.5)
+    w[ne:] = -rng.random((ni, k), dtype=np.float32)
+    rp = (np.arange(n + 1, dtype=np.uint64) * k).astype(np.uint32)
+    spec = NetSpec("izhikevich", [rs, fs], rp, tgt.reshape(-1), w.reshape(-1),
+                   np.zeros(n * k, dtype=np.uint8), 1.0, steps)
+    if noise:
+        spec.noise.append((0, np.full(ne, 5.0, np.float32), seed * 7919 + 17))
+        spec.noise.append((1, np.full(ni, 2.0, np.float32), seed * 7919 + 18))
+    return spec
+
+
+def brunel_lif_network(n: int = 10_000, p: float = 0.01, seed: int = 1, steps: int = 10_000,
+                       rate_hz: float = 2000.0) -> NetSpec:
+    """Brunel-style current-based LIF network (80% exc / 20% inh).
+
+    Erdős–Rényi connectivity with probability p, no autapses; J_E=+0.1 mV and
+    J_I=-0.5 mV expressed as the current the forward-Euler LIF integrates for
+    one step (i = J*tau_m/dt: +20 and -100 at tau_m=20 ms, dt=0.1 ms); delays
+    uniform 1..15 steps; tau_m=20 ms, V_rest=V_reset=-65, V_th=-50,
+    refractory 2 ms; one independent Poisson source per neuron at rate_hz
+    with weight J_E; v0 ~ U(-65,-50).
+    """
+    rng = np.random.default_rng(seed)
+    dt = 0.1
+    tau_m = 20.0
+    ne = n * 4 // 5
+    # Bernoulli(p) per ordered pair, drawn row by row
+    pres, posts = [], []
+    for i in range(n):
+        m = rng.random(n) < p
+        m[i] = False
+        j = np.nonzero(m)[0]
+        pres.append(np.full(j.size, i, np.uint32))
+        posts.append(j.astype(np.uint32))
+    pre = np.concatenate(pres)
+    post = np.concatenate(posts)
+    je = np.float32(0.1 * tau_m / dt)
+    ji = np.float32(-0.5 * tau_m / dt)
+    w = np.where(pre < ne, je, ji).astype(np.float32)
+    d = rng.integers(1, 16, size=pre.size).astype(np.uint8)
+    rp, col, w, d = csr_from_triplets(n, pre, post, w, d)
+    v0 = rng.uniform(-65.0, -50.0, size=n).astype(np.float32)
+    pop = lif_pop(n, dt, 1.0 / tau_m, -65.0, -50.0, -65.0, 2.0, v0)
+    spec = NetSpec("brunel_lif", [pop], rp, col, w, d, dt, steps)
+    seeds = (np.arange(n, dtype=np.uint64) * np.uint64(2654435761) + np.uint64(seed)) & np.uint64(0xFFFFFFFF)
+    for i in range(n):
+        spec.sources.append(Source(rate_hz, dt, int(seeds[i]), np.array([i], np.uint32),
+                                   np.array([je], np.float32), np.array([0], np.uint8)))
+    return spec
+
+
+def hh_network(n: int = 200_000, seed: int = 1, steps: int = 50_000, mean_deg: float = 200.0,
+               w_scale: float = 1.0, noise_sd: float = 3.0, stdp: bool = True) -> NetSpec:
+    """Hodgkin–Huxley network with heavy-tailed out-degree and pair STDP.
+
+    Out-degree ceil(x_m * U^(-1/2)) (Pareto alpha=2, x_m = mean/2, mean 200),
+    capped at n-1; excitatory weights lognormal(mu=-1, sigma=1) scaled by
+    w_scale (current pulses delivered through the ring), 20% inhibitory
+    neurons with weight -lognormal; dt = 0.01 ms; Gaussian current noise
+    keeps the network active.  STDP (A+=0.01, A-=0.012,
+    tau=20 ms, clip [0,1]) on excitatory synapses.
+    """
+    rng = np.random.default_rng(seed)
+    dt = 0.01
+    xm = mean_deg / 2.0
+    u = rng.random(n)
+    deg = np.minimum(np.ceil(xm * u ** -0.5), n - 1).astype(np.int64)
+    rp = np.zeros(n + 1, dtype=np.uint64)
+    np.cumsum(deg, out=rp[1:])
+    ns = int(rp[-1])
+    pre = np.repeat(np.arange(n, dtype=np.int64), deg)
+    post = rng.integers(0, n - 1, size=ns, dtype=np.int64)
+    post += post >= pre
+    ne = n * 4 // 5
+    w = rng.lognormal(-1.0, 1.0, size=ns).astype(np.float32) * np.float32(w_scale)
+    exc = pre < ne
+    w = np.where(exc, np.minimum(w, 1.0), -w).astype(np.float32)
+    pop = hh_pop(n, dt, tau_ref=2.0, v0=rng.uniform(-70.0, -60.0, size=n).astype(np.float32))
+    spec = NetSpec("hh", [pop], rp.astype(np.uint32), post.astype(np.uint32), w,
+                   np.ones(ns, dtype=np.uint8), dt, steps)
+    if noise_sd > 0:
+        spec.noise.append((0, np.full(n, noise_sd, np.float32), seed * 104729 + 3))
+    if stdp:
+        spec.stdp = dict(a_plus=0.01, a_minus=0.012, tau_plus=20.0, tau_minus=20.0, w_min=0.0, w_max=1.0,
+                         plastic=exc.astype(np.uint8))
+    return spec
+
+
+def build_device(spec: NetSpec, graph_steps: int = 8) -> CompiledNetwork:
+    """Instantiate a spec on the device."""
+    net = CompiledNetwork(spec.pops, spec.row_ptr, spec.col, spec.w, spec.delay, spec.dt)
+    for s in spec.sources:
+        net.add_poisson_source(s.rate_hz, s.dt_ms, s.seed, s.posts, s.weights, s.delays)
+    for pop, sigma, seed in spec.noise:
+        net.set_noise(pop, sigma, seed)
+    if spec.stdp:
+        st = dict(spec.stdp)
+        plastic = st.pop("plastic", None)
+        net.set_plasticity(plastic=plastic, **st)
+    net.set_graph_steps(graph_steps)
+    return net

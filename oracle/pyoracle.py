@@ -1,0 +1,271 @@
+"""ctypes front end of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker and the timed CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class OrcPop(C.Structure):
+    _fields_ = [("model", C.c_int32), ("n", C.c_uint32), ("ns", C.c_uint32), ("np", C.c_uint32),
+                ("stride", C.c_uint32), ("states", C.POINTER(C.c_float)), ("params", C.POINTER(C.c_float))]
+
+
+class OrcStdp(C.Structure):
+    _fields_ = [("a_plus", C.c_float), ("a_minus", C.c_float), ("tau_plus", C.c_float),
+                ("tau_minus", C.c_float), ("w_min", C.c_float), ("w_max", C.c_float)]
+
+
+_lib = None
+P = C.c_void_p
+u32p, u64p, i32p = C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_int32)
+f32p, f64p, u8p = C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_uint8)
+
+_SIGS = {
+    "orc_net_create": ([C.c_int, C.POINTER(OrcPop), C.c_int32, u32p, u32p, f32p, u8p, C.c_uint64], P),
+    "orc_net_destroy": ([P], None),
+    "orc_net_add_poisson_source": ([P, C.c_float, C.c_float, C.c_uint32, u32p, f32p, u8p, C.c_uint32], C.c_int),
+    "orc_net_set_noise": ([P, C.c_int32, f32p, C.c_uint64], C.c_int),
+    "orc_net_set_plasticity": ([P, C.POINTER(OrcStdp), u8p], C.c_int),
+    "orc_net_inject_current": ([P, C.c_int32, u32p, f32p, C.c_uint32], C.c_int),
+    "orc_net_step": ([P, C.c_uint32], None),
+    "orc_net_step_counter": ([P], C.c_uint64),
+    "orc_net_spikes": ([P, i32p], None),
+    "orc_net_active": ([P, u32p], C.c_uint32),
+    "orc_net_state": ([P, C.c_int32, C.c_int32, f64p], C.c_int),
+    "orc_net_ring": ([P, f64p], None),
+    "orc_net_weights": ([P, f64p], None),
+    "orc_net_total_spikes": ([P], C.c_uint64),
+    "orc_net_total_events": ([P], C.c_uint64),
+    "orc_net_record": ([P, C.c_int32], None),
+    "orc_net_raster_steps": ([P], C.c_uint64),
+    "orc_net_raster": ([P, i32p], None),
+    "orc_net_set_threads": ([P, C.c_int32], None),
+    "orc_poisson_spike": ([C.c_uint32, C.c_uint32, C.c_uint64, C.c_float], C.c_int32),
+    "orc_gauss": ([C.c_uint64, C.c_uint32, C.c_uint64], C.c_float),
+    "orc_exp_f32": ([C.c_float], C.c_float),
+    "orc_ln_f32": ([C.c_float], C.c_float),
+    "orc_vtrap_f32": ([C.c_float, C.c_float], C.c_float),
+    "orc_vtrap_f64": ([C.c_double, C.c_double], C.c_double),
+    "orc_compact": ([i32p, C.c_uint32, u32p], C.c_uint32),
+    "orc_kura_create": ([C.c_int, C.c_uint32, u32p, u32p, f32p, f32p, f32p, C.c_float], P),
+    "orc_kura_destroy": ([P], None),
+    "orc_kura_step_rk4": ([P, C.c_uint32], None),
+    "orc_kura_step_euler": ([P, C.c_uint32], None),
+    "orc_kura_phases": ([P, f64p], None),
+    "orc_kura_order": ([P], C.c_double),
+    "orc_firing_rates": ([i32p, C.c_uint32, C.c_uint32, C.c_float, f32p], None),
+    "orc_isi_cv": ([i32p, C.c_uint32, C.c_uint32, C.c_float, f32p], None),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class OracleNetwork:
+    """CPU restatement of Network::step built from a NetSpec (kind 32 or 64)."""
+
+    def __init__(self, spec, kind: int = 32, threads: int = 1):
+        L = lib()
+        self.spec = spec
+        self.kind = kind
+        arr = (OrcPop * len(spec.pops))()
+        keep = []
+        for i, p in enumerate(spec.pops):
+            st = np.ascontiguousarray(p.states, dtype=np.float32)
+            pr = np.ascontiguousarray(p.params, dtype=np.float32)
+            keep += [st, pr]
+            arr[i] = OrcPop(p.model, p.n, st.shape[0], pr.shape[0], st.shape[1], _p(st, C.c_float),
+                            _p(pr, C.c_float))
+        rp = np.ascontiguousarray(spec.row_ptr, np.uint32)
+        col = np.ascontiguousarray(spec.col, np.uint32)
+        w = np.ascontiguousarray(spec.w, np.float32)
+        d = np.ascontiguousarray(spec.delay, np.uint8)
+        self.h = L.orc_net_create(kind, arr, len(spec.pops), _p(rp, C.c_uint32), _p(col, C.c_uint32),
+                                  _p(w, C.c_float), _p(d, C.c_uint8), col.size)
+        if not self.h:
+            raise ValueError("oracle rejected the network")
+        self.N = spec.n_total
+        for s in spec.sources:
+            pp = np.ascontiguousarray(s.posts, np.uint32)
+            ww = np.ascontiguousarray(s.weights, np.float32)
+            dd = np.ascontiguousarray(s.delays, np.uint8)
+            rc = L.orc_net_add_poisson_source(self.h, s.rate_hz, s.dt_ms, s.seed & 0xFFFFFFFF, _p(pp, C.c_uint32),
+                                              _p(ww, C.c_float), _p(dd, C.c_uint8), pp.size)
+            assert rc == 0
+        for pop, sigma, seed in spec.noise:
+            sg = np.ascontiguousarray(sigma, np.float32)
+            assert L.orc_net_set_noise(self.h, pop, _p(sg, C.c_float), seed) == 0
+        if spec.stdp:
+            st = dict(spec.stdp)
+            plastic = st.pop("plastic", None)
+            if plastic is None:
+                plastic = np.ones(col.size, np.uint8)
+            plastic = np.ascontiguousarray(plastic, np.uint8)
+            sp = OrcStdp(st["a_plus"], st["a_minus"], st["tau_plus"], st["tau_minus"], st["w_min"], st["w_max"])
+            assert L.orc_net_set_plasticity(self.h, C.byref(sp), _p(plastic, C.c_uint8)) == 0
+        if threads > 1:
+            L.orc_net_set_threads(self.h, threads)
+
+    def close(self):
+        if self.h:
+            lib().orc_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n: int = 1):
+        lib().orc_net_step(self.h, n)
+
+    def inject_current(self, pop, idx, val):
+        i = np.ascontiguousarray(idx, np.uint32)
+        v = np.ascontiguousarray(val, np.float32)
+        assert lib().orc_net_inject_current(self.h, pop, _p(i, C.c_uint32), _p(v, C.c_float), i.size) == 0
+
+    def step_counter(self):
+        return int(lib().orc_net_step_counter(self.h))
+
+    def spikes(self):
+        out = np.zeros(self.N, np.int32)
+        lib().orc_net_spikes(self.h, _p(out, C.c_int32))
+        return out
+
+    def active(self):
+        out = np.zeros(self.N + 1, np.uint32)
+        k = lib().orc_net_active(self.h, _p(out, C.c_uint32))
+        return out[:k].copy()
+
+    def state(self, pop, col):
+        out = np.zeros(self.spec.pops[pop].n, np.float64)
+        assert lib().orc_net_state(self.h, pop, col, _p(out, C.c_double)) == 0
+        return out
+
+    def ring(self):
+        out = np.zeros((32, self.N), np.float64)
+        lib().orc_net_ring(self.h, _p(out, C.c_double))
+        return out
+
+    def weights(self):
+        out = np.zeros(self.spec.n_syn, np.float64)
+        lib().orc_net_weights(self.h, _p(out, C.c_double))
+        return out
+
+    def counters(self):
+        return int(lib().orc_net_total_spikes(self.h)), int(lib().orc_net_total_events(self.h))
+
+    def record(self, on=True):
+        lib().orc_net_record(self.h, 1 if on else 0)
+
+    def raster(self):
+        t = int(lib().orc_net_raster_steps(self.h))
+        out = np.zeros((t, self.N), np.int32)
+        if t:
+            lib().orc_net_raster(self.h, _p(out, C.c_int32))
+        return out
+
+
+class OracleKuramoto:
+    def __init__(self, row_ptr, col, w, omega, theta0, dt, kind=64):
+        rp = np.ascontiguousarray(row_ptr, np.uint32)
+        cc = np.ascontiguousarray(col, np.uint32)
+        ww = np.ascontiguousarray(w, np.float32)
+        om = np.ascontiguousarray(omega, np.float32)
+        th = np.ascontiguousarray(theta0, np.float32)
+        self.n = om.size
+        self.h = lib().orc_kura_create(kind, self.n, _p(rp, C.c_uint32), _p(cc, C.c_uint32), _p(ww, C.c_float),
+                                       _p(om, C.c_float), _p(th, C.c_float), dt)
+
+    def rk4(self, n):
+        lib().orc_kura_step_rk4(self.h, n)
+
+    def euler(self, n):
+        lib().orc_kura_step_euler(self.h, n)
+
+    def phases(self):
+        out = np.zeros(self.n, np.float64)
+        lib().orc_kura_phases(self.h, _p(out, C.c_double))
+        return out
+
+    def order(self):
+        return float(lib().orc_kura_order(self.h))
+
+    def __del__(self):
+        try:
+            lib().orc_kura_destroy(self.h)
+        except Exception:
+            pass
+
+
+def poisson_spike(seed, stream, step, p):
+    return bool(lib().orc_poisson_spike(seed, stream, step, p))
+
+
+def gauss(seed, i, step):
+    return float(lib().orc_gauss(seed, i, step))
+
+
+def exp_f32(x):
+    return float(lib().orc_exp_f32(x))
+
+
+def ln_f32(x):
+    return float(lib().orc_ln_f32(x))
+
+
+def vtrap_f32(x, y):
+    return float(lib().orc_vtrap_f32(x, y))
+
+
+def vtrap_f64(x, y):
+    return float(lib().orc_vtrap_f64(x, y))
+
+
+def compact(spikes):
+    s = np.ascontiguousarray(spikes, np.int32)
+    out = np.zeros(s.size + 1, np.uint32)
+    k = lib().orc_compact(_p(s, C.c_int32), s.size, _p(out, C.c_uint32))
+    return out[:k].copy()
+
+
+def firing_rates(raster, dt_s):
+    r = np.ascontiguousarray(raster, np.int32)
+    t, n = r.shape
+    out = np.zeros(n, np.float32)
+    lib().orc_firing_rates(_p(r, C.c_int32), n, t, dt_s, _p(out, C.c_float))
+    return out
+
+
+def isi_cv(raster, dt_s):
+    r = np.ascontiguousarray(raster, np.int32)
+    t, n = r.shape
+    out = np.zeros(n, np.float32)
+    lib().orc_isi_cv(_p(r, C.c_int32), n, t, dt_s, _p(out, C.c_float))
+    return out

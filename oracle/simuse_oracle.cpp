@@ -1,0 +1,796 @@
+// simuse_oracle.cpp — CPU oracle for the SiMuse per-timestep network step.
+//
+// TEST INFRASTRUCTURE ONLY: the checker for the HIP product path and the
+// timed CPU baseline.  Nothing in the product library links or calls this.
+//
+// Every function cites the reference file:line it restates.  Path prefixes:
+//   P/  = crates/SiMuse/crates/pillars/
+//   PC/ = crates/SiMuse/crates/pillars-cuda/
+//
+// Build: see oracle/Makefile (g++ -O3 -ffp-contract=off, no fast-math), which
+// keeps float arithmetic unfused exactly like the reference's default non-FMA
+// build (P/src/math/fast_exp.rs:32-55, PC/src/engine.rs:762-770).
+
+#include "simuse_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+constexpr uint32_t kSlots = 32;  // MAX_DELAY_SLOTS, P/src/api/net/graph.rs:33
+constexpr float kClampLo = -100.0f, kClampHi = 100.0f;  // P/src/core/exec/population.rs:69
+
+// ---------------------------------------------------------------------------
+// Integer hashing
+// ---------------------------------------------------------------------------
+
+// poisson_spike — P/src/api/net/network/sources.rs:5-14 (SplitMix64 finaliser
+// over ((seed<<32)|stream) ^ step*phi64, 24-bit uniform compared with p).
+static inline bool poisson_spike(uint32_t seed, uint32_t stream, uint64_t step, float p) {
+    uint64_t z = (static_cast<uint64_t>(seed) << 32) | static_cast<uint64_t>(stream);
+    z ^= step * 0x9E3779B97F4A7C15ull;
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    float u = static_cast<float>(z >> 40) / static_cast<float>(1ull << 24);
+    return u < p;
+}
+
+// SplitMix64 step (Steele, Lea & Flood 2014) used for this build's own
+// counter-based Gaussian noise; not a reference function.
+static inline uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static inline float bits_to_f(uint32_t b) { float f; std::memcpy(&f, &b, 4); return f; }
+static inline uint32_t f_to_bits(float f) { uint32_t b; std::memcpy(&b, &f, 4); return b; }
+
+// ---------------------------------------------------------------------------
+// Deterministic float math built from IEEE +,-,*,/ only, so that the CPU and
+// the GPU produce identical bits.  Accuracy is checked against libm in tests.
+// ---------------------------------------------------------------------------
+
+// ln(x) for finite normal x > 0: exponent split, m in (sqrt(1/2), sqrt(2)],
+// ln(m) = 2*atanh(f/(2+f)) odd series to s^11.
+static inline float ln_f32(float x) {
+    uint32_t b = f_to_bits(x);
+    int e = static_cast<int>((b >> 23) & 0xFFu) - 127;
+    float m = bits_to_f((b & 0x7FFFFFu) | 0x3F800000u);
+    if (m > 1.41421356f) { m = m * 0.5f; e += 1; }
+    float f = m - 1.0f;
+    float s = f / (2.0f + f);
+    float s2 = s * s;
+    float p = 0.181818187f;
+    p = p * s2 + 0.222222224f;
+    p = p * s2 + 0.285714298f;
+    p = p * s2 + 0.400000006f;
+    p = p * s2 + 0.666666687f;
+    p = p * s2 + 2.0f;
+    float lnm = s * p;
+    float ef = static_cast<float>(e);
+    return ef * 0.693145752f + (lnm + ef * 1.42860677e-06f);
+}
+
+// exp(x): Cody-Waite reduction x = k*ln2 + r, degree-7 Taylor on |r|<=0.35,
+// scale by 2^k through the exponent field.  Results below 2^-126 flush to 0.
+static inline float exp_f32(float x) {
+    if (!(x == x)) return x;
+    if (x > 88.72f) return std::numeric_limits<float>::infinity();
+    if (x < -87.33f) return 0.0f;
+    float kf = std::floor(x * 1.44269502f + 0.5f);
+    float r = (x - kf * 0.693145752f) - kf * 1.42860677e-06f;
+    float p = 1.98412701e-04f;
+    p = p * r + 1.38888892e-03f;
+    p = p * r + 8.33333377e-03f;
+    p = p * r + 4.16666679e-02f;
+    p = p * r + 0.166666672f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    int k = static_cast<int>(kf);
+    if (k > 127) return p * bits_to_f(0x7F000000u) * 2.0f;
+    if (k < -126) return 0.0f;
+    return p * bits_to_f(static_cast<uint32_t>(k + 127) << 23);
+}
+
+// Standard normal draw for (seed, neuron, step): Marsaglia polar method on
+// 24-bit uniforms from mix64.  sqrt and / are IEEE correctly rounded.
+static inline float gauss(uint64_t seed, uint32_t i, uint64_t step) {
+    uint64_t base = mix64(seed ^ (static_cast<uint64_t>(i) * 0xD1B54A32D192ED03ull));
+    for (uint32_t k = 0; k < 64; ++k) {
+        uint64_t h = mix64(base ^ (step * 0xA24BAED4963EE407ull + k));
+        float u1 = static_cast<float>(h >> 40) * 5.9604644775390625e-08f;
+        float u2 = static_cast<float>((h >> 16) & 0xFFFFFFu) * 5.9604644775390625e-08f;
+        float x = u1 * 2.0f - 1.0f;
+        float y = u2 * 2.0f - 1.0f;
+        float s = x * x + y * y;
+        if (s >= 1.0f || s == 0.0f) continue;
+        float q = std::sqrt((-2.0f * ln_f32(s)) / s);
+        return x * q;
+    }
+    return 0.0f;
+}
+
+// vtrap(x, y) = x / (exp(x/y) - 1) — PC/kernels/hh.cu:28-34 (NEURON hh.mod).
+// Near the removable 0/0 point (|x/y| < 0.25) the FP32 difference exp-1
+// cancels; there the value is y * r/(e^r - 1) from its Bernoulli series,
+// evaluated in FP64.
+static inline float vtrap_f32(float x, float y) {
+    float r = x / y;
+    if (std::fabs(r) < 0.25f) {
+        double rd = static_cast<double>(x) / static_cast<double>(y);
+        double r2 = rd * rd;
+        double ev = -691.0 / 1307674368000.0;
+        ev = ev * r2 + 1.0 / 47900160.0;
+        ev = ev * r2 - 1.0 / 1209600.0;
+        ev = ev * r2 + 1.0 / 30240.0;
+        ev = ev * r2 - 1.0 / 720.0;
+        ev = ev * r2 + 1.0 / 12.0;
+        double bern = (1.0 - 0.5 * rd) + r2 * ev;
+        return static_cast<float>(static_cast<double>(y) * bern);
+    }
+    return x / (exp_f32(r) - 1.0f);
+}
+static inline double vtrap_f64(double x, double y) {
+    double r = x / y;
+    if (r == 0.0) return y;
+    return x / std::expm1(r);
+}
+
+// ---------------------------------------------------------------------------
+// Numeric twin policy
+// ---------------------------------------------------------------------------
+template <class R> struct Math;
+template <> struct Math<float> {
+    static float exp(float x) { return exp_f32(x); }
+    static float vtrap(float x, float y) { return vtrap_f32(x, y); }
+};
+template <> struct Math<double> {
+    static double exp(double x) { return std::exp(x); }
+    static double vtrap(double x, double y) { return vtrap_f64(x, y); }
+};
+
+// f32::max / SimdLane::max for non-NaN operands.
+template <class R> static inline R maxr(R a, R b) { return a > b ? a : b; }
+
+// ---------------------------------------------------------------------------
+// Neuron models
+// ---------------------------------------------------------------------------
+
+// LIFModel::step — P/src/zoo/neurons/lif.rs:47-78.
+// params: [0]dt [1]inv_tau_m [2]v_rest [3]v_thresh [4]v_reset [5]tau_ref
+// state:  [0]v  [1]refr
+template <class R>
+static inline int lif_step(R& v, R& refr, R I, R dt, R inv_tau, R v_rest, R v_th,
+                           R v_reset, R tau_ref) {
+    const R zero = R(0);
+    bool is_refr = refr > zero;
+    R dv = ((v_rest - v) + I) * inv_tau * dt;
+    R v_new = v + dv;
+    bool spike_raw = v_new >= v_th;
+    bool spike = is_refr ? false : spike_raw;
+    R v_final = spike ? v_reset : v_new;
+    v = is_refr ? v : v_final;
+    R refr_dec = refr - dt;
+    R refr_set = spike ? tau_ref : refr_dec;
+    refr = maxr(refr_set, zero);
+    return spike ? 1 : 0;
+}
+
+// IzhikevichModel::step — P/src/zoo/neurons/izhikevich.rs:71-125.
+// params: [0]dt [1]a [2]b [3]c [4]d [5]tau_ref [6]v_thresh
+// state:  [0]v [1]u [2]refr
+template <class R>
+static inline int izh_step(R& v, R& u, R& refr, R I, R dt, R a, R b, R c, R d,
+                           R tau_ref, R v_th) {
+    const R zero = R(0);
+    bool is_refr = refr > zero;
+    R vv = is_refr ? c : v;
+    R half_dt = dt * R(0.5);
+    const R c04 = R(0.04), c5 = R(5.0), c140 = R(140.0);
+    R v_poly1 = vv * (c04 * vv + c5);
+    R dv1 = (v_poly1 + c140 - u + I) * half_dt;
+    R v_mid = vv + dv1;
+    R v_poly2 = v_mid * (c04 * v_mid + c5);
+    R dv2 = (v_poly2 + c140 - u + I) * half_dt;
+    R dv_total = is_refr ? zero : (dv1 + dv2);
+    R v_new = vv + dv_total;
+    R du = a * (b * vv - u) * dt;
+    R u_new = is_refr ? u : (u + du);
+    bool spike = v_new >= v_th;
+    v = spike ? c : v_new;
+    u = spike ? (u_new + d) : u_new;
+    R refr_dec = refr - dt;
+    R refr_set = spike ? tau_ref : refr_dec;
+    refr = maxr(refr_set, zero);
+    return spike ? 1 : 0;
+}
+
+// Hodgkin–Huxley step — equations of PC/kernels/hh.cu:107-216 (the CPU HHModel
+// at P/src/zoo/neurons/hh.rs:1-51 is a no-op stub, so hh.cu is the only live
+// specification).  Column layout follows the stub's field order, with the
+// refractory period appended as param 12.
+// params: [0]dt [1]g_na [2]g_k [3]g_leak [4]e_na [5]e_k [6]e_leak [7]inv_cm
+//         [8]q10 [9]v_thresh [10]g_ca_t [11]e_ca_t [12]tau_ref
+// state:  [0]v [1]m [2]h [3]n [4]refr [5]c (T-type Ca gate)
+template <class R>
+static inline void hh_gate(R& g, R alpha, R beta, R dt, R q10) {
+    R sum_q10 = (alpha + beta + R(1e-12)) * q10;       // hh.cu:174
+    R m_inf = alpha / (alpha + beta + R(1e-12));        // hh.cu:175
+    R decay = Math<R>::exp(-dt * sum_q10);              // hh.cu:176
+    g = m_inf + (g - m_inf) * decay;                    // hh.cu:182
+}
+template <class R>
+static inline int hh_step(R* st, const R* pr, R I) {
+    using M = Math<R>;
+    R dt = pr[0], g_na = pr[1], g_k = pr[2], g_l = pr[3], e_na = pr[4], e_k = pr[5];
+    R e_l = pr[6], inv_cm = pr[7], q10 = pr[8], v_th = pr[9], g_cat = pr[10], e_cat = pr[11];
+    R tau_ref = pr[12];
+    R& v = st[0]; R& m = st[1]; R& h = st[2]; R& n = st[3]; R& refr = st[4]; R& cg = st[5];
+    R ref = refr;                                        // hh.cu:142-146
+    if (ref > R(0)) refr = maxr(R(0), ref - dt);
+    R vi = v;
+    // NaV m^3 h (hh.cu:40-53)
+    R am = R(0.1) * M::vtrap(-(vi + R(40)), R(10));
+    R bm = R(4) * M::exp(-(vi + R(65)) / R(18));
+    hh_gate(m, am, bm, dt, q10);
+    R ah = R(0.07) * M::exp(-(vi + R(65)) / R(20));
+    R bh = R(1) / (R(1) + M::exp(-(vi + R(35)) / R(10)));
+    hh_gate(h, ah, bh, dt, q10);
+    R gna_ch = g_na;
+    gna_ch *= m; gna_ch *= m; gna_ch *= m; gna_ch *= h;
+    // KV n^4 (hh.cu:59-64)
+    R an = R(0.01) * M::vtrap(-(vi + R(55)), R(10));
+    R bn = R(0.125) * M::exp(-(vi + R(65)) / R(80));
+    hh_gate(n, an, bn, dt, q10);
+    R gk_ch = g_k;
+    gk_ch *= n; gk_ch *= n; gk_ch *= n; gk_ch *= n;
+    R g_ion = R(0) + gna_ch;
+    R i_ion = R(0) + gna_ch * e_na;
+    g_ion += gk_ch;
+    i_ion += gk_ch * e_k;
+    if (g_cat != R(0)) {  // CaV-T (hh.cu:73-80), present only when g_ca_t != 0
+        R minf = R(1) / (R(1) + M::exp(-(vi + R(56)) / R(6.2)));
+        R ac = minf / R(0.6);
+        R bc = (R(1) - minf) / R(0.6);
+        hh_gate(cg, ac, bc, dt, q10);
+        R gc = g_cat * cg;
+        g_ion += gc;
+        i_ion += gc * e_cat;
+    }
+    // Exponential-Euler membrane (hh.cu:194-204), tau_v = C_m/g_tot.
+    R g_tot = g_ion + g_l + R(1e-12);
+    R v_inf = (I + i_ion + g_l * e_l) / g_tot;
+    R decay_v = M::exp(-dt * (g_tot * inv_cm));
+    R v_prev = vi;
+    vi = v_inf + (v_prev - v_inf) * decay_v;
+    int spiked = 0;                                      // hh.cu:206-213
+    if (ref <= R(0) && v_prev < v_th && vi >= v_th) {
+        spiked = 1;
+        refr = tau_ref;
+    }
+    v = vi;
+    return spiked;
+}
+
+static inline int n_state(int model) {
+    return model == ORC_MODEL_LIF ? 2 : model == ORC_MODEL_IZHIKEVICH ? 3 : 6;
+}
+static inline int n_param(int model) {
+    return model == ORC_MODEL_LIF ? 6 : model == ORC_MODEL_IZHIKEVICH ? 7 : 13;
+}
+static inline bool clamps(int model) {  // #[no_clamp] on HH (hh.rs:13)
+    return model != ORC_MODEL_HH;
+}
+
+// ---------------------------------------------------------------------------
+// Network
+// ---------------------------------------------------------------------------
+
+struct NetBase {
+    virtual ~NetBase() {}
+    virtual int add_source(float rate_hz, float dt_ms, uint32_t seed, const uint32_t* posts,
+                           const float* w, const uint8_t* d, uint32_t count) = 0;
+    virtual int set_noise(int pop, const float* sigma, uint64_t seed) = 0;
+    virtual int set_plasticity(const orc_stdp_params* p, const uint8_t* plastic) = 0;
+    virtual int inject(int pop, const uint32_t* idx, const float* val, uint32_t count) = 0;
+    virtual void step(uint32_t n) = 0;
+    virtual uint64_t step_counter() const = 0;
+    virtual void spikes(int32_t* out) const = 0;
+    virtual uint32_t active(uint32_t* out) const = 0;
+    virtual int state(int pop, int col, double* out) const = 0;
+    virtual void ring(double* out) const = 0;
+    virtual void weights(double* out) const = 0;
+    virtual uint64_t total_spikes() const = 0;
+    virtual uint64_t total_events() const = 0;
+    virtual void record(bool on) = 0;
+    virtual uint64_t raster_steps() const = 0;
+    virtual void raster(int32_t* out) const = 0;
+    int n_threads = 1;
+};
+
+template <class R>
+struct Net final : NetBase {
+    struct Pop {
+        int model;
+        uint32_t n, off, cap;
+        std::vector<R> st, pr;  // [col*cap + i]
+        bool noise = false;
+        uint64_t seed = 0;
+        std::vector<float> sigma;
+    };
+    struct Src {
+        float p;
+        uint32_t seed;
+        std::vector<uint32_t> post;
+        std::vector<float> w;
+        std::vector<uint8_t> d;
+    };
+    std::vector<Pop> pops;
+    uint32_t N = 0;
+    std::vector<uint32_t> row_ptr, col;
+    std::vector<R> w;            // per-edge weight in the twin's precision
+    std::vector<uint8_t> d;
+    std::vector<R> ringv;        // [32][N]
+    std::vector<R> i_in;
+    std::vector<int32_t> spk;
+    std::vector<uint32_t> act;
+    std::vector<Src> srcs;
+    uint64_t stepc = 0, n_spikes = 0, n_events = 0;
+    bool rec = false;
+    std::vector<int32_t> rast;
+    uint64_t rast_steps = 0;
+    // pair-based STDP
+    bool stdp = false;
+    orc_stdp_params sp{};
+    std::vector<uint8_t> plastic;
+    std::vector<R> tr_pre, tr_post;
+    std::vector<uint32_t> in_ptr, in_edge, edge_pre;
+
+    int init(const orc_pop_desc* pd, int np_, const uint32_t* rp, const uint32_t* c,
+             const float* wf, const uint8_t* dl, uint64_t n_syn) {
+        uint32_t off = 0;
+        for (int p = 0; p < np_; ++p) {
+            Pop P;
+            P.model = pd[p].model;
+            P.n = pd[p].n;
+            P.off = off;
+            P.cap = (pd[p].n + 7u) / 8u * 8u;
+            int ns = n_state(P.model), npar = n_param(P.model);
+            if (pd[p].ns != static_cast<uint32_t>(ns) || pd[p].np != static_cast<uint32_t>(npar)) return -2;
+            uint32_t hs = pd[p].stride;
+            P.st.assign(static_cast<size_t>(ns) * P.cap, R(0));
+            P.pr.assign(static_cast<size_t>(npar) * P.cap, R(0));
+            for (int k = 0; k < ns; ++k)
+                for (uint32_t i = 0; i < P.n; ++i)
+                    P.st[static_cast<size_t>(k) * P.cap + i] = R(pd[p].states[static_cast<size_t>(k) * hs + i]);
+            for (int k = 0; k < npar; ++k)
+                for (uint32_t i = 0; i < P.n; ++i)
+                    P.pr[static_cast<size_t>(k) * P.cap + i] = R(pd[p].params[static_cast<size_t>(k) * hs + i]);
+            off += P.n;
+            pops.push_back(std::move(P));
+        }
+        N = off;
+        row_ptr.assign(rp, rp + N + 1);
+        col.assign(c, c + n_syn);
+        w.resize(n_syn);
+        for (uint64_t e = 0; e < n_syn; ++e) w[e] = R(wf[e]);
+        d.assign(dl, dl + n_syn);
+        ringv.assign(static_cast<size_t>(kSlots) * N, R(0));
+        i_in.assign(N, R(0));
+        spk.assign(N, 0);
+        return 0;
+    }
+
+    int add_source(float rate_hz, float dt_ms, uint32_t seed, const uint32_t* posts,
+                   const float* wf, const uint8_t* dl, uint32_t count) override {
+        // Network::add_poisson_source — P/src/api/net/network/core.rs:288-318
+        Src s;
+        s.p = std::min(std::max(rate_hz * dt_ms / 1000.0f, 0.0f), 1.0f);
+        s.seed = seed;
+        for (uint32_t i = 0; i < count; ++i) {
+            if (posts[i] >= N || dl[i] >= kSlots) return -3;
+            s.post.push_back(posts[i]);
+            s.w.push_back(wf[i]);
+            s.d.push_back(dl[i]);
+        }
+        srcs.push_back(std::move(s));
+        return 0;
+    }
+    int set_noise(int p, const float* sigma, uint64_t seed) override {
+        if (p < 0 || p >= static_cast<int>(pops.size())) return -3;
+        pops[p].noise = true;
+        pops[p].seed = seed;
+        pops[p].sigma.assign(sigma, sigma + pops[p].n);
+        return 0;
+    }
+    int set_plasticity(const orc_stdp_params* p, const uint8_t* pl) override {
+        stdp = true;
+        sp = *p;
+        size_t ns = col.size();
+        plastic.assign(pl, pl + ns);
+        tr_pre.assign(N, R(0));
+        tr_post.assign(N, R(0));
+        edge_pre.resize(ns);
+        for (uint32_t i = 0; i < N; ++i)
+            for (uint32_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) edge_pre[e] = i;
+        // in-edge lists: for each post, its in-edges in ascending edge order
+        in_ptr.assign(N + 1, 0);
+        for (size_t e = 0; e < ns; ++e) in_ptr[col[e] + 1]++;
+        for (uint32_t i = 0; i < N; ++i) in_ptr[i + 1] += in_ptr[i];
+        in_edge.assign(ns, 0);
+        std::vector<uint32_t> cur(in_ptr.begin(), in_ptr.end() - 1);
+        for (size_t e = 0; e < ns; ++e) in_edge[cur[col[e]]++] = static_cast<uint32_t>(e);
+        return 0;
+    }
+    int inject(int p, const uint32_t* idx, const float* val, uint32_t count) override {
+        // Network::inject_current — P/src/api/net/network/build.rs:243-275
+        if (p < 0 || p >= static_cast<int>(pops.size())) return -3;
+        uint32_t slot = static_cast<uint32_t>(stepc % kSlots);
+        for (uint32_t i = 0; i < count; ++i) {
+            if (idx[i] >= pops[p].n) return -3;
+            ringv[static_cast<size_t>(slot) * N + pops[p].off + idx[i]] += R(val[i]);
+        }
+        return 0;
+    }
+
+    void step_pop(Pop& P) {
+        const uint32_t cap = P.cap;
+        // dt broadcast from param column 0 of neuron 0 (population.rs:307,361)
+        const R dt = P.pr[0];
+        const bool cl = clamps(P.model);
+        const int64_t n = P.n;
+        const int nst = n_state(P.model), npar = n_param(P.model);
+#pragma omp parallel for schedule(static) num_threads(n_threads) if (n_threads > 1 && n > 4096)
+        for (int64_t ii = 0; ii < n; ++ii) {
+            uint32_t i = static_cast<uint32_t>(ii);
+            R I = i_in[P.off + i];
+            if (cl) I = std::min(std::max(I, R(kClampLo)), R(kClampHi));  // population.rs:79-88
+            R* s = P.st.data();
+            const R* pr = P.pr.data();
+            int spike;
+            if (P.model == ORC_MODEL_LIF) {
+                spike = lif_step<R>(s[i], s[cap + i], I, dt, pr[cap + i], pr[2 * cap + i],
+                                    pr[3 * cap + i], pr[4 * cap + i], pr[5 * cap + i]);
+            } else if (P.model == ORC_MODEL_IZHIKEVICH) {
+                spike = izh_step<R>(s[i], s[cap + i], s[2 * cap + i], I, dt, pr[cap + i],
+                                    pr[2 * cap + i], pr[3 * cap + i], pr[4 * cap + i],
+                                    pr[5 * cap + i], pr[6 * cap + i]);
+            } else {
+                R stv[6], prv[13];
+                for (int k = 0; k < nst; ++k) stv[k] = s[static_cast<size_t>(k) * cap + i];
+                for (int k = 0; k < npar; ++k) prv[k] = pr[static_cast<size_t>(k) * cap + i];
+                prv[0] = dt;
+                spike = hh_step<R>(stv, prv, I);
+                for (int k = 0; k < nst; ++k) s[static_cast<size_t>(k) * cap + i] = stv[k];
+            }
+            spk[P.off + i] = spike;
+        }
+    }
+
+    // Pair-based STDP applied on the CSR, event-driven: rule of
+    // P/src/ops/plasticity/stdp.rs:34-75 (traces decay by e^{-dt/tau} and add
+    // the spike; LTP = A+ * pre-trace at a post spike, LTD = A- * post-trace at
+    // a pre spike) with the spike-triggered in-edge/out-edge event scheme of
+    // P/src/api/learn/brain.rs:204-286 (traces first, fused decay+add).
+    // LTP pass (in-edges of spiking posts) then LTD pass (out-edges of
+    // spiking pres), each clipped to [w_min, w_max].  The per-step decay
+    // factors are e^{-dt/tau} in the twin's own precision.
+    void stdp_update(R dt) {
+        const R dpre = R(std::exp(-static_cast<double>(dt) / static_cast<double>(sp.tau_plus)));
+        const R dpost = R(std::exp(-static_cast<double>(dt) / static_cast<double>(sp.tau_minus)));
+        for (uint32_t i = 0; i < N; ++i) {
+            R add = spk[i] ? R(1) : R(0);
+            tr_pre[i] = std::fma(tr_pre[i], dpre, add);
+            tr_post[i] = std::fma(tr_post[i], dpost, add);
+        }
+        const R ap = R(sp.a_plus), am = R(sp.a_minus), wlo = R(sp.w_min), whi = R(sp.w_max);
+        for (uint32_t post : act)
+            for (uint32_t k = in_ptr[post]; k < in_ptr[post + 1]; ++k) {
+                uint32_t e = in_edge[k];
+                if (!plastic[e]) continue;
+                R nw = w[e] + ap * tr_pre[edge_pre[e]];
+                w[e] = std::min(std::max(nw, wlo), whi);
+            }
+        for (uint32_t pre : act)
+            for (uint32_t e = row_ptr[pre]; e < row_ptr[pre + 1]; ++e) {
+                if (!plastic[e]) continue;
+                R nw = w[e] - am * tr_post[col[e]];
+                w[e] = std::min(std::max(nw, wlo), whi);
+            }
+    }
+
+    void step_once() {
+        // Network::step — P/src/api/net/network/step.rs:18-189
+        if (N == 0) { ++stepc; return; }
+        const uint32_t slot = static_cast<uint32_t>(stepc % kSlots);
+        R* cur = ringv.data() + static_cast<size_t>(slot) * N;
+        // Gaussian current noise enters the current slot like inject_current
+        // (build.rs:243-275) issued just before the step.
+        for (auto& P : pops)
+            if (P.noise)
+                for (uint32_t i = 0; i < P.n; ++i)
+                    cur[P.off + i] += R(P.sigma[i] * gauss(P.seed, i, stepc));
+        // 1-2: read slot -> i_in, clear slot (delay.rs:68-77)
+        std::memcpy(i_in.data(), cur, sizeof(R) * N);
+        std::fill(cur, cur + N, R(0));
+        // 3: populations (population.rs:284-420)
+        for (auto& P : pops) step_pop(P);
+        // 4: compaction, ascending (compact.rs:30-61)
+        act.clear();
+        for (uint32_t i = 0; i < N; ++i)
+            if (spk[i] != 0) act.push_back(i);
+        n_spikes += act.size();
+        if (rec) {
+            rast.insert(rast.end(), spk.begin(), spk.end());
+            ++rast_steps;
+        }
+        if (stdp) stdp_update(pops.empty() ? R(0) : pops[0].pr[0]);
+        // 5: push, pre ascending then CSR row order (push.rs:126-173); the
+        //    rayon variant (push.rs:184-237) applies triples in the same order.
+        for (uint32_t pre : act) {
+            for (uint32_t e = row_ptr[pre]; e < row_ptr[pre + 1]; ++e) {
+                uint32_t dd = d[e] < 1 ? 1u : d[e];
+                uint32_t ts = static_cast<uint32_t>((stepc + dd) % kSlots);
+                ringv[static_cast<size_t>(ts) * N + col[e]] += w[e];
+            }
+            n_events += row_ptr[pre + 1] - row_ptr[pre];
+        }
+        // 5b: Poisson sources (step.rs:166-185)
+        for (auto& s : srcs) {
+            if (s.p <= 0.0f || !poisson_spike(s.seed, 0, stepc, s.p)) continue;
+            for (size_t i = 0; i < s.post.size(); ++i) {
+                uint32_t dd = s.d[i] < 1 ? 1u : s.d[i];
+                uint32_t ts = static_cast<uint32_t>((stepc + dd) % kSlots);
+                ringv[static_cast<size_t>(ts) * N + s.post[i]] += R(s.w[i]);
+            }
+            n_events += s.post.size();
+        }
+        ++stepc;  // 6
+    }
+    void step(uint32_t n) override { for (uint32_t k = 0; k < n; ++k) step_once(); }
+    uint64_t step_counter() const override { return stepc; }
+    void spikes(int32_t* out) const override { std::memcpy(out, spk.data(), 4 * static_cast<size_t>(N)); }
+    uint32_t active(uint32_t* out) const override {
+        if (out) std::memcpy(out, act.data(), 4 * act.size());
+        return static_cast<uint32_t>(act.size());
+    }
+    int state(int p, int c, double* out) const override {
+        if (p < 0 || p >= static_cast<int>(pops.size())) return -3;
+        const Pop& P = pops[p];
+        if (c < 0 || c >= n_state(P.model)) return -3;
+        for (uint32_t i = 0; i < P.n; ++i) out[i] = static_cast<double>(P.st[static_cast<size_t>(c) * P.cap + i]);
+        return 0;
+    }
+    void ring(double* out) const override {
+        for (size_t k = 0; k < ringv.size(); ++k) out[k] = static_cast<double>(ringv[k]);
+    }
+    void weights(double* out) const override {
+        for (size_t k = 0; k < w.size(); ++k) out[k] = static_cast<double>(w[k]);
+    }
+    uint64_t total_spikes() const override { return n_spikes; }
+    uint64_t total_events() const override { return n_events; }
+    void record(bool on) override { rec = on; }
+    uint64_t raster_steps() const override { return rast_steps; }
+    void raster(int32_t* out) const override { std::memcpy(out, rast.data(), 4 * rast.size()); }
+};
+
+// ---------------------------------------------------------------------------
+// Kuramoto
+// ---------------------------------------------------------------------------
+// Coupling c_i = sum_j W_ij sin(theta_j - theta_i) over the CSR row of i
+// (P/src/kernels/phase/sparse.rs:63-96, written there as
+// sin_j cos_i - cos_j sin_i).  The reference integrates by forward Euler and
+// wraps to [-pi, pi) (P/src/kernels/phase/kuramoto.rs:42-68,
+// P/src/kernels/mod.rs:74-77); RK4 is this build's extension.  The oracle
+// evaluates sin/cos with libm in its own precision.
+struct KuraBase {
+    virtual ~KuraBase() {}
+    virtual void rk4(uint32_t n) = 0;
+    virtual void euler(uint32_t n) = 0;
+    virtual void phases(double* out) const = 0;
+    virtual double order() const = 0;
+};
+template <class R>
+struct Kura final : KuraBase {
+    uint32_t n;
+    std::vector<uint32_t> rp, col;
+    std::vector<R> w, om, th, k1, k2, k3, k4, tmp, sn, cs;
+    R dt;
+    void deriv(const std::vector<R>& t, std::vector<R>& out) {
+        for (uint32_t i = 0; i < n; ++i) { sn[i] = std::sin(t[i]); cs[i] = std::cos(t[i]); }
+        for (uint32_t i = 0; i < n; ++i) {
+            R acc = R(0);
+            for (uint32_t e = rp[i]; e < rp[i + 1]; ++e) {
+                uint32_t j = col[e];
+                acc += w[e] * (sn[j] * cs[i] - cs[j] * sn[i]);
+            }
+            out[i] = om[i] + acc;
+        }
+    }
+    static R wrap(R p) {
+        const R tau = R(6.283185307179586476925286766559);
+        return p - tau * std::round(p / tau);
+    }
+    void rk4(uint32_t steps) override {
+        for (uint32_t s = 0; s < steps; ++s) {
+            deriv(th, k1);
+            for (uint32_t i = 0; i < n; ++i) tmp[i] = th[i] + R(0.5) * dt * k1[i];
+            deriv(tmp, k2);
+            for (uint32_t i = 0; i < n; ++i) tmp[i] = th[i] + R(0.5) * dt * k2[i];
+            deriv(tmp, k3);
+            for (uint32_t i = 0; i < n; ++i) tmp[i] = th[i] + dt * k3[i];
+            deriv(tmp, k4);
+            for (uint32_t i = 0; i < n; ++i)
+                th[i] = wrap(th[i] + dt / R(6) * (k1[i] + R(2) * k2[i] + R(2) * k3[i] + k4[i]));
+        }
+    }
+    void euler(uint32_t steps) override {
+        for (uint32_t s = 0; s < steps; ++s) {
+            deriv(th, k1);
+            for (uint32_t i = 0; i < n; ++i) th[i] = wrap(th[i] + dt * k1[i]);
+        }
+    }
+    void phases(double* out) const override {
+        for (uint32_t i = 0; i < n; ++i) out[i] = static_cast<double>(th[i]);
+    }
+    // phase_coherence with unit amplitudes (P/src/kernels/measure/mod.rs:76-106)
+    double order() const override {
+        double ss = 0, cc = 0;
+        for (uint32_t i = 0; i < n; ++i) { ss += std::sin(static_cast<double>(th[i])); cc += std::cos(static_cast<double>(th[i])); }
+        return std::sqrt(ss * ss + cc * cc) / (n ? n : 1);
+    }
+};
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+void* orc_net_create(int kind, const orc_pop_desc* pops, int32_t n_pops, const uint32_t* row_ptr,
+                     const uint32_t* col, const float* w, const uint8_t* delay, uint64_t n_syn) {
+    NetBase* nb = nullptr;
+    int rc;
+    if (kind == 64) {
+        auto* n = new Net<double>();
+        rc = n->init(pops, n_pops, row_ptr, col, w, delay, n_syn);
+        nb = n;
+    } else {
+        auto* n = new Net<float>();
+        rc = n->init(pops, n_pops, row_ptr, col, w, delay, n_syn);
+        nb = n;
+    }
+    if (rc != 0) { delete nb; return nullptr; }
+    return nb;
+}
+void orc_net_destroy(void* net) { delete static_cast<NetBase*>(net); }
+int orc_net_add_poisson_source(void* net, float rate_hz, float dt_ms, uint32_t seed, const uint32_t* posts,
+                               const float* w, const uint8_t* delay, uint32_t count) {
+    return static_cast<NetBase*>(net)->add_source(rate_hz, dt_ms, seed, posts, w, delay, count);
+}
+int orc_net_set_noise(void* net, int32_t pop, const float* sigma, uint64_t seed) {
+    return static_cast<NetBase*>(net)->set_noise(pop, sigma, seed);
+}
+int orc_net_set_plasticity(void* net, const orc_stdp_params* p, const uint8_t* plastic) {
+    return static_cast<NetBase*>(net)->set_plasticity(p, plastic);
+}
+int orc_net_inject_current(void* net, int32_t pop, const uint32_t* idx, const float* val, uint32_t count) {
+    return static_cast<NetBase*>(net)->inject(pop, idx, val, count);
+}
+void orc_net_step(void* net, uint32_t n_steps) { static_cast<NetBase*>(net)->step(n_steps); }
+uint64_t orc_net_step_counter(void* net) { return static_cast<NetBase*>(net)->step_counter(); }
+void orc_net_spikes(void* net, int32_t* out) { static_cast<NetBase*>(net)->spikes(out); }
+uint32_t orc_net_active(void* net, uint32_t* out) { return static_cast<NetBase*>(net)->active(out); }
+int orc_net_state(void* net, int32_t pop, int32_t col, double* out) {
+    return static_cast<NetBase*>(net)->state(pop, col, out);
+}
+void orc_net_ring(void* net, double* out) { static_cast<NetBase*>(net)->ring(out); }
+void orc_net_weights(void* net, double* out) { static_cast<NetBase*>(net)->weights(out); }
+uint64_t orc_net_total_spikes(void* net) { return static_cast<NetBase*>(net)->total_spikes(); }
+uint64_t orc_net_total_events(void* net) { return static_cast<NetBase*>(net)->total_events(); }
+void orc_net_record(void* net, int32_t on) { static_cast<NetBase*>(net)->record(on != 0); }
+uint64_t orc_net_raster_steps(void* net) { return static_cast<NetBase*>(net)->raster_steps(); }
+void orc_net_raster(void* net, int32_t* out) { static_cast<NetBase*>(net)->raster(out); }
+void orc_net_set_threads(void* net, int32_t n_threads) {
+    static_cast<NetBase*>(net)->n_threads = n_threads < 1 ? 1 : n_threads;
+}
+
+int32_t orc_poisson_spike(uint32_t seed, uint32_t stream, uint64_t step, float p) {
+    return poisson_spike(seed, stream, step, p) ? 1 : 0;
+}
+float orc_gauss(uint64_t seed, uint32_t i, uint64_t step) { return gauss(seed, i, step); }
+float orc_exp_f32(float x) { return exp_f32(x); }
+float orc_ln_f32(float x) { return ln_f32(x); }
+float orc_vtrap_f32(float x, float y) { return vtrap_f32(x, y); }
+double orc_vtrap_f64(double x, double y) { return vtrap_f64(x, y); }
+
+// compact_spikes_into — P/src/api/net/compact.rs:30-61 (ascending indices).
+uint32_t orc_compact(const int32_t* spikes, uint32_t n, uint32_t* out) {
+    uint32_t k = 0;
+    for (uint32_t i = 0; i < n; ++i)
+        if (spikes[i] != 0) out[k++] = i;
+    return k;
+}
+
+void* orc_kura_create(int kind, uint32_t n, const uint32_t* row_ptr, const uint32_t* col, const float* w,
+                      const float* omega, const float* theta0, float dt) {
+    auto fill = [&](auto* k) {
+        using R = typename std::remove_reference<decltype(k->w[0])>::type;
+        k->n = n;
+        k->rp.assign(row_ptr, row_ptr + n + 1);
+        uint64_t ne = row_ptr[n];
+        k->col.assign(col, col + ne);
+        k->w.resize(ne);
+        for (uint64_t e = 0; e < ne; ++e) k->w[e] = R(w[e]);
+        k->om.resize(n); k->th.resize(n);
+        for (uint32_t i = 0; i < n; ++i) { k->om[i] = R(omega[i]); k->th[i] = R(theta0[i]); }
+        k->k1.assign(n, R(0)); k->k2 = k->k1; k->k3 = k->k1; k->k4 = k->k1;
+        k->tmp = k->k1; k->sn = k->k1; k->cs = k->k1;
+        k->dt = R(dt);
+    };
+    if (kind == 64) { auto* k = new Kura<double>(); fill(k); return static_cast<KuraBase*>(k); }
+    auto* k = new Kura<float>(); fill(k); return static_cast<KuraBase*>(k);
+}
+void orc_kura_destroy(void* k) { delete static_cast<KuraBase*>(k); }
+void orc_kura_step_rk4(void* k, uint32_t n) { static_cast<KuraBase*>(k)->rk4(n); }
+void orc_kura_step_euler(void* k, uint32_t n) { static_cast<KuraBase*>(k)->euler(n); }
+void orc_kura_phases(void* k, double* out) { static_cast<KuraBase*>(k)->phases(out); }
+double orc_kura_order(void* k) { return static_cast<KuraBase*>(k)->order(); }
+
+// firing_rates — P/src/kernels/stats/rates.rs:6-35 (dt in seconds).
+void orc_firing_rates(const int32_t* raster, uint32_t n, uint32_t steps, float dt_s, float* out) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = 0.0f;
+    for (uint32_t t = 0; t < steps; ++t)
+        for (uint32_t i = 0; i < n; ++i)
+            if (raster[static_cast<size_t>(t) * n + i] != 0) out[i] += 1.0f;
+    float dur = static_cast<float>(steps) * dt_s;
+    if (dur > 0.0f)
+        for (uint32_t i = 0; i < n; ++i) out[i] /= dur;
+}
+
+// compute_cvs — P/src/kernels/stats/isi.rs:30-67 (n-1 variance, >= 2 ISIs).
+void orc_isi_cv(const int32_t* raster, uint32_t n, uint32_t steps, float dt_s, float* out) {
+    std::vector<float> isi;
+    for (uint32_t i = 0; i < n; ++i) {
+        out[i] = 0.0f;
+        isi.clear();
+        int64_t last = -1;
+        for (uint32_t t = 0; t < steps; ++t) {
+            if (raster[static_cast<size_t>(t) * n + i] != 0) {
+                if (last >= 0) {
+                    float iv = static_cast<float>(t - last) * dt_s * 1000.0f;
+                    if (iv > 0.0f) isi.push_back(iv);
+                }
+                last = t;
+            }
+        }
+        if (isi.size() >= 2) {
+            float mean = 0.0f;
+            for (float x : isi) mean += x;
+            mean /= static_cast<float>(isi.size());
+            if (mean > 0.0f) {
+                float var = 0.0f;
+                for (float x : isi) var += (x - mean) * (x - mean);
+                var /= static_cast<float>(isi.size() - 1);
+                out[i] = std::sqrt(var) / mean;
+            }
+        }
+    }
+}
+
+}  // extern "C"

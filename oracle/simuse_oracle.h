@@ -1,0 +1,106 @@
+/*
+ * simuse_oracle.h — C ABI of the CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY.  The oracle is a plain C++ restatement of the
+ * reference's CPU canon for the per-timestep spiking-network step
+ * (pillars::api::Network::step, crates/SiMuse/crates/pillars/src/api/net/
+ * network/step.rs:18-189).  It exists to check the HIP product path and to
+ * time a CPU baseline.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it; the product library never links or calls it.
+ *
+ * Two numeric twins share one template:
+ *   kind 32 — FP32 "reference-faithful": IEEE float, no FMA contraction
+ *             (build with -ffp-contract=off), the reference's association
+ *             order, ±100 input clamp, dt broadcast from param column 0.
+ *   kind 64 — FP64 "truth": the same equations evaluated in double, libm
+ *             exp/expm1, used to measure FP32 error.
+ */
+#ifndef SIMUSE_ORACLE_H
+#define SIMUSE_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Model ids (same numbering as the product's simuse.h). */
+#define ORC_MODEL_LIF 0
+#define ORC_MODEL_IZHIKEVICH 1
+#define ORC_MODEL_HH 2
+
+/* One population: column-major SoA with host stride ceil8(n)
+ * (P/src/core/exec/population.rs:106-131). */
+typedef struct {
+    int32_t model;
+    uint32_t n;
+    uint32_t ns;
+    uint32_t np;
+    uint32_t stride;
+    const float* states;
+    const float* params;
+} orc_pop_desc;
+
+typedef struct {
+    float a_plus;     /* LTP amplitude (per post spike, x pre-trace) */
+    float a_minus;    /* LTD amplitude (per pre spike, x post-trace) */
+    float tau_plus;   /* pre-trace time constant (ms) */
+    float tau_minus;  /* post-trace time constant (ms) */
+    float w_min;
+    float w_max;
+} orc_stdp_params;
+
+void* orc_net_create(int kind, const orc_pop_desc* pops, int32_t n_pops,
+                     const uint32_t* row_ptr, const uint32_t* col,
+                     const float* w, const uint8_t* delay, uint64_t n_syn);
+void orc_net_destroy(void* net);
+int orc_net_add_poisson_source(void* net, float rate_hz, float dt_ms, uint32_t seed,
+                               const uint32_t* posts, const float* w,
+                               const uint8_t* delay, uint32_t count);
+int orc_net_set_noise(void* net, int32_t pop, const float* sigma, uint64_t seed);
+int orc_net_set_plasticity(void* net, const orc_stdp_params* p, const uint8_t* plastic);
+int orc_net_inject_current(void* net, int32_t pop, const uint32_t* idx,
+                           const float* val, uint32_t count);
+void orc_net_step(void* net, uint32_t n_steps);
+uint64_t orc_net_step_counter(void* net);
+void orc_net_spikes(void* net, int32_t* out);
+uint32_t orc_net_active(void* net, uint32_t* out);
+/* State column `col` of population `pop`, widened to double. */
+int orc_net_state(void* net, int32_t pop, int32_t col, double* out);
+/* Ring buffer [32][N] widened to double. */
+void orc_net_ring(void* net, double* out);
+void orc_net_weights(void* net, double* out);
+uint64_t orc_net_total_spikes(void* net);
+uint64_t orc_net_total_events(void* net);
+/* Per-step spike recording (dense int32 raster [steps x N]). */
+void orc_net_record(void* net, int32_t on);
+uint64_t orc_net_raster_steps(void* net);
+void orc_net_raster(void* net, int32_t* out);
+void orc_net_set_threads(void* net, int32_t n_threads);
+
+/* Stand-alone pieces for known-answer tests. */
+int32_t orc_poisson_spike(uint32_t seed, uint32_t stream, uint64_t step, float p);
+float orc_gauss(uint64_t seed, uint32_t i, uint64_t step);
+float orc_exp_f32(float x);
+float orc_ln_f32(float x);
+float orc_vtrap_f32(float x, float y);
+double orc_vtrap_f64(double x, double y);
+uint32_t orc_compact(const int32_t* spikes, uint32_t n, uint32_t* out);
+
+/* Kuramoto (P/src/kernels/phase/sparse.rs:40-97 in Euler form, plus RK4). */
+void* orc_kura_create(int kind, uint32_t n, const uint32_t* row_ptr, const uint32_t* col,
+                      const float* w, const float* omega, const float* theta0, float dt);
+void orc_kura_destroy(void* k);
+void orc_kura_step_rk4(void* k, uint32_t n_steps);
+void orc_kura_step_euler(void* k, uint32_t n_steps);
+void orc_kura_phases(void* k, double* out);
+double orc_kura_order(void* k);
+
+/* Statistics (P/src/kernels/stats/rates.rs:6-35, isi.rs:30-67). */
+void orc_firing_rates(const int32_t* raster, uint32_t n, uint32_t steps, float dt_s, float* out);
+void orc_isi_cv(const int32_t* raster, uint32_t n, uint32_t steps, float dt_s, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,382 @@
+"""Pins the CPU oracle against the reference's own known-answer tests and
+against closed forms (CPU only).
+
+Reference paths are relative to crates/SiMuse/crates/.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from nets import delay_net, empty_csr
+from simuse.models import hh_pop, izh_pop, lif_pop
+from simuse.workloads import NetSpec, csr_from_triplets
+
+
+def single_pop(pop, dt):
+    rp, col, w, d = empty_csr(pop.n)
+    return NetSpec("single", [pop], rp, col, w, d, dt, 0)
+
+
+# --- compaction: pillars/src/api/net/compact.rs:79-140 -----------------------
+def test_compact_known_answers():
+    assert list(po.compact(np.zeros(100, np.int32))) == []
+    s = np.array([1 if i % 3 == 0 else 0 for i in range(20)], np.int32)
+    assert list(po.compact(s)) == [0, 3, 6, 9, 12, 15, 18]
+    assert list(po.compact(np.ones(10, np.int32))) == list(range(10))
+    assert list(po.compact(np.array([0, 1] * 5, np.int32))) == [1, 3, 5, 7, 9]
+    n = 1003
+    i = np.arange(n, dtype=np.uint64)
+    s = (((i * np.uint64(2654435761)) % np.uint64(997)) < 97).astype(np.int32)
+    assert np.array_equal(po.compact(s), np.nonzero(s)[0].astype(np.uint32))
+
+
+# --- push / delay ring: pillars/src/api/net/push.rs:300-431 ------------------
+def _push_net(n, pre, post, w, d):
+    pops = [lif_pop(n, 0.5, 0.5, -70.0, -54.0, -70.0, 2.0, -70.0)]
+    rp, col, ww, dd = csr_from_triplets(n, pre, post, np.asarray(w, np.float32), np.asarray(d, np.uint8))
+    return NetSpec("push", pops, rp, col, ww, dd, 0.5, 0)
+
+
+def _force_fire(o, pop, idx):
+    # clamp(500) = 100; dv = 100*0.5*0.5 = 25 mV from -70 -> -45 >= -54 fires now
+    o.inject_current(pop, idx, np.full(len(idx), 500.0, np.float32))
+
+
+def test_push_delays_to_slots():
+    o = po.OracleNetwork(_push_net(6, [0, 1, 2], [3, 4, 5], [1.0, 2.0, 0.5], [0, 1, 5]))
+    _force_fire(o, 0, [0, 1, 2])
+    o.step()
+    assert list(o.active()) == [0, 1, 2]
+    ring = o.ring()
+    assert ring[1][3] == 1.0 and ring[1][4] == 2.0 and ring[5][5] == 0.5
+    assert ring[0][3] == 0.0 and ring[0][4] == 0.0
+    nz = {(s, i) for s, i in zip(*np.nonzero(ring))}
+    assert nz == {(1, 3), (1, 4), (5, 5)}
+
+
+def test_push_wraparound_and_step_counter():
+    o = po.OracleNetwork(_push_net(2, [0], [1], [1.0], [31]))
+    o.step()  # step 0: nothing fires
+    _force_fire(o, 0, [0])
+    o.step()  # step 1: fires, (1+31)%32 = 0
+    assert o.ring()[0][1] == 1.0
+    o2 = po.OracleNetwork(_push_net(2, [0], [1], [1.0], [2]))
+    o2.step(5)
+    _force_fire(o2, 0, [0])
+    o2.step()  # step 5: (5+2)%32 = 7
+    r = o2.ring()
+    assert r[7][1] == 1.0 and r[0][1] == 0.0
+
+
+def test_push_accumulates():
+    o = po.OracleNetwork(_push_net(3, [0, 1], [2, 2], [3.0, 4.0], [0, 0]))
+    _force_fire(o, 0, [0, 1])
+    o.step()
+    assert abs(o.ring()[1][2] - 7.0) < 1e-6
+
+
+def test_inject_same_step_arrival():
+    # build.rs:243-275 — current injected before step s drives step s
+    o = po.OracleNetwork(_push_net(2, [0], [1], [1.0], [0]))
+    o.inject_current(0, [1], [5.0])
+    assert o.ring()[0][1] == 5.0
+    o.step()
+    assert o.ring()[0][1] == 0.0  # read and cleared
+
+
+# --- delivery through Network::step: pillars/tests/network_delays.rs --------
+@pytest.mark.parametrize("n", [4, 80])
+def test_network_delays_exact_arrival(n):
+    delays = [0, 2, 5, 31] + [0] * (n - 4)
+    o = po.OracleNetwork(delay_net(delays))
+    raster = []
+    for step in range(70):
+        o.step()
+        s = o.spikes()
+        if step == 0:
+            assert s[:n].all()
+        raster.append(s[n:].copy())
+    raster = np.array(raster)
+    for i, d in enumerate(delays):
+        arrival = max(d, 1)
+        want = np.zeros(70, np.int32)
+        want[arrival] = 1
+        assert np.array_equal(raster[:, i], want), (i, d)
+    assert raster.sum() == n
+
+
+# --- LIF: pillars/tests/lif_vs_neuron.rs:77-145 + closed form ----------------
+@pytest.mark.parametrize("i_syn,ref", [(0.0, 0), (30.0, 51), (50.0, 83)])
+def test_lif_spike_counts_vs_reference(i_syn, ref):
+    dt, t_ms, n = 0.1, 500.0, 8
+    o = po.OracleNetwork(single_pop(lif_pop(n, dt, 0.1, -70.0, -54.0, -70.0, 2.0, -70.0), dt))
+    on = int(10.0 / dt)
+    counts = np.zeros(n)
+    for step in range(int(t_ms / dt)):
+        if step >= on:
+            o.inject_current(0, np.arange(n), np.full(n, i_syn, np.float32))
+        o.step()
+        counts += o.spikes()
+    assert abs(counts.mean() - ref) <= 2.0
+
+
+def test_lif_first_spike_matches_closed_form():
+    # forward Euler with constant I: v_k = v_inf + (v0 - v_inf)(1-a)^k, a = dt/tau
+    dt, tau, I = 0.1, 10.0, 30.0
+    v0, vth = -70.0, -54.0
+    o = po.OracleNetwork(single_pop(lif_pop(1, dt, 1.0 / tau, -70.0, vth, -70.0, 2.0, v0), dt))
+    a = dt / tau
+    v_inf = -70.0 + I
+    k_star = math.ceil(math.log((vth - v_inf) / (v0 - v_inf)) / math.log(1 - a))
+    for step in range(200):
+        o.inject_current(0, [0], [I])
+        o.step()
+        if o.spikes()[0]:
+            break
+    # step index s fires after s+1 updates
+    assert abs((step + 1) - k_star) <= 1
+
+
+# --- Izhikevich: pillars/tests/izhikevich_vs_neuron.rs:31-214 ---------------
+@pytest.mark.parametrize("label,a,b,c,d,i_syn,lo,hi,min_cv", [
+    ("RS", 0.02, 0.2, -65.0, 8.0, 10.0, 15, 30, 0.0),
+    ("FS", 0.1, 0.2, -65.0, 2.0, 30.0, 180, 330, 0.0),
+    ("IB", 0.02, 0.2, -55.0, 4.0, 10.0, 20, 55, 0.5),
+    ("CH", 0.02, 0.2, -50.0, 2.0, 15.0, 70, 165, 0.3),
+])
+def test_izhikevich_patterns(label, a, b, c, d, i_syn, lo, hi, min_cv):
+    dt, t_ms, n = 0.5, 1000.0, 8
+    o = po.OracleNetwork(single_pop(izh_pop(n, dt, a, b, c, d, 2.0, 30.0, c, 0.0), dt))
+    on = int(10.0 / dt)
+    per = np.zeros(n)
+    # ISIs pooled over the population in neuron order within a step, as the
+    # reference test computes them (izhikevich_vs_neuron.rs:129-148)
+    isi, last, first = [], 0.0, True
+    for step in range(int(t_ms / dt)):
+        if step >= on:
+            o.inject_current(0, np.arange(n), np.full(n, i_syn, np.float32))
+        o.step()
+        s = o.spikes()
+        per += s
+        for j in range(n):
+            if s[j]:
+                t_now = step * dt
+                if not first:
+                    isi.append(t_now - last)
+                last, first = t_now, False
+    assert lo <= per.mean() <= hi, (label, per.mean())
+    isi = np.array(isi, np.float32)
+    cv = float(np.sqrt(np.mean((isi - isi.mean()) ** 2)) / max(isi.mean(), 1e-6)) if isi.size > 2 else 0.0
+    assert cv >= min_cv
+
+
+def test_izhikevich_fp32_tracks_fp64():
+    dt = 0.5
+    spec = single_pop(izh_pop(4, dt, 0.02, 0.2, -65.0, 8.0, 2.0, 30.0, -65.0, -13.0), dt)
+    o32, o64 = po.OracleNetwork(spec, 32), po.OracleNetwork(spec, 64)
+    c32 = c64 = 0
+    for _ in range(400):
+        for o in (o32, o64):
+            o.inject_current(0, np.arange(4), np.full(4, 10.0, np.float32))
+            o.step()
+        c32 += o32.spikes().sum()
+        c64 += o64.spikes().sum()
+    assert c32 == c64 and c32 > 0
+
+
+# --- counter-based inputs ------------------------------------------------------
+def test_poisson_roll_rate_and_independence():
+    # P/src/api/net/network/sources.rs:5-14: u < p with u uniform on 24 bits
+    p, T = 0.2, 20000
+    for seed in (1, 7, 12345):
+        k = sum(po.poisson_spike(seed, 0, t, p) for t in range(T))
+        assert abs(k - p * T) < 4 * math.sqrt(T * p * (1 - p))
+    a = [po.poisson_spike(1, 0, t, 0.5) for t in range(512)]
+    b = [po.poisson_spike(2, 0, t, 0.5) for t in range(512)]
+    assert a != b
+    assert not any(po.poisson_spike(3, 0, t, 0.0) for t in range(100))
+    assert all(po.poisson_spike(3, 0, t, 1.0) for t in range(100))
+
+
+def test_gauss_moments():
+    z = np.array([po.gauss(42, i, t) for i in range(200) for t in range(500)])
+    assert abs(z.mean()) < 0.01
+    assert abs(z.var() - 1.0) < 0.015
+    assert abs(np.mean(np.abs(z) > 2.0) - 0.0455) < 0.004
+    kurt = np.mean(z ** 4) / z.var() ** 2
+    assert abs(kurt - 3.0) < 0.1
+
+
+def test_exp_ln_accuracy():
+    xs = np.linspace(-80, 80, 20001).astype(np.float32)
+    got = np.array([po.exp_f32(float(x)) for x in xs])
+    want = np.exp(xs.astype(np.float64))
+    assert np.max(np.abs(got - want) / want) < 4e-7
+    ys = np.geomspace(1e-12, 1.0, 20001).astype(np.float32)
+    got = np.array([po.ln_f32(float(y)) for y in ys])
+    want = np.log(ys.astype(np.float64))
+    assert np.max(np.abs(got - want) / np.maximum(np.abs(want), 1.0)) < 4e-7
+
+
+def test_vtrap_known_answers_and_window():
+    # P/src/math/vtrap.rs:73-117
+    assert abs(po.vtrap_f32(0.0, 10.0) - 10.0) < 0.01
+    assert abs(po.vtrap_f32(10.0, 1.0) - 10.0 / (math.exp(10.0) - 1.0)) < 1e-3
+    assert abs(po.vtrap_f32(-10.0, 1.0) - (-10.0) / (math.exp(-10.0) - 1.0)) < 1e-3
+    for x in (-0.01, -0.001, 0.0, 0.001, 0.01):
+        assert math.isfinite(po.vtrap_f32(x, 10.0))
+    # FP32 value vs FP64 formula across the 0/0 window and outside it
+    for x in np.linspace(-60, 60, 4001):
+        x = float(np.float32(x))
+        want = po.vtrap_f64(x, 10.0)
+        assert abs(po.vtrap_f32(x, 10.0) - want) <= 2e-6 * max(abs(want), 1.0)
+
+
+# --- STDP: pillars/src/ops/plasticity/stdp.rs:182-216 + closed form --------
+def _stdp_pair(k, pre_first, tau=20.0, a_plus=0.01, a_minus=0.012, w0=0.5):
+    dt = 0.5
+    pops = [lif_pop(2, dt, 0.5, -70.0, -54.0, -70.0, 0.0, [100.0 if pre_first else -70.0,
+                                                           -70.0 if pre_first else 100.0])]
+    rp, col, w, d = csr_from_triplets(2, [0], [1], np.array([w0], np.float32), np.array([5], np.uint8))
+    spec = NetSpec("stdp", pops, rp, col, w, d, dt, 0)
+    spec.stdp = dict(a_plus=a_plus, a_minus=a_minus, tau_plus=tau, tau_minus=tau, w_min=0.0, w_max=1.0)
+    o = po.OracleNetwork(spec)
+    o.step()  # step 0: the forced neuron fires
+    o.step(k - 1)
+    _force_fire(o, 0, [1 if pre_first else 0])
+    o.step()  # step k: the partner fires
+    return o.weights()[0]
+
+
+@pytest.mark.parametrize("k", [1, 3, 10])
+def test_stdp_pair_closed_form(k):
+    dt, tau = 0.5, 20.0
+    dec = float(np.float32(math.exp(-dt / tau)))
+    assert abs(_stdp_pair(k, True) - (0.5 + 0.01 * dec ** k)) < 1e-6      # pre then post: LTP
+    assert abs(_stdp_pair(k, False) - (0.5 - 0.012 * dec ** k)) < 1e-6    # post then pre: LTD
+
+
+def test_stdp_sign_known_answers():
+    # stdp.rs:182-216: lr 0.1, trace 0.8 -> 0.5 becomes 0.58 (LTP) / 0.42 (LTD)
+    tau = 0.5 / -math.log(0.8)
+    assert abs(_stdp_pair(1, True, tau=tau, a_plus=0.1) - 0.58) < 1e-6
+    assert abs(_stdp_pair(1, False, tau=tau, a_minus=0.1) - 0.42) < 1e-6
+
+
+# --- statistics: pillars/src/kernels/stats/{rates,isi}.rs -----------------------
+def test_rates_and_cv_brute_force():
+    rng = np.random.default_rng(3)
+    T, n, dt_s = 3000, 40, 0.001
+    r = (rng.random((T, n)) < rng.uniform(0.001, 0.05, n)).astype(np.int32)
+    rates = po.firing_rates(r, dt_s)
+    assert np.allclose(rates, r.sum(0) / (T * dt_s), rtol=1e-5)
+    cvs = po.isi_cv(r, dt_s)
+    for i in range(n):
+        t = np.nonzero(r[:, i])[0]
+        isi = np.diff(t).astype(np.float64)
+        want = isi.std(ddof=1) / isi.mean() if isi.size >= 2 else 0.0
+        assert abs(cvs[i] - want) < 1e-4 * max(1.0, want)
+
+
+# --- Kuramoto: tests/kuramoto_tests.rs:22-94 + closed forms -----------------
+def _all_to_all(n, k):
+    rp = np.arange(n + 1, dtype=np.uint32) * (n - 1)
+    col = np.array([j for i in range(n) for j in range(n) if j != i], np.uint32)
+    return rp, col, np.full(col.size, k / n, np.float32)
+
+
+def test_kuramoto_sync_from_zero():
+    n = 8
+    rp, col, w = _all_to_all(n, 1.0)
+    k = po.OracleKuramoto(rp, col, w, np.ones(n, np.float32), np.zeros(n, np.float32), 0.5)
+    k.euler(100)
+    p = k.phases()
+    assert np.all(np.abs(p - p[0]) < 0.5)
+
+
+def test_kuramoto_identical_frequencies_synchronise():
+    n = 64
+    rng = np.random.default_rng(1)
+    rp, col, w = _all_to_all(n, 2.0)
+    k = po.OracleKuramoto(rp, col, w, np.full(n, 0.3, np.float32),
+                          rng.uniform(-1.5, 1.5, n).astype(np.float32), 0.05)
+    r0 = k.order()
+    k.rk4(2000)
+    assert k.order() > 0.999 > r0
+
+
+def test_kuramoto_uncoupled_is_linear():
+    n = 16
+    rng = np.random.default_rng(2)
+    om = rng.normal(size=n).astype(np.float32)
+    th = rng.uniform(-3, 3, n).astype(np.float32)
+    rp = np.zeros(n + 1, np.uint32)
+    k = po.OracleKuramoto(rp, np.zeros(0, np.uint32), np.zeros(0, np.float32), om, th, 0.01)
+    k.rk4(1000)
+    want = th.astype(np.float64) + om.astype(np.float64) * (1000 * float(np.float32(0.01)))
+    diff = np.angle(np.exp(1j * (k.phases() - want)))
+    assert np.max(np.abs(diff)) < 1e-9
+
+
+def test_kuramoto_rk4_fourth_order():
+    n = 12
+    rng = np.random.default_rng(4)
+    rp, col, w = _all_to_all(n, 1.5)
+    om = rng.normal(size=n).astype(np.float32)
+    th = rng.uniform(-3, 3, n).astype(np.float32)
+
+    def run(dt, steps):
+        k = po.OracleKuramoto(rp, col, w, om, th, dt)
+        k.rk4(steps)
+        return k.phases()
+
+    ref = run(0.0025, 800)
+    e1 = np.max(np.abs(np.angle(np.exp(1j * (run(0.04, 50) - ref)))))
+    e2 = np.max(np.abs(np.angle(np.exp(1j * (run(0.02, 100) - ref)))))
+    assert 12.0 < e1 / e2 < 20.0
+
+
+# --- Hodgkin–Huxley: textbook closed forms -----------------------------------
+def test_hh_resting_gates_are_steady_state():
+    # m_inf, h_inf, n_inf at -65 mV equal the initial gates the reference uses
+    # (pillars-connectome/src/sim/hh.rs:34-38)
+    v = -65.0
+    am = 0.1 * (v + 40) / (1 - math.exp(-(v + 40) / 10))
+    bm = 4 * math.exp(-(v + 65) / 18)
+    ah = 0.07 * math.exp(-(v + 65) / 20)
+    bh = 1 / (1 + math.exp(-(v + 35) / 10))
+    an = 0.01 * (v + 55) / (1 - math.exp(-(v + 55) / 10))
+    bn = 0.125 * math.exp(-(v + 65) / 80)
+    assert abs(am / (am + bm) - 0.0529) < 1e-3
+    assert abs(ah / (ah + bh) - 0.596) < 1e-3
+    assert abs(an / (an + bn) - 0.318) < 1e-3
+    # with the squid-axon leak reversal the membrane stays at rest
+    dt = 0.01
+    spec = single_pop(hh_pop(2, dt, e_leak=-54.387, tau_ref=2.0), dt)
+    for kind in (32, 64):
+        o = po.OracleNetwork(spec, kind)
+        o.step(2000)
+        assert np.all(np.abs(o.state(0, 0) + 65.0) < 0.5)
+        assert o.counters()[0] == 0
+
+
+def test_hh_repetitive_firing_frequency():
+    # textbook HH at 6.3 C fires ~68 Hz for a 10 uA/cm^2 step
+    dt = 0.01
+    spec = single_pop(hh_pop(1, dt, e_leak=-54.387, tau_ref=1.0), dt)
+    res = {}
+    for kind in (32, 64):
+        o = po.OracleNetwork(spec, kind)
+        times = []
+        for step in range(50000):
+            o.inject_current(0, [0], [10.0])
+            o.step()
+            if o.spikes()[0]:
+                times.append(step)
+        res[kind] = times
+        isi = np.diff(times[2:]) * dt
+        assert 60.0 < 1000.0 / isi.mean() < 76.0
+    assert len(res[32]) == len(res[64])
+    assert np.max(np.abs(np.array(res[32]) - np.array(res[64]))) <= 2
