@@ -28,6 +28,7 @@ HIPCC_FLAGS = [
     "-fno-fast-math",
     "-fPIC",
     "-shared",
+    "-Wl,--no-undefined",
     "-Wall",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]

@@ -1,0 +1,474 @@
+// engine.hpp — host-side runtime shared by the single-network, batch and
+// serving C APIs.
+//
+// NetState: every device buffer of one network plus its host bookkeeping.
+// Engine:   one HIP stream stepping a set of networks together.  The step
+//           kernels read a device array of NetArgs (one per network) and
+//           per-launch work lists of (network, tile) pairs, so one launch
+//           covers every network of the set.  Each network keeps its own
+//           ring, step counter and RNG keys, so its results do not depend on
+//           which other networks share the launch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/simuse.h"
+#include "net_kernels.hpp"
+
+namespace simuse {
+
+int fail(int code, const char* fmt, ...);
+
+void launch_neuron(int model, const NetArgs* nets, const uint2* tiles, uint32_t n, hipStream_t s);
+void launch_stdp(const NetArgs* nets, const uint2* tiles, uint32_t n, hipStream_t s);
+void launch_insert(int sched, const NetArgs* nets, const uint2* blocks, uint32_t n, hipStream_t s);
+void launch_fold(int sched, const NetArgs* nets, const uint2* blocks, uint32_t n, hipStream_t s);
+void launch_inject(float* ring, const uint64_t* step, uint32_t N, const uint32_t* idx, const float* val,
+                   uint32_t count, int serial, hipStream_t s);
+void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+void launch_widen(const float* w, double* w64, size_t n, hipStream_t s);
+void launch_raster_stats(const uint64_t* raster, uint32_t n_words, uint32_t steps, const uint32_t* word_of,
+                         const uint32_t* bit_of, uint32_t count, float dt_s, float* rates, float* cvs,
+                         hipStream_t s);
+
+#define HIP_TRY(expr)                                                                           \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return ::simuse::fail(SIMUSE_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+inline int model_states(int m) { return m == SIMUSE_MODEL_LIF ? 2 : m == SIMUSE_MODEL_IZHIKEVICH ? 3 : 6; }
+inline int model_params(int m) { return m == SIMUSE_MODEL_LIF ? 6 : m == SIMUSE_MODEL_IZHIKEVICH ? 7 : 13; }
+// column of tau_ref and of the refractory state (LIF/Izhikevich only)
+inline int model_tau_ref_col(int m) { return m == SIMUSE_MODEL_HH ? -1 : 5; }
+inline int model_refr_state(int m) { return m == SIMUSE_MODEL_LIF ? 1 : m == SIMUSE_MODEL_IZHIKEVICH ? 2 : -1; }
+
+struct HostPop {
+    int model = 0;
+    uint32_t n = 0, ns = 0, np = 0, host_stride = 0;
+    uint32_t off = 0, tile0 = 0, ntiles = 0, word0 = 0, dstride = 0;
+    uint32_t uni_mask = 0, flags = 0;
+    float puni[kMaxParams] = {};
+    float* st = nullptr;
+    float* pr = nullptr;
+    float* sigma = nullptr;
+    float sigma_uni = 0.0f;
+    uint64_t seed = 0;
+};
+
+struct SourceHost {
+    float p;
+    uint32_t seed;
+    std::vector<uint32_t> post;
+    std::vector<float> w;
+    std::vector<uint8_t> d;
+};
+
+struct Engine;
+
+// Validation shared by create and reset.
+int validate_pops(const simuse_pop_desc* pops, int32_t n_pops, uint64_t* n_total);
+int validate_csr(uint64_t n_total, const uint32_t* row_ptr, const uint32_t* col, const float* w,
+                 const uint8_t* delay, uint64_t n_syn);
+
+struct NetState {
+    Engine* eng = nullptr;
+    std::vector<HostPop> pops;
+    uint32_t N = 0, n_tiles = 0, n_words = 0;
+    uint64_t n_syn = 0;
+    std::vector<void*> allocs;
+    uint64_t bytes = 0;
+    PopDev* d_pops = nullptr;
+    uint32_t* d_tile_pop = nullptr;
+    float* ring = nullptr;
+    uint32_t* head = nullptr;
+    uint32_t* next = nullptr;
+    uint32_t* row_ptr = nullptr;
+    uint32_t* col = nullptr;
+    float* w = nullptr;
+    uint8_t* delay = nullptr;
+    uint32_t* tile_count = nullptr;
+    uint32_t* active = nullptr;
+    uint64_t* bits = nullptr;
+    uint64_t* tile_spikes = nullptr;
+    uint64_t* tile_events = nullptr;
+    uint64_t* step = nullptr;  // [0] step, [1] raster base
+    uint32_t* done = nullptr;
+    std::vector<SourceHost> srcs;
+    uint32_t n_src = 0, n_src_tiles = 0;
+    uint64_t n_src_edges = 0;
+    uint32_t* src_row_ptr = nullptr;
+    uint32_t* src_post = nullptr;
+    float* src_w = nullptr;
+    uint8_t* src_delay = nullptr;
+    float* src_p = nullptr;
+    uint32_t* src_seed = nullptr;
+    uint64_t* raster = nullptr;
+    uint32_t raster_cap = 0;
+    uint64_t raster_start = 0;
+    bool stdp = false;
+    simuse_stdp_params sp{};
+    double* w64 = nullptr;
+    float* tr_pre = nullptr;
+    float* tr_post = nullptr;
+    uint32_t* in_ptr = nullptr;
+    uint32_t* in_edge = nullptr;
+    uint32_t* edge_pre = nullptr;
+    uint8_t* plastic = nullptr;
+    float stdp_dpre = 0, stdp_dpost = 0;
+    bool finalized = false;
+    uint64_t host_step = 0;
+    uint64_t step_limit = std::numeric_limits<uint64_t>::max();
+    uint32_t* inj_idx = nullptr;
+    float* inj_val = nullptr;
+    uint32_t inj_cap = 0;
+    std::vector<uint32_t> h_row_ptr;
+
+    NetState() = default;
+    NetState(const NetState&) = delete;
+    NetState& operator=(const NetState&) = delete;
+    ~NetState() {
+        for (void* p : allocs) (void)hipFree(p);
+    }
+
+    template <class T>
+    int alloc(T** p, size_t count) {
+        *p = nullptr;
+        if (count == 0) count = 1;
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, count * sizeof(T));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(SIMUSE_E_NOMEM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T),
+                        hipGetErrorString(e));
+        }
+        allocs.push_back(q);
+        bytes += count * sizeof(T);
+        *p = static_cast<T*>(q);
+        return SIMUSE_OK;
+    }
+
+    NetArgs args() const {
+        NetArgs a{};
+        a.pops = d_pops;
+        a.tile_pop = d_tile_pop;
+        a.N = N;
+        a.n_tiles = n_tiles;
+        a.n_src_tiles = n_src_tiles;
+        a.n_src = n_src;
+        a.n_blocks = n_tiles + n_src_tiles;
+        a.n_syn = n_syn;
+        a.step_limit = step_limit;
+        a.ring = ring;
+        a.head = head;
+        a.next = next;
+        a.row_ptr = row_ptr;
+        a.col = col;
+        a.w = w;
+        a.delay = delay;
+        a.src_row_ptr = src_row_ptr;
+        a.src_post = src_post;
+        a.src_w = src_w;
+        a.src_delay = src_delay;
+        a.src_p = src_p;
+        a.src_seed = src_seed;
+        a.tile_count = tile_count;
+        a.active = active;
+        a.bits = bits;
+        a.tile_spikes = tile_spikes;
+        a.tile_events = tile_events;
+        a.step = step;
+        a.done = done;
+        a.raster = raster;
+        a.raster_cap = raster_cap;
+        a.n_words = n_words;
+        a.raster_base = step + 1;
+        a.stdp = stdp ? 1 : 0;
+        a.w64 = w64;
+        a.tr_pre = tr_pre;
+        a.tr_post = tr_post;
+        a.in_ptr = in_ptr;
+        a.in_edge = in_edge;
+        a.edge_pre = edge_pre;
+        a.plastic = plastic;
+        a.stdp_dpre = stdp_dpre;
+        a.stdp_dpost = stdp_dpost;
+        a.a_plus = sp.a_plus;
+        a.a_minus = sp.a_minus;
+        a.w_min = sp.w_min;
+        a.w_max = sp.w_max;
+        return a;
+    }
+
+    // All creation work after validation.  `stream` orders the device fills.
+    int init(const simuse_pop_desc* pd, int32_t n_pops, const uint32_t* rp, const uint32_t* c, const float* wf,
+             const uint8_t* dl, uint64_t nsyn, hipStream_t stream);
+    int finalize(hipStream_t stream);
+    int upload_pops(const simuse_pop_desc* pd);
+    int upload_pop_table();
+    int reset_dynamic(hipStream_t stream);
+    int add_source(float rate_hz, float dt_ms, uint32_t seed, const uint32_t* posts, const float* wf,
+                   const uint8_t* dl, uint32_t count);
+    int set_noise(int32_t pop, const float* sigma, uint64_t seed);
+    int set_plasticity(const simuse_stdp_params* p, const uint8_t* pl, hipStream_t stream);
+    int download_spikes(int32_t* out, uint64_t len, hipStream_t stream);
+    int download_spike_ids(uint32_t* ids, uint32_t cap, uint32_t* count, hipStream_t stream);
+    int download_state(int32_t pop, float* out, uint64_t len, hipStream_t stream);
+    int download_ring(float* out, uint64_t len, hipStream_t stream);
+    int upload_weights(const float* wf, uint64_t n, hipStream_t stream);
+    int download_weights(float* wf, uint64_t n, hipStream_t stream);
+    int inject(int32_t pop, const uint32_t* idx, const float* val, uint32_t count, hipStream_t stream);
+    int reset(const simuse_pop_desc* pd, int32_t n_pops, hipStream_t stream);
+    int counters(uint64_t* spikes, uint64_t* events, hipStream_t stream);
+    int record(uint32_t max_steps, hipStream_t stream);
+    int download_raster(uint64_t* out, uint64_t n_words_buf, uint32_t* steps, hipStream_t stream);
+    int raster_stats(uint32_t first, uint32_t count, float dt_s, float* rates, float* cvs, hipStream_t stream);
+};
+
+struct Engine {
+    hipStream_t stream = nullptr;
+    std::vector<NetState*> nets;
+    bool dirty = true;
+    // device tables
+    NetArgs* d_nets = nullptr;
+    uint32_t nets_cap = 0;
+    uint2* d_tiles[3] = {nullptr, nullptr, nullptr};
+    uint32_t n_tiles_m[3] = {0, 0, 0}, cap_tiles_m[3] = {0, 0, 0};
+    uint2* d_blocks = nullptr;
+    uint32_t n_blocks = 0, cap_blocks = 0;
+    uint2* d_stdp = nullptr;
+    uint32_t n_stdp = 0, cap_stdp = 0;
+    // execution
+    int sched = SIMUSE_SCATTER_TILE;
+    uint32_t graph_steps = 8;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    static constexpr int kPhases = 4;  // neuron, stdp, insert, fold
+    bool profiling = false;
+    std::vector<hipEvent_t> ev;
+    uint32_t prof_steps = 0;
+
+    Engine() = default;
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+    ~Engine() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        drop_graph();
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        for (void* p : {static_cast<void*>(d_nets), static_cast<void*>(d_tiles[0]), static_cast<void*>(d_tiles[1]),
+                        static_cast<void*>(d_tiles[2]), static_cast<void*>(d_blocks), static_cast<void*>(d_stdp)})
+            if (p) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    int init() {
+        // a blocking stream is ordered after the synchronous null-stream uploads
+        HIP_TRY(hipStreamCreate(&stream));
+        return SIMUSE_OK;
+    }
+    void drop_graph() {
+        if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
+        if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+    }
+    void mark_dirty() {
+        dirty = true;
+        drop_graph();
+    }
+
+    template <class T>
+    static int grow(T** p, uint32_t* cap, uint32_t need) {
+        if (need <= *cap && *p) return SIMUSE_OK;
+        uint32_t nc = std::max<uint32_t>(need, std::max<uint32_t>(64, *cap * 2));
+        if (*p) { (void)hipFree(*p); *p = nullptr; *cap = 0; }
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, sizeof(T) * static_cast<size_t>(nc));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(SIMUSE_E_NOMEM, "engine table allocation failed: %s", hipGetErrorString(e));
+        }
+        *p = static_cast<T*>(q);
+        *cap = nc;
+        return SIMUSE_OK;
+    }
+
+    // Rebuild the NetArgs array and the (network, tile) work lists.
+    int ensure_tables() {
+        if (!dirty) return SIMUSE_OK;
+        HIP_TRY(hipStreamSynchronize(stream));
+        std::vector<NetArgs> na;
+        std::vector<uint2> tl[3], bl, sl;
+        for (uint32_t k = 0; k < nets.size(); ++k) {
+            NetState* s = nets[k];
+            na.push_back(s->args());
+            for (const auto& P : s->pops)
+                for (uint32_t t = 0; t < P.ntiles; ++t) tl[P.model].push_back(make_uint2(k, P.tile0 + t));
+            for (uint32_t b = 0; b < s->n_tiles + s->n_src_tiles; ++b) bl.push_back(make_uint2(k, b));
+            if (s->stdp)
+                for (uint32_t t = 0; t < s->n_tiles; ++t) sl.push_back(make_uint2(k, t));
+        }
+        int rc;
+        if (!na.empty()) {
+            if ((rc = grow(&d_nets, &nets_cap, static_cast<uint32_t>(na.size())))) return rc;
+            HIP_TRY(hipMemcpy(d_nets, na.data(), sizeof(NetArgs) * na.size(), hipMemcpyHostToDevice));
+        }
+        for (int m = 0; m < 3; ++m) {
+            n_tiles_m[m] = static_cast<uint32_t>(tl[m].size());
+            if (tl[m].empty()) continue;
+            if ((rc = grow(&d_tiles[m], &cap_tiles_m[m], n_tiles_m[m]))) return rc;
+            HIP_TRY(hipMemcpy(d_tiles[m], tl[m].data(), sizeof(uint2) * tl[m].size(), hipMemcpyHostToDevice));
+        }
+        n_blocks = static_cast<uint32_t>(bl.size());
+        if (n_blocks) {
+            if ((rc = grow(&d_blocks, &cap_blocks, n_blocks))) return rc;
+            HIP_TRY(hipMemcpy(d_blocks, bl.data(), sizeof(uint2) * bl.size(), hipMemcpyHostToDevice));
+        }
+        n_stdp = static_cast<uint32_t>(sl.size());
+        if (n_stdp) {
+            if ((rc = grow(&d_stdp, &cap_stdp, n_stdp))) return rc;
+            HIP_TRY(hipMemcpy(d_stdp, sl.data(), sizeof(uint2) * sl.size(), hipMemcpyHostToDevice));
+        }
+        drop_graph();
+        dirty = false;
+        return SIMUSE_OK;
+    }
+
+    void launch_step(hipEvent_t* e) {
+        if (e) (void)hipEventRecord(e[0], stream);
+        for (int m = 0; m < 3; ++m) launch_neuron(m, d_nets, d_tiles[m], n_tiles_m[m], stream);
+        if (e) (void)hipEventRecord(e[1], stream);
+        launch_stdp(d_nets, d_stdp, n_stdp, stream);
+        if (e) (void)hipEventRecord(e[2], stream);
+        launch_insert(sched, d_nets, d_blocks, n_blocks, stream);
+        if (e) (void)hipEventRecord(e[3], stream);
+        launch_fold(sched, d_nets, d_blocks, n_blocks, stream);
+        if (e) (void)hipEventRecord(e[4], stream);
+    }
+
+    int enqueue_steps(uint32_t n) {
+        for (uint32_t k = 0; k < n; ++k) launch_step(nullptr);
+        HIP_TRY(hipGetLastError());
+        return SIMUSE_OK;
+    }
+
+    int build_graph() {
+        drop_graph();
+        HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_steps(graph_steps);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(stream, &g);
+        if (rc != SIMUSE_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(SIMUSE_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        graph = g;
+        e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            drop_graph();
+            return fail(SIMUSE_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+        }
+        return SIMUSE_OK;
+    }
+
+    int prepare() {
+        int rc;
+        for (NetState* s : nets)
+            if (!s->finalized) {
+                if ((rc = s->finalize(stream))) return rc;
+                dirty = true;
+            }
+        return ensure_tables();
+    }
+
+    // Enqueue n steps of every network of the set.
+    int step_async(uint32_t n) {
+        int rc = prepare();
+        if (rc) return rc;
+        if (n == 0 || nets.empty()) return SIMUSE_OK;
+        if (profiling) {
+            const size_t need = static_cast<size_t>(prof_steps + n) * (kPhases + 1);
+            while (ev.size() < need) {
+                hipEvent_t e;
+                HIP_TRY(hipEventCreate(&e));
+                ev.push_back(e);
+            }
+            for (uint32_t k = 0; k < n; ++k)
+                launch_step(ev.data() + static_cast<size_t>(prof_steps + k) * (kPhases + 1));
+            prof_steps += n;
+            HIP_TRY(hipGetLastError());
+        } else {
+            const uint32_t full = n / graph_steps, rest = n % graph_steps;
+            if (full) {
+                if (!gexec && (rc = build_graph())) return rc;
+                for (uint32_t k = 0; k < full; ++k) HIP_TRY(hipGraphLaunch(gexec, stream));
+            }
+            if (rest && (rc = enqueue_steps(rest))) return rc;
+        }
+        for (NetState* s : nets) s->host_step = std::min(s->host_step + n, std::max(s->host_step, s->step_limit));
+        return SIMUSE_OK;
+    }
+
+    int sync() {
+        HIP_TRY(hipStreamSynchronize(stream));
+        return SIMUSE_OK;
+    }
+
+    int set_profiling(bool on) {
+        HIP_TRY(hipStreamSynchronize(stream));
+        profiling = on;
+        prof_steps = 0;
+        return SIMUSE_OK;
+    }
+
+    int kernel_times(float* ms, int32_t n) {
+        if (!ms || n < 1) return fail(SIMUSE_E_INVALID, "null output");
+        HIP_TRY(hipStreamSynchronize(stream));
+        double acc[kPhases] = {0, 0, 0, 0};
+        for (uint32_t k = 0; k < prof_steps; ++k) {
+            hipEvent_t* e = ev.data() + static_cast<size_t>(k) * (kPhases + 1);
+            for (int p = 0; p < kPhases; ++p) {
+                float t = 0;
+                HIP_TRY(hipEventElapsedTime(&t, e[p], e[p + 1]));
+                acc[p] += t;
+            }
+        }
+        for (int p = 0; p < n && p < kPhases; ++p)
+            ms[p] = prof_steps ? static_cast<float>(acc[p] / prof_steps) : 0.0f;
+        prof_steps = 0;
+        return SIMUSE_OK;
+    }
+
+    int set_sched(int mode) {
+        if (mode != SIMUSE_SCATTER_TILE && mode != SIMUSE_SCATTER_WAVE)
+            return fail(SIMUSE_E_INVALID, "unknown scatter mode %d", mode);
+        if (sched != mode) {
+            sched = mode;
+            drop_graph();
+        }
+        return SIMUSE_OK;
+    }
+
+    int set_graph_steps(uint32_t steps) {
+        if (steps < 1 || steps > 64) return fail(SIMUSE_E_INVALID, "graph steps must be 1..64");
+        if (graph_steps != steps) {
+            graph_steps = steps;
+            drop_graph();
+        }
+        return SIMUSE_OK;
+    }
+
+    int autotune(uint32_t trial_steps);
+};
+
+}  // namespace simuse

@@ -1,0 +1,428 @@
+// kuramoto.hip — Kuramoto phase coupling on a CSR graph, Euler or RK4.
+//
+// Coupling of node i: c_i = sum_j W_ij sin(theta_j - theta_i)
+//                         = cos_i * sum_j W_ij sin_j - sin_i * sum_j W_ij cos_j
+// (P/src/kernels/phase/sparse.rs:63-96 computes the same sum as
+// w * (sin_j cos_i - cos_j sin_i); GPU twin PC/kernels/kuramoto.cu:32-75).
+// The reference integrates with forward Euler and wraps to [-pi, pi)
+// (P/src/kernels/phase/kuramoto.rs:42-68, P/src/kernels/mod.rs:74-77); the
+// RK4 integrator is this engine's extension for long horizons.
+//
+// Precision: the phase is accumulated in FP64 (theta, 8 B/node); stage phases,
+// their sin/cos and the derivatives are FP32.  One kernel per RK4 stage:
+// it gathers the current stage's (sin, cos) over the row, forms the stage
+// derivative, folds it into the RK4 accumulator, and writes (sin, cos) of the
+// next stage's phase into the other buffer.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/simuse.h"
+
+namespace simuse {
+int fail(int code, const char* fmt, ...);
+}
+
+namespace {
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+struct KuraArgs {
+    uint32_t n;
+    const uint32_t* row_ptr;
+    const uint32_t* col;
+    const float* w;
+    const float* omega;
+    double* theta;        // FP64 master phase, wrapped to [-pi, pi)
+    float* acc;           // RK4 accumulator k1 + 2k2 + 2k3
+    const float2* sc_in;  // (sin, cos) of the current stage phase
+    float2* sc_out;       // (sin, cos) of the next stage phase
+    double dt;
+};
+
+__device__ __forceinline__ double wrap_phase(double p) {
+    // p - 2pi * round(p / 2pi), round half away from zero (Rust f32::round)
+    return p - kTwoPi * round(p / kTwoPi);
+}
+
+// Gather sum_j w_ij (sin_j, cos_j) for row i with G lanes per row.
+template <int G>
+__device__ __forceinline__ float2 row_gather(const KuraArgs& a, uint32_t i, uint32_t sub) {
+    float s = 0.0f, c = 0.0f;
+    const uint32_t r0 = a.row_ptr[i], r1 = a.row_ptr[i + 1];
+    for (uint32_t e = r0 + sub; e < r1; e += G) {
+        const float wv = a.w[e];
+        const float2 sc = a.sc_in[a.col[e]];
+        s += wv * sc.x;
+        c += wv * sc.y;
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o, G);
+        c += __shfl_xor(c, o, G);
+    }
+    return make_float2(s, c);
+}
+
+// STAGE: 0 = Euler step; 1..4 = RK4 stages.
+template <int G, int STAGE>
+__global__ __launch_bounds__(256) void kura_stage_kernel(KuraArgs a) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = gid / G;
+    const uint32_t sub = gid % G;
+    if (i >= a.n) return;  // whole groups exit together (blockDim % G == 0)
+    const float2 sum = row_gather<G>(a, i, sub);
+    if (sub != 0) return;
+    const float2 me = a.sc_in[i];
+    const float k = a.omega[i] + (me.y * sum.x - me.x * sum.y);
+    const double th = a.theta[i];
+    double next;
+    if (STAGE == 0) {
+        next = wrap_phase(th + a.dt * static_cast<double>(k));
+        a.theta[i] = next;
+    } else if (STAGE == 1) {
+        a.acc[i] = k;
+        next = th + 0.5 * a.dt * static_cast<double>(k);
+    } else if (STAGE == 2) {
+        a.acc[i] += 2.0f * k;
+        next = th + 0.5 * a.dt * static_cast<double>(k);
+    } else if (STAGE == 3) {
+        a.acc[i] += 2.0f * k;
+        next = th + a.dt * static_cast<double>(k);
+    } else {
+        const double inc = a.dt / 6.0 * (static_cast<double>(a.acc[i]) + static_cast<double>(k));
+        next = wrap_phase(th + inc);
+        a.theta[i] = next;
+    }
+    float sn, cs;
+    sincosf(static_cast<float>(next), &sn, &cs);
+    a.sc_out[i] = make_float2(sn, cs);
+}
+
+__global__ void kura_prepare_kernel(const double* theta, float2* sc, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float sn, cs;
+    sincosf(static_cast<float>(theta[i]), &sn, &cs);
+    sc[i] = make_float2(sn, cs);
+}
+
+// Order parameter R = |sum_i e^{i theta_i}| / n, deterministic two-pass sum
+// in FP64 (P/src/kernels/measure/mod.rs:76-106 with unit amplitudes).
+__global__ void kura_order_partial(const double* theta, uint32_t n, double2* part) {
+    __shared__ double ss[256], cc[256];
+    double s = 0, c = 0;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        double sn, cs;
+        sincos(theta[i], &sn, &cs);
+        s += sn;
+        c += cs;
+    }
+    ss[threadIdx.x] = s;
+    cc[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < static_cast<uint32_t>(o)) {
+            ss[threadIdx.x] += ss[threadIdx.x + o];
+            cc[threadIdx.x] += cc[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = make_double2(ss[0], cc[0]);
+}
+
+template <int G>
+void launch_stage(int stage, const KuraArgs& a, hipStream_t s) {
+    const uint64_t threads = static_cast<uint64_t>(a.n) * G;
+    const uint32_t blocks = static_cast<uint32_t>((threads + 255) / 256);
+    switch (stage) {
+        case 0: kura_stage_kernel<G, 0><<<blocks, 256, 0, s>>>(a); break;
+        case 1: kura_stage_kernel<G, 1><<<blocks, 256, 0, s>>>(a); break;
+        case 2: kura_stage_kernel<G, 2><<<blocks, 256, 0, s>>>(a); break;
+        case 3: kura_stage_kernel<G, 3><<<blocks, 256, 0, s>>>(a); break;
+        default: kura_stage_kernel<G, 4><<<blocks, 256, 0, s>>>(a); break;
+    }
+}
+
+void launch_stage_g(int g, int stage, const KuraArgs& a, hipStream_t s) {
+    switch (g) {
+        case 1: launch_stage<1>(stage, a, s); break;
+        case 2: launch_stage<2>(stage, a, s); break;
+        case 4: launch_stage<4>(stage, a, s); break;
+        case 8: launch_stage<8>(stage, a, s); break;
+        case 16: launch_stage<16>(stage, a, s); break;
+        case 32: launch_stage<32>(stage, a, s); break;
+        default: launch_stage<64>(stage, a, s); break;
+    }
+}
+
+}  // namespace
+
+using simuse::fail;
+
+#define KHIP(expr)                                                                        \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess) return fail(SIMUSE_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+struct simuse_kura {
+    hipStream_t stream = nullptr;
+    uint32_t n = 0;
+    uint64_t n_edges = 0;
+    int method = SIMUSE_KURA_RK4;
+    int group = 4;
+    double dt = 0;
+    std::vector<void*> allocs;
+    uint32_t *row_ptr = nullptr, *col = nullptr;
+    float *w = nullptr, *omega = nullptr, *acc = nullptr;
+    double* theta = nullptr;
+    float2* sc[2] = {nullptr, nullptr};
+    double2* part = nullptr;
+    int cur = 0;
+    uint64_t steps = 0;
+    uint32_t graph_steps = 8;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool profiling = false;
+    std::vector<hipEvent_t> ev;
+    uint32_t prof_steps = 0;
+
+    template <class T>
+    int alloc(T** p, size_t count) {
+        void* q = nullptr;
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc(&q, count * sizeof(T));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(SIMUSE_E_NOMEM, "hipMalloc(%zu): %s", count * sizeof(T), hipGetErrorString(e));
+        }
+        allocs.push_back(q);
+        *p = static_cast<T*>(q);
+        return SIMUSE_OK;
+    }
+    void drop_graph() {
+        if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
+        if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+    }
+    ~simuse_kura() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        drop_graph();
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        for (void* p : allocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    KuraArgs args(int in) const {
+        KuraArgs a;
+        a.n = n; a.row_ptr = row_ptr; a.col = col; a.w = w; a.omega = omega; a.theta = theta; a.acc = acc;
+        a.sc_in = sc[in]; a.sc_out = sc[in ^ 1]; a.dt = dt;
+        return a;
+    }
+    // One step = 1 (Euler) or 4 (RK4) stage kernels; the (sin, cos) buffers
+    // alternate every stage, so after a step the parity of `cur` flips once
+    // per stage.
+    int stages() const { return method == SIMUSE_KURA_EULER ? 1 : 4; }
+    void enqueue(uint32_t steps_n, int& c) {
+        for (uint32_t k = 0; k < steps_n; ++k) {
+            if (method == SIMUSE_KURA_EULER) {
+                launch_stage_g(group, 0, args(c), stream);
+                c ^= 1;
+            } else {
+                for (int s = 1; s <= 4; ++s) {
+                    launch_stage_g(group, s, args(c), stream);
+                    c ^= 1;
+                }
+            }
+        }
+    }
+};
+
+extern "C" {
+
+int simuse_kura_create(const uint32_t* row_ptr, const uint32_t* col, const float* w, const float* omega,
+                       const float* theta0, uint32_t n, float dt, int32_t method, simuse_kura** out) {
+    if (!out) return fail(SIMUSE_E_INVALID, "null output handle");
+    *out = nullptr;
+    if (n == 0) return fail(SIMUSE_E_INVALID, "kuramoto: no oscillators");
+    if (!row_ptr || !omega || !theta0) return fail(SIMUSE_E_INVALID, "kuramoto: null arrays");
+    if (!(dt > 0.0f) || !std::isfinite(dt)) return fail(SIMUSE_E_INVALID, "kuramoto: dt must be finite and > 0");
+    if (method != SIMUSE_KURA_EULER && method != SIMUSE_KURA_RK4)
+        return fail(SIMUSE_E_INVALID, "kuramoto: unknown method %d", method);
+    if (row_ptr[0] != 0) return fail(SIMUSE_E_INVALID, "CSR row_ptr must start at 0");
+    for (uint32_t i = 0; i < n; ++i)
+        if (row_ptr[i] > row_ptr[i + 1]) return fail(SIMUSE_E_INVALID, "CSR row_ptr must be non-decreasing");
+    const uint64_t ne = row_ptr[n];
+    if (ne && (!col || !w)) return fail(SIMUSE_E_INVALID, "kuramoto: null CSR arrays");
+    for (uint64_t e = 0; e < ne; ++e) {
+        if (col[e] >= n) return fail(SIMUSE_E_INVALID, "col index %u >= %u", col[e], n);
+        if (!std::isfinite(w[e])) return fail(SIMUSE_E_INVALID, "weight %llu must be finite",
+                                              static_cast<unsigned long long>(e));
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        if (!std::isfinite(omega[i]) || !std::isfinite(theta0[i]))
+            return fail(SIMUSE_E_INVALID, "kuramoto: non-finite omega/theta at %u", i);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        return fail(SIMUSE_E_HIP, "no HIP device available");
+    }
+    std::unique_ptr<simuse_kura> k(new (std::nothrow) simuse_kura());
+    if (!k) return fail(SIMUSE_E_NOMEM, "host allocation failed");
+    KHIP(hipStreamCreate(&k->stream));
+    k->n = n;
+    k->n_edges = ne;
+    k->method = method;
+    k->dt = static_cast<double>(dt);
+    // lanes per row: about a quarter of the mean degree, a power of two
+    const double mean = static_cast<double>(ne) / n;
+    int g = 1;
+    while (g < 64 && g * 4 < mean) g <<= 1;
+    k->group = g;
+    int rc;
+    if ((rc = k->alloc(&k->row_ptr, n + 1))) return rc;
+    if ((rc = k->alloc(&k->col, ne))) return rc;
+    if ((rc = k->alloc(&k->w, ne))) return rc;
+    if ((rc = k->alloc(&k->omega, n))) return rc;
+    if ((rc = k->alloc(&k->acc, n))) return rc;
+    if ((rc = k->alloc(&k->theta, n))) return rc;
+    if ((rc = k->alloc(&k->sc[0], n))) return rc;
+    if ((rc = k->alloc(&k->sc[1], n))) return rc;
+    if ((rc = k->alloc(&k->part, 1024))) return rc;
+    KHIP(hipMemcpy(k->row_ptr, row_ptr, 4ull * (n + 1), hipMemcpyHostToDevice));
+    if (ne) {
+        KHIP(hipMemcpy(k->col, col, 4 * ne, hipMemcpyHostToDevice));
+        KHIP(hipMemcpy(k->w, w, 4 * ne, hipMemcpyHostToDevice));
+    }
+    KHIP(hipMemcpy(k->omega, omega, 4ull * n, hipMemcpyHostToDevice));
+    std::vector<double> th(n);
+    for (uint32_t i = 0; i < n; ++i) th[i] = static_cast<double>(theta0[i]);
+    KHIP(hipMemcpy(k->theta, th.data(), 8ull * n, hipMemcpyHostToDevice));
+    kura_prepare_kernel<<<(n + 255) / 256, 256, 0, k->stream>>>(k->theta, k->sc[0], n);
+    KHIP(hipGetLastError());
+    KHIP(hipStreamSynchronize(k->stream));
+    *out = k.release();
+    return SIMUSE_OK;
+}
+
+void simuse_kura_destroy(simuse_kura* k) { delete k; }
+
+int simuse_kura_set_group(simuse_kura* k, int32_t lanes) {
+    if (!k) return fail(SIMUSE_E_INVALID, "null handle");
+    if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
+        return fail(SIMUSE_E_INVALID, "lanes per row must be a power of two in 1..64");
+    if (k->group != lanes) { k->group = lanes; k->drop_graph(); }
+    return SIMUSE_OK;
+}
+
+int simuse_kura_step_async(simuse_kura* k, uint32_t n_steps) {
+    if (!k) return fail(SIMUSE_E_INVALID, "null handle");
+    if (n_steps == 0) return SIMUSE_OK;
+    if (k->profiling) {
+        const int per = k->stages();
+        const size_t need = static_cast<size_t>(k->prof_steps + n_steps) * 2;
+        while (k->ev.size() < need) {
+            hipEvent_t e;
+            KHIP(hipEventCreate(&e));
+            k->ev.push_back(e);
+        }
+        for (uint32_t s = 0; s < n_steps; ++s) {
+            hipEvent_t* e = k->ev.data() + static_cast<size_t>(k->prof_steps + s) * 2;
+            KHIP(hipEventRecord(e[0], k->stream));
+            k->enqueue(1, k->cur);
+            KHIP(hipEventRecord(e[1], k->stream));
+        }
+        (void)per;
+        k->prof_steps += n_steps;
+        k->steps += n_steps;
+        KHIP(hipGetLastError());
+        return SIMUSE_OK;
+    }
+    // The graph covers graph_steps steps; since the stage count per step is
+    // even for RK4, the buffer parity at graph entry is the same every replay.
+    uint32_t full = (k->stages() % 2 == 0) ? n_steps / k->graph_steps : 0;
+    uint32_t rest = n_steps - full * k->graph_steps;
+    if (full) {
+        if (!k->gexec) {
+            KHIP(hipStreamBeginCapture(k->stream, hipStreamCaptureModeThreadLocal));
+            int c = k->cur;
+            k->enqueue(k->graph_steps, c);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(k->stream, &g);
+            if (e != hipSuccess) return fail(SIMUSE_E_HIP, "kuramoto capture: %s", hipGetErrorString(e));
+            k->graph = g;
+            e = hipGraphInstantiate(&k->gexec, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) { k->drop_graph(); return fail(SIMUSE_E_HIP, "kuramoto instantiate: %s", hipGetErrorString(e)); }
+        }
+        for (uint32_t r = 0; r < full; ++r) KHIP(hipGraphLaunch(k->gexec, k->stream));
+    }
+    if (rest) k->enqueue(rest, k->cur);
+    k->steps += n_steps;
+    KHIP(hipGetLastError());
+    return SIMUSE_OK;
+}
+
+int simuse_kura_sync(simuse_kura* k) {
+    if (!k) return fail(SIMUSE_E_INVALID, "null handle");
+    KHIP(hipStreamSynchronize(k->stream));
+    return SIMUSE_OK;
+}
+
+int simuse_kura_download_phases(simuse_kura* k, double* out, uint32_t n) {
+    if (!k || !out || n != k->n) return fail(SIMUSE_E_INVALID, "phases buffer must hold n values");
+    KHIP(hipMemcpyAsync(out, k->theta, 8ull * n, hipMemcpyDeviceToHost, k->stream));
+    KHIP(hipStreamSynchronize(k->stream));
+    return SIMUSE_OK;
+}
+
+int simuse_kura_order_parameter(simuse_kura* k, double* r) {
+    if (!k || !r) return fail(SIMUSE_E_INVALID, "null argument");
+    const uint32_t blocks = std::min<uint32_t>(1024, (k->n + 255) / 256);
+    kura_order_partial<<<blocks, 256, 0, k->stream>>>(k->theta, k->n, k->part);
+    KHIP(hipGetLastError());
+    std::vector<double2> p(blocks);
+    KHIP(hipMemcpyAsync(p.data(), k->part, sizeof(double2) * blocks, hipMemcpyDeviceToHost, k->stream));
+    KHIP(hipStreamSynchronize(k->stream));
+    double s = 0, c = 0;
+    for (const auto& x : p) { s += x.x; c += x.y; }
+    *r = std::sqrt(s * s + c * c) / k->n;
+    return SIMUSE_OK;
+}
+
+int simuse_kura_set_profiling(simuse_kura* k, int32_t on) {
+    if (!k) return fail(SIMUSE_E_INVALID, "null handle");
+    KHIP(hipStreamSynchronize(k->stream));
+    k->profiling = on != 0;
+    k->prof_steps = 0;
+    return SIMUSE_OK;
+}
+
+int simuse_kura_kernel_times(simuse_kura* k, float* ms, int32_t n) {
+    if (!k || !ms || n < 1) return fail(SIMUSE_E_INVALID, "null argument");
+    KHIP(hipStreamSynchronize(k->stream));
+    double acc = 0;
+    for (uint32_t s = 0; s < k->prof_steps; ++s) {
+        float t = 0;
+        KHIP(hipEventElapsedTime(&t, k->ev[2 * s], k->ev[2 * s + 1]));
+        acc += t;
+    }
+    ms[0] = k->prof_steps ? static_cast<float>(acc / k->prof_steps) : 0.0f;
+    k->prof_steps = 0;
+    return SIMUSE_OK;
+}
+
+int simuse_kura_step_counter(simuse_kura* k, uint64_t* steps) {
+    if (!k) return fail(SIMUSE_E_INVALID, "null handle");
+    if (steps) *steps = k->steps;
+    return SIMUSE_OK;
+}
+
+}  // extern "C"

@@ -154,10 +154,14 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t* s
 }
 
 template <int MODEL>
-__global__ __launch_bounds__(kTile) void neuron_kernel(NetArgs a, const uint32_t* __restrict__ tiles) {
-    const uint32_t tile = tiles[blockIdx.x];
-    const PopDev& P = a.pops[a.tile_pop[tile]];
+__global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict__ nets,
+                                                      const uint2* __restrict__ tiles) {
+    const uint2 ref = tiles[blockIdx.x];
+    const NetArgs& a = nets[ref.x];
+    const uint32_t tile = ref.y;
     const uint64_t step = *a.step;
+    if (step >= a.step_limit) return;
+    const PopDev& P = a.pops[a.tile_pop[tile]];
     const uint32_t slot = static_cast<uint32_t>(step & (kSlots - 1));
     const uint32_t li = (tile - P.tile0) * kTile + threadIdx.x;
     int spike = 0;
@@ -243,10 +247,9 @@ struct TileRows {
 };
 
 template <int SCHED, class Visit>
-__device__ __forceinline__ void for_tile_edges(const NetArgs& a, uint64_t step, bool is_src,
+__device__ __forceinline__ void for_tile_edges(const NetArgs& a, uint32_t b, uint64_t step, bool is_src,
                                                uint32_t* s_rs, uint32_t* s_pref, uint32_t* s_wave,
                                                uint32_t& out_total, Visit visit) {
-    const uint32_t b = blockIdx.x;
     const uint32_t t = threadIdx.x;
     uint32_t cnt;
     uint32_t rs = 0, dg = 0;
@@ -323,22 +326,27 @@ __device__ __forceinline__ void for_tile_edges(const NetArgs& a, uint64_t step, 
 }
 
 template <int SCHED>
-__global__ __launch_bounds__(kTile) void insert_kernel(NetArgs a) {
+__global__ __launch_bounds__(kTile) void insert_kernel(const NetArgs* __restrict__ nets,
+                                                      const uint2* __restrict__ blocks) {
     __shared__ uint32_t s_rs[kTile], s_pref[kTile], s_wave[kWavesPerTile];
+    const uint2 ref = blocks[blockIdx.x];
+    const NetArgs& a = nets[ref.x];
+    const uint32_t b = ref.y;
     const uint64_t step = *a.step;
-    const bool is_src = blockIdx.x >= a.n_tiles;
+    if (step >= a.step_limit) return;
+    const bool is_src = b >= a.n_tiles;
     uint32_t total;
     uint32_t* head = a.head;
     uint32_t* next = a.next;
     const uint32_t N = a.N;
-    for_tile_edges<SCHED>(a, step, is_src, s_rs, s_pref, s_wave, total,
+    for_tile_edges<SCHED>(a, b, step, is_src, s_rs, s_pref, s_wave, total,
                           [&](uint32_t eid, uint32_t post, uint8_t d) {
                               const uint32_t dd = d < 1 ? 1u : static_cast<uint32_t>(d);
                               const uint32_t ts = static_cast<uint32_t>((step + dd) & (kSlots - 1));
                               const size_t key = static_cast<size_t>(ts) * N + post;
                               next[eid] = atomicExch(&head[key], eid);
                           });
-    if (threadIdx.x == 0 && total) a.tile_events[blockIdx.x] += total;
+    if (threadIdx.x == 0 && total) a.tile_events[b] += total;
 }
 
 __device__ __forceinline__ float event_weight(const NetArgs& a, uint32_t eid) {
@@ -346,19 +354,24 @@ __device__ __forceinline__ float event_weight(const NetArgs& a, uint32_t eid) {
 }
 
 template <int SCHED>
-__global__ __launch_bounds__(kTile) void fold_kernel(NetArgs a) {
+__global__ __launch_bounds__(kTile) void fold_kernel(const NetArgs* __restrict__ nets,
+                                                    const uint2* __restrict__ blocks) {
     __shared__ uint32_t s_rs[kTile], s_pref[kTile], s_wave[kWavesPerTile];
     constexpr int kChain = 8;
     __shared__ uint32_t s_chain[kTile * kChain];
+    const uint2 ref = blocks[blockIdx.x];
+    const NetArgs& a = nets[ref.x];
+    const uint32_t b = ref.y;
     const uint64_t step = *a.step;
-    const bool is_src = blockIdx.x >= a.n_tiles;
+    if (step >= a.step_limit) return;  // every block of the network sees the same value
+    const bool is_src = b >= a.n_tiles;
     uint32_t total;
     uint32_t* head = a.head;
     const uint32_t* next = a.next;
     float* ring = a.ring;
     const uint32_t N = a.N;
     uint32_t* my = s_chain + threadIdx.x * kChain;
-    for_tile_edges<SCHED>(a, step, is_src, s_rs, s_pref, s_wave, total,
+    for_tile_edges<SCHED>(a, b, step, is_src, s_rs, s_pref, s_wave, total,
                           [&](uint32_t eid, uint32_t post, uint8_t d) {
         if (next[eid] != kEmpty) return;  // not the first event of its key
         const uint32_t dd = d < 1 ? 1u : static_cast<uint32_t>(d);
@@ -399,13 +412,13 @@ __global__ __launch_bounds__(kTile) void fold_kernel(NetArgs a) {
         }
         ring[key] = r;
     });
-    // The last block to finish advances the device step counter; every block
-    // read *a.step before its increment of *a.done.
+    // The last of the network's blocks to finish advances its step counter;
+    // every block read *a.step before its increment of *a.done.
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
         const uint32_t prev = atomicAdd(a.done, 1u);
-        if (prev == gridDim.x - 1) {
+        if (prev == a.n_blocks - 1) {
             *a.done = 0u;
             *a.step = step + 1;
         }
@@ -415,9 +428,13 @@ __global__ __launch_bounds__(kTile) void fold_kernel(NetArgs a) {
 // Pair-based STDP, LTP pass: in-edges of every spiking post gain
 // A+ * pre-trace.  Each edge has one post, so each edge is written by at most
 // one thread.  FP64 master weight, FP32 mirror for the scatter.
-__global__ __launch_bounds__(kTile) void ltp_kernel(NetArgs a) {
+__global__ __launch_bounds__(kTile) void ltp_kernel(const NetArgs* __restrict__ nets,
+                                                  const uint2* __restrict__ tiles) {
     __shared__ uint32_t s_rs[kTile], s_pref[kTile], s_wave[kWavesPerTile];
-    const uint32_t b = blockIdx.x, t = threadIdx.x;
+    const uint2 ref = tiles[blockIdx.x];
+    const NetArgs& a = nets[ref.x];
+    if (*a.step >= a.step_limit) return;
+    const uint32_t b = ref.y, t = threadIdx.x;
     const uint32_t cnt = a.tile_count[b];
     if (cnt == 0) return;
     uint32_t rs = 0, dg = 0;
@@ -448,9 +465,13 @@ __global__ __launch_bounds__(kTile) void ltp_kernel(NetArgs a) {
 }
 
 // LTD pass: out-edges of every spiking pre lose A- * post-trace.
-__global__ __launch_bounds__(kTile) void ltd_kernel(NetArgs a) {
+__global__ __launch_bounds__(kTile) void ltd_kernel(const NetArgs* __restrict__ nets,
+                                                  const uint2* __restrict__ tiles) {
     __shared__ uint32_t s_rs[kTile], s_pref[kTile], s_wave[kWavesPerTile];
-    const uint32_t b = blockIdx.x, t = threadIdx.x;
+    const uint2 ref = tiles[blockIdx.x];
+    const NetArgs& a = nets[ref.x];
+    if (*a.step >= a.step_limit) return;
+    const uint32_t b = ref.y, t = threadIdx.x;
     const uint32_t cnt = a.tile_count[b];
     if (cnt == 0) return;
     uint32_t rs = 0, dg = 0;
@@ -557,26 +578,26 @@ __global__ void raster_stats_kernel(const uint64_t* raster, uint32_t n_words, ui
 // ---------------------------------------------------------------------------
 // Host-side launchers (called from net_api.hip)
 // ---------------------------------------------------------------------------
-void launch_neuron(int model, const NetArgs& a, const uint32_t* tiles, uint32_t n, hipStream_t s) {
+void launch_neuron(int model, const NetArgs* nets, const uint2* tiles, uint32_t n, hipStream_t s) {
     if (n == 0) return;
-    if (model == 0) neuron_kernel<0><<<n, kTile, 0, s>>>(a, tiles);
-    else if (model == 1) neuron_kernel<1><<<n, kTile, 0, s>>>(a, tiles);
-    else neuron_kernel<2><<<n, kTile, 0, s>>>(a, tiles);
+    if (model == 0) neuron_kernel<0><<<n, kTile, 0, s>>>(nets, tiles);
+    else if (model == 1) neuron_kernel<1><<<n, kTile, 0, s>>>(nets, tiles);
+    else neuron_kernel<2><<<n, kTile, 0, s>>>(nets, tiles);
 }
-void launch_stdp(const NetArgs& a, hipStream_t s) {
-    if (a.n_tiles == 0) return;
-    ltp_kernel<<<a.n_tiles, kTile, 0, s>>>(a);
-    ltd_kernel<<<a.n_tiles, kTile, 0, s>>>(a);
+void launch_stdp(const NetArgs* nets, const uint2* tiles, uint32_t n, hipStream_t s) {
+    if (n == 0) return;
+    ltp_kernel<<<n, kTile, 0, s>>>(nets, tiles);
+    ltd_kernel<<<n, kTile, 0, s>>>(nets, tiles);
 }
-void launch_insert(int sched, const NetArgs& a, hipStream_t s) {
-    const uint32_t g = a.n_tiles + a.n_src_tiles;
-    if (sched == 2) insert_kernel<2><<<g, kTile, 0, s>>>(a);
-    else insert_kernel<1><<<g, kTile, 0, s>>>(a);
+void launch_insert(int sched, const NetArgs* nets, const uint2* blocks, uint32_t n, hipStream_t s) {
+    if (n == 0) return;
+    if (sched == 2) insert_kernel<2><<<n, kTile, 0, s>>>(nets, blocks);
+    else insert_kernel<1><<<n, kTile, 0, s>>>(nets, blocks);
 }
-void launch_fold(int sched, const NetArgs& a, hipStream_t s) {
-    const uint32_t g = a.n_tiles + a.n_src_tiles;
-    if (sched == 2) fold_kernel<2><<<g, kTile, 0, s>>>(a);
-    else fold_kernel<1><<<g, kTile, 0, s>>>(a);
+void launch_fold(int sched, const NetArgs* nets, const uint2* blocks, uint32_t n, hipStream_t s) {
+    if (n == 0) return;
+    if (sched == 2) fold_kernel<2><<<n, kTile, 0, s>>>(nets, blocks);
+    else fold_kernel<1><<<n, kTile, 0, s>>>(nets, blocks);
 }
 void launch_inject(float* ring, const uint64_t* step, uint32_t N, const uint32_t* idx, const float* val,
                    uint32_t count, int serial, hipStream_t s) {

@@ -40,7 +40,8 @@ struct PopDev {
     float puni[kMaxParams];
 };
 
-// Everything one step of one network touches.
+// Everything one step of one network touches.  An engine keeps an array of
+// these in device memory; every kernel block is handed a (network, tile) pair.
 struct NetArgs {
     const PopDev* pops;
     const uint32_t* tile_pop;   // tile -> population
@@ -48,7 +49,9 @@ struct NetArgs {
     uint32_t n_tiles;           // neuron tiles
     uint32_t n_src_tiles;       // Poisson-source tiles (after the neuron tiles)
     uint32_t n_src;
+    uint32_t n_blocks;          // n_tiles + n_src_tiles (scatter blocks of this network)
     uint64_t n_syn;
+    uint64_t step_limit;        // kernels skip the network once *step reaches it
     float* ring;                // [32][N]
     uint32_t* head;             // [32][N] chain heads, kEmpty when idle
     uint32_t* next;             // [n_syn + n_src_edges] chain links

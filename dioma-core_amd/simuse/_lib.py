@@ -22,6 +22,9 @@ MODEL_LIF = 0
 MODEL_IZHIKEVICH = 1
 MODEL_HH = 2
 
+KURA_EULER = 0
+KURA_RK4 = 1
+
 SCATTER_AUTO = 0
 SCATTER_TILE = 1
 SCATTER_WAVE = 2
@@ -106,12 +109,14 @@ _SIGS = {
 
 # Optional families (present when their sources are built in).
 _SIGS_OPT = {
-    "simuse_kura_create": ([u32p, u32p, f32p, f32p, f32p, C.c_uint32, C.c_float, C.POINTER(P)], C.c_int),
+    "simuse_kura_create": ([u32p, u32p, f32p, f32p, f32p, C.c_uint32, C.c_float, C.c_int32, C.POINTER(P)], C.c_int),
     "simuse_kura_destroy": ([P], None),
+    "simuse_kura_set_group": ([P, C.c_int32], C.c_int),
     "simuse_kura_step_async": ([P, C.c_uint32], C.c_int),
     "simuse_kura_sync": ([P], C.c_int),
-    "simuse_kura_download_phases": ([P, f32p, C.c_uint32], C.c_int),
+    "simuse_kura_download_phases": ([P, f64p, C.c_uint32], C.c_int),
     "simuse_kura_order_parameter": ([P, f64p], C.c_int),
+    "simuse_kura_step_counter": ([P, u64p], C.c_int),
     "simuse_kura_set_profiling": ([P, C.c_int32], C.c_int),
     "simuse_kura_kernel_times": ([P, f32p, C.c_int32], C.c_int),
 }

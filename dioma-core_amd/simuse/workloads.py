@@ -216,3 +216,57 @@ def build_device(spec: NetSpec, graph_steps: int = 8) -> CompiledNetwork:
         net.set_plasticity(plastic=plastic, **st)
     net.set_graph_steps(graph_steps)
     return net
+
+
+def modular_small_world(n: int, modules: int = 64, k: int = 20, beta: float = 0.1, inter: float = 0.05,
+                        coupling: float = 2.0, seed: int = 1):
+    """Modular Watts–Strogatz graph for Kuramoto oscillators.
+
+    n nodes in `modules` contiguous modules; inside a module each node links
+    to its k/2 right neighbours on the module ring; each such edge is rewired
+    with probability `inter` to a uniform node of another module, otherwise
+    with probability `beta` to a uniform node of its own module.  Edges are
+    made symmetric; W_ij = coupling / k_i (k_i = degree of i).  Returns
+    (row_ptr, col, w, omega ~ N(0,1), theta0 ~ U(-pi, pi)).
+    """
+    rng = np.random.default_rng(seed)
+    m = n // modules
+    if m < k + 2:
+        raise ValueError("modules too small for the lattice degree")
+    sizes = np.full(modules, m, np.int64)
+    sizes[: n - m * modules] += 1
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    node = np.arange(n, dtype=np.int64)
+    mod = np.repeat(np.arange(modules), sizes)
+    base = starts[mod]
+    size = sizes[mod]
+    local = node - base
+    half = k // 2
+    src = np.repeat(node, half)
+    off = np.tile(np.arange(1, half + 1), n)
+    dst = base.repeat(half) + (local.repeat(half) + off) % size.repeat(half)
+    u = rng.random(src.size)
+    inter_m = u < inter
+    intra_m = (~inter_m) & (u < inter + beta)
+    # rewire inside the module
+    ni = int(intra_m.sum())
+    s_mod = mod[src[intra_m]]
+    r = rng.integers(0, sizes[s_mod] - 1)
+    loc = local[src[intra_m]]
+    r = r + (r >= loc)
+    dst[intra_m] = starts[s_mod] + r
+    # rewire to another module
+    nx = int(inter_m.sum())
+    om = (mod[src[inter_m]] + rng.integers(1, modules, size=nx)) % modules
+    dst[inter_m] = starts[om] + rng.integers(0, sizes[om])
+    a = np.concatenate([src, dst])
+    b = np.concatenate([dst, src])
+    order = np.lexsort((b, a))
+    a, b = a[order], b[order]
+    deg = np.bincount(a, minlength=n)
+    rp = np.zeros(n + 1, np.uint64)
+    np.cumsum(deg, out=rp[1:])
+    w = (coupling / deg[a]).astype(np.float32)
+    omega = rng.normal(size=n).astype(np.float32)
+    theta0 = rng.uniform(-np.pi, np.pi, n).astype(np.float32)
+    return rp.astype(np.uint32), b.astype(np.uint32), w, omega, theta0

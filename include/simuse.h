@@ -171,6 +171,35 @@ int simuse_net_device_bytes(simuse_net* net, uint64_t* bytes);
 int simuse_net_set_profiling(simuse_net* net, int32_t on);
 int simuse_net_kernel_times(simuse_net* net, float* ms, int32_t n);
 
+/* ---------------------------------------------------------------------------
+ * Kuramoto oscillators on a CSR graph.
+ * Coupling c_i = sum_j W_ij sin(theta_j - theta_i) over row i; theta wraps to
+ * [-pi, pi).  SIMUSE_KURA_EULER is the reference's integrator
+ * (P/src/kernels/phase/sparse.rs:40-97, kuramoto.rs:42-68; GPU
+ * PC/kernels/kuramoto.cu:9-75,163-195); SIMUSE_KURA_RK4 is classic RK4 with
+ * the phase accumulated in FP64.
+ * ------------------------------------------------------------------------- */
+#define SIMUSE_KURA_EULER 0
+#define SIMUSE_KURA_RK4 1
+
+typedef struct simuse_kura simuse_kura;
+
+int simuse_kura_create(const uint32_t* row_ptr, const uint32_t* col, const float* w, const float* omega,
+                       const float* theta0, uint32_t n, float dt, int32_t method, simuse_kura** out);
+void simuse_kura_destroy(simuse_kura* k);
+/* lanes per CSR row in the gather (power of two 1..64; default ~ mean degree / 4) */
+int simuse_kura_set_group(simuse_kura* k, int32_t lanes);
+int simuse_kura_step_async(simuse_kura* k, uint32_t n_steps);
+int simuse_kura_sync(simuse_kura* k);
+int simuse_kura_download_phases(simuse_kura* k, double* out, uint32_t n);
+/* R = |sum_i e^{i theta_i}| / n (phase_coherence with unit amplitudes,
+ * P/src/kernels/measure/mod.rs:76-106), deterministic FP64 reduction. */
+int simuse_kura_order_parameter(simuse_kura* k, double* r);
+int simuse_kura_step_counter(simuse_kura* k, uint64_t* steps);
+/* HIP-event timing of whole steps (all stage kernels of a step). */
+int simuse_kura_set_profiling(simuse_kura* k, int32_t on);
+int simuse_kura_kernel_times(simuse_kura* k, float* ms, int32_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,57 @@
+"""Kuramoto coupling on the device against the FP64 oracle (needs an MI355X).
+
+Tolerances: FP32 sin/cos and derivatives with an FP64 phase give phase
+errors of order 1e-6 rad per unit time; the tests allow 1e-4 rad.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from simuse.kuramoto import KuramotoGraph
+from simuse.workloads import modular_small_world
+
+pytestmark = pytest.mark.gpu
+
+
+def wrapped_err(a, b):
+    return float(np.max(np.abs(np.angle(np.exp(1j * (a - b))))))
+
+
+@pytest.mark.parametrize("method,lanes", [("rk4", None), ("rk4", 1), ("rk4", 16), ("euler", None)])
+def test_kuramoto_matches_fp64_oracle(method, lanes):
+    print("kuramoto", method, lanes, flush=True)
+    rp, col, w, om, th = modular_small_world(20_000, seed=3)
+    dt, steps = 0.001, 500
+    dev = KuramotoGraph(rp, col, w, om, th, dt, method)
+    if lanes:
+        dev.set_group(lanes)
+    ora = po.OracleKuramoto(rp, col, w, om, th, dt, kind=64)
+    dev.run(steps)
+    (ora.rk4 if method == "rk4" else ora.euler)(steps)
+    err = wrapped_err(dev.phases(), ora.phases())
+    print("max phase error", err, flush=True)
+    assert err < 1e-4
+    assert abs(dev.order_parameter() - ora.order()) < 1e-6
+    assert dev.step_counter() == steps
+
+
+def test_kuramoto_identical_frequencies_synchronise():
+    n = 4096
+    rp, col, w, _, th = modular_small_world(n, modules=16, k=20, beta=0.3, inter=0.1, coupling=4.0, seed=1)
+    dev = KuramotoGraph(rp, col, w, np.full(n, 0.5, np.float32), th, 0.01, "rk4")
+    r0 = dev.order_parameter()
+    dev.run(3000)
+    assert dev.order_parameter() > 0.99 > r0
+
+
+def test_kuramoto_uncoupled_linear_phase():
+    n = 1000
+    rng = np.random.default_rng(0)
+    om = rng.normal(size=n).astype(np.float32)
+    th = rng.uniform(-3, 3, n).astype(np.float32)
+    rp = np.zeros(n + 1, np.uint32)
+    dev = KuramotoGraph(rp, np.zeros(0, np.uint32), np.zeros(0, np.float32), om, th, 0.01, "rk4")
+    dev.run(1000)
+    t = 1000 * float(np.float32(0.01))
+    want = th.astype(np.float64) + om.astype(np.float64) * t
+    assert wrapped_err(dev.phases(), want) < 1e-9
