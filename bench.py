@@ -59,8 +59,29 @@ def cpu_baseline(spec, steps: int, threads: int):
     return spec.n_total * steps / dt, dt
 
 
+def copy_calibration(mib: int = 1024) -> int:
+    """Device-to-device copy of a known size (the HIP runtime's copy kernel),
+    used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 3}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 3 device copies of 1 GiB (traffic counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
@@ -68,10 +89,12 @@ def main() -> int:
     ap.add_argument("--synapses-per-neuron", type=int, default=100)
     ap.add_argument("--graph-steps", type=int, default=16)
     ap.add_argument("--profile-steps", type=int, default=100)
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scatter", choices=["auto", "tile", "wave"], default="auto")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "

@@ -64,6 +64,8 @@ struct HostPop {
     float* pr = nullptr;
     float* sigma = nullptr;
     float sigma_uni = 0.0f;
+    float drive_p = 0.0f, drive_w = 0.0f;
+    uint32_t drive_seed0 = 0;
     uint64_t seed = 0;
 };
 
@@ -221,6 +223,7 @@ struct NetState {
     int add_source(float rate_hz, float dt_ms, uint32_t seed, const uint32_t* posts, const float* wf,
                    const uint8_t* dl, uint32_t count);
     int set_noise(int32_t pop, const float* sigma, uint64_t seed);
+    int set_drive(int32_t pop, float rate_hz, float dt_ms, float weight, uint32_t seed0);
     int set_plasticity(const simuse_stdp_params* p, const uint8_t* pl, hipStream_t stream);
     int download_spikes(int32_t* out, uint64_t len, hipStream_t stream);
     int download_spike_ids(uint32_t* ids, uint32_t cap, uint32_t* count, hipStream_t stream);

@@ -43,7 +43,7 @@ struct KuraArgs {
     const float* w;
     const float* omega;
     double* theta;        // FP64 master phase, wrapped to [-pi, pi)
-    float* acc;           // RK4 accumulator k1 + 2k2 + 2k3
+    double* acc;          // RK4 accumulator k1 + 2k2 + 2k3 (FP64)
     const float2* sc_in;  // (sin, cos) of the current stage phase
     float2* sc_out;       // (sin, cos) of the next stage phase
     double dt;
@@ -90,16 +90,16 @@ __global__ __launch_bounds__(256) void kura_stage_kernel(KuraArgs a) {
         next = wrap_phase(th + a.dt * static_cast<double>(k));
         a.theta[i] = next;
     } else if (STAGE == 1) {
-        a.acc[i] = k;
+        a.acc[i] = static_cast<double>(k);
         next = th + 0.5 * a.dt * static_cast<double>(k);
     } else if (STAGE == 2) {
-        a.acc[i] += 2.0f * k;
+        a.acc[i] += 2.0 * static_cast<double>(k);
         next = th + 0.5 * a.dt * static_cast<double>(k);
     } else if (STAGE == 3) {
-        a.acc[i] += 2.0f * k;
+        a.acc[i] += 2.0 * static_cast<double>(k);
         next = th + a.dt * static_cast<double>(k);
     } else {
-        const double inc = a.dt / 6.0 * (static_cast<double>(a.acc[i]) + static_cast<double>(k));
+        const double inc = a.dt / 6.0 * (a.acc[i] + static_cast<double>(k));
         next = wrap_phase(th + inc);
         a.theta[i] = next;
     }
@@ -138,6 +138,57 @@ __global__ void kura_order_partial(const double* theta, uint32_t n, double2* par
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = make_double2(ss[0], cc[0]);
+}
+
+// Directed modular small-world graph: node i of module m has k in-edges from
+// its lattice neighbours i±1..±k/2 on the module ring; each is redrawn with
+// probability `inter` from a uniform node of another module, otherwise with
+// probability `beta` from a uniform node of its own module.  W = coupling/k.
+__device__ __forceinline__ uint64_t kmix(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__global__ void kura_gen_kernel(uint32_t* row_ptr, uint32_t* col, float* w, float* omega, float* theta0, uint32_t n,
+                                uint32_t modules, uint32_t k, float beta, float inter, float coupling, uint64_t seed) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    if (i == n) { row_ptr[n] = n * k; return; }
+    const uint32_t msize = n / modules;
+    uint32_t m = i / msize;
+    if (m >= modules) m = modules - 1;
+    const uint32_t base = m * msize;
+    const uint32_t size = (m == modules - 1) ? n - base : msize;
+    const uint32_t l = i - base;
+    row_ptr[i] = i * k;
+    const float wv = coupling / static_cast<float>(k);
+    for (uint32_t j = 0; j < k; ++j) {
+        const int32_t o = (j < k / 2) ? -static_cast<int32_t>(j + 1) : static_cast<int32_t>(j - k / 2 + 1);
+        uint32_t src = base + static_cast<uint32_t>((static_cast<int64_t>(l) + o + size) % size);
+        const uint64_t h = kmix(seed ^ (static_cast<uint64_t>(i) * 0xD1B54A32D192ED03ull + j));
+        const float u = static_cast<float>(h >> 40) * 5.9604644775390625e-08f;
+        const uint64_t h2 = kmix(h);
+        if (modules > 1 && u < inter) {
+            const uint32_t om = (m + 1 + static_cast<uint32_t>(h2 % (modules - 1))) % modules;
+            const uint32_t ob = om * msize;
+            const uint32_t os = (om == modules - 1) ? n - ob : msize;
+            src = ob + static_cast<uint32_t>((h2 >> 32) % os);
+        } else if (u < inter + beta) {
+            uint32_t r = static_cast<uint32_t>(h2 % (size - 1));
+            r += r >= l ? 1u : 0u;
+            src = base + r;
+        }
+        col[static_cast<size_t>(i) * k + j] = src;
+        w[static_cast<size_t>(i) * k + j] = wv;
+    }
+    // omega ~ N(0,1) (Box-Muller on two 24-bit uniforms), theta0 ~ U(-pi, pi)
+    const uint64_t a = kmix(seed ^ 0x5851F42D4C957F2Dull ^ (static_cast<uint64_t>(i) << 1));
+    const uint64_t b = kmix(a);
+    const float u1 = (static_cast<float>(a >> 40) + 1.0f) * 5.9604644775390625e-08f;
+    const float u2 = static_cast<float>(b >> 40) * 5.9604644775390625e-08f;
+    omega[i] = sqrtf(-2.0f * logf(u1)) * cosf(6.2831853f * u2);
+    theta0[i] = -3.14159265f + 6.2831853f * static_cast<float>(kmix(b) >> 40) * 5.9604644775390625e-08f;
 }
 
 template <int G>
@@ -184,7 +235,8 @@ struct simuse_kura {
     double dt = 0;
     std::vector<void*> allocs;
     uint32_t *row_ptr = nullptr, *col = nullptr;
-    float *w = nullptr, *omega = nullptr, *acc = nullptr;
+    float *w = nullptr, *omega = nullptr;
+    double* acc = nullptr;
     double* theta = nullptr;
     float2* sc[2] = {nullptr, nullptr};
     double2* part = nullptr;
@@ -310,6 +362,80 @@ int simuse_kura_create(const uint32_t* row_ptr, const uint32_t* col, const float
     KHIP(hipGetLastError());
     KHIP(hipStreamSynchronize(k->stream));
     *out = k.release();
+    return SIMUSE_OK;
+}
+
+int simuse_kura_create_generated(const simuse_kura_gen_desc* g, int32_t method, simuse_kura** out) {
+    if (!out || !g) return fail(SIMUSE_E_INVALID, "null argument");
+    *out = nullptr;
+    if (g->modules == 0 || g->k < 2 || g->k % 2 || g->n / g->modules < g->k + 2)
+        return fail(SIMUSE_E_INVALID, "kuramoto generator: need even k >= 2 and modules of at least k+2 nodes");
+    if (static_cast<uint64_t>(g->n) * g->k >= 0xFFFFFFFFull)
+        return fail(SIMUSE_E_INVALID, "kuramoto generator: n*k exceeds the u32 CSR");
+    if (!(g->dt > 0.0f) || !std::isfinite(g->coupling) || !(g->beta >= 0.0f) || !(g->inter >= 0.0f))
+        return fail(SIMUSE_E_INVALID, "kuramoto generator: invalid parameters");
+    if (method != SIMUSE_KURA_EULER && method != SIMUSE_KURA_RK4)
+        return fail(SIMUSE_E_INVALID, "kuramoto: unknown method %d", method);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        return fail(SIMUSE_E_HIP, "no HIP device available");
+    }
+    std::unique_ptr<simuse_kura> k(new (std::nothrow) simuse_kura());
+    if (!k) return fail(SIMUSE_E_NOMEM, "host allocation failed");
+    KHIP(hipStreamCreate(&k->stream));
+    const uint32_t n = g->n;
+    const uint64_t ne = static_cast<uint64_t>(n) * g->k;
+    k->n = n;
+    k->n_edges = ne;
+    k->method = method;
+    k->dt = static_cast<double>(g->dt);
+    int grp = 1;
+    while (grp < 64 && grp * 4 < static_cast<int>(g->k)) grp <<= 1;
+    k->group = grp;
+    int rc;
+    float* th0 = nullptr;
+    if ((rc = k->alloc(&k->row_ptr, n + 1))) return rc;
+    if ((rc = k->alloc(&k->col, ne))) return rc;
+    if ((rc = k->alloc(&k->w, ne))) return rc;
+    if ((rc = k->alloc(&k->omega, n))) return rc;
+    if ((rc = k->alloc(&k->acc, n))) return rc;
+    if ((rc = k->alloc(&k->theta, n))) return rc;
+    if ((rc = k->alloc(&k->sc[0], n))) return rc;
+    if ((rc = k->alloc(&k->sc[1], n))) return rc;
+    if ((rc = k->alloc(&k->part, 1024))) return rc;
+    if ((rc = k->alloc(&th0, n))) return rc;
+    kura_gen_kernel<<<(n + 256) / 256, 256, 0, k->stream>>>(k->row_ptr, k->col, k->w, k->omega, th0, n, g->modules,
+                                                             g->k, g->beta, g->inter, g->coupling, g->seed);
+    KHIP(hipGetLastError());
+    std::vector<float> th(n);
+    KHIP(hipMemcpyAsync(th.data(), th0, 4ull * n, hipMemcpyDeviceToHost, k->stream));
+    KHIP(hipStreamSynchronize(k->stream));
+    std::vector<double> thd(th.begin(), th.end());
+    KHIP(hipMemcpy(k->theta, thd.data(), 8ull * n, hipMemcpyHostToDevice));
+    kura_prepare_kernel<<<(n + 255) / 256, 256, 0, k->stream>>>(k->theta, k->sc[0], n);
+    KHIP(hipGetLastError());
+    KHIP(hipStreamSynchronize(k->stream));
+    *out = k.release();
+    return SIMUSE_OK;
+}
+
+int simuse_kura_download_graph(simuse_kura* k, uint32_t* row_ptr, uint32_t* col, float* w, float* omega) {
+    if (!k) return fail(SIMUSE_E_INVALID, "null handle");
+    KHIP(hipStreamSynchronize(k->stream));
+    if (row_ptr) KHIP(hipMemcpy(row_ptr, k->row_ptr, 4ull * (k->n + 1), hipMemcpyDeviceToHost));
+    if (k->n_edges) {
+        if (col) KHIP(hipMemcpy(col, k->col, 4 * k->n_edges, hipMemcpyDeviceToHost));
+        if (w) KHIP(hipMemcpy(w, k->w, 4 * k->n_edges, hipMemcpyDeviceToHost));
+    }
+    if (omega) KHIP(hipMemcpy(omega, k->omega, 4ull * k->n, hipMemcpyDeviceToHost));
+    return SIMUSE_OK;
+}
+
+int simuse_kura_info(simuse_kura* k, uint32_t* n, uint64_t* n_edges) {
+    if (!k) return fail(SIMUSE_E_INVALID, "null handle");
+    if (n) *n = k->n;
+    if (n_edges) *n_edges = k->n_edges;
     return SIMUSE_OK;
 }
 

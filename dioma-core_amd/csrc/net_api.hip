@@ -231,6 +231,9 @@ int NetState::upload_pop_table() {
         D.pr = P.pr;
         D.sigma = P.sigma;
         D.sigma_uni = P.sigma_uni;
+        D.drive_p = P.drive_p;
+        D.drive_w = P.drive_w;
+        D.drive_seed0 = P.drive_seed0;
         D.seed = P.seed;
         std::memcpy(D.puni, P.puni, sizeof(D.puni));
     }
@@ -303,6 +306,20 @@ int NetState::set_noise(int32_t pop, const float* sigma, uint64_t seed) {
         P.seed = seed;
         P.flags |= kPopNoise;
     }
+    return upload_pop_table();
+}
+
+int NetState::set_drive(int32_t pop, float rate_hz, float dt_ms, float weight, uint32_t seed0) {
+    if (pop < 0 || pop >= static_cast<int32_t>(pops.size())) return fail(SIMUSE_E_INVALID, "pop %d out of range", pop);
+    if (!(std::isfinite(rate_hz) && rate_hz >= 0.0f) || !(std::isfinite(dt_ms) && dt_ms > 0.0f) ||
+        !std::isfinite(weight))
+        return fail(SIMUSE_E_INVALID, "poisson drive: rate >= 0, dt > 0 and a finite weight are required");
+    HostPop& P = pops[pop];
+    P.drive_p = std::min(std::max(rate_hz * dt_ms / 1000.0f, 0.0f), 1.0f);  // core.rs:301
+    P.drive_w = weight;
+    P.drive_seed0 = seed0;
+    if (rate_hz > 0.0f) P.flags |= kPopDrive;
+    else P.flags &= ~kPopDrive;
     return upload_pop_table();
 }
 
@@ -745,6 +762,16 @@ int simuse_net_set_noise(simuse_net* net, int32_t pop, const float* sigma, uint6
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(net->eng.stream));
     rc = net->st.set_noise(pop, sigma, seed);
+    net->eng.mark_dirty();
+    return rc;
+}
+
+int simuse_net_set_poisson_drive(simuse_net* net, int32_t pop, float rate_hz, float dt_ms, float weight,
+                                 uint32_t seed0) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(net->eng.stream));
+    rc = net->st.set_drive(pop, rate_hz, dt_ms, weight, seed0);
     net->eng.mark_dirty();
     return rc;
 }

@@ -165,11 +165,20 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
     const uint32_t slot = static_cast<uint32_t>(step & (kSlots - 1));
     const uint32_t li = (tile - P.tile0) * kTile + threadIdx.x;
     int spike = 0;
+    int drive_now = 0;  // drive source of this neuron fires this step (one event)
     if (li < P.n) {
         const uint32_t gi = P.off + li;
         float* rs = a.ring + static_cast<size_t>(slot) * a.N + gi;
         float I = *rs;
         *rs = 0.0f;
+        // A per-neuron Poisson source with one edge of delay <= 1 adds its
+        // weight into the slot after every neuron push of the previous step
+        // (P/src/api/net/network/step.rs:166-185), i.e. as the last add.
+        if ((P.flags & kPopDrive) && P.drive_p > 0.0f) {
+            const uint32_t sd = P.drive_seed0 + li * 2654435761u;
+            if (step > 0 && poisson_roll(sd, 0u, step - 1, P.drive_p)) I = I + P.drive_w;
+            drive_now = poisson_roll(sd, 0u, step, P.drive_p) ? 1 : 0;
+        }
         if (P.flags & kPopNoise) {
             const float sg = (P.flags & kPopNoiseScalar) ? P.sigma_uni : P.sigma[li];
             I = I + sg * normal_draw(P.seed, li, step);
@@ -205,11 +214,13 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
     }
     // Ordered compaction: wave ballot + block prefix over the 4 waves.
     const uint64_t m = __ballot(spike);
+    const uint64_t dm = __ballot(drive_now);
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    __shared__ uint32_t s_cnt[kWavesPerTile];
+    __shared__ uint32_t s_cnt[kWavesPerTile], s_drv[kWavesPerTile];
     const uint32_t local_tile = tile - P.tile0;
     if (lane == 0) {
         s_cnt[wave] = static_cast<uint32_t>(__popcll(m));
+        s_drv[wave] = static_cast<uint32_t>(__popcll(dm));
         const uint32_t word = P.word0 + local_tile * kWavesPerTile + wave;
         a.bits[word] = m;
         if (a.raster) {
@@ -232,6 +243,9 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
     if (threadIdx.x == 0) {
         a.tile_count[tile] = total;
         a.tile_spikes[tile] += total;
+        // each firing drive source is one synaptic event, counted at its roll
+        const uint32_t dtot = s_drv[0] + s_drv[1] + s_drv[2] + s_drv[3];
+        if (dtot) a.tile_events[tile] += dtot;
     }
 }
 
@@ -412,11 +426,13 @@ __global__ __launch_bounds__(kTile) void fold_kernel(const NetArgs* __restrict__
         }
         ring[key] = r;
     });
-    // The last of the network's blocks to finish advances its step counter;
-    // every block read *a.step before its increment of *a.done.
+    // The last of the network's blocks to finish advances its step counter.
+    // Every block consumed its load of *a.step before reaching this point, so
+    // a relaxed counter suffices; the ring and chain writes reach the next
+    // kernel through the kernel boundary.  (No device-scope fence here: on
+    // gfx950 a release fence writes back the XCD's L2, once per block.)
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
         const uint32_t prev = atomicAdd(a.done, 1u);
         if (prev == a.n_blocks - 1) {
             *a.done = 0u;

@@ -16,6 +16,7 @@ enum : uint32_t {
     kPopRefrElided = 1u,  // refractory timer provably stays 0: not loaded/stored
     kPopNoise = 2u,       // Gaussian current noise enabled
     kPopNoiseScalar = 4u, // every neuron has the same sigma (sigma_uni)
+    kPopDrive = 8u,       // per-neuron Poisson drive (one source per neuron)
 };
 
 // One population as the kernels see it.  Parameter column k is either a
@@ -36,6 +37,9 @@ struct PopDev {
     const float* pr;    // [np][dstride]
     const float* sigma; // [n] noise amplitude
     float sigma_uni;    // noise amplitude when kPopNoiseScalar
+    float drive_p;      // Poisson drive: per-step probability
+    float drive_w;      //                weight of one drive event
+    uint32_t drive_seed0;  // seed of neuron i: drive_seed0 + i * 2654435761 (mod 2^32)
     uint64_t seed;
     float puni[kMaxParams];
 };

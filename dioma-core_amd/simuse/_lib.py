@@ -77,6 +77,7 @@ _SIGS = {
     "simuse_net_destroy": ([P], None),
     "simuse_net_add_poisson_source": ([P, C.c_float, C.c_float, C.c_uint32, u32p, f32p, u8p, C.c_uint32], C.c_int),
     "simuse_net_set_noise": ([P, C.c_int32, f32p, C.c_uint64], C.c_int),
+    "simuse_net_set_poisson_drive": ([P, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_uint32], C.c_int),
     "simuse_net_set_plasticity": ([P, C.POINTER(StdpParams), u8p], C.c_int),
     "simuse_net_set_scatter_mode": ([P, C.c_int32], C.c_int),
     "simuse_net_get_scatter_mode": ([P, i32p], C.c_int),

@@ -127,6 +127,12 @@ class CompiledNetwork:
             raise ValueError("sigma must have one entry per neuron")
         check(lib().simuse_net_set_noise(self._h, pop, _ptr(s, C.c_float), int(seed) & ((1 << 64) - 1)))
 
+    def set_poisson_drive(self, pop: int, rate_hz: float, dt_ms: float, weight: float, seed0: int) -> None:
+        """One Poisson source per neuron (seed seed0 + i*2654435761 mod 2^32, one
+        edge of `weight`, delay <= 1), rolled inside the neuron kernel."""
+        check(lib().simuse_net_set_poisson_drive(self._h, pop, float(rate_hz), float(dt_ms), float(weight),
+                                                 int(seed0) & 0xFFFFFFFF))
+
     def set_plasticity(self, a_plus, a_minus, tau_plus, tau_minus, w_min, w_max, plastic=None) -> None:
         sp = StdpParams(a_plus, a_minus, tau_plus, tau_minus, w_min, w_max)
         if plastic is None:

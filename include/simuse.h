@@ -102,6 +102,14 @@ int simuse_net_add_poisson_source(simuse_net* net, float rate_hz, float dt_ms, u
  * neuron of `pop` each step (thalamic input).  No reference counterpart: the
  * reference drives this through inject_current from the host every step. */
 int simuse_net_set_noise(simuse_net* net, int32_t pop, const float* sigma, uint64_t seed);
+/* Per-neuron Poisson drive: neuron i of `pop` gets its own Poisson source
+ * (rate_hz, seed drive_seed0 + i*2654435761 mod 2^32, stream 0) with one edge
+ * of weight `weight` and delay <= 1, exactly as add_poisson_source with one
+ * post would give, but rolled inside the neuron kernel instead of scattered.
+ * (Bit-identical to the explicit sources unless inject_current adds to the
+ * same neuron in the same step.) */
+int simuse_net_set_poisson_drive(simuse_net* net, int32_t pop, float rate_hz, float dt_ms, float weight,
+                                 uint32_t drive_seed0);
 /* Enables pair-based STDP; plastic[e] != 0 marks plastic edges (NULL = all).
  * Weights are accumulated in FP64 with an FP32 mirror used by the scatter. */
 int simuse_net_set_plasticity(simuse_net* net, const simuse_stdp_params* p, const uint8_t* plastic);
@@ -172,6 +180,145 @@ int simuse_net_set_profiling(simuse_net* net, int32_t on);
 int simuse_net_kernel_times(simuse_net* net, float* ms, int32_t n);
 
 /* ---------------------------------------------------------------------------
+ * Batches: many independent networks stepped by one launch sequence.
+ * No reference counterpart (the reference steps one CompiledNetwork per call);
+ * each network keeps the simuse_net step semantics, its own ring, step
+ * counter and RNG keys, so its results are identical alone or in a batch.
+ * ------------------------------------------------------------------------- */
+typedef struct simuse_batch simuse_batch;
+
+/* Seeded random network built on the device.  Neurons [0, round(n*exc_frac))
+ * are excitatory.  Each neuron has k outgoing synapses to uniform random
+ * other neurons (multi-edges possible), weights U(0,w_exc) (exc rows) or
+ * -U(0,w_inh) (inh rows), delays uniform in [d_min, d_max].
+ * LIF: one population, tau_m 20 ms, v_rest = v_reset = -65, v_th -50,
+ *      refractory 2 ms, v0 ~ U(-65,-50).
+ * Izhikevich: RS population (exc) and FS population (inh) with the 2003
+ *      randomisation, v_th 30, no refractory period, v0 -65, u0 = b v0.
+ * Optional per-neuron Poisson drive (simuse_net_set_poisson_drive semantics)
+ * and Gaussian current noise. */
+typedef struct simuse_gen_desc {
+    int32_t model; /* SIMUSE_MODEL_LIF or SIMUSE_MODEL_IZHIKEVICH */
+    uint32_t n;
+    uint32_t k;
+    uint64_t seed;
+    float dt;
+    float exc_frac;
+    float w_exc;
+    float w_inh;
+    uint8_t d_min;
+    uint8_t d_max;
+    float drive_rate_hz;
+    float drive_weight;
+    float noise_sigma;
+} simuse_gen_desc;
+
+int simuse_batch_create(simuse_batch** out);
+void simuse_batch_destroy(simuse_batch* b);
+int simuse_batch_add_network(simuse_batch* b, const simuse_pop_desc* pops, int32_t n_pops, const uint32_t* row_ptr,
+                             const uint32_t* col, const float* w, const uint8_t* delay, uint64_t n_syn,
+                             int32_t* net_id);
+int simuse_batch_add_generated(simuse_batch* b, const simuse_gen_desc* g, int32_t* net_id);
+/* Frees the network's device memory; its id may be reused. */
+int simuse_batch_remove_network(simuse_batch* b, int32_t net_id);
+int simuse_batch_n_networks(simuse_batch* b, int32_t* n);
+int simuse_batch_add_poisson_source(simuse_batch* b, int32_t net_id, float rate_hz, float dt_ms, uint32_t seed,
+                                    const uint32_t* posts, const float* w, const uint8_t* delay, uint32_t count);
+int simuse_batch_set_noise(simuse_batch* b, int32_t net_id, int32_t pop, const float* sigma, uint64_t seed);
+int simuse_batch_set_poisson_drive(simuse_batch* b, int32_t net_id, int32_t pop, float rate_hz, float dt_ms,
+                                   float weight, uint32_t drive_seed0);
+/* The network stops advancing once its step counter reaches `steps`. */
+int simuse_batch_set_horizon(simuse_batch* b, int32_t net_id, uint64_t steps);
+int simuse_batch_set_graph_steps(simuse_batch* b, uint32_t steps);
+int simuse_batch_set_scatter_mode(simuse_batch* b, int32_t mode);
+int simuse_batch_autotune(simuse_batch* b, uint32_t trial_steps, int32_t* mode);
+int simuse_batch_step_async(simuse_batch* b, uint32_t n_steps);
+int simuse_batch_sync(simuse_batch* b);
+int simuse_batch_step_counter(simuse_batch* b, int32_t net_id, uint64_t* step);
+int simuse_batch_net_info(simuse_batch* b, int32_t net_id, uint64_t* n_total, uint64_t* n_syn, int32_t* n_pops);
+int simuse_batch_pop_info(simuse_batch* b, int32_t net_id, int32_t pop, int32_t* model, uint64_t* n);
+int simuse_batch_download_state(simuse_batch* b, int32_t net_id, int32_t pop, float* out, uint64_t len);
+int simuse_batch_download_params(simuse_batch* b, int32_t net_id, int32_t pop, float* out, uint64_t len);
+int simuse_batch_download_csr(simuse_batch* b, int32_t net_id, uint32_t* row_ptr, uint32_t* col, float* w,
+                              uint8_t* delay);
+int simuse_batch_download_spike_ids(simuse_batch* b, int32_t net_id, uint32_t* ids, uint32_t cap, uint32_t* count);
+int simuse_batch_counters(simuse_batch* b, int32_t net_id, uint64_t* spikes, uint64_t* events);
+int simuse_batch_record(simuse_batch* b, int32_t net_id, uint32_t max_steps);
+int simuse_batch_download_raster(simuse_batch* b, int32_t net_id, uint64_t* out, uint64_t n_words, uint32_t* steps,
+                                 uint64_t* words_per_step);
+int simuse_batch_device_bytes(simuse_batch* b, uint64_t* bytes);
+int simuse_batch_set_profiling(simuse_batch* b, int32_t on);
+int simuse_batch_kernel_times(simuse_batch* b, float* ms, int32_t n);
+/* Device bytes a generated network will take (admission control). */
+uint64_t simuse_gen_bytes(const simuse_gen_desc* g);
+
+/* ---------------------------------------------------------------------------
+ * Serving: continuous batching of simulation requests.  No reference
+ * counterpart.  A server thread admits queued requests in arrival order while
+ * the HBM budget allows (estimated bytes of the request), builds them on the
+ * device, steps every live network together (one simuse_batch) and every live
+ * Kuramoto graph on its own stream, and retires each request at its horizon.
+ * ------------------------------------------------------------------------- */
+/* Directed modular small-world graph built on the device: `modules` rings of
+ * n/modules nodes; node i takes k in-edges from its lattice neighbours
+ * i±1..±k/2, each redrawn with probability `inter` from another module, else
+ * with probability `beta` from its own module; W_ij = coupling/k;
+ * omega ~ N(0,1), theta0 ~ U(-pi, pi). */
+typedef struct simuse_kura_gen_desc {
+    uint32_t n;
+    uint32_t modules;
+    uint32_t k;
+    float beta;
+    float inter;
+    float coupling;
+    float dt;
+    uint64_t seed;
+} simuse_kura_gen_desc;
+
+#define SIMUSE_REQ_NETWORK 0
+#define SIMUSE_REQ_KURAMOTO 1
+
+#define SIMUSE_REQ_QUEUED 0
+#define SIMUSE_REQ_RUNNING 1
+#define SIMUSE_REQ_DONE 2
+#define SIMUSE_REQ_REJECTED (-1) /* larger than the whole budget */
+#define SIMUSE_REQ_FAILED (-2)
+
+typedef struct simuse_server simuse_server;
+
+typedef struct simuse_request {
+    int32_t kind;                /* SIMUSE_REQ_NETWORK or SIMUSE_REQ_KURAMOTO */
+    simuse_gen_desc net;         /* network requests */
+    simuse_kura_gen_desc kura;   /* Kuramoto requests (RK4) */
+    uint64_t horizon;            /* steps */
+} simuse_request;
+
+typedef struct simuse_result {
+    int32_t status;              /* SIMUSE_REQ_* */
+    uint64_t steps;
+    uint64_t spikes;             /* network: spikes over the horizon */
+    uint64_t events;             /* network: synaptic events over the horizon */
+    double order_parameter;      /* Kuramoto: R at the horizon */
+    double t_submit, t_admit, t_done; /* seconds since the server started */
+    uint64_t bytes;              /* admission estimate */
+} simuse_result;
+
+typedef struct simuse_server_stats_t {
+    uint64_t submitted, completed, live, queued;
+    uint64_t live_bytes, peak_bytes;
+    uint32_t peak_live;
+    double now;
+} simuse_server_stats_t;
+
+int simuse_server_create(uint64_t hbm_budget_bytes, uint32_t chunk_steps, simuse_server** out);
+int simuse_server_submit(simuse_server* s, const simuse_request* r, uint64_t* id);
+int simuse_server_result(simuse_server* s, uint64_t id, simuse_result* out);
+/* Waits until every submitted request has completed (or timeout_s). */
+int simuse_server_drain(simuse_server* s, double timeout_s);
+int simuse_server_stats(simuse_server* s, simuse_server_stats_t* out);
+void simuse_server_destroy(simuse_server* s);
+
+/* ---------------------------------------------------------------------------
  * Kuramoto oscillators on a CSR graph.
  * Coupling c_i = sum_j W_ij sin(theta_j - theta_i) over row i; theta wraps to
  * [-pi, pi).  SIMUSE_KURA_EULER is the reference's integrator
@@ -187,6 +334,10 @@ typedef struct simuse_kura simuse_kura;
 int simuse_kura_create(const uint32_t* row_ptr, const uint32_t* col, const float* w, const float* omega,
                        const float* theta0, uint32_t n, float dt, int32_t method, simuse_kura** out);
 void simuse_kura_destroy(simuse_kura* k);
+
+int simuse_kura_create_generated(const simuse_kura_gen_desc* g, int32_t method, simuse_kura** out);
+int simuse_kura_download_graph(simuse_kura* k, uint32_t* row_ptr, uint32_t* col, float* w, float* omega);
+int simuse_kura_info(simuse_kura* k, uint32_t* n, uint64_t* n_edges);
 /* lanes per CSR row in the gather (power of two 1..64; default ~ mean degree / 4) */
 int simuse_kura_set_group(simuse_kura* k, int32_t lanes);
 int simuse_kura_step_async(simuse_kura* k, uint32_t n_steps);

@@ -79,7 +79,7 @@ def random_net(n, k_mean, seed, model="izh", max_delay=0, inh_frac=0.2, noise=5.
     elif model == "lif":
         dt = 0.1 if dt is None else dt
         v0 = rng.uniform(-65, -50, n).astype(np.float32)
-        pop = lif_pop(n, dt, 0.05, -65.0, -50.0, -65.0, 2.0, v0)
+        pop = lif_pop(n, dt, 0.05, -48.0, -50.0, -65.0, 2.0, v0)  # v_rest above threshold: tonic firing
     else:
         dt = 0.01 if dt is None else dt
         pop = hh_pop(n, dt, v0=rng.uniform(-70, -60, n).astype(np.float32))

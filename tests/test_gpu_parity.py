@@ -189,7 +189,7 @@ def test_brunel_lif_small_bitexact():
     # configs[0] shape at reduced size: LIF, p=0.01, delays 1..15, Poisson 2 kHz per neuron
     from simuse.workloads import brunel_lif_network
     say("brunel_small")
-    spec = brunel_lif_network(n=2000, p=0.05, seed=2, steps=1500)
+    spec = brunel_lif_network(n=2000, p=0.05, seed=2, steps=1500, rate_hz=20000.0)
     dev, ora = run_pair(spec, 1500, async_steps=True)
     assert ora.counters()[0] > 0
     assert dev.counters() == ora.counters()
