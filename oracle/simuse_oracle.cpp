@@ -360,6 +360,7 @@ struct Net final : NetBase {
     std::vector<uint8_t> plastic;
     std::vector<R> tr_pre, tr_post;
     std::vector<uint32_t> in_ptr, in_edge, edge_pre;
+    std::vector<double> w64;  // FP64 master weights while plasticity is on
 
     int init(const orc_pop_desc* pd, int np_, const uint32_t* rp, const uint32_t* c,
              const float* wf, const uint8_t* dl, uint64_t n_syn) {
@@ -423,6 +424,8 @@ struct Net final : NetBase {
         sp = *p;
         size_t ns = col.size();
         plastic.assign(pl, pl + ns);
+        w64.resize(ns);
+        for (size_t e = 0; e < ns; ++e) w64[e] = static_cast<double>(w[e]);
         tr_pre.assign(N, R(0));
         tr_post.assign(N, R(0));
         edge_pre.resize(ns);
@@ -498,19 +501,26 @@ struct Net final : NetBase {
             tr_pre[i] = std::fma(tr_pre[i], dpre, add);
             tr_post[i] = std::fma(tr_post[i], dpost, add);
         }
-        const R ap = R(sp.a_plus), am = R(sp.a_minus), wlo = R(sp.w_min), whi = R(sp.w_max);
+        // Weights accumulate in FP64 (w64) and are mirrored to the twin's
+        // precision; the increment itself is formed in the twin's precision.
+        const R ap = R(sp.a_plus), am = R(sp.a_minus);
+        const double wlo = static_cast<double>(sp.w_min), whi = static_cast<double>(sp.w_max);
         for (uint32_t post : act)
             for (uint32_t k = in_ptr[post]; k < in_ptr[post + 1]; ++k) {
                 uint32_t e = in_edge[k];
                 if (!plastic[e]) continue;
-                R nw = w[e] + ap * tr_pre[edge_pre[e]];
-                w[e] = std::min(std::max(nw, wlo), whi);
+                const R dw = ap * tr_pre[edge_pre[e]];
+                const double nw = std::min(std::max(w64[e] + static_cast<double>(dw), wlo), whi);
+                w64[e] = nw;
+                w[e] = R(nw);
             }
         for (uint32_t pre : act)
             for (uint32_t e = row_ptr[pre]; e < row_ptr[pre + 1]; ++e) {
                 if (!plastic[e]) continue;
-                R nw = w[e] - am * tr_post[col[e]];
-                w[e] = std::min(std::max(nw, wlo), whi);
+                const R dw = am * tr_post[col[e]];
+                const double nw = std::min(std::max(w64[e] - static_cast<double>(dw), wlo), whi);
+                w64[e] = nw;
+                w[e] = R(nw);
             }
     }
 
