@@ -191,7 +191,7 @@ int validate_gen(const simuse_gen_desc* g) {
     if (!g) return fail(SIMUSE_E_INVALID, "null generator description");
     if (g->model != SIMUSE_MODEL_LIF && g->model != SIMUSE_MODEL_IZHIKEVICH)
         return fail(SIMUSE_E_UNSUPPORTED, "generator supports LIF and Izhikevich networks");
-    if (g->n < 2 || g->n >= 0x7FFFFFFFu) return fail(SIMUSE_E_INVALID, "generator: n must be in [2, 2^31)");
+    if (g->n < 2 || g->n > kMaxNeurons) return fail(SIMUSE_E_INVALID, "generator: n must be in [2, 2^27]");
     if (static_cast<uint64_t>(g->n) * g->k >= 0xFFFFFFFFull)
         return fail(SIMUSE_E_INVALID, "generator: n*k exceeds the u32 CSR");
     if (!(g->dt > 0.0f) || !std::isfinite(g->dt)) return fail(SIMUSE_E_INVALID, "generator: dt must be > 0");
@@ -207,8 +207,9 @@ int validate_gen(const simuse_gen_desc* g) {
 uint64_t generated_bytes(const simuse_gen_desc& g) {
     const uint64_t n = g.n, ns = n * g.k;
     const uint64_t tiles = (n + kTile - 1) / kTile + 2;
-    uint64_t b = ns * (4 + 4 + 1 + 4);                       // col, w, delay, chain links
-    b += n * 4 * 2 + n * 32 * (4 + 4);                       // row_ptr, ring, chain heads
+    uint64_t b = ns * (4 + 4 + 1);                           // col, w, delay
+    b += ns * (8 + 4);                                       // event records, collision lists
+    b += n * 4 * 2 + n * 32 * (4 + 8);                       // row_ptr, ring, tagged heads
     b += n * 4 * 10 + tiles * (kTile * 4 + 4 + 8 + 8 + 8);   // state/params, compaction, counters
     return b + (1u << 20);
 }
@@ -368,6 +369,11 @@ int simuse_batch_set_scatter_mode(simuse_batch* b, int32_t mode) {
     if (!b) return fail(SIMUSE_E_INVALID, "null batch handle");
     if (mode == SIMUSE_SCATTER_AUTO) return b->eng.autotune(16);
     return b->eng.set_sched(mode);
+}
+
+int simuse_batch_set_block_tiles(simuse_batch* b, uint32_t tiles) {
+    if (!b) return fail(SIMUSE_E_INVALID, "null batch handle");
+    return b->eng.set_group(tiles);
 }
 
 int simuse_batch_autotune(simuse_batch* b, uint32_t trial_steps, int32_t* mode) {

@@ -29,10 +29,11 @@ namespace simuse {
 
 int fail(int code, const char* fmt, ...);
 
-void launch_neuron(int model, const NetArgs* nets, const uint2* tiles, uint32_t n, hipStream_t s);
+void launch_neuron(int model, int sched, const NetArgs* nets, const uint4* items, uint32_t n, hipStream_t s);
 void launch_stdp(const NetArgs* nets, const uint2* tiles, uint32_t n, hipStream_t s);
 void launch_insert(int sched, const NetArgs* nets, const uint2* blocks, uint32_t n, hipStream_t s);
-void launch_fold(int sched, const NetArgs* nets, const uint2* blocks, uint32_t n, hipStream_t s);
+void launch_fixup(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s);
+void launch_head_clear(unsigned long long* head, const uint64_t* step, size_t n, hipStream_t s);
 void launch_inject(float* ring, const uint64_t* step, uint32_t N, const uint32_t* idx, const float* val,
                    uint32_t count, int serial, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
@@ -94,8 +95,13 @@ struct NetState {
     PopDev* d_pops = nullptr;
     uint32_t* d_tile_pop = nullptr;
     float* ring = nullptr;
-    uint32_t* head = nullptr;
-    uint32_t* next = nullptr;
+    unsigned long long* head = nullptr;
+    uint2* rec = nullptr;
+    uint2* coll = nullptr;
+    uint32_t* tile_coll = nullptr;
+    uint32_t* tile_ebase = nullptr;
+    uint32_t n_fix = 1;
+    uint64_t since_clear = 0;  // steps enqueued since the heads were last rewritten
     uint32_t* row_ptr = nullptr;
     uint32_t* col = nullptr;
     float* w = nullptr;
@@ -170,11 +176,15 @@ struct NetState {
         a.n_src_tiles = n_src_tiles;
         a.n_src = n_src;
         a.n_blocks = n_tiles + n_src_tiles;
+        a.n_fix = n_fix;
         a.n_syn = n_syn;
         a.step_limit = step_limit;
         a.ring = ring;
         a.head = head;
-        a.next = next;
+        a.rec = rec;
+        a.coll = coll;
+        a.tile_coll = tile_coll;
+        a.tile_ebase = tile_ebase;
         a.row_ptr = row_ptr;
         a.col = col;
         a.w = w;
@@ -220,6 +230,10 @@ struct NetState {
     int upload_pops(const simuse_pop_desc* pd);
     int upload_pop_table();
     int reset_dynamic(hipStream_t stream);
+    void clear_heads(hipStream_t stream) {
+        launch_head_clear(head, step, static_cast<size_t>(kSlots) * N, stream);
+        since_clear = 0;
+    }
     int add_source(float rate_hz, float dt_ms, uint32_t seed, const uint32_t* posts, const float* wf,
                    const uint8_t* dl, uint32_t count);
     int set_noise(int32_t pop, const float* sigma, uint64_t seed);
@@ -246,18 +260,22 @@ struct Engine {
     // device tables
     NetArgs* d_nets = nullptr;
     uint32_t nets_cap = 0;
-    uint2* d_tiles[3] = {nullptr, nullptr, nullptr};
-    uint32_t n_tiles_m[3] = {0, 0, 0}, cap_tiles_m[3] = {0, 0, 0};
-    uint2* d_blocks = nullptr;
+    uint4* d_tiles[3] = {nullptr, nullptr, nullptr};  // neuron work items per model:
+    uint32_t n_tiles_m[3] = {0, 0, 0}, cap_tiles_m[3] = {0, 0, 0};  // {net, tile, tiles, pop}
+    uint2* d_blocks = nullptr;  // insert_kernel work: plastic networks' tiles, source tiles
     uint32_t n_blocks = 0, cap_blocks = 0;
     uint2* d_stdp = nullptr;
     uint32_t n_stdp = 0, cap_stdp = 0;
+    uint2* d_fix = nullptr;
+    uint32_t n_fix = 0, cap_fix = 0;
+    bool direct_fix = false;
     // execution
     int sched = SIMUSE_SCATTER_TILE;
+    uint32_t group = kMaxGroup;  // neuron tiles per block
     uint32_t graph_steps = 8;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    static constexpr int kPhases = 4;  // neuron, stdp, insert, fold
+    static constexpr int kPhases = 4;  // neuron (+ fused scatter), stdp, insert, fix-up
     bool profiling = false;
     std::vector<hipEvent_t> ev;
     uint32_t prof_steps = 0;
@@ -270,7 +288,8 @@ struct Engine {
         drop_graph();
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         for (void* p : {static_cast<void*>(d_nets), static_cast<void*>(d_tiles[0]), static_cast<void*>(d_tiles[1]),
-                        static_cast<void*>(d_tiles[2]), static_cast<void*>(d_blocks), static_cast<void*>(d_stdp)})
+                        static_cast<void*>(d_tiles[2]), static_cast<void*>(d_blocks), static_cast<void*>(d_stdp),
+                        static_cast<void*>(d_fix)})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -310,16 +329,24 @@ struct Engine {
         if (!dirty) return SIMUSE_OK;
         HIP_TRY(hipStreamSynchronize(stream));
         std::vector<NetArgs> na;
-        std::vector<uint2> tl[3], bl, sl;
+        std::vector<uint4> tl[3];
+        std::vector<uint2> bl, sl, fl;
         for (uint32_t k = 0; k < nets.size(); ++k) {
             NetState* s = nets[k];
             na.push_back(s->args());
-            for (const auto& P : s->pops)
-                for (uint32_t t = 0; t < P.ntiles; ++t) tl[P.model].push_back(make_uint2(k, P.tile0 + t));
-            for (uint32_t b = 0; b < s->n_tiles + s->n_src_tiles; ++b) bl.push_back(make_uint2(k, b));
+            for (uint32_t p = 0; p < s->pops.size(); ++p) {
+                const HostPop& P = s->pops[p];
+                for (uint32_t t = 0; t < P.ntiles; t += group)
+                    tl[P.model].push_back(make_uint4(k, P.tile0 + t, std::min(group, P.ntiles - t), p));
+            }
+            // plastic networks scatter after the STDP kernels; sources always
+            for (uint32_t b = s->stdp ? 0 : s->n_tiles; b < s->n_tiles + s->n_src_tiles; ++b)
+                bl.push_back(make_uint2(k, b));
             if (s->stdp)
                 for (uint32_t t = 0; t < s->n_tiles; ++t) sl.push_back(make_uint2(k, t));
+            for (uint32_t j = 0; j < s->n_fix; ++j) fl.push_back(make_uint2(k, j));
         }
+        direct_fix = nets.size() == 1;  // one network: fix-up block j is blockIdx.x
         int rc;
         if (!na.empty()) {
             if ((rc = grow(&d_nets, &nets_cap, static_cast<uint32_t>(na.size())))) return rc;
@@ -329,7 +356,7 @@ struct Engine {
             n_tiles_m[m] = static_cast<uint32_t>(tl[m].size());
             if (tl[m].empty()) continue;
             if ((rc = grow(&d_tiles[m], &cap_tiles_m[m], n_tiles_m[m]))) return rc;
-            HIP_TRY(hipMemcpy(d_tiles[m], tl[m].data(), sizeof(uint2) * tl[m].size(), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(d_tiles[m], tl[m].data(), sizeof(uint4) * tl[m].size(), hipMemcpyHostToDevice));
         }
         n_blocks = static_cast<uint32_t>(bl.size());
         if (n_blocks) {
@@ -341,6 +368,11 @@ struct Engine {
             if ((rc = grow(&d_stdp, &cap_stdp, n_stdp))) return rc;
             HIP_TRY(hipMemcpy(d_stdp, sl.data(), sizeof(uint2) * sl.size(), hipMemcpyHostToDevice));
         }
+        n_fix = static_cast<uint32_t>(fl.size());
+        if (n_fix) {
+            if ((rc = grow(&d_fix, &cap_fix, n_fix))) return rc;
+            HIP_TRY(hipMemcpy(d_fix, fl.data(), sizeof(uint2) * fl.size(), hipMemcpyHostToDevice));
+        }
         drop_graph();
         dirty = false;
         return SIMUSE_OK;
@@ -348,13 +380,13 @@ struct Engine {
 
     void launch_step(hipEvent_t* e) {
         if (e) (void)hipEventRecord(e[0], stream);
-        for (int m = 0; m < 3; ++m) launch_neuron(m, d_nets, d_tiles[m], n_tiles_m[m], stream);
+        for (int m = 0; m < 3; ++m) launch_neuron(m, sched, d_nets, d_tiles[m], n_tiles_m[m], stream);
         if (e) (void)hipEventRecord(e[1], stream);
         launch_stdp(d_nets, d_stdp, n_stdp, stream);
         if (e) (void)hipEventRecord(e[2], stream);
         launch_insert(sched, d_nets, d_blocks, n_blocks, stream);
         if (e) (void)hipEventRecord(e[3], stream);
-        launch_fold(sched, d_nets, d_blocks, n_blocks, stream);
+        launch_fixup(d_nets, direct_fix ? nullptr : d_fix, n_fix, stream);
         if (e) (void)hipEventRecord(e[4], stream);
     }
 
@@ -399,6 +431,16 @@ struct Engine {
         int rc = prepare();
         if (rc) return rc;
         if (n == 0 || nets.empty()) return SIMUSE_OK;
+        while (n > (1u << 30)) {
+            if ((rc = step_async(1u << 30))) return rc;
+            n -= 1u << 30;
+        }
+        // Scatter heads carry the low 32 bits of the step as a tag; rewrite
+        // them before 2^31 steps pass so that no stale tag comes round again.
+        for (NetState* s : nets) {
+            if (s->since_clear + n >= (1ull << 31)) s->clear_heads(stream);
+            s->since_clear += n;
+        }
         if (profiling) {
             const size_t need = static_cast<size_t>(prof_steps + n) * (kPhases + 1);
             while (ev.size() < need) {
@@ -458,6 +500,15 @@ struct Engine {
         if (sched != mode) {
             sched = mode;
             drop_graph();
+        }
+        return SIMUSE_OK;
+    }
+
+    int set_group(uint32_t g) {
+        if (g < 1 || g > kMaxGroup) return fail(SIMUSE_E_INVALID, "tiles per block must be 1..%u", kMaxGroup);
+        if (group != g) {
+            group = g;
+            mark_dirty();
         }
         return SIMUSE_OK;
     }

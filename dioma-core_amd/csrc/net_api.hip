@@ -40,8 +40,10 @@ int validate_pops(const simuse_pop_desc* pops, int32_t n_pops, uint64_t* n_total
         tot += d.n;
     }
     if (tot == 0) return fail(SIMUSE_E_INVALID, "CompiledNetwork: no neurons");
-    if (tot >= 0xFFFFFFFFull)
-        return fail(SIMUSE_E_INVALID, "total neuron count %llu exceeds u32", static_cast<unsigned long long>(tot));
+    // Ring cells are addressed as slot*N + post in 32 bits (32 slots).
+    if (tot > kMaxNeurons)
+        return fail(SIMUSE_E_INVALID, "total neuron count %llu exceeds %u per network",
+                    static_cast<unsigned long long>(tot), kMaxNeurons);
     *n_total = tot;
     return SIMUSE_OK;
 }
@@ -152,7 +154,23 @@ int NetState::finalize(hipStream_t stream) {
         return fail(SIMUSE_E_INVALID, "edges + source edges (%llu) exceed the u32 event id range",
                     static_cast<unsigned long long>(n_syn + n_src_edges));
     int rc;
-    if ((rc = alloc(&next, n_syn + n_src_edges))) return rc;
+    const uint64_t n_ev = n_syn + n_src_edges;
+    const uint32_t n_blk = n_tiles + n_src_tiles;
+    if ((rc = alloc(&rec, n_ev))) return rc;
+    if ((rc = alloc(&coll, n_ev))) return rc;
+    if ((rc = alloc(&tile_coll, n_blk))) return rc;
+    if ((rc = alloc(&tile_ebase, n_blk))) return rc;
+    HIP_TRY(hipMemsetAsync(tile_coll, 0, sizeof(uint32_t) * n_blk, stream));
+    // First event id of every scatter tile's rows: a tile's events, and so its
+    // collisions, fit in [tile_ebase, next tile's tile_ebase).
+    {
+        std::vector<uint32_t> eb(n_blk);
+        for (const HostPop& P : pops)
+            for (uint32_t t = 0; t < P.ntiles; ++t) eb[P.tile0 + t] = h_row_ptr[P.off + t * kTile];
+        for (uint32_t j = 0; j < n_src_tiles; ++j) eb[n_tiles + j] = static_cast<uint32_t>(n_syn + rp[j * kTile]);
+        if (n_blk) HIP_TRY(hipMemcpy(tile_ebase, eb.data(), 4 * eb.size(), hipMemcpyHostToDevice));
+    }
+    n_fix = std::max<uint32_t>(1u, (n_blk + kFixTiles - 1) / kFixTiles);
     if ((rc = alloc(&tile_events, static_cast<size_t>(n_tiles) + n_src_tiles))) return rc;
     HIP_TRY(hipMemsetAsync(tile_events, 0, sizeof(uint64_t) * (n_tiles + n_src_tiles), stream));
     if (n_src) {
@@ -243,8 +261,8 @@ int NetState::upload_pop_table() {
 
 int NetState::reset_dynamic(hipStream_t stream) {
     HIP_TRY(hipMemsetAsync(ring, 0, sizeof(float) * static_cast<size_t>(kSlots) * N, stream));
-    launch_fill_u32(head, kEmpty, static_cast<size_t>(kSlots) * N, stream);
     HIP_TRY(hipMemsetAsync(step, 0, sizeof(uint64_t) * 2, stream));
+    clear_heads(stream);
     HIP_TRY(hipMemsetAsync(done, 0, sizeof(uint32_t), stream));
     HIP_TRY(hipMemsetAsync(tile_count, 0, sizeof(uint32_t) * n_tiles, stream));
     HIP_TRY(hipMemsetAsync(bits, 0, sizeof(uint64_t) * n_words, stream));
@@ -650,18 +668,33 @@ int Engine::autotune(uint32_t trial_steps) {
         }
     }
     if ((rc = snap.copy(stream, true))) return rc;
-    const int cand[2] = {SIMUSE_SCATTER_TILE, SIMUSE_SCATTER_WAVE};
+    // Restoring rewinds the step counters; the scatter heads written by the
+    // trial steps carry later tags and are rewritten as idle.
+    auto restore = [&]() {
+        int r = snap.copy(stream, false);
+        if (r) return r;
+        for (NetState* s : nets) s->clear_heads(stream);
+        HIP_TRY(hipGetLastError());
+        return SIMUSE_OK;
+    };
+    // Candidates: scatter schedule x neuron tiles per block.
+    const int cand_s[2] = {SIMUSE_SCATTER_TILE, SIMUSE_SCATTER_WAVE};
+    const uint32_t cand_g[3] = {kMaxGroup, 2, 1};
     const int saved = sched;
+    const uint32_t saved_g = group;
     float best_ms = 0;
     int best = sched;
+    uint32_t best_g = group;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    for (int c = 0; c < 2 && rc == SIMUSE_OK; ++c) {
-        sched = cand[c];
-        if ((rc = snap.copy(stream, false))) break;
+    for (int c = 0; c < 6 && rc == SIMUSE_OK; ++c) {
+        sched = cand_s[c % 2];
+        if ((rc = set_group(cand_g[c / 2]))) break;
+        if ((rc = ensure_tables())) break;
+        if ((rc = restore())) break;
         if ((rc = enqueue_steps(2))) break;  // warm-up
-        if ((rc = snap.copy(stream, false))) break;
+        if ((rc = restore())) break;
         hipError_t e = hipEventRecord(e0, stream);
         if (e == hipSuccess && (rc = enqueue_steps(trial_steps))) break;
         if (e == hipSuccess) e = hipEventRecord(e1, stream);
@@ -674,15 +707,20 @@ int Engine::autotune(uint32_t trial_steps) {
         }
         if (c == 0 || ms < best_ms) {
             best_ms = ms;
-            best = cand[c];
+            best = sched;
+            best_g = group;
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    const int rc2 = snap.copy(stream, false);
+    const int rc2 = restore();
     sched = saved;
-    if (rc) return rc;
+    if (rc) {
+        (void)set_group(saved_g);
+        return rc;
+    }
     if (rc2) return rc2;
+    if ((rc = set_group(best_g))) return rc;
     return set_sched(best);
 }
 
@@ -795,6 +833,19 @@ int simuse_net_get_scatter_mode(simuse_net* net, int32_t* mode) {
     int rc = check_net(net);
     if (rc) return rc;
     if (mode) *mode = net->eng.sched;
+    return SIMUSE_OK;
+}
+
+int simuse_net_set_block_tiles(simuse_net* net, uint32_t tiles) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    return net->eng.set_group(tiles);
+}
+
+int simuse_net_get_block_tiles(simuse_net* net, uint32_t* tiles) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    if (tiles) *tiles = net->eng.group;
     return SIMUSE_OK;
 }
 

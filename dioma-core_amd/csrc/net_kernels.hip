@@ -1,20 +1,19 @@
 // net_kernels.hip — per-timestep kernels of the SiMuse network step on gfx950.
 //
 // One step of one network is, in stream order:
-//   1. neuron_kernel<MODEL>   (one launch per model present)
-//        read ring slot s%32, zero it, add Gaussian noise, clamp (LIF/Izh),
-//        advance the neuron, update STDP traces, wave-ballot the spike mask
-//        and write the tile's spiking ids in ascending order.
-//   2. ltp_kernel / ltd_kernel (only with pair-based STDP)
-//   3. insert_kernel          every synaptic event (e = edge id) pushes
-//        itself onto the chain of its (target slot, post) key with one
-//        atomicExch; Poisson sources roll and push their edges the same way.
-//   4. fold_kernel            the first event of each key walks the chain,
-//        orders it by edge id — which is exactly the CPU push order: ascending
-//        pre, CSR row order, then sources (P/src/api/net/push.rs:143-171,
-//        P/src/api/net/network/step.rs:166-185) — and adds the weights into
-//        the ring one by one.  No float atomics: the ring is bit-identical to
-//        the sequential CPU push.  The last block advances the step counter.
+//   1. neuron_kernel<MODEL,SCHED>  (one launch per model present)
+//        read ring slot s%32, zero it, add drive and Gaussian noise, clamp
+//        (LIF/Izh), advance the neuron, update STDP traces, wave-ballot the
+//        spike mask and write the tile's spiking ids in ascending order; for
+//        networks without plasticity, scatter the tile's spikes right away.
+//   2. ltp_kernel / ltd_kernel     (only with pair-based STDP)
+//   3. insert_kernel<SCHED>        scatter of plastic networks' spikes and of
+//        the Poisson sources that fire this step
+//   4. fixup_kernel                re-sums, in ascending event id, every ring
+//        cell that received more than one event; the last block advances the
+//        step counter.
+// The ring ends each step bit-identical to the CPU's sequential push
+// (P/src/api/net/push.rs:143-171): no float atomics.
 //
 // All arithmetic is compiled with -ffp-contract=off so every multiply and add
 // rounds separately, as in the reference's non-FMA build.
@@ -153,121 +152,324 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t* s
     return total;
 }
 
-template <int MODEL>
-__global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict__ nets,
-                                                      const uint2* __restrict__ tiles) {
-    const uint2 ref = tiles[blockIdx.x];
-    const NetArgs& a = nets[ref.x];
-    const uint32_t tile = ref.y;
-    const uint64_t step = *a.step;
-    if (step >= a.step_limit) return;
-    const PopDev& P = a.pops[a.tile_pop[tile]];
-    const uint32_t slot = static_cast<uint32_t>(step & (kSlots - 1));
-    const uint32_t li = (tile - P.tile0) * kTile + threadIdx.x;
-    int spike = 0;
-    int drive_now = 0;  // drive source of this neuron fires this step (one event)
-    if (li < P.n) {
-        const uint32_t gi = P.off + li;
-        float* rs = a.ring + static_cast<size_t>(slot) * a.N + gi;
-        float I = *rs;
-        *rs = 0.0f;
-        // A per-neuron Poisson source with one edge of delay <= 1 adds its
-        // weight into the slot after every neuron push of the previous step
-        // (P/src/api/net/network/step.rs:166-185), i.e. as the last add.
-        if ((P.flags & kPopDrive) && P.drive_p > 0.0f) {
-            const uint32_t sd = P.drive_seed0 + li * 2654435761u;
-            if (step > 0 && poisson_roll(sd, 0u, step - 1, P.drive_p)) I = I + P.drive_w;
-            drive_now = poisson_roll(sd, 0u, step, P.drive_p) ? 1 : 0;
-        }
-        if (P.flags & kPopNoise) {
-            const float sg = (P.flags & kPopNoiseScalar) ? P.sigma_uni : P.sigma[li];
-            I = I + sg * normal_draw(P.seed, li, step);
-        }
-        const float dt = P.puni[0];
-        if (MODEL == 0) {
-            I = fminf(fmaxf(I, -100.0f), 100.0f);  // population.rs:79-88
-            float* s = P.st;
-            float v = s[li];
-            float refr = (P.flags & kPopRefrElided) ? 0.0f : s[P.dstride + li];
-            spike = lif_update(v, refr, I, dt, par(P, 1, li), par(P, 2, li), par(P, 3, li),
-                               par(P, 4, li), par(P, 5, li));
-            s[li] = v;
-            if (!(P.flags & kPopRefrElided)) s[P.dstride + li] = refr;
-        } else if (MODEL == 1) {
-            I = fminf(fmaxf(I, -100.0f), 100.0f);
-            float* s = P.st;
-            float v = s[li], u = s[P.dstride + li];
-            float refr = (P.flags & kPopRefrElided) ? 0.0f : s[2 * P.dstride + li];
-            spike = izh_update(v, u, refr, I, dt, par(P, 1, li), par(P, 2, li), par(P, 3, li),
-                               par(P, 4, li), par(P, 5, li), par(P, 6, li));
-            s[li] = v;
-            s[P.dstride + li] = u;
-            if (!(P.flags & kPopRefrElided)) s[2 * P.dstride + li] = refr;
-        } else {
-            spike = hh_update(P, li, I, dt);  // #[no_clamp] (hh.rs:13)
-        }
-        if (a.stdp) {
-            const float add = spike ? 1.0f : 0.0f;
-            a.tr_pre[gi] = fmaf(a.tr_pre[gi], a.stdp_dpre, add);
-            a.tr_post[gi] = fmaf(a.tr_post[gi], a.stdp_dpost, add);
-        }
-    }
-    // Ordered compaction: wave ballot + block prefix over the 4 waves.
-    const uint64_t m = __ballot(spike);
-    const uint64_t dm = __ballot(drive_now);
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    __shared__ uint32_t s_cnt[kWavesPerTile], s_drv[kWavesPerTile];
-    const uint32_t local_tile = tile - P.tile0;
-    if (lane == 0) {
-        s_cnt[wave] = static_cast<uint32_t>(__popcll(m));
-        s_drv[wave] = static_cast<uint32_t>(__popcll(dm));
-        const uint32_t word = P.word0 + local_tile * kWavesPerTile + wave;
-        a.bits[word] = m;
-        if (a.raster) {
-            const uint64_t rs = step - *a.raster_base;
-            if (rs < a.raster_cap) a.raster[rs * a.n_words + word] = m;
-        }
-    }
-    __syncthreads();
-    uint32_t base = 0, total = 0;
-#pragma unroll
-    for (uint32_t w = 0; w < kWavesPerTile; ++w) {
-        const uint32_t c = s_cnt[w];
-        if (w < wave) base += c;
-        total += c;
-    }
-    if (spike) {
-        const uint32_t rank = static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
-        a.active[static_cast<size_t>(tile) * kTile + base + rank] = P.off + li;
-    }
-    if (threadIdx.x == 0) {
-        a.tile_count[tile] = total;
-        a.tile_spikes[tile] += total;
-        // each firing drive source is one synaptic event, counted at its roll
-        const uint32_t dtot = s_drv[0] + s_drv[1] + s_drv[2] + s_drv[3];
-        if (dtot) a.tile_events[tile] += dtot;
+// ---------------------------------------------------------------------------
+// Scatter.  The CPU push adds each synaptic event's weight into ring cell
+// key = ((step + max(d,1)) % 32) * N + post, pre ascending and in CSR row
+// order, then the Poisson sources (P/src/api/net/push.rs:143-171,
+// P/src/api/net/network/step.rs:166-185): per key the adds happen in
+// ascending event id.  On the device every event exchanges itself into the
+// key's tagged head.  The event that finds the head idle (another step's tag)
+// is the key's first: it adds its weight into the ring at once and keeps the
+// value it read.  An event that finds the head taken links to the event it
+// displaced and enters its tile's collision list.  fixup_kernel re-sums each
+// key that received more than one event, from the kept value in ascending
+// event id, so the ring ends bit-identical to the sequential push.  An event
+// alone on its key (the common case) costs one exchange and one ring
+// read-modify-write; no float atomics anywhere.
+// ---------------------------------------------------------------------------
+
+struct ScatterCtx {
+    uint64_t step;
+    uint32_t tag;       // low 32 bits of step
+    uint32_t ebase;     // collision list base of the tile
+    uint32_t* s_ncoll;  // collisions of the block so far (LDS)
+};
+
+__device__ __forceinline__ void scatter_event(const NetArgs& a, const ScatterCtx& c, uint32_t eid, uint32_t post,
+                                              uint32_t d, float wt) {
+    const uint32_t dd = d < 1 ? 1u : d;
+    const size_t key = static_cast<size_t>((c.step + dd) & (kSlots - 1)) * a.N + post;
+    // The ring read does not depend on the exchange, so both round trips
+    // overlap; only the key's first event uses the value.
+    const float r0 = a.ring[key];
+    const unsigned long long mine = (static_cast<unsigned long long>(c.tag) << 32) | eid;
+    const unsigned long long old = atomicExch(a.head + key, mine);
+    if (static_cast<uint32_t>(old >> 32) != c.tag) {
+        a.ring[key] = r0 + wt;
+        a.rec[eid] = make_uint2(kEmpty, __float_as_uint(r0));
+    } else {
+        a.rec[eid] = make_uint2(static_cast<uint32_t>(old), 0u);
+        const uint32_t pos = atomicAdd(c.s_ncoll, 1u);
+        a.coll[c.ebase + pos] = make_uint2(eid, static_cast<uint32_t>(key));
     }
 }
 
-// Per-tile event enumeration shared by insert and fold.  A neuron tile holds
-// up to kTile spiking neurons (ascending ids); a source tile holds up to kTile
-// Poisson sources that fired this step.  `visit(eid, post, d)` is called for
-// every outgoing edge; eid is the global event id (edge id, sources after
-// n_syn).
-struct TileRows {
-    uint32_t cnt;
-    uint32_t total;
-    uint32_t base_eid;  // 0 for neuron edges, n_syn for source edges
+// Visit every edge of `cnt` rows staged in LDS: row k starts at edge s_rs[k]
+// and its edges are numbered from s_pref[k] in the concatenation (`total`
+// edges).  SCHED 1: thread f takes edge f of the concatenation (binary search
+// over the prefix); SCHED 2: one wavefront per row, lanes stride the row.
+template <int SCHED, class Visit>
+__device__ __forceinline__ void for_rows(uint32_t cnt, uint32_t total, const uint32_t* s_rs, const uint32_t* s_pref,
+                                         Visit visit) {
+    const uint32_t t = threadIdx.x;
+    if (SCHED == 2) {
+        const uint32_t lane = t & 63u, wave = t >> 6;
+        for (uint32_t k = wave; k < cnt; k += kWavesPerTile) {
+            const uint32_t r0 = s_rs[k];
+            const uint32_t len = (k + 1 < cnt ? s_pref[k + 1] : total) - s_pref[k];
+            for (uint32_t j = lane; j < len; j += 64) visit(r0 + j);
+        }
+    } else {
+        for (uint32_t f = t; f < total; f += kTile) {
+            uint32_t lo = 0, hi = cnt;  // largest k with s_pref[k] <= f
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_pref[mid] <= f) lo = mid; else hi = mid;
+            }
+            visit(s_rs[lo] + (f - s_pref[lo]));
+        }
+    }
+}
+
+template <int SCHED>
+__device__ __forceinline__ void scatter_rows(const NetArgs& a, const ScatterCtx& c, bool is_src, uint32_t cnt,
+                                             uint32_t total, const uint32_t* s_rs, const uint32_t* s_pref) {
+    if (!is_src) {
+        const uint32_t* col = a.col;
+        const uint8_t* dly = a.delay;
+        const float* w = a.w;
+        for_rows<SCHED>(cnt, total, s_rs, s_pref,
+                        [&](uint32_t e) { scatter_event(a, c, e, col[e], dly[e], w[e]); });
+    } else {
+        const uint32_t base = static_cast<uint32_t>(a.n_syn);
+        const uint32_t* col = a.src_post;
+        const uint8_t* dly = a.src_delay;
+        const float* w = a.src_w;
+        for_rows<SCHED>(cnt, total, s_rs, s_pref,
+                        [&](uint32_t e) { scatter_event(a, c, base + e, col[e], dly[e], w[e]); });
+    }
+}
+
+// Per-model register state: load everything a neuron-step reads, advance,
+// store.  Splitting the three lets a block issue the loads of all its tiles
+// before the first one is needed.
+template <int MODEL>
+struct Neuron;
+
+template <>
+struct Neuron<0> {  // LIF: states v, refr; params dt, inv_tau, v_rest, v_th, v_reset, tau_ref
+    float v, refr, p1, p2, p3, p4, p5;
+    __device__ __forceinline__ void load(const PopDev& P, uint32_t li) {
+        v = P.st[li];
+        refr = (P.flags & kPopRefrElided) ? 0.0f : P.st[P.dstride + li];
+        p1 = par(P, 1, li); p2 = par(P, 2, li); p3 = par(P, 3, li); p4 = par(P, 4, li); p5 = par(P, 5, li);
+    }
+    __device__ __forceinline__ int step(const PopDev&, uint32_t, float I, float dt) {
+        I = fminf(fmaxf(I, -100.0f), 100.0f);  // population.rs:79-88
+        return lif_update(v, refr, I, dt, p1, p2, p3, p4, p5);
+    }
+    __device__ __forceinline__ void store(const PopDev& P, uint32_t li) const {
+        P.st[li] = v;
+        if (!(P.flags & kPopRefrElided)) P.st[P.dstride + li] = refr;
+    }
 };
 
-template <int SCHED, class Visit>
-__device__ __forceinline__ void for_tile_edges(const NetArgs& a, uint32_t b, uint64_t step, bool is_src,
-                                               uint32_t* s_rs, uint32_t* s_pref, uint32_t* s_wave,
-                                               uint32_t& out_total, Visit visit) {
-    const uint32_t t = threadIdx.x;
-    uint32_t cnt;
-    uint32_t rs = 0, dg = 0;
+template <>
+struct Neuron<1> {  // Izhikevich: states v, u, refr; params dt, a, b, c, d, tau_ref, v_th
+    float v, u, refr, p1, p2, p3, p4, p5, p6;
+    __device__ __forceinline__ void load(const PopDev& P, uint32_t li) {
+        v = P.st[li];
+        u = P.st[P.dstride + li];
+        refr = (P.flags & kPopRefrElided) ? 0.0f : P.st[2 * P.dstride + li];
+        p1 = par(P, 1, li); p2 = par(P, 2, li); p3 = par(P, 3, li);
+        p4 = par(P, 4, li); p5 = par(P, 5, li); p6 = par(P, 6, li);
+    }
+    __device__ __forceinline__ int step(const PopDev&, uint32_t, float I, float dt) {
+        I = fminf(fmaxf(I, -100.0f), 100.0f);
+        return izh_update(v, u, refr, I, dt, p1, p2, p3, p4, p5, p6);
+    }
+    __device__ __forceinline__ void store(const PopDev& P, uint32_t li) const {
+        P.st[li] = v;
+        P.st[P.dstride + li] = u;
+        if (!(P.flags & kPopRefrElided)) P.st[2 * P.dstride + li] = refr;
+    }
+};
+
+template <>
+struct Neuron<2> {  // Hodgkin-Huxley: reads and writes its columns inside the update
+    __device__ __forceinline__ void load(const PopDev&, uint32_t) {}
+    __device__ __forceinline__ int step(const PopDev& P, uint32_t li, float I, float dt) {
+        return hh_update(P, li, I, dt);  // #[no_clamp] (hh.rs:13)
+    }
+    __device__ __forceinline__ void store(const PopDev&, uint32_t) const {}
+};
+
+// One step of up to kMaxGroup consecutive tiles of one population: read ring
+// slot s%32 and zero it, add the Poisson drive and Gaussian noise, clamp
+// (LIF/Izhikevich), advance the neuron, update STDP traces, then wave-ballot
+// each tile's spike mask and write its spiking ids in ascending order.  For
+// networks without plasticity the block's spikes are scattered here as well
+// (the push only needs the spiking ids, which the block already holds); with
+// plasticity the weights change between update and push, so insert_kernel
+// scatters after the STDP kernels.  Work item: {network, first tile, tiles,
+// population}.
+template <int MODEL, int SCHED>
+__global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict__ nets,
+                                                      const uint4* __restrict__ items) {
+    __shared__ uint32_t s_rs[kMaxGroup * kTile], s_pref[kMaxGroup * kTile];
+    __shared__ uint32_t s_cnt[kMaxGroup][kWavesPerTile], s_deg[kMaxGroup][kWavesPerTile];
+    __shared__ uint32_t s_drv[kMaxGroup][kWavesPerTile];
+    __shared__ uint32_t s_ncoll;
+    const uint4 it = items[blockIdx.x];
+    const NetArgs& a = nets[it.x];
+    const PopDev& P = a.pops[it.w];
+    const uint32_t tile0 = it.y, ntile = it.z;
+    const bool fused = a.stdp == 0;
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    const uint32_t li0 = (tile0 - P.tile0) * kTile + t;
+
+    // Phase 1: the loads that do not need the step counter, for every tile.
+    Neuron<MODEL> nrn[kMaxGroup];
+    uint32_t row0[kMaxGroup], deg[kMaxGroup];
+#pragma unroll
+    for (uint32_t g = 0; g < kMaxGroup; ++g) {
+        row0[g] = deg[g] = 0u;
+        const uint32_t li = li0 + g * kTile;
+        if (g < ntile && li < P.n) {
+            nrn[g].load(P, li);
+            if (fused) {
+                row0[g] = a.row_ptr[P.off + li];
+                deg[g] = a.row_ptr[P.off + li + 1] - row0[g];
+            }
+        }
+    }
+    const uint64_t step = *a.step;
+    if (step >= a.step_limit) return;
+    float* ring_slot = a.ring + static_cast<size_t>(step & (kSlots - 1)) * a.N + P.off;
+    float cur[kMaxGroup];
+#pragma unroll
+    for (uint32_t g = 0; g < kMaxGroup; ++g) {
+        const uint32_t li = li0 + g * kTile;
+        cur[g] = 0.0f;
+        if (g < ntile && li < P.n) {
+            cur[g] = ring_slot[li];
+            ring_slot[li] = 0.0f;
+        }
+    }
+
+    // Phase 2: advance every neuron; per tile ballot the spikes and scan the
+    // spiking rows' lengths within each wave.
+    const float dt = P.puni[0];
+    uint64_t m[kMaxGroup];
+    uint32_t excl[kMaxGroup];  // edge prefix of the thread's row within its wave
+#pragma unroll
+    for (uint32_t g = 0; g < kMaxGroup; ++g) {
+        const uint32_t li = li0 + g * kTile;
+        int spike = 0, drive_now = 0;
+        if (g < ntile && li < P.n) {
+            float I = cur[g];
+            // A per-neuron Poisson source with one edge of delay <= 1 adds
+            // its weight into the slot after every neuron push of the previous
+            // step (P/src/api/net/network/step.rs:166-185): the last add.
+            if ((P.flags & kPopDrive) && P.drive_p > 0.0f) {
+                const uint32_t sd = P.drive_seed0 + li * 2654435761u;
+                if (step > 0 && poisson_roll(sd, 0u, step - 1, P.drive_p)) I = I + P.drive_w;
+                drive_now = poisson_roll(sd, 0u, step, P.drive_p) ? 1 : 0;
+            }
+            if (P.flags & kPopNoise) {
+                const float sg = (P.flags & kPopNoiseScalar) ? P.sigma_uni : P.sigma[li];
+                I = I + sg * normal_draw(P.seed, li, step);
+            }
+            spike = nrn[g].step(P, li, I, dt);
+            nrn[g].store(P, li);
+            if (a.stdp) {
+                const uint32_t gi = P.off + li;
+                const float add = spike ? 1.0f : 0.0f;
+                a.tr_pre[gi] = fmaf(a.tr_pre[gi], a.stdp_dpre, add);
+                a.tr_post[gi] = fmaf(a.tr_post[gi], a.stdp_dpost, add);
+            }
+        }
+        m[g] = __ballot(spike);
+        const uint32_t drive_events = static_cast<uint32_t>(__popcll(__ballot(drive_now)));
+        const uint32_t d = spike ? deg[g] : 0u;
+        uint32_t inc = d;
+        if (m[g]) {
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(inc, o, 64);
+                if (lane >= static_cast<uint32_t>(o)) inc += y;
+            }
+        }
+        excl[g] = inc - d;
+        if (g < ntile) {
+            if (lane == 63) s_deg[g][wave] = inc;
+            if (lane == 0) {
+                s_cnt[g][wave] = static_cast<uint32_t>(__popcll(m[g]));
+                s_drv[g][wave] = drive_events;
+                const uint32_t word = P.word0 + (tile0 - P.tile0 + g) * kWavesPerTile + wave;
+                a.bits[word] = m[g];
+                if (a.raster) {
+                    const uint64_t rs = step - *a.raster_base;
+                    if (rs < a.raster_cap) a.raster[rs * a.n_words + word] = m[g];
+                }
+            }
+        }
+    }
+    if (t == 0) s_ncoll = 0u;
+    __syncthreads();
+
+    // Phase 3: per-tile ordered compaction; the block's spiking rows are
+    // staged in (tile, wave, lane) order with their edge prefix.
+    uint32_t blk_sp = 0, blk_ed = 0, drv = 0;
+#pragma unroll
+    for (uint32_t g = 0; g < kMaxGroup; ++g) {
+        if (g < ntile) {
+            uint32_t base = 0, total = 0, dbase = 0, dtot = 0;
+#pragma unroll
+            for (uint32_t w = 0; w < kWavesPerTile; ++w) {
+                const uint32_t c = s_cnt[g][w], d = s_deg[g][w];
+                if (w < wave) { base += c; dbase += d; }
+                total += c;
+                dtot += d;
+                drv += s_drv[g][w];
+            }
+            if ((m[g] >> lane) & 1ull) {
+                const uint32_t rank = base + static_cast<uint32_t>(__popcll(m[g] & ((1ull << lane) - 1ull)));
+                a.active[static_cast<size_t>(tile0 + g) * kTile + rank] = P.off + li0 + g * kTile;
+                if (fused) {
+                    s_rs[blk_sp + rank] = row0[g];
+                    s_pref[blk_sp + rank] = blk_ed + dbase + excl[g];
+                }
+            }
+            if (t == 0) a.tile_count[tile0 + g] = total;
+            blk_sp += total;
+            blk_ed += dtot;
+        }
+    }
+    if (t == 0) {
+        if (blk_sp) atomicAdd(reinterpret_cast<unsigned long long*>(a.tile_spikes + tile0), blk_sp);
+        // each firing drive source is one synaptic event, counted at its roll;
+        // the fused scatter's events are counted here too
+        const uint32_t ev = drv + (fused ? blk_ed : 0u);
+        if (ev) atomicAdd(reinterpret_cast<unsigned long long*>(a.tile_events + tile0), ev);
+    }
+    if (!fused) return;
+    // the block's collisions are listed from its first tile's base
+    if (t < ntile && (t > 0 || blk_sp == 0)) a.tile_coll[tile0 + t] = 0u;
+    if (blk_sp == 0) return;
+    __syncthreads();
+    const ScatterCtx c{step, static_cast<uint32_t>(step), a.tile_ebase[tile0], &s_ncoll};
+    scatter_rows<SCHED>(a, c, false, blk_sp, blk_ed, s_rs, s_pref);
+    __syncthreads();
+    if (t == 0) a.tile_coll[tile0] = s_ncoll;
+}
+
+// Scatter of the neuron tiles of networks with plasticity (after the STDP
+// kernels) and of every Poisson-source tile.  A source tile rolls its up to
+// kTile sources for this step (P/src/api/net/network/step.rs:166-185).
+template <int SCHED>
+__global__ __launch_bounds__(kTile) void insert_kernel(const NetArgs* __restrict__ nets,
+                                                      const uint2* __restrict__ blocks) {
+    __shared__ uint32_t s_rs[kTile], s_pref[kTile], s_wave[kWavesPerTile], s_c[kWavesPerTile];
+    __shared__ uint32_t s_ncoll;
+    const uint2 ref = blocks[blockIdx.x];
+    const NetArgs& a = nets[ref.x];
+    const uint32_t b = ref.y, t = threadIdx.x;
+    const uint64_t step = *a.step;
+    if (step >= a.step_limit) return;
+    const bool is_src = b >= a.n_tiles;
     const uint32_t* rp = is_src ? a.src_row_ptr : a.row_ptr;
+    uint32_t cnt, rs = 0, dg = 0;
     if (!is_src) {
         cnt = a.tile_count[b];
         if (t < cnt) {
@@ -283,7 +485,6 @@ __device__ __forceinline__ void for_tile_edges(const NetArgs& a, uint32_t b, uin
             fired = (p > 0.0f && poisson_roll(a.src_seed[s], 0u, step, p)) ? 1 : 0;
         }
         const uint64_t m = __ballot(fired);
-        __shared__ uint32_t s_c[kWavesPerTile];
         const uint32_t lane = t & 63u, wave = t >> 6;
         if (lane == 0) s_c[wave] = static_cast<uint32_t>(__popcll(m));
         __syncthreads();
@@ -293,10 +494,7 @@ __device__ __forceinline__ void for_tile_edges(const NetArgs& a, uint32_t b, uin
             if (w < wave) base += s_c[w];
             cnt += s_c[w];
         }
-        if (fired) {
-            const uint32_t rank = static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
-            s_rs[base + rank] = s;  // staged source ids
-        }
+        if (fired) s_rs[base + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)))] = s;
         __syncthreads();
         if (t < cnt) {
             const uint32_t sid = s_rs[t];
@@ -305,140 +503,139 @@ __device__ __forceinline__ void for_tile_edges(const NetArgs& a, uint32_t b, uin
         }
         __syncthreads();
     }
-    if (cnt == 0) { out_total = 0; return; }
+    if (cnt == 0) {
+        if (t == 0) a.tile_coll[b] = 0u;
+        return;
+    }
+    if (t == 0) s_ncoll = 0u;
     uint32_t excl;
     const uint32_t total = block_exclusive_scan(dg, s_wave, excl);
     if (t < cnt) { s_rs[t] = rs; s_pref[t] = excl; }
     __syncthreads();
-    out_total = total;
-    const uint32_t* col = is_src ? a.src_post : a.col;
-    const uint8_t* dly = is_src ? a.src_delay : a.delay;
-    const uint32_t base_eid = is_src ? static_cast<uint32_t>(a.n_syn) : 0u;
-    if (SCHED == 2) {
-        // one wavefront per spiking neuron, lanes stride over its row
-        const uint32_t lane = t & 63u, wave = t >> 6;
-        for (uint32_t k = wave; k < cnt; k += kWavesPerTile) {
-            const uint32_t r0 = s_rs[k];
-            const uint32_t len = (k + 1 < cnt ? s_pref[k + 1] : total) - s_pref[k];
-            for (uint32_t j = lane; j < len; j += 64) {
-                const uint32_t e = r0 + j;
-                visit(base_eid + e, col[e], dly[e]);
-            }
-        }
-    } else {
-        // block-flattened: thread f takes edge f of the tile's concatenated rows
-        for (uint32_t f = t; f < total; f += kTile) {
-            uint32_t lo = 0, hi = cnt;  // largest k with s_pref[k] <= f
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s_pref[mid] <= f) lo = mid; else hi = mid;
-            }
-            const uint32_t e = s_rs[lo] + (f - s_pref[lo]);
-            visit(base_eid + e, col[e], dly[e]);
-        }
+    const ScatterCtx c{step, static_cast<uint32_t>(step), a.tile_ebase[b], &s_ncoll};
+    scatter_rows<SCHED>(a, c, is_src, cnt, total, s_rs, s_pref);
+    __syncthreads();
+    if (t == 0) {
+        a.tile_coll[b] = s_ncoll;
+        if (total) atomicAdd(reinterpret_cast<unsigned long long*>(a.tile_events + b), total);
     }
-}
-
-template <int SCHED>
-__global__ __launch_bounds__(kTile) void insert_kernel(const NetArgs* __restrict__ nets,
-                                                      const uint2* __restrict__ blocks) {
-    __shared__ uint32_t s_rs[kTile], s_pref[kTile], s_wave[kWavesPerTile];
-    const uint2 ref = blocks[blockIdx.x];
-    const NetArgs& a = nets[ref.x];
-    const uint32_t b = ref.y;
-    const uint64_t step = *a.step;
-    if (step >= a.step_limit) return;
-    const bool is_src = b >= a.n_tiles;
-    uint32_t total;
-    uint32_t* head = a.head;
-    uint32_t* next = a.next;
-    const uint32_t N = a.N;
-    for_tile_edges<SCHED>(a, b, step, is_src, s_rs, s_pref, s_wave, total,
-                          [&](uint32_t eid, uint32_t post, uint8_t d) {
-                              const uint32_t dd = d < 1 ? 1u : static_cast<uint32_t>(d);
-                              const uint32_t ts = static_cast<uint32_t>((step + dd) & (kSlots - 1));
-                              const size_t key = static_cast<size_t>(ts) * N + post;
-                              next[eid] = atomicExch(&head[key], eid);
-                          });
-    if (threadIdx.x == 0 && total) a.tile_events[b] += total;
 }
 
 __device__ __forceinline__ float event_weight(const NetArgs& a, uint32_t eid) {
     return eid < a.n_syn ? a.w[eid] : a.src_w[eid - a.n_syn];
 }
 
-template <int SCHED>
-__global__ __launch_bounds__(kTile) void fold_kernel(const NetArgs* __restrict__ nets,
-                                                    const uint2* __restrict__ blocks) {
-    __shared__ uint32_t s_rs[kTile], s_pref[kTile], s_wave[kWavesPerTile];
-    constexpr int kChain = 8;
+// Re-sum one key that received more than one event this step.  `c` is one
+// of the key's late events; only the key's last event (the current head)
+// does the work.  The chain runs last -> ... -> first event; the first
+// event's record holds the ring value before this step's adds.
+__device__ __forceinline__ void fix_key(const NetArgs& a, uint32_t c, uint32_t key, uint32_t* my) {
+    if (static_cast<uint32_t>(a.head[key]) != c) return;
+    // Walk the chain, keeping its kChain smallest ids sorted in LDS.
+    int len = 0;
+    bool overflow = false;
+    uint2 r;
+    for (uint32_t e = c;;) {
+        r = a.rec[e];
+        if (len < kChain) {
+            int j = len++;
+            while (j > 0 && my[j - 1] > e) { my[j] = my[j - 1]; --j; }
+            my[j] = e;
+        } else {
+            overflow = true;
+        }
+        if (r.x == kEmpty) break;
+        e = r.x;
+    }
+    float acc = __uint_as_float(r.y);
+    if (!overflow) {
+        for (int k = 0; k < len; ++k) acc = acc + event_weight(a, my[k]);
+    } else {
+        // Long chain: passes over the chain, each taking the kChain smallest
+        // ids above the last one added.
+        int64_t last = -1;
+        for (;;) {
+            int n = 0;
+            for (uint32_t e = c;;) {
+                if (static_cast<int64_t>(e) > last) {
+                    if (n < kChain) {
+                        int j = n++;
+                        while (j > 0 && my[j - 1] > e) { my[j] = my[j - 1]; --j; }
+                        my[j] = e;
+                    } else if (e < my[kChain - 1]) {
+                        int j = kChain - 1;
+                        while (j > 0 && my[j - 1] > e) { my[j] = my[j - 1]; --j; }
+                        my[j] = e;
+                    }
+                }
+                const uint32_t nx = a.rec[e].x;
+                if (nx == kEmpty) break;
+                e = nx;
+            }
+            if (n == 0) break;
+            for (int k = 0; k < n; ++k) acc = acc + event_weight(a, my[k]);
+            last = my[n - 1];
+        }
+    }
+    a.ring[key] = acc;
+}
+
+// Fix-up of the step's collisions: block j of a network covers scatter tiles
+// [j*kFixTiles, (j+1)*kFixTiles).  The network's last block to finish
+// advances its step counter.  `refs` null: single network, block j = blockIdx.x.
+__global__ __launch_bounds__(kTile) void fixup_kernel(const NetArgs* __restrict__ nets,
+                                                     const uint2* __restrict__ refs) {
+    __shared__ uint32_t s_base[kFixTiles], s_pref[kFixTiles], s_wave[kWavesPerTile];
     __shared__ uint32_t s_chain[kTile * kChain];
-    const uint2 ref = blocks[blockIdx.x];
+    const uint2 ref = refs ? refs[blockIdx.x] : make_uint2(0u, blockIdx.x);
     const NetArgs& a = nets[ref.x];
-    const uint32_t b = ref.y;
     const uint64_t step = *a.step;
     if (step >= a.step_limit) return;  // every block of the network sees the same value
-    const bool is_src = b >= a.n_tiles;
-    uint32_t total;
-    uint32_t* head = a.head;
-    const uint32_t* next = a.next;
-    float* ring = a.ring;
-    const uint32_t N = a.N;
-    uint32_t* my = s_chain + threadIdx.x * kChain;
-    for_tile_edges<SCHED>(a, b, step, is_src, s_rs, s_pref, s_wave, total,
-                          [&](uint32_t eid, uint32_t post, uint8_t d) {
-        if (next[eid] != kEmpty) return;  // not the first event of its key
-        const uint32_t dd = d < 1 ? 1u : static_cast<uint32_t>(d);
-        const uint32_t ts = static_cast<uint32_t>((step + dd) & (kSlots - 1));
-        const size_t key = static_cast<size_t>(ts) * N + post;
-        const uint32_t h = head[key];
-        head[key] = kEmpty;
-        float r = ring[key];
-        if (h == eid) {  // single contribution
-            ring[key] = r + event_weight(a, eid);
-            return;
-        }
-        // Collect the chain (insertion order is arbitrary), order by edge id.
-        int len = 0;
-        bool overflow = false;
-        for (uint32_t c = h; c != kEmpty; c = next[c]) {
-            if (len == kChain) { overflow = true; break; }
-            uint32_t x = c;
-            int j = len++;
-            while (j > 0 && my[j - 1] > x) { my[j] = my[j - 1]; --j; }
-            my[j] = x;
-        }
-        if (!overflow) {
-            for (int k = 0; k < len; ++k) r = r + event_weight(a, my[k]);
-        } else {
-            // Long chain: repeatedly take the smallest edge id above the last.
-            uint32_t last = 0;
-            bool first = true;
-            for (;;) {
-                uint32_t best = kEmpty;
-                for (uint32_t c = h; c != kEmpty; c = next[c])
-                    if ((first || c > last) && c < best) best = c;
-                if (best == kEmpty) break;
-                r = r + event_weight(a, best);
-                last = best;
-                first = false;
-            }
-        }
-        ring[key] = r;
-    });
-    // The last of the network's blocks to finish advances its step counter.
-    // Every block consumed its load of *a.step before reaching this point, so
-    // a relaxed counter suffices; the ring and chain writes reach the next
-    // kernel through the kernel boundary.  (No device-scope fence here: on
-    // gfx950 a release fence writes back the XCD's L2, once per block.)
+    const uint32_t t = threadIdx.x;
+    const uint32_t t0 = ref.y * kFixTiles;
+    const uint32_t nt = min(kFixTiles, a.n_blocks - t0);
+    uint32_t cnt = 0, base = 0;
+    if (t < nt) {
+        cnt = a.tile_coll[t0 + t];
+        base = a.tile_ebase[t0 + t];
+    }
+    uint32_t excl;
+    const uint32_t total = block_exclusive_scan(cnt, s_wave, excl);
+    if (t < nt) { s_base[t] = base; s_pref[t] = excl; }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    uint32_t* my = s_chain + t * kChain;
+    for (uint32_t f = t; f < total; f += kTile) {
+        uint32_t lo = 0, hi = nt;  // largest k with s_pref[k] <= f
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_pref[mid] <= f) lo = mid; else hi = mid;
+        }
+        const uint2 ce = a.coll[s_base[lo] + (f - s_pref[lo])];
+        fix_key(a, ce.x, ce.y, my);
+    }
+    // Every block consumed its load of *a.step before this point, so a
+    // relaxed counter suffices; the ring writes reach the next kernel through
+    // the kernel boundary.  (No device-scope fence: on gfx950 a release fence
+    // writes back the XCD's L2, once per block.)
+    __syncthreads();
+    if (t == 0) {
         const uint32_t prev = atomicAdd(a.done, 1u);
-        if (prev == a.n_blocks - 1) {
+        if (prev == a.n_fix - 1) {
             *a.done = 0u;
             *a.step = step + 1;
         }
     }
+}
+
+// Heads tagged with a step before the current one are idle.  Rewriting them
+// all with the tag of step-1 keeps that true for the next 2^32-1 steps; the
+// runtime does it at reset, after restoring a snapshot and every 2^31 steps.
+__global__ void head_clear_kernel(unsigned long long* head, const uint64_t* step, size_t n) {
+    const unsigned long long v =
+        (static_cast<unsigned long long>(static_cast<uint32_t>(*step - 1u)) << 32) | kEmpty;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        head[i] = v;
 }
 
 // Pair-based STDP, LTP pass: in-edges of every spiking post gain
@@ -594,11 +791,17 @@ __global__ void raster_stats_kernel(const uint64_t* raster, uint32_t n_words, ui
 // ---------------------------------------------------------------------------
 // Host-side launchers (called from net_api.hip)
 // ---------------------------------------------------------------------------
-void launch_neuron(int model, const NetArgs* nets, const uint2* tiles, uint32_t n, hipStream_t s) {
+void launch_neuron(int model, int sched, const NetArgs* nets, const uint4* items, uint32_t n, hipStream_t s) {
     if (n == 0) return;
-    if (model == 0) neuron_kernel<0><<<n, kTile, 0, s>>>(nets, tiles);
-    else if (model == 1) neuron_kernel<1><<<n, kTile, 0, s>>>(nets, tiles);
-    else neuron_kernel<2><<<n, kTile, 0, s>>>(nets, tiles);
+    if (sched == 2) {
+        if (model == 0) neuron_kernel<0, 2><<<n, kTile, 0, s>>>(nets, items);
+        else if (model == 1) neuron_kernel<1, 2><<<n, kTile, 0, s>>>(nets, items);
+        else neuron_kernel<2, 2><<<n, kTile, 0, s>>>(nets, items);
+    } else {
+        if (model == 0) neuron_kernel<0, 1><<<n, kTile, 0, s>>>(nets, items);
+        else if (model == 1) neuron_kernel<1, 1><<<n, kTile, 0, s>>>(nets, items);
+        else neuron_kernel<2, 1><<<n, kTile, 0, s>>>(nets, items);
+    }
 }
 void launch_stdp(const NetArgs* nets, const uint2* tiles, uint32_t n, hipStream_t s) {
     if (n == 0) return;
@@ -610,10 +813,15 @@ void launch_insert(int sched, const NetArgs* nets, const uint2* blocks, uint32_t
     if (sched == 2) insert_kernel<2><<<n, kTile, 0, s>>>(nets, blocks);
     else insert_kernel<1><<<n, kTile, 0, s>>>(nets, blocks);
 }
-void launch_fold(int sched, const NetArgs* nets, const uint2* blocks, uint32_t n, hipStream_t s) {
+void launch_fixup(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s) {
     if (n == 0) return;
-    if (sched == 2) fold_kernel<2><<<n, kTile, 0, s>>>(nets, blocks);
-    else fold_kernel<1><<<n, kTile, 0, s>>>(nets, blocks);
+    fixup_kernel<<<n, kTile, 0, s>>>(nets, refs);
+}
+void launch_head_clear(unsigned long long* head, const uint64_t* step, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    head_clear_kernel<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(head, step, n);
 }
 void launch_inject(float* ring, const uint64_t* step, uint32_t N, const uint32_t* idx, const float* val,
                    uint32_t count, int serial, hipStream_t s) {

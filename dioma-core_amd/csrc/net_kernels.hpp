@@ -11,6 +11,10 @@ constexpr uint32_t kWavesPerTile = kTile / 64;
 constexpr uint32_t kSlots = 32;          // P/src/api/net/graph.rs:33
 constexpr uint32_t kEmpty = 0xFFFFFFFFu; // end of a scatter chain
 constexpr int kMaxParams = 16;
+constexpr uint32_t kFixTiles = 8;        // scatter tiles per fix-up block
+constexpr uint32_t kMaxGroup = 4;        // neuron tiles per block of the neuron kernel
+constexpr uint32_t kMaxNeurons = 1u << 27;  // per network: slot*N + post fits 32 bits
+constexpr int kChain = 8;                // chain entries sorted in LDS per thread
 
 enum : uint32_t {
     kPopRefrElided = 1u,  // refractory timer provably stays 0: not loaded/stored
@@ -53,12 +57,21 @@ struct NetArgs {
     uint32_t n_tiles;           // neuron tiles
     uint32_t n_src_tiles;       // Poisson-source tiles (after the neuron tiles)
     uint32_t n_src;
-    uint32_t n_blocks;          // n_tiles + n_src_tiles (scatter blocks of this network)
+    uint32_t n_blocks;          // n_tiles + n_src_tiles (scatter tiles of this network)
+    uint32_t n_fix;             // fix-up blocks: ceil(n_blocks / kFixTiles)
     uint64_t n_syn;
     uint64_t step_limit;        // kernels skip the network once *step reaches it
     float* ring;                // [32][N]
-    uint32_t* head;             // [32][N] chain heads, kEmpty when idle
-    uint32_t* next;             // [n_syn + n_src_edges] chain links
+    // Scatter bookkeeping.  head[key] = (step tag << 32) | event id of the
+    // last event that reached key = slot*N + post in the step with that tag;
+    // a head with another tag is idle, so heads are never cleared per step.
+    unsigned long long* head;   // [32][N]
+    uint2* rec;                 // [events] per event: {earlier event of its key or kEmpty,
+                                //   ring value before the first add (first event only)}
+    uint2* coll;                // [events] {event, key} of events that found their key
+                                //   taken, per block from tile_ebase[first tile]
+    uint32_t* tile_coll;        // [n_blocks] how many, this step
+    const uint32_t* tile_ebase; // [n_blocks] first event id of the tile's rows
     const uint32_t* row_ptr;    // [N+1]
     const uint32_t* col;        // [n_syn]
     const float* w;             // [n_syn]
@@ -75,7 +88,7 @@ struct NetArgs {
     uint64_t* tile_spikes;      // [n_tiles] running totals
     uint64_t* tile_events;      // [n_tiles + n_src_tiles] running totals
     uint64_t* step;             // device step counter
-    uint32_t* done;             // block completion counter of the fold kernel
+    uint32_t* done;             // block completion counter of the fix-up kernel
     uint64_t* raster;           // optional [raster_cap][n_words]
     uint32_t raster_cap;
     uint32_t n_words;

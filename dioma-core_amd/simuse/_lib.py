@@ -82,6 +82,8 @@ _SIGS = {
     "simuse_net_set_scatter_mode": ([P, C.c_int32], C.c_int),
     "simuse_net_get_scatter_mode": ([P, i32p], C.c_int),
     "simuse_net_autotune": ([P, C.c_uint32], C.c_int),
+    "simuse_net_set_block_tiles": ([P, C.c_uint32], C.c_int),
+    "simuse_net_get_block_tiles": ([P, u32p], C.c_int),
     "simuse_net_set_graph_steps": ([P, C.c_uint32], C.c_int),
     "simuse_net_step": ([P], C.c_int),
     "simuse_net_step_async": ([P, C.c_uint32], C.c_int),

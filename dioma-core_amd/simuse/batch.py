@@ -40,6 +40,7 @@ _SIGS = {
     "simuse_batch_set_graph_steps": ([P, C.c_uint32], C.c_int),
     "simuse_batch_set_scatter_mode": ([P, C.c_int32], C.c_int),
     "simuse_batch_autotune": ([P, C.c_uint32, i32p], C.c_int),
+    "simuse_batch_set_block_tiles": ([P, C.c_uint32], C.c_int),
     "simuse_batch_step_async": ([P, C.c_uint32], C.c_int),
     "simuse_batch_sync": ([P], C.c_int),
     "simuse_batch_step_counter": ([P, C.c_int32, u64p], C.c_int),
@@ -157,6 +158,9 @@ class Batch:
 
     def set_scatter_mode(self, mode):
         check(_L().simuse_batch_set_scatter_mode(self._h, int(mode)))
+
+    def set_block_tiles(self, tiles):
+        check(_L().simuse_batch_set_block_tiles(self._h, int(tiles)))
 
     def autotune(self, trial_steps=16) -> int:
         m = C.c_int32()

@@ -149,6 +149,14 @@ class CompiledNetwork:
         check(lib().simuse_net_get_scatter_mode(self._h, C.byref(m)))
         return int(m.value)
 
+    def set_block_tiles(self, tiles: int) -> None:
+        check(lib().simuse_net_set_block_tiles(self._h, int(tiles)))
+
+    def block_tiles(self) -> int:
+        g = C.c_uint32()
+        check(lib().simuse_net_get_block_tiles(self._h, C.byref(g)))
+        return int(g.value)
+
     def autotune(self, trial_steps: int = 16) -> int:
         check(lib().simuse_net_autotune(self._h, int(trial_steps)))
         return self.scatter_mode()
@@ -280,7 +288,7 @@ class CompiledNetwork:
         check(lib().simuse_net_set_profiling(self._h, 1 if on else 0))
 
     def kernel_times(self):
-        """Mean ms per step of [neuron, stdp, insert, fold] over profiled steps."""
+        """Mean ms per step of [neuron (+ fused scatter), stdp, insert, fix-up] over profiled steps."""
         out = (C.c_float * 4)()
         check(lib().simuse_net_kernel_times(self._h, out, 4))
         return [float(x) for x in out]

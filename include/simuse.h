@@ -115,8 +115,12 @@ int simuse_net_set_poisson_drive(simuse_net* net, int32_t pop, float rate_hz, fl
 int simuse_net_set_plasticity(simuse_net* net, const simuse_stdp_params* p, const uint8_t* plastic);
 int simuse_net_set_scatter_mode(simuse_net* net, int32_t mode);
 int simuse_net_get_scatter_mode(simuse_net* net, int32_t* mode);
-/* Times each scatter schedule on a snapshot of the live state for
- * `trial_steps` steps, restores the snapshot, keeps the faster schedule. */
+/* Neuron tiles (256 neurons each) one block of the step kernel advances,
+ * 1..4.  No effect on results. */
+int simuse_net_set_block_tiles(simuse_net* net, uint32_t tiles);
+int simuse_net_get_block_tiles(simuse_net* net, uint32_t* tiles);
+/* Times each scatter schedule and block size on a snapshot of the live
+ * state for `trial_steps` steps, restores the snapshot, keeps the fastest. */
 int simuse_net_autotune(simuse_net* net, uint32_t trial_steps);
 /* Steps captured per hipGraph replay (1..64). */
 int simuse_net_set_graph_steps(simuse_net* net, uint32_t steps);
@@ -175,7 +179,8 @@ int simuse_net_raster_stats(simuse_net* net, uint32_t first, uint32_t count, flo
 int simuse_net_device_bytes(simuse_net* net, uint64_t* bytes);
 /* Mean duration (ms) of each step kernel over the last simuse_net_step_async
  * window measured with HIP events on the handle's stream when profiling is
- * on: [neuron, insert, fold, stdp]. */
+ * on: [neuron (with the scatter of networks without plasticity), stdp,
+ * insert (plastic networks' scatter and Poisson sources), collision fix-up]. */
 int simuse_net_set_profiling(simuse_net* net, int32_t on);
 int simuse_net_kernel_times(simuse_net* net, float* ms, int32_t n);
 
@@ -231,6 +236,7 @@ int simuse_batch_set_poisson_drive(simuse_batch* b, int32_t net_id, int32_t pop,
 int simuse_batch_set_horizon(simuse_batch* b, int32_t net_id, uint64_t steps);
 int simuse_batch_set_graph_steps(simuse_batch* b, uint32_t steps);
 int simuse_batch_set_scatter_mode(simuse_batch* b, int32_t mode);
+int simuse_batch_set_block_tiles(simuse_batch* b, uint32_t tiles);
 int simuse_batch_autotune(simuse_batch* b, uint32_t trial_steps, int32_t* mode);
 int simuse_batch_step_async(simuse_batch* b, uint32_t n_steps);
 int simuse_batch_sync(simuse_batch* b);

@@ -41,9 +41,11 @@ def assert_same_ring(dev, ora):
     assert bad.shape[0] == 0, f"ring differs at {bad[:5].tolist()}"
 
 
-def run_pair(spec, steps, sched=SCATTER_TILE, inject=None, check_every=1, graph_steps=8, async_steps=False):
+def run_pair(spec, steps, sched=SCATTER_TILE, inject=None, check_every=1, graph_steps=8, async_steps=False,
+             tiles=4):
     dev = build_device(spec, graph_steps=graph_steps)
     dev.set_scatter_mode(sched)
+    dev.set_block_tiles(tiles)
     ora = po.OracleNetwork(spec, 32)
     if async_steps and inject is None:
         dev.record(steps)
@@ -152,12 +154,13 @@ def test_network_delays_exact_arrival(n):
         assert np.array_equal(raster[:, i], want), (i, d)
 
 
-@pytest.mark.parametrize("sched", [SCATTER_TILE, SCATTER_WAVE])
-def test_random_izhikevich_network_bitexact(sched):
+@pytest.mark.parametrize("sched,tiles", [(SCATTER_TILE, 4), (SCATTER_WAVE, 4), (SCATTER_TILE, 1),
+                                         (SCATTER_WAVE, 2)])
+def test_random_izhikevich_network_bitexact(sched, tiles):
     # 10k neurons, ~100 synapses each, multi-edges, noise; 1000 steps
-    say("random_izhikevich", sched)
+    say("random_izhikevich", sched, tiles)
     spec = random_net(10_000, 100, seed=5, model="izh", max_delay=0)
-    dev, ora = run_pair(spec, 1000, sched=sched, check_every=1)
+    dev, ora = run_pair(spec, 1000, sched=sched, check_every=1, tiles=tiles)
     s, e = ora.counters()
     assert s > 1000 and e > 0
     assert dev.counters() == (s, e)
@@ -186,7 +189,7 @@ def test_graph_replay_matches_stepwise():
 
 
 def test_brunel_lif_small_bitexact():
-    # configs[0] shape at reduced size: LIF, p=0.01, delays 1..15, Poisson 2 kHz per neuron
+    # the Brunel LIF benchmark network at reduced size: delays 1..15, Poisson drive per neuron
     from simuse.workloads import brunel_lif_network
     say("brunel_small")
     spec = brunel_lif_network(n=2000, p=0.05, seed=2, steps=1500, rate_hz=20000.0)
@@ -266,6 +269,38 @@ def test_edge_cases():
     spec = NetSpec("mixed", pops, rp, col, w, d, 0.5, 0)
     spec.noise += [(0, np.full(65, 20.0, np.float32), 1), (2, np.full(130, 30.0, np.float32), 2)]
     dev, ora = run_pair(spec, 300)
+    assert_same_state(dev, ora, spec)
+
+
+@pytest.mark.parametrize("sched", [SCATTER_TILE, SCATTER_WAVE])
+def test_collisions_across_tiles_and_sources(sched):
+    """Many events per ring cell in one step: half of all edges aim at 40 hot
+    posts (chains of ~100 per cell, beyond the 8 sorted in LDS), spread over
+    79 neuron tiles (several fix-up blocks) and mixed with Poisson sources
+    that fire every step into the same cells."""
+    say("collisions", sched)
+    from simuse.workloads import Source
+    rng = np.random.default_rng(5)
+    n, deg, hot = 20000, 30, 40
+    pre = np.repeat(np.arange(n, dtype=np.uint32), deg)
+    post = np.where(rng.random(pre.size) < 0.5, rng.integers(0, hot, pre.size),
+                    rng.integers(0, n, pre.size)).astype(np.uint32)
+    w = rng.normal(0.0, 1.0, pre.size).astype(np.float32)
+    d = rng.integers(1, 4, pre.size).astype(np.uint8)
+    rp, col, w, d = csr_from_triplets(n, pre, post, w, d)
+    spec = NetSpec("hot", [lif_pop(n, 0.5, 0.5, -70.0, -54.0, -70.0, 0.0, -70.0)], rp, col, w, d, 0.5, 0)
+    for s in range(300):
+        k = int(rng.integers(1, 6))
+        spec.sources.append(Source(1e7, 0.5, 1000 + s, rng.integers(0, hot, k).astype(np.uint32),
+                                   rng.normal(0.0, 1.0, k).astype(np.float32),
+                                   rng.integers(0, 4, k).astype(np.uint8)))
+    kick = [rng.choice(n, 1000, replace=False).astype(np.uint32) for _ in range(25)]
+    big = np.full(1000, 500.0, np.float32)
+    dev, ora = run_pair(spec, 25, sched=sched,
+                        inject=lambda dv, o, t: (dv.inject_current(0, kick[t], big), o.inject_current(0, kick[t], big)))
+    assert ora.counters()[1] > 25 * 1000 * deg // 2
+    assert dev.counters() == ora.counters()
+    assert_same_ring(dev, ora)
     assert_same_state(dev, ora, spec)
 
 
