@@ -122,7 +122,9 @@ def main() -> int:
     net = build_device(spec, graph_steps=args.graph_steps)
     log(f"  device bytes {net.device_bytes() / 2**30:.2f} GiB")
     if args.scatter == "auto":
-        mode = net.autotune(16)
+        # tune on the network's steady state, not on the initial transient
+        net.run(max(1, args.warmup // 2))
+        mode = net.autotune(32)
     else:
         net.set_scatter_mode(simuse.SCATTER_TILE if args.scatter == "tile" else simuse.SCATTER_WAVE)
         mode = net.scatter_mode()

@@ -183,7 +183,6 @@ int build_generated(NetState& s, const simuse_gen_desc& g, hipStream_t stream) {
     if ((rc = s.alloc(&s.bits, s.n_words))) return rc;
     if ((rc = s.alloc(&s.tile_spikes, s.n_tiles))) return rc;
     if ((rc = s.alloc(&s.step, 2))) return rc;
-    if ((rc = s.alloc(&s.done, 1))) return rc;
     return s.reset_dynamic(stream);
 }
 

@@ -75,23 +75,37 @@ __device__ __forceinline__ float det_expf(float x) {
     return p * __uint_as_float(static_cast<uint32_t>(k + 127) << 23);
 }
 
-// Standard normal draw keyed by (seed, neuron, step): Marsaglia's polar
-// method on two 24-bit uniforms per attempt.  Counter-based, so any neuron
-// of any step can be drawn independently and repeatably.
+// cos(2*pi*u) for u in [0, 1) on a 2^-24 grid: quadrant from 4u (exact),
+// then sin or cos of (pi/2)*g, g in [0, 1/2], by their Taylor series through
+// x^9 / x^10 (truncation < 2e-9 on [0, pi/4]), Horner in FP32.
+__device__ __forceinline__ float det_cos2pi(float u) {
+    const float t = u * 4.0f;
+    const float q = floorf(t);
+    const float f = t - q;
+    const bool hi = f > 0.5f;
+    const float g = hi ? 1.0f - f : f;
+    const float x = g * 1.57079637f;
+    const float x2 = x * x;
+    const float sn = x * (1.0f + x2 * (-0.166666672f + x2 * (0.00833333377f + x2 * (-0.000198412701f +
+                                                                                     x2 * 2.75573188e-06f))));
+    const float cs = 1.0f + x2 * (-0.5f + x2 * (0.0416666679f + x2 * (-0.00138888892f +
+                                                                     x2 * (2.48015876e-05f + x2 * -2.75573188e-07f))));
+    const float c = hi ? sn : cs;  // cos((pi/2) f)
+    const float s = hi ? cs : sn;  // sin((pi/2) f)
+    const int qi = static_cast<int>(q);
+    return qi == 0 ? c : qi == 1 ? -s : qi == 2 ? -c : s;
+}
+
+// Standard normal draw keyed by (seed, neuron, step): Box & Muller (1958),
+// z = sqrt(-2 ln u1) cos(2 pi u2), on two 24-bit uniforms of one SplitMix64
+// hash of the counter (u1 in (0, 1], u2 in [0, 1)).  Counter-based and
+// branch-free, so every lane of a wavefront draws in lockstep.
 __device__ __forceinline__ float normal_draw(uint64_t seed, uint32_t i, uint64_t step) {
     const uint64_t base = splitmix(seed ^ (static_cast<uint64_t>(i) * 0xD1B54A32D192ED03ull));
-    for (uint32_t k = 0; k < 64; ++k) {
-        const uint64_t h = splitmix(base ^ (step * 0xA24BAED4963EE407ull + k));
-        const float u1 = static_cast<float>(h >> 40) * 5.9604644775390625e-08f;
-        const float u2 = static_cast<float>((h >> 16) & 0xFFFFFFu) * 5.9604644775390625e-08f;
-        const float x = u1 * 2.0f - 1.0f;
-        const float y = u2 * 2.0f - 1.0f;
-        const float s = x * x + y * y;
-        if (s >= 1.0f || s == 0.0f) continue;
-        const float q = __builtin_sqrtf((-2.0f * det_logf(s)) / s);
-        return x * q;
-    }
-    return 0.0f;
+    const uint64_t h = splitmix(base ^ (step * 0xA24BAED4963EE407ull));
+    const float u1 = static_cast<float>((h >> 40) + 1u) * 5.9604644775390625e-08f;
+    const float u2 = static_cast<float>((h >> 16) & 0xFFFFFFu) * 5.9604644775390625e-08f;
+    return __builtin_sqrtf(-2.0f * det_logf(u1)) * det_cos2pi(u2);
 }
 
 // x / (e^{x/y} - 1) (NEURON hh.mod vtrap, PC/kernels/hh.cu:28-34).  For

@@ -112,7 +112,6 @@ struct NetState {
     uint64_t* tile_spikes = nullptr;
     uint64_t* tile_events = nullptr;
     uint64_t* step = nullptr;  // [0] step, [1] raster base
-    uint32_t* done = nullptr;
     std::vector<SourceHost> srcs;
     uint32_t n_src = 0, n_src_tiles = 0;
     uint64_t n_src_edges = 0;
@@ -201,7 +200,6 @@ struct NetState {
         a.tile_spikes = tile_spikes;
         a.tile_events = tile_events;
         a.step = step;
-        a.done = done;
         a.raster = raster;
         a.raster_cap = raster_cap;
         a.n_words = n_words;

@@ -128,7 +128,6 @@ int NetState::init(const simuse_pop_desc* pd, int32_t n_pops, const uint32_t* rp
     if ((rc = alloc(&bits, n_words))) return rc;
     if ((rc = alloc(&tile_spikes, n_tiles))) return rc;
     if ((rc = alloc(&step, 2))) return rc;
-    if ((rc = alloc(&done, 1))) return rc;
     if ((rc = reset_dynamic(stream))) return rc;
     HIP_TRY(hipDeviceSynchronize());
     return SIMUSE_OK;
@@ -263,7 +262,7 @@ int NetState::reset_dynamic(hipStream_t stream) {
     HIP_TRY(hipMemsetAsync(ring, 0, sizeof(float) * static_cast<size_t>(kSlots) * N, stream));
     HIP_TRY(hipMemsetAsync(step, 0, sizeof(uint64_t) * 2, stream));
     clear_heads(stream);
-    HIP_TRY(hipMemsetAsync(done, 0, sizeof(uint32_t), stream));
+    if (tile_coll) HIP_TRY(hipMemsetAsync(tile_coll, 0, sizeof(uint32_t) * (n_tiles + n_src_tiles), stream));
     HIP_TRY(hipMemsetAsync(tile_count, 0, sizeof(uint32_t) * n_tiles, stream));
     HIP_TRY(hipMemsetAsync(bits, 0, sizeof(uint64_t) * n_words, stream));
     HIP_TRY(hipMemsetAsync(tile_spikes, 0, sizeof(uint64_t) * n_tiles, stream));
@@ -679,16 +678,18 @@ int Engine::autotune(uint32_t trial_steps) {
     };
     // Candidates: scatter schedule x neuron tiles per block.
     const int cand_s[2] = {SIMUSE_SCATTER_TILE, SIMUSE_SCATTER_WAVE};
-    const uint32_t cand_g[3] = {kMaxGroup, 2, 1};
+    const uint32_t cand_g[3] = {4, 2, 1};
     const int saved = sched;
     const uint32_t saved_g = group;
     float best_ms = 0;
+    bool have = false;
     int best = sched;
     uint32_t best_g = group;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     for (int c = 0; c < 6 && rc == SIMUSE_OK; ++c) {
+        if (cand_g[c / 2] > kMaxGroup) continue;
         sched = cand_s[c % 2];
         if ((rc = set_group(cand_g[c / 2]))) break;
         if ((rc = ensure_tables())) break;
@@ -705,7 +706,8 @@ int Engine::autotune(uint32_t trial_steps) {
             rc = fail(SIMUSE_E_HIP, "autotune timing: %s", hipGetErrorString(e));
             break;
         }
-        if (c == 0 || ms < best_ms) {
+        if (!have || ms < best_ms) {
+            have = true;
             best_ms = ms;
             best = sched;
             best_g = group;

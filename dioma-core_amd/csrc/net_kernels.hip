@@ -581,16 +581,13 @@ __device__ __forceinline__ void fix_key(const NetArgs& a, uint32_t c, uint32_t k
 }
 
 // Fix-up of the step's collisions: block j of a network covers scatter tiles
-// [j*kFixTiles, (j+1)*kFixTiles).  The network's last block to finish
-// advances its step counter.  `refs` null: single network, block j = blockIdx.x.
+// [j*kFixTiles, (j+1)*kFixTiles); block 0 advances the step counter.  `refs` null: single network, block j = blockIdx.x.
 __global__ __launch_bounds__(kTile) void fixup_kernel(const NetArgs* __restrict__ nets,
                                                      const uint2* __restrict__ refs) {
     __shared__ uint32_t s_base[kFixTiles], s_pref[kFixTiles], s_wave[kWavesPerTile];
     __shared__ uint32_t s_chain[kTile * kChain];
     const uint2 ref = refs ? refs[blockIdx.x] : make_uint2(0u, blockIdx.x);
     const NetArgs& a = nets[ref.x];
-    const uint64_t step = *a.step;
-    if (step >= a.step_limit) return;  // every block of the network sees the same value
     const uint32_t t = threadIdx.x;
     const uint32_t t0 = ref.y * kFixTiles;
     const uint32_t nt = min(kFixTiles, a.n_blocks - t0);
@@ -601,7 +598,11 @@ __global__ __launch_bounds__(kTile) void fixup_kernel(const NetArgs* __restrict_
     }
     uint32_t excl;
     const uint32_t total = block_exclusive_scan(cnt, s_wave, excl);
-    if (t < nt) { s_base[t] = base; s_pref[t] = excl; }
+    if (t < nt) {
+        s_base[t] = base;
+        s_pref[t] = excl;
+        if (cnt) a.tile_coll[t0 + t] = 0u;  // consumed: a network past its horizon re-reads zeros
+    }
     __syncthreads();
     uint32_t* my = s_chain + t * kChain;
     for (uint32_t f = t; f < total; f += kTile) {
@@ -613,17 +614,12 @@ __global__ __launch_bounds__(kTile) void fixup_kernel(const NetArgs* __restrict_
         const uint2 ce = a.coll[s_base[lo] + (f - s_pref[lo])];
         fix_key(a, ce.x, ce.y, my);
     }
-    // Every block consumed its load of *a.step before this point, so a
-    // relaxed counter suffices; the ring writes reach the next kernel through
-    // the kernel boundary.  (No device-scope fence: on gfx950 a release fence
-    // writes back the XCD's L2, once per block.)
-    __syncthreads();
-    if (t == 0) {
-        const uint32_t prev = atomicAdd(a.done, 1u);
-        if (prev == a.n_fix - 1) {
-            *a.done = 0u;
-            *a.step = step + 1;
-        }
+    // No other block of this kernel reads the step counter, so block 0 of
+    // the network advances it without waiting for the rest; the next step's
+    // kernels see it, and the ring writes, through the kernel boundary.
+    if (ref.y == 0 && t == 0) {
+        const uint64_t step = *a.step;
+        if (step < a.step_limit) *a.step = step + 1;
     }
 }
 

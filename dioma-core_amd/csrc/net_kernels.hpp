@@ -12,7 +12,7 @@ constexpr uint32_t kSlots = 32;          // P/src/api/net/graph.rs:33
 constexpr uint32_t kEmpty = 0xFFFFFFFFu; // end of a scatter chain
 constexpr int kMaxParams = 16;
 constexpr uint32_t kFixTiles = 8;        // scatter tiles per fix-up block
-constexpr uint32_t kMaxGroup = 4;        // neuron tiles per block of the neuron kernel
+constexpr uint32_t kMaxGroup = 2;        // neuron tiles per block of the neuron kernel
 constexpr uint32_t kMaxNeurons = 1u << 27;  // per network: slot*N + post fits 32 bits
 constexpr int kChain = 8;                // chain entries sorted in LDS per thread
 
@@ -88,7 +88,6 @@ struct NetArgs {
     uint64_t* tile_spikes;      // [n_tiles] running totals
     uint64_t* tile_events;      // [n_tiles + n_src_tiles] running totals
     uint64_t* step;             // device step counter
-    uint32_t* done;             // block completion counter of the fix-up kernel
     uint64_t* raster;           // optional [raster_cap][n_words]
     uint32_t raster_cap;
     uint32_t n_words;

@@ -52,6 +52,7 @@ _SIGS = {
     "orc_net_set_threads": ([P, C.c_int32], None),
     "orc_poisson_spike": ([C.c_uint32, C.c_uint32, C.c_uint64, C.c_float], C.c_int32),
     "orc_gauss": ([C.c_uint64, C.c_uint32, C.c_uint64], C.c_float),
+    "orc_cos2pi_f32": ([C.c_float], C.c_float),
     "orc_exp_f32": ([C.c_float], C.c_float),
     "orc_ln_f32": ([C.c_float], C.c_float),
     "orc_vtrap_f32": ([C.c_float, C.c_float], C.c_float),
@@ -230,6 +231,10 @@ def poisson_spike(seed, stream, step, p):
 
 def gauss(seed, i, step):
     return float(lib().orc_gauss(seed, i, step))
+
+
+def cos2pi_f32(u):
+    return float(lib().orc_cos2pi_f32(u))
 
 
 def exp_f32(x):

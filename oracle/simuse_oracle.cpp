@@ -109,20 +109,36 @@ static inline float exp_f32(float x) {
 
 // Standard normal draw for (seed, neuron, step): Marsaglia polar method on
 // 24-bit uniforms from mix64.  sqrt and / are IEEE correctly rounded.
+// cos(2*pi*u), u in [0,1) on a 2^-24 grid: exact quadrant split of 4u, then
+// the Taylor series of sin/cos((pi/2) g), g in [0, 1/2], through x^9 / x^10,
+// Horner in FP32 (same operation sequence as the device).
+static inline float cos2pi_f32(float u) {
+    const float t = u * 4.0f;
+    const float q = std::floor(t);
+    const float f = t - q;
+    const bool hi = f > 0.5f;
+    const float g = hi ? 1.0f - f : f;
+    const float x = g * 1.57079637f;
+    const float x2 = x * x;
+    const float sn = x * (1.0f + x2 * (-0.166666672f + x2 * (0.00833333377f + x2 * (-0.000198412701f +
+                                                                                     x2 * 2.75573188e-06f))));
+    const float cs = 1.0f + x2 * (-0.5f + x2 * (0.0416666679f + x2 * (-0.00138888892f +
+                                                                     x2 * (2.48015876e-05f + x2 * -2.75573188e-07f))));
+    const float c = hi ? sn : cs;
+    const float s = hi ? cs : sn;
+    const int qi = static_cast<int>(q);
+    return qi == 0 ? c : qi == 1 ? -s : qi == 2 ? -c : s;
+}
+
+// Thalamic Gaussian input sigma * N(0,1) per neuron and step: Box-Muller on
+// one SplitMix64 hash of (seed, neuron, step), z = sqrt(-2 ln u1) cos(2 pi u2)
+// with 24-bit u1 in (0,1], u2 in [0,1).
 static inline float gauss(uint64_t seed, uint32_t i, uint64_t step) {
-    uint64_t base = mix64(seed ^ (static_cast<uint64_t>(i) * 0xD1B54A32D192ED03ull));
-    for (uint32_t k = 0; k < 64; ++k) {
-        uint64_t h = mix64(base ^ (step * 0xA24BAED4963EE407ull + k));
-        float u1 = static_cast<float>(h >> 40) * 5.9604644775390625e-08f;
-        float u2 = static_cast<float>((h >> 16) & 0xFFFFFFu) * 5.9604644775390625e-08f;
-        float x = u1 * 2.0f - 1.0f;
-        float y = u2 * 2.0f - 1.0f;
-        float s = x * x + y * y;
-        if (s >= 1.0f || s == 0.0f) continue;
-        float q = std::sqrt((-2.0f * ln_f32(s)) / s);
-        return x * q;
-    }
-    return 0.0f;
+    const uint64_t base = mix64(seed ^ (static_cast<uint64_t>(i) * 0xD1B54A32D192ED03ull));
+    const uint64_t h = mix64(base ^ (step * 0xA24BAED4963EE407ull));
+    const float u1 = static_cast<float>((h >> 40) + 1u) * 5.9604644775390625e-08f;
+    const float u2 = static_cast<float>((h >> 16) & 0xFFFFFFu) * 5.9604644775390625e-08f;
+    return std::sqrt(-2.0f * ln_f32(u1)) * cos2pi_f32(u2);
 }
 
 // vtrap(x, y) = x / (exp(x/y) - 1) — PC/kernels/hh.cu:28-34 (NEURON hh.mod).
@@ -724,6 +740,7 @@ int32_t orc_poisson_spike(uint32_t seed, uint32_t stream, uint64_t step, float p
     return poisson_spike(seed, stream, step, p) ? 1 : 0;
 }
 float orc_gauss(uint64_t seed, uint32_t i, uint64_t step) { return gauss(seed, i, step); }
+float orc_cos2pi_f32(float u) { return cos2pi_f32(u); }
 float orc_exp_f32(float x) { return exp_f32(x); }
 float orc_ln_f32(float x) { return ln_f32(x); }
 float orc_vtrap_f32(float x, float y) { return vtrap_f32(x, y); }

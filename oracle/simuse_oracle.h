@@ -81,6 +81,7 @@ void orc_net_set_threads(void* net, int32_t n_threads);
 /* Stand-alone pieces for known-answer tests. */
 int32_t orc_poisson_spike(uint32_t seed, uint32_t stream, uint64_t step, float p);
 float orc_gauss(uint64_t seed, uint32_t i, uint64_t step);
+float orc_cos2pi_f32(float u);
 float orc_exp_f32(float x);
 float orc_ln_f32(float x);
 float orc_vtrap_f32(float x, float y);

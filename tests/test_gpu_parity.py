@@ -42,7 +42,7 @@ def assert_same_ring(dev, ora):
 
 
 def run_pair(spec, steps, sched=SCATTER_TILE, inject=None, check_every=1, graph_steps=8, async_steps=False,
-             tiles=4):
+             tiles=2):
     dev = build_device(spec, graph_steps=graph_steps)
     dev.set_scatter_mode(sched)
     dev.set_block_tiles(tiles)
@@ -154,8 +154,8 @@ def test_network_delays_exact_arrival(n):
         assert np.array_equal(raster[:, i], want), (i, d)
 
 
-@pytest.mark.parametrize("sched,tiles", [(SCATTER_TILE, 4), (SCATTER_WAVE, 4), (SCATTER_TILE, 1),
-                                         (SCATTER_WAVE, 2)])
+@pytest.mark.parametrize("sched,tiles", [(SCATTER_TILE, 2), (SCATTER_WAVE, 2), (SCATTER_TILE, 1),
+                                         (SCATTER_WAVE, 1)])
 def test_random_izhikevich_network_bitexact(sched, tiles):
     # 10k neurons, ~100 synapses each, multi-edges, noise; 1000 steps
     say("random_izhikevich", sched, tiles)

@@ -200,6 +200,39 @@ def test_poisson_roll_rate_and_independence():
     assert all(po.poisson_spike(3, 0, t, 1.0) for t in range(100))
 
 
+M64 = (1 << 64) - 1
+
+
+def _splitmix(z):
+    z = (z + 0x9E3779B97F4A7C15) & M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
+def test_cos2pi_accuracy():
+    # every 97th point of the 2^24 grid, plus the quadrant edges
+    k = np.concatenate([np.arange(0, 1 << 24, 97), [0, 1 << 22, 1 << 23, 3 << 22, (1 << 24) - 1]])
+    u = (k.astype(np.float64) / (1 << 24)).astype(np.float32)
+    got = np.array([po.cos2pi_f32(float(x)) for x in u[::7]])
+    want = np.cos(2.0 * np.pi * u[::7].astype(np.float64))
+    assert np.max(np.abs(got - want)) < 3e-7
+    assert po.cos2pi_f32(0.0) == 1.0 and po.cos2pi_f32(0.5) == -1.0
+    assert abs(po.cos2pi_f32(0.25)) < 1e-7 and abs(po.cos2pi_f32(0.75)) < 1e-7
+
+
+def test_gauss_is_box_muller_of_the_counter_hash():
+    # z = sqrt(-2 ln u1) cos(2 pi u2) with u1 = (h>>40 + 1)/2^24, u2 = ((h>>16)&0xFFFFFF)/2^24,
+    # h = SplitMix64(SplitMix64(seed ^ i*C1) ^ step*C2): recomputed in float64 here
+    for seed, i, t in [(42, 0, 0), (42, 7, 3), (1, 123456, 999), (2 ** 40 + 5, 3, 2 ** 33)]:
+        base = _splitmix((seed ^ (i * 0xD1B54A32D192ED03)) & M64)
+        h = _splitmix(base ^ ((t * 0xA24BAED4963EE407) & M64))
+        u1 = ((h >> 40) + 1) / 2.0 ** 24
+        u2 = ((h >> 16) & 0xFFFFFF) / 2.0 ** 24
+        want = math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+        assert abs(po.gauss(seed, i, t) - want) < 2e-6 * max(1.0, abs(want)), (seed, i, t)
+
+
 def test_gauss_moments():
     z = np.array([po.gauss(42, i, t) for i in range(200) for t in range(500)])
     assert abs(z.mean()) < 0.01
