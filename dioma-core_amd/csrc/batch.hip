@@ -168,6 +168,7 @@ int build_generated(NetState& s, const simuse_gen_desc& g, hipStream_t stream) {
     if ((rc = s.alloc(&s.w, s.n_syn))) return rc;
     if ((rc = s.alloc(&s.delay, s.n_syn))) return rc;
     const uint32_t d_min = g.d_min, d_span = static_cast<uint32_t>(g.d_max) - g.d_min + 1u;
+    s.max_delay = g.d_max;
     uint64_t blocks = (s.n_syn + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     if (blocks == 0) blocks = 1;

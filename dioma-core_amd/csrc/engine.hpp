@@ -101,6 +101,7 @@ struct NetState {
     uint32_t* tile_coll = nullptr;
     uint32_t* tile_ebase = nullptr;
     uint32_t n_fix = 1;
+    uint32_t max_delay = 0;    // largest synapse or source delay (steps)
     uint64_t since_clear = 0;  // steps enqueued since the heads were last rewritten
     uint32_t* row_ptr = nullptr;
     uint32_t* col = nullptr;
@@ -179,6 +180,7 @@ struct NetState {
         a.n_syn = n_syn;
         a.step_limit = step_limit;
         a.ring = ring;
+        a.ring_fresh = max_delay <= 1 ? 1 : 0;
         a.head = head;
         a.rec = rec;
         a.coll = coll;
@@ -269,7 +271,7 @@ struct Engine {
     bool direct_fix = false;
     // execution
     int sched = SIMUSE_SCATTER_TILE;
-    uint32_t group = kMaxGroup;  // neuron tiles per block
+    uint32_t group = 1;  // neuron tiles per block (1..kMaxGroup; the autotuner may change it)
     uint32_t graph_steps = 8;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;

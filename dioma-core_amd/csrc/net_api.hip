@@ -121,6 +121,8 @@ int NetState::init(const simuse_pop_desc* pd, int32_t n_pops, const uint32_t* rp
         HIP_TRY(hipMemcpy(delay, dl, nsyn, hipMemcpyHostToDevice));
     }
     h_row_ptr.assign(rp, rp + N + 1);
+    max_delay = 0;
+    for (uint64_t e = 0; e < nsyn; ++e) max_delay = std::max<uint32_t>(max_delay, dl[e]);
     if ((rc = alloc(&ring, static_cast<size_t>(kSlots) * N))) return rc;
     if ((rc = alloc(&head, static_cast<size_t>(kSlots) * N))) return rc;
     if ((rc = alloc(&tile_count, n_tiles))) return rc;
@@ -297,6 +299,7 @@ int NetState::add_source(float rate_hz, float dt_ms, uint32_t seed, const uint32
     s.post.assign(posts, posts + count);
     s.w.assign(wf, wf + count);
     s.d.assign(dl, dl + count);
+    for (uint32_t i = 0; i < count; ++i) max_delay = std::max<uint32_t>(max_delay, dl[i]);
     srcs.push_back(std::move(s));
     return SIMUSE_OK;
 }

@@ -180,8 +180,10 @@ __device__ __forceinline__ void scatter_event(const NetArgs& a, const ScatterCtx
     const uint32_t dd = d < 1 ? 1u : d;
     const size_t key = static_cast<size_t>((c.step + dd) & (kSlots - 1)) * a.N + post;
     // The ring read does not depend on the exchange, so both round trips
-    // overlap; only the key's first event uses the value.
-    const float r0 = a.ring[key];
+    // overlap; only the key's first event uses the value.  When every delay
+    // is <= 1 step, the target slot is untouched since it was consumed and
+    // zeroed, so the read is skipped.
+    const float r0 = a.ring_fresh ? 0.0f : a.ring[key];
     const unsigned long long mine = (static_cast<unsigned long long>(c.tag) << 32) | eid;
     const unsigned long long old = atomicExch(a.head + key, mine);
     if (static_cast<uint32_t>(old >> 32) != c.tag) {
@@ -522,59 +524,63 @@ __global__ __launch_bounds__(kTile) void insert_kernel(const NetArgs* __restrict
 }
 
 __device__ __forceinline__ float event_weight(const NetArgs& a, uint32_t eid) {
-    return eid < a.n_syn ? a.w[eid] : a.src_w[eid - a.n_syn];
+    const float* p = eid < a.n_syn ? a.w + eid : a.src_w + (eid - a.n_syn);
+    return *p;
+}
+
+__device__ __forceinline__ void chain_insert(uint32_t* my, float* myw, int n, uint32_t e, float w) {
+    int j = n;
+    while (j > 0 && my[j - 1] > e) {
+        my[j] = my[j - 1];
+        myw[j] = myw[j - 1];
+        --j;
+    }
+    my[j] = e;
+    myw[j] = w;
 }
 
 // Re-sum one key that received more than one event this step.  `c` is one
 // of the key's late events; only the key's last event (the current head)
 // does the work.  The chain runs last -> ... -> first event; the first
-// event's record holds the ring value before this step's adds.
-__device__ __forceinline__ void fix_key(const NetArgs& a, uint32_t c, uint32_t key, uint32_t* my) {
-    if (static_cast<uint32_t>(a.head[key]) != c) return;
-    // Walk the chain, keeping its kChain smallest ids sorted in LDS.
+// event's record holds the ring value before this step's adds.  Each hop
+// loads an event's record and weight together.
+__device__ __forceinline__ void fix_key(const NetArgs& a, uint32_t c, uint32_t key, uint32_t* my, float* myw) {
+    const uint32_t last = reinterpret_cast<const uint32_t*>(a.head + key)[0];  // event id half
+    uint2 r = a.rec[c];
+    float we = event_weight(a, c);
+    if (last != c) return;
     int len = 0;
     bool overflow = false;
-    uint2 r;
     for (uint32_t e = c;;) {
-        r = a.rec[e];
-        if (len < kChain) {
-            int j = len++;
-            while (j > 0 && my[j - 1] > e) { my[j] = my[j - 1]; --j; }
-            my[j] = e;
-        } else {
-            overflow = true;
-        }
+        if (len < kChain) chain_insert(my, myw, len++, e, we);
+        else overflow = true;
         if (r.x == kEmpty) break;
         e = r.x;
+        r = a.rec[e];
+        we = event_weight(a, e);
     }
     float acc = __uint_as_float(r.y);
     if (!overflow) {
-        for (int k = 0; k < len; ++k) acc = acc + event_weight(a, my[k]);
+        for (int k = 0; k < len; ++k) acc = acc + myw[k];
     } else {
         // Long chain: passes over the chain, each taking the kChain smallest
         // ids above the last one added.
-        int64_t last = -1;
+        int64_t done_to = -1;
         for (;;) {
             int n = 0;
             for (uint32_t e = c;;) {
-                if (static_cast<int64_t>(e) > last) {
-                    if (n < kChain) {
-                        int j = n++;
-                        while (j > 0 && my[j - 1] > e) { my[j] = my[j - 1]; --j; }
-                        my[j] = e;
-                    } else if (e < my[kChain - 1]) {
-                        int j = kChain - 1;
-                        while (j > 0 && my[j - 1] > e) { my[j] = my[j - 1]; --j; }
-                        my[j] = e;
-                    }
+                if (static_cast<int64_t>(e) > done_to && (n < kChain || e < my[kChain - 1])) {
+                    const float w = event_weight(a, e);
+                    if (n < kChain) chain_insert(my, myw, n++, e, w);
+                    else chain_insert(my, myw, kChain - 1, e, w);
                 }
                 const uint32_t nx = a.rec[e].x;
                 if (nx == kEmpty) break;
                 e = nx;
             }
             if (n == 0) break;
-            for (int k = 0; k < n; ++k) acc = acc + event_weight(a, my[k]);
-            last = my[n - 1];
+            for (int k = 0; k < n; ++k) acc = acc + myw[k];
+            done_to = my[n - 1];
         }
     }
     a.ring[key] = acc;
@@ -586,6 +592,7 @@ __global__ __launch_bounds__(kTile) void fixup_kernel(const NetArgs* __restrict_
                                                      const uint2* __restrict__ refs) {
     __shared__ uint32_t s_base[kFixTiles], s_pref[kFixTiles], s_wave[kWavesPerTile];
     __shared__ uint32_t s_chain[kTile * kChain];
+    __shared__ float s_chain_w[kTile * kChain];
     const uint2 ref = refs ? refs[blockIdx.x] : make_uint2(0u, blockIdx.x);
     const NetArgs& a = nets[ref.x];
     const uint32_t t = threadIdx.x;
@@ -605,6 +612,7 @@ __global__ __launch_bounds__(kTile) void fixup_kernel(const NetArgs* __restrict_
     }
     __syncthreads();
     uint32_t* my = s_chain + t * kChain;
+    float* myw = s_chain_w + t * kChain;
     for (uint32_t f = t; f < total; f += kTile) {
         uint32_t lo = 0, hi = nt;  // largest k with s_pref[k] <= f
         while (hi - lo > 1) {
@@ -612,7 +620,7 @@ __global__ __launch_bounds__(kTile) void fixup_kernel(const NetArgs* __restrict_
             if (s_pref[mid] <= f) lo = mid; else hi = mid;
         }
         const uint2 ce = a.coll[s_base[lo] + (f - s_pref[lo])];
-        fix_key(a, ce.x, ce.y, my);
+        fix_key(a, ce.x, ce.y, my, myw);
     }
     // No other block of this kernel reads the step counter, so block 0 of
     // the network advances it without waiting for the rest; the next step's

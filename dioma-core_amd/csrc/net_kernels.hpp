@@ -62,6 +62,7 @@ struct NetArgs {
     uint64_t n_syn;
     uint64_t step_limit;        // kernels skip the network once *step reaches it
     float* ring;                // [32][N]
+    int32_t ring_fresh;         // every delay <= 1: this step's target slot holds zeros
     // Scatter bookkeeping.  head[key] = (step tag << 32) | event id of the
     // last event that reached key = slot*N + post in the step with that tag;
     // a head with another tag is idle, so heads are never cleared per step.

@@ -179,6 +179,26 @@ def test_lif_delays_sources_bitexact(sched):
     assert_same_ring(dev, ora)
 
 
+@pytest.mark.parametrize("sched", [SCATTER_TILE, SCATTER_WAVE])
+def test_unit_delay_network_with_sources_and_injection(sched):
+    # every synapse and source delay <= 1: the scatter skips the ring read
+    say("unit_delay", sched)
+    spec = random_net(3000, 60, seed=13, model="lif", max_delay=1, sources=30)
+    idx = np.arange(0, 3000, 7)
+    cur = np.linspace(-20.0, 40.0, idx.size).astype(np.float32)
+
+    def inj(dv, o, t):
+        if t % 3 == 0:
+            dv.inject_current(0, idx, cur)
+            o.inject_current(0, idx, cur)
+
+    dev, ora = run_pair(spec, 400, sched=sched, inject=inj)
+    assert ora.counters()[0] > 100
+    assert dev.counters() == ora.counters()
+    assert_same_state(dev, ora, spec)
+    assert_same_ring(dev, ora)
+
+
 def test_graph_replay_matches_stepwise():
     say("graph_replay")
     spec = random_net(5000, 60, seed=3, model="izh", max_delay=7, sources=5)
