@@ -340,6 +340,21 @@ int simuse_batch_set_noise(simuse_batch* b, int32_t id, int32_t pop, const float
     return rc;
 }
 
+int simuse_batch_set_plasticity(simuse_batch* b, int32_t id, const simuse_stdp_params* p, const uint8_t* plastic) {
+    BATCH_NET(b, id, s);
+    HIP_TRY(hipStreamSynchronize(b->eng.stream));
+    int rc = s->set_plasticity(p, plastic, b->eng.stream);
+    b->eng.mark_dirty();
+    return rc;
+}
+
+int simuse_batch_set_short_term_plasticity(simuse_batch* b, int32_t id, const simuse_stp_profile* profiles,
+                                           int32_t n_profiles, float dt, const uint8_t* syn_types,
+                                           const uint8_t* src_types, uint64_t n_src_edges) {
+    BATCH_NET(b, id, s);
+    return s->set_stp(profiles, n_profiles, dt, syn_types, src_types, n_src_edges);
+}
+
 int simuse_batch_set_poisson_drive(simuse_batch* b, int32_t id, int32_t pop, float rate_hz, float dt_ms,
                                    float weight, uint32_t seed0) {
     BATCH_NET(b, id, s);

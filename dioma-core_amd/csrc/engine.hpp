@@ -29,7 +29,11 @@ namespace simuse {
 
 int fail(int code, const char* fmt, ...);
 
-void launch_neuron(int model, int sched, const NetArgs* nets, const uint4* items, uint32_t n, hipStream_t s);
+void launch_neuron(int model, int sched, const NetArgs* nets, const uint4* items, uint32_t n, hipStream_t s,
+                   uint32_t defer_scatter);
+void launch_dense_trace(const NetArgs* nets, uint32_t n_tiles, float* trace, float decay, hipStream_t s);
+void launch_dense_weights(const NetArgs* nets, uint32_t N, const float* trace, float lr_ltp, float lr_ltd,
+                          float w_min, float w_max, hipStream_t s);
 void launch_stdp(const NetArgs* nets, const uint2* tiles, uint32_t n, hipStream_t s);
 void launch_insert(int sched, const NetArgs* nets, const uint2* blocks, uint32_t n, hipStream_t s);
 void launch_fixup(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s);
@@ -37,6 +41,8 @@ void launch_head_clear(unsigned long long* head, const uint64_t* step, size_t n,
 void launch_inject(float* ring, const uint64_t* step, uint32_t N, const uint32_t* idx, const float* val,
                    uint32_t count, int serial, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+void launch_tm_init(const uint8_t* type, const float4* prof, float* r, float* u, uint32_t* last, size_t n,
+                    hipStream_t s);
 void launch_widen(const float* w, double* w64, size_t n, hipStream_t s);
 void launch_raster_stats(const uint64_t* raster, uint32_t n_words, uint32_t steps, const uint32_t* word_of,
                          const uint32_t* bit_of, uint32_t count, float dt_s, float* rates, float* cvs,
@@ -135,6 +141,17 @@ struct NetState {
     uint32_t* edge_pre = nullptr;
     uint8_t* plastic = nullptr;
     float stdp_dpre = 0, stdp_dpost = 0;
+    float* dtrace = nullptr;      // shared trace of the dense stdp_step rule
+    // Tsodyks-Markram short-term plasticity (host copies until finalize)
+    std::vector<float4> h_tm_prof;
+    std::vector<uint8_t> h_tm_type;  // synapses then source edges; empty = off
+    uint8_t* tm_type = nullptr;
+    float* tm_r = nullptr;
+    float* tm_u = nullptr;
+    uint32_t* tm_last = nullptr;
+    float4* tm_prof = nullptr;
+    float* ev_w = nullptr;
+    uint2* d_all_blocks = nullptr;  // (0, b) for every scatter tile: dense-step insert list
     bool finalized = false;
     uint64_t host_step = 0;
     uint64_t step_limit = std::numeric_limits<uint64_t>::max();
@@ -220,6 +237,12 @@ struct NetState {
         a.a_minus = sp.a_minus;
         a.w_min = sp.w_min;
         a.w_max = sp.w_max;
+        a.tm_type = tm_type;
+        a.tm_r = tm_r;
+        a.tm_u = tm_u;
+        a.tm_last = tm_last;
+        a.tm_prof = tm_prof;
+        a.ev_w = ev_w;
         return a;
     }
 
@@ -239,6 +262,8 @@ struct NetState {
     int set_noise(int32_t pop, const float* sigma, uint64_t seed);
     int set_drive(int32_t pop, float rate_hz, float dt_ms, float weight, uint32_t seed0);
     int set_plasticity(const simuse_stdp_params* p, const uint8_t* pl, hipStream_t stream);
+    int set_stp(const simuse_stp_profile* profiles, int32_t n_profiles, float dt, const uint8_t* syn_types,
+                const uint8_t* src_types, uint64_t n_src_edges_given);
     int download_spikes(int32_t* out, uint64_t len, hipStream_t stream);
     int download_spike_ids(uint32_t* ids, uint32_t cap, uint32_t* count, hipStream_t stream);
     int download_state(int32_t pop, float* out, uint64_t len, hipStream_t stream);
@@ -380,7 +405,7 @@ struct Engine {
 
     void launch_step(hipEvent_t* e) {
         if (e) (void)hipEventRecord(e[0], stream);
-        for (int m = 0; m < 3; ++m) launch_neuron(m, sched, d_nets, d_tiles[m], n_tiles_m[m], stream);
+        for (int m = 0; m < 3; ++m) launch_neuron(m, sched, d_nets, d_tiles[m], n_tiles_m[m], stream, 0u);
         if (e) (void)hipEventRecord(e[1], stream);
         launch_stdp(d_nets, d_stdp, n_stdp, stream);
         if (e) (void)hipEventRecord(e[2], stream);

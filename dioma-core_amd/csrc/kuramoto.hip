@@ -118,14 +118,15 @@ __global__ void kura_prepare_kernel(const double* theta, float2* sc, uint32_t n)
 
 // Order parameter R = |sum_i e^{i theta_i}| / n, deterministic two-pass sum
 // in FP64 (P/src/kernels/measure/mod.rs:76-106 with unit amplitudes).
-__global__ void kura_order_partial(const double* theta, uint32_t n, double2* part) {
+__global__ void kura_order_partial(const double* theta, const float* amp, uint32_t n, double2* part) {
     __shared__ double ss[256], cc[256];
     double s = 0, c = 0;
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         double sn, cs;
         sincos(theta[i], &sn, &cs);
-        s += sn;
-        c += cs;
+        const double a = amp ? static_cast<double>(amp[i]) : 1.0;
+        s += a * sn;
+        c += a * cs;
     }
     ss[threadIdx.x] = s;
     cc[threadIdx.x] = c;
@@ -240,6 +241,7 @@ struct simuse_kura {
     double* theta = nullptr;
     float2* sc[2] = {nullptr, nullptr};
     double2* part = nullptr;
+    float* amp = nullptr;  // amplitudes of phase_coherence
     int cur = 0;
     uint64_t steps = 0;
     uint32_t graph_steps = 8;
@@ -509,17 +511,48 @@ int simuse_kura_download_phases(simuse_kura* k, double* out, uint32_t n) {
     return SIMUSE_OK;
 }
 
-int simuse_kura_order_parameter(simuse_kura* k, double* r) {
-    if (!k || !r) return fail(SIMUSE_E_INVALID, "null argument");
+static int kura_coherence(simuse_kura* k, const float* amp_dev, double* s_out, double* c_out) {
     const uint32_t blocks = std::min<uint32_t>(1024, (k->n + 255) / 256);
-    kura_order_partial<<<blocks, 256, 0, k->stream>>>(k->theta, k->n, k->part);
+    kura_order_partial<<<blocks, 256, 0, k->stream>>>(k->theta, amp_dev, k->n, k->part);
     KHIP(hipGetLastError());
     std::vector<double2> p(blocks);
     KHIP(hipMemcpyAsync(p.data(), k->part, sizeof(double2) * blocks, hipMemcpyDeviceToHost, k->stream));
     KHIP(hipStreamSynchronize(k->stream));
     double s = 0, c = 0;
     for (const auto& x : p) { s += x.x; c += x.y; }
+    *s_out = s;
+    *c_out = c;
+    return SIMUSE_OK;
+}
+
+int simuse_kura_order_parameter(simuse_kura* k, double* r) {
+    if (!k || !r) return fail(SIMUSE_E_INVALID, "null argument");
+    double s, c;
+    int rc = kura_coherence(k, nullptr, &s, &c);
+    if (rc) return rc;
     *r = std::sqrt(s * s + c * c) / k->n;
+    return SIMUSE_OK;
+}
+
+int simuse_kura_phase_coherence(simuse_kura* k, const float* amplitudes, double* r) {
+    if (!k || !r || !amplitudes) return fail(SIMUSE_E_INVALID, "null argument");
+    if (!k->amp) {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, sizeof(float) * k->n);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(SIMUSE_E_NOMEM, "phase_coherence: %s", hipGetErrorString(e));
+        }
+        k->allocs.push_back(q);
+        k->amp = static_cast<float*>(q);
+    }
+    double sa = 0;
+    for (uint32_t i = 0; i < k->n; ++i) sa += amplitudes[i];
+    KHIP(hipMemcpyAsync(k->amp, amplitudes, sizeof(float) * k->n, hipMemcpyHostToDevice, k->stream));
+    double s, c;
+    int rc = kura_coherence(k, k->amp, &s, &c);
+    if (rc) return rc;
+    *r = std::sqrt(s * s + c * c) / std::max(sa, 1e-12);
     return SIMUSE_OK;
 }
 

@@ -172,6 +172,19 @@ int NetState::finalize(hipStream_t stream) {
         if (n_blk) HIP_TRY(hipMemcpy(tile_ebase, eb.data(), 4 * eb.size(), hipMemcpyHostToDevice));
     }
     n_fix = std::max<uint32_t>(1u, (n_blk + kFixTiles - 1) / kFixTiles);
+    if (!h_tm_type.empty()) {
+        h_tm_type.resize(n_ev, 0);  // source edges added after set_stp are static
+        if ((rc = alloc(&tm_type, n_ev))) return rc;
+        if ((rc = alloc(&tm_r, n_ev))) return rc;
+        if ((rc = alloc(&tm_u, n_ev))) return rc;
+        if ((rc = alloc(&tm_last, n_ev))) return rc;
+        if ((rc = alloc(&ev_w, n_ev))) return rc;
+        if ((rc = alloc(&tm_prof, h_tm_prof.size()))) return rc;
+        HIP_TRY(hipMemcpy(tm_type, h_tm_type.data(), n_ev, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(tm_prof, h_tm_prof.data(), sizeof(float4) * h_tm_prof.size(), hipMemcpyHostToDevice));
+        launch_tm_init(tm_type, tm_prof, tm_r, tm_u, tm_last, n_ev, stream);
+        HIP_TRY(hipGetLastError());
+    }
     if ((rc = alloc(&tile_events, static_cast<size_t>(n_tiles) + n_src_tiles))) return rc;
     HIP_TRY(hipMemsetAsync(tile_events, 0, sizeof(uint64_t) * (n_tiles + n_src_tiles), stream));
     if (n_src) {
@@ -301,6 +314,48 @@ int NetState::add_source(float rate_hz, float dt_ms, uint32_t seed, const uint32
     s.d.assign(dl, dl + count);
     for (uint32_t i = 0; i < count; ++i) max_delay = std::max<uint32_t>(max_delay, dl[i]);
     srcs.push_back(std::move(s));
+    return SIMUSE_OK;
+}
+
+// Network::register_plasticity (P/src/api/net/network/core.rs:597-619) for
+// every profile plus the per-edge profile ids (Edge::with_synapse_plasticity;
+// source edges in add order).  a_rec = e^(-dt/tau_rec) in FP32, as the
+// reference computes it.
+int NetState::set_stp(const simuse_stp_profile* profiles, int32_t n_profiles, float dt, const uint8_t* syn_types,
+                      const uint8_t* src_types, uint64_t n_src_edges_given) {
+    if (finalized) return fail(SIMUSE_E_STATE, "short-term plasticity must be set before the first step");
+    if (n_profiles < 1 || n_profiles > 255 || !profiles)
+        return fail(SIMUSE_E_INVALID, "short-term plasticity: 1..255 profiles required");
+    if (!(std::isfinite(dt) && dt > 0.0f)) return fail(SIMUSE_E_INVALID, "register_plasticity: dt must be > 0");
+    if (!syn_types && n_syn) return fail(SIMUSE_E_INVALID, "short-term plasticity: null syn_types");
+    uint64_t n_src = 0;
+    for (const auto& s : srcs) n_src += s.post.size();
+    if (src_types && n_src_edges_given != n_src)
+        return fail(SIMUSE_E_INVALID, "short-term plasticity: %llu source edge types for %llu source edges",
+                    static_cast<unsigned long long>(n_src_edges_given), static_cast<unsigned long long>(n_src));
+    std::vector<float4> prof;
+    for (int32_t k = 0; k < n_profiles; ++k) {
+        const simuse_stp_profile& p = profiles[k];
+        if (!std::isfinite(p.u0) || p.u0 <= 0.0f || p.u0 > 1.0f)
+            return fail(SIMUSE_E_INVALID, "profile %d: init_release_prob must be in (0, 1], got %g", k, p.u0);
+        if (!std::isfinite(p.tau_rec) || p.tau_rec <= 0.0f)
+            return fail(SIMUSE_E_INVALID, "profile %d: tau_rec must be finite and > 0, got %g", k, p.tau_rec);
+        if (!std::isfinite(p.tau_fac))
+            return fail(SIMUSE_E_INVALID, "profile %d: tau_fac must be finite (<= 0: none)", k);
+        const bool fac = p.tau_fac > 0.0f;
+        prof.push_back(make_float4(p.u0, std::exp(-dt / p.tau_rec), fac ? std::exp(-dt / p.tau_fac) : 0.0f,
+                                   fac ? 1.0f : 0.0f));
+    }
+    std::vector<uint8_t> types(n_syn + n_src, 0);
+    for (uint64_t e = 0; e < n_syn; ++e) types[e] = syn_types[e];
+    if (src_types)
+        for (uint64_t e = 0; e < n_src; ++e) types[n_syn + e] = src_types[e];
+    for (uint64_t e = 0; e < types.size(); ++e)
+        if (types[e] > n_profiles)
+            return fail(SIMUSE_E_INVALID, "edge %llu: plasticity profile %u not registered",
+                        static_cast<unsigned long long>(e), types[e]);
+    h_tm_prof = std::move(prof);
+    h_tm_type = std::move(types);
     return SIMUSE_OK;
 }
 
@@ -518,6 +573,11 @@ int NetState::reset(const simuse_pop_desc* pd, int32_t n_pops, hipStream_t strea
             return fail(SIMUSE_E_INVALID, "reset: population %d differs from the network", p);
     HIP_TRY(hipStreamSynchronize(stream));
     if ((rc = upload_pops(pd))) return rc;
+    if (dtrace) HIP_TRY(hipMemsetAsync(dtrace, 0, sizeof(float) * N, stream));
+    if (tm_type) {
+        launch_tm_init(tm_type, tm_prof, tm_r, tm_u, tm_last, n_syn + n_src_edges, stream);
+        HIP_TRY(hipGetLastError());
+    }
     return reset_dynamic(stream);
 }
 
@@ -661,6 +721,12 @@ int Engine::autotune(uint32_t trial_steps) {
         if ((rc = snap.add(s->tile_events, sizeof(uint64_t) * (s->n_tiles + s->n_src_tiles)))) return rc;
         if (s->raster) {
             if ((rc = snap.add(s->raster, sizeof(uint64_t) * s->raster_cap * s->n_words))) return rc;
+        }
+        if (s->tm_type) {
+            const size_t n_ev = s->n_syn + s->n_src_edges;
+            if ((rc = snap.add(s->tm_r, sizeof(float) * n_ev))) return rc;
+            if ((rc = snap.add(s->tm_u, sizeof(float) * n_ev))) return rc;
+            if ((rc = snap.add(s->tm_last, sizeof(uint32_t) * n_ev))) return rc;
         }
         if (s->stdp) {
             if ((rc = snap.add(s->w, sizeof(float) * s->n_syn))) return rc;
@@ -882,6 +948,51 @@ int simuse_net_step(simuse_net* net) {
     int rc = simuse_net_step_async(net, 1);
     if (rc) return rc;
     return simuse_net_sync(net);
+}
+
+int simuse_net_set_short_term_plasticity(simuse_net* net, const simuse_stp_profile* profiles, int32_t n_profiles,
+                                         float dt, const uint8_t* syn_types, const uint8_t* src_types,
+                                         uint64_t n_src_edges) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    return net->st.set_stp(profiles, n_profiles, dt, syn_types, src_types, n_src_edges);
+}
+
+int simuse_net_stdp_step(simuse_net* net, float trace_decay, float lr_ltp, float lr_ltd, float w_min,
+                         float w_max) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    if (!std::isfinite(trace_decay) || !std::isfinite(lr_ltp) || !std::isfinite(lr_ltd) || !std::isfinite(w_min) ||
+        !std::isfinite(w_max) || w_min > w_max)
+        return fail(SIMUSE_E_INVALID, "stdp_step: parameters must be finite with w_min <= w_max");
+    NetState& s = net->st;
+    Engine& e = net->eng;
+    if (s.stdp) return fail(SIMUSE_E_STATE, "stdp_step: pair-based plasticity is enabled on this network");
+    if ((rc = e.prepare())) return rc;
+    const uint32_t n_blk = s.n_tiles + s.n_src_tiles;
+    if (!s.dtrace) {
+        if ((rc = s.alloc(&s.dtrace, s.N))) return rc;
+        HIP_TRY(hipMemsetAsync(s.dtrace, 0, sizeof(float) * s.N, e.stream));
+    }
+    if (!s.d_all_blocks) {
+        std::vector<uint2> bl(n_blk);
+        for (uint32_t b = 0; b < n_blk; ++b) bl[b] = make_uint2(0u, b);
+        if ((rc = s.alloc(&s.d_all_blocks, n_blk))) return rc;
+        if (n_blk) HIP_TRY(hipMemcpy(s.d_all_blocks, bl.data(), sizeof(uint2) * n_blk, hipMemcpyHostToDevice));
+    }
+    if (s.step_limit != std::numeric_limits<uint64_t>::max() && s.host_step >= s.step_limit) return SIMUSE_OK;
+    if (s.since_clear + 1 >= (1ull << 31)) s.clear_heads(e.stream);
+    ++s.since_clear;
+    // trace from the previous step's spikes; update without the inline
+    // scatter; every synapse; then the push with the new weights.
+    launch_dense_trace(e.d_nets, s.n_tiles, s.dtrace, trace_decay, e.stream);
+    for (int m = 0; m < 3; ++m) launch_neuron(m, e.sched, e.d_nets, e.d_tiles[m], e.n_tiles_m[m], e.stream, 1u);
+    launch_dense_weights(e.d_nets, s.N, s.dtrace, lr_ltp, lr_ltd, w_min, w_max, e.stream);
+    launch_insert(e.sched, e.d_nets, s.d_all_blocks, n_blk, e.stream);
+    launch_fixup(e.d_nets, nullptr, s.n_fix, e.stream);
+    HIP_TRY(hipGetLastError());
+    ++s.host_step;
+    return SIMUSE_OK;
 }
 
 int simuse_net_download_spikes(simuse_net* net, int32_t* out, uint64_t len) {

@@ -168,6 +168,45 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t* s
 // read-modify-write; no float atomics anywhere.
 // ---------------------------------------------------------------------------
 
+// x^n, n >= 1, by right-to-left binary exponentiation (the expansion of
+// f32::powi used by the reference's recovery step).
+__device__ __forceinline__ float powi_f32(float a, int n) {
+    float r = 1.0f;
+    for (;;) {
+        if (n & 1) r *= a;
+        n /= 2;
+        if (n == 0) break;
+        a *= a;
+    }
+    return r;
+}
+
+// tm_spike (P/src/api/net/push.rs:60-86) for event e: lazy exponential
+// recovery over the steps since the edge last fired (capped at 1e6), deliver
+// weight * R*U with the pre-spike factor, then deplete R (facilitate U).  An
+// edge fires at most once per step, so its state has a single writer.
+__device__ __forceinline__ float tm_deliver(const NetArgs& a, uint32_t e, float wt, uint64_t step) {
+    const uint32_t t = a.tm_type[e];
+    if (t == 0) return wt;
+    const float4 p = a.tm_prof[t - 1];
+    float r = a.tm_r[e], u = a.tm_u[e];
+    const uint64_t last = a.tm_last[e];
+    uint64_t gap = step > last ? step - last : 0u;
+    if (gap > 1000000u) gap = 1000000u;
+    const int n = static_cast<int>(gap);
+    if (n > 0) {
+        r = 1.0f + (r - 1.0f) * powi_f32(p.y, n);
+        if (p.w != 0.0f) u = p.x + (u - p.x) * powi_f32(p.z, n);
+    }
+    const float scale = r * u;
+    r *= 1.0f - u;
+    if (p.w != 0.0f) u += p.x * (1.0f - u);
+    a.tm_r[e] = r;
+    a.tm_u[e] = u;
+    a.tm_last[e] = static_cast<uint32_t>(step);
+    return wt * scale;
+}
+
 struct ScatterCtx {
     uint64_t step;
     uint32_t tag;       // low 32 bits of step
@@ -184,6 +223,7 @@ __device__ __forceinline__ void scatter_event(const NetArgs& a, const ScatterCtx
     // is <= 1 step, the target slot is untouched since it was consumed and
     // zeroed, so the read is skipped.
     const float r0 = a.ring_fresh ? 0.0f : a.ring[key];
+    if (a.ev_w) a.ev_w[eid] = wt;
     const unsigned long long mine = (static_cast<unsigned long long>(c.tag) << 32) | eid;
     const unsigned long long old = atomicExch(a.head + key, mine);
     if (static_cast<uint32_t>(old >> 32) != c.tag) {
@@ -230,15 +270,19 @@ __device__ __forceinline__ void scatter_rows(const NetArgs& a, const ScatterCtx&
         const uint32_t* col = a.col;
         const uint8_t* dly = a.delay;
         const float* w = a.w;
-        for_rows<SCHED>(cnt, total, s_rs, s_pref,
-                        [&](uint32_t e) { scatter_event(a, c, e, col[e], dly[e], w[e]); });
+        for_rows<SCHED>(cnt, total, s_rs, s_pref, [&](uint32_t e) {
+            const float wt = a.tm_type ? tm_deliver(a, e, w[e], c.step) : w[e];
+            scatter_event(a, c, e, col[e], dly[e], wt);
+        });
     } else {
         const uint32_t base = static_cast<uint32_t>(a.n_syn);
         const uint32_t* col = a.src_post;
         const uint8_t* dly = a.src_delay;
         const float* w = a.src_w;
-        for_rows<SCHED>(cnt, total, s_rs, s_pref,
-                        [&](uint32_t e) { scatter_event(a, c, base + e, col[e], dly[e], w[e]); });
+        for_rows<SCHED>(cnt, total, s_rs, s_pref, [&](uint32_t e) {
+            const float wt = a.tm_type ? tm_deliver(a, base + e, w[e], c.step) : w[e];
+            scatter_event(a, c, base + e, col[e], dly[e], wt);
+        });
     }
 }
 
@@ -307,7 +351,7 @@ struct Neuron<2> {  // Hodgkin-Huxley: reads and writes its columns inside the u
 // population}.
 template <int MODEL, int SCHED>
 __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict__ nets,
-                                                      const uint4* __restrict__ items) {
+                                                      const uint4* __restrict__ items, uint32_t defer_scatter) {
     __shared__ uint32_t s_rs[kMaxGroup * kTile], s_pref[kMaxGroup * kTile];
     __shared__ uint32_t s_cnt[kMaxGroup][kWavesPerTile], s_deg[kMaxGroup][kWavesPerTile];
     __shared__ uint32_t s_drv[kMaxGroup][kWavesPerTile];
@@ -316,7 +360,7 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
     const NetArgs& a = nets[it.x];
     const PopDev& P = a.pops[it.w];
     const uint32_t tile0 = it.y, ntile = it.z;
-    const bool fused = a.stdp == 0;
+    const bool fused = a.stdp == 0 && !defer_scatter;
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     const uint32_t li0 = (tile0 - P.tile0) * kTile + t;
 
@@ -524,7 +568,7 @@ __global__ __launch_bounds__(kTile) void insert_kernel(const NetArgs* __restrict
 }
 
 __device__ __forceinline__ float event_weight(const NetArgs& a, uint32_t eid) {
-    const float* p = eid < a.n_syn ? a.w + eid : a.src_w + (eid - a.n_syn);
+    const float* p = a.ev_w ? a.ev_w + eid : eid < a.n_syn ? a.w + eid : a.src_w + (eid - a.n_syn);
     return *p;
 }
 
@@ -718,6 +762,69 @@ __global__ __launch_bounds__(kTile) void ltd_kernel(const NetArgs* __restrict__ 
     }
 }
 
+// Initial short-term plasticity state: R = 1, U = U0 of the edge's profile
+// (0 for static edges), last = 0 (Network::build, build.rs:70-84).
+__global__ void tm_init_kernel(const uint8_t* type, const float4* prof, float* r, float* u, uint32_t* last,
+                               size_t n) {
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const uint32_t t = type[i];
+        r[i] = 1.0f;
+        u[i] = t ? prof[t - 1].x : 0.0f;
+        last[i] = 0u;
+    }
+}
+
+// Dense every-synapse STDP of CompiledNetwork::stdp_step
+// (PC/src/network.rs:53-83, driven by :495-718), single network.  Trace
+// update from the previous step's spike words: trace = trace*decay + spike.
+__global__ __launch_bounds__(kTile) void dense_trace_kernel(const NetArgs* __restrict__ nets, float* trace,
+                                                          float decay) {
+    const NetArgs& a = nets[0];
+    const uint32_t tile = blockIdx.x;
+    const PopDev& P = a.pops[a.tile_pop[tile]];
+    const uint32_t lt = tile - P.tile0, li = lt * kTile + threadIdx.x;
+    if (li >= P.n) return;
+    const uint64_t word = a.bits[P.word0 + lt * kWavesPerTile + (threadIdx.x >> 6)];
+    const float s = ((word >> (threadIdx.x & 63u)) & 1ull) ? 1.0f : 0.0f;
+    const uint32_t gi = P.off + li;
+    trace[gi] = trace[gi] * decay + s;
+}
+
+// Every synapse of a tile of 256 pre neurons (rows flattened over the
+// block): w = min(w_max, max(w_min, (w + lr_ltp*tr[pre]) - lr_ltd*tr[post])),
+// between the population update and the push.
+__global__ __launch_bounds__(kTile) void dense_stdp_kernel(const NetArgs* __restrict__ nets,
+                                                         const float* __restrict__ trace, float lr_ltp,
+                                                         float lr_ltd, float w_min, float w_max) {
+    __shared__ uint32_t s_rs[kTile], s_pref[kTile], s_wave[kWavesPerTile];
+    const NetArgs& a = nets[0];
+    const uint32_t t = threadIdx.x, pre0 = blockIdx.x * kTile;
+    const uint32_t cnt = min(kTile, a.N - pre0);
+    uint32_t rs = 0, dg = 0;
+    if (t < cnt) {
+        rs = a.row_ptr[pre0 + t];
+        dg = a.row_ptr[pre0 + t + 1] - rs;
+    }
+    uint32_t excl;
+    const uint32_t total = block_exclusive_scan(dg, s_wave, excl);
+    s_rs[t] = rs;
+    s_pref[t] = excl;
+    __syncthreads();
+    float* w = const_cast<float*>(a.w);
+    for (uint32_t f = t; f < total; f += kTile) {
+        uint32_t lo = 0, hi = cnt;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_pref[mid] <= f) lo = mid; else hi = mid;
+        }
+        const uint32_t e = s_rs[lo] + (f - s_pref[lo]);
+        const float nw = (w[e] + lr_ltp * trace[pre0 + lo]) - lr_ltd * trace[a.col[e]];
+        const float cl = w_min < nw ? nw : w_min;  // max(w_min, nw), argument order kept
+        w[e] = cl < w_max ? cl : w_max;            // min(w_max, cl)
+    }
+}
+
 // inject_current into the slot the next step reads (build.rs:243-275).
 __global__ void inject_kernel(float* ring, const uint64_t* step, uint32_t N, const uint32_t* idx,
                               const float* val, uint32_t count, int serial) {
@@ -795,17 +902,27 @@ __global__ void raster_stats_kernel(const uint64_t* raster, uint32_t n_words, ui
 // ---------------------------------------------------------------------------
 // Host-side launchers (called from net_api.hip)
 // ---------------------------------------------------------------------------
-void launch_neuron(int model, int sched, const NetArgs* nets, const uint4* items, uint32_t n, hipStream_t s) {
+void launch_neuron(int model, int sched, const NetArgs* nets, const uint4* items, uint32_t n, hipStream_t s,
+                   uint32_t defer_scatter) {
     if (n == 0) return;
     if (sched == 2) {
-        if (model == 0) neuron_kernel<0, 2><<<n, kTile, 0, s>>>(nets, items);
-        else if (model == 1) neuron_kernel<1, 2><<<n, kTile, 0, s>>>(nets, items);
-        else neuron_kernel<2, 2><<<n, kTile, 0, s>>>(nets, items);
+        if (model == 0) neuron_kernel<0, 2><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
+        else if (model == 1) neuron_kernel<1, 2><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
+        else neuron_kernel<2, 2><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
     } else {
-        if (model == 0) neuron_kernel<0, 1><<<n, kTile, 0, s>>>(nets, items);
-        else if (model == 1) neuron_kernel<1, 1><<<n, kTile, 0, s>>>(nets, items);
-        else neuron_kernel<2, 1><<<n, kTile, 0, s>>>(nets, items);
+        if (model == 0) neuron_kernel<0, 1><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
+        else if (model == 1) neuron_kernel<1, 1><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
+        else neuron_kernel<2, 1><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
     }
+}
+void launch_dense_trace(const NetArgs* nets, uint32_t n_tiles, float* trace, float decay, hipStream_t s) {
+    if (n_tiles == 0) return;
+    dense_trace_kernel<<<n_tiles, kTile, 0, s>>>(nets, trace, decay);
+}
+void launch_dense_weights(const NetArgs* nets, uint32_t N, const float* trace, float lr_ltp, float lr_ltd,
+                          float w_min, float w_max, hipStream_t s) {
+    if (N == 0) return;
+    dense_stdp_kernel<<<(N + kTile - 1) / kTile, kTile, 0, s>>>(nets, trace, lr_ltp, lr_ltd, w_min, w_max);
 }
 void launch_stdp(const NetArgs* nets, const uint2* tiles, uint32_t n, hipStream_t s) {
     if (n == 0) return;
@@ -832,6 +949,13 @@ void launch_inject(float* ring, const uint64_t* step, uint32_t N, const uint32_t
     if (count == 0) return;
     if (serial) inject_kernel<<<1, 1, 0, s>>>(ring, step, N, idx, val, count, 1);
     else inject_kernel<<<(count + 255) / 256, 256, 0, s>>>(ring, step, N, idx, val, count, 0);
+}
+void launch_tm_init(const uint8_t* type, const float4* prof, float* r, float* u, uint32_t* last, size_t n,
+                    hipStream_t s) {
+    if (n == 0) return;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    tm_init_kernel<<<static_cast<uint32_t>(blocks), 256, 0, s>>>(type, prof, r, u, last, n);
 }
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
     if (n == 0) return;

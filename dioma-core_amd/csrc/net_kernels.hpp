@@ -103,6 +103,14 @@ struct NetArgs {
     const uint32_t* edge_pre;   // [n_syn]
     const uint8_t* plastic;     // [n_syn] or null (all)
     float stdp_dpre, stdp_dpost, a_plus, a_minus, w_min, w_max;
+    // Tsodyks-Markram short-term plasticity per event id (synapses, then
+    // source edges); null when no edge carries a profile
+    const uint8_t* tm_type;     // profile id, 0 = static
+    float* tm_r;
+    float* tm_u;
+    uint32_t* tm_last;
+    const float4* tm_prof;      // {u0, e^(-dt/tau_rec), e^(-dt/tau_fac), facilitating}
+    float* ev_w;                // delivered weight of each event this step (for the fix-up)
 };
 
 }  // namespace simuse

@@ -53,6 +53,10 @@ class StdpParams(C.Structure):
     ]
 
 
+class StpProfile(C.Structure):
+    _fields_ = [("u0", C.c_float), ("tau_rec", C.c_float), ("tau_fac", C.c_float)]
+
+
 class SimuseError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"[simuse {code}] {msg}")
@@ -82,6 +86,9 @@ _SIGS = {
     "simuse_net_set_scatter_mode": ([P, C.c_int32], C.c_int),
     "simuse_net_get_scatter_mode": ([P, i32p], C.c_int),
     "simuse_net_autotune": ([P, C.c_uint32], C.c_int),
+    "simuse_net_stdp_step": ([P, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float], C.c_int),
+    "simuse_net_set_short_term_plasticity": ([P, C.POINTER(StpProfile), C.c_int32, C.c_float, u8p, u8p,
+                                              C.c_uint64], C.c_int),
     "simuse_net_set_block_tiles": ([P, C.c_uint32], C.c_int),
     "simuse_net_get_block_tiles": ([P, u32p], C.c_int),
     "simuse_net_set_graph_steps": ([P, C.c_uint32], C.c_int),
@@ -119,6 +126,7 @@ _SIGS_OPT = {
     "simuse_kura_sync": ([P], C.c_int),
     "simuse_kura_download_phases": ([P, f64p, C.c_uint32], C.c_int),
     "simuse_kura_order_parameter": ([P, f64p], C.c_int),
+    "simuse_kura_phase_coherence": ([P, f32p, f64p], C.c_int),
     "simuse_kura_step_counter": ([P, u64p], C.c_int),
     "simuse_kura_set_profiling": ([P, C.c_int32], C.c_int),
     "simuse_kura_kernel_times": ([P, f32p, C.c_int32], C.c_int),

@@ -12,7 +12,7 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from ._lib import P, check, f32p, i32p, lib, u8p, u32p, u64p
+from ._lib import P, StdpParams, StpProfile, check, f32p, i32p, lib, u8p, u32p, u64p
 from .network import N_PARAMS, N_STATE, NetworkPop, _descs, _ptr
 
 
@@ -34,6 +34,9 @@ _SIGS = {
     "simuse_batch_add_poisson_source": ([P, C.c_int32, C.c_float, C.c_float, C.c_uint32, u32p, f32p, u8p,
                                          C.c_uint32], C.c_int),
     "simuse_batch_set_noise": ([P, C.c_int32, C.c_int32, f32p, C.c_uint64], C.c_int),
+    "simuse_batch_set_plasticity": ([P, C.c_int32, C.POINTER(StdpParams), u8p], C.c_int),
+    "simuse_batch_set_short_term_plasticity": ([P, C.c_int32, C.POINTER(StpProfile), C.c_int32, C.c_float, u8p,
+                                                u8p, C.c_uint64], C.c_int),
     "simuse_batch_set_poisson_drive": ([P, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_uint32],
                                        C.c_int),
     "simuse_batch_set_horizon": ([P, C.c_int32, C.c_uint64], C.c_int),
@@ -119,6 +122,22 @@ class Batch:
             self.add_poisson_source(nid, s.rate_hz, s.dt_ms, s.seed, s.posts, s.weights, s.delays)
         for pop, sigma, seed in spec.noise:
             self.set_noise(nid, pop, sigma, seed)
+        if spec.stdp:
+            st = dict(spec.stdp)
+            plastic = st.pop("plastic", None)
+            sp = StdpParams(st["a_plus"], st["a_minus"], st["tau_plus"], st["tau_minus"], st["w_min"], st["w_max"])
+            pl = None if plastic is None else np.ascontiguousarray(plastic, np.uint8)
+            check(_L().simuse_batch_set_plasticity(self._h, nid, C.byref(sp),
+                                                   _ptr(pl, C.c_uint8) if pl is not None else None))
+        if spec.stp:
+            profs = spec.stp["profiles"]
+            arr = (StpProfile * len(profs))(*[StpProfile(*map(float, p)) for p in profs])
+            syn = np.ascontiguousarray(spec.stp["syn_types"], np.uint8)
+            src = spec.stp.get("src_types")
+            src = None if src is None else np.ascontiguousarray(src, np.uint8)
+            check(_L().simuse_batch_set_short_term_plasticity(
+                self._h, nid, arr, len(profs), float(spec.stp["dt"]), _ptr(syn, C.c_uint8),
+                _ptr(src, C.c_uint8) if src is not None else None, 0 if src is None else src.size))
         return nid
 
     def add_generated(self, g: GenDesc) -> int:

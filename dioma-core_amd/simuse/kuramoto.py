@@ -68,6 +68,15 @@ class KuramotoGraph:
         check(lib().simuse_kura_order_parameter(self._h, C.byref(r)))
         return float(r.value)
 
+    def phase_coherence(self, amplitudes) -> float:
+        """|sum a e^{i theta}| / sum a (measure/mod.rs:76-106)."""
+        a = np.ascontiguousarray(amplitudes, np.float32)
+        if a.size != self.n:
+            raise ValueError("one amplitude per oscillator")
+        r = C.c_double()
+        check(lib().simuse_kura_phase_coherence(self._h, _p(a, C.c_float), C.byref(r)))
+        return float(r.value)
+
     def step_counter(self) -> int:
         s = C.c_uint64()
         check(lib().simuse_kura_step_counter(self._h, C.byref(s)))

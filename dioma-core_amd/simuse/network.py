@@ -19,7 +19,7 @@ from typing import Optional, Sequence
 import numpy as np
 
 from . import _lib
-from ._lib import PopDesc, StdpParams, check, lib
+from ._lib import PopDesc, StdpParams, StpProfile, check, lib
 
 N_STATE = {_lib.MODEL_LIF: 2, _lib.MODEL_IZHIKEVICH: 3, _lib.MODEL_HH: 6}
 N_PARAMS = {_lib.MODEL_LIF: 6, _lib.MODEL_IZHIKEVICH: 7, _lib.MODEL_HH: 13}
@@ -141,6 +141,19 @@ class CompiledNetwork:
             m = np.ascontiguousarray(plastic, dtype=np.uint8)
             check(lib().simuse_net_set_plasticity(self._h, C.byref(sp), _ptr(m, C.c_uint8)))
 
+    def set_short_term_plasticity(self, profiles, dt: float, syn_types, src_types=None) -> None:
+        """Tsodyks-Markram synapses: profiles = [(u0, tau_rec, tau_fac or 0)], per-edge profile
+        ids (1-based, 0 = static) for the synapses and optionally the source edges
+        (Network::register_plasticity, core.rs:597-619; tm_spike, push.rs:60-86)."""
+        arr = (StpProfile * len(profiles))(*[StpProfile(*map(float, p)) for p in profiles])
+        syn = np.ascontiguousarray(syn_types, dtype=np.uint8)
+        if syn.size != self.n_syn:
+            raise ValueError("syn_types must have one entry per synapse")
+        src = None if src_types is None else np.ascontiguousarray(src_types, dtype=np.uint8)
+        check(lib().simuse_net_set_short_term_plasticity(
+            self._h, arr, len(profiles), float(dt), _ptr(syn, C.c_uint8),
+            _ptr(src, C.c_uint8) if src is not None else None, 0 if src is None else src.size))
+
     def set_scatter_mode(self, mode: int) -> None:
         check(lib().simuse_net_set_scatter_mode(self._h, int(mode)))
 
@@ -168,6 +181,12 @@ class CompiledNetwork:
     def step(self) -> None:
         """One step, synchronous (CompiledNetwork::step, network.rs:411)."""
         check(lib().simuse_net_step(self._h))
+
+    def stdp_step(self, trace_decay: float, lr_ltp: float, lr_ltd: float, w_min: float, w_max: float) -> None:
+        """One step with the dense every-synapse STDP rule (CompiledNetwork::stdp_step,
+        network.rs:495-718); asynchronous."""
+        check(lib().simuse_net_stdp_step(self._h, float(trace_decay), float(lr_ltp), float(lr_ltd),
+                                         float(w_min), float(w_max)))
 
     def step_graph(self, n_steps: int = 1) -> None:
         """Enqueue n steps of graph replay and return (network.rs:431)."""

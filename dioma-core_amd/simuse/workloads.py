@@ -39,6 +39,9 @@ class NetSpec:
     sources: List[Source] = field(default_factory=list)
     noise: List[Tuple[int, np.ndarray, int]] = field(default_factory=list)
     stdp: Optional[dict] = None
+    # Tsodyks-Markram short-term plasticity: {profiles: [(u0, tau_rec, tau_fac)], dt,
+    # syn_types: uint8[n_syn] (0 = static), src_types: uint8[source edges] or absent}
+    stp: Optional[dict] = None
 
     @property
     def n_total(self) -> int:
@@ -214,6 +217,9 @@ def build_device(spec: NetSpec, graph_steps: int = 8) -> CompiledNetwork:
         st = dict(spec.stdp)
         plastic = st.pop("plastic", None)
         net.set_plasticity(plastic=plastic, **st)
+    if spec.stp:
+        net.set_short_term_plasticity(spec.stp["profiles"], spec.stp["dt"], spec.stp["syn_types"],
+                                      spec.stp.get("src_types"))
     net.set_graph_steps(graph_steps)
     return net
 

@@ -125,8 +125,36 @@ int simuse_net_autotune(simuse_net* net, uint32_t trial_steps);
 /* Steps captured per hipGraph replay (1..64). */
 int simuse_net_set_graph_steps(simuse_net* net, uint32_t steps);
 
+/* Tsodyks-Markram short-term plasticity profile (PlasticityProfile,
+ * P/src/api/net/synapse.rs:240-285).  tau_fac <= 0: depression only. */
+typedef struct {
+    float u0;       /* baseline release probability, (0, 1] */
+    float tau_rec;  /* recovery time constant (ms), > 0 */
+    float tau_fac;  /* facilitation time constant (ms), <= 0 for none */
+} simuse_stp_profile;
+
+/* Network::register_plasticity (P/src/api/net/network/core.rs:597-619) for
+ * profiles 1..n_profiles, and per-edge profile ids (0 = static) for the
+ * synapses (Edge::with_synapse_plasticity) and, optionally, for the Poisson
+ * source edges in add order (n_src_edges must then equal their count).  The
+ * push delivers weight*R*U and advances R, U (tm_spike, P/src/api/net/
+ * push.rs:60-86); state starts at R = 1, U = U0 (build.rs:70-84).  Call after
+ * add_poisson_source and before the first step. */
+int simuse_net_set_short_term_plasticity(simuse_net* net, const simuse_stp_profile* profiles, int32_t n_profiles,
+                                         float dt, const uint8_t* syn_types, const uint8_t* src_types,
+                                         uint64_t n_src_edges);
+
 /* CompiledNetwork::step (PC/src/network.rs:411): one step, synchronous. */
 int simuse_net_step(simuse_net* net);
+/* CompiledNetwork::stdp_step (PC/src/network.rs:495-718, kernels :53-83):
+ * one step with the dense every-synapse rule.  trace = trace*decay + spike of
+ * the previous step (one trace per neuron, zero at create/reset), then the
+ * population update, then for every synapse
+ * w = min(w_max, max(w_min, (w + lr_ltp*trace[pre]) - lr_ltd*trace[post])),
+ * then the push with the updated weights.  Asynchronous like step_async.
+ * Returns SIMUSE_E_STATE when pair-based plasticity is enabled. */
+int simuse_net_stdp_step(simuse_net* net, float trace_decay, float lr_ltp, float lr_ltd, float w_min,
+                         float w_max);
 /* step_graph (PC/src/network.rs:431) generalised to n steps: enqueue only.
  * The slot and step counter advance on the device, so back-to-back calls
  * need no host synchronisation. */
@@ -230,6 +258,12 @@ int simuse_batch_n_networks(simuse_batch* b, int32_t* n);
 int simuse_batch_add_poisson_source(simuse_batch* b, int32_t net_id, float rate_hz, float dt_ms, uint32_t seed,
                                     const uint32_t* posts, const float* w, const uint8_t* delay, uint32_t count);
 int simuse_batch_set_noise(simuse_batch* b, int32_t net_id, int32_t pop, const float* sigma, uint64_t seed);
+/* simuse_net_set_plasticity / simuse_net_set_short_term_plasticity for one
+ * network of the batch (before its first step). */
+int simuse_batch_set_plasticity(simuse_batch* b, int32_t net_id, const simuse_stdp_params* p, const uint8_t* plastic);
+int simuse_batch_set_short_term_plasticity(simuse_batch* b, int32_t net_id, const simuse_stp_profile* profiles,
+                                           int32_t n_profiles, float dt, const uint8_t* syn_types,
+                                           const uint8_t* src_types, uint64_t n_src_edges);
 int simuse_batch_set_poisson_drive(simuse_batch* b, int32_t net_id, int32_t pop, float rate_hz, float dt_ms,
                                    float weight, uint32_t drive_seed0);
 /* The network stops advancing once its step counter reaches `steps`. */
@@ -352,6 +386,9 @@ int simuse_kura_download_phases(simuse_kura* k, double* out, uint32_t n);
 /* R = |sum_i e^{i theta_i}| / n (phase_coherence with unit amplitudes,
  * P/src/kernels/measure/mod.rs:76-106), deterministic FP64 reduction. */
 int simuse_kura_order_parameter(simuse_kura* k, double* r);
+/* phase_coherence (P/src/kernels/measure/mod.rs:76-106): |sum a_i e^{i theta_i}| /
+ * max(sum a_i, 1e-12) with host amplitudes a (n values), summed in FP64. */
+int simuse_kura_phase_coherence(simuse_kura* k, const float* amplitudes, double* r);
 int simuse_kura_step_counter(simuse_kura* k, uint64_t* steps);
 /* HIP-event timing of whole steps (all stage kernels of a step). */
 int simuse_kura_set_profiling(simuse_kura* k, int32_t on);

@@ -38,6 +38,7 @@ _SIGS = {
     "orc_net_set_plasticity": ([P, C.POINTER(OrcStdp), u8p], C.c_int),
     "orc_net_inject_current": ([P, C.c_int32, u32p, f32p, C.c_uint32], C.c_int),
     "orc_net_step": ([P, C.c_uint32], None),
+    "orc_net_stdp_step": ([P, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float], None),
     "orc_net_step_counter": ([P], C.c_uint64),
     "orc_net_spikes": ([P, i32p], None),
     "orc_net_active": ([P, u32p], C.c_uint32),
@@ -52,6 +53,10 @@ _SIGS = {
     "orc_net_set_threads": ([P, C.c_int32], None),
     "orc_poisson_spike": ([C.c_uint32, C.c_uint32, C.c_uint64, C.c_float], C.c_int32),
     "orc_gauss": ([C.c_uint64, C.c_uint32, C.c_uint64], C.c_float),
+    "orc_tm_spike": ([C.c_float, C.c_float, C.c_float, C.c_int32, f32p, f32p, u32p, C.c_float, C.c_uint64],
+                     C.c_float),
+    "orc_powi_f32": ([C.c_float, C.c_int32], C.c_float),
+    "orc_net_set_stp": ([P, C.c_void_p, C.c_int32, C.c_float, C.c_void_p, C.c_void_p, C.c_uint64], C.c_int),
     "orc_cos2pi_f32": ([C.c_float], C.c_float),
     "orc_exp_f32": ([C.c_float], C.c_float),
     "orc_ln_f32": ([C.c_float], C.c_float),
@@ -85,6 +90,37 @@ def lib():
 
 def _p(a, t):
     return a.ctypes.data_as(C.POINTER(t))
+
+
+class OrcStp(C.Structure):
+    _fields_ = [("u0", C.c_float), ("tau_rec", C.c_float), ("tau_fac", C.c_float)]
+
+
+def stp_arrays(spec):
+    """Profiles and per-edge type arrays of spec.stp = {profiles: [(u0, tau_rec, tau_fac)],
+    dt, syn_types (n_syn), src_types (all source edges, optional)}."""
+    st = spec.stp
+    profs = st["profiles"]
+    arr = (OrcStp * len(profs))(*[OrcStp(*map(float, p)) for p in profs])
+    syn = np.ascontiguousarray(st["syn_types"], np.uint8)
+    src = st.get("src_types")
+    n_src = sum(len(s.posts) for s in spec.sources)
+    if src is not None:
+        src = np.ascontiguousarray(src, np.uint8)
+    return arr, len(profs), syn, src, n_src
+
+
+def tm_spike(u0, a_rec, a_fac, is_fac, state, weight, step):
+    """tm_spike on one synapse; state = [r, u, last] is advanced in place."""
+    r, u, last = C.c_float(state[0]), C.c_float(state[1]), C.c_uint32(state[2])
+    out = lib().orc_tm_spike(u0, a_rec, a_fac, 1 if is_fac else 0, C.byref(r), C.byref(u), C.byref(last),
+                             weight, step)
+    state[0], state[1], state[2] = r.value, u.value, last.value
+    return float(out)
+
+
+def powi_f32(a, n):
+    return float(lib().orc_powi_f32(a, n))
 
 
 class OracleNetwork:
@@ -129,6 +165,11 @@ class OracleNetwork:
             plastic = np.ascontiguousarray(plastic, np.uint8)
             sp = OrcStdp(st["a_plus"], st["a_minus"], st["tau_plus"], st["tau_minus"], st["w_min"], st["w_max"])
             assert L.orc_net_set_plasticity(self.h, C.byref(sp), _p(plastic, C.c_uint8)) == 0
+        if getattr(spec, "stp", None):
+            prof, n_prof, syn, src, n_src = stp_arrays(spec)
+            assert L.orc_net_set_stp(self.h, prof, n_prof, float(spec.stp["dt"]), syn.ctypes.data,
+                                     src.ctypes.data if src is not None else None, n_src) == 0
+            self._stp_keep = (prof, syn, src)
         if threads > 1:
             L.orc_net_set_threads(self.h, threads)
 
@@ -145,6 +186,10 @@ class OracleNetwork:
 
     def step(self, n: int = 1):
         lib().orc_net_step(self.h, n)
+
+    def stdp_step(self, decay, lr_ltp, lr_ltd, w_min, w_max):
+        """One step with the dense every-synapse STDP (CompiledNetwork::stdp_step)."""
+        lib().orc_net_stdp_step(self.h, decay, lr_ltp, lr_ltd, w_min, w_max)
 
     def inject_current(self, pop, idx, val):
         i = np.ascontiguousarray(idx, np.uint32)

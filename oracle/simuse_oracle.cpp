@@ -109,6 +109,48 @@ static inline float exp_f32(float x) {
 
 // Standard normal draw for (seed, neuron, step): Marsaglia polar method on
 // 24-bit uniforms from mix64.  sqrt and / are IEEE correctly rounded.
+// x^n for n >= 0 by right-to-left binary exponentiation, the expansion of
+// f32::powi (compiler-rt __powisf2): r *= a on set bits, a *= a between.
+template <class R>
+static inline R powi_r(R a, int n) {
+    R r = R(1);
+    for (;;) {
+        if (n & 1) r *= a;
+        n /= 2;
+        if (n == 0) break;
+        a *= a;
+    }
+    return r;
+}
+
+struct TmProfile {
+    float u0, a_rec, a_fac;
+    bool is_fac;
+};
+
+// tm_spike, P/src/api/net/push.rs:60-86: lazy exponential recovery over the
+// steps since the edge's last spike (capped at 1e6), then deliver
+// weight * R*U with the pre-spike factor, then deplete R (and facilitate U).
+template <class R>
+static inline R tm_spike_r(const TmProfile& p, R& r, R& u, uint32_t& last, R weight, uint64_t step) {
+    uint64_t gap = step > last ? step - last : 0;
+    if (gap > 1000000u) gap = 1000000u;
+    const int n = static_cast<int>(gap);
+    if (n > 0) {
+        const R ar = powi_r(R(p.a_rec), n);
+        r = R(1) + (r - R(1)) * ar;
+        if (p.is_fac) {
+            const R af = powi_r(R(p.a_fac), n);
+            u = R(p.u0) + (u - R(p.u0)) * af;
+        }
+    }
+    last = static_cast<uint32_t>(step);
+    const R scale = r * u;
+    r *= R(1) - u;
+    if (p.is_fac) u += R(p.u0) * (R(1) - u);
+    return weight * scale;
+}
+
 // cos(2*pi*u), u in [0,1) on a 2^-24 grid: exact quadrant split of 4u, then
 // the Taylor series of sin/cos((pi/2) g), g in [0, 1/2], through x^9 / x^10,
 // Horner in FP32 (same operation sequence as the device).
@@ -323,8 +365,11 @@ struct NetBase {
                            const float* w, const uint8_t* d, uint32_t count) = 0;
     virtual int set_noise(int pop, const float* sigma, uint64_t seed) = 0;
     virtual int set_plasticity(const orc_stdp_params* p, const uint8_t* plastic) = 0;
+    virtual int set_stp(const orc_stp_profile* profiles, int n_profiles, float dt, const uint8_t* syn_types,
+                        const uint8_t* src_types, uint64_t n_src_edges) = 0;
     virtual int inject(int pop, const uint32_t* idx, const float* val, uint32_t count) = 0;
     virtual void step(uint32_t n) = 0;
+    virtual void stdp_step(float decay, float lr_ltp, float lr_ltd, float w_min, float w_max) = 0;
     virtual uint64_t step_counter() const = 0;
     virtual void spikes(int32_t* out) const = 0;
     virtual uint32_t active(uint32_t* out) const = 0;
@@ -377,6 +422,16 @@ struct Net final : NetBase {
     std::vector<R> tr_pre, tr_post;
     std::vector<uint32_t> in_ptr, in_edge, edge_pre;
     std::vector<double> w64;  // FP64 master weights while plasticity is on
+    // Tsodyks-Markram state per edge: synapses first, then source edges in
+    // add order (Network::build, P/src/api/net/network/build.rs:70-84)
+    std::vector<TmProfile> tm_prof;
+    std::vector<uint8_t> tm_type;
+    std::vector<R> tm_r, tm_u;
+    std::vector<uint32_t> tm_last;
+    // dense every-synapse STDP of CompiledNetwork::stdp_step (one shared trace)
+    std::vector<R> dtrace;
+    bool dense = false;
+    float d_lr_ltp = 0, d_lr_ltd = 0, d_wmin = 0, d_wmax = 0;
 
     int init(const orc_pop_desc* pd, int np_, const uint32_t* rp, const uint32_t* c,
              const float* wf, const uint8_t* dl, uint64_t n_syn) {
@@ -455,6 +510,39 @@ struct Net final : NetBase {
         std::vector<uint32_t> cur(in_ptr.begin(), in_ptr.end() - 1);
         for (size_t e = 0; e < ns; ++e) in_edge[cur[col[e]]++] = static_cast<uint32_t>(e);
         return 0;
+    }
+    int set_stp(const orc_stp_profile* profiles, int n_profiles, float dt, const uint8_t* syn_types,
+                const uint8_t* src_types, uint64_t n_src_edges) override {
+        // Network::register_plasticity, P/src/api/net/network/core.rs:597-619
+        tm_prof.clear();
+        for (int k = 0; k < n_profiles; ++k) {
+            TmProfile t;
+            t.u0 = profiles[k].u0;
+            t.a_rec = std::exp(-dt / profiles[k].tau_rec);
+            t.is_fac = profiles[k].tau_fac > 0.0f;
+            t.a_fac = t.is_fac ? std::exp(-dt / profiles[k].tau_fac) : 0.0f;
+            tm_prof.push_back(t);
+        }
+        uint64_t n_src = 0;
+        for (auto& s : srcs) n_src += s.post.size();
+        if (src_types && n_src_edges != n_src) return -3;
+        const size_t ne = col.size() + n_src;
+        tm_type.assign(ne, 0);
+        for (size_t e = 0; e < col.size(); ++e) tm_type[e] = syn_types ? syn_types[e] : 0;
+        if (src_types)
+            for (uint64_t e = 0; e < n_src; ++e) tm_type[col.size() + e] = src_types[e];
+        for (uint8_t t : tm_type)
+            if (t > tm_prof.size()) return -3;
+        tm_r.assign(ne, R(1));
+        tm_u.assign(ne, R(0));
+        for (size_t e = 0; e < ne; ++e)
+            if (tm_type[e]) tm_u[e] = R(tm_prof[tm_type[e] - 1].u0);
+        tm_last.assign(ne, 0u);
+        return 0;
+    }
+    R delivered(size_t e, R weight) {
+        if (tm_type.empty() || tm_type[e] == 0) return weight;
+        return tm_spike_r(tm_prof[tm_type[e] - 1], tm_r[e], tm_u[e], tm_last[e], weight, stepc);
     }
     int inject(int p, const uint32_t* idx, const float* val, uint32_t count) override {
         // Network::inject_current — P/src/api/net/network/build.rs:243-275
@@ -566,29 +654,61 @@ struct Net final : NetBase {
             ++rast_steps;
         }
         if (stdp) stdp_update(pops.empty() ? R(0) : pops[0].pr[0]);
+        if (dense) dense_update();
         // 5: push, pre ascending then CSR row order (push.rs:126-173); the
         //    rayon variant (push.rs:184-237) applies triples in the same order.
         for (uint32_t pre : act) {
             for (uint32_t e = row_ptr[pre]; e < row_ptr[pre + 1]; ++e) {
                 uint32_t dd = d[e] < 1 ? 1u : d[e];
                 uint32_t ts = static_cast<uint32_t>((stepc + dd) % kSlots);
-                ringv[static_cast<size_t>(ts) * N + col[e]] += w[e];
+                ringv[static_cast<size_t>(ts) * N + col[e]] += delivered(e, w[e]);
             }
             n_events += row_ptr[pre + 1] - row_ptr[pre];
         }
         // 5b: Poisson sources (step.rs:166-185)
+        size_t src_edge = col.size();
         for (auto& s : srcs) {
+            const size_t first = src_edge;
+            src_edge += s.post.size();
             if (s.p <= 0.0f || !poisson_spike(s.seed, 0, stepc, s.p)) continue;
             for (size_t i = 0; i < s.post.size(); ++i) {
                 uint32_t dd = s.d[i] < 1 ? 1u : s.d[i];
                 uint32_t ts = static_cast<uint32_t>((stepc + dd) % kSlots);
-                ringv[static_cast<size_t>(ts) * N + s.post[i]] += R(s.w[i]);
+                ringv[static_cast<size_t>(ts) * N + s.post[i]] += delivered(first + i, R(s.w[i]));
             }
             n_events += s.post.size();
         }
         ++stepc;  // 6
     }
     void step(uint32_t n) override { for (uint32_t k = 0; k < n; ++k) step_once(); }
+
+    // CompiledNetwork::stdp_step, PC/src/network.rs:495-718 with the kernels
+    // of :53-83: trace = trace*decay + (spike of the previous step), then the
+    // step itself with every synapse updated between the population update
+    // and the push, w = min(w_max, max(w_min, (w + lr_ltp*tr[pre]) -
+    // lr_ltd*tr[post])), each operation rounded separately.
+    void stdp_step(float decay, float lr_ltp, float lr_ltd, float w_min, float w_max) override {
+        if (dtrace.empty()) dtrace.assign(N, R(0));
+        const R dc = R(decay);
+        for (uint32_t i = 0; i < N; ++i) dtrace[i] = dtrace[i] * dc + (spk[i] != 0 ? R(1) : R(0));
+        dense = true;
+        d_lr_ltp = lr_ltp;
+        d_lr_ltd = lr_ltd;
+        d_wmin = w_min;
+        d_wmax = w_max;
+        step_once();
+        dense = false;
+    }
+    void dense_update() {
+        const R lp = R(d_lr_ltp), ld = R(d_lr_ltd), lo = R(d_wmin), hi = R(d_wmax);
+        for (uint32_t pre = 0; pre < N; ++pre) {
+            const R tp = dtrace[pre];
+            for (uint32_t e = row_ptr[pre]; e < row_ptr[pre + 1]; ++e) {
+                const R nw = (w[e] + lp * tp) - ld * dtrace[col[e]];
+                w[e] = std::min(hi, std::max(lo, nw));
+            }
+        }
+    }
     uint64_t step_counter() const override { return stepc; }
     void spikes(int32_t* out) const override { std::memcpy(out, spk.data(), 4 * static_cast<size_t>(N)); }
     uint32_t active(uint32_t* out) const override {
@@ -719,6 +839,19 @@ int orc_net_inject_current(void* net, int32_t pop, const uint32_t* idx, const fl
     return static_cast<NetBase*>(net)->inject(pop, idx, val, count);
 }
 void orc_net_step(void* net, uint32_t n_steps) { static_cast<NetBase*>(net)->step(n_steps); }
+int orc_net_set_stp(void* net, const orc_stp_profile* profiles, int32_t n_profiles, float dt,
+                    const uint8_t* syn_types, const uint8_t* src_types, uint64_t n_src_edges) {
+    return static_cast<NetBase*>(net)->set_stp(profiles, n_profiles, dt, syn_types, src_types, n_src_edges);
+}
+float orc_tm_spike(float u0, float a_rec, float a_fac, int32_t is_fac, float* r, float* u, uint32_t* last,
+                   float weight, uint64_t step) {
+    TmProfile p{u0, a_rec, a_fac, is_fac != 0};
+    return tm_spike_r<float>(p, *r, *u, *last, weight, step);
+}
+float orc_powi_f32(float a, int32_t n) { return powi_r<float>(a, n); }
+void orc_net_stdp_step(void* net, float decay, float lr_ltp, float lr_ltd, float w_min, float w_max) {
+    static_cast<NetBase*>(net)->stdp_step(decay, lr_ltp, lr_ltd, w_min, w_max);
+}
 uint64_t orc_net_step_counter(void* net) { return static_cast<NetBase*>(net)->step_counter(); }
 void orc_net_spikes(void* net, int32_t* out) { static_cast<NetBase*>(net)->spikes(out); }
 uint32_t orc_net_active(void* net, uint32_t* out) { return static_cast<NetBase*>(net)->active(out); }

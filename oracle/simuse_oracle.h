@@ -50,6 +50,14 @@ typedef struct {
     float w_max;
 } orc_stdp_params;
 
+/* Tsodyks-Markram short-term plasticity profile (PlasticityProfile,
+ * P/src/api/net/synapse.rs:240-285); tau_fac <= 0 means depression only. */
+typedef struct {
+    float u0;
+    float tau_rec;
+    float tau_fac;
+} orc_stp_profile;
+
 void* orc_net_create(int kind, const orc_pop_desc* pops, int32_t n_pops,
                      const uint32_t* row_ptr, const uint32_t* col,
                      const float* w, const uint8_t* delay, uint64_t n_syn);
@@ -59,9 +67,15 @@ int orc_net_add_poisson_source(void* net, float rate_hz, float dt_ms, uint32_t s
                                const uint8_t* delay, uint32_t count);
 int orc_net_set_noise(void* net, int32_t pop, const float* sigma, uint64_t seed);
 int orc_net_set_plasticity(void* net, const orc_stdp_params* p, const uint8_t* plastic);
+/* Per-edge TM profile ids (0 = static) for the synapses and, in add order,
+ * for the Poisson-source edges (src_types may be NULL); profiles are 1-based. */
+int orc_net_set_stp(void* net, const orc_stp_profile* profiles, int32_t n_profiles, float dt,
+                    const uint8_t* syn_types, const uint8_t* src_types, uint64_t n_src_edges);
 int orc_net_inject_current(void* net, int32_t pop, const uint32_t* idx,
                            const float* val, uint32_t count);
 void orc_net_step(void* net, uint32_t n_steps);
+/* One step with the dense every-synapse STDP of CompiledNetwork::stdp_step. */
+void orc_net_stdp_step(void* net, float decay, float lr_ltp, float lr_ltd, float w_min, float w_max);
 uint64_t orc_net_step_counter(void* net);
 void orc_net_spikes(void* net, int32_t* out);
 uint32_t orc_net_active(void* net, uint32_t* out);
@@ -81,6 +95,11 @@ void orc_net_set_threads(void* net, int32_t n_threads);
 /* Stand-alone pieces for known-answer tests. */
 int32_t orc_poisson_spike(uint32_t seed, uint32_t stream, uint64_t step, float p);
 float orc_gauss(uint64_t seed, uint32_t i, uint64_t step);
+/* tm_spike (P/src/api/net/push.rs:60-86) on one synapse: returns the delivered
+ * weight and advances (r, u, last). */
+float orc_tm_spike(float u0, float a_rec, float a_fac, int32_t is_fac, float* r, float* u, uint32_t* last,
+                   float weight, uint64_t step);
+float orc_powi_f32(float a, int32_t n);
 float orc_cos2pi_f32(float u);
 float orc_exp_f32(float x);
 float orc_ln_f32(float x);

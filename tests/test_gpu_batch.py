@@ -154,3 +154,34 @@ def test_horizon_and_removal():
     assert b.n_networks() == 1
     b.run(20)
     assert np.array_equal(b.download_state(i2, 0), st120)
+
+
+def test_batch_with_plasticity_alone_vs_mix():
+    # pair STDP and Tsodyks-Markram networks keep their results inside a batch
+    print("plastic_mix", flush=True)
+    rng = np.random.default_rng(3)
+    a = random_net(2000, 40, seed=51, model="lif", max_delay=4, sources=5)
+    a.w = np.abs(a.w) / 40.0
+    a.stdp = dict(a_plus=0.01, a_minus=0.012, tau_plus=20.0, tau_minus=20.0, w_min=0.0, w_max=1.0)
+    b = random_net(2500, 40, seed=52, model="lif", max_delay=0, sources=5)
+    n_src = sum(len(s.posts) for s in b.sources)
+    b.stp = dict(profiles=[(0.5, 100.0, 0.0), (0.2, 50.0, 20.0)], dt=b.dt,
+                 syn_types=rng.integers(0, 3, b.n_syn).astype(np.uint8),
+                 src_types=rng.integers(0, 3, n_src).astype(np.uint8))
+    c = random_net(1500, 30, seed=53, model="izh", max_delay=2)
+    specs = [a, b, c]
+    T = 300
+    alone = []
+    for s in specs:
+        d = build_device(s)
+        d.run(T)
+        alone.append(([d.download_states(p) for p in range(len(s.pops))], d.counters()))
+        d.close()
+    bt = Batch()
+    ids = [bt.add_spec(s) for s in specs]
+    bt.set_graph_steps(5)
+    bt.run(T)
+    for s, i, (sts, cnt) in zip(specs, ids, alone):
+        for p in range(len(s.pops)):
+            assert np.array_equal(bt.download_state(i, p).view(np.uint32), sts[p].view(np.uint32))
+        assert bt.counters(i) == cnt

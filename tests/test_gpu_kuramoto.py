@@ -55,3 +55,18 @@ def test_kuramoto_uncoupled_linear_phase():
     t = 1000 * float(np.float32(0.01))
     want = th.astype(np.float64) + om.astype(np.float64) * t
     assert wrapped_err(dev.phases(), want) < 1e-9
+
+
+def test_phase_coherence_weighted():
+    # phase_coherence (pillars/src/kernels/measure/mod.rs:76-106) against the FP64 formula on the
+    # device's own phases; unit amplitudes reduce to the order parameter
+    n = 5000
+    rp, col, w, om, th = modular_small_world(n, modules=8, seed=4)
+    dev = KuramotoGraph(rp, col, w, om, th, 0.01, "rk4")
+    dev.run(50)
+    p = dev.phases()
+    a = np.random.default_rng(1).uniform(0.0, 2.0, n).astype(np.float32)
+    want = abs(np.sum(a.astype(np.float64) * np.exp(1j * p))) / np.sum(a.astype(np.float64))
+    assert abs(dev.phase_coherence(a) - want) < 1e-9
+    assert abs(dev.phase_coherence(np.ones(n, np.float32)) - dev.order_parameter()) < 1e-12
+    assert dev.phase_coherence(np.zeros(n, np.float32)) == 0.0

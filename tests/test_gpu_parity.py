@@ -263,6 +263,61 @@ def test_stdp_weights_bitexact():
     assert_same_state(dev, ora, spec)
 
 
+@pytest.mark.parametrize("sched", [SCATTER_TILE, SCATTER_WAVE])
+def test_dense_stdp_step_bitexact(sched):
+    # CompiledNetwork::stdp_step (pillars-cuda/src/network.rs:495-718): dense rule on every
+    # synapse, interleaved with ordinary steps; spikes, weights, state and ring bitwise
+    say("dense_stdp", sched)
+    spec = random_net(2500, 40, seed=31, model="lif", max_delay=3, sources=6)
+    dev = build_device(spec)
+    dev.set_scatter_mode(sched)
+    ora = po.OracleNetwork(spec, 32)
+    args = (0.95, 0.002, 0.0025, -4.0, 12.0)
+    for t in range(300):
+        if t % 10 < 7:
+            dev.stdp_step(*args)
+            ora.stdp_step(*args)
+        else:
+            dev.step()
+            ora.step()
+        assert np.array_equal(dev.download_spike_ids(), ora.active()), t
+    assert ora.counters()[0] > 500
+    assert np.array_equal(dev.download_weights().view(np.uint32), ora.weights().astype(np.float32).view(np.uint32))
+    assert not np.array_equal(ora.weights().astype(np.float32), spec.w)
+    assert dev.counters() == ora.counters()
+    assert_same_state(dev, ora, spec)
+    assert_same_ring(dev, ora)
+
+
+def _tm_spec(seed, max_delay):
+    spec = random_net(3000, 50, seed=seed, model="lif", max_delay=max_delay, sources=12)
+    rng = np.random.default_rng(seed)
+    n_src = sum(len(s.posts) for s in spec.sources)
+    spec.stp = dict(profiles=[(0.5, 100.0, 0.0), (0.1, 1e6, 50.0), (0.3, 20.0, 5.0)], dt=spec.dt,
+                    syn_types=rng.integers(0, 4, spec.n_syn).astype(np.uint8),
+                    src_types=rng.integers(0, 4, n_src).astype(np.uint8))
+    return spec
+
+
+@pytest.mark.parametrize("max_delay", [0, 7])
+def test_short_term_plasticity_bitexact(max_delay):
+    # Tsodyks-Markram delivery (push.rs:60-86) on synapses and source edges, mixed profiles,
+    # many events per ring cell (the fix-up re-sums delivered, not static, weights)
+    say("stp", max_delay)
+    spec = _tm_spec(41 + max_delay, max_delay)
+    dev, ora = run_pair(spec, 400)
+    assert ora.counters()[0] > 500
+    assert dev.counters() == ora.counters()
+    assert_same_state(dev, ora, spec)
+    assert_same_ring(dev, ora)
+    # a reset restores R = 1, U = U0 and replays the same trajectory
+    dev.reset(spec.pops)
+    ora2 = po.OracleNetwork(spec, 32)
+    dev.run(150)
+    ora2.step(150)
+    assert_same_state(dev, ora2, spec)
+
+
 def test_edge_cases():
     say("edge_cases")
     # n = 1 with a self loop firing every step under strong drive

@@ -298,6 +298,104 @@ def test_stdp_sign_known_answers():
     assert abs(_stdp_pair(1, False, tau=tau, a_minus=0.1) - 0.42) < 1e-6
 
 
+# --- Tsodyks-Markram short-term plasticity: pillars/src/api/net/push.rs:515-695 ---------
+REF_DEP = [0.500000, 0.295317, 0.211527, 0.177227, 0.163185, 0.157437, 0.155084, 0.154121,
+           0.153726, 0.153565, 0.153499, 0.153472]
+REF_FAC = [0.100000, 0.144296, 0.148667, 0.132751, 0.110006, 0.087407, 0.067775, 0.051800,
+           0.039252, 0.029592, 0.022241, 0.016687]
+
+
+def _f32exp(x):
+    return float(np.exp(np.float32(x)))
+
+
+def test_tm_matches_neuron_delivered_weights():
+    # NEURON tm.mod golden vectors (push.rs:646-695): spikes at steps 20 + 40k, dt 0.5 ms
+    dep = [1.0, 0.5, 0]
+    fac = [1.0, 0.1, 0]
+    for k in range(12):
+        step = 20 + 40 * k
+        w = po.tm_spike(0.5, _f32exp(-0.5 / 100.0), 0.0, False, dep, 1.0, step)
+        assert abs(w - REF_DEP[k]) < 1e-4, (k, w)
+        w = po.tm_spike(0.1, _f32exp(-0.5 / 1e6), _f32exp(-0.5 / 50.0), True, fac, 1.0, step)
+        assert abs(w - REF_FAC[k]) < 1e-4, (k, w)
+
+
+def test_tm_depression_recovery_and_facilitation_rules():
+    # push.rs:519-543: negligible recovery -> 0.5, 0.25, 0.125, 0.0625
+    st = [1.0, 0.5, 0]
+    a = _f32exp(-0.5e-6 / 1e6)
+    got = [po.tm_spike(0.5, a, 0.0, False, st, 1.0, s) for s in (0, 10, 20, 30)]
+    assert np.allclose(got, [0.5, 0.25, 0.125, 0.0625], atol=1e-3)
+    # push.rs:546-588: spaced spikes recover to ~U0, tight ones keep decaying
+    t, s = [1.0, 0.5, 0], [1.0, 0.5, 0]
+    tight = [po.tm_spike(0.5, _f32exp(-1.0), 0.0, False, t, 1.0, k) for k in range(4)]
+    spaced = [po.tm_spike(0.5, _f32exp(-1.0), 0.0, False, s, 1.0, k) for k in (0, 1000, 2000, 3000)]
+    assert abs(tight[0] - spaced[0]) < 1e-4
+    assert all(spaced[i] > tight[i] and abs(spaced[i] - 0.5) < 0.05 for i in range(1, 4))
+    # push.rs:591-612: facilitation, U 0.1 -> 0.19 -> ~0.2198
+    f = [1.0, 0.1, 0]
+    w0 = po.tm_spike(0.1, _f32exp(-1e-6), _f32exp(-1.0), True, f, 1.0, 0)
+    w1 = po.tm_spike(0.1, _f32exp(-1e-6), _f32exp(-1.0), True, f, 1.0, 1)
+    assert w1 > w0 and abs(f[1] - 0.2198) < 1e-3
+    # gap capped at 1e6 steps, recovery exact at a_rec = 1 (closed form r = (1-U)^k)
+    g = [1.0, 0.5, 0]
+    ws = [po.tm_spike(0.5, 1.0, 0.0, False, g, 2.0, k) for k in range(5)]
+    assert ws == [2.0 * 0.5 * 0.5 ** k for k in range(5)]
+
+
+def test_powi_is_binary_exponentiation():
+    # exact for powers of two; elsewhere within the repeated-squaring bound 2n*2^-24
+    assert po.powi_f32(0.5, 64) == 2.0 ** -64 and po.powi_f32(2.0, 100) == 2.0 ** 100
+    for a in (0.99, 1.0001, 0.36787945, 0.9999):
+        for n in (1, 2, 3, 7, 64, 1000, 12345):
+            want = float(np.float64(np.float32(a)) ** n)
+            if not (1e-37 < want < 1e37):
+                continue
+            assert abs(po.powi_f32(a, n) - want) <= 2 * n * 2.0 ** -24 * want, (a, n)
+
+
+def test_tm_network_depresses_repeated_drive():
+    # pillars/src/api/net/network/tests.rs:530-570: pre driven every step; the static edge keeps
+    # post 1 firing, the depressing edge (U0 0.5, tau_rec 10 ms) silences post 2
+    pop = lif_pop(3, 0.5, 0.1, -70.0, -65.0, -70.0, 1.0, -70.0)
+    rp, col, w, d = csr_from_triplets(3, [0, 0], [1, 2], np.array([100.0, 100.0], np.float32),
+                                      np.zeros(2, np.uint8))
+    spec = NetSpec("tm", [pop], rp, col, w, d, 0.5, 0)
+    spec.stp = dict(profiles=[(0.5, 10.0, 0.0)], dt=0.5, syn_types=np.array([0, 1], np.uint8))
+    o = po.OracleNetwork(spec)
+    fired = np.zeros(3, np.int64)
+    for _ in range(500):
+        o.inject_current(0, [0], [100.0])
+        o.step()
+        fired += o.spikes()
+    assert fired[1] > 20 and fired[2] == 0
+
+
+# --- dense STDP of CompiledNetwork::stdp_step (pillars-cuda/src/network.rs:53-83) -------
+def _dense_run(k, pre_first, decay=0.9, lr_ltp=0.01, lr_ltd=0.02, w0=0.5, w_max=1.0):
+    dt = 0.5
+    pops = [lif_pop(2, dt, 0.5, -70.0, -54.0, -70.0, 0.0, [100.0 if pre_first else -70.0,
+                                                           -70.0 if pre_first else 100.0])]
+    rp, col, w, d = csr_from_triplets(2, [0], [1], np.array([w0], np.float32), np.array([5], np.uint8))
+    o = po.OracleNetwork(NetSpec("dense", pops, rp, col, w, d, dt, 0))
+    for _ in range(k + 1):  # call 1 is the step in which the forced neuron fires
+        o.stdp_step(decay, lr_ltp, lr_ltd, 0.0, w_max)
+    return o.weights()[0]
+
+
+@pytest.mark.parametrize("k", [1, 4, 12])
+def test_dense_stdp_closed_form(k):
+    # the trace of the neuron that fired at step 0 is 1, dec, dec^2, ... from the next
+    # call on (traces see the previous step's spikes); every call adds lr * trace
+    dec = 0.9
+    geo = (1 - dec ** k) / (1 - dec)
+    assert abs(_dense_run(k, True) - (0.5 + 0.01 * geo)) < 1e-6    # pre fired: LTP only
+    assert abs(_dense_run(k, False) - (0.5 - 0.02 * geo)) < 1e-6   # post fired: LTD only
+    # clipping at w_max
+    assert _dense_run(k, True, lr_ltp=1.0) == 1.0
+
+
 # --- statistics: pillars/src/kernels/stats/{rates,isi}.rs -----------------------
 def test_rates_and_cv_brute_force():
     rng = np.random.default_rng(3)
