@@ -75,6 +75,25 @@ __device__ __forceinline__ float det_expf(float x) {
     return p * __uint_as_float(static_cast<uint32_t>(k + 127) << 23);
 }
 
+// fast_exp (P/src/math/fast_exp.rs:21-63): 2^n * p(f), t = x*log2(e),
+// n = floor(t) (saturating), f = t - n, degree-6 Horner rounded step by step.
+__device__ __forceinline__ float fast_expf(float x) {
+    const float t = x * 1.44269502f;
+    float tc = t;
+    if (tc > 1000.0f) tc = 1000.0f;
+    if (tc < -1000.0f) tc = -1000.0f;
+    const int n = tc == tc ? static_cast<int>(floorf(tc)) : 0;
+    const float f = t - static_cast<float>(n);
+    const float c1 = 0.693147182f, c2 = 0.2402265f, c3 = 0.05550411f, c4 = 0.009618129f, c5 = 0.001333356f,
+                c6 = 0.0001540353f;
+    float poly = (c6 * f + c5) * f + c4;
+    poly = (poly * f + c3) * f + c2;
+    poly = (poly * f + c1) * f + 1.0f;
+    if (n < -126) return 0.0f;
+    if (n > 127) return __builtin_inff();
+    return __uint_as_float(static_cast<uint32_t>(n + 127) << 23) * poly;
+}
+
 // cos(2*pi*u) for u in [0, 1) on a 2^-24 grid: quadrant from 4u (exact),
 // then sin or cos of (pi/2)*g, g in [0, 1/2], by their Taylor series through
 // x^9 / x^10 (truncation < 2e-9 on [0, pi/4]), Horner in FP32.

@@ -55,10 +55,14 @@ void launch_raster_stats(const uint64_t* raster, uint32_t n_words, uint32_t step
             return ::simuse::fail(SIMUSE_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
-inline int model_states(int m) { return m == SIMUSE_MODEL_LIF ? 2 : m == SIMUSE_MODEL_IZHIKEVICH ? 3 : 6; }
-inline int model_params(int m) { return m == SIMUSE_MODEL_LIF ? 6 : m == SIMUSE_MODEL_IZHIKEVICH ? 7 : 13; }
+inline int model_states(int m) {
+    return m == SIMUSE_MODEL_LIF ? 2 : m == SIMUSE_MODEL_IZHIKEVICH ? 3 : m == SIMUSE_MODEL_HH ? 6 : 9;
+}
+inline int model_params(int m) {
+    return m == SIMUSE_MODEL_LIF ? 6 : m == SIMUSE_MODEL_IZHIKEVICH ? 7 : m == SIMUSE_MODEL_HH ? 13 : 14;
+}
 // column of tau_ref and of the refractory state (LIF/Izhikevich only)
-inline int model_tau_ref_col(int m) { return m == SIMUSE_MODEL_HH ? -1 : 5; }
+inline int model_tau_ref_col(int m) { return m == SIMUSE_MODEL_LIF || m == SIMUSE_MODEL_IZHIKEVICH ? 5 : -1; }
 inline int model_refr_state(int m) { return m == SIMUSE_MODEL_LIF ? 1 : m == SIMUSE_MODEL_IZHIKEVICH ? 2 : -1; }
 
 struct HostPop {
@@ -285,8 +289,9 @@ struct Engine {
     // device tables
     NetArgs* d_nets = nullptr;
     uint32_t nets_cap = 0;
-    uint4* d_tiles[3] = {nullptr, nullptr, nullptr};  // neuron work items per model:
-    uint32_t n_tiles_m[3] = {0, 0, 0}, cap_tiles_m[3] = {0, 0, 0};  // {net, tile, tiles, pop}
+    static constexpr int kModels = 4;  // LIF, Izhikevich, HH, GPEA
+    uint4* d_tiles[kModels] = {};      // neuron work items per model: {net, tile, tiles, pop}
+    uint32_t n_tiles_m[kModels] = {}, cap_tiles_m[kModels] = {};
     uint2* d_blocks = nullptr;  // insert_kernel work: plastic networks' tiles, source tiles
     uint32_t n_blocks = 0, cap_blocks = 0;
     uint2* d_stdp = nullptr;
@@ -312,8 +317,9 @@ struct Engine {
         if (stream) (void)hipStreamSynchronize(stream);
         drop_graph();
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-        for (void* p : {static_cast<void*>(d_nets), static_cast<void*>(d_tiles[0]), static_cast<void*>(d_tiles[1]),
-                        static_cast<void*>(d_tiles[2]), static_cast<void*>(d_blocks), static_cast<void*>(d_stdp),
+        for (uint4* t : d_tiles)
+            if (t) (void)hipFree(t);
+        for (void* p : {static_cast<void*>(d_nets), static_cast<void*>(d_blocks), static_cast<void*>(d_stdp),
                         static_cast<void*>(d_fix)})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
@@ -354,7 +360,7 @@ struct Engine {
         if (!dirty) return SIMUSE_OK;
         HIP_TRY(hipStreamSynchronize(stream));
         std::vector<NetArgs> na;
-        std::vector<uint4> tl[3];
+        std::vector<uint4> tl[kModels];
         std::vector<uint2> bl, sl, fl;
         for (uint32_t k = 0; k < nets.size(); ++k) {
             NetState* s = nets[k];
@@ -377,7 +383,7 @@ struct Engine {
             if ((rc = grow(&d_nets, &nets_cap, static_cast<uint32_t>(na.size())))) return rc;
             HIP_TRY(hipMemcpy(d_nets, na.data(), sizeof(NetArgs) * na.size(), hipMemcpyHostToDevice));
         }
-        for (int m = 0; m < 3; ++m) {
+        for (int m = 0; m < kModels; ++m) {
             n_tiles_m[m] = static_cast<uint32_t>(tl[m].size());
             if (tl[m].empty()) continue;
             if ((rc = grow(&d_tiles[m], &cap_tiles_m[m], n_tiles_m[m]))) return rc;
@@ -405,7 +411,7 @@ struct Engine {
 
     void launch_step(hipEvent_t* e) {
         if (e) (void)hipEventRecord(e[0], stream);
-        for (int m = 0; m < 3; ++m) launch_neuron(m, sched, d_nets, d_tiles[m], n_tiles_m[m], stream, 0u);
+        for (int m = 0; m < kModels; ++m) launch_neuron(m, sched, d_nets, d_tiles[m], n_tiles_m[m], stream, 0u);
         if (e) (void)hipEventRecord(e[1], stream);
         launch_stdp(d_nets, d_stdp, n_stdp, stream);
         if (e) (void)hipEventRecord(e[2], stream);

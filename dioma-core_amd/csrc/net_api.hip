@@ -29,7 +29,7 @@ int validate_pops(const simuse_pop_desc* pops, int32_t n_pops, uint64_t* n_total
     uint64_t tot = 0;
     for (int32_t p = 0; p < n_pops; ++p) {
         const simuse_pop_desc& d = pops[p];
-        if (d.model < SIMUSE_MODEL_LIF || d.model > SIMUSE_MODEL_HH)
+        if (d.model < SIMUSE_MODEL_LIF || d.model > SIMUSE_MODEL_GPEA)
             return fail(SIMUSE_E_UNSUPPORTED, "population %d: unknown model %d", p, d.model);
         if (d.ns != static_cast<uint32_t>(model_states(d.model)) ||
             d.np != static_cast<uint32_t>(model_params(d.model)))
@@ -986,7 +986,8 @@ int simuse_net_stdp_step(simuse_net* net, float trace_decay, float lr_ltp, float
     // trace from the previous step's spikes; update without the inline
     // scatter; every synapse; then the push with the new weights.
     launch_dense_trace(e.d_nets, s.n_tiles, s.dtrace, trace_decay, e.stream);
-    for (int m = 0; m < 3; ++m) launch_neuron(m, e.sched, e.d_nets, e.d_tiles[m], e.n_tiles_m[m], e.stream, 1u);
+    for (int m = 0; m < Engine::kModels; ++m)
+        launch_neuron(m, e.sched, e.d_nets, e.d_tiles[m], e.n_tiles_m[m], e.stream, 1u);
     launch_dense_weights(e.d_nets, s.N, s.dtrace, lr_ltp, lr_ltd, w_min, w_max, e.stream);
     launch_insert(e.sched, e.d_nets, s.d_all_blocks, n_blk, e.stream);
     launch_fixup(e.d_nets, nullptr, s.n_fix, e.stream);

@@ -286,6 +286,83 @@ __device__ __forceinline__ void scatter_rows(const NetArgs& a, const ScatterCtx&
     }
 }
 
+// GpeaModel::step, P/src/zoo/neurons/gpea.rs:266-360 (pGPeA_k.mod): gates by
+// exponential Euler at the old V (fused decay*(x - x_inf) + x_inf), r from the
+// old cai, Ca by cnexp with I_Ca from the old state, Nernst potentials through
+// ln, implicit-Euler membrane, rising-edge spike through v_thresh.  Same
+// operation sequence as the oracle; #[no_clamp].
+// params: [0]dt [1]celsius [2]inv_cm [3]v_thresh [4]area [5]gnabar [6]gkdrbar
+//         [7]gl [8]el [9]gcatbar [10]gkcabar [11]kca [12]vol [13]caGain
+// state:  [0]v [1]m [2]h [3]n [4]p [5]q [6]cai [7]r [8]was_below
+__device__ __forceinline__ float gpea_sigmoid(float x, float th, float k) {
+    return 1.0f / (1.0f + fast_expf((x - th) * (1.0f / k)));
+}
+__device__ __forceinline__ float gpea_tau2(float v, float t0, float t1, float tht1, float sig1, float tht2,
+                                           float sig2) {
+    const float e1 = fast_expf((v - tht1) * (-1.0f / sig1));
+    const float e2 = fast_expf(-(v - tht2) * (1.0f / sig2));
+    return t0 + t1 / (e1 + e2);
+}
+__device__ __forceinline__ float gpea_exp_euler(float x, float x_inf, float tau, float dt) {
+    const float decay = fast_expf(-dt / tau);
+    return fmaf(decay, x - x_inf, x_inf);
+}
+__device__ __forceinline__ int gpea_update(const PopDev& P, uint32_t li, float I, float dt) {
+    float* s = P.st;
+    const size_t ds = P.dstride;
+    const float celsius = par(P, 1, li), inv_cm = par(P, 2, li), v_th = par(P, 3, li), area = par(P, 4, li);
+    const float gnabar = par(P, 5, li), gkdrbar = par(P, 6, li), gl = par(P, 7, li), el = par(P, 8, li);
+    const float gcatbar = par(P, 9, li), gkcabar = par(P, 10, li), kca = par(P, 11, li), vol = par(P, 12, li);
+    const float ca_gain = par(P, 13, li);
+    const float F = 96485.0f;
+    const float rt = 8.31441f * ((273.0f + celsius) - 9.5f) / F * 1000.0f;
+    const float ena = -rt * det_logf(10.0f / 140.0f);
+    const float ek = rt * det_logf(5.0f / 140.0f);
+    const float v = s[li];
+    const float m_new = gpea_exp_euler(s[ds + li], gpea_sigmoid(v, -38.0f, -7.0f),
+                                       0.001f + 0.1f / (1.0f + fast_expf((v - -53.0f) * (-1.0f / -0.7f))), dt);
+    const float h_new = gpea_exp_euler(s[2 * ds + li], gpea_sigmoid(v, -45.5f, 6.4f),
+                                       gpea_tau2(v, 0.0f, 4.5f, -50.0f, -15.0f, -50.0f, 16.0f), dt);
+    const float n_new = gpea_exp_euler(s[3 * ds + li], gpea_sigmoid(v, -42.0f, -14.0f),
+                                       gpea_tau2(v, 0.0f, 2.4f, -40.0f, -40.0f, -40.0f, 50.0f), dt);
+    const float p_old = s[4 * ds + li], q_old = s[5 * ds + li];
+    const float p_new = gpea_exp_euler(p_old, gpea_sigmoid(v, -56.0f, -6.7f),
+                                       gpea_tau2(v, 5.0f, 0.33f, -27.0f, -10.0f, -102.0f, 15.0f), dt);
+    const float q_new = gpea_exp_euler(q_old, gpea_sigmoid(v, -85.0f, 5.8f),
+                                       gpea_tau2(v, 0.0f, 400.0f, -50.0f, -15.0f, -50.0f, 16.0f), dt);
+    const float cai = s[6 * ds + li];
+    const float r_new = gpea_exp_euler(s[7 * ds + li], gpea_sigmoid(cai, 0.17e-3f, -0.08e-3f), 2.0f, dt);
+    const float eca_old = -rt * det_logf(cai / 2.0f) / 2.0f;
+    const float gcat_old = gcatbar * p_old * p_old * q_old;
+    const float ica_old = gcat_old * (v - eca_old);
+    const float j = -ica_old * area * 1e-14f / (2.0f * F * vol);
+    const float k_ca = ca_gain * kca;
+    const float cai_decay = fast_expf(-dt * k_ca);
+    const float cai_new = cai + (1.0f - cai_decay) * (ca_gain * j / k_ca - cai);
+    const float eca = -rt * det_logf(cai_new / 2.0f) / 2.0f;
+    const float g_na = gnabar * (m_new * m_new * m_new) * h_new;
+    const float n2 = n_new * n_new;
+    const float g_k = gkdrbar * (n2 * n2);
+    const float g_cat = gcatbar * p_new * p_new * q_new;
+    const float g_ahp = gkcabar * r_new * r_new;
+    const float g_sum = g_na + g_k + g_cat + g_ahp + gl;
+    const float g_e_sum = g_na * ena + (g_k + g_ahp) * ek + g_cat * eca + gl * el;
+    const float c_over_dt = 1.0f / (inv_cm * dt);
+    const float v_new = (c_over_dt * v + I + g_e_sum) / (c_over_dt + g_sum);
+    const bool above = v_new >= v_th;  // edge_spike, P/src/ops/spike.rs:70-74
+    const int spike = (above && s[8 * ds + li] > 0.0f) ? 1 : 0;
+    s[li] = v_new;
+    s[ds + li] = m_new;
+    s[2 * ds + li] = h_new;
+    s[3 * ds + li] = n_new;
+    s[4 * ds + li] = p_new;
+    s[5 * ds + li] = q_new;
+    s[6 * ds + li] = cai_new;
+    s[7 * ds + li] = r_new;
+    s[8 * ds + li] = above ? 0.0f : 1.0f;
+    return spike;
+}
+
 // Per-model register state: load everything a neuron-step reads, advance,
 // store.  Splitting the three lets a block issue the loads of all its tiles
 // before the first one is needed.
@@ -329,6 +406,15 @@ struct Neuron<1> {  // Izhikevich: states v, u, refr; params dt, a, b, c, d, tau
         P.st[P.dstride + li] = u;
         if (!(P.flags & kPopRefrElided)) P.st[2 * P.dstride + li] = refr;
     }
+};
+
+template <>
+struct Neuron<3> {  // GPEA: reads and writes its columns inside the update
+    __device__ __forceinline__ void load(const PopDev&, uint32_t) {}
+    __device__ __forceinline__ int step(const PopDev& P, uint32_t li, float I, float dt) {
+        return gpea_update(P, li, I, dt);
+    }
+    __device__ __forceinline__ void store(const PopDev&, uint32_t) const {}
 };
 
 template <>
@@ -908,11 +994,13 @@ void launch_neuron(int model, int sched, const NetArgs* nets, const uint4* items
     if (sched == 2) {
         if (model == 0) neuron_kernel<0, 2><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
         else if (model == 1) neuron_kernel<1, 2><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
-        else neuron_kernel<2, 2><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
+        else if (model == 2) neuron_kernel<2, 2><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
+        else neuron_kernel<3, 2><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
     } else {
         if (model == 0) neuron_kernel<0, 1><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
         else if (model == 1) neuron_kernel<1, 1><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
-        else neuron_kernel<2, 1><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
+        else if (model == 2) neuron_kernel<2, 1><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
+        else neuron_kernel<3, 1><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
     }
 }
 void launch_dense_trace(const NetArgs* nets, uint32_t n_tiles, float* trace, float decay, hipStream_t s) {

@@ -21,6 +21,7 @@ E_DEVICE = -6
 MODEL_LIF = 0
 MODEL_IZHIKEVICH = 1
 MODEL_HH = 2
+MODEL_GPEA = 3
 
 KURA_EULER = 0
 KURA_RK4 = 1

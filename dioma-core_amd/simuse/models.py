@@ -17,7 +17,7 @@ from typing import Sequence, Union
 
 import numpy as np
 
-from ._lib import MODEL_HH, MODEL_IZHIKEVICH, MODEL_LIF
+from ._lib import MODEL_GPEA, MODEL_HH, MODEL_IZHIKEVICH, MODEL_LIF
 from .network import N_PARAMS, N_STATE, NetworkPop, stride_for
 
 Num = Union[float, np.ndarray]
@@ -45,6 +45,26 @@ def lif_pop(n: int, dt: Num, inv_tau_m: Num, v_rest: Num, v_thresh: Num, v_reset
 def izh_pop(n: int, dt: Num, a: Num, b: Num, c: Num, d: Num, tau_ref: Num, v_thresh: Num, v0: Num,
             u0: Num, refr0: Num = 0.0) -> NetworkPop:
     return make_pop(MODEL_IZHIKEVICH, n, [dt, a, b, c, d, tau_ref, v_thresh], [v0, u0, refr0])
+
+
+# NEURON reference geometry of the GPEA tests (soma L=20, diam=40 um): area pi*40*20 um^2
+GPEA_AREA = 2513.2742
+# 0.5 nA IClamp over that area, in uA/cm^2
+GPEA_PULSE = 0.5e-3 / (GPEA_AREA * 1e-8)
+
+
+def gpea_pop(n: int, dt: Num = 0.025, celsius: Num = 24.0, inv_cm: Num = 1.0, v_thresh: Num = -20.0,
+             area: Num = GPEA_AREA, gnabar: Num = 49.0, gkdrbar: Num = 57.0, gl: Num = 0.35, el: Num = -60.0,
+             gcatbar: Num = 5.0, gkcabar: Num = 1.0, kca: Num = 2.0, vol: Num = 3.355e-11, ca_gain: Num = 0.1,
+             v0: Num = -60.0, m0: Num = 0.0414, h0: Num = 0.906, n0: Num = 0.2165, p0: Num = 0.355,
+             q0: Num = 0.01325, cai0: Num = 5e-5, r0: Num = 0.1821261, was_below0: Num = 1.0) -> NetworkPop:
+    """GPEA (pGPeA_k.mod) population with the NEURON reference parameters and initial
+    state of crates/SiMuse/crates/pillars/src/zoo/neurons/gpea.rs:363-424 (V=-60 mV, gates at
+    steady state, cai = 5e-5 as NEURON's finitialize sets it, dt 0.025 ms, 24 C)."""
+    return make_pop(MODEL_GPEA, n,
+                    [dt, celsius, inv_cm, v_thresh, area, gnabar, gkdrbar, gl, el, gcatbar, gkcabar, kca, vol,
+                     ca_gain],
+                    [v0, m0, h0, n0, p0, q0, cai0, r0, was_below0])
 
 
 def hh_pop(n: int, dt: Num, g_na: Num = 120.0, g_k: Num = 36.0, g_leak: Num = 0.3, e_na: Num = 50.0,

@@ -21,8 +21,8 @@ import numpy as np
 from . import _lib
 from ._lib import PopDesc, StdpParams, StpProfile, check, lib
 
-N_STATE = {_lib.MODEL_LIF: 2, _lib.MODEL_IZHIKEVICH: 3, _lib.MODEL_HH: 6}
-N_PARAMS = {_lib.MODEL_LIF: 6, _lib.MODEL_IZHIKEVICH: 7, _lib.MODEL_HH: 13}
+N_STATE = {_lib.MODEL_LIF: 2, _lib.MODEL_IZHIKEVICH: 3, _lib.MODEL_HH: 6, _lib.MODEL_GPEA: 9}
+N_PARAMS = {_lib.MODEL_LIF: 6, _lib.MODEL_IZHIKEVICH: 7, _lib.MODEL_HH: 13, _lib.MODEL_GPEA: 14}
 
 
 def _ptr(a: np.ndarray, ctype):

@@ -46,6 +46,7 @@ extern "C" {
 #define SIMUSE_MODEL_LIF 0        /* P/src/zoo/neurons/lif.rs:14-78: 2 states, 6 params */
 #define SIMUSE_MODEL_IZHIKEVICH 1 /* P/src/zoo/neurons/izhikevich.rs:30-125: 3 states, 7 params */
 #define SIMUSE_MODEL_HH 2         /* PC/kernels/hh.cu:107-216: 6 states, 13 params */
+#define SIMUSE_MODEL_GPEA 3       /* P/src/zoo/neurons/gpea.rs:266-360: 9 states, 14 params */
 
 /* Scatter schedules the autotuner chooses between. */
 #define SIMUSE_SCATTER_AUTO 0

@@ -56,6 +56,8 @@ _SIGS = {
     "orc_tm_spike": ([C.c_float, C.c_float, C.c_float, C.c_int32, f32p, f32p, u32p, C.c_float, C.c_uint64],
                      C.c_float),
     "orc_powi_f32": ([C.c_float, C.c_int32], C.c_float),
+    "orc_fast_exp": ([C.c_float], C.c_float),
+    "orc_gpea_rates": ([C.c_float, C.c_float, f32p], None),
     "orc_net_set_stp": ([P, C.c_void_p, C.c_int32, C.c_float, C.c_void_p, C.c_void_p, C.c_uint64], C.c_int),
     "orc_cos2pi_f32": ([C.c_float], C.c_float),
     "orc_exp_f32": ([C.c_float], C.c_float),
@@ -121,6 +123,17 @@ def tm_spike(u0, a_rec, a_fac, is_fac, state, weight, step):
 
 def powi_f32(a, n):
     return float(lib().orc_powi_f32(a, n))
+
+
+def fast_exp(x):
+    return float(lib().orc_fast_exp(x))
+
+
+def gpea_rates(v, cai):
+    """m_inf, h_inf, n_inf, p_inf, q_inf, r_inf, tau_m, tau_h, tau_n, tau_p, tau_q."""
+    out = np.zeros(11, np.float32)
+    lib().orc_gpea_rates(v, cai, _p(out, C.c_float))
+    return out
 
 
 class OracleNetwork:

@@ -213,13 +213,43 @@ static inline double vtrap_f64(double x, double y) {
 // Numeric twin policy
 // ---------------------------------------------------------------------------
 template <class R> struct Math;
+// fast_exp, P/src/math/fast_exp.rs:21-63: exp(x) = 2^n * p(f), t = x*log2(e),
+// n = floor(t), f = t - n, p a degree-6 polynomial; the Horner steps are
+// rounded separately (the reference's build without the FMA target feature).
+static inline float fast_exp_f32(float x) {
+    const float t = x * 1.44269502f;
+    // floor(t) as i32 (saturating); beyond +-1000 the result is 0 or inf anyway
+    float tc = t;
+    if (tc > 1000.0f) tc = 1000.0f;
+    if (tc < -1000.0f) tc = -1000.0f;
+    const int n = tc == tc ? static_cast<int>(std::floor(tc)) : 0;
+    const float f = t - static_cast<float>(n);
+    const float c1 = 0.693147182f, c2 = 0.2402265f, c3 = 0.05550411f, c4 = 0.009618129f, c5 = 0.001333356f,
+                c6 = 0.0001540353f;
+    float poly = (c6 * f + c5) * f + c4;
+    poly = (poly * f + c3) * f + c2;
+    poly = (poly * f + c1) * f + 1.0f;
+    if (n < -126) return 0.0f;
+    if (n > 127) return INFINITY;
+    uint32_t bits = static_cast<uint32_t>(n + 127) << 23;
+    float scale;
+    std::memcpy(&scale, &bits, 4);
+    return scale * poly;
+}
+
 template <> struct Math<float> {
     static float exp(float x) { return exp_f32(x); }
     static float vtrap(float x, float y) { return vtrap_f32(x, y); }
+    static float fast_exp(float x) { return fast_exp_f32(x); }
+    static float ln(float x) { return ln_f32(x); }
+    static float mul_add(float a, float b, float c) { return std::fma(a, b, c); }
 };
 template <> struct Math<double> {
     static double exp(double x) { return std::exp(x); }
     static double vtrap(double x, double y) { return vtrap_f64(x, y); }
+    static double fast_exp(double x) { return std::exp(x); }
+    static double ln(double x) { return std::log(x); }
+    static double mul_add(double a, double b, double c) { return std::fma(a, b, c); }
 };
 
 // f32::max / SimdLane::max for non-NaN operands.
@@ -345,14 +375,96 @@ static inline int hh_step(R* st, const R* pr, R I) {
     return spiked;
 }
 
+// GpeaModel::step, P/src/zoo/neurons/gpea.rs:266-360 (port of pGPeA_k.mod):
+// gates by exponential Euler at the old V (decay.mul_add(x - x_inf, x_inf),
+// fused), r from the old cai, Ca by cnexp with I_Ca frozen from the old state,
+// Nernst E_Na, E_K, E_Ca through ln, membrane by implicit Euler, rising-edge
+// spike through v_thresh with the was_below flag.  #[no_clamp].
+// params: [0]dt [1]celsius [2]inv_cm [3]v_thresh [4]area [5]gnabar [6]gkdrbar
+//         [7]gl [8]el [9]gcatbar [10]gkcabar [11]kca [12]vol [13]caGain
+// state:  [0]v [1]m [2]h [3]n [4]p [5]q [6]cai [7]r [8]was_below
+template <class R>
+struct Gpea {
+    using M = Math<R>;
+    // sigmoid(x, th, k) = 1/(1 + fast_exp((x - th) * (1/k)))   gpea.rs:130-133
+    static R sigmoid(R x, float th, float k) {
+        return R(1) / (R(1) + M::fast_exp((x - R(th)) * R(1.0f / k)));
+    }
+    // tau = t0 + t1/(fast_exp((v-tht1)*(-1/sig1)) + fast_exp(-(v-tht2)*(1/sig2)))   gpea.rs:148-180
+    static R tau2(R v, float t0, float t1, float tht1, float sig1, float tht2, float sig2) {
+        const R e1 = M::fast_exp((v - R(tht1)) * R(-1.0f / sig1));
+        const R e2 = M::fast_exp(-(v - R(tht2)) * R(1.0f / sig2));
+        return R(t0) + R(t1) / (e1 + e2);
+    }
+    static R tau_m(R v) {  // gpea.rs:141-145
+        const R den = R(1) + M::fast_exp((v - R(-53.0f)) * R(-1.0f / -0.7f));
+        return R(0.001f) + R(0.1f) / den;
+    }
+    static R tau_h(R v) { return tau2(v, 0.0f, 4.5f, -50.0f, -15.0f, -50.0f, 16.0f); }
+    static R tau_n(R v) { return tau2(v, 0.0f, 2.4f, -40.0f, -40.0f, -40.0f, 50.0f); }
+    static R tau_p(R v) { return tau2(v, 5.0f, 0.33f, -27.0f, -10.0f, -102.0f, 15.0f); }
+    static R tau_q(R v) { return tau2(v, 0.0f, 400.0f, -50.0f, -15.0f, -50.0f, 16.0f); }
+    static R m_inf(R v) { return sigmoid(v, -38.0f, -7.0f); }
+    static R h_inf(R v) { return sigmoid(v, -45.5f, 6.4f); }
+    static R n_inf(R v) { return sigmoid(v, -42.0f, -14.0f); }
+    static R p_inf(R v) { return sigmoid(v, -56.0f, -6.7f); }
+    static R q_inf(R v) { return sigmoid(v, -85.0f, 5.8f); }
+    static R r_inf(R cai) { return sigmoid(cai, 0.17e-3f, -0.08e-3f); }
+    static R exp_euler(R x, R x_inf, R tau, R dt) {  // gpea.rs:136-139
+        const R decay = M::fast_exp(-dt / tau);
+        return M::mul_add(decay, x - x_inf, x_inf);
+    }
+    static int step(R* st, const R* pr, R I) {
+        const R dt = pr[0], celsius = pr[1], inv_cm = pr[2], v_th = pr[3], area = pr[4];
+        const R gnabar = pr[5], gkdrbar = pr[6], gl = pr[7], el = pr[8], gcatbar = pr[9];
+        const R gkcabar = pr[10], kca = pr[11], vol = pr[12], ca_gain = pr[13];
+        const R F = R(96485.0f);
+        const R rt = R(8.31441f) * ((R(273.0f) + celsius) - R(9.5f)) / F * R(1000.0f);
+        const R ena = -rt * M::ln(R(10.0f) / R(140.0f));
+        const R ek = rt * M::ln(R(5.0f) / R(140.0f));
+        const R v = st[0];
+        const R m_new = exp_euler(st[1], m_inf(v), tau_m(v), dt);
+        const R h_new = exp_euler(st[2], h_inf(v), tau_h(v), dt);
+        const R n_new = exp_euler(st[3], n_inf(v), tau_n(v), dt);
+        const R p_new = exp_euler(st[4], p_inf(v), tau_p(v), dt);
+        const R q_new = exp_euler(st[5], q_inf(v), tau_q(v), dt);
+        const R cai = st[6];
+        const R r_new = exp_euler(st[7], r_inf(cai), R(2.0f), dt);
+        // Ca (cnexp): I_Ca from the old state
+        const R eca_old = -rt * M::ln(cai / R(2.0f)) / R(2.0f);
+        const R gcat_old = gcatbar * st[4] * st[4] * st[5];
+        const R ica_old = gcat_old * (v - eca_old);
+        const R j = -ica_old * area * R(1e-14f) / (R(2.0f) * F * vol);
+        const R k_ca = ca_gain * kca;
+        const R cai_decay = M::fast_exp(-dt * k_ca);
+        const R cai_new = cai + (R(1) - cai_decay) * (ca_gain * j / k_ca - cai);
+        const R eca = -rt * M::ln(cai_new / R(2.0f)) / R(2.0f);
+        const R g_na = gnabar * (m_new * m_new * m_new) * h_new;
+        const R n2 = n_new * n_new;
+        const R g_k = gkdrbar * (n2 * n2);
+        const R g_cat = gcatbar * p_new * p_new * q_new;
+        const R g_ahp = gkcabar * r_new * r_new;
+        const R g_sum = g_na + g_k + g_cat + g_ahp + gl;
+        const R g_e_sum = g_na * ena + (g_k + g_ahp) * ek + g_cat * eca + gl * el;
+        const R c_over_dt = R(1) / (inv_cm * dt);
+        const R v_new = (c_over_dt * v + I + g_e_sum) / (c_over_dt + g_sum);
+        // edge_spike, P/src/ops/spike.rs:70-74
+        const bool above = v_new >= v_th;
+        const int spike = (above && st[8] > R(0)) ? 1 : 0;
+        st[0] = v_new; st[1] = m_new; st[2] = h_new; st[3] = n_new; st[4] = p_new; st[5] = q_new;
+        st[6] = cai_new; st[7] = r_new; st[8] = above ? R(0) : R(1);
+        return spike;
+    }
+};
+
 static inline int n_state(int model) {
-    return model == ORC_MODEL_LIF ? 2 : model == ORC_MODEL_IZHIKEVICH ? 3 : 6;
+    return model == ORC_MODEL_LIF ? 2 : model == ORC_MODEL_IZHIKEVICH ? 3 : model == ORC_MODEL_HH ? 6 : 9;
 }
 static inline int n_param(int model) {
-    return model == ORC_MODEL_LIF ? 6 : model == ORC_MODEL_IZHIKEVICH ? 7 : 13;
+    return model == ORC_MODEL_LIF ? 6 : model == ORC_MODEL_IZHIKEVICH ? 7 : model == ORC_MODEL_HH ? 13 : 14;
 }
-static inline bool clamps(int model) {  // #[no_clamp] on HH (hh.rs:13)
-    return model != ORC_MODEL_HH;
+static inline bool clamps(int model) {  // #[no_clamp] on HH (hh.rs:13) and GPEA (gpea.rs:213)
+    return model == ORC_MODEL_LIF || model == ORC_MODEL_IZHIKEVICH;
 }
 
 // ---------------------------------------------------------------------------
@@ -578,11 +690,11 @@ struct Net final : NetBase {
                                     pr[2 * cap + i], pr[3 * cap + i], pr[4 * cap + i],
                                     pr[5 * cap + i], pr[6 * cap + i]);
             } else {
-                R stv[6], prv[13];
+                R stv[9], prv[14];
                 for (int k = 0; k < nst; ++k) stv[k] = s[static_cast<size_t>(k) * cap + i];
                 for (int k = 0; k < npar; ++k) prv[k] = pr[static_cast<size_t>(k) * cap + i];
                 prv[0] = dt;
-                spike = hh_step<R>(stv, prv, I);
+                spike = P.model == ORC_MODEL_HH ? hh_step<R>(stv, prv, I) : Gpea<R>::step(stv, prv, I);
                 for (int k = 0; k < nst; ++k) s[static_cast<size_t>(k) * cap + i] = stv[k];
             }
             spk[P.off + i] = spike;
@@ -849,6 +961,13 @@ float orc_tm_spike(float u0, float a_rec, float a_fac, int32_t is_fac, float* r,
     return tm_spike_r<float>(p, *r, *u, *last, weight, step);
 }
 float orc_powi_f32(float a, int32_t n) { return powi_r<float>(a, n); }
+float orc_fast_exp(float x) { return fast_exp_f32(x); }
+void orc_gpea_rates(float v, float cai, float* out) {
+    using G = Gpea<float>;
+    out[0] = G::m_inf(v); out[1] = G::h_inf(v); out[2] = G::n_inf(v); out[3] = G::p_inf(v);
+    out[4] = G::q_inf(v); out[5] = G::r_inf(cai); out[6] = G::tau_m(v); out[7] = G::tau_h(v);
+    out[8] = G::tau_n(v); out[9] = G::tau_p(v); out[10] = G::tau_q(v);
+}
 void orc_net_stdp_step(void* net, float decay, float lr_ltp, float lr_ltd, float w_min, float w_max) {
     static_cast<NetBase*>(net)->stdp_step(decay, lr_ltp, lr_ltd, w_min, w_max);
 }

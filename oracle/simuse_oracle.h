@@ -28,6 +28,7 @@ extern "C" {
 #define ORC_MODEL_LIF 0
 #define ORC_MODEL_IZHIKEVICH 1
 #define ORC_MODEL_HH 2
+#define ORC_MODEL_GPEA 3
 
 /* One population: column-major SoA with host stride ceil8(n)
  * (P/src/core/exec/population.rs:106-131). */
@@ -100,6 +101,11 @@ float orc_gauss(uint64_t seed, uint32_t i, uint64_t step);
 float orc_tm_spike(float u0, float a_rec, float a_fac, int32_t is_fac, float* r, float* u, uint32_t* last,
                    float weight, uint64_t step);
 float orc_powi_f32(float a, int32_t n);
+/* fast_exp (P/src/math/fast_exp.rs:21-63, the separately rounded Horner). */
+float orc_fast_exp(float x);
+/* GPEA steady states and time constants at (v, cai): out = m_inf, h_inf, n_inf,
+ * p_inf, q_inf, r_inf, tau_m, tau_h, tau_n, tau_p, tau_q. */
+void orc_gpea_rates(float v, float cai, float* out);
 float orc_cos2pi_f32(float u);
 float orc_exp_f32(float x);
 float orc_ln_f32(float x);

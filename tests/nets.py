@@ -7,7 +7,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from simuse.models import hh_pop, izh_pop, lif_pop
+from simuse.models import gpea_pop, hh_pop, izh_pop, lif_pop
 from simuse.workloads import NetSpec, Source, csr_from_triplets
 
 
@@ -80,6 +80,9 @@ def random_net(n, k_mean, seed, model="izh", max_delay=0, inh_frac=0.2, noise=5.
         dt = 0.1 if dt is None else dt
         v0 = rng.uniform(-65, -50, n).astype(np.float32)
         pop = lif_pop(n, dt, 0.05, -48.0, -50.0, -65.0, 2.0, v0)  # v_rest above threshold: tonic firing
+    elif model == "gpea":
+        dt = 0.025 if dt is None else dt
+        pop = gpea_pop(n, dt, v0=rng.uniform(-62, -58, n).astype(np.float32))
     else:
         dt = 0.01 if dt is None else dt
         pop = hh_pop(n, dt, v0=rng.uniform(-70, -60, n).astype(np.float32))

@@ -443,3 +443,34 @@ def test_izhikevich_1m_full_size_bitexact():
     assert dev.counters() == ora.counters()
     assert_same_state(dev, ora, spec)
     say("izhikevich_1m: ok", ora.counters())
+
+
+def test_gpea_population_bitwise():
+    # gpea.rs pulse protocol (rest, then a constant current) on 1024 neurons with currents
+    # from 0 to 3x the 0.5 nA pulse; spikes every step and the final 9 state columns bitwise
+    from simuse.models import GPEA_PULSE, gpea_pop
+    say("gpea_population")
+    n = 1024
+    spec = single_pop(gpea_pop(n), 0.025)
+    idx = np.arange(n)
+    cur = np.linspace(0.0, 3.0 * GPEA_PULSE, n).astype(np.float32)
+
+    def inj(dv, o, t):
+        if t >= 400:
+            dv.inject_current(0, idx, cur)
+            o.inject_current(0, idx, cur)
+
+    dev, ora = run_pair(spec, 2400, inject=inj)
+    assert ora.counters()[0] > 1000
+    assert dev.counters() == ora.counters()
+    assert_same_state(dev, ora, spec)
+
+
+def test_gpea_network_bitexact():
+    say("gpea_network")
+    spec = random_net(3000, 40, seed=61, model="gpea", max_delay=5, noise=60.0, sources=5)
+    dev, ora = run_pair(spec, 800, check_every=1)
+    assert ora.counters()[0] > 200
+    assert dev.counters() == ora.counters()
+    assert_same_state(dev, ora, spec)
+    assert_same_ring(dev, ora)

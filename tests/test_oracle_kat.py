@@ -372,6 +372,94 @@ def test_tm_network_depresses_repeated_drive():
     assert fired[1] > 20 and fired[2] == 0
 
 
+# --- GPEA: pillars/src/zoo/neurons/gpea.rs:430-611 (NEURON 9.0.2 values) -----------
+def test_fast_exp_accuracy():
+    xs = np.linspace(-87.0, 87.0, 40001).astype(np.float32)
+    got = np.array([po.fast_exp(float(x)) for x in xs])
+    want = np.exp(xs.astype(np.float64))
+    assert np.max(np.abs(got - want) / want) < 2e-5
+    assert po.fast_exp(0.0) == 1.0 and po.fast_exp(-200.0) == 0.0 and math.isinf(po.fast_exp(200.0))
+
+
+def test_gpea_rates_vs_neuron():
+    r = po.gpea_rates(-65.0, 1e-4)
+    want = [0.02069111227, 0.9546474194, 0.1620766655, 0.2069705933, 0.03082031845, 0.2942149722]
+    assert np.all(np.abs(r[:6] - want) < 1e-4)
+    assert np.all(np.abs(r[6:10] - [0.1009999964, 1.540321035, 1.098909804, 8.077265928]) < 1e-3)
+    assert abs(r[10] - 136.9174254) < 5e-2
+    r = po.gpea_rates(0.0, 1e-4)
+    assert abs(r[0] - 0.9956298205) < 1e-4 and abs(r[1] - 0.0008167381047) < 1e-4
+    assert abs(r[3] - 0.9997655911) < 1e-4 and abs(r[10] - 14.24726609) < 1e-2
+    r = po.gpea_rates(-80.0, 5e-4)
+    assert abs(r[0] - 0.002472623157) < 1e-4 and abs(r[1] - 0.9954615683) < 1e-4
+    assert abs(r[4] - 0.2969072608) < 1e-4 and abs(r[5] - 0.9840936083) < 1e-4
+    assert np.all(np.abs(r[7:10] - [0.6760660476, 0.925418813, 6.400175291]) < 1e-3)
+    assert abs(r[10] - 60.09475979) < 5e-2
+
+
+def _gpea_run(steps, current, n=8, kind=32, first_spike_after=None):
+    from simuse.models import gpea_pop
+    pop = gpea_pop(n)
+    rp, col, w, d = csr_from_triplets(n, [], [], np.zeros(0, np.float32), np.zeros(0, np.uint8))
+    o = po.OracleNetwork(NetSpec("gpea", [pop], rp, col, w, d, 0.025, 0), kind)
+    idx = np.arange(n)
+    total, first = 0, None
+    for k in range(steps):
+        if current:
+            o.inject_current(0, idx, np.full(n, current, np.float32))
+        o.step()
+        s = int(o.spikes().sum())
+        total += s
+        if s and first is None:
+            first = k
+    return o, total, first
+
+
+def test_gpea_resting_and_silence_match_neuron():
+    o, total, _ = _gpea_run(200, 0.0)
+    assert abs(o.state(0, 0)[0] - -61.72048753) < 0.05          # t = 5 ms
+    o, total, _ = _gpea_run(2000, 0.0)
+    assert abs(o.state(0, 0)[0] - -61.2831801) < 0.05           # t = 50 ms
+    o, total, _ = _gpea_run(8000, 0.0)
+    assert abs(o.state(0, 0)[0] - -61.04569455) < 0.05          # t = 200 ms
+    assert total == 0 and np.all(np.isfinite(o.state(0, 0)))
+
+
+def test_gpea_pulse_matches_neuron():
+    from simuse.models import GPEA_PULSE, gpea_pop
+    n = 8
+    pop = gpea_pop(n)
+    rp, col, w, d = csr_from_triplets(n, [], [], np.zeros(0, np.float32), np.zeros(0, np.uint8))
+    o = po.OracleNetwork(NetSpec("gpea", [pop], rp, col, w, d, 0.025, 0))
+    o.step(2000)                                                # 0..50 ms at rest
+    idx = np.arange(n)
+    first, total = None, 0
+    for k in range(4000):                                       # 50..150 ms, 0.5 nA
+        o.inject_current(0, idx, np.full(n, GPEA_PULSE, np.float32))
+        o.step()
+        s = int(o.spikes().sum())
+        total += s
+        if s and first is None:
+            first = 50.0 + (k + 0.5) * 0.025
+    assert first is not None and abs(first - 50.8) < 0.5       # NEURON: 50.8 ms
+    assert abs(total / n - 37.0) <= 3.0                         # NEURON: 37 spikes
+
+
+def test_gpea_stress_strong_input():
+    o, total, _ = _gpea_run(8000, 500.0)                        # 0.5 mA/cm^2, 200 ms
+    v = o.state(0, 0)
+    assert np.all(np.isfinite(v)) and np.all(np.abs(v) < 200.0)
+    assert abs(total / 8 - 220.0) <= 22.0                       # NEURON: 220 spikes
+
+
+def test_gpea_fp32_tracks_fp64_truth():
+    # the FP64 twin uses exact exp/ln; spike counts under the pulse agree
+    from simuse.models import GPEA_PULSE
+    _, t32, f32 = _gpea_run(4000, GPEA_PULSE)
+    _, t64, f64 = _gpea_run(4000, GPEA_PULSE, kind=64)
+    assert abs(t32 - t64) <= 8 and abs(f32 - f64) <= 2
+
+
 # --- dense STDP of CompiledNetwork::stdp_step (pillars-cuda/src/network.rs:53-83) -------
 def _dense_run(k, pre_first, decay=0.9, lr_ltp=0.01, lr_ltd=0.02, w0=0.5, w_max=1.0):
     dt = 0.5
