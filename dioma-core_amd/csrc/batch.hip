@@ -76,6 +76,21 @@ __global__ void gen_izh_kernel(float* st, float* pr, uint32_t n, uint32_t dstrid
     st[2 * static_cast<size_t>(dstride) + i] = 0.0f;
 }
 
+// Hodgkin-Huxley: v0 ~ U(-70, -60), gates at the standard resting values
+// m 0.0529, h 0.596, n 0.318 (CN/src/sim/hh.rs:34-38), refr 0, CaT gate 0.
+__global__ void gen_hh_state_kernel(float* st, uint32_t n, uint32_t dstride, uint64_t seed) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float u = unit24(splitmix(seed ^ (static_cast<uint64_t>(i) * 0xA24BAED4963EE407ull)));
+    const size_t ds = dstride;
+    st[i] = -70.0f + 10.0f * u;
+    st[ds + i] = 0.0529f;
+    st[2 * ds + i] = 0.596f;
+    st[3 * ds + i] = 0.318f;
+    st[4 * ds + i] = 0.0f;
+    st[5 * ds + i] = 0.0f;
+}
+
 }  // namespace
 
 // Build a generated network in place (replaces NetState::init for it).
@@ -86,7 +101,7 @@ int build_generated(NetState& s, const simuse_gen_desc& g, hipStream_t stream) {
     // populations: LIF one population; Izhikevich RS then FS
     struct Spec { uint32_t n; int fs; };
     std::vector<Spec> specs;
-    if (g.model == SIMUSE_MODEL_LIF) specs.push_back({n, 0});
+    if (g.model == SIMUSE_MODEL_LIF || g.model == SIMUSE_MODEL_HH) specs.push_back({n, 0});
     else {
         if (ne) specs.push_back({ne, 0});
         if (n - ne) specs.push_back({n - ne, 1});
@@ -114,6 +129,12 @@ int build_generated(NetState& s, const simuse_gen_desc& g, hipStream_t stream) {
             const float vals[6] = {g.dt, 1.0f / 20.0f, -65.0f, -50.0f, -65.0f, 2.0f};
             for (int k = 0; k < 6; ++k) P.puni[k] = vals[k];
             P.uni_mask = 0x3Fu;
+        } else if (g.model == SIMUSE_MODEL_HH) {
+            // standard HH (CN/src/sim/hh.rs:26-31), threshold 0 mV, tau_ref 2 ms
+            const float vals[13] = {g.dt, 120.0f, 36.0f, 0.3f, 50.0f, -77.0f, -65.0f, 1.0f, 1.0f, 0.0f,
+                                    0.0f, 120.0f, 2.0f};
+            for (int k = 0; k < 13; ++k) P.puni[k] = vals[k];
+            P.uni_mask = 0x1FFFu;
         } else if (!sp.fs) {
             const float vals[7] = {g.dt, 0.02f, 0.2f, 0.0f, 0.0f, 0.0f, 30.0f};
             for (int k = 0; k < 7; ++k) P.puni[k] = vals[k];
@@ -151,6 +172,8 @@ int build_generated(NetState& s, const simuse_gen_desc& g, hipStream_t stream) {
         const uint64_t pseed = g.seed * 0xD1B54A32D192ED03ull + 1000003ull * (p + 1);
         if (g.model == SIMUSE_MODEL_LIF)
             gen_lif_state_kernel<<<(P.n + 255) / 256, 256, 0, stream>>>(P.st, P.n, P.dstride, pseed);
+        else if (g.model == SIMUSE_MODEL_HH)
+            gen_hh_state_kernel<<<(P.n + 255) / 256, 256, 0, stream>>>(P.st, P.n, P.dstride, pseed);
         else
             gen_izh_kernel<<<(P.n + 255) / 256, 256, 0, stream>>>(P.st, P.pr, P.n, P.dstride, pseed,
                                                                   specs[p].fs);
@@ -189,8 +212,8 @@ int build_generated(NetState& s, const simuse_gen_desc& g, hipStream_t stream) {
 
 int validate_gen(const simuse_gen_desc* g) {
     if (!g) return fail(SIMUSE_E_INVALID, "null generator description");
-    if (g->model != SIMUSE_MODEL_LIF && g->model != SIMUSE_MODEL_IZHIKEVICH)
-        return fail(SIMUSE_E_UNSUPPORTED, "generator supports LIF and Izhikevich networks");
+    if (g->model != SIMUSE_MODEL_LIF && g->model != SIMUSE_MODEL_IZHIKEVICH && g->model != SIMUSE_MODEL_HH)
+        return fail(SIMUSE_E_UNSUPPORTED, "generator supports LIF, Izhikevich and HH networks");
     if (g->n < 2 || g->n > kMaxNeurons) return fail(SIMUSE_E_INVALID, "generator: n must be in [2, 2^27]");
     if (static_cast<uint64_t>(g->n) * g->k >= 0xFFFFFFFFull)
         return fail(SIMUSE_E_INVALID, "generator: n*k exceeds the u32 CSR");
@@ -208,9 +231,10 @@ uint64_t generated_bytes(const simuse_gen_desc& g) {
     const uint64_t n = g.n, ns = n * g.k;
     const uint64_t tiles = (n + kTile - 1) / kTile + 2;
     uint64_t b = ns * (4 + 4 + 1);                           // col, w, delay
-    b += ns * (8 + 4);                                       // event records, collision lists
+    b += ns * (8 + 8);                                       // event records, collision lists
     b += n * 4 * 2 + n * 32 * (4 + 8);                       // row_ptr, ring, tagged heads
-    b += n * 4 * 10 + tiles * (kTile * 4 + 4 + 8 + 8 + 8);   // state/params, compaction, counters
+    const uint64_t cols = static_cast<uint64_t>(model_states(g.model) + model_params(g.model));
+    b += n * 4 * cols + tiles * (kTile * 4 + 4 + 8 + 8 + 8);  // state/params, compaction, counters
     return b + (1u << 20);
 }
 

@@ -246,8 +246,9 @@ int simuse_server_submit(simuse_server* s, const simuse_request* r, uint64_t* id
     Req q;
     q.r = *r;
     if (r->kind == SIMUSE_REQ_NETWORK) {
-        if (r->net.model != SIMUSE_MODEL_LIF && r->net.model != SIMUSE_MODEL_IZHIKEVICH)
-            return fail(SIMUSE_E_UNSUPPORTED, "server: network requests are LIF or Izhikevich");
+        if (r->net.model != SIMUSE_MODEL_LIF && r->net.model != SIMUSE_MODEL_IZHIKEVICH &&
+            r->net.model != SIMUSE_MODEL_HH)
+            return fail(SIMUSE_E_UNSUPPORTED, "server: network requests are LIF, Izhikevich or HH");
         q.bytes = simuse_gen_bytes(&r->net);
     } else if (r->kind == SIMUSE_REQ_KURAMOTO) {
         q.bytes = kura_bytes(r->kura);

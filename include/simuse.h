@@ -229,10 +229,12 @@ typedef struct simuse_batch simuse_batch;
  *      refractory 2 ms, v0 ~ U(-65,-50).
  * Izhikevich: RS population (exc) and FS population (inh) with the 2003
  *      randomisation, v_th 30, no refractory period, v0 -65, u0 = b v0.
+ * HH: one population, standard Na/K/leak (CN/src/sim/hh.rs:26-38), v_th 0,
+ *      refractory 2 ms, v0 ~ U(-70,-60), resting gates.
  * Optional per-neuron Poisson drive (simuse_net_set_poisson_drive semantics)
  * and Gaussian current noise. */
 typedef struct simuse_gen_desc {
-    int32_t model; /* SIMUSE_MODEL_LIF or SIMUSE_MODEL_IZHIKEVICH */
+    int32_t model; /* SIMUSE_MODEL_LIF, SIMUSE_MODEL_IZHIKEVICH or SIMUSE_MODEL_HH */
     uint32_t n;
     uint32_t k;
     uint64_t seed;

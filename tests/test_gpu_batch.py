@@ -5,7 +5,7 @@ import pytest
 
 import pyoracle as po
 from nets import random_net
-from simuse import MODEL_IZHIKEVICH, MODEL_LIF
+from simuse import MODEL_HH, MODEL_IZHIKEVICH, MODEL_LIF
 from simuse.batch import Batch, gen_desc
 from simuse.workloads import NetSpec, Source, build_device
 
@@ -40,23 +40,27 @@ def spec_from_generated(b, nid, g, steps):
     return spec
 
 
-@pytest.mark.parametrize("model", [MODEL_IZHIKEVICH, MODEL_LIF])
+@pytest.mark.parametrize("model", [MODEL_IZHIKEVICH, MODEL_LIF, MODEL_HH])
 def test_generated_network_matches_oracle(model):
     print("generated", model, flush=True)
     if model == MODEL_IZHIKEVICH:
         g = gen_desc(model, 3000, 60, seed=11, dt=1.0, w_exc=0.5, w_inh=1.0, drive_rate_hz=40.0,
                      drive_weight=8.0, noise_sigma=3.0)
+    elif model == MODEL_HH:
+        g = gen_desc(model, 2000, 40, seed=13, dt=0.01, w_exc=2.0, w_inh=6.0, d_min=1, d_max=4,
+                     drive_rate_hz=2000.0, drive_weight=40.0, noise_sigma=30.0)
     else:
         g = gen_desc(model, 3000, 60, seed=12, dt=0.1, w_exc=20.0, w_inh=60.0, d_min=1, d_max=15,
                      drive_rate_hz=3000.0, drive_weight=60.0, noise_sigma=0.0)
+    T = 1500 if model == MODEL_HH else 300
     b = Batch()
     nid = b.add_generated(g)
-    spec = spec_from_generated(b, nid, g, 300)
+    spec = spec_from_generated(b, nid, g, T)
     o = po.OracleNetwork(spec, 32)
-    b.record(nid, 300)
+    b.record(nid, T)
     o.record(True)
-    b.run(300)
-    o.step(300)
+    b.run(T)
+    o.step(T)
     bits = b.download_raster_bits(nid)
     ras = o.raster()
     assert ras.sum() > 100
