@@ -65,6 +65,115 @@ def cpu_baseline(spec, steps: int, threads: int):
     return spec.n_total * steps / dt, dt
 
 
+def bench_batching(args) -> dict:
+    """Batching configuration: 512 generated networks (log-uniform 1k-200k
+    neurons, LIF/Izhikevich, 100 synapses each, Poisson drive U(1,10) kHz)
+    stepped together for `batch_steps` steps in 16-step graphs, against plain
+    per-network runs (one network per engine, same graphs) on a random sample.
+    networks/s = networks simulated for the full horizon per second of stepping
+    (network construction excluded on both sides)."""
+    import simuse
+    from simuse.batch import Batch
+    from simuse.workloads import batching_mix
+    descs = batching_mix(args.batch_networks, seed=1)
+    T = args.batch_steps
+    log(f"batching: {len(descs)} networks, sum N {sum(d.n for d in descs)}, {T} steps")
+    t0 = time.perf_counter()
+    b = Batch()
+    ids = [b.add_generated(d) for d in descs]
+    b.set_graph_steps(16)
+    b.run(1)  # finalize every network
+    built = time.perf_counter() - t0
+    mode = b.autotune(16)
+    b.run(16)
+    c0 = [b.counters(i) for i in ids]
+    b.sync()
+    t0 = time.perf_counter()
+    b.step_async(T)
+    b.sync()
+    wall = time.perf_counter() - t0
+    c1 = [b.counters(i) for i in ids]
+    ev = sum(y[1] - x[1] for x, y in zip(c0, c1))
+    nsteps = sum(d.n for d in descs) * T
+    b.close()
+    batched = {"networks": len(descs), "steps": T, "wall_s": wall, "networks_per_s": len(descs) / wall,
+               "neuron_steps_per_s": nsteps / wall, "synaptic_events_per_s": ev / wall,
+               "scatter": "tile" if mode == simuse.SCATTER_TILE else "wave", "build_s": built}
+    log(f"  batched: {wall:.3f} s for {T} steps, {len(descs) / wall:.1f} networks/s")
+    rng = np.random.default_rng(2)
+    sample = rng.choice(len(descs), size=min(args.plain_sample, len(descs)), replace=False)
+    plain_wall = 0.0
+    plain_nsteps = 0
+    for i in sample:
+        d = descs[int(i)]
+        pb = Batch()
+        nid = pb.add_generated(d)
+        pb.set_graph_steps(16)
+        pb.run(17)
+        t0 = time.perf_counter()
+        pb.step_async(T)
+        pb.sync()
+        plain_wall += time.perf_counter() - t0
+        plain_nsteps += d.n * T
+        pb.close()
+    plain = {"sample_networks": int(len(sample)), "wall_s": plain_wall,
+             "networks_per_s": len(sample) / plain_wall, "neuron_steps_per_s": plain_nsteps / plain_wall}
+    log(f"  plain: {len(sample)} networks one at a time in {plain_wall:.3f} s, "
+        f"{len(sample) / plain_wall:.1f} networks/s")
+    return {"workload": f"{len(descs)} generated networks, 1k-200k neurons log-uniform, LIF/Izhikevich 50/50, "
+                        f"100 syn/neuron, Poisson drive U(1,10) kHz, {T} steps",
+            "batched": batched, "plain": plain,
+            "batched_vs_plain": batched["networks_per_s"] / plain["networks_per_s"]}
+
+
+def bench_serving(args) -> dict:
+    """Serving configuration: Poisson arrivals (200/s) of LIF/Izhikevich/HH
+    networks (5k-500k neurons) and Kuramoto graphs (10k-1M oscillators, RK4),
+    horizons 500-20,000 steps, admitted by an HBM budget and stepped together
+    (continuous batching).  Latency = completion - submission."""
+    from simuse.server import DONE, Server
+    from simuse.workloads import serving_mix
+    reqs = serving_mix(args.serve_requests, rate_hz=args.serve_rate, seed=3)
+    log(f"serving: {len(reqs)} requests at {args.serve_rate}/s, budget {args.serve_budget_gb} GB")
+    srv = Server(int(args.serve_budget_gb * (1 << 30)), chunk_steps=16)
+    ids = []
+    t0 = time.perf_counter()
+    for t, r in reqs:
+        now = time.perf_counter() - t0
+        if t > now:
+            time.sleep(t - now)
+        ids.append(srv.submit(r))
+    last = time.perf_counter()
+    while True:
+        st = srv.stats()
+        if st.completed >= st.submitted or time.perf_counter() - t0 > args.serve_timeout:
+            break
+        time.sleep(0.05)
+        if time.perf_counter() - last > 5.0:
+            log(f"  t={time.perf_counter() - t0:.1f}s completed {st.completed}/{st.submitted}, live {st.live}, "
+                f"queued {st.queued}")
+            last = time.perf_counter()
+    srv.drain(5.0)
+    wall = time.perf_counter() - t0
+    res = [srv.result(i) for i in ids]
+    st = srv.stats()
+    srv.close()
+    done = [x for x in res if x.status == DONE]
+    lat = np.array([x.t_done - x.t_submit for x in done])
+    wait = np.array([x.t_admit - x.t_submit for x in done])
+    span = max(x.t_done for x in done) - min(x.t_submit for x in res) if done else float("nan")
+    out = {"workload": f"{len(reqs)} requests, Poisson arrivals {args.serve_rate}/s, LIF/Izh/HH 5k-500k neurons "
+                       "or Kuramoto 10k-1M (RK4), horizons 500-20000 steps",
+           "completed": len(done), "rejected_or_failed": len(res) - len(done),
+           "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+           "mean_admission_wait_s": float(wait.mean()), "networks_per_s": len(done) / span,
+           "wall_s": wall, "peak_live": int(st.peak_live), "peak_bytes": int(st.peak_bytes),
+           "budget_bytes": int(args.serve_budget_gb * (1 << 30))}
+    log(f"  {len(done)} done, p50 {out['p50_latency_s']:.3f} s, p99 {out['p99_latency_s']:.3f} s, "
+        f"{out['networks_per_s']:.1f} networks/s, peak live {st.peak_live}")
+    return out
+
+
 def copy_calibration(mib: int = 1024) -> int:
     """Device-to-device copy of a known size (the HIP runtime's copy kernel),
     used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes."""
@@ -98,6 +207,15 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scatter", choices=["auto", "tile", "wave"], default="auto")
+    ap.add_argument("--no-batching", action="store_true", help="skip the batched-vs-plain leg")
+    ap.add_argument("--batch-networks", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=2000)
+    ap.add_argument("--plain-sample", type=int, default=64)
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=160.0)
+    ap.add_argument("--serve-timeout", type=float, default=300.0)
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
@@ -215,8 +333,12 @@ def main() -> int:
         "step_frac_of_peak": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
     net.close()
+    if not args.no_batching:
+        out["batching"] = bench_batching(args)
+    if not args.no_serving:
+        out["serving"] = bench_serving(args)
+    print(json.dumps(out), flush=True)
     return 0
 
 

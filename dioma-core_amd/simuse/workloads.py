@@ -276,3 +276,73 @@ def modular_small_world(n: int, modules: int = 64, k: int = 20, beta: float = 0.
     omega = rng.normal(size=n).astype(np.float32)
     theta0 = rng.uniform(-np.pi, np.pi, n).astype(np.float32)
     return rp.astype(np.uint32), b.astype(np.uint32), w, omega, theta0
+
+
+def _log_uniform(rng: np.random.Generator, lo: float, hi: float) -> int:
+    return int(round(float(np.exp(rng.uniform(np.log(lo), np.log(hi))))))
+
+
+def _net_seed(master: int, k: int) -> int:
+    z = (master * 0x9E3779B97F4A7C15 + (k + 1) * 0xD1B54A32D192ED03) & ((1 << 64) - 1)
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & ((1 << 64) - 1)
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & ((1 << 64) - 1)
+    return z ^ (z >> 31)
+
+
+def _gen_network(model: int, n: int, seed: int, rate_hz: float, k: int = 100):
+    """One generated network of the batching and serving mixes: LIF (dt 0.1 ms,
+    Brunel-style currents J*tau/dt: +20 / -100, delays 1..15), Izhikevich (dt 1 ms,
+    exc U(0,0.5) / inh U(-1,0), delay 0, thalamic noise sd 5) or HH (dt 0.01 ms,
+    delays 1..4, noise sd 30); every neuron has its own Poisson drive at rate_hz."""
+    from . import _lib
+    from .batch import gen_desc
+    if model == _lib.MODEL_LIF:
+        return gen_desc(model, n, k, seed, 0.1, w_exc=20.0, w_inh=100.0, d_min=1, d_max=15,
+                        drive_rate_hz=rate_hz, drive_weight=20.0)
+    if model == _lib.MODEL_IZHIKEVICH:
+        return gen_desc(model, n, k, seed, 1.0, w_exc=0.5, w_inh=1.0, d_min=0, d_max=0,
+                        drive_rate_hz=rate_hz, drive_weight=2.0, noise_sigma=5.0)
+    return gen_desc(model, n, k, seed, 0.01, w_exc=2.0, w_inh=6.0, d_min=1, d_max=4,
+                    drive_rate_hz=rate_hz, drive_weight=40.0, noise_sigma=30.0)
+
+
+def batching_mix(count: int = 512, seed: int = 1, lo: int = 1_000, hi: int = 200_000, k: int = 100):
+    """The batching configuration: `count` independent networks with sizes
+    log-uniform in [lo, hi], model 50/50 LIF/Izhikevich, k synapses per neuron,
+    per-network seed hash(seed, index) and Poisson drive rate U(1, 10) kHz."""
+    from . import _lib
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = _log_uniform(rng, lo, hi)
+        model = _lib.MODEL_LIF if rng.random() < 0.5 else _lib.MODEL_IZHIKEVICH
+        out.append(_gen_network(model, n, _net_seed(seed, i), float(rng.uniform(1000.0, 10000.0)), k))
+    return out
+
+
+def serving_mix(count: int = 64, rate_hz: float = 200.0, seed: int = 1):
+    """The serving configuration: Poisson arrivals at rate_hz; each request is a
+    LIF, Izhikevich or HH network (5k-500k neurons, log-uniform) or a Kuramoto
+    graph (10k-1M oscillators, 64 modules, Watts-Strogatz k=20, beta 0.1, 5%
+    inter-module edges, K=2, RK4 at dt 1 ms), horizon U{500..20000} steps.
+    Returns [(arrival time s, Request)]."""
+    from . import _lib
+    from .server import kuramoto_request, network_request
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += float(rng.exponential(1.0 / rate_hz))
+        kind = int(rng.integers(0, 4))
+        horizon = int(rng.integers(500, 20001))
+        s = _net_seed(seed + 7, i)
+        if kind == 3:
+            n = _log_uniform(rng, 1e4, 1e6)
+            out.append((t, kuramoto_request(n, modules=64, k=20, beta=0.1, inter=0.05, coupling=2.0, dt=0.001,
+                                            seed=s, horizon=horizon)))
+        else:
+            model = (_lib.MODEL_LIF, _lib.MODEL_IZHIKEVICH, _lib.MODEL_HH)[kind]
+            n = _log_uniform(rng, 5e3, 5e5)
+            out.append((t, network_request(_gen_network(model, n, s, float(rng.uniform(1000.0, 10000.0))),
+                                           horizon)))
+    return out
