@@ -37,6 +37,8 @@ void launch_dense_weights(const NetArgs* nets, uint32_t N, const float* trace, f
 void launch_stdp(const NetArgs* nets, const uint2* tiles, uint32_t n, hipStream_t s);
 void launch_insert(int sched, const NetArgs* nets, const uint2* blocks, uint32_t n, hipStream_t s);
 void launch_fixup(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s);
+void launch_bin(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s);
+void launch_bin_apply(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s);
 void launch_head_clear(unsigned long long* head, const uint64_t* step, size_t n, hipStream_t s);
 void launch_inject(float* ring, const uint64_t* step, uint32_t N, const uint32_t* idx, const float* val,
                    uint32_t count, int serial, hipStream_t s);
@@ -112,6 +114,12 @@ struct NetState {
     uint32_t* tile_ebase = nullptr;
     uint32_t n_fix = 1;
     uint32_t max_delay = 0;    // largest synapse or source delay (steps)
+    // binned scatter (every delay <= 1 step): bins of kBinPosts posts
+    bool binned = false;
+    uint32_t n_bins = 0, n_bblk = 0, bblk_tiles = 0;
+    uint32_t* bin_off = nullptr;
+    uint4* bin_ev = nullptr;
+    uint32_t* bin_slot = nullptr;
     uint64_t since_clear = 0;  // steps enqueued since the heads were last rewritten
     uint32_t* row_ptr = nullptr;
     uint32_t* col = nullptr;
@@ -202,6 +210,12 @@ struct NetState {
         a.step_limit = step_limit;
         a.ring = ring;
         a.ring_fresh = max_delay <= 1 ? 1 : 0;
+        a.n_bins = binned ? n_bins : 0u;
+        a.n_bblk = n_bblk;
+        a.bblk_tiles = bblk_tiles;
+        a.bin_off = bin_off;
+        a.bin_ev = bin_ev;
+        a.bin_slot = bin_slot;
         a.head = head;
         a.rec = rec;
         a.coll = coll;
@@ -258,7 +272,7 @@ struct NetState {
     int upload_pop_table();
     int reset_dynamic(hipStream_t stream);
     void clear_heads(hipStream_t stream) {
-        launch_head_clear(head, step, static_cast<size_t>(kSlots) * N, stream);
+        if (head) launch_head_clear(head, step, static_cast<size_t>(kSlots) * N, stream);
         since_clear = 0;
     }
     int add_source(float rate_hz, float dt_ms, uint32_t seed, const uint32_t* posts, const float* wf,
@@ -298,6 +312,10 @@ struct Engine {
     uint32_t n_stdp = 0, cap_stdp = 0;
     uint2* d_fix = nullptr;
     uint32_t n_fix = 0, cap_fix = 0;
+    uint2* d_bin = nullptr;     // bin blocks of binned networks
+    uint32_t n_bin = 0, cap_bin = 0;
+    uint2* d_apply = nullptr;   // bins of binned networks
+    uint32_t n_apply = 0, cap_apply = 0;
     bool direct_fix = false;
     // execution
     int sched = SIMUSE_SCATTER_TILE;
@@ -305,7 +323,7 @@ struct Engine {
     uint32_t graph_steps = 8;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    static constexpr int kPhases = 4;  // neuron (+ fused scatter), stdp, insert, fix-up
+    static constexpr int kPhases = 4;  // neuron (+ fused scatter), stdp, insert + bin, apply + fix-up
     bool profiling = false;
     std::vector<hipEvent_t> ev;
     uint32_t prof_steps = 0;
@@ -320,7 +338,7 @@ struct Engine {
         for (uint4* t : d_tiles)
             if (t) (void)hipFree(t);
         for (void* p : {static_cast<void*>(d_nets), static_cast<void*>(d_blocks), static_cast<void*>(d_stdp),
-                        static_cast<void*>(d_fix)})
+                        static_cast<void*>(d_fix), static_cast<void*>(d_bin), static_cast<void*>(d_apply)})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -361,7 +379,7 @@ struct Engine {
         HIP_TRY(hipStreamSynchronize(stream));
         std::vector<NetArgs> na;
         std::vector<uint4> tl[kModels];
-        std::vector<uint2> bl, sl, fl;
+        std::vector<uint2> bl, sl, fl, binl, apl;
         for (uint32_t k = 0; k < nets.size(); ++k) {
             NetState* s = nets[k];
             na.push_back(s->args());
@@ -370,11 +388,17 @@ struct Engine {
                 for (uint32_t t = 0; t < P.ntiles; t += group)
                     tl[P.model].push_back(make_uint4(k, P.tile0 + t, std::min(group, P.ntiles - t), p));
             }
-            // plastic networks scatter after the STDP kernels; sources always
-            for (uint32_t b = s->stdp ? 0 : s->n_tiles; b < s->n_tiles + s->n_src_tiles; ++b)
-                bl.push_back(make_uint2(k, b));
             if (s->stdp)
                 for (uint32_t t = 0; t < s->n_tiles; ++t) sl.push_back(make_uint2(k, t));
+            if (s->binned) {
+                for (uint32_t j = 0; j < s->n_bblk; ++j) binl.push_back(make_uint2(k, j));
+                for (uint32_t j = 0; j < s->n_bins; ++j) apl.push_back(make_uint2(k, j));
+                continue;
+            }
+            // chained scatter: plastic networks scatter after the STDP kernels;
+            // sources always
+            for (uint32_t b = s->stdp ? 0 : s->n_tiles; b < s->n_tiles + s->n_src_tiles; ++b)
+                bl.push_back(make_uint2(k, b));
             for (uint32_t j = 0; j < s->n_fix; ++j) fl.push_back(make_uint2(k, j));
         }
         direct_fix = nets.size() == 1;  // one network: fix-up block j is blockIdx.x
@@ -404,6 +428,16 @@ struct Engine {
             if ((rc = grow(&d_fix, &cap_fix, n_fix))) return rc;
             HIP_TRY(hipMemcpy(d_fix, fl.data(), sizeof(uint2) * fl.size(), hipMemcpyHostToDevice));
         }
+        n_bin = static_cast<uint32_t>(binl.size());
+        if (n_bin) {
+            if ((rc = grow(&d_bin, &cap_bin, n_bin))) return rc;
+            HIP_TRY(hipMemcpy(d_bin, binl.data(), sizeof(uint2) * binl.size(), hipMemcpyHostToDevice));
+        }
+        n_apply = static_cast<uint32_t>(apl.size());
+        if (n_apply) {
+            if ((rc = grow(&d_apply, &cap_apply, n_apply))) return rc;
+            HIP_TRY(hipMemcpy(d_apply, apl.data(), sizeof(uint2) * apl.size(), hipMemcpyHostToDevice));
+        }
         drop_graph();
         dirty = false;
         return SIMUSE_OK;
@@ -416,7 +450,9 @@ struct Engine {
         launch_stdp(d_nets, d_stdp, n_stdp, stream);
         if (e) (void)hipEventRecord(e[2], stream);
         launch_insert(sched, d_nets, d_blocks, n_blocks, stream);
+        launch_bin(d_nets, d_bin, n_bin, stream);
         if (e) (void)hipEventRecord(e[3], stream);
+        launch_bin_apply(d_nets, d_apply, n_apply, stream);
         launch_fixup(d_nets, direct_fix ? nullptr : d_fix, n_fix, stream);
         if (e) (void)hipEventRecord(e[4], stream);
     }

@@ -124,7 +124,6 @@ int NetState::init(const simuse_pop_desc* pd, int32_t n_pops, const uint32_t* rp
     max_delay = 0;
     for (uint64_t e = 0; e < nsyn; ++e) max_delay = std::max<uint32_t>(max_delay, dl[e]);
     if ((rc = alloc(&ring, static_cast<size_t>(kSlots) * N))) return rc;
-    if ((rc = alloc(&head, static_cast<size_t>(kSlots) * N))) return rc;
     if ((rc = alloc(&tile_count, n_tiles))) return rc;
     if ((rc = alloc(&active, static_cast<size_t>(n_tiles) * kTile))) return rc;
     if ((rc = alloc(&bits, n_words))) return rc;
@@ -157,8 +156,22 @@ int NetState::finalize(hipStream_t stream) {
     int rc;
     const uint64_t n_ev = n_syn + n_src_edges;
     const uint32_t n_blk = n_tiles + n_src_tiles;
-    if ((rc = alloc(&rec, n_ev))) return rc;
-    if ((rc = alloc(&coll, n_ev))) return rc;
+    // Every delay <= 1 step (synapses and sources): the binned scatter;
+    // otherwise the chained one with its tagged heads and event records.
+    binned = max_delay <= 1 && N <= kBinPosts * kMaxBins && n_blk > 0 && n_blk <= kMaxBinBlocks * kTile;
+    if (binned) {
+        n_bins = (N + kBinPosts - 1) / kBinPosts;
+        bblk_tiles = (n_blk + kMaxBinBlocks - 1) / kMaxBinBlocks;
+        n_bblk = (n_blk + bblk_tiles - 1) / bblk_tiles;
+        if ((rc = alloc(&bin_ev, n_ev))) return rc;
+        if ((rc = alloc(&bin_off, static_cast<size_t>(n_bblk) * (n_bins + 1)))) return rc;
+        if ((rc = alloc(&bin_slot, 1))) return rc;
+    } else {
+        if ((rc = alloc(&head, static_cast<size_t>(kSlots) * N))) return rc;
+        if ((rc = alloc(&rec, n_ev))) return rc;
+        if ((rc = alloc(&coll, n_ev))) return rc;
+        clear_heads(stream);
+    }
     if ((rc = alloc(&tile_coll, n_blk))) return rc;
     if ((rc = alloc(&tile_ebase, n_blk))) return rc;
     HIP_TRY(hipMemsetAsync(tile_coll, 0, sizeof(uint32_t) * n_blk, stream));
@@ -178,7 +191,7 @@ int NetState::finalize(hipStream_t stream) {
         if ((rc = alloc(&tm_r, n_ev))) return rc;
         if ((rc = alloc(&tm_u, n_ev))) return rc;
         if ((rc = alloc(&tm_last, n_ev))) return rc;
-        if ((rc = alloc(&ev_w, n_ev))) return rc;
+        if (!binned && (rc = alloc(&ev_w, n_ev))) return rc;  // the fix-up re-reads delivered weights
         if ((rc = alloc(&tm_prof, h_tm_prof.size()))) return rc;
         HIP_TRY(hipMemcpy(tm_type, h_tm_type.data(), n_ev, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(tm_prof, h_tm_prof.data(), sizeof(float4) * h_tm_prof.size(), hipMemcpyHostToDevice));
@@ -989,8 +1002,13 @@ int simuse_net_stdp_step(simuse_net* net, float trace_decay, float lr_ltp, float
     for (int m = 0; m < Engine::kModels; ++m)
         launch_neuron(m, e.sched, e.d_nets, e.d_tiles[m], e.n_tiles_m[m], e.stream, 1u);
     launch_dense_weights(e.d_nets, s.N, s.dtrace, lr_ltp, lr_ltd, w_min, w_max, e.stream);
-    launch_insert(e.sched, e.d_nets, s.d_all_blocks, n_blk, e.stream);
-    launch_fixup(e.d_nets, nullptr, s.n_fix, e.stream);
+    if (s.binned) {
+        launch_bin(e.d_nets, e.d_bin, e.n_bin, e.stream);
+        launch_bin_apply(e.d_nets, e.d_apply, e.n_apply, e.stream);
+    } else {
+        launch_insert(e.sched, e.d_nets, s.d_all_blocks, n_blk, e.stream);
+        launch_fixup(e.d_nets, nullptr, s.n_fix, e.stream);
+    }
     HIP_TRY(hipGetLastError());
     ++s.host_step;
     return SIMUSE_OK;

@@ -446,7 +446,7 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
     const NetArgs& a = nets[it.x];
     const PopDev& P = a.pops[it.w];
     const uint32_t tile0 = it.y, ntile = it.z;
-    const bool fused = a.stdp == 0 && !defer_scatter;
+    const bool fused = a.stdp == 0 && !defer_scatter && a.n_bins == 0;
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     const uint32_t li0 = (tile0 - P.tile0) * kTile + t;
 
@@ -586,19 +586,16 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
     if (t == 0) a.tile_coll[tile0] = s_ncoll;
 }
 
-// Scatter of the neuron tiles of networks with plasticity (after the STDP
-// kernels) and of every Poisson-source tile.  A source tile rolls its up to
-// kTile sources for this step (P/src/api/net/network/step.rs:166-185).
-template <int SCHED>
-__global__ __launch_bounds__(kTile) void insert_kernel(const NetArgs* __restrict__ nets,
-                                                      const uint2* __restrict__ blocks) {
-    __shared__ uint32_t s_rs[kTile], s_pref[kTile], s_wave[kWavesPerTile], s_c[kWavesPerTile];
-    __shared__ uint32_t s_ncoll;
-    const uint2 ref = blocks[blockIdx.x];
-    const NetArgs& a = nets[ref.x];
-    const uint32_t b = ref.y, t = threadIdx.x;
-    const uint64_t step = *a.step;
-    if (step >= a.step_limit) return;
+// Stage the spiking rows of scatter tile b in LDS: a neuron tile's ascending
+// spiking ids (from the neuron kernel), or the sources of a source tile that
+// fire this step (P/src/api/net/network/step.rs:166-185).  Row k starts at
+// edge s_rs[k] (of the tile's own CSR), s_pref[k] is its first edge in the
+// concatenation.  Returns the number of rows; *total gets the edge count.
+// Every thread of the block must call it (it synchronises).
+__device__ __forceinline__ uint32_t stage_tile_rows(const NetArgs& a, uint32_t b, uint64_t step, uint32_t* s_rs,
+                                                    uint32_t* s_pref, uint32_t* s_wave, uint32_t* s_c,
+                                                    uint32_t* total) {
+    const uint32_t t = threadIdx.x;
     const bool is_src = b >= a.n_tiles;
     const uint32_t* rp = is_src ? a.src_row_ptr : a.row_ptr;
     uint32_t cnt, rs = 0, dg = 0;
@@ -635,22 +632,443 @@ __global__ __launch_bounds__(kTile) void insert_kernel(const NetArgs* __restrict
         }
         __syncthreads();
     }
-    if (cnt == 0) {
-        if (t == 0) a.tile_coll[b] = 0u;
-        return;
-    }
-    if (t == 0) s_ncoll = 0u;
     uint32_t excl;
-    const uint32_t total = block_exclusive_scan(dg, s_wave, excl);
-    if (t < cnt) { s_rs[t] = rs; s_pref[t] = excl; }
+    *total = block_exclusive_scan(dg, s_wave, excl);  // uniform; also a barrier
+    if (t < cnt) {
+        s_rs[t] = rs;
+        s_pref[t] = excl;
+    }
     __syncthreads();
-    const ScatterCtx c{step, static_cast<uint32_t>(step), a.tile_ebase[b], &s_ncoll};
-    scatter_rows<SCHED>(a, c, is_src, cnt, total, s_rs, s_pref);
+    return cnt;
+}
+
+// Scatter of the neuron tiles of networks with plasticity (after the STDP
+// kernels) and of every Poisson-source tile, for networks on the chained
+// scatter.
+template <int SCHED>
+__global__ __launch_bounds__(kTile) void insert_kernel(const NetArgs* __restrict__ nets,
+                                                      const uint2* __restrict__ blocks) {
+    __shared__ uint32_t s_rs[kTile], s_pref[kTile], s_wave[kWavesPerTile], s_c[kWavesPerTile];
+    __shared__ uint32_t s_ncoll;
+    const uint2 ref = blocks[blockIdx.x];
+    const NetArgs& a = nets[ref.x];
+    const uint32_t b = ref.y, t = threadIdx.x;
+    const uint64_t step = *a.step;
+    if (step >= a.step_limit) return;
+    if (t == 0) s_ncoll = 0u;
+    uint32_t total;
+    const uint32_t cnt = stage_tile_rows(a, b, step, s_rs, s_pref, s_wave, s_c, &total);
+    if (cnt) {
+        const ScatterCtx c{step, static_cast<uint32_t>(step), a.tile_ebase[b], &s_ncoll};
+        scatter_rows<SCHED>(a, c, b >= a.n_tiles, cnt, total, s_rs, s_pref);
+    }
     __syncthreads();
     if (t == 0) {
         a.tile_coll[b] = s_ncoll;
         if (total) atomicAdd(reinterpret_cast<unsigned long long*>(a.tile_events + b), total);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Binned scatter for networks whose delays are all <= 1 step: every event of
+// step s lands in slot (s+1) % 32, whose cells are all zero (consumed and
+// cleared 31 steps ago).  Per step:
+//   bin_kernel    block j of a network takes scatter tiles [j*T, (j+1)*T),
+//                 counts their events per bin of kBinPosts posts in LDS,
+//                 writes its row of bin offsets and then every event
+//                 {id, weight, post within bin} into its own region (the
+//                 block's edge-id range, so no global counter);
+//   bin_apply     block k of a network takes bin k: it reads bin k's segment
+//                 of every bin block, in block order, which is ascending event
+//                 id (blocks cover consecutive tiles; sources after neurons),
+//                 orders each chunk by (post, event id) in LDS and adds the
+//                 weights into an LDS copy of the bin's slot slice, then writes
+//                 the slice out whole.  Block 0 advances the step counter.
+// Sums per cell run in ascending event id from zero: the sequential push.  No
+// atomics outside LDS, no random ring traffic.
+// ---------------------------------------------------------------------------
+
+// Exclusive scan of n <= kTile*per entries held in LDS (in place); returns the total.
+template <uint32_t kPer>
+__device__ __forceinline__ uint32_t lds_exclusive_scan(uint32_t* v, uint32_t n, uint32_t* s_wave) {
+    const uint32_t t = threadIdx.x;
+    const uint32_t lo = t * kPer;
+    uint32_t loc[kPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kPer; ++i) {
+        loc[i] = lo + i < n ? v[lo + i] : 0u;
+        sum += loc[i];
+    }
+    uint32_t excl;
+    const uint32_t total = block_exclusive_scan(sum, s_wave, excl);
+#pragma unroll
+    for (uint32_t i = 0; i < kPer; ++i) {
+        if (lo + i < n) v[lo + i] = excl;
+        excl += loc[i];
+    }
+    __syncthreads();
+    return total;
+}
+
+// Edge of row-concatenation position f: row k holds positions
+// [s_pref[k], s_pref[k+1]) and starts at edge s_rs[k].
+__device__ __forceinline__ uint32_t concat_edge(uint32_t f, uint32_t cnt, const uint32_t* s_rs,
+                                                const uint32_t* s_pref) {
+    uint32_t lo = 0, hi = cnt;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_pref[mid] <= f) lo = mid; else hi = mid;
+    }
+    return s_rs[lo] + (f - s_pref[lo]);
+}
+
+// Count (kWrite false) or write out (kWrite true) the events of `cnt` staged
+// rows; four events per thread are loaded before any is used.
+template <bool kWrite>
+__device__ __forceinline__ void bin_events(const NetArgs& a, bool is_src, uint32_t cnt, uint32_t total,
+                                           const uint32_t* s_rs, const uint32_t* s_pref, uint32_t* s_cnt,
+                                           uint32_t base, uint64_t step) {
+    constexpr uint32_t kU = 4;
+    const uint32_t* col = is_src ? a.src_post : a.col;
+    const float* w = is_src ? a.src_w : a.w;
+    const uint32_t eb = is_src ? static_cast<uint32_t>(a.n_syn) : 0u;
+    for (uint32_t f0 = threadIdx.x; f0 < total; f0 += kU * kTile) {
+        uint32_t e[kU], post[kU];
+        float wt[kU];
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+            const uint32_t f = f0 + u * kTile;
+            e[u] = f < total ? concat_edge(f, cnt, s_rs, s_pref) : 0u;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+            if (f0 + u * kTile < total) {
+                post[u] = col[e[u]];
+                if (kWrite) wt[u] = w[e[u]];
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+            if (f0 + u * kTile >= total) continue;
+            const uint32_t bin = post[u] >> kBinLog2;
+            if (!kWrite) {
+                atomicAdd(&s_cnt[bin], 1u);
+            } else {
+                const float x = a.tm_type ? tm_deliver(a, eb + e[u], wt[u], step) : wt[u];
+                const uint32_t pos = atomicAdd(&s_cnt[bin], 1u);
+                a.bin_ev[base + pos] = make_uint4(eb + e[u], __float_as_uint(x), post[u] & (kBinPosts - 1u), 0u);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kTile) void bin_kernel(const NetArgs* __restrict__ nets,
+                                                   const uint2* __restrict__ refs) {
+    constexpr uint32_t kRows = 1024;  // spiking rows staged per chunk
+    __shared__ uint32_t s_rs[kRows], s_pref[kRows];
+    __shared__ uint32_t s_srs[kTile], s_spref[kTile];
+    __shared__ uint32_t s_tp[kTile];  // first spiking row of each neuron tile of the block
+    __shared__ uint32_t s_wave[kWavesPerTile], s_c[kWavesPerTile];
+    __shared__ uint32_t s_cnt[kMaxBins];
+    const uint2 ref = refs[blockIdx.x];
+    const NetArgs& a = nets[ref.x];
+    const uint32_t j = ref.y, t = threadIdx.x;
+    const uint64_t step = *a.step;
+    const bool live = step < a.step_limit;
+    if (j == 0 && t == 0) *a.bin_slot = live ? static_cast<uint32_t>((step + 1) & (kSlots - 1)) : kEmpty;
+    if (!live) return;
+    const uint32_t nb = a.n_bins;
+    const uint32_t b0 = j * a.bblk_tiles, b1 = min(b0 + a.bblk_tiles, a.n_blocks);
+    const uint32_t n_nt = b0 < a.n_tiles ? min(b1, a.n_tiles) - b0 : 0u;  // neuron tiles (<= kTile)
+    const uint32_t base = a.tile_ebase[b0];
+    for (uint32_t k = t; k < nb; k += kTile) s_cnt[k] = 0u;
+    // spiking rows of all the block's neuron tiles at once
+    const uint32_t tc = t < n_nt ? a.tile_count[b0 + t] : 0u;
+    uint32_t excl;
+    const uint32_t n_rows = block_exclusive_scan(tc, s_wave, excl);
+    if (t < n_nt) s_tp[t] = excl;
+    __syncthreads();
+    const bool one = n_rows <= kRows;  // rows staged once, for both passes
+    uint32_t events = 0, kept = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (uint32_t r0 = 0; r0 < n_rows; r0 += kRows) {
+            const uint32_t nr = min(kRows, n_rows - r0);
+            uint32_t total = kept;
+            if (!(one && pass == 1)) {
+                for (uint32_t i = t; i < nr; i += kTile) {
+                    const uint32_t r = r0 + i;
+                    uint32_t lo = 0, hi = n_nt;  // tile holding row r
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (s_tp[mid] <= r) lo = mid; else hi = mid;
+                    }
+                    const uint32_t id = a.active[static_cast<size_t>(b0 + lo) * kTile + (r - s_tp[lo])];
+                    const uint32_t rs = a.row_ptr[id];
+                    s_rs[i] = rs;
+                    s_pref[i] = a.row_ptr[id + 1] - rs;
+                }
+                __syncthreads();
+                total = lds_exclusive_scan<kRows / kTile>(s_pref, nr, s_wave);
+                kept = total;
+            }
+            if (pass == 0) {
+                events += total;
+                bin_events<false>(a, false, nr, total, s_rs, s_pref, s_cnt, base, step);
+            } else {
+                bin_events<true>(a, false, nr, total, s_rs, s_pref, s_cnt, base, step);
+            }
+            if (!one) __syncthreads();  // before the next chunk is staged
+        }
+        // source tiles (their rows are the sources that fire this step)
+        for (uint32_t b = max(b0, a.n_tiles); b < b1; ++b) {
+            uint32_t total;
+            const uint32_t cnt = stage_tile_rows(a, b, step, s_srs, s_spref, s_wave, s_c, &total);
+            if (pass == 0) {
+                events += total;
+                bin_events<false>(a, true, cnt, total, s_srs, s_spref, s_cnt, base, step);
+            } else {
+                bin_events<true>(a, true, cnt, total, s_srs, s_spref, s_cnt, base, step);
+            }
+            __syncthreads();
+        }
+        if (pass == 0) {
+            __syncthreads();
+            // bin offsets within the block's region
+            if (nb <= kTile) lds_exclusive_scan<1>(s_cnt, nb, s_wave);
+            else if (nb <= 4 * kTile) lds_exclusive_scan<4>(s_cnt, nb, s_wave);
+            else lds_exclusive_scan<kMaxBins / kTile>(s_cnt, nb, s_wave);
+            uint32_t* off = a.bin_off + static_cast<size_t>(j) * (nb + 1);
+            for (uint32_t k = t; k < nb; k += kTile) off[k] = s_cnt[k];
+            if (t == 0) off[nb] = events;
+            __syncthreads();
+        }
+    }
+    if (t == 0 && events) atomicAdd(reinterpret_cast<unsigned long long*>(a.tile_events + b0), events);
+}
+
+// LDS of one bin_apply block.  The chunk's records live in r_w and in `u`:
+// while lists are built, u holds r_eid[kBinCap] (u32), nxt[kBinCap] (u16,
+// the per-post list links) and r_post[kBinCap] (u16); the sort of long lists
+// reuses all of u as kBinCap 64-bit keys.
+struct BinApplyLds {
+    float slice[kBinPosts];
+    uint32_t head[kBinPosts];
+    float r_w[kBinCap];
+    uint64_t u[kBinCap];
+    uint32_t seg_base[kMaxBinBlocks];
+    uint32_t seg_pref[kMaxBinBlocks + 1];
+    uint32_t wave[kWavesPerTile];
+    uint32_t m, next;
+    __device__ uint32_t* r_eid() { return reinterpret_cast<uint32_t*>(u); }
+    __device__ uint16_t* nxt() { return reinterpret_cast<uint16_t*>(u + kBinCap / 2); }
+    __device__ uint16_t* r_post() { return reinterpret_cast<uint16_t*>(u + 3 * kBinCap / 4); }
+};
+constexpr uint32_t kListNil = 0xFFFFFFFFu, kListLong = 0xFFFFFFFEu;
+constexpr uint16_t kLinkNil = 0xFFFFu;
+constexpr int kShortList = 8;
+
+// Record q of the chunk: post `post` (within the bin), event id, weight.
+__device__ __forceinline__ void bin_link(BinApplyLds& L, uint32_t q, uint32_t post, uint32_t eid, float w) {
+    L.r_eid()[q] = eid;
+    L.r_w[q] = w;
+    L.r_post()[q] = static_cast<uint16_t>(post);
+    L.nxt()[q] = static_cast<uint16_t>(atomicExch(&L.head[post], q));  // kListNil -> kLinkNil
+}
+
+// Add the chunk's m records (linked per post from head[]) into the slice,
+// each cell in ascending event id, and leave every head empty.  Lists of up
+// to kShortList records are sorted by their owning thread; longer ones (hub
+// posts) are sorted together by (post, event id) with a block bitonic sort.
+__device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
+    const uint32_t t = threadIdx.x;
+    const uint32_t* r_eid = L.r_eid();
+    const uint16_t* nxt = L.nxt();
+    int longs = 0;
+    for (uint32_t p = t; p < kBinPosts; p += kTile) {
+        const uint32_t h = L.head[p];
+        if (h == kListNil) continue;
+        uint32_t ids[kShortList];
+        float ws[kShortList];
+        int n = 0;
+        bool fits = true;
+        for (uint32_t i = h;;) {
+            if (n == kShortList) {
+                fits = false;
+                break;
+            }
+            const uint32_t e = r_eid[i];
+            const float w = L.r_w[i];
+            int q = n++;
+            while (q > 0 && ids[q - 1] > e) {
+                ids[q] = ids[q - 1];
+                ws[q] = ws[q - 1];
+                --q;
+            }
+            ids[q] = e;
+            ws[q] = w;
+            const uint16_t nx = nxt[i];
+            if (nx == kLinkNil) break;
+            i = nx;
+        }
+        if (!fits) {
+            L.head[p] = kListLong;
+            longs = 1;
+            continue;
+        }
+        L.head[p] = kListNil;
+        float acc = L.slice[p];
+        for (int q = 0; q < n; ++q) acc = acc + ws[q];
+        L.slice[p] = acc;
+    }
+    if (!__syncthreads_or(longs)) return;
+    // keys (post << 43 | eid << 11 | record) of records on long lists; others sort last
+    constexpr uint32_t kPer = kBinCap / kTile;
+    uint32_t np2 = 1;
+    while (np2 < m) np2 <<= 1;
+    uint64_t kk[kPer];
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; ++r) {
+        const uint32_t i = t + r * kTile;
+        kk[r] = ~0ull;
+        if (i < m) {
+            const uint32_t post = L.r_post()[i];
+            if (L.head[post] == kListLong)
+                kk[r] = (static_cast<uint64_t>(post) << 43) | (static_cast<uint64_t>(r_eid[i]) << 11) | i;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; ++r)
+        if (t + r * kTile < np2) L.u[t + r * kTile] = kk[r];
+    __syncthreads();
+    for (uint32_t k = 2; k <= np2; k <<= 1) {
+        for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+            for (uint32_t i = t; i < np2; i += kTile) {
+                const uint32_t l = i ^ jj;
+                if (l > i) {
+                    const uint64_t x = L.u[i], y = L.u[l];
+                    if ((x > y) == ((i & k) == 0)) {
+                        L.u[i] = y;
+                        L.u[l] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // one thread per long list sums its sorted run
+    for (uint32_t i = t; i < m; i += kTile) {
+        const uint64_t k0 = L.u[i];
+        const uint32_t post = static_cast<uint32_t>(k0 >> 43);
+        if (post >= kBinPosts) break;  // past the last long-list record
+        if (i > 0 && static_cast<uint32_t>(L.u[i - 1] >> 43) == post) continue;
+        float acc = L.slice[post];
+        for (uint32_t q = i; q < m; ++q) {
+            const uint64_t kq = L.u[q];
+            if (static_cast<uint32_t>(kq >> 43) != post) break;
+            acc = acc + L.r_w[kq & (kBinCap - 1)];
+        }
+        L.slice[post] = acc;
+        L.head[post] = kListNil;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kTile) void bin_apply_kernel(const NetArgs* __restrict__ nets,
+                                                         const uint2* __restrict__ refs) {
+    __shared__ BinApplyLds L;
+    const uint2 ref = refs[blockIdx.x];
+    const NetArgs& a = nets[ref.x];
+    const uint32_t k = ref.y, t = threadIdx.x;
+    const uint32_t slot = *a.bin_slot;
+    if (slot == kEmpty) return;
+    const uint32_t nb = a.n_bins, nblk = a.n_bblk;
+    // bin k's segment of every bin block, in block order
+    constexpr uint32_t kPer = kMaxBinBlocks / kTile;
+    for (uint32_t r = 0; r < kPer; ++r) {
+        const uint32_t jb = t * kPer + r;
+        uint32_t c = 0;
+        if (jb < nblk) {
+            const uint32_t* off = a.bin_off + static_cast<size_t>(jb) * (nb + 1);
+            const uint32_t o = off[k];
+            c = off[k + 1] - o;
+            L.seg_base[jb] = a.tile_ebase[jb * a.bblk_tiles] + o;
+        }
+        L.seg_pref[jb] = c;
+    }
+    for (uint32_t p = t; p < kBinPosts; p += kTile) {
+        L.slice[p] = 0.0f;
+        L.head[p] = kListNil;
+    }
+    __syncthreads();
+    const uint32_t total = lds_exclusive_scan<kPer>(L.seg_pref, nblk, L.wave);
+    if (t == 0) L.seg_pref[nblk] = total;
+    __syncthreads();
+    uint32_t ja = 0;
+    while (ja < nblk) {
+        const uint32_t f0 = L.seg_pref[ja];
+        // the longest run of whole segments [ja, jb) that fits the chunk
+        uint32_t lo = ja, hi = nblk;  // largest jb with seg_pref[jb] <= f0 + kBinCap
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (L.seg_pref[mid] - f0 <= kBinCap) lo = mid; else hi = mid - 1;
+        }
+        const uint32_t jb = lo;
+        if (jb == ja) {
+            // Segment ja alone exceeds the chunk; its records are unordered,
+            // so take them in windows of kBinCap consecutive event ids (at
+            // most kBinCap records each), lowest first.
+            const uint4* rec = a.bin_ev + L.seg_base[ja];
+            const uint32_t seg = L.seg_pref[ja + 1] - f0;
+            if (t == 0) L.next = kListNil;
+            __syncthreads();
+            for (uint32_t i = t; i < seg; i += kTile) atomicMin(&L.next, rec[i].x);
+            __syncthreads();
+            uint32_t w0 = L.next;
+            for (;;) {
+                __syncthreads();
+                if (t == 0) {
+                    L.m = 0u;
+                    L.next = kListNil;
+                }
+                __syncthreads();
+                for (uint32_t i = t; i < seg; i += kTile) {
+                    const uint4 r = rec[i];
+                    if (r.x < w0) continue;
+                    if (r.x - w0 < kBinCap) bin_link(L, atomicAdd(&L.m, 1u), r.z, r.x, __uint_as_float(r.y));
+                    else atomicMin(&L.next, r.x);
+                }
+                __syncthreads();
+                bin_accumulate(L, L.m);
+                if (L.next == kListNil) break;
+                w0 = L.next;
+            }
+            __syncthreads();
+            ++ja;
+            continue;
+        }
+        const uint32_t m = L.seg_pref[jb] - f0;
+        for (uint32_t f = t; f < m; f += kTile) {
+            uint32_t l2 = ja, h2 = jb;  // segment of record f0 + f
+            while (h2 - l2 > 1) {
+                const uint32_t mid = (l2 + h2) >> 1;
+                if (L.seg_pref[mid] <= f0 + f) l2 = mid; else h2 = mid;
+            }
+            const uint4 r = a.bin_ev[L.seg_base[l2] + (f0 + f - L.seg_pref[l2])];
+            bin_link(L, f, r.z, r.x, __uint_as_float(r.y));
+        }
+        __syncthreads();
+        if (m) bin_accumulate(L, m);
+        ja = jb;
+    }
+    // the slice of slot (s+1) % 32 for posts [k*kBinPosts, ...)
+    __syncthreads();
+    const uint32_t p0 = k * kBinPosts;
+    const uint32_t np = min(kBinPosts, a.N - p0);
+    float* out = a.ring + static_cast<size_t>(slot) * a.N + p0;
+    for (uint32_t p = t; p < np; p += kTile) out[p] = L.slice[p];
+    if (k == 0 && t == 0) *a.step = *a.step + 1;  // no other block of this kernel reads it
 }
 
 __device__ __forceinline__ float event_weight(const NetArgs& a, uint32_t eid) {
@@ -1021,6 +1439,14 @@ void launch_insert(int sched, const NetArgs* nets, const uint2* blocks, uint32_t
     if (n == 0) return;
     if (sched == 2) insert_kernel<2><<<n, kTile, 0, s>>>(nets, blocks);
     else insert_kernel<1><<<n, kTile, 0, s>>>(nets, blocks);
+}
+void launch_bin(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s) {
+    if (n == 0) return;
+    bin_kernel<<<n, kTile, 0, s>>>(nets, refs);
+}
+void launch_bin_apply(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s) {
+    if (n == 0) return;
+    bin_apply_kernel<<<n, kTile, 0, s>>>(nets, refs);
 }
 void launch_fixup(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s) {
     if (n == 0) return;

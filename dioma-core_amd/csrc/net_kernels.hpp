@@ -15,6 +15,14 @@ constexpr uint32_t kFixTiles = 8;        // scatter tiles per fix-up block
 constexpr uint32_t kMaxGroup = 2;        // neuron tiles per block of the neuron kernel
 constexpr uint32_t kMaxNeurons = 1u << 27;  // per network: slot*N + post fits 32 bits
 constexpr int kChain = 8;                // chain entries sorted in LDS per thread
+// Binned scatter (networks whose delays are all <= 1 step): posts per bin,
+// most bins per network, events one apply block orders in LDS at a time, and
+// most bin blocks (groups of consecutive scatter tiles) per network.
+constexpr uint32_t kBinLog2 = 11;
+constexpr uint32_t kBinPosts = 1u << kBinLog2;
+constexpr uint32_t kMaxBins = 4096;
+constexpr uint32_t kBinCap = 2048;
+constexpr uint32_t kMaxBinBlocks = 1024;
 
 enum : uint32_t {
     kPopRefrElided = 1u,  // refractory timer provably stays 0: not loaded/stored
@@ -63,6 +71,16 @@ struct NetArgs {
     uint64_t step_limit;        // kernels skip the network once *step reaches it
     float* ring;                // [32][N]
     int32_t ring_fresh;         // every delay <= 1: this step's target slot holds zeros
+    // Binned scatter (n_bins > 0): bin block j covers scatter tiles
+    // [j*bblk_tiles, (j+1)*bblk_tiles) and writes its events, grouped by bin of
+    // kBinPosts posts, from tile_ebase[first tile]; bin_off[j][k] is the start
+    // of bin k in that region (n_bins + 1 entries per block).
+    uint32_t n_bins;
+    uint32_t n_bblk;
+    uint32_t bblk_tiles;
+    uint32_t* bin_off;          // [n_bblk][n_bins + 1]
+    uint4* bin_ev;              // [events] {event id, weight bits, post within the bin, 0}
+    uint32_t* bin_slot;         // ring slot this step's events target, kEmpty past the horizon
     // Scatter bookkeeping.  head[key] = (step tag << 32) | event id of the
     // last event that reached key = slot*N + post in the step with that tag;
     // a head with another tag is idle, so heads are never cleared per step.

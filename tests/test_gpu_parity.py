@@ -199,6 +199,63 @@ def test_unit_delay_network_with_sources_and_injection(sched):
     assert_same_ring(dev, ora)
 
 
+def test_binned_scatter_oversized_segments_and_hubs():
+    # delays <= 1 (binned scatter): every neuron fires every step and sends 40 edges into the
+    # first 2048 posts, so one tile puts 10240 records into one bin (more than one LDS chunk)
+    # and a few hub posts collect hundreds of events per step
+    say("binned_oversized")
+    rng = np.random.default_rng(17)
+    n, deg = 4096, 40
+    pre = np.repeat(np.arange(n, dtype=np.uint32), deg)
+    post = np.where(rng.random(pre.size) < 0.1, rng.integers(0, 4, pre.size),
+                    rng.integers(0, 2048, pre.size)).astype(np.uint32)
+    w = rng.normal(0.0, 1.0, pre.size).astype(np.float32)
+    d = rng.integers(0, 2, pre.size).astype(np.uint8)
+    rp, col, w, d = csr_from_triplets(n, pre, post, w, d)
+    spec = NetSpec("hubs", [lif_pop(n, 0.5, 0.5, -70.0, -54.0, -70.0, 0.0, -70.0)], rp, col, w, d, 0.5, 0)
+    idx = np.arange(n)
+    big = np.full(n, 500.0, np.float32)
+    dev, ora = run_pair(spec, 12, inject=lambda dv, o, t: (dv.inject_current(0, idx, big),
+                                                         o.inject_current(0, idx, big)))
+    assert ora.counters()[1] >= 12 * n * deg
+    assert dev.counters() == ora.counters()
+    assert_same_ring(dev, ora)
+    assert_same_state(dev, ora, spec)
+
+
+def test_binned_scatter_many_tiles_with_sources():
+    # 80k neurons = 313 neuron tiles + source tiles: several tiles per bin block
+    say("binned_many_tiles")
+    spec = random_net(80_000, 30, seed=19, model="lif", max_delay=1, sources=300)
+    dev, ora = run_pair(spec, 150, check_every=10)
+    assert ora.counters()[0] > 1000
+    assert dev.counters() == ora.counters()
+    assert_same_state(dev, ora, spec)
+    assert_same_ring(dev, ora)
+
+
+def test_binned_scatter_multi_chunk_blocks():
+    # 1.2M neurons = 4688 neuron tiles, so a bin block covers 5 tiles; with every
+    # neuron firing a block stages 1280 spiking rows, more than one LDS chunk
+    say("binned_multi_chunk")
+    rng = np.random.default_rng(23)
+    n, deg = 1_200_000, 3
+    pre = np.repeat(np.arange(n, dtype=np.uint32), deg)
+    post = rng.integers(0, n, pre.size).astype(np.uint32)
+    w = rng.normal(0.0, 1.0, pre.size).astype(np.float32)
+    d = np.ones(pre.size, np.uint8)
+    rp, col, w, d = csr_from_triplets(n, pre, post, w, d)
+    spec = NetSpec("wide", [lif_pop(n, 0.5, 0.5, -70.0, -54.0, -70.0, 0.0, -70.0)], rp, col, w, d, 0.5, 0)
+    idx = np.arange(n, dtype=np.uint32)
+    big = np.full(n, 500.0, np.float32)
+    dev, ora = run_pair(spec, 3, inject=lambda dv, o, t: (dv.inject_current(0, idx, big),
+                                                        o.inject_current(0, idx, big)))
+    assert ora.counters()[0] >= 2 * n
+    assert dev.counters() == ora.counters()
+    assert_same_ring(dev, ora)
+    assert_same_state(dev, ora, spec)
+
+
 def test_graph_replay_matches_stepwise():
     say("graph_replay")
     spec = random_net(5000, 60, seed=3, model="izh", max_delay=7, sources=5)
