@@ -30,15 +30,16 @@ namespace simuse {
 int fail(int code, const char* fmt, ...);
 
 void launch_neuron(int model, int sched, const NetArgs* nets, const uint4* items, uint32_t n, hipStream_t s,
-                   uint32_t defer_scatter);
+                   uint32_t defer_scatter, uint32_t bins);
 void launch_dense_trace(const NetArgs* nets, uint32_t n_tiles, float* trace, float decay, hipStream_t s);
 void launch_dense_weights(const NetArgs* nets, uint32_t N, const float* trace, float lr_ltp, float lr_ltd,
                           float w_min, float w_max, hipStream_t s);
 void launch_stdp(const NetArgs* nets, const uint2* tiles, uint32_t n, hipStream_t s);
 void launch_insert(int sched, const NetArgs* nets, const uint2* blocks, uint32_t n, hipStream_t s);
 void launch_fixup(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s);
-void launch_bin(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s);
+void launch_bin(const NetArgs* nets, const uint4* refs, uint32_t n, uint32_t bins, hipStream_t s);
 void launch_bin_apply(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s);
+
 void launch_head_clear(unsigned long long* head, const uint64_t* step, size_t n, hipStream_t s);
 void launch_inject(float* ring, const uint64_t* step, uint32_t N, const uint32_t* idx, const float* val,
                    uint32_t count, int serial, hipStream_t s);
@@ -116,8 +117,11 @@ struct NetState {
     uint32_t max_delay = 0;    // largest synapse or source delay (steps)
     // binned scatter (every delay <= 1 step): bins of kBinPosts posts
     bool binned = false;
-    uint32_t n_bins = 0, n_bblk = 0, bblk_tiles = 0;
-    uint32_t* bin_off = nullptr;
+    uint32_t n_bins = 0, bin_rows = 0;
+    uint2* bin_hdr = nullptr;
+    std::vector<uint4> dense_writers;  // every writer as a bin_kernel item (dense STDP step)
+    uint4* d_dense_writers = nullptr;  // [bin_rows + kXcds]
+    bool dense_dirty = true;
     uint4* bin_ev = nullptr;
     uint32_t* bin_slot = nullptr;
     uint64_t since_clear = 0;  // steps enqueued since the heads were last rewritten
@@ -211,9 +215,8 @@ struct NetState {
         a.ring = ring;
         a.ring_fresh = max_delay <= 1 ? 1 : 0;
         a.n_bins = binned ? n_bins : 0u;
-        a.n_bblk = n_bblk;
-        a.bblk_tiles = bblk_tiles;
-        a.bin_off = bin_off;
+        a.bin_hdr = bin_hdr;
+        a.bin_rows = bin_rows;
         a.bin_ev = bin_ev;
         a.bin_slot = bin_slot;
         a.head = head;
@@ -312,9 +315,10 @@ struct Engine {
     uint32_t n_stdp = 0, cap_stdp = 0;
     uint2* d_fix = nullptr;
     uint32_t n_fix = 0, cap_fix = 0;
-    uint2* d_bin = nullptr;     // bin blocks of binned networks
+    uint4* d_bin = nullptr;     // bin_kernel writers of binned networks
     uint32_t n_bin = 0, cap_bin = 0;
-    uint2* d_apply = nullptr;   // bins of binned networks
+    uint32_t bins_fused = 0, bins_bin = 0;  // most bins of a network with neuron-kernel / bin_kernel writers
+    uint2* d_apply = nullptr;   // bins of binned networks, XCD-ordered
     uint32_t n_apply = 0, cap_apply = 0;
     bool direct_fix = false;
     // execution
@@ -373,26 +377,74 @@ struct Engine {
         return SIMUSE_OK;
     }
 
+    // Workgroup b of a launch runs on XCD b % kXcds: reorder a work list so
+    // that XCD x takes the contiguous items [x*per, (x+1)*per) (padded with
+    // `pad`, which the kernels skip).  Neighbouring tiles and bins then share
+    // that XCD's L2.
+    template <class T>
+    static void xcd_order(std::vector<T>& v, const T& pad) {
+        const size_t n = v.size(), per = (n + kXcds - 1) / kXcds;
+        std::vector<T> o(per * kXcds, pad);
+        for (size_t b = 0; b < o.size(); ++b) {
+            const size_t i = (b % kXcds) * per + b / kXcds;
+            if (i < n) o[b] = v[i];
+        }
+        v.swap(o);
+    }
+
     // Rebuild the NetArgs array and the (network, tile) work lists.
     int ensure_tables() {
         if (!dirty) return SIMUSE_OK;
         HIP_TRY(hipStreamSynchronize(stream));
         std::vector<NetArgs> na;
-        std::vector<uint4> tl[kModels];
-        std::vector<uint2> bl, sl, fl, binl, apl;
+        std::vector<uint4> tl[kModels], binl;
+        std::vector<uint2> bl, sl, fl, apl;
+        bins_fused = bins_bin = 0;
         for (uint32_t k = 0; k < nets.size(); ++k) {
             NetState* s = nets[k];
             na.push_back(s->args());
+            NetArgs& A = na.back();
+            const uint32_t n_blk = s->n_tiles + s->n_src_tiles;
+            uint32_t ng = 0;  // neuron blocks
+            for (const HostPop& P : s->pops) ng += (P.ntiles + group - 1) / group;
+            const uint32_t ns = (s->n_src_tiles + kBinBlockTiles - 1) / kBinBlockTiles;
+            // binned networks without pair-based plasticity: neuron block j is writer j
+            const bool fz = s->binned && !s->stdp && ng + ns <= kMaxWriters;
+            uint32_t j = 0;
             for (uint32_t p = 0; p < s->pops.size(); ++p) {
                 const HostPop& P = s->pops[p];
-                for (uint32_t t = 0; t < P.ntiles; t += group)
-                    tl[P.model].push_back(make_uint4(k, P.tile0 + t, std::min(group, P.ntiles - t), p));
+                for (uint32_t t = 0; t < P.ntiles; t += group, ++j)
+                    tl[P.model].push_back(make_uint4(k, P.tile0 + t, std::min(group, P.ntiles - t) | (j << 4), p));
             }
             if (s->stdp)
                 for (uint32_t t = 0; t < s->n_tiles; ++t) sl.push_back(make_uint2(k, t));
             if (s->binned) {
-                for (uint32_t j = 0; j < s->n_bblk; ++j) binl.push_back(make_uint2(k, j));
-                for (uint32_t j = 0; j < s->n_bins; ++j) apl.push_back(make_uint2(k, j));
+                std::vector<uint4>& wl = s->dense_writers;
+                wl.clear();
+                j = 0;
+                if (fz) {
+                    for (const HostPop& P : s->pops)
+                        for (uint32_t t = 0; t < P.ntiles; t += group, ++j)
+                            wl.push_back(make_uint4(k, P.tile0 + t, std::min(group, P.ntiles - t), j));
+                    for (uint32_t b = s->n_tiles; b < n_blk; b += kBinBlockTiles, ++j) {
+                        wl.push_back(make_uint4(k, b, std::min(kBinBlockTiles, n_blk - b), j));
+                        binl.push_back(wl.back());
+                    }
+                } else {
+                    const uint32_t T = std::max(kBinBlockTiles, (n_blk + kMaxWriters - 1) / kMaxWriters);
+                    for (uint32_t b = 0; b < n_blk; b += T, ++j) {
+                        wl.push_back(make_uint4(k, b, std::min(T, n_blk - b), j));
+                        binl.push_back(wl.back());
+                    }
+                }
+                if (j > s->bin_rows) return fail(SIMUSE_E_STATE, "binned scatter: %u writers, %u rows", j, s->bin_rows);
+                xcd_order(wl, make_uint4(kEmpty, 0u, 0u, 0u));
+                s->dense_dirty = true;
+                A.n_bblk = j;
+                A.bin_fused = fz ? 1 : 0;
+                if (fz) bins_fused = std::max(bins_fused, s->n_bins);
+                bins_bin = std::max(bins_bin, s->n_bins);
+                for (uint32_t b = 0; b < s->n_bins; ++b) apl.push_back(make_uint2(k, b));
                 continue;
             }
             // chained scatter: plastic networks scatter after the STDP kernels;
@@ -408,6 +460,7 @@ struct Engine {
             HIP_TRY(hipMemcpy(d_nets, na.data(), sizeof(NetArgs) * na.size(), hipMemcpyHostToDevice));
         }
         for (int m = 0; m < kModels; ++m) {
+            xcd_order(tl[m], make_uint4(kEmpty, 0u, 0u, 0u));
             n_tiles_m[m] = static_cast<uint32_t>(tl[m].size());
             if (tl[m].empty()) continue;
             if ((rc = grow(&d_tiles[m], &cap_tiles_m[m], n_tiles_m[m]))) return rc;
@@ -428,11 +481,13 @@ struct Engine {
             if ((rc = grow(&d_fix, &cap_fix, n_fix))) return rc;
             HIP_TRY(hipMemcpy(d_fix, fl.data(), sizeof(uint2) * fl.size(), hipMemcpyHostToDevice));
         }
+        xcd_order(binl, make_uint4(kEmpty, 0u, 0u, 0u));
         n_bin = static_cast<uint32_t>(binl.size());
         if (n_bin) {
             if ((rc = grow(&d_bin, &cap_bin, n_bin))) return rc;
-            HIP_TRY(hipMemcpy(d_bin, binl.data(), sizeof(uint2) * binl.size(), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(d_bin, binl.data(), sizeof(uint4) * binl.size(), hipMemcpyHostToDevice));
         }
+        xcd_order(apl, make_uint2(kEmpty, 0u));
         n_apply = static_cast<uint32_t>(apl.size());
         if (n_apply) {
             if ((rc = grow(&d_apply, &cap_apply, n_apply))) return rc;
@@ -445,12 +500,13 @@ struct Engine {
 
     void launch_step(hipEvent_t* e) {
         if (e) (void)hipEventRecord(e[0], stream);
-        for (int m = 0; m < kModels; ++m) launch_neuron(m, sched, d_nets, d_tiles[m], n_tiles_m[m], stream, 0u);
+        for (int m = 0; m < kModels; ++m)
+            launch_neuron(m, sched, d_nets, d_tiles[m], n_tiles_m[m], stream, 0u, bins_fused);
         if (e) (void)hipEventRecord(e[1], stream);
         launch_stdp(d_nets, d_stdp, n_stdp, stream);
         if (e) (void)hipEventRecord(e[2], stream);
         launch_insert(sched, d_nets, d_blocks, n_blocks, stream);
-        launch_bin(d_nets, d_bin, n_bin, stream);
+        launch_bin(d_nets, d_bin, n_bin, bins_bin, stream);
         if (e) (void)hipEventRecord(e[3], stream);
         launch_bin_apply(d_nets, d_apply, n_apply, stream);
         launch_fixup(d_nets, direct_fix ? nullptr : d_fix, n_fix, stream);

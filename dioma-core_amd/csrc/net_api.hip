@@ -158,13 +158,15 @@ int NetState::finalize(hipStream_t stream) {
     const uint32_t n_blk = n_tiles + n_src_tiles;
     // Every delay <= 1 step (synapses and sources): the binned scatter;
     // otherwise the chained one with its tagged heads and event records.
-    binned = max_delay <= 1 && N <= kBinPosts * kMaxBins && n_blk > 0 && n_blk <= kMaxBinBlocks * kTile;
+    binned = max_delay <= 1 && N <= kBinPosts * kMaxBins && n_blk > 0;
     if (binned) {
         n_bins = (N + kBinPosts - 1) / kBinPosts;
-        bblk_tiles = (n_blk + kMaxBinBlocks - 1) / kMaxBinBlocks;
-        n_bblk = (n_blk + bblk_tiles - 1) / bblk_tiles;
+        // writers: at most one per neuron tile plus one per kBinBlockTiles source tiles
+        bin_rows = std::min(kMaxWriters, n_tiles + (n_src_tiles + kBinBlockTiles - 1) / kBinBlockTiles);
+        bin_rows += bin_rows & 1u;  // even: bin_apply reads two writers' headers at once
         if ((rc = alloc(&bin_ev, n_ev))) return rc;
-        if ((rc = alloc(&bin_off, static_cast<size_t>(n_bblk) * (n_bins + 1)))) return rc;
+        if ((rc = alloc(&bin_hdr, static_cast<size_t>(n_bins) * bin_rows))) return rc;
+        if ((rc = alloc(&d_dense_writers, bin_rows + kXcds))) return rc;  // XCD-ordered, padded
         if ((rc = alloc(&bin_slot, 1))) return rc;
     } else {
         if ((rc = alloc(&head, static_cast<size_t>(kSlots) * N))) return rc;
@@ -1000,10 +1002,15 @@ int simuse_net_stdp_step(simuse_net* net, float trace_decay, float lr_ltp, float
     // scatter; every synapse; then the push with the new weights.
     launch_dense_trace(e.d_nets, s.n_tiles, s.dtrace, trace_decay, e.stream);
     for (int m = 0; m < Engine::kModels; ++m)
-        launch_neuron(m, e.sched, e.d_nets, e.d_tiles[m], e.n_tiles_m[m], e.stream, 1u);
+        launch_neuron(m, e.sched, e.d_nets, e.d_tiles[m], e.n_tiles_m[m], e.stream, 1u, 0u);
     launch_dense_weights(e.d_nets, s.N, s.dtrace, lr_ltp, lr_ltd, w_min, w_max, e.stream);
     if (s.binned) {
-        launch_bin(e.d_nets, e.d_bin, e.n_bin, e.stream);
+        if (s.dense_dirty) {
+            HIP_TRY(hipMemcpy(s.d_dense_writers, s.dense_writers.data(), sizeof(uint4) * s.dense_writers.size(),
+                              hipMemcpyHostToDevice));
+            s.dense_dirty = false;
+        }
+        launch_bin(e.d_nets, s.d_dense_writers, static_cast<uint32_t>(s.dense_writers.size()), s.n_bins, e.stream);
         launch_bin_apply(e.d_nets, e.d_apply, e.n_apply, e.stream);
     } else {
         launch_insert(e.sched, e.d_nets, s.d_all_blocks, n_blk, e.stream);

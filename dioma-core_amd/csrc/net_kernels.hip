@@ -426,6 +426,178 @@ struct Neuron<2> {  // Hodgkin-Huxley: reads and writes its columns inside the u
     __device__ __forceinline__ void store(const PopDev&, uint32_t) const {}
 };
 
+// ---------------------------------------------------------------------------
+// Binned scatter for networks whose delays are all <= 1 step: every event of
+// step s lands in slot (s+1) % 32, whose cells are all zero (consumed and
+// cleared 31 steps ago).  Posts are grouped in bins of kBinPosts.  A
+// network's scatter tiles are split among at most kMaxWriters writers, in
+// ascending tile (so event id) order: the neuron kernel's blocks when the
+// network has no pair-based plasticity, else stand-alone bin_kernel blocks;
+// source tiles always go to bin_kernel blocks after them.  Per step:
+//   writer j   counts its events per bin in LDS, writes them grouped by bin
+//              from its first edge id (regions never overlap) as {event id,
+//              weight, post within bin}, and writes column j of bin_hdr:
+//              {first record, count} of each bin's group;
+//   bin_apply  block k reads bin k's group from every writer, in writer order
+//              (ascending event id), orders each post's records by event id in
+//              LDS, adds them into an LDS copy of the bin's slot slice and
+//              writes the slice out whole.  Its work list is permuted so that
+//              each XCD takes a contiguous range of bins, whose record lines
+//              then meet in that XCD's L2.
+// Sums per cell run in ascending event id from zero: the sequential push.  No
+// global atomics, no random ring traffic.
+// ---------------------------------------------------------------------------
+
+// Exclusive scan of n <= kTile*per entries held in LDS (in place); returns the total.
+template <uint32_t kPer>
+__device__ __forceinline__ uint32_t lds_exclusive_scan(uint32_t* v, uint32_t n, uint32_t* s_wave) {
+    const uint32_t t = threadIdx.x;
+    const uint32_t lo = t * kPer;
+    uint32_t loc[kPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kPer; ++i) {
+        loc[i] = lo + i < n ? v[lo + i] : 0u;
+        sum += loc[i];
+    }
+    uint32_t excl;
+    const uint32_t total = block_exclusive_scan(sum, s_wave, excl);
+#pragma unroll
+    for (uint32_t i = 0; i < kPer; ++i) {
+        if (lo + i < n) v[lo + i] = excl;
+        excl += loc[i];
+    }
+    __syncthreads();
+    return total;
+}
+
+// Largest index i in [lo0, hi) with v[i] <= x, given v[lo0] <= x and v
+// nondecreasing; fixed trip count (steps of kSpan/2, ..., 1) so independent
+// searches interleave.
+template <uint32_t kSpan>
+__device__ __forceinline__ uint32_t floor_index(const uint32_t* v, uint32_t lo0, uint32_t hi, uint32_t x) {
+    uint32_t lo = lo0;
+#pragma unroll
+    for (uint32_t s = kSpan / 2; s > 0; s >>= 1)
+        if (lo + s < hi && v[lo + s] <= x) lo += s;
+    return lo;
+}
+
+// Count (kWrite false) or write out (kWrite true) the events of `cnt` staged
+// rows (cnt <= 1024): row k starts at edge s_rs[k] and holds concatenation
+// positions [s_pref[k], s_pref[k+1]).  Counting adds 1 to s_bin[bin]; writing
+// puts the event at record s_bin[bin] (absolute in bin_ev) and advances it.
+// Four events per thread are loaded before any is used.
+template <bool kWrite>
+__device__ __forceinline__ void bin_events(const NetArgs& a, bool is_src, uint32_t cnt, uint32_t total,
+                                           const uint32_t* s_rs, const uint32_t* s_pref, uint32_t* s_bin,
+                                           uint64_t step) {
+    constexpr uint32_t kU = 4;
+    const uint32_t* col = is_src ? a.src_post : a.col;
+    const float* w = is_src ? a.src_w : a.w;
+    const uint32_t eb = is_src ? static_cast<uint32_t>(a.n_syn) : 0u;
+    for (uint32_t f0 = threadIdx.x; f0 < total; f0 += kU * kTile) {
+        uint32_t e[kU], post[kU];
+        float wt[kU];
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+            const uint32_t f = f0 + u * kTile;
+            e[u] = 0u;
+            if (f < total) {
+                const uint32_t k = floor_index<1024>(s_pref, 0, cnt, f);
+                e[u] = s_rs[k] + (f - s_pref[k]);
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+            if (f0 + u * kTile < total) {
+                post[u] = col[e[u]];
+                if (kWrite) wt[u] = w[e[u]];
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+            if (f0 + u * kTile >= total) continue;
+            const uint32_t bin = post[u] >> kBinLog2;
+            if (!kWrite) {
+                atomicAdd(&s_bin[bin], 1u);
+            } else {
+                const float x = a.tm_type ? tm_deliver(a, eb + e[u], wt[u], step) : wt[u];
+                const uint32_t pos = atomicAdd(&s_bin[bin], 1u);
+                a.bin_ev[pos] = make_uint4(eb + e[u], __float_as_uint(x), post[u] & (kBinPosts - 1u), 0u);
+            }
+        }
+    }
+}
+
+// The block's events when there are at most kEvCache per thread: loaded once
+// (edge id, post, weight) and held in registers from the count to the write.
+constexpr uint32_t kEvCache = 4;
+struct EvCache {
+    uint32_t e[kEvCache], post[kEvCache];
+    float w[kEvCache];
+};
+
+__device__ __forceinline__ void ev_cache_load(const NetArgs& a, bool is_src, uint32_t cnt, uint32_t total,
+                                              const uint32_t* s_rs, const uint32_t* s_pref, EvCache& c) {
+    const uint32_t* col = is_src ? a.src_post : a.col;
+    const float* w = is_src ? a.src_w : a.w;
+#pragma unroll
+    for (uint32_t u = 0; u < kEvCache; ++u) {
+        const uint32_t f = threadIdx.x + u * kTile;
+        c.e[u] = 0u;
+        if (f < total) {
+            const uint32_t k = floor_index<1024>(s_pref, 0, cnt, f);
+            c.e[u] = s_rs[k] + (f - s_pref[k]);
+        }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kEvCache; ++u) {
+        if (threadIdx.x + u * kTile < total) {
+            c.post[u] = col[c.e[u]];
+            c.w[u] = w[c.e[u]];
+        }
+    }
+}
+
+__device__ __forceinline__ void ev_cache_count(const EvCache& c, uint32_t total, uint32_t* s_bin) {
+#pragma unroll
+    for (uint32_t u = 0; u < kEvCache; ++u)
+        if (threadIdx.x + u * kTile < total) atomicAdd(&s_bin[c.post[u] >> kBinLog2], 1u);
+}
+
+__device__ __forceinline__ void ev_cache_write(const NetArgs& a, bool is_src, const EvCache& c, uint32_t total,
+                                               uint32_t* s_bin, uint64_t step) {
+    const uint32_t eb = is_src ? static_cast<uint32_t>(a.n_syn) : 0u;
+#pragma unroll
+    for (uint32_t u = 0; u < kEvCache; ++u) {
+        if (threadIdx.x + u * kTile >= total) continue;
+        const uint32_t e = eb + c.e[u];
+        const float x = a.tm_type ? tm_deliver(a, e, c.w[u], step) : c.w[u];
+        const uint32_t pos = atomicAdd(&s_bin[c.post[u] >> kBinLog2], 1u);
+        a.bin_ev[pos] = make_uint4(e, __float_as_uint(x), c.post[u] & (kBinPosts - 1u), 0u);
+    }
+}
+
+// Writer j's per-bin counts (s_bin) become the absolute first record of each
+// bin's group, from `base`; column j of bin_hdr gets {start, count} per bin.
+// Synchronises before and after.
+__device__ __forceinline__ void bin_offsets(const NetArgs& a, uint32_t* s_bin, uint32_t* s_wave, uint32_t j,
+                                            uint32_t base, uint32_t events) {
+    __syncthreads();
+    const uint32_t nb = a.n_bins, t = threadIdx.x;
+    if (nb <= kTile) lds_exclusive_scan<1>(s_bin, nb, s_wave);
+    else if (nb <= 4 * kTile) lds_exclusive_scan<4>(s_bin, nb, s_wave);
+    else lds_exclusive_scan<kMaxBins / kTile>(s_bin, nb, s_wave);
+    for (uint32_t k = t; k < nb; k += kTile) {
+        const uint32_t o = s_bin[k], e = k + 1 < nb ? s_bin[k + 1] : events;
+        a.bin_hdr[static_cast<size_t>(k) * a.bin_rows + j] = make_uint2(base + o, e - o);
+    }
+    __syncthreads();
+    for (uint32_t k = t; k < nb; k += kTile) s_bin[k] += base;
+    __syncthreads();
+}
+
 // One step of up to kMaxGroup consecutive tiles of one population: read ring
 // slot s%32 and zero it, add the Poisson drive and Gaussian noise, clamp
 // (LIF/Izhikevich), advance the neuron, update STDP traces, then wave-ballot
@@ -442,11 +614,17 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
     __shared__ uint32_t s_cnt[kMaxGroup][kWavesPerTile], s_deg[kMaxGroup][kWavesPerTile];
     __shared__ uint32_t s_drv[kMaxGroup][kWavesPerTile];
     __shared__ uint32_t s_ncoll;
+    __shared__ uint32_t s_bw[kWavesPerTile];
+    extern __shared__ uint32_t s_bin[];  // [n_bins] of binned writers (dynamic)
     const uint4 it = items[blockIdx.x];
+    if (it.x == kEmpty) return;  // pad of the XCD-ordered list
     const NetArgs& a = nets[it.x];
     const PopDev& P = a.pops[it.w];
-    const uint32_t tile0 = it.y, ntile = it.z;
-    const bool fused = a.stdp == 0 && !defer_scatter && a.n_bins == 0;
+    const uint32_t tile0 = it.y, ntile = it.z & 15u, wj = it.z >> 4;
+    const bool fused = a.stdp == 0 && !defer_scatter && a.n_bins == 0;  // chained scatter of the block's spikes
+    const bool fused_bin = a.bin_fused && !defer_scatter;               // binned writer wj of its spikes
+    const bool inline_push = fused || fused_bin;
+    const uint32_t ebase = fused_bin ? a.tile_ebase[tile0] : 0u;
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     const uint32_t li0 = (tile0 - P.tile0) * kTile + t;
 
@@ -459,14 +637,18 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
         const uint32_t li = li0 + g * kTile;
         if (g < ntile && li < P.n) {
             nrn[g].load(P, li);
-            if (fused) {
+            if (inline_push) {
                 row0[g] = a.row_ptr[P.off + li];
                 deg[g] = a.row_ptr[P.off + li + 1] - row0[g];
             }
         }
     }
     const uint64_t step = *a.step;
+    if (a.n_bins && tile0 == 0 && t == 0)  // the slot bin_apply fills
+        *a.bin_slot = step < a.step_limit ? static_cast<uint32_t>((step + 1) & (kSlots - 1)) : kEmpty;
     if (step >= a.step_limit) return;
+    if (fused_bin)
+        for (uint32_t k = t; k < a.n_bins; k += kTile) s_bin[k] = 0u;
     float* ring_slot = a.ring + static_cast<size_t>(step & (kSlots - 1)) * a.N + P.off;
     float cur[kMaxGroup];
 #pragma unroll
@@ -558,7 +740,7 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
             if ((m[g] >> lane) & 1ull) {
                 const uint32_t rank = base + static_cast<uint32_t>(__popcll(m[g] & ((1ull << lane) - 1ull)));
                 a.active[static_cast<size_t>(tile0 + g) * kTile + rank] = P.off + li0 + g * kTile;
-                if (fused) {
+                if (inline_push) {
                     s_rs[blk_sp + rank] = row0[g];
                     s_pref[blk_sp + rank] = blk_ed + dbase + excl[g];
                 }
@@ -572,8 +754,23 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
         if (blk_sp) atomicAdd(reinterpret_cast<unsigned long long*>(a.tile_spikes + tile0), blk_sp);
         // each firing drive source is one synaptic event, counted at its roll;
         // the fused scatter's events are counted here too
-        const uint32_t ev = drv + (fused ? blk_ed : 0u);
+        const uint32_t ev = drv + (inline_push ? blk_ed : 0u);
         if (ev) atomicAdd(reinterpret_cast<unsigned long long*>(a.tile_events + tile0), ev);
+    }
+    if (fused_bin) {
+        __syncthreads();
+        const bool cached = blk_ed <= kEvCache * kTile;
+        EvCache ec;
+        if (cached) {
+            ev_cache_load(a, false, blk_sp, blk_ed, s_rs, s_pref, ec);
+            ev_cache_count(ec, blk_ed, s_bin);
+        } else {
+            bin_events<false>(a, false, blk_sp, blk_ed, s_rs, s_pref, s_bin, step);
+        }
+        bin_offsets(a, s_bin, s_bw, wj, ebase, blk_ed);
+        if (cached) ev_cache_write(a, false, ec, blk_ed, s_bin, step);
+        else bin_events<true>(a, false, blk_sp, blk_ed, s_rs, s_pref, s_bin, step);
+        return;
     }
     if (!fused) return;
     // the block's collisions are listed from its first tile's base
@@ -669,121 +866,28 @@ __global__ __launch_bounds__(kTile) void insert_kernel(const NetArgs* __restrict
     }
 }
 
-// ---------------------------------------------------------------------------
-// Binned scatter for networks whose delays are all <= 1 step: every event of
-// step s lands in slot (s+1) % 32, whose cells are all zero (consumed and
-// cleared 31 steps ago).  Per step:
-//   bin_kernel    block j of a network takes scatter tiles [j*T, (j+1)*T),
-//                 counts their events per bin of kBinPosts posts in LDS,
-//                 writes its row of bin offsets and then every event
-//                 {id, weight, post within bin} into its own region (the
-//                 block's edge-id range, so no global counter);
-//   bin_apply     block k of a network takes bin k: it reads bin k's segment
-//                 of every bin block, in block order, which is ascending event
-//                 id (blocks cover consecutive tiles; sources after neurons),
-//                 orders each chunk by (post, event id) in LDS and adds the
-//                 weights into an LDS copy of the bin's slot slice, then writes
-//                 the slice out whole.  Block 0 advances the step counter.
-// Sums per cell run in ascending event id from zero: the sequential push.  No
-// atomics outside LDS, no random ring traffic.
-// ---------------------------------------------------------------------------
-
-// Exclusive scan of n <= kTile*per entries held in LDS (in place); returns the total.
-template <uint32_t kPer>
-__device__ __forceinline__ uint32_t lds_exclusive_scan(uint32_t* v, uint32_t n, uint32_t* s_wave) {
-    const uint32_t t = threadIdx.x;
-    const uint32_t lo = t * kPer;
-    uint32_t loc[kPer];
-    uint32_t sum = 0;
-#pragma unroll
-    for (uint32_t i = 0; i < kPer; ++i) {
-        loc[i] = lo + i < n ? v[lo + i] : 0u;
-        sum += loc[i];
-    }
-    uint32_t excl;
-    const uint32_t total = block_exclusive_scan(sum, s_wave, excl);
-#pragma unroll
-    for (uint32_t i = 0; i < kPer; ++i) {
-        if (lo + i < n) v[lo + i] = excl;
-        excl += loc[i];
-    }
-    __syncthreads();
-    return total;
-}
-
-// Edge of row-concatenation position f: row k holds positions
-// [s_pref[k], s_pref[k+1]) and starts at edge s_rs[k].
-__device__ __forceinline__ uint32_t concat_edge(uint32_t f, uint32_t cnt, const uint32_t* s_rs,
-                                                const uint32_t* s_pref) {
-    uint32_t lo = 0, hi = cnt;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (s_pref[mid] <= f) lo = mid; else hi = mid;
-    }
-    return s_rs[lo] + (f - s_pref[lo]);
-}
-
-// Count (kWrite false) or write out (kWrite true) the events of `cnt` staged
-// rows; four events per thread are loaded before any is used.
-template <bool kWrite>
-__device__ __forceinline__ void bin_events(const NetArgs& a, bool is_src, uint32_t cnt, uint32_t total,
-                                           const uint32_t* s_rs, const uint32_t* s_pref, uint32_t* s_cnt,
-                                           uint32_t base, uint64_t step) {
-    constexpr uint32_t kU = 4;
-    const uint32_t* col = is_src ? a.src_post : a.col;
-    const float* w = is_src ? a.src_w : a.w;
-    const uint32_t eb = is_src ? static_cast<uint32_t>(a.n_syn) : 0u;
-    for (uint32_t f0 = threadIdx.x; f0 < total; f0 += kU * kTile) {
-        uint32_t e[kU], post[kU];
-        float wt[kU];
-#pragma unroll
-        for (uint32_t u = 0; u < kU; ++u) {
-            const uint32_t f = f0 + u * kTile;
-            e[u] = f < total ? concat_edge(f, cnt, s_rs, s_pref) : 0u;
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < kU; ++u) {
-            if (f0 + u * kTile < total) {
-                post[u] = col[e[u]];
-                if (kWrite) wt[u] = w[e[u]];
-            }
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < kU; ++u) {
-            if (f0 + u * kTile >= total) continue;
-            const uint32_t bin = post[u] >> kBinLog2;
-            if (!kWrite) {
-                atomicAdd(&s_cnt[bin], 1u);
-            } else {
-                const float x = a.tm_type ? tm_deliver(a, eb + e[u], wt[u], step) : wt[u];
-                const uint32_t pos = atomicAdd(&s_cnt[bin], 1u);
-                a.bin_ev[base + pos] = make_uint4(eb + e[u], __float_as_uint(x), post[u] & (kBinPosts - 1u), 0u);
-            }
-        }
-    }
-}
-
+// Stand-alone binned writer: work item {network, first scatter tile, tiles,
+// writer index}; neuron tiles (of plastic networks, after the STDP kernels,
+// or of every network in the dense STDP step) and/or Poisson-source tiles.
+// The spiking rows of all of its neuron tiles are staged together.
 __global__ __launch_bounds__(kTile) void bin_kernel(const NetArgs* __restrict__ nets,
-                                                   const uint2* __restrict__ refs) {
+                                                   const uint4* __restrict__ refs) {
     constexpr uint32_t kRows = 1024;  // spiking rows staged per chunk
+    constexpr uint32_t kRowsPer = kRows / kTile;
     __shared__ uint32_t s_rs[kRows], s_pref[kRows];
     __shared__ uint32_t s_srs[kTile], s_spref[kTile];
     __shared__ uint32_t s_tp[kTile];  // first spiking row of each neuron tile of the block
     __shared__ uint32_t s_wave[kWavesPerTile], s_c[kWavesPerTile];
-    __shared__ uint32_t s_cnt[kMaxBins];
-    const uint2 ref = refs[blockIdx.x];
+    extern __shared__ uint32_t s_bin[];  // [n_bins] (dynamic)
+    const uint4 ref = refs[blockIdx.x];
+    if (ref.x == kEmpty) return;  // pad of the XCD-ordered list
     const NetArgs& a = nets[ref.x];
-    const uint32_t j = ref.y, t = threadIdx.x;
+    const uint32_t b0 = ref.y, b1 = min(ref.y + ref.z, a.n_blocks), j = ref.w, t = threadIdx.x;
     const uint64_t step = *a.step;
-    const bool live = step < a.step_limit;
-    if (j == 0 && t == 0) *a.bin_slot = live ? static_cast<uint32_t>((step + 1) & (kSlots - 1)) : kEmpty;
-    if (!live) return;
-    const uint32_t nb = a.n_bins;
-    const uint32_t b0 = j * a.bblk_tiles, b1 = min(b0 + a.bblk_tiles, a.n_blocks);
+    if (step >= a.step_limit) return;
     const uint32_t n_nt = b0 < a.n_tiles ? min(b1, a.n_tiles) - b0 : 0u;  // neuron tiles (<= kTile)
     const uint32_t base = a.tile_ebase[b0];
-    for (uint32_t k = t; k < nb; k += kTile) s_cnt[k] = 0u;
-    // spiking rows of all the block's neuron tiles at once
+    for (uint32_t k = t; k < a.n_bins; k += kTile) s_bin[k] = 0u;
     const uint32_t tc = t < n_nt ? a.tile_count[b0 + t] : 0u;
     uint32_t excl;
     const uint32_t n_rows = block_exclusive_scan(tc, s_wave, excl);
@@ -791,32 +895,50 @@ __global__ __launch_bounds__(kTile) void bin_kernel(const NetArgs* __restrict__ 
     __syncthreads();
     const bool one = n_rows <= kRows;  // rows staged once, for both passes
     uint32_t events = 0, kept = 0;
+    bool cached = false;  // one chunk of at most kEvCache events per thread
+    EvCache ec;
     for (int pass = 0; pass < 2; ++pass) {
         for (uint32_t r0 = 0; r0 < n_rows; r0 += kRows) {
             const uint32_t nr = min(kRows, n_rows - r0);
             uint32_t total = kept;
             if (!(one && pass == 1)) {
-                for (uint32_t i = t; i < nr; i += kTile) {
-                    const uint32_t r = r0 + i;
-                    uint32_t lo = 0, hi = n_nt;  // tile holding row r
-                    while (hi - lo > 1) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if (s_tp[mid] <= r) lo = mid; else hi = mid;
+                uint32_t id[kRowsPer];
+#pragma unroll
+                for (uint32_t u = 0; u < kRowsPer; ++u) {
+                    const uint32_t i = t + u * kTile;
+                    id[u] = 0u;
+                    if (i < nr) {
+                        const uint32_t r = r0 + i;
+                        const uint32_t lo = floor_index<kTile>(s_tp, 0, n_nt, r);  // tile holding row r
+                        id[u] = a.active[static_cast<size_t>(b0 + lo) * kTile + (r - s_tp[lo])];
                     }
-                    const uint32_t id = a.active[static_cast<size_t>(b0 + lo) * kTile + (r - s_tp[lo])];
-                    const uint32_t rs = a.row_ptr[id];
-                    s_rs[i] = rs;
-                    s_pref[i] = a.row_ptr[id + 1] - rs;
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < kRowsPer; ++u) {
+                    const uint32_t i = t + u * kTile;
+                    if (i < nr) {
+                        const uint32_t rs = a.row_ptr[id[u]];
+                        s_rs[i] = rs;
+                        s_pref[i] = a.row_ptr[id[u] + 1] - rs;
+                    }
                 }
                 __syncthreads();
-                total = lds_exclusive_scan<kRows / kTile>(s_pref, nr, s_wave);
+                total = lds_exclusive_scan<kRowsPer>(s_pref, nr, s_wave);
                 kept = total;
             }
             if (pass == 0) {
                 events += total;
-                bin_events<false>(a, false, nr, total, s_rs, s_pref, s_cnt, base, step);
+                cached = one && total <= kEvCache * kTile;
+                if (cached) {
+                    ev_cache_load(a, false, nr, total, s_rs, s_pref, ec);
+                    ev_cache_count(ec, total, s_bin);
+                } else {
+                    bin_events<false>(a, false, nr, total, s_rs, s_pref, s_bin, step);
+                }
+            } else if (cached) {
+                ev_cache_write(a, false, ec, total, s_bin, step);
             } else {
-                bin_events<true>(a, false, nr, total, s_rs, s_pref, s_cnt, base, step);
+                bin_events<true>(a, false, nr, total, s_rs, s_pref, s_bin, step);
             }
             if (!one) __syncthreads();  // before the next chunk is staged
         }
@@ -826,28 +948,46 @@ __global__ __launch_bounds__(kTile) void bin_kernel(const NetArgs* __restrict__ 
             const uint32_t cnt = stage_tile_rows(a, b, step, s_srs, s_spref, s_wave, s_c, &total);
             if (pass == 0) {
                 events += total;
-                bin_events<false>(a, true, cnt, total, s_srs, s_spref, s_cnt, base, step);
+                bin_events<false>(a, true, cnt, total, s_srs, s_spref, s_bin, step);
             } else {
-                bin_events<true>(a, true, cnt, total, s_srs, s_spref, s_cnt, base, step);
+                bin_events<true>(a, true, cnt, total, s_srs, s_spref, s_bin, step);
             }
             __syncthreads();
         }
-        if (pass == 0) {
-            __syncthreads();
-            // bin offsets within the block's region
-            if (nb <= kTile) lds_exclusive_scan<1>(s_cnt, nb, s_wave);
-            else if (nb <= 4 * kTile) lds_exclusive_scan<4>(s_cnt, nb, s_wave);
-            else lds_exclusive_scan<kMaxBins / kTile>(s_cnt, nb, s_wave);
-            uint32_t* off = a.bin_off + static_cast<size_t>(j) * (nb + 1);
-            for (uint32_t k = t; k < nb; k += kTile) off[k] = s_cnt[k];
-            if (t == 0) off[nb] = events;
-            __syncthreads();
-        }
+        if (pass == 0) bin_offsets(a, s_bin, s_wave, j, base, events);
     }
     if (t == 0 && events) atomicAdd(reinterpret_cast<unsigned long long*>(a.tile_events + b0), events);
 }
 
-// LDS of one bin_apply block.  The chunk's records live in r_w and in `u`:
+// bin_apply runs kApply threads per block: two posts, two records and two
+// writer headers per thread.
+constexpr uint32_t kApply = 1024;
+constexpr uint32_t kApplyWaves = kApply / 64;
+
+// Exclusive prefix sum over one bin_apply block; returns the total.
+__device__ __forceinline__ uint32_t apply_exclusive_scan(uint32_t x, uint32_t* s_wave, uint32_t& excl) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t inc = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o, 64);
+        if (lane >= static_cast<uint32_t>(o)) inc += y;
+    }
+    if (lane == 63) s_wave[wave] = inc;
+    __syncthreads();
+    uint32_t base = 0, total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kApplyWaves; ++w) {
+        const uint32_t v = s_wave[w];
+        if (w < wave) base += v;
+        total += v;
+    }
+    excl = base + inc - x;
+    __syncthreads();
+    return total;
+}
+
+// LDS of one bin_apply block.  A pass's records live in r_w and in `u`:
 // while lists are built, u holds r_eid[kBinCap] (u32), nxt[kBinCap] (u16,
 // the per-post list links) and r_post[kBinCap] (u16); the sort of long lists
 // reuses all of u as kBinCap 64-bit keys.
@@ -856,9 +996,9 @@ struct BinApplyLds {
     uint32_t head[kBinPosts];
     float r_w[kBinCap];
     uint64_t u[kBinCap];
-    uint32_t seg_base[kMaxBinBlocks];
-    uint32_t seg_pref[kMaxBinBlocks + 1];
-    uint32_t wave[kWavesPerTile];
+    uint32_t seg_base[kMaxWriters];
+    uint32_t seg_pref[kMaxWriters + 1];
+    uint32_t wave[kApplyWaves];
     uint32_t m, next;
     __device__ uint32_t* r_eid() { return reinterpret_cast<uint32_t*>(u); }
     __device__ uint16_t* nxt() { return reinterpret_cast<uint16_t*>(u + kBinCap / 2); }
@@ -868,7 +1008,7 @@ constexpr uint32_t kListNil = 0xFFFFFFFFu, kListLong = 0xFFFFFFFEu;
 constexpr uint16_t kLinkNil = 0xFFFFu;
 constexpr int kShortList = 8;
 
-// Record q of the chunk: post `post` (within the bin), event id, weight.
+// Record q of the pass: post `post` (within the bin), event id, weight.
 __device__ __forceinline__ void bin_link(BinApplyLds& L, uint32_t q, uint32_t post, uint32_t eid, float w) {
     L.r_eid()[q] = eid;
     L.r_w[q] = w;
@@ -876,20 +1016,28 @@ __device__ __forceinline__ void bin_link(BinApplyLds& L, uint32_t q, uint32_t po
     L.nxt()[q] = static_cast<uint16_t>(atomicExch(&L.head[post], q));  // kListNil -> kLinkNil
 }
 
-// Add the chunk's m records (linked per post from head[]) into the slice,
+// Add the pass's m records (linked per post from head[]) into the slice,
 // each cell in ascending event id, and leave every head empty.  Lists of up
-// to kShortList records are sorted by their owning thread; longer ones (hub
-// posts) are sorted together by (post, event id) with a block bitonic sort.
+// to kShortList records are insertion-sorted in registers by their owning
+// thread; longer ones (hub posts) are sorted together by (post, event id)
+// with a block bitonic sort.
 __device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
     const uint32_t t = threadIdx.x;
     const uint32_t* r_eid = L.r_eid();
     const uint16_t* nxt = L.nxt();
     int longs = 0;
-    for (uint32_t p = t; p < kBinPosts; p += kTile) {
+#pragma unroll
+    for (uint32_t pr = 0; pr < kBinPosts / kApply; ++pr) {
+        const uint32_t p = t + pr * kApply;
         const uint32_t h = L.head[p];
         if (h == kListNil) continue;
         uint32_t ids[kShortList];
         float ws[kShortList];
+#pragma unroll
+        for (int q = 0; q < kShortList; ++q) {
+            ids[q] = 0xFFFFFFFFu;  // above every event id
+            ws[q] = 0.0f;
+        }
         int n = 0;
         bool fits = true;
         for (uint32_t i = h;;) {
@@ -899,15 +1047,19 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
             }
             const uint32_t e = r_eid[i];
             const float w = L.r_w[i];
-            int q = n++;
-            while (q > 0 && ids[q - 1] > e) {
-                ids[q] = ids[q - 1];
-                ws[q] = ws[q - 1];
-                --q;
-            }
-            ids[q] = e;
-            ws[q] = w;
             const uint16_t nx = nxt[i];
+            // insert (e, w) into the ascending ids[]: every slot holding a larger
+            // id takes its left neighbour's entry, or (e, w) if that one is smaller
+#pragma unroll
+            for (int q = kShortList - 1; q >= 0; --q) {
+                const int qp = q > 0 ? q - 1 : 0;
+                const bool prev_gt = q > 0 && ids[qp] > e;
+                if (ids[q] > e) {
+                    ids[q] = prev_gt ? ids[qp] : e;
+                    ws[q] = prev_gt ? ws[qp] : w;
+                }
+            }
+            ++n;
             if (nx == kLinkNil) break;
             i = nx;
         }
@@ -918,18 +1070,20 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
         }
         L.head[p] = kListNil;
         float acc = L.slice[p];
-        for (int q = 0; q < n; ++q) acc = acc + ws[q];
+#pragma unroll
+        for (int q = 0; q < kShortList; ++q)
+            if (q < n) acc = acc + ws[q];
         L.slice[p] = acc;
     }
     if (!__syncthreads_or(longs)) return;
     // keys (post << 43 | eid << 11 | record) of records on long lists; others sort last
-    constexpr uint32_t kPer = kBinCap / kTile;
+    constexpr uint32_t kPer = kBinCap / kApply;
     uint32_t np2 = 1;
     while (np2 < m) np2 <<= 1;
     uint64_t kk[kPer];
 #pragma unroll
     for (uint32_t r = 0; r < kPer; ++r) {
-        const uint32_t i = t + r * kTile;
+        const uint32_t i = t + r * kApply;
         kk[r] = ~0ull;
         if (i < m) {
             const uint32_t post = L.r_post()[i];
@@ -940,11 +1094,11 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
     __syncthreads();
 #pragma unroll
     for (uint32_t r = 0; r < kPer; ++r)
-        if (t + r * kTile < np2) L.u[t + r * kTile] = kk[r];
+        if (t + r * kApply < np2) L.u[t + r * kApply] = kk[r];
     __syncthreads();
     for (uint32_t k = 2; k <= np2; k <<= 1) {
         for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
-            for (uint32_t i = t; i < np2; i += kTile) {
+            for (uint32_t i = t; i < np2; i += kApply) {
                 const uint32_t l = i ^ jj;
                 if (l > i) {
                     const uint64_t x = L.u[i], y = L.u[l];
@@ -958,7 +1112,7 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
         }
     }
     // one thread per long list sums its sorted run
-    for (uint32_t i = t; i < m; i += kTile) {
+    for (uint32_t i = t; i < m; i += kApply) {
         const uint64_t k0 = L.u[i];
         const uint32_t post = static_cast<uint32_t>(k0 >> 43);
         if (post >= kBinPosts) break;  // past the last long-list record
@@ -975,55 +1129,56 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
     __syncthreads();
 }
 
-__global__ __launch_bounds__(kTile) void bin_apply_kernel(const NetArgs* __restrict__ nets,
-                                                         const uint2* __restrict__ refs) {
+// Work item {network, bin}; {kEmpty, 0} pads the XCD-ordered list.
+__global__ __launch_bounds__(kApply) void bin_apply_kernel(const NetArgs* __restrict__ nets,
+                                                          const uint2* __restrict__ refs) {
     __shared__ BinApplyLds L;
     const uint2 ref = refs[blockIdx.x];
+    if (ref.x == kEmpty) return;
     const NetArgs& a = nets[ref.x];
     const uint32_t k = ref.y, t = threadIdx.x;
     const uint32_t slot = *a.bin_slot;
     if (slot == kEmpty) return;
-    const uint32_t nb = a.n_bins, nblk = a.n_bblk;
-    // bin k's segment of every bin block, in block order
-    constexpr uint32_t kPer = kMaxBinBlocks / kTile;
-    for (uint32_t r = 0; r < kPer; ++r) {
-        const uint32_t jb = t * kPer + r;
-        uint32_t c = 0;
-        if (jb < nblk) {
-            const uint32_t* off = a.bin_off + static_cast<size_t>(jb) * (nb + 1);
-            const uint32_t o = off[k];
-            c = off[k + 1] - o;
-            L.seg_base[jb] = a.tile_ebase[jb * a.bblk_tiles] + o;
-        }
-        L.seg_pref[jb] = c;
+    const uint32_t nw = a.n_bblk;
+    // bin k's {start, count} from every writer, in writer order: writers
+    // 2t and 2t+1 (rows hold an even number of writers)
+    static_assert(kMaxWriters == 2 * kApply, "two writers per thread");
+    uint4 hw = make_uint4(0u, 0u, 0u, 0u);
+    if (2 * t < nw) hw = reinterpret_cast<const uint4*>(a.bin_hdr + static_cast<size_t>(k) * a.bin_rows)[t];
+    if (2 * t + 1 >= nw) hw.w = 0u;
+#pragma unroll
+    for (uint32_t pr = 0; pr < kBinPosts / kApply; ++pr) {
+        L.slice[t + pr * kApply] = 0.0f;
+        L.head[t + pr * kApply] = kListNil;
     }
-    for (uint32_t p = t; p < kBinPosts; p += kTile) {
-        L.slice[p] = 0.0f;
-        L.head[p] = kListNil;
-    }
-    __syncthreads();
-    const uint32_t total = lds_exclusive_scan<kPer>(L.seg_pref, nblk, L.wave);
-    if (t == 0) L.seg_pref[nblk] = total;
+    uint32_t excl;
+    const uint32_t total = apply_exclusive_scan(hw.y + hw.w, L.wave, excl);
+    L.seg_base[2 * t] = hw.x;
+    L.seg_base[2 * t + 1] = hw.z;
+    L.seg_pref[2 * t] = excl;
+    L.seg_pref[2 * t + 1] = excl + hw.y;
+    if (t == 0) L.seg_pref[kMaxWriters] = total;
     __syncthreads();
     uint32_t ja = 0;
-    while (ja < nblk) {
+    while (ja < nw) {
         const uint32_t f0 = L.seg_pref[ja];
-        // the longest run of whole segments [ja, jb) that fits the chunk
-        uint32_t lo = ja, hi = nblk;  // largest jb with seg_pref[jb] <= f0 + kBinCap
+        if (f0 == total) break;  // nothing left
+        // the longest run of whole groups [ja, jb) that fits one pass
+        uint32_t lo = ja, hi = nw;  // largest jb with seg_pref[jb] <= f0 + kBinCap
         while (lo < hi) {
             const uint32_t mid = (lo + hi + 1) >> 1;
             if (L.seg_pref[mid] - f0 <= kBinCap) lo = mid; else hi = mid - 1;
         }
         const uint32_t jb = lo;
         if (jb == ja) {
-            // Segment ja alone exceeds the chunk; its records are unordered,
-            // so take them in windows of kBinCap consecutive event ids (at
-            // most kBinCap records each), lowest first.
+            // Group ja alone exceeds a pass; its records are unordered, so
+            // take them in windows of kBinCap consecutive event ids (at most
+            // kBinCap records each), lowest first.
             const uint4* rec = a.bin_ev + L.seg_base[ja];
             const uint32_t seg = L.seg_pref[ja + 1] - f0;
             if (t == 0) L.next = kListNil;
             __syncthreads();
-            for (uint32_t i = t; i < seg; i += kTile) atomicMin(&L.next, rec[i].x);
+            for (uint32_t i = t; i < seg; i += kApply) atomicMin(&L.next, rec[i].x);
             __syncthreads();
             uint32_t w0 = L.next;
             for (;;) {
@@ -1033,7 +1188,7 @@ __global__ __launch_bounds__(kTile) void bin_apply_kernel(const NetArgs* __restr
                     L.next = kListNil;
                 }
                 __syncthreads();
-                for (uint32_t i = t; i < seg; i += kTile) {
+                for (uint32_t i = t; i < seg; i += kApply) {
                     const uint4 r = rec[i];
                     if (r.x < w0) continue;
                     if (r.x - w0 < kBinCap) bin_link(L, atomicAdd(&L.m, 1u), r.z, r.x, __uint_as_float(r.y));
@@ -1049,17 +1204,23 @@ __global__ __launch_bounds__(kTile) void bin_apply_kernel(const NetArgs* __restr
             continue;
         }
         const uint32_t m = L.seg_pref[jb] - f0;
-        for (uint32_t f = t; f < m; f += kTile) {
-            uint32_t l2 = ja, h2 = jb;  // segment of record f0 + f
-            while (h2 - l2 > 1) {
-                const uint32_t mid = (l2 + h2) >> 1;
-                if (L.seg_pref[mid] <= f0 + f) l2 = mid; else h2 = mid;
+        constexpr uint32_t kRec = kBinCap / kApply;
+        uint4 r[kRec];
+#pragma unroll
+        for (uint32_t u = 0; u < kRec; ++u) {
+            const uint32_t f = t + u * kApply;
+            if (f < m) {
+                const uint32_t sg = floor_index<kMaxWriters>(L.seg_pref, ja, jb, f0 + f);  // group of the record
+                r[u] = a.bin_ev[L.seg_base[sg] + (f0 + f - L.seg_pref[sg])];
             }
-            const uint4 r = a.bin_ev[L.seg_base[l2] + (f0 + f - L.seg_pref[l2])];
-            bin_link(L, f, r.z, r.x, __uint_as_float(r.y));
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kRec; ++u) {
+            const uint32_t f = t + u * kApply;
+            if (f < m) bin_link(L, f, r[u].z, r[u].x, __uint_as_float(r[u].y));
         }
         __syncthreads();
-        if (m) bin_accumulate(L, m);
+        bin_accumulate(L, m);
         ja = jb;
     }
     // the slice of slot (s+1) % 32 for posts [k*kBinPosts, ...)
@@ -1067,7 +1228,7 @@ __global__ __launch_bounds__(kTile) void bin_apply_kernel(const NetArgs* __restr
     const uint32_t p0 = k * kBinPosts;
     const uint32_t np = min(kBinPosts, a.N - p0);
     float* out = a.ring + static_cast<size_t>(slot) * a.N + p0;
-    for (uint32_t p = t; p < np; p += kTile) out[p] = L.slice[p];
+    for (uint32_t p = t; p < np; p += kApply) out[p] = L.slice[p];
     if (k == 0 && t == 0) *a.step = *a.step + 1;  // no other block of this kernel reads it
 }
 
@@ -1407,18 +1568,19 @@ __global__ void raster_stats_kernel(const uint64_t* raster, uint32_t n_words, ui
 // Host-side launchers (called from net_api.hip)
 // ---------------------------------------------------------------------------
 void launch_neuron(int model, int sched, const NetArgs* nets, const uint4* items, uint32_t n, hipStream_t s,
-                   uint32_t defer_scatter) {
+                   uint32_t defer_scatter, uint32_t bins) {
     if (n == 0) return;
+    const size_t lds = sizeof(uint32_t) * bins;
     if (sched == 2) {
-        if (model == 0) neuron_kernel<0, 2><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
-        else if (model == 1) neuron_kernel<1, 2><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
-        else if (model == 2) neuron_kernel<2, 2><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
-        else neuron_kernel<3, 2><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
+        if (model == 0) neuron_kernel<0, 2><<<n, kTile, lds, s>>>(nets, items, defer_scatter);
+        else if (model == 1) neuron_kernel<1, 2><<<n, kTile, lds, s>>>(nets, items, defer_scatter);
+        else if (model == 2) neuron_kernel<2, 2><<<n, kTile, lds, s>>>(nets, items, defer_scatter);
+        else neuron_kernel<3, 2><<<n, kTile, lds, s>>>(nets, items, defer_scatter);
     } else {
-        if (model == 0) neuron_kernel<0, 1><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
-        else if (model == 1) neuron_kernel<1, 1><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
-        else if (model == 2) neuron_kernel<2, 1><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
-        else neuron_kernel<3, 1><<<n, kTile, 0, s>>>(nets, items, defer_scatter);
+        if (model == 0) neuron_kernel<0, 1><<<n, kTile, lds, s>>>(nets, items, defer_scatter);
+        else if (model == 1) neuron_kernel<1, 1><<<n, kTile, lds, s>>>(nets, items, defer_scatter);
+        else if (model == 2) neuron_kernel<2, 1><<<n, kTile, lds, s>>>(nets, items, defer_scatter);
+        else neuron_kernel<3, 1><<<n, kTile, lds, s>>>(nets, items, defer_scatter);
     }
 }
 void launch_dense_trace(const NetArgs* nets, uint32_t n_tiles, float* trace, float decay, hipStream_t s) {
@@ -1440,13 +1602,13 @@ void launch_insert(int sched, const NetArgs* nets, const uint2* blocks, uint32_t
     if (sched == 2) insert_kernel<2><<<n, kTile, 0, s>>>(nets, blocks);
     else insert_kernel<1><<<n, kTile, 0, s>>>(nets, blocks);
 }
-void launch_bin(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s) {
+void launch_bin(const NetArgs* nets, const uint4* refs, uint32_t n, uint32_t bins, hipStream_t s) {
     if (n == 0) return;
-    bin_kernel<<<n, kTile, 0, s>>>(nets, refs);
+    bin_kernel<<<n, kTile, sizeof(uint32_t) * bins, s>>>(nets, refs);
 }
 void launch_bin_apply(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s) {
     if (n == 0) return;
-    bin_apply_kernel<<<n, kTile, 0, s>>>(nets, refs);
+    bin_apply_kernel<<<n, kApply, 0, s>>>(nets, refs);
 }
 void launch_fixup(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s) {
     if (n == 0) return;

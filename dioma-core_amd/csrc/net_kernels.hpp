@@ -21,8 +21,10 @@ constexpr int kChain = 8;                // chain entries sorted in LDS per thre
 constexpr uint32_t kBinLog2 = 11;
 constexpr uint32_t kBinPosts = 1u << kBinLog2;
 constexpr uint32_t kMaxBins = 4096;
-constexpr uint32_t kBinCap = 2048;
-constexpr uint32_t kMaxBinBlocks = 1024;
+constexpr uint32_t kXcds = 8;          // MI355X: workgroups go round-robin over 8 XCDs
+constexpr uint32_t kBinCap = 2048;      // records bin_apply orders in one LDS pass
+constexpr uint32_t kMaxWriters = 2048;  // binned writers per network
+constexpr uint32_t kBinBlockTiles = 4;  // source tiles per bin_kernel writer
 
 enum : uint32_t {
     kPopRefrElided = 1u,  // refractory timer provably stays 0: not loaded/stored
@@ -71,14 +73,15 @@ struct NetArgs {
     uint64_t step_limit;        // kernels skip the network once *step reaches it
     float* ring;                // [32][N]
     int32_t ring_fresh;         // every delay <= 1: this step's target slot holds zeros
-    // Binned scatter (n_bins > 0): bin block j covers scatter tiles
-    // [j*bblk_tiles, (j+1)*bblk_tiles) and writes its events, grouped by bin of
-    // kBinPosts posts, from tile_ebase[first tile]; bin_off[j][k] is the start
-    // of bin k in that region (n_bins + 1 entries per block).
+    // Binned scatter (n_bins > 0): bins of kBinPosts posts; writer j (of
+    // n_bblk, ascending event ids) puts its events grouped by bin from its
+    // first edge id, and bin_hdr[k][j] = {first record, count} of its group
+    // for bin k.
     uint32_t n_bins;
-    uint32_t n_bblk;
-    uint32_t bblk_tiles;
-    uint32_t* bin_off;          // [n_bblk][n_bins + 1]
+    uint32_t n_bblk;            // writers
+    int32_t bin_fused;          // the neuron kernel's blocks are the neuron-tile writers
+    uint32_t bin_rows;          // row length of bin_hdr (even, >= n_bblk)
+    uint2* bin_hdr;             // [n_bins][bin_rows]
     uint4* bin_ev;              // [events] {event id, weight bits, post within the bin, 0}
     uint32_t* bin_slot;         // ring slot this step's events target, kEmpty past the horizon
     // Scatter bookkeeping.  head[key] = (step tag << 32) | event id of the
