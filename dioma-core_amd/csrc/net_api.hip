@@ -166,6 +166,7 @@ int NetState::finalize(hipStream_t stream) {
         bin_rows += bin_rows & 1u;  // even: bin_apply reads two writers' headers at once
         if ((rc = alloc(&bin_ev, n_ev))) return rc;
         if ((rc = alloc(&bin_hdr, static_cast<size_t>(n_bins) * bin_rows))) return rc;
+        HIP_TRY(hipMemsetAsync(bin_hdr, 0, sizeof(uint2) * n_bins * bin_rows, stream));
         if ((rc = alloc(&d_dense_writers, bin_rows + kXcds))) return rc;  // XCD-ordered, padded
         if ((rc = alloc(&bin_slot, 1))) return rc;
     } else {
@@ -293,6 +294,7 @@ int NetState::reset_dynamic(hipStream_t stream) {
     HIP_TRY(hipMemsetAsync(step, 0, sizeof(uint64_t) * 2, stream));
     clear_heads(stream);
     if (tile_coll) HIP_TRY(hipMemsetAsync(tile_coll, 0, sizeof(uint32_t) * (n_tiles + n_src_tiles), stream));
+    if (bin_hdr) HIP_TRY(hipMemsetAsync(bin_hdr, 0, sizeof(uint2) * n_bins * bin_rows, stream));
     HIP_TRY(hipMemsetAsync(tile_count, 0, sizeof(uint32_t) * n_tiles, stream));
     HIP_TRY(hipMemsetAsync(bits, 0, sizeof(uint64_t) * n_words, stream));
     HIP_TRY(hipMemsetAsync(tile_spikes, 0, sizeof(uint64_t) * n_tiles, stream));

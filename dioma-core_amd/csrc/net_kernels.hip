@@ -580,7 +580,9 @@ __device__ __forceinline__ void ev_cache_write(const NetArgs& a, bool is_src, co
 }
 
 // Writer j's per-bin counts (s_bin) become the absolute first record of each
-// bin's group, from `base`; column j of bin_hdr gets {start, count} per bin.
+// bin's group, from `base`; column j of bin_hdr gets {start, count} for each
+// bin it sends events to (bin_apply zeroes the entries it consumes, so the
+// others read as empty).
 // Synchronises before and after.
 __device__ __forceinline__ void bin_offsets(const NetArgs& a, uint32_t* s_bin, uint32_t* s_wave, uint32_t j,
                                             uint32_t base, uint32_t events) {
@@ -591,7 +593,7 @@ __device__ __forceinline__ void bin_offsets(const NetArgs& a, uint32_t* s_bin, u
     else lds_exclusive_scan<kMaxBins / kTile>(s_bin, nb, s_wave);
     for (uint32_t k = t; k < nb; k += kTile) {
         const uint32_t o = s_bin[k], e = k + 1 < nb ? s_bin[k + 1] : events;
-        a.bin_hdr[static_cast<size_t>(k) * a.bin_rows + j] = make_uint2(base + o, e - o);
+        if (e != o) a.bin_hdr[static_cast<size_t>(k) * a.bin_rows + j] = make_uint2(base + o, e - o);
     }
     __syncthreads();
     for (uint32_t k = t; k < nb; k += kTile) s_bin[k] += base;
@@ -628,12 +630,18 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     const uint32_t li0 = (tile0 - P.tile0) * kTile + t;
 
-    // Phase 1: the loads that do not need the step counter, for every tile.
+    // Phase 1: every load of the tile(s).  The step counter comes first: the
+    // ring slot's address depends on it, and the slot's read then overlaps
+    // the state loads.
+    const uint64_t step = *a.step;
+    float* ring_slot = a.ring + static_cast<size_t>(step & (kSlots - 1)) * a.N + P.off;
     Neuron<MODEL> nrn[kMaxGroup];
     uint32_t row0[kMaxGroup], deg[kMaxGroup];
+    float cur[kMaxGroup];
 #pragma unroll
     for (uint32_t g = 0; g < kMaxGroup; ++g) {
         row0[g] = deg[g] = 0u;
+        cur[g] = 0.0f;
         const uint32_t li = li0 + g * kTile;
         if (g < ntile && li < P.n) {
             nrn[g].load(P, li);
@@ -641,24 +649,18 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
                 row0[g] = a.row_ptr[P.off + li];
                 deg[g] = a.row_ptr[P.off + li + 1] - row0[g];
             }
+            cur[g] = ring_slot[li];
         }
     }
-    const uint64_t step = *a.step;
     if (a.n_bins && tile0 == 0 && t == 0)  // the slot bin_apply fills
         *a.bin_slot = step < a.step_limit ? static_cast<uint32_t>((step + 1) & (kSlots - 1)) : kEmpty;
     if (step >= a.step_limit) return;
     if (fused_bin)
         for (uint32_t k = t; k < a.n_bins; k += kTile) s_bin[k] = 0u;
-    float* ring_slot = a.ring + static_cast<size_t>(step & (kSlots - 1)) * a.N + P.off;
-    float cur[kMaxGroup];
 #pragma unroll
     for (uint32_t g = 0; g < kMaxGroup; ++g) {
         const uint32_t li = li0 + g * kTile;
-        cur[g] = 0.0f;
-        if (g < ntile && li < P.n) {
-            cur[g] = ring_slot[li];
-            ring_slot[li] = 0.0f;
-        }
+        if (g < ntile && li < P.n) ring_slot[li] = 0.0f;
     }
 
     // Phase 2: advance every neuron; per tile ballot the spikes and scan the
@@ -1143,9 +1145,10 @@ __global__ __launch_bounds__(kApply) void bin_apply_kernel(const NetArgs* __rest
     // bin k's {start, count} from every writer, in writer order: writers
     // 2t and 2t+1 (rows hold an even number of writers)
     static_assert(kMaxWriters == 2 * kApply, "two writers per thread");
+    uint4* hrow = reinterpret_cast<uint4*>(a.bin_hdr + static_cast<size_t>(k) * a.bin_rows);
     uint4 hw = make_uint4(0u, 0u, 0u, 0u);
-    if (2 * t < nw) hw = reinterpret_cast<const uint4*>(a.bin_hdr + static_cast<size_t>(k) * a.bin_rows)[t];
-    if (2 * t + 1 >= nw) hw.w = 0u;
+    if (2 * t < nw) hw = hrow[t];
+    if (hw.y | hw.w) hrow[t] = make_uint4(0u, 0u, 0u, 0u);  // consumed: empty for the next step
 #pragma unroll
     for (uint32_t pr = 0; pr < kBinPosts / kApply; ++pr) {
         L.slice[t + pr * kApply] = 0.0f;
