@@ -230,8 +230,10 @@ uint64_t generated_bytes(const simuse_gen_desc& g) {
     const uint64_t n = g.n, ns = n * g.k;
     const uint64_t tiles = (n + kTile - 1) / kTile + 2;
     uint64_t b = ns * (4 + 4 + 1);                           // col, w, delay
-    b += ns * (8 + 8);                                       // event records, collision lists
-    b += n * 4 * 2 + n * 32 * (4 + 8);                       // row_ptr, ring, tagged heads
+    if (g.d_max <= 1)
+        b += ns * 32 + n * 4 * 2 + n * 32 * 4;               // binned records (two buffers), row_ptr, ring
+    else
+        b += ns * (8 + 8) + n * 4 * 2 + n * 32 * (4 + 8);    // records, collision lists, row_ptr, ring, heads
     const uint64_t cols = static_cast<uint64_t>(model_states(g.model) + model_params(g.model));
     b += n * 4 * cols + tiles * (kTile * 4 + 4 + 8 + 8 + 8);  // state/params, compaction, counters
     return b + (1u << 20);
@@ -365,6 +367,10 @@ int simuse_batch_set_noise(simuse_batch* b, int32_t id, int32_t pop, const float
 
 int simuse_batch_set_plasticity(simuse_batch* b, int32_t id, const simuse_stdp_params* p, const uint8_t* plastic) {
     BATCH_NET(b, id, s);
+    if (p) {
+        int rl = b->eng.leave_megastep(s);
+        if (rl) return rl;
+    }
     HIP_TRY(hipStreamSynchronize(b->eng.stream));
     int rc = s->set_plasticity(p, plastic, b->eng.stream);
     b->eng.mark_dirty();

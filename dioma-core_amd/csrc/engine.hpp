@@ -38,7 +38,8 @@ void launch_stdp(const NetArgs* nets, const uint2* tiles, uint32_t n, hipStream_
 void launch_insert(int sched, const NetArgs* nets, const uint2* blocks, uint32_t n, hipStream_t s);
 void launch_fixup(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s);
 void launch_bin(const NetArgs* nets, const uint4* refs, uint32_t n, uint32_t bins, hipStream_t s);
-void launch_bin_apply(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s);
+void launch_bin_apply(const NetArgs* nets, const uint2* refs, uint32_t n, uint32_t mode, hipStream_t s);
+void launch_megastep(int model, const NetArgs* nets, const uint2* items, uint32_t n, hipStream_t s);
 
 void launch_head_clear(unsigned long long* head, const uint64_t* step, size_t n, hipStream_t s);
 void launch_inject(float* ring, const uint64_t* step, uint32_t N, const uint32_t* idx, const float* val,
@@ -119,6 +120,17 @@ struct NetState {
     bool binned = false;
     uint32_t n_bins = 0, bin_rows = 0;
     uint2* bin_hdr = nullptr;
+    // megastep (binned, one model, no pair-based plasticity): records and
+    // headers double-buffered by step parity
+    bool mega_ok = false;   // buffers for it exist
+    bool no_mega = false;   // left for the two-kernel path (dense STDP step, plasticity)
+    bool mega = false;      // running it (set with the work lists)
+    int mega_model = 0;
+    uint32_t* bin_done = nullptr;
+    uint32_t* bin_mat = nullptr;
+    std::vector<uint2> mat_items;      // (engine index, bin) of every bin, XCD-ordered
+    uint2* d_mat_items = nullptr;      // [n_bins + kXcds]
+    bool mat_dirty = true;
     std::vector<uint4> dense_writers;  // every writer as a bin_kernel item (dense STDP step)
     uint4* d_dense_writers = nullptr;  // [bin_rows + kXcds]
     bool dense_dirty = true;
@@ -217,6 +229,11 @@ struct NetState {
         a.n_bins = binned ? n_bins : 0u;
         a.bin_hdr = bin_hdr;
         a.bin_rows = bin_rows;
+        a.n_pops = static_cast<uint32_t>(pops.size());
+        a.n_ev = n_syn + n_src_edges;
+        a.hdr_stride = static_cast<uint64_t>(n_bins) * bin_rows;
+        a.bin_done = bin_done;
+        a.bin_mat = bin_mat;
         a.bin_ev = bin_ev;
         a.bin_slot = bin_slot;
         a.head = head;
@@ -307,8 +324,10 @@ struct Engine {
     NetArgs* d_nets = nullptr;
     uint32_t nets_cap = 0;
     static constexpr int kModels = 4;  // LIF, Izhikevich, HH, GPEA
-    uint4* d_tiles[kModels] = {};      // neuron work items per model: {net, tile, tiles, pop}
+    uint4* d_tiles[kModels] = {};      // neuron work items per model: {net, tile, tiles | writer << 4, pop}
     uint32_t n_tiles_m[kModels] = {}, cap_tiles_m[kModels] = {};
+    uint2* d_mega[kModels] = {};       // megastep work items per model: {net, bin}
+    uint32_t n_mega_m[kModels] = {}, cap_mega_m[kModels] = {};
     uint2* d_blocks = nullptr;  // insert_kernel work: plastic networks' tiles, source tiles
     uint32_t n_blocks = 0, cap_blocks = 0;
     uint2* d_stdp = nullptr;
@@ -340,6 +359,8 @@ struct Engine {
         drop_graph();
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         for (uint4* t : d_tiles)
+            if (t) (void)hipFree(t);
+        for (uint2* t : d_mega)
             if (t) (void)hipFree(t);
         for (void* p : {static_cast<void*>(d_nets), static_cast<void*>(d_blocks), static_cast<void*>(d_stdp),
                         static_cast<void*>(d_fix), static_cast<void*>(d_bin), static_cast<void*>(d_apply)})
@@ -398,7 +419,7 @@ struct Engine {
         HIP_TRY(hipStreamSynchronize(stream));
         std::vector<NetArgs> na;
         std::vector<uint4> tl[kModels], binl;
-        std::vector<uint2> bl, sl, fl, apl;
+        std::vector<uint2> bl, sl, fl, apl, ml[kModels];
         bins_fused = bins_bin = 0;
         for (uint32_t k = 0; k < nets.size(); ++k) {
             NetState* s = nets[k];
@@ -410,6 +431,26 @@ struct Engine {
             const uint32_t ns = (s->n_src_tiles + kBinBlockTiles - 1) / kBinBlockTiles;
             // binned networks without pair-based plasticity: neuron block j is writer j
             const bool fz = s->binned && !s->stdp && ng + ns <= kMaxWriters;
+            // megastep: bin b's block is writer b, the source blocks follow
+            s->mega = s->binned && s->mega_ok && !s->no_mega && !s->stdp;
+            if (s->mega) {
+                s->mat_items.clear();
+                for (uint32_t b = 0; b < s->n_bins; ++b) {
+                    ml[s->mega_model].push_back(make_uint2(k, b));
+                    s->mat_items.push_back(make_uint2(k, b));
+                }
+                xcd_order(s->mat_items, make_uint2(kEmpty, 0u));
+                s->mat_dirty = true;
+                uint32_t jw = s->n_bins;
+                for (uint32_t b = s->n_tiles; b < n_blk; b += kBinBlockTiles, ++jw)
+                    binl.push_back(make_uint4(k, b, std::min(kBinBlockTiles, n_blk - b), jw));
+                if (jw > s->bin_rows) return fail(SIMUSE_E_STATE, "megastep: %u writers, %u rows", jw, s->bin_rows);
+                A.n_bblk = jw;
+                A.mega = 1;
+                A.bin_fused = 0;
+                if (s->n_src_tiles) bins_bin = std::max(bins_bin, s->n_bins);
+                continue;
+            }
             uint32_t j = 0;
             for (uint32_t p = 0; p < s->pops.size(); ++p) {
                 const HostPop& P = s->pops[p];
@@ -460,6 +501,12 @@ struct Engine {
             HIP_TRY(hipMemcpy(d_nets, na.data(), sizeof(NetArgs) * na.size(), hipMemcpyHostToDevice));
         }
         for (int m = 0; m < kModels; ++m) {
+            xcd_order(ml[m], make_uint2(kEmpty, 0u));
+            n_mega_m[m] = static_cast<uint32_t>(ml[m].size());
+            if (!ml[m].empty()) {
+                if ((rc = grow(&d_mega[m], &cap_mega_m[m], n_mega_m[m]))) return rc;
+                HIP_TRY(hipMemcpy(d_mega[m], ml[m].data(), sizeof(uint2) * ml[m].size(), hipMemcpyHostToDevice));
+            }
             xcd_order(tl[m], make_uint4(kEmpty, 0u, 0u, 0u));
             n_tiles_m[m] = static_cast<uint32_t>(tl[m].size());
             if (tl[m].empty()) continue;
@@ -500,6 +547,7 @@ struct Engine {
 
     void launch_step(hipEvent_t* e) {
         if (e) (void)hipEventRecord(e[0], stream);
+        for (int m = 0; m < kModels; ++m) launch_megastep(m, d_nets, d_mega[m], n_mega_m[m], stream);
         for (int m = 0; m < kModels; ++m)
             launch_neuron(m, sched, d_nets, d_tiles[m], n_tiles_m[m], stream, 0u, bins_fused);
         if (e) (void)hipEventRecord(e[1], stream);
@@ -508,7 +556,7 @@ struct Engine {
         launch_insert(sched, d_nets, d_blocks, n_blocks, stream);
         launch_bin(d_nets, d_bin, n_bin, bins_bin, stream);
         if (e) (void)hipEventRecord(e[3], stream);
-        launch_bin_apply(d_nets, d_apply, n_apply, stream);
+        launch_bin_apply(d_nets, d_apply, n_apply, 0u, stream);
         launch_fixup(d_nets, direct_fix ? nullptr : d_fix, n_fix, stream);
         if (e) (void)hipEventRecord(e[4], stream);
     }
@@ -535,6 +583,34 @@ struct Engine {
         if (e != hipSuccess) {
             drop_graph();
             return fail(SIMUSE_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+        }
+        return SIMUSE_OK;
+    }
+
+    // A megastep network keeps its pending events (the last step's records)
+    // out of the ring; apply them into ring slot step % 32 before anything
+    // reads or adds to that slot from the host side.  Idempotent.
+    int materialize(NetState* s) {
+        int rc;
+        if ((rc = ensure_tables())) return rc;
+        if (!s->finalized || !s->mega) return SIMUSE_OK;
+        if (s->mat_dirty) {
+            HIP_TRY(hipMemcpy(s->d_mat_items, s->mat_items.data(), sizeof(uint2) * s->mat_items.size(),
+                              hipMemcpyHostToDevice));
+            s->mat_dirty = false;
+        }
+        launch_bin_apply(d_nets, s->d_mat_items, static_cast<uint32_t>(s->mat_items.size()), 1u, stream);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(s->bin_mat), 1, 1, stream));
+        return SIMUSE_OK;
+    }
+    // Move a megastep network to the two-kernel path for good.
+    int leave_megastep(NetState* s) {
+        int rc;
+        if ((rc = materialize(s))) return rc;
+        if (s->mega_ok && !s->no_mega) {
+            s->no_mega = true;
+            mark_dirty();
         }
         return SIMUSE_OK;
     }

@@ -158,17 +158,33 @@ int NetState::finalize(hipStream_t stream) {
     const uint32_t n_blk = n_tiles + n_src_tiles;
     // Every delay <= 1 step (synapses and sources): the binned scatter;
     // otherwise the chained one with its tagged heads and event records.
-    binned = max_delay <= 1 && N <= kBinPosts * kMaxBins && n_blk > 0;
+    const uint32_t n_bins_t = (n_tiles + kBinTiles - 1) / kBinTiles;
+    bool one_model = true;
+    for (const HostPop& P : pops) one_model = one_model && P.model == pops[0].model;
+    binned = max_delay <= 1 && n_tiles > 0 && n_bins_t <= kMaxBins && pops.size() <= kMaxBinPops;
     if (binned) {
-        n_bins = (N + kBinPosts - 1) / kBinPosts;
+        n_bins = n_bins_t;
+        const uint32_t ns = (n_src_tiles + kBinBlockTiles - 1) / kBinBlockTiles;
         // writers: at most one per neuron tile plus one per kBinBlockTiles source tiles
-        bin_rows = std::min(kMaxWriters, n_tiles + (n_src_tiles + kBinBlockTiles - 1) / kBinBlockTiles);
-        bin_rows += bin_rows & 1u;  // even: bin_apply reads two writers' headers at once
-        if ((rc = alloc(&bin_ev, n_ev))) return rc;
-        if ((rc = alloc(&bin_hdr, static_cast<size_t>(n_bins) * bin_rows))) return rc;
-        HIP_TRY(hipMemsetAsync(bin_hdr, 0, sizeof(uint2) * n_bins * bin_rows, stream));
+        bin_rows = std::min(kMaxWriters, n_tiles + ns);
+        bin_rows += bin_rows & 1u;  // even: consumers read two writers' headers at once
+        // one kernel per step needs one model, no pair-based plasticity and a
+        // writer per bin
+        mega_ok = one_model && !stdp && n_bins + ns <= kMaxWriters;
+        mega_model = pops[0].model;
+        const size_t nbuf = mega_ok ? 2 : 1;
+        if ((rc = alloc(&bin_ev, nbuf * n_ev))) return rc;
+        if ((rc = alloc(&bin_hdr, nbuf * n_bins * bin_rows))) return rc;
+        HIP_TRY(hipMemsetAsync(bin_hdr, 0, sizeof(uint2) * nbuf * n_bins * bin_rows, stream));
         if ((rc = alloc(&d_dense_writers, bin_rows + kXcds))) return rc;  // XCD-ordered, padded
+        if ((rc = alloc(&d_mat_items, n_bins + kXcds))) return rc;
         if ((rc = alloc(&bin_slot, 1))) return rc;
+        if ((rc = alloc(&bin_done, 1))) return rc;
+        if ((rc = alloc(&bin_mat, 1))) return rc;
+        HIP_TRY(hipMemsetAsync(bin_done, 0, sizeof(uint32_t), stream));
+        // before the first step the ring holds the step's input (zeros, or
+        // currents injected so far): the megastep reads it from there
+        HIP_TRY(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(bin_mat), 1, 1, stream));
     } else {
         if ((rc = alloc(&head, static_cast<size_t>(kSlots) * N))) return rc;
         if ((rc = alloc(&rec, n_ev))) return rc;
@@ -294,7 +310,10 @@ int NetState::reset_dynamic(hipStream_t stream) {
     HIP_TRY(hipMemsetAsync(step, 0, sizeof(uint64_t) * 2, stream));
     clear_heads(stream);
     if (tile_coll) HIP_TRY(hipMemsetAsync(tile_coll, 0, sizeof(uint32_t) * (n_tiles + n_src_tiles), stream));
-    if (bin_hdr) HIP_TRY(hipMemsetAsync(bin_hdr, 0, sizeof(uint2) * n_bins * bin_rows, stream));
+    if (bin_hdr)
+        HIP_TRY(hipMemsetAsync(bin_hdr, 0, sizeof(uint2) * (mega_ok ? 2 : 1) * n_bins * bin_rows, stream));
+    if (bin_done) HIP_TRY(hipMemsetAsync(bin_done, 0, sizeof(uint32_t), stream));
+    if (bin_mat) HIP_TRY(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(bin_mat), 1, 1, stream));  // zeroed ring
     HIP_TRY(hipMemsetAsync(tile_count, 0, sizeof(uint32_t) * n_tiles, stream));
     HIP_TRY(hipMemsetAsync(bits, 0, sizeof(uint64_t) * n_words, stream));
     HIP_TRY(hipMemsetAsync(tile_spikes, 0, sizeof(uint64_t) * n_tiles, stream));
@@ -518,6 +537,8 @@ int NetState::download_state(int32_t pop, float* out, uint64_t len, hipStream_t 
 
 int NetState::download_ring(float* out, uint64_t len, hipStream_t stream) {
     if (!out || len != static_cast<uint64_t>(kSlots) * N) return fail(SIMUSE_E_INVALID, "ring len must be 32*n_total");
+    int rc;
+    if (eng && (rc = eng->materialize(this))) return rc;
     HIP_TRY(hipMemcpyAsync(out, ring, 4 * len, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     return SIMUSE_OK;
@@ -563,6 +584,7 @@ int NetState::inject(int32_t pop, const uint32_t* idx, const float* val, uint32_
     std::sort(sorted.begin(), sorted.end());
     const bool dup = std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end();
     int rc;
+    if (eng && (rc = eng->materialize(this))) return rc;  // injection adds to the slot's event sums
     HIP_TRY(hipStreamSynchronize(stream));
     if (count > inj_cap) {
         // grow the staging buffers (old ones stay owned by the network)
@@ -723,9 +745,13 @@ int Engine::autotune(uint32_t trial_steps) {
     int rc = prepare();
     if (rc) return rc;
     if (trial_steps == 0) trial_steps = 16;
+    // megastep networks: the pending events into the ring, which the snapshot keeps
+    for (NetState* s : nets)
+        if ((rc = materialize(s))) return rc;
     HIP_TRY(hipStreamSynchronize(stream));
     Snapshot snap;
     for (NetState* s : nets) {
+        if (s->bin_mat && (rc = snap.add(s->bin_mat, sizeof(uint32_t)))) return rc;
         for (auto& P : s->pops)
             if ((rc = snap.add(P.st, sizeof(float) * P.ns * P.dstride))) return rc;
         const size_t N = s->N;
@@ -758,7 +784,13 @@ int Engine::autotune(uint32_t trial_steps) {
     auto restore = [&]() {
         int r = snap.copy(stream, false);
         if (r) return r;
-        for (NetState* s : nets) s->clear_heads(stream);
+        for (NetState* s : nets) {
+            s->clear_heads(stream);
+            if (s->bin_hdr)  // the trial steps' pending events
+                HIP_TRY(hipMemsetAsync(s->bin_hdr, 0, sizeof(uint2) * (s->mega_ok ? 2 : 1) * s->n_bins * s->bin_rows,
+                                       stream));
+            if (s->bin_done) HIP_TRY(hipMemsetAsync(s->bin_done, 0, sizeof(uint32_t), stream));
+        }
         HIP_TRY(hipGetLastError());
         return SIMUSE_OK;
     };
@@ -905,6 +937,7 @@ int simuse_net_set_poisson_drive(simuse_net* net, int32_t pop, float rate_hz, fl
 int simuse_net_set_plasticity(simuse_net* net, const simuse_stdp_params* p, const uint8_t* plastic) {
     int rc = check_net(net);
     if (rc) return rc;
+    if (p && (rc = net->eng.leave_megastep(&net->st))) return rc;
     rc = net->st.set_plasticity(p, plastic, net->eng.stream);
     net->eng.mark_dirty();
     return rc;
@@ -986,6 +1019,8 @@ int simuse_net_stdp_step(simuse_net* net, float trace_decay, float lr_ltp, float
     Engine& e = net->eng;
     if (s.stdp) return fail(SIMUSE_E_STATE, "stdp_step: pair-based plasticity is enabled on this network");
     if ((rc = e.prepare())) return rc;
+    if ((rc = e.leave_megastep(&s))) return rc;  // this step's update, weights and push are separate kernels
+    if ((rc = e.prepare())) return rc;
     const uint32_t n_blk = s.n_tiles + s.n_src_tiles;
     if (!s.dtrace) {
         if ((rc = s.alloc(&s.dtrace, s.N))) return rc;
@@ -1013,7 +1048,7 @@ int simuse_net_stdp_step(simuse_net* net, float trace_decay, float lr_ltp, float
             s.dense_dirty = false;
         }
         launch_bin(e.d_nets, s.d_dense_writers, static_cast<uint32_t>(s.dense_writers.size()), s.n_bins, e.stream);
-        launch_bin_apply(e.d_nets, e.d_apply, e.n_apply, e.stream);
+        launch_bin_apply(e.d_nets, e.d_apply, e.n_apply, 0u, e.stream);
     } else {
         launch_insert(e.sched, e.d_nets, s.d_all_blocks, n_blk, e.stream);
         launch_fixup(e.d_nets, nullptr, s.n_fix, e.stream);

@@ -428,31 +428,59 @@ struct Neuron<2> {  // Hodgkin-Huxley: reads and writes its columns inside the u
 
 // ---------------------------------------------------------------------------
 // Binned scatter for networks whose delays are all <= 1 step: every event of
-// step s lands in slot (s+1) % 32, whose cells are all zero (consumed and
-// cleared 31 steps ago).  Posts are grouped in bins of kBinPosts.  A
+// step s lands in slot (s+1) % 32.  A bin is kBinTiles consecutive tiles of
+// the network (tiles never straddle populations); an event's cell is its
+// post's tile within the bin * kTile + the post's index within the tile.  A
 // network's scatter tiles are split among at most kMaxWriters writers, in
-// ascending tile (so event id) order: the neuron kernel's blocks when the
-// network has no pair-based plasticity, else stand-alone bin_kernel blocks;
-// source tiles always go to bin_kernel blocks after them.  Per step:
-//   writer j   counts its events per bin in LDS, writes them grouped by bin
-//              from its first edge id (regions never overlap) as {event id,
-//              weight, post within bin}, and writes column j of bin_hdr:
-//              {first record, count} of each bin's group;
-//   bin_apply  block k reads bin k's group from every writer, in writer order
-//              (ascending event id), orders each post's records by event id in
-//              LDS, adds them into an LDS copy of the bin's slot slice and
-//              writes the slice out whole.  Its work list is permuted so that
-//              each XCD takes a contiguous range of bins, whose record lines
-//              then meet in that XCD's L2.
-// Sums per cell run in ascending event id from zero: the sequential push.  No
-// global atomics, no random ring traffic.
+// ascending tile (so event id) order; writer j counts its events per bin in
+// LDS, writes them grouped by bin from its first edge id (regions never
+// overlap) as {event id, weight, cell}, and writes column j of bin_hdr:
+// {first record, count} of each bin's group.  A consumer of bin k reads bin
+// k's group of every writer in writer order (ascending event id), orders each
+// cell's records by event id in LDS and adds them from zero: the sequential
+// push.  Two layouts share this:
+//   megastep   one kernel per step: block (network, bin k) gathers the
+//              previous step's events of bin k's cells, advances those
+//              neurons with them and writes its own spikes' events; records
+//              and headers alternate between two buffers by step parity, and
+//              the network's last block to finish advances the step counter;
+//   two-kernel the writers are the neuron kernel's blocks (or bin_kernel after
+//              the STDP kernels), then bin_apply writes each bin's slice of
+//              ring slot (s+1) % 32 whole and advances the step.
+// No global atomics beyond the megastep's block count, no random ring
+// traffic.
 // ---------------------------------------------------------------------------
 
-// Exclusive scan of n <= kTile*per entries held in LDS (in place); returns the total.
-template <uint32_t kPer>
-__device__ __forceinline__ uint32_t lds_exclusive_scan(uint32_t* v, uint32_t n, uint32_t* s_wave) {
-    const uint32_t t = threadIdx.x;
-    const uint32_t lo = t * kPer;
+// Exclusive prefix sum over a block of kThreads threads; returns the total.
+template <uint32_t kThreads>
+__device__ __forceinline__ uint32_t block_scan(uint32_t x, uint32_t* s_wave, uint32_t& excl) {
+    constexpr uint32_t kW = kThreads / 64;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t inc = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o, 64);
+        if (lane >= static_cast<uint32_t>(o)) inc += y;
+    }
+    if (lane == 63) s_wave[wave] = inc;
+    __syncthreads();
+    uint32_t base = 0, total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kW; ++w) {
+        const uint32_t v = s_wave[w];
+        if (w < wave) base += v;
+        total += v;
+    }
+    excl = base + inc - x;
+    __syncthreads();
+    return total;
+}
+
+// Exclusive scan of n <= kThreads*kPer entries held in LDS (in place), thread
+// t owning entries [t*kPer, (t+1)*kPer); returns the total.
+template <uint32_t kThreads, uint32_t kPer>
+__device__ __forceinline__ uint32_t lds_scan(uint32_t* v, uint32_t n, uint32_t* s_wave) {
+    const uint32_t lo = threadIdx.x * kPer;
     uint32_t loc[kPer];
     uint32_t sum = 0;
 #pragma unroll
@@ -461,7 +489,7 @@ __device__ __forceinline__ uint32_t lds_exclusive_scan(uint32_t* v, uint32_t n, 
         sum += loc[i];
     }
     uint32_t excl;
-    const uint32_t total = block_exclusive_scan(sum, s_wave, excl);
+    const uint32_t total = block_scan<kThreads>(sum, s_wave, excl);
 #pragma unroll
     for (uint32_t i = 0; i < kPer; ++i) {
         if (lo + i < n) v[lo + i] = excl;
@@ -469,6 +497,11 @@ __device__ __forceinline__ uint32_t lds_exclusive_scan(uint32_t* v, uint32_t n, 
     }
     __syncthreads();
     return total;
+}
+
+template <uint32_t kPer>
+__device__ __forceinline__ uint32_t lds_exclusive_scan(uint32_t* v, uint32_t n, uint32_t* s_wave) {
+    return lds_scan<kTile, kPer>(v, n, s_wave);
 }
 
 // Largest index i in [lo0, hi) with v[i] <= x, given v[lo0] <= x and v
@@ -483,121 +516,166 @@ __device__ __forceinline__ uint32_t floor_index(const uint32_t* v, uint32_t lo0,
     return lo;
 }
 
+// The network's population table in LDS: first neuron and first tile.
+struct BinMap {
+    uint32_t off[kMaxBinPops];
+    uint32_t tile0[kMaxBinPops];
+};
+
+// Every thread of the block calls it; the table is valid after the caller's
+// next barrier.
+__device__ __forceinline__ void bin_map_load(const NetArgs& a, BinMap& M) {
+    for (uint32_t p = threadIdx.x; p < a.n_pops; p += blockDim.x) {
+        M.off[p] = a.pops[p].off;
+        M.tile0[p] = a.pops[p].tile0;
+    }
+}
+
+// (bin << 11) | cell of global post `post`.
+__device__ __forceinline__ uint32_t post_bin_cell(const BinMap& M, uint32_t n_pops, uint32_t post) {
+    uint32_t p = 0;  // last population starting at or before post
+#pragma unroll
+    for (uint32_t s = kMaxBinPops / 2; s > 0; s >>= 1)
+        if (p + s < n_pops && M.off[p + s] <= post) p += s;
+    const uint32_t r = post - M.off[p];
+    const uint32_t tile = M.tile0[p] + r / kTile;
+    return ((tile / kBinTiles) << 11) | ((tile % kBinTiles) * kTile + r % kTile);
+}
+
 // Count (kWrite false) or write out (kWrite true) the events of `cnt` staged
-// rows (cnt <= 1024): row k starts at edge s_rs[k] and holds concatenation
+// rows (cnt <= 2048): row k starts at edge s_rs[k] and holds concatenation
 // positions [s_pref[k], s_pref[k+1]).  Counting adds 1 to s_bin[bin]; writing
-// puts the event at record s_bin[bin] (absolute in bin_ev) and advances it.
-// Four events per thread are loaded before any is used.
-template <bool kWrite>
-__device__ __forceinline__ void bin_events(const NetArgs& a, bool is_src, uint32_t cnt, uint32_t total,
-                                           const uint32_t* s_rs, const uint32_t* s_pref, uint32_t* s_bin,
-                                           uint64_t step) {
+// puts the event at record s_bin[bin] of `ev` and advances it.  Four events
+// per thread are loaded before any is used.
+template <bool kWrite, uint32_t kThreads>
+__device__ __forceinline__ void bin_events(const NetArgs& a, const BinMap& M, bool is_src, uint32_t cnt,
+                                           uint32_t total, const uint32_t* s_rs, const uint32_t* s_pref,
+                                           uint32_t* s_bin, uint4* ev, uint64_t step) {
     constexpr uint32_t kU = 4;
     const uint32_t* col = is_src ? a.src_post : a.col;
     const float* w = is_src ? a.src_w : a.w;
-    const uint32_t eb = is_src ? static_cast<uint32_t>(a.n_syn) : 0u;
-    for (uint32_t f0 = threadIdx.x; f0 < total; f0 += kU * kTile) {
+    const uint32_t eb = is_src ? static_cast<uint32_t>(a.n_syn) : 0u, np = a.n_pops;
+    for (uint32_t f0 = threadIdx.x; f0 < total; f0 += kU * kThreads) {
         uint32_t e[kU], post[kU];
         float wt[kU];
 #pragma unroll
         for (uint32_t u = 0; u < kU; ++u) {
-            const uint32_t f = f0 + u * kTile;
+            const uint32_t f = f0 + u * kThreads;
             e[u] = 0u;
             if (f < total) {
-                const uint32_t k = floor_index<1024>(s_pref, 0, cnt, f);
+                const uint32_t k = floor_index<2048>(s_pref, 0, cnt, f);
                 e[u] = s_rs[k] + (f - s_pref[k]);
             }
         }
 #pragma unroll
         for (uint32_t u = 0; u < kU; ++u) {
-            if (f0 + u * kTile < total) {
+            if (f0 + u * kThreads < total) {
                 post[u] = col[e[u]];
                 if (kWrite) wt[u] = w[e[u]];
             }
         }
 #pragma unroll
         for (uint32_t u = 0; u < kU; ++u) {
-            if (f0 + u * kTile >= total) continue;
-            const uint32_t bin = post[u] >> kBinLog2;
+            if (f0 + u * kThreads >= total) continue;
+            const uint32_t bc = post_bin_cell(M, np, post[u]);
             if (!kWrite) {
-                atomicAdd(&s_bin[bin], 1u);
+                atomicAdd(&s_bin[bc >> 11], 1u);
             } else {
                 const float x = a.tm_type ? tm_deliver(a, eb + e[u], wt[u], step) : wt[u];
-                const uint32_t pos = atomicAdd(&s_bin[bin], 1u);
-                a.bin_ev[pos] = make_uint4(eb + e[u], __float_as_uint(x), post[u] & (kBinPosts - 1u), 0u);
+                const uint32_t pos = atomicAdd(&s_bin[bc >> 11], 1u);
+                ev[pos] = make_uint4(eb + e[u], __float_as_uint(x), bc & 2047u, 0u);
             }
         }
     }
 }
 
 // The block's events when there are at most kEvCache per thread: loaded once
-// (edge id, post, weight) and held in registers from the count to the write.
+// (edge id, bin and cell, weight) and held in registers from the count to
+// the write.
 constexpr uint32_t kEvCache = 4;
 struct EvCache {
-    uint32_t e[kEvCache], post[kEvCache];
+    uint32_t e[kEvCache], bc[kEvCache];
     float w[kEvCache];
 };
 
-__device__ __forceinline__ void ev_cache_load(const NetArgs& a, bool is_src, uint32_t cnt, uint32_t total,
-                                              const uint32_t* s_rs, const uint32_t* s_pref, EvCache& c) {
+template <uint32_t kThreads>
+__device__ __forceinline__ void ev_cache_load(const NetArgs& a, const BinMap& M, bool is_src, uint32_t cnt,
+                                              uint32_t total, const uint32_t* s_rs, const uint32_t* s_pref,
+                                              EvCache& c) {
     const uint32_t* col = is_src ? a.src_post : a.col;
     const float* w = is_src ? a.src_w : a.w;
+    uint32_t post[kEvCache];
 #pragma unroll
     for (uint32_t u = 0; u < kEvCache; ++u) {
-        const uint32_t f = threadIdx.x + u * kTile;
+        const uint32_t f = threadIdx.x + u * kThreads;
         c.e[u] = 0u;
         if (f < total) {
-            const uint32_t k = floor_index<1024>(s_pref, 0, cnt, f);
+            const uint32_t k = floor_index<2048>(s_pref, 0, cnt, f);
             c.e[u] = s_rs[k] + (f - s_pref[k]);
         }
     }
 #pragma unroll
     for (uint32_t u = 0; u < kEvCache; ++u) {
-        if (threadIdx.x + u * kTile < total) {
-            c.post[u] = col[c.e[u]];
+        if (threadIdx.x + u * kThreads < total) {
+            post[u] = col[c.e[u]];
             c.w[u] = w[c.e[u]];
         }
     }
+#pragma unroll
+    for (uint32_t u = 0; u < kEvCache; ++u)
+        if (threadIdx.x + u * kThreads < total) c.bc[u] = post_bin_cell(M, a.n_pops, post[u]);
 }
 
+template <uint32_t kThreads>
 __device__ __forceinline__ void ev_cache_count(const EvCache& c, uint32_t total, uint32_t* s_bin) {
 #pragma unroll
     for (uint32_t u = 0; u < kEvCache; ++u)
-        if (threadIdx.x + u * kTile < total) atomicAdd(&s_bin[c.post[u] >> kBinLog2], 1u);
+        if (threadIdx.x + u * kThreads < total) atomicAdd(&s_bin[c.bc[u] >> 11], 1u);
 }
 
+template <uint32_t kThreads>
 __device__ __forceinline__ void ev_cache_write(const NetArgs& a, bool is_src, const EvCache& c, uint32_t total,
-                                               uint32_t* s_bin, uint64_t step) {
+                                               uint32_t* s_bin, uint4* ev, uint64_t step) {
     const uint32_t eb = is_src ? static_cast<uint32_t>(a.n_syn) : 0u;
 #pragma unroll
     for (uint32_t u = 0; u < kEvCache; ++u) {
-        if (threadIdx.x + u * kTile >= total) continue;
+        if (threadIdx.x + u * kThreads >= total) continue;
         const uint32_t e = eb + c.e[u];
         const float x = a.tm_type ? tm_deliver(a, e, c.w[u], step) : c.w[u];
-        const uint32_t pos = atomicAdd(&s_bin[c.post[u] >> kBinLog2], 1u);
-        a.bin_ev[pos] = make_uint4(e, __float_as_uint(x), c.post[u] & (kBinPosts - 1u), 0u);
+        const uint32_t pos = atomicAdd(&s_bin[c.bc[u] >> 11], 1u);
+        ev[pos] = make_uint4(e, __float_as_uint(x), c.bc[u] & 2047u, 0u);
     }
 }
 
 // Writer j's per-bin counts (s_bin) become the absolute first record of each
-// bin's group, from `base`; column j of bin_hdr gets {start, count} for each
-// bin it sends events to (bin_apply zeroes the entries it consumes, so the
-// others read as empty).
-// Synchronises before and after.
+// bin's group, from `base`; column j of `hdr` gets {start, count} for each
+// bin it sends events to (consumers zero the entries they read, so the
+// others read as empty).  Synchronises before and after.
+template <uint32_t kThreads>
 __device__ __forceinline__ void bin_offsets(const NetArgs& a, uint32_t* s_bin, uint32_t* s_wave, uint32_t j,
-                                            uint32_t base, uint32_t events) {
+                                            uint32_t base, uint32_t events, uint2* hdr) {
     __syncthreads();
     const uint32_t nb = a.n_bins, t = threadIdx.x;
-    if (nb <= kTile) lds_exclusive_scan<1>(s_bin, nb, s_wave);
-    else if (nb <= 4 * kTile) lds_exclusive_scan<4>(s_bin, nb, s_wave);
-    else lds_exclusive_scan<kMaxBins / kTile>(s_bin, nb, s_wave);
-    for (uint32_t k = t; k < nb; k += kTile) {
+    if (nb <= kThreads) lds_scan<kThreads, 1>(s_bin, nb, s_wave);
+    else if (nb <= 4 * kThreads) lds_scan<kThreads, 4>(s_bin, nb, s_wave);
+    else lds_scan<kThreads, (kMaxBins + kThreads - 1) / kThreads>(s_bin, nb, s_wave);
+    for (uint32_t k = t; k < nb; k += kThreads) {
         const uint32_t o = s_bin[k], e = k + 1 < nb ? s_bin[k + 1] : events;
-        if (e != o) a.bin_hdr[static_cast<size_t>(k) * a.bin_rows + j] = make_uint2(base + o, e - o);
+        if (e != o) hdr[static_cast<size_t>(k) * a.bin_rows + j] = make_uint2(base + o, e - o);
     }
     __syncthreads();
-    for (uint32_t k = t; k < nb; k += kTile) s_bin[k] += base;
+    for (uint32_t k = t; k < nb; k += kThreads) s_bin[k] += base;
     __syncthreads();
+}
+
+// Megastep: the network's last block of the step (of n_bblk writers) moves
+// the step counter on and clears the block count and the ring flag.
+__device__ __forceinline__ void mega_done(const NetArgs& a, uint64_t step) {
+    if (atomicAdd(a.bin_done, 1u) + 1u == a.n_bblk) {
+        *a.bin_done = 0u;
+        *a.bin_mat = 0u;
+        *a.step = step + 1;
+    }
 }
 
 // One step of up to kMaxGroup consecutive tiles of one population: read ring
@@ -617,6 +695,7 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
     __shared__ uint32_t s_drv[kMaxGroup][kWavesPerTile];
     __shared__ uint32_t s_ncoll;
     __shared__ uint32_t s_bw[kWavesPerTile];
+    __shared__ BinMap s_map;
     extern __shared__ uint32_t s_bin[];  // [n_bins] of binned writers (dynamic)
     const uint4 it = items[blockIdx.x];
     if (it.x == kEmpty) return;  // pad of the XCD-ordered list
@@ -655,8 +734,10 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
     if (a.n_bins && tile0 == 0 && t == 0)  // the slot bin_apply fills
         *a.bin_slot = step < a.step_limit ? static_cast<uint32_t>((step + 1) & (kSlots - 1)) : kEmpty;
     if (step >= a.step_limit) return;
-    if (fused_bin)
+    if (fused_bin) {
         for (uint32_t k = t; k < a.n_bins; k += kTile) s_bin[k] = 0u;
+        bin_map_load(a, s_map);
+    }
 #pragma unroll
     for (uint32_t g = 0; g < kMaxGroup; ++g) {
         const uint32_t li = li0 + g * kTile;
@@ -764,14 +845,14 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
         const bool cached = blk_ed <= kEvCache * kTile;
         EvCache ec;
         if (cached) {
-            ev_cache_load(a, false, blk_sp, blk_ed, s_rs, s_pref, ec);
-            ev_cache_count(ec, blk_ed, s_bin);
+            ev_cache_load<kTile>(a, s_map, false, blk_sp, blk_ed, s_rs, s_pref, ec);
+            ev_cache_count<kTile>(ec, blk_ed, s_bin);
         } else {
-            bin_events<false>(a, false, blk_sp, blk_ed, s_rs, s_pref, s_bin, step);
+            bin_events<false, kTile>(a, s_map, false, blk_sp, blk_ed, s_rs, s_pref, s_bin, a.bin_ev, step);
         }
-        bin_offsets(a, s_bin, s_bw, wj, ebase, blk_ed);
-        if (cached) ev_cache_write(a, false, ec, blk_ed, s_bin, step);
-        else bin_events<true>(a, false, blk_sp, blk_ed, s_rs, s_pref, s_bin, step);
+        bin_offsets<kTile>(a, s_bin, s_bw, wj, ebase, blk_ed, a.bin_hdr);
+        if (cached) ev_cache_write<kTile>(a, false, ec, blk_ed, s_bin, a.bin_ev, step);
+        else bin_events<true, kTile>(a, s_map, false, blk_sp, blk_ed, s_rs, s_pref, s_bin, a.bin_ev, step);
         return;
     }
     if (!fused) return;
@@ -880,6 +961,7 @@ __global__ __launch_bounds__(kTile) void bin_kernel(const NetArgs* __restrict__ 
     __shared__ uint32_t s_srs[kTile], s_spref[kTile];
     __shared__ uint32_t s_tp[kTile];  // first spiking row of each neuron tile of the block
     __shared__ uint32_t s_wave[kWavesPerTile], s_c[kWavesPerTile];
+    __shared__ BinMap M;
     extern __shared__ uint32_t s_bin[];  // [n_bins] (dynamic)
     const uint4 ref = refs[blockIdx.x];
     if (ref.x == kEmpty) return;  // pad of the XCD-ordered list
@@ -889,7 +971,12 @@ __global__ __launch_bounds__(kTile) void bin_kernel(const NetArgs* __restrict__ 
     if (step >= a.step_limit) return;
     const uint32_t n_nt = b0 < a.n_tiles ? min(b1, a.n_tiles) - b0 : 0u;  // neuron tiles (<= kTile)
     const uint32_t base = a.tile_ebase[b0];
+    // megastep networks: this step's buffer of records and headers
+    const uint32_t par = a.mega ? static_cast<uint32_t>(step & 1u) : 0u;
+    uint4* ev = a.bin_ev + par * a.n_ev;
+    uint2* hdr = a.bin_hdr + par * a.hdr_stride;
     for (uint32_t k = t; k < a.n_bins; k += kTile) s_bin[k] = 0u;
+    bin_map_load(a, M);
     const uint32_t tc = t < n_nt ? a.tile_count[b0 + t] : 0u;
     uint32_t excl;
     const uint32_t n_rows = block_exclusive_scan(tc, s_wave, excl);
@@ -932,15 +1019,15 @@ __global__ __launch_bounds__(kTile) void bin_kernel(const NetArgs* __restrict__ 
                 events += total;
                 cached = one && total <= kEvCache * kTile;
                 if (cached) {
-                    ev_cache_load(a, false, nr, total, s_rs, s_pref, ec);
-                    ev_cache_count(ec, total, s_bin);
+                    ev_cache_load<kTile>(a, M, false, nr, total, s_rs, s_pref, ec);
+                    ev_cache_count<kTile>(ec, total, s_bin);
                 } else {
-                    bin_events<false>(a, false, nr, total, s_rs, s_pref, s_bin, step);
+                    bin_events<false, kTile>(a, M, false, nr, total, s_rs, s_pref, s_bin, ev, step);
                 }
             } else if (cached) {
-                ev_cache_write(a, false, ec, total, s_bin, step);
+                ev_cache_write<kTile>(a, false, ec, total, s_bin, ev, step);
             } else {
-                bin_events<true>(a, false, nr, total, s_rs, s_pref, s_bin, step);
+                bin_events<true, kTile>(a, M, false, nr, total, s_rs, s_pref, s_bin, ev, step);
             }
             if (!one) __syncthreads();  // before the next chunk is staged
         }
@@ -950,49 +1037,30 @@ __global__ __launch_bounds__(kTile) void bin_kernel(const NetArgs* __restrict__ 
             const uint32_t cnt = stage_tile_rows(a, b, step, s_srs, s_spref, s_wave, s_c, &total);
             if (pass == 0) {
                 events += total;
-                bin_events<false>(a, true, cnt, total, s_srs, s_spref, s_bin, step);
+                bin_events<false, kTile>(a, M, true, cnt, total, s_srs, s_spref, s_bin, ev, step);
             } else {
-                bin_events<true>(a, true, cnt, total, s_srs, s_spref, s_bin, step);
+                bin_events<true, kTile>(a, M, true, cnt, total, s_srs, s_spref, s_bin, ev, step);
             }
             __syncthreads();
         }
-        if (pass == 0) bin_offsets(a, s_bin, s_wave, j, base, events);
+        if (pass == 0) bin_offsets<kTile>(a, s_bin, s_wave, j, base, events, hdr);
     }
     if (t == 0 && events) atomicAdd(reinterpret_cast<unsigned long long*>(a.tile_events + b0), events);
+    if (a.mega) {
+        __syncthreads();
+        if (t == 0) mega_done(a, step);
+    }
 }
 
-// bin_apply runs kApply threads per block: two posts, two records and two
-// writer headers per thread.
+// bin_apply and the megastep run kApply threads per block: two cells, two
+// records and two writer headers per thread.
 constexpr uint32_t kApply = 1024;
 constexpr uint32_t kApplyWaves = kApply / 64;
 
-// Exclusive prefix sum over one bin_apply block; returns the total.
-__device__ __forceinline__ uint32_t apply_exclusive_scan(uint32_t x, uint32_t* s_wave, uint32_t& excl) {
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    uint32_t inc = x;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(inc, o, 64);
-        if (lane >= static_cast<uint32_t>(o)) inc += y;
-    }
-    if (lane == 63) s_wave[wave] = inc;
-    __syncthreads();
-    uint32_t base = 0, total = 0;
-#pragma unroll
-    for (uint32_t w = 0; w < kApplyWaves; ++w) {
-        const uint32_t v = s_wave[w];
-        if (w < wave) base += v;
-        total += v;
-    }
-    excl = base + inc - x;
-    __syncthreads();
-    return total;
-}
-
-// LDS of one bin_apply block.  A pass's records live in r_w and in `u`:
-// while lists are built, u holds r_eid[kBinCap] (u32), nxt[kBinCap] (u16,
-// the per-post list links) and r_post[kBinCap] (u16); the sort of long lists
-// reuses all of u as kBinCap 64-bit keys.
+// LDS of a consumer block.  A pass's records live in r_w and in `u`: while
+// lists are built, u holds r_eid[kBinCap] (u32), nxt[kBinCap] (u16, the
+// per-cell list links) and r_post[kBinCap] (u16, the cell); the sort of long
+// lists reuses all of u as kBinCap 64-bit keys.
 struct BinApplyLds {
     float slice[kBinPosts];
     uint32_t head[kBinPosts];
@@ -1010,18 +1078,18 @@ constexpr uint32_t kListNil = 0xFFFFFFFFu, kListLong = 0xFFFFFFFEu;
 constexpr uint16_t kLinkNil = 0xFFFFu;
 constexpr int kShortList = 8;
 
-// Record q of the pass: post `post` (within the bin), event id, weight.
-__device__ __forceinline__ void bin_link(BinApplyLds& L, uint32_t q, uint32_t post, uint32_t eid, float w) {
+// Record q of the pass: cell, event id, weight.
+__device__ __forceinline__ void bin_link(BinApplyLds& L, uint32_t q, uint32_t cell, uint32_t eid, float w) {
     L.r_eid()[q] = eid;
     L.r_w[q] = w;
-    L.r_post()[q] = static_cast<uint16_t>(post);
-    L.nxt()[q] = static_cast<uint16_t>(atomicExch(&L.head[post], q));  // kListNil -> kLinkNil
+    L.r_post()[q] = static_cast<uint16_t>(cell);
+    L.nxt()[q] = static_cast<uint16_t>(atomicExch(&L.head[cell], q));  // kListNil -> kLinkNil
 }
 
-// Add the pass's m records (linked per post from head[]) into the slice,
+// Add the pass's m records (linked per cell from head[]) into the slice,
 // each cell in ascending event id, and leave every head empty.  Lists of up
 // to kShortList records are insertion-sorted in registers by their owning
-// thread; longer ones (hub posts) are sorted together by (post, event id)
+// thread; longer ones (hub posts) are sorted together by (cell, event id)
 // with a block bitonic sort.
 __device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
     const uint32_t t = threadIdx.x;
@@ -1078,7 +1146,7 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
         L.slice[p] = acc;
     }
     if (!__syncthreads_or(longs)) return;
-    // keys (post << 43 | eid << 11 | record) of records on long lists; others sort last
+    // keys (cell << 43 | eid << 11 | record) of records on long lists; others sort last
     constexpr uint32_t kPer = kBinCap / kApply;
     uint32_t np2 = 1;
     while (np2 < m) np2 <<= 1;
@@ -1088,9 +1156,9 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
         const uint32_t i = t + r * kApply;
         kk[r] = ~0ull;
         if (i < m) {
-            const uint32_t post = L.r_post()[i];
-            if (L.head[post] == kListLong)
-                kk[r] = (static_cast<uint64_t>(post) << 43) | (static_cast<uint64_t>(r_eid[i]) << 11) | i;
+            const uint32_t cell = L.r_post()[i];
+            if (L.head[cell] == kListLong)
+                kk[r] = (static_cast<uint64_t>(cell) << 43) | (static_cast<uint64_t>(r_eid[i]) << 11) | i;
         }
     }
     __syncthreads();
@@ -1116,46 +1184,35 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
     // one thread per long list sums its sorted run
     for (uint32_t i = t; i < m; i += kApply) {
         const uint64_t k0 = L.u[i];
-        const uint32_t post = static_cast<uint32_t>(k0 >> 43);
-        if (post >= kBinPosts) break;  // past the last long-list record
-        if (i > 0 && static_cast<uint32_t>(L.u[i - 1] >> 43) == post) continue;
-        float acc = L.slice[post];
+        const uint32_t cell = static_cast<uint32_t>(k0 >> 43);
+        if (cell >= kBinPosts) break;  // past the last long-list record
+        if (i > 0 && static_cast<uint32_t>(L.u[i - 1] >> 43) == cell) continue;
+        float acc = L.slice[cell];
         for (uint32_t q = i; q < m; ++q) {
             const uint64_t kq = L.u[q];
-            if (static_cast<uint32_t>(kq >> 43) != post) break;
+            if (static_cast<uint32_t>(kq >> 43) != cell) break;
             acc = acc + L.r_w[kq & (kBinCap - 1)];
         }
-        L.slice[post] = acc;
-        L.head[post] = kListNil;
+        L.slice[cell] = acc;
+        L.head[cell] = kListNil;
     }
     __syncthreads();
 }
 
-// Work item {network, bin}; {kEmpty, 0} pads the XCD-ordered list.
-__global__ __launch_bounds__(kApply) void bin_apply_kernel(const NetArgs* __restrict__ nets,
-                                                          const uint2* __restrict__ refs) {
-    __shared__ BinApplyLds L;
-    const uint2 ref = refs[blockIdx.x];
-    if (ref.x == kEmpty) return;
-    const NetArgs& a = nets[ref.x];
-    const uint32_t k = ref.y, t = threadIdx.x;
-    const uint32_t slot = *a.bin_slot;
-    if (slot == kEmpty) return;
-    const uint32_t nw = a.n_bblk;
-    // bin k's {start, count} from every writer, in writer order: writers
-    // 2t and 2t+1 (rows hold an even number of writers)
+// Gather one bin's records of one step into L.slice (each cell from zero in
+// ascending event id): hrow = the bin's row of headers (nw writers, two per
+// thread; the entries read are zeroed), ev = the records.  Every thread of
+// the block calls it after zeroing L.slice and emptying L.head; it ends with
+// a barrier.
+__device__ __forceinline__ void bin_gather(BinApplyLds& L, uint2* hrow2, const uint4* ev, uint32_t nw) {
     static_assert(kMaxWriters == 2 * kApply, "two writers per thread");
-    uint4* hrow = reinterpret_cast<uint4*>(a.bin_hdr + static_cast<size_t>(k) * a.bin_rows);
+    const uint32_t t = threadIdx.x;
+    uint4* hrow = reinterpret_cast<uint4*>(hrow2);  // rows hold an even number of writers
     uint4 hw = make_uint4(0u, 0u, 0u, 0u);
     if (2 * t < nw) hw = hrow[t];
-    if (hw.y | hw.w) hrow[t] = make_uint4(0u, 0u, 0u, 0u);  // consumed: empty for the next step
-#pragma unroll
-    for (uint32_t pr = 0; pr < kBinPosts / kApply; ++pr) {
-        L.slice[t + pr * kApply] = 0.0f;
-        L.head[t + pr * kApply] = kListNil;
-    }
+    if (hw.y | hw.w) hrow[t] = make_uint4(0u, 0u, 0u, 0u);  // consumed: empty for the next use
     uint32_t excl;
-    const uint32_t total = apply_exclusive_scan(hw.y + hw.w, L.wave, excl);
+    const uint32_t total = block_scan<kApply>(hw.y + hw.w, L.wave, excl);
     L.seg_base[2 * t] = hw.x;
     L.seg_base[2 * t + 1] = hw.z;
     L.seg_pref[2 * t] = excl;
@@ -1177,7 +1234,7 @@ __global__ __launch_bounds__(kApply) void bin_apply_kernel(const NetArgs* __rest
             // Group ja alone exceeds a pass; its records are unordered, so
             // take them in windows of kBinCap consecutive event ids (at most
             // kBinCap records each), lowest first.
-            const uint4* rec = a.bin_ev + L.seg_base[ja];
+            const uint4* rec = ev + L.seg_base[ja];
             const uint32_t seg = L.seg_pref[ja + 1] - f0;
             if (t == 0) L.next = kListNil;
             __syncthreads();
@@ -1214,7 +1271,7 @@ __global__ __launch_bounds__(kApply) void bin_apply_kernel(const NetArgs* __rest
             const uint32_t f = t + u * kApply;
             if (f < m) {
                 const uint32_t sg = floor_index<kMaxWriters>(L.seg_pref, ja, jb, f0 + f);  // group of the record
-                r[u] = a.bin_ev[L.seg_base[sg] + (f0 + f - L.seg_pref[sg])];
+                r[u] = ev[L.seg_base[sg] + (f0 + f - L.seg_pref[sg])];
             }
         }
 #pragma unroll
@@ -1226,13 +1283,255 @@ __global__ __launch_bounds__(kApply) void bin_apply_kernel(const NetArgs* __rest
         bin_accumulate(L, m);
         ja = jb;
     }
-    // the slice of slot (s+1) % 32 for posts [k*kBinPosts, ...)
     __syncthreads();
-    const uint32_t p0 = k * kBinPosts;
-    const uint32_t np = min(kBinPosts, a.N - p0);
-    float* out = a.ring + static_cast<size_t>(slot) * a.N + p0;
-    for (uint32_t p = t; p < np; p += kApply) out[p] = L.slice[p];
-    if (k == 0 && t == 0) *a.step = *a.step + 1;  // no other block of this kernel reads it
+}
+
+// Two-kernel consumer, work item {network, bin}; {kEmpty, 0} pads the
+// XCD-ordered list.  mode 0: the step's apply (slot from bin_slot, then the
+// step counter moves on); mode 1: a megastep network's pending events (the
+// previous step's buffer) into ring slot step % 32, unless already there.
+__global__ __launch_bounds__(kApply) void bin_apply_kernel(const NetArgs* __restrict__ nets,
+                                                          const uint2* __restrict__ refs, uint32_t mode) {
+    __shared__ BinApplyLds L;
+    const uint2 ref = refs[blockIdx.x];
+    if (ref.x == kEmpty) return;
+    const NetArgs& a = nets[ref.x];
+    const uint32_t k = ref.y, t = threadIdx.x;
+    uint32_t slot;
+    uint2* hdr = a.bin_hdr;
+    const uint4* ev = a.bin_ev;
+    if (mode == 0) {
+        slot = *a.bin_slot;
+        if (slot == kEmpty) return;
+    } else {
+        if (*a.bin_mat) return;
+        const uint64_t step = *a.step;
+        slot = static_cast<uint32_t>(step & (kSlots - 1));
+        const uint32_t par = static_cast<uint32_t>((step + 1) & 1u);
+        hdr += par * a.hdr_stride;
+        ev += par * a.n_ev;
+    }
+#pragma unroll
+    for (uint32_t pr = 0; pr < kBinPosts / kApply; ++pr) {
+        L.slice[t + pr * kApply] = 0.0f;
+        L.head[t + pr * kApply] = kListNil;
+    }
+    bin_gather(L, hdr + static_cast<size_t>(k) * a.bin_rows, ev, a.n_bblk);
+    // cell c of the bin -> its post in the slot
+    float* out = a.ring + static_cast<size_t>(slot) * a.N;
+#pragma unroll
+    for (uint32_t pr = 0; pr < kBinPosts / kApply; ++pr) {
+        const uint32_t c = t + pr * kApply, tile = k * kBinTiles + c / kTile;
+        if (tile < a.n_tiles) {
+            const PopDev& P = a.pops[a.tile_pop[tile]];
+            const uint32_t li = (tile - P.tile0) * kTile + c % kTile;
+            if (li < P.n) out[P.off + li] = L.slice[c];
+        }
+    }
+    if (mode == 0 && k == 0 && t == 0) *a.step = *a.step + 1;  // no other block of this kernel reads it
+}
+
+// Megastep: one kernel per step for a binned network whose populations share
+// MODEL and have no pair-based plasticity.  Work item {network, bin k}: the
+// block owns tiles [k*kBinTiles, (k+1)*kBinTiles), thread t neurons t and
+// t + kApply of them (cells of bin k).  It gathers the previous step's events
+// for its cells (or reads ring slot step % 32 when a host call has put them
+// there), advances the neurons exactly as neuron_kernel does, and writes its
+// spikes' events as writer k for the next step.
+// LIF and Izhikevich fit two blocks per CU (eight waves per SIMD); HH and
+// GPEA keep one, their updates need the registers.
+template <int MODEL>
+constexpr int kMegaWaves = MODEL <= 1 ? 8 : 4;
+
+template <int MODEL>
+__global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(const NetArgs* __restrict__ nets,
+                                                         const uint2* __restrict__ items) {
+    __shared__ BinApplyLds L;
+    __shared__ BinMap M;
+    __shared__ uint32_t s_cnt[kBinTiles][kWavesPerTile], s_deg[kBinTiles][kWavesPerTile];
+    __shared__ uint32_t s_drv[kBinTiles][kWavesPerTile];
+    __shared__ uint32_t s_tp[kBinTiles + 1][3];  // per tile: first spiking row, first edge; then block totals
+    const uint2 it = items[blockIdx.x];
+    if (it.x == kEmpty) return;  // pad of the XCD-ordered list
+    const NetArgs& a = nets[it.x];
+    const uint32_t k = it.y, t = threadIdx.x, lane = t & 63u, wq = (t / 64u) % kWavesPerTile, tl = t % kTile;
+    const uint64_t step = *a.step;
+    if (step >= a.step_limit) return;
+    const uint32_t wpar = static_cast<uint32_t>(step & 1u);
+
+    // the block's neurons: population and index (their loads follow the gather,
+    // which needs every register it can get)
+    const PopDev* PP[2];
+    uint32_t li[2], tile[2];
+    bool on[2];
+#pragma unroll
+    for (uint32_t q = 0; q < 2; ++q) {
+        tile[q] = k * kBinTiles + t / kTile + q * (kApply / kTile);
+        PP[q] = a.pops;
+        li[q] = 0u;
+        on[q] = false;
+        if (tile[q] < a.n_tiles) {
+            if (a.n_pops > 1) PP[q] = a.pops + a.tile_pop[tile[q]];
+            li[q] = (tile[q] - PP[q]->tile0) * kTile + tl;
+            on[q] = li[q] < PP[q]->n;
+        }
+    }
+    bin_map_load(a, M);
+    float cur[2] = {0.0f, 0.0f};
+    if (*a.bin_mat) {
+        // a host call (current injection, ring download) has applied them
+        float* ring_slot = a.ring + static_cast<size_t>(step & (kSlots - 1)) * a.N;
+#pragma unroll
+        for (uint32_t q = 0; q < 2; ++q) {
+            if (!on[q]) continue;
+            const uint32_t i = PP[q]->off + li[q];
+            cur[q] = ring_slot[i];
+            ring_slot[i] = 0.0f;
+        }
+        __syncthreads();
+    } else {
+#pragma unroll
+        for (uint32_t pr = 0; pr < kBinPosts / kApply; ++pr) {
+            L.slice[t + pr * kApply] = 0.0f;
+            L.head[t + pr * kApply] = kListNil;
+        }
+        const uint32_t rpar = wpar ^ 1u;
+        bin_gather(L, a.bin_hdr + rpar * a.hdr_stride + static_cast<size_t>(k) * a.bin_rows, a.bin_ev + rpar * a.n_ev,
+                   a.n_bblk);
+#pragma unroll
+        for (uint32_t q = 0; q < 2; ++q) cur[q] = L.slice[t + q * kApply];
+    }
+    uint32_t row0[2], deg[2];
+    Neuron<MODEL> nrn[2];
+#pragma unroll
+    for (uint32_t q = 0; q < 2; ++q) {
+        row0[q] = deg[q] = 0u;
+        if (on[q]) {
+            nrn[q].load(*PP[q], li[q]);
+            row0[q] = a.row_ptr[PP[q]->off + li[q]];
+            deg[q] = a.row_ptr[PP[q]->off + li[q] + 1] - row0[q];
+        }
+    }
+
+    // advance every neuron; per tile ballot the spikes and scan the spiking
+    // rows' lengths within each wave
+    uint64_t m[2];
+    uint32_t excl[2];
+#pragma unroll
+    for (uint32_t q = 0; q < 2; ++q) {
+        const PopDev& P = *PP[q];
+        const uint32_t ti = t / kTile + q * (kApply / kTile);
+        int spike = 0, drive_now = 0;
+        if (on[q]) {
+            const uint32_t i = li[q];
+            float I = cur[q];
+            // A per-neuron Poisson source with one edge of delay <= 1 adds
+            // its weight into the slot after every neuron push of the previous
+            // step (P/src/api/net/network/step.rs:166-185): the last add.
+            if ((P.flags & kPopDrive) && P.drive_p > 0.0f) {
+                const uint32_t sd = P.drive_seed0 + i * 2654435761u;
+                if (step > 0 && poisson_roll(sd, 0u, step - 1, P.drive_p)) I = I + P.drive_w;
+                drive_now = poisson_roll(sd, 0u, step, P.drive_p) ? 1 : 0;
+            }
+            if (P.flags & kPopNoise) {
+                const float sg = (P.flags & kPopNoiseScalar) ? P.sigma_uni : P.sigma[i];
+                I = I + sg * normal_draw(P.seed, i, step);
+            }
+            spike = nrn[q].step(P, i, I, P.puni[0]);
+            nrn[q].store(P, i);
+        }
+        m[q] = __ballot(spike);
+        const uint32_t drive_events = static_cast<uint32_t>(__popcll(__ballot(drive_now)));
+        const uint32_t d = spike ? deg[q] : 0u;
+        uint32_t inc = d;
+        if (m[q]) {
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(inc, o, 64);
+                if (lane >= static_cast<uint32_t>(o)) inc += y;
+            }
+        }
+        excl[q] = inc - d;
+        if (lane == 63) s_deg[ti][wq] = inc;
+        if (lane == 0) {
+            s_cnt[ti][wq] = static_cast<uint32_t>(__popcll(m[q]));
+            s_drv[ti][wq] = drive_events;
+            if (tile[q] < a.n_tiles) {
+                const uint32_t word = P.word0 + (tile[q] - P.tile0) * kWavesPerTile + wq;
+                a.bits[word] = m[q];
+                if (a.raster) {
+                    const uint64_t rs = step - *a.raster_base;
+                    if (rs < a.raster_cap) a.raster[rs * a.n_words + word] = m[q];
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // ordered compaction: the block's spiking rows in (tile, wave, lane)
+    // order, i.e. ascending neuron, with their edge prefix
+    if (t == 0) {
+        uint32_t sp = 0, ed = 0, dv = 0;
+        for (uint32_t ti = 0; ti < kBinTiles; ++ti) {
+            s_tp[ti][0] = sp;
+            s_tp[ti][1] = ed;
+            for (uint32_t w = 0; w < kWavesPerTile; ++w) {
+                sp += s_cnt[ti][w];
+                ed += s_deg[ti][w];
+                dv += s_drv[ti][w];
+            }
+        }
+        s_tp[kBinTiles][0] = sp;
+        s_tp[kBinTiles][1] = ed;
+        s_tp[kBinTiles][2] = dv;
+    }
+    __syncthreads();
+    const uint32_t blk_sp = s_tp[kBinTiles][0], blk_ed = s_tp[kBinTiles][1], drv = s_tp[kBinTiles][2];
+    uint32_t* s_rs = reinterpret_cast<uint32_t*>(L.u);  // the gather is done with u
+    uint32_t* s_pref = s_rs + kBinCap;
+#pragma unroll
+    for (uint32_t q = 0; q < 2; ++q) {
+        const uint32_t ti = t / kTile + q * (kApply / kTile);
+        const uint32_t sp0 = s_tp[ti][0], ed0 = s_tp[ti][1];  // rows and edges of the tiles before ti
+        if ((m[q] >> lane) & 1ull) {
+            uint32_t base = 0, dbase = 0;
+            for (uint32_t w = 0; w < wq; ++w) {
+                base += s_cnt[ti][w];
+                dbase += s_deg[ti][w];
+            }
+            const uint32_t rank = base + static_cast<uint32_t>(__popcll(m[q] & ((1ull << lane) - 1ull)));
+            a.active[static_cast<size_t>(tile[q]) * kTile + rank] = PP[q]->off + li[q];
+            s_rs[sp0 + rank] = row0[q];
+            s_pref[sp0 + rank] = ed0 + dbase + excl[q];
+        }
+        if (tl == 0 && tile[q] < a.n_tiles) a.tile_count[tile[q]] = s_tp[ti + 1][0] - sp0;
+    }
+    if (t == 0) {
+        const uint32_t t0 = k * kBinTiles;
+        if (blk_sp) atomicAdd(reinterpret_cast<unsigned long long*>(a.tile_spikes + t0), blk_sp);
+        // each firing drive source is one synaptic event, counted at its roll
+        if (drv + blk_ed) atomicAdd(reinterpret_cast<unsigned long long*>(a.tile_events + t0), drv + blk_ed);
+    }
+    uint32_t* s_bin = L.seg_base;  // n_bins <= kMaxWriters; the gather is done with it
+    for (uint32_t b = t; b < a.n_bins; b += kApply) s_bin[b] = 0u;
+    __syncthreads();
+
+    // writer k of this step's events
+    uint4* ev = a.bin_ev + wpar * a.n_ev;
+    uint2* hdr = a.bin_hdr + wpar * a.hdr_stride;
+    const bool cached = blk_ed <= kEvCache * kApply;
+    EvCache ec;
+    if (cached) {
+        ev_cache_load<kApply>(a, M, false, blk_sp, blk_ed, s_rs, s_pref, ec);
+        ev_cache_count<kApply>(ec, blk_ed, s_bin);
+    } else {
+        bin_events<false, kApply>(a, M, false, blk_sp, blk_ed, s_rs, s_pref, s_bin, ev, step);
+    }
+    bin_offsets<kApply>(a, s_bin, L.wave, k, a.tile_ebase[k * kBinTiles], blk_ed, hdr);
+    if (cached) ev_cache_write<kApply>(a, false, ec, blk_ed, s_bin, ev, step);
+    else bin_events<true, kApply>(a, M, false, blk_sp, blk_ed, s_rs, s_pref, s_bin, ev, step);
+    __syncthreads();
+    if (t == 0) mega_done(a, step);
 }
 
 __device__ __forceinline__ float event_weight(const NetArgs& a, uint32_t eid) {
@@ -1609,9 +1908,16 @@ void launch_bin(const NetArgs* nets, const uint4* refs, uint32_t n, uint32_t bin
     if (n == 0) return;
     bin_kernel<<<n, kTile, sizeof(uint32_t) * bins, s>>>(nets, refs);
 }
-void launch_bin_apply(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s) {
+void launch_bin_apply(const NetArgs* nets, const uint2* refs, uint32_t n, uint32_t mode, hipStream_t s) {
     if (n == 0) return;
-    bin_apply_kernel<<<n, kApply, 0, s>>>(nets, refs);
+    bin_apply_kernel<<<n, kApply, 0, s>>>(nets, refs, mode);
+}
+void launch_megastep(int model, const NetArgs* nets, const uint2* items, uint32_t n, hipStream_t s) {
+    if (n == 0) return;
+    if (model == 0) megastep_kernel<0><<<n, kApply, 0, s>>>(nets, items);
+    else if (model == 1) megastep_kernel<1><<<n, kApply, 0, s>>>(nets, items);
+    else if (model == 2) megastep_kernel<2><<<n, kApply, 0, s>>>(nets, items);
+    else megastep_kernel<3><<<n, kApply, 0, s>>>(nets, items);
 }
 void launch_fixup(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s) {
     if (n == 0) return;

@@ -22,6 +22,8 @@ constexpr uint32_t kBinLog2 = 11;
 constexpr uint32_t kBinPosts = 1u << kBinLog2;
 constexpr uint32_t kMaxBins = 4096;
 constexpr uint32_t kXcds = 8;          // MI355X: workgroups go round-robin over 8 XCDs
+constexpr uint32_t kBinTiles = 8;       // tiles per bin
+constexpr uint32_t kMaxBinPops = 64;    // populations of a binned network
 constexpr uint32_t kBinCap = 2048;      // records bin_apply orders in one LDS pass
 constexpr uint32_t kMaxWriters = 2048;  // binned writers per network
 constexpr uint32_t kBinBlockTiles = 4;  // source tiles per bin_kernel writer
@@ -73,17 +75,24 @@ struct NetArgs {
     uint64_t step_limit;        // kernels skip the network once *step reaches it
     float* ring;                // [32][N]
     int32_t ring_fresh;         // every delay <= 1: this step's target slot holds zeros
-    // Binned scatter (n_bins > 0): bins of kBinPosts posts; writer j (of
+    // Binned scatter (n_bins > 0): bins of kBinTiles tiles; writer j (of
     // n_bblk, ascending event ids) puts its events grouped by bin from its
     // first edge id, and bin_hdr[k][j] = {first record, count} of its group
-    // for bin k.
+    // for bin k.  Megastep networks keep two buffers of each, by step parity:
+    // bin_ev + p*n_ev and bin_hdr + p*hdr_stride.
     uint32_t n_bins;
     uint32_t n_bblk;            // writers
     int32_t bin_fused;          // the neuron kernel's blocks are the neuron-tile writers
+    int32_t mega;               // megastep network: one kernel per step
     uint32_t bin_rows;          // row length of bin_hdr (even, >= n_bblk)
-    uint2* bin_hdr;             // [n_bins][bin_rows]
-    uint4* bin_ev;              // [events] {event id, weight bits, post within the bin, 0}
-    uint32_t* bin_slot;         // ring slot this step's events target, kEmpty past the horizon
+    uint32_t n_pops;
+    uint64_t n_ev;              // synapses + source edges
+    uint64_t hdr_stride;        // n_bins * bin_rows
+    uint2* bin_hdr;             // [1 or 2][n_bins][bin_rows]
+    uint4* bin_ev;              // [1 or 2][n_ev] {event id, weight bits, cell, 0}
+    uint32_t* bin_slot;         // ring slot bin_apply fills, kEmpty past the horizon
+    uint32_t* bin_done;         // megastep: blocks of the step that finished
+    uint32_t* bin_mat;          // megastep: the pending events are in ring slot step % 32         // ring slot this step's events target, kEmpty past the horizon
     // Scatter bookkeeping.  head[key] = (step tag << 32) | event id of the
     // last event that reached key = slot*N + post in the step with that tag;
     // a head with another tag is idle, so heads are never cleared per step.
