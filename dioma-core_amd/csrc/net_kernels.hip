@@ -372,10 +372,14 @@ struct Neuron;
 template <>
 struct Neuron<0> {  // LIF: states v, refr; params dt, inv_tau, v_rest, v_th, v_reset, tau_ref
     float v, refr, p1, p2, p3, p4, p5;
-    __device__ __forceinline__ void load(const PopDev& P, uint32_t li) {
-        v = P.st[li];
+    __device__ __forceinline__ void load_state(const PopDev& P, uint32_t li) { v = P.st[li]; }
+    __device__ __forceinline__ void load_params(const PopDev& P, uint32_t li) {
         refr = (P.flags & kPopRefrElided) ? 0.0f : P.st[P.dstride + li];
         p1 = par(P, 1, li); p2 = par(P, 2, li); p3 = par(P, 3, li); p4 = par(P, 4, li); p5 = par(P, 5, li);
+    }
+    __device__ __forceinline__ void load(const PopDev& P, uint32_t li) {
+        load_state(P, li);
+        load_params(P, li);
     }
     __device__ __forceinline__ int step(const PopDev&, uint32_t, float I, float dt) {
         I = fminf(fmaxf(I, -100.0f), 100.0f);  // population.rs:79-88
@@ -390,12 +394,18 @@ struct Neuron<0> {  // LIF: states v, refr; params dt, inv_tau, v_rest, v_th, v_
 template <>
 struct Neuron<1> {  // Izhikevich: states v, u, refr; params dt, a, b, c, d, tau_ref, v_th
     float v, u, refr, p1, p2, p3, p4, p5, p6;
-    __device__ __forceinline__ void load(const PopDev& P, uint32_t li) {
+    __device__ __forceinline__ void load_state(const PopDev& P, uint32_t li) {
         v = P.st[li];
         u = P.st[P.dstride + li];
+    }
+    __device__ __forceinline__ void load_params(const PopDev& P, uint32_t li) {
         refr = (P.flags & kPopRefrElided) ? 0.0f : P.st[2 * P.dstride + li];
         p1 = par(P, 1, li); p2 = par(P, 2, li); p3 = par(P, 3, li);
         p4 = par(P, 4, li); p5 = par(P, 5, li); p6 = par(P, 6, li);
+    }
+    __device__ __forceinline__ void load(const PopDev& P, uint32_t li) {
+        load_state(P, li);
+        load_params(P, li);
     }
     __device__ __forceinline__ int step(const PopDev&, uint32_t, float I, float dt) {
         I = fminf(fmaxf(I, -100.0f), 100.0f);
@@ -410,6 +420,8 @@ struct Neuron<1> {  // Izhikevich: states v, u, refr; params dt, a, b, c, d, tau
 
 template <>
 struct Neuron<3> {  // GPEA: reads and writes its columns inside the update
+    __device__ __forceinline__ void load_state(const PopDev&, uint32_t) {}
+    __device__ __forceinline__ void load_params(const PopDev&, uint32_t) {}
     __device__ __forceinline__ void load(const PopDev&, uint32_t) {}
     __device__ __forceinline__ int step(const PopDev& P, uint32_t li, float I, float dt) {
         return gpea_update(P, li, I, dt);
@@ -419,6 +431,8 @@ struct Neuron<3> {  // GPEA: reads and writes its columns inside the update
 
 template <>
 struct Neuron<2> {  // Hodgkin-Huxley: reads and writes its columns inside the update
+    __device__ __forceinline__ void load_state(const PopDev&, uint32_t) {}
+    __device__ __forceinline__ void load_params(const PopDev&, uint32_t) {}
     __device__ __forceinline__ void load(const PopDev&, uint32_t) {}
     __device__ __forceinline__ int step(const PopDev& P, uint32_t li, float I, float dt) {
         return hh_update(P, li, I, dt);  // #[no_clamp] (hh.rs:13)
@@ -1146,25 +1160,34 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
         L.slice[p] = acc;
     }
     if (!__syncthreads_or(longs)) return;
-    // keys (cell << 43 | eid << 11 | record) of records on long lists; others sort last
+    // the records of long lists, compacted as keys (cell << 43 | eid << 11 |
+    // record), then sorted
     constexpr uint32_t kPer = kBinCap / kApply;
-    uint32_t np2 = 1;
-    while (np2 < m) np2 <<= 1;
     uint64_t kk[kPer];
+    uint32_t nk = 0;
 #pragma unroll
     for (uint32_t r = 0; r < kPer; ++r) {
         const uint32_t i = t + r * kApply;
         kk[r] = ~0ull;
         if (i < m) {
             const uint32_t cell = L.r_post()[i];
-            if (L.head[cell] == kListLong)
+            if (L.head[cell] == kListLong) {
                 kk[r] = (static_cast<uint64_t>(cell) << 43) | (static_cast<uint64_t>(r_eid[i]) << 11) | i;
+                ++nk;
+            }
         }
     }
-    __syncthreads();
+    if (t == 0) L.m = 0u;
+    __syncthreads();  // every key read; u is free
+    uint32_t pos = nk ? atomicAdd(&L.m, nk) : 0u;
 #pragma unroll
     for (uint32_t r = 0; r < kPer; ++r)
-        if (t + r * kApply < np2) L.u[t + r * kApply] = kk[r];
+        if (kk[r] != ~0ull) L.u[pos++] = kk[r];
+    __syncthreads();
+    const uint32_t n_long = L.m;
+    uint32_t np2 = 1;
+    while (np2 < n_long) np2 <<= 1;
+    for (uint32_t i = n_long + t; i < np2; i += kApply) L.u[i] = ~0ull;
     __syncthreads();
     for (uint32_t k = 2; k <= np2; k <<= 1) {
         for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
@@ -1182,13 +1205,12 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
         }
     }
     // one thread per long list sums its sorted run
-    for (uint32_t i = t; i < m; i += kApply) {
+    for (uint32_t i = t; i < n_long; i += kApply) {
         const uint64_t k0 = L.u[i];
         const uint32_t cell = static_cast<uint32_t>(k0 >> 43);
-        if (cell >= kBinPosts) break;  // past the last long-list record
         if (i > 0 && static_cast<uint32_t>(L.u[i - 1] >> 43) == cell) continue;
         float acc = L.slice[cell];
-        for (uint32_t q = i; q < m; ++q) {
+        for (uint32_t q = i; q < n_long; ++q) {
             const uint64_t kq = L.u[q];
             if (static_cast<uint32_t>(kq >> 43) != cell) break;
             acc = acc + L.r_w[kq & (kBinCap - 1)];
@@ -1359,30 +1381,29 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
     if (step >= a.step_limit) return;
     const uint32_t wpar = static_cast<uint32_t>(step & 1u);
 
-    // the block's neurons: population and index (their loads follow the gather,
-    // which needs every register it can get)
-    const PopDev* PP[2];
-    uint32_t li[2], tile[2];
-    bool on[2];
+    // the block's neurons' populations; their state loads follow the gather
+    // (issued before it, they slow its latency-bound reads more than they gain)
+    uint32_t pop[2];
 #pragma unroll
     for (uint32_t q = 0; q < 2; ++q) {
-        tile[q] = k * kBinTiles + t / kTile + q * (kApply / kTile);
-        PP[q] = a.pops;
-        li[q] = 0u;
-        on[q] = false;
-        if (tile[q] < a.n_tiles) {
-            if (a.n_pops > 1) PP[q] = a.pops + a.tile_pop[tile[q]];
-            li[q] = (tile[q] - PP[q]->tile0) * kTile + tl;
-            on[q] = li[q] < PP[q]->n;
-        }
+        const uint32_t tile = k * kBinTiles + t / kTile + q * (kApply / kTile);
+        pop[q] = 0u;
+        if (tile < a.n_tiles && a.n_pops > 1) pop[q] = a.tile_pop[tile];
     }
     bin_map_load(a, M);
     float cur[2] = {0.0f, 0.0f};
+    const PopDev* PP[2];
+    uint32_t li[2], tile[2];
+    bool on[2];
     if (*a.bin_mat) {
         // a host call (current injection, ring download) has applied them
         float* ring_slot = a.ring + static_cast<size_t>(step & (kSlots - 1)) * a.N;
 #pragma unroll
         for (uint32_t q = 0; q < 2; ++q) {
+            tile[q] = k * kBinTiles + t / kTile + q * (kApply / kTile);
+            PP[q] = a.pops + pop[q];
+            li[q] = (tile[q] - PP[q]->tile0) * kTile + tl;
+            on[q] = tile[q] < a.n_tiles && li[q] < PP[q]->n;
             if (!on[q]) continue;
             const uint32_t i = PP[q]->off + li[q];
             cur[q] = ring_slot[i];
@@ -1405,6 +1426,10 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
     Neuron<MODEL> nrn[2];
 #pragma unroll
     for (uint32_t q = 0; q < 2; ++q) {
+        tile[q] = k * kBinTiles + t / kTile + q * (kApply / kTile);
+        PP[q] = a.pops + pop[q];
+        li[q] = (tile[q] - PP[q]->tile0) * kTile + tl;
+        on[q] = tile[q] < a.n_tiles && li[q] < PP[q]->n;
         row0[q] = deg[q] = 0u;
         if (on[q]) {
             nrn[q].load(*PP[q], li[q]);
@@ -1470,20 +1495,32 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
 
     // ordered compaction: the block's spiking rows in (tile, wave, lane)
     // order, i.e. ascending neuron, with their edge prefix
-    if (t == 0) {
+    if (t < kBinTiles) {  // lane t of wave 0: tile t's totals, then their prefix
         uint32_t sp = 0, ed = 0, dv = 0;
-        for (uint32_t ti = 0; ti < kBinTiles; ++ti) {
-            s_tp[ti][0] = sp;
-            s_tp[ti][1] = ed;
-            for (uint32_t w = 0; w < kWavesPerTile; ++w) {
-                sp += s_cnt[ti][w];
-                ed += s_deg[ti][w];
-                dv += s_drv[ti][w];
+#pragma unroll
+        for (uint32_t w = 0; w < kWavesPerTile; ++w) {
+            sp += s_cnt[t][w];
+            ed += s_deg[t][w];
+            dv += s_drv[t][w];
+        }
+        uint32_t isp = sp, ied = ed, idv = dv;
+#pragma unroll
+        for (uint32_t o = 1; o < kBinTiles; o <<= 1) {
+            const uint32_t a1 = __shfl_up(isp, o, kBinTiles), a2 = __shfl_up(ied, o, kBinTiles),
+                           a3 = __shfl_up(idv, o, kBinTiles);
+            if (t >= o) {
+                isp += a1;
+                ied += a2;
+                idv += a3;
             }
         }
-        s_tp[kBinTiles][0] = sp;
-        s_tp[kBinTiles][1] = ed;
-        s_tp[kBinTiles][2] = dv;
+        s_tp[t][0] = isp - sp;
+        s_tp[t][1] = ied - ed;
+        if (t == kBinTiles - 1) {
+            s_tp[kBinTiles][0] = isp;
+            s_tp[kBinTiles][1] = ied;
+            s_tp[kBinTiles][2] = idv;
+        }
     }
     __syncthreads();
     const uint32_t blk_sp = s_tp[kBinTiles][0], blk_ed = s_tp[kBinTiles][1], drv = s_tp[kBinTiles][2];
