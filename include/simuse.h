@@ -114,10 +114,13 @@ int simuse_net_set_poisson_drive(simuse_net* net, int32_t pop, float rate_hz, fl
 /* Enables pair-based STDP; plastic[e] != 0 marks plastic edges (NULL = all).
  * Weights are accumulated in FP64 with an FP32 mirror used by the scatter. */
 int simuse_net_set_plasticity(simuse_net* net, const simuse_stdp_params* p, const uint8_t* plastic);
+/* Schedule of the chained scatter (networks with a delay > 1 step): one
+ * spiking row per wave or a block-flattened row list.  No effect on results. */
 int simuse_net_set_scatter_mode(simuse_net* net, int32_t mode);
 int simuse_net_get_scatter_mode(simuse_net* net, int32_t* mode);
-/* Neuron tiles (256 neurons each) one block of the step kernel advances,
- * 1..4.  No effect on results. */
+/* Neuron tiles (256 neurons each) one block of the neuron kernel advances,
+ * 1..2 (the one-kernel step of networks with delays <= 1 always takes 8).
+ * No effect on results. */
 int simuse_net_set_block_tiles(simuse_net* net, uint32_t tiles);
 int simuse_net_get_block_tiles(simuse_net* net, uint32_t* tiles);
 /* Times each scatter schedule and block size on a snapshot of the live
@@ -206,10 +209,12 @@ int simuse_net_raster_stats(simuse_net* net, uint32_t first, uint32_t count, flo
 
 /* Bytes of device memory the handle holds. */
 int simuse_net_device_bytes(simuse_net* net, uint64_t* bytes);
-/* Mean duration (ms) of each step kernel over the last simuse_net_step_async
- * window measured with HIP events on the handle's stream when profiling is
- * on: [neuron (with the scatter of networks without plasticity), stdp,
- * insert (plastic networks' scatter and Poisson sources), collision fix-up]. */
+/* Mean duration (ms) of each phase of the step over the last
+ * simuse_net_step_async window, measured with HIP events on the handle's
+ * stream when profiling is on: [step kernels (the one-kernel step of networks
+ * with delays <= 1, or the neuron update with its fused push), stdp, push
+ * writers after STDP and of Poisson sources, binned apply / chained
+ * fix-up]. */
 int simuse_net_set_profiling(simuse_net* net, int32_t on);
 int simuse_net_kernel_times(simuse_net* net, float* ms, int32_t n);
 
