@@ -177,6 +177,61 @@ def bench_serving(args) -> dict:
     return out
 
 
+def bench_precision(args) -> dict:
+    """Precision configuration: the Hodgkin-Huxley network with heavy-tailed
+    out-degree and pair STDP on its excitatory synapses (dt = 0.01 ms).  The
+    device state (FP32) is compared with the oracle's FP64 twin after every
+    window of 1000 steps on a reduced network (the oracle runs on the host
+    cores), and the device's step time is taken at the full 200k-neuron size."""
+    import pyoracle as po
+    from simuse.workloads import build_device, hh_network
+    n, windows = args.prec_neurons, args.prec_windows
+    spec = hh_network(n=n, seed=3, steps=windows * 1000, noise_sd=30.0)
+    log(f"precision: HH {n} neurons, {spec.n_syn} synapses, pair STDP, {windows} windows of 1000 steps vs FP64")
+    dev = build_device(spec, graph_steps=16)
+    o64 = po.OracleNetwork(spec, 64, threads=min(16, os.cpu_count() or 1))
+    plastic = spec.stdp["plastic"].astype(bool)
+    rows = []
+    s_dev0, s_ora0 = 0, 0
+    for k in range(windows):
+        dev.run(1000)
+        o64.step(1000)
+        v_dev = dev.download_states(0)[0, :n].astype(np.float64)
+        v64 = o64.state(0, 0)
+        dv = np.abs(v_dev - v64)
+        w_dev = dev.download_weights().astype(np.float64)[plastic]
+        w64 = o64.weights()[plastic]
+        dw = np.abs(w_dev - w64)
+        s_dev, s_ora = dev.counters()[0], o64.counters()[0]
+        rate_dev = (s_dev - s_dev0) / n / (1000 * spec.dt * 1e-3)
+        rate_ora = (s_ora - s_ora0) / n / (1000 * spec.dt * 1e-3)
+        s_dev0, s_ora0 = s_dev, s_ora
+        rows.append({"window": k, "v_mean_abs_err_mV": float(dv.mean()), "v_max_abs_err_mV": float(dv.max()),
+                     "w_rms_err": float(np.sqrt(np.mean(dw ** 2))), "w_max_abs_err": float(dw.max()),
+                     "rate_hz_fp32": float(rate_dev), "rate_hz_fp64": float(rate_ora),
+                     "rate_rel_err": float(abs(rate_dev - rate_ora) / max(rate_ora, 1e-12))})
+        log(f"  window {k}: |dv| mean {dv.mean():.3e} max {dv.max():.3e} mV, w rms {rows[-1]['w_rms_err']:.3e}, "
+            f"rate {rate_dev:.2f} vs {rate_ora:.2f} Hz")
+    dev.close()
+    big = hh_network(n=args.prec_full_neurons, seed=3, steps=args.prec_full_steps + 200, noise_sd=30.0)
+    net = build_device(big, graph_steps=16)
+    net.run(200)
+    net.sync()
+    t0 = time.perf_counter()
+    net.step_graph(args.prec_full_steps)
+    net.sync()
+    us = (time.perf_counter() - t0) * 1e6 / args.prec_full_steps
+    spikes, events = net.counters()
+    net.close()
+    log(f"  full size {args.prec_full_neurons} neurons / {big.n_syn} synapses: {us:.2f} us/step")
+    return {"workload": f"HH {n} neurons ({spec.n_syn} synapses), Pareto out-degree, lognormal weights, "
+                        "pair STDP (A+ 0.01, A- 0.012, tau 20 ms, [0,1]) on excitatory synapses, Gaussian input sd 30, "
+                        "dt 0.01 ms",
+            "reference": "oracle FP64 twin", "windows": rows,
+            "full_size": {"neurons": args.prec_full_neurons, "synapses": big.n_syn,
+                          "steps": args.prec_full_steps, "us_per_step": us}}
+
+
 def copy_calibration(mib: int = 1024) -> int:
     """Device-to-device copy of a known size (the HIP runtime's copy kernel),
     used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes."""
@@ -219,6 +274,11 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-gb", type=float, default=160.0)
     ap.add_argument("--serve-timeout", type=float, default=300.0)
+    ap.add_argument("--no-precision", action="store_true", help="skip the HH precision leg")
+    ap.add_argument("--prec-neurons", type=int, default=10_000)
+    ap.add_argument("--prec-windows", type=int, default=3)
+    ap.add_argument("--prec-full-neurons", type=int, default=200_000)
+    ap.add_argument("--prec-full-steps", type=int, default=2000)
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
@@ -344,6 +404,8 @@ def main() -> int:
         out["batching"] = bench_batching(args)
     if not args.no_serving:
         out["serving"] = bench_serving(args)
+    if not args.no_precision:
+        out["precision"] = bench_precision(args)
     print(json.dumps(out), flush=True)
     return 0
 
