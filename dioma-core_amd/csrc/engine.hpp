@@ -314,6 +314,10 @@ struct NetState {
     int record(uint32_t max_steps, hipStream_t stream);
     int download_raster(uint64_t* out, uint64_t n_words_buf, uint32_t* steps, hipStream_t stream);
     int raster_stats(uint32_t first, uint32_t count, float dt_s, float* rates, float* cvs, hipStream_t stream);
+    // checkpoint / resume as a .taas v2 file (snapshot.hip)
+    int fingerprint(std::vector<uint64_t>& fp) const;
+    int save_state(const char* path, hipStream_t stream);
+    int load_state(const char* path, hipStream_t stream);
 };
 
 struct Engine {

@@ -1108,6 +1108,22 @@ int simuse_net_reset(simuse_net* net, const simuse_pop_desc* pops, int32_t n_pop
     return rc;
 }
 
+int simuse_net_save_state(simuse_net* net, const char* path) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    if ((rc = net->eng.prepare())) return rc;
+    return net->st.save_state(path, net->eng.stream);
+}
+
+int simuse_net_load_state(simuse_net* net, const char* path) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    if ((rc = net->eng.prepare())) return rc;
+    rc = net->st.load_state(path, net->eng.stream);
+    net->eng.mark_dirty();
+    return rc;
+}
+
 int simuse_net_step_counter(simuse_net* net, uint64_t* step) {
     int rc = check_net(net);
     if (rc) return rc;

@@ -13,6 +13,7 @@ Steps follow the CPU canon ``pillars::api::Network::step``
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
@@ -243,6 +244,16 @@ class CompiledNetwork:
         self.pops = list(pops)
         descs = _descs(self.pops)
         check(lib().simuse_net_reset(self._h, descs, len(self.pops)))
+
+    # -- checkpoint / resume --------------------------------------------------------
+    def save_state(self, path) -> None:
+        """Write the network's whole dynamic state to a ``.taas`` v2 file."""
+        check(lib().simuse_net_save_state(self._h, os.fsencode(path)))
+
+    def load_state(self, path) -> None:
+        """Restore a state written by :meth:`save_state` into a network built
+        the same way; stepping then continues exactly as the saved one."""
+        check(lib().simuse_net_load_state(self._h, os.fsencode(path)))
 
     # -- introspection -------------------------------------------------------------
     def step_counter(self) -> int:

@@ -187,6 +187,25 @@ int simuse_net_inject_current(simuse_net* net, int32_t pop, const uint32_t* idx,
  * step counter and traces. */
 int simuse_net_reset(simuse_net* net, const simuse_pop_desc* pops, int32_t n_pops);
 
+/* Checkpoint / resume: the reference saves download_states /
+ * download_weights (PC/src/network.rs:481,736) as named sections of a
+ * `.taas` v2 file (S/crates/taas-format/src/writer.rs:73-99, lib.rs:8-24)
+ * and restores them with upload_weights and reset(pops)
+ * (PC/src/network.rs:721,827-842).  save_state writes a `.taas` v2 file
+ * (magic "TAAS2", u64 LE manifest length, JSON manifest, CRC-32 sections)
+ * holding every piece of dynamic state: population states and parameters,
+ * the 32-slot ring with the pending events, the step counter, weights,
+ * last-step spikes, event counters, STDP traces and FP64 weights, synapse
+ * resources.  load_state restores it into a network built the same way
+ * (populations, CSR, sources, noise, drive and plasticity set-up; checked
+ * against a fingerprint in the file, SIMUSE_E_INVALID naming the first
+ * difference); stepping then continues bit for bit as the saved network
+ * would have.  A bad file (magic, CRC, kinds, sizes) is rejected with
+ * SIMUSE_E_INVALID before the network is changed.  Both finalize the
+ * network if it has not stepped yet.  zstd-compressed sections are refused. */
+int simuse_net_save_state(simuse_net* net, const char* path);
+int simuse_net_load_state(simuse_net* net, const char* path);
+
 /* step_counter / n_total / n_pops / pop_n (PC/src/network.rs:389-401,820). */
 int simuse_net_step_counter(simuse_net* net, uint64_t* step);
 int simuse_net_n_total(simuse_net* net, uint64_t* n);
