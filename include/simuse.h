@@ -8,6 +8,7 @@
  * network/step.rs:18-189).  Path prefixes in citations:
  *   P/  = crates/SiMuse/crates/pillars/
  *   PC/ = crates/SiMuse/crates/pillars-cuda/
+ *   S/  = crates/SiMuse/
  *
  * Conventions
  *   - Every call returns SIMUSE_OK (0) or a negative SIMUSE_E_* code; the
