@@ -77,7 +77,9 @@ __device__ __forceinline__ float det_expf(float x) {
 
 // fast_exp (P/src/math/fast_exp.rs:21-63): 2^n * p(f), t = x*log2(e),
 // n = floor(t) (saturating), f = t - n, degree-6 Horner rounded step by step.
-__device__ __forceinline__ float fast_expf(float x) {
+// Host and device evaluate the same operations (the host computes the
+// per-population receptor decay factors with it).
+__host__ __device__ __forceinline__ float fast_expf(float x) {
     const float t = x * 1.44269502f;
     float tc = t;
     if (tc > 1000.0f) tc = 1000.0f;
@@ -91,7 +93,10 @@ __device__ __forceinline__ float fast_expf(float x) {
     poly = (poly * f + c1) * f + 1.0f;
     if (n < -126) return 0.0f;
     if (n > 127) return __builtin_inff();
-    return __uint_as_float(static_cast<uint32_t>(n + 127) << 23) * poly;
+    const uint32_t bits = static_cast<uint32_t>(n + 127) << 23;
+    float scale;
+    __builtin_memcpy(&scale, &bits, 4);
+    return scale * poly;
 }
 
 // cos(2*pi*u) for u in [0, 1) on a 2^-24 grid: quadrant from 4u (exact),

@@ -39,6 +39,7 @@ void launch_insert(int sched, const NetArgs* nets, const uint2* blocks, uint32_t
 void launch_fixup(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s);
 void launch_bin(const NetArgs* nets, const uint4* refs, uint32_t n, uint32_t bins, hipStream_t s);
 void launch_bin_apply(const NetArgs* nets, const uint2* refs, uint32_t n, uint32_t mode, hipStream_t s);
+void launch_bin_apply_rec(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s);
 void launch_megastep(int model, const NetArgs* nets, const uint2* items, uint32_t n, hipStream_t s);
 
 void launch_head_clear(unsigned long long* head, const uint64_t* step, size_t n, hipStream_t s);
@@ -179,6 +180,13 @@ struct NetState {
     uint32_t* tm_last = nullptr;
     float4* tm_prof = nullptr;
     float* ev_w = nullptr;
+    // conductance-based receptor synapses (host copies until finalize)
+    std::vector<uint8_t> h_rec_code;  // per synapse: 0 current-based, 1 + receptor type; empty = off
+    std::vector<float> h_g_ext;       // constant external current per neuron (empty: none)
+    float rec_scale = 1.0f;
+    uint8_t* rec_code = nullptr;
+    float* g = nullptr;               // [kReceptors][N]
+    float* g_ext = nullptr;
     uint2* d_all_blocks = nullptr;  // (0, b) for every scatter tile: dense-step insert list
     bool finalized = false;
     uint64_t host_step = 0;
@@ -281,6 +289,11 @@ struct NetState {
         a.tm_last = tm_last;
         a.tm_prof = tm_prof;
         a.ev_w = ev_w;
+        a.rec_code = rec_code;
+        a.g = g;
+        a.g_ext = g_ext;
+        a.rec_scale = rec_scale;
+        a.n8 = N & ~7u;
         return a;
     }
 
@@ -302,6 +315,8 @@ struct NetState {
     int set_plasticity(const simuse_stdp_params* p, const uint8_t* pl, hipStream_t stream);
     int set_stp(const simuse_stp_profile* profiles, int32_t n_profiles, float dt, const uint8_t* syn_types,
                 const uint8_t* src_types, uint64_t n_src_edges_given);
+    int set_receptors(const uint8_t* types, float scale, const float* ext);
+    int download_conductances(float* out, uint64_t len, hipStream_t stream);
     int download_spikes(int32_t* out, uint64_t len, hipStream_t stream);
     int download_spike_ids(uint32_t* ids, uint32_t cap, uint32_t* count, hipStream_t stream);
     int download_state(int32_t pop, float* out, uint64_t len, hipStream_t stream);
@@ -343,6 +358,8 @@ struct Engine {
     uint32_t bins_fused = 0, bins_bin = 0;  // most bins of a network with neuron-kernel / bin_kernel writers
     uint2* d_apply = nullptr;   // bins of binned networks, XCD-ordered
     uint32_t n_apply = 0, cap_apply = 0;
+    uint2* d_apply_rec = nullptr;  // the same for networks with receptor synapses
+    uint32_t n_apply_rec = 0, cap_apply_rec = 0;
     bool direct_fix = false;
     // execution
     int sched = SIMUSE_SCATTER_TILE;
@@ -367,7 +384,8 @@ struct Engine {
         for (uint2* t : d_mega)
             if (t) (void)hipFree(t);
         for (void* p : {static_cast<void*>(d_nets), static_cast<void*>(d_blocks), static_cast<void*>(d_stdp),
-                        static_cast<void*>(d_fix), static_cast<void*>(d_bin), static_cast<void*>(d_apply)})
+                        static_cast<void*>(d_fix), static_cast<void*>(d_bin), static_cast<void*>(d_apply),
+                        static_cast<void*>(d_apply_rec)})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -423,7 +441,7 @@ struct Engine {
         HIP_TRY(hipStreamSynchronize(stream));
         std::vector<NetArgs> na;
         std::vector<uint4> tl[kModels], binl;
-        std::vector<uint2> bl, sl, fl, apl, ml[kModels];
+        std::vector<uint2> bl, sl, fl, apl, aplr, ml[kModels];
         bins_fused = bins_bin = 0;
         for (uint32_t k = 0; k < nets.size(); ++k) {
             NetState* s = nets[k];
@@ -434,7 +452,8 @@ struct Engine {
             for (const HostPop& P : s->pops) ng += (P.ntiles + group - 1) / group;
             const uint32_t ns = (s->n_src_tiles + kBinBlockTiles - 1) / kBinBlockTiles;
             // binned networks without pair-based plasticity: neuron block j is writer j
-            const bool fz = s->binned && !s->stdp && ng + ns <= kMaxWriters;
+            // (receptor networks: bin_kernel writers, which tag each event with its receptor)
+            const bool fz = s->binned && !s->stdp && !s->rec_code && ng + ns <= kMaxWriters;
             // megastep: bin b's block is writer b, the source blocks follow
             s->mega = s->binned && s->mega_ok && !s->no_mega && !s->stdp;
             if (s->mega) {
@@ -489,7 +508,7 @@ struct Engine {
                 A.bin_fused = fz ? 1 : 0;
                 if (fz) bins_fused = std::max(bins_fused, s->n_bins);
                 bins_bin = std::max(bins_bin, s->n_bins);
-                for (uint32_t b = 0; b < s->n_bins; ++b) apl.push_back(make_uint2(k, b));
+                for (uint32_t b = 0; b < s->n_bins; ++b) (s->rec_code ? aplr : apl).push_back(make_uint2(k, b));
                 continue;
             }
             // chained scatter: plastic networks scatter after the STDP kernels;
@@ -544,6 +563,12 @@ struct Engine {
             if ((rc = grow(&d_apply, &cap_apply, n_apply))) return rc;
             HIP_TRY(hipMemcpy(d_apply, apl.data(), sizeof(uint2) * apl.size(), hipMemcpyHostToDevice));
         }
+        xcd_order(aplr, make_uint2(kEmpty, 0u));
+        n_apply_rec = static_cast<uint32_t>(aplr.size());
+        if (n_apply_rec) {
+            if ((rc = grow(&d_apply_rec, &cap_apply_rec, n_apply_rec))) return rc;
+            HIP_TRY(hipMemcpy(d_apply_rec, aplr.data(), sizeof(uint2) * aplr.size(), hipMemcpyHostToDevice));
+        }
         drop_graph();
         dirty = false;
         return SIMUSE_OK;
@@ -561,6 +586,7 @@ struct Engine {
         launch_bin(d_nets, d_bin, n_bin, bins_bin, stream);
         if (e) (void)hipEventRecord(e[3], stream);
         launch_bin_apply(d_nets, d_apply, n_apply, 0u, stream);
+        launch_bin_apply_rec(d_nets, d_apply_rec, n_apply_rec, stream);
         launch_fixup(d_nets, direct_fix ? nullptr : d_fix, n_fix, stream);
         if (e) (void)hipEventRecord(e[4], stream);
     }

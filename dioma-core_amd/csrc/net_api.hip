@@ -8,6 +8,7 @@
 // need no host round trip (the reference patches a host-written slot scalar
 // before every replay, PC/src/network.rs:431-460).
 
+#include "device_math.hpp"
 #include "engine.hpp"
 
 namespace simuse {
@@ -170,7 +171,7 @@ int NetState::finalize(hipStream_t stream) {
         bin_rows += bin_rows & 1u;  // even: consumers read two writers' headers at once
         // one kernel per step needs one model, no pair-based plasticity and a
         // writer per bin
-        mega_ok = one_model && !stdp && n_bins + ns <= kMaxWriters;
+        mega_ok = one_model && !stdp && h_rec_code.empty() && n_bins + ns <= kMaxWriters;
         mega_model = pops[0].model;
         const size_t nbuf = mega_ok ? 2 : 1;
         if ((rc = alloc(&bin_ev, nbuf * n_ev))) return rc;
@@ -216,6 +217,23 @@ int NetState::finalize(hipStream_t stream) {
         HIP_TRY(hipMemcpy(tm_prof, h_tm_prof.data(), sizeof(float4) * h_tm_prof.size(), hipMemcpyHostToDevice));
         launch_tm_init(tm_type, tm_prof, tm_r, tm_u, tm_last, n_ev, stream);
         HIP_TRY(hipGetLastError());
+    }
+    if (!h_rec_code.empty()) {
+        // conductance jumps land in the step after the spike, like a delay of
+        // one; the chained scatter (longer delays) has no receptor form
+        if (!binned)
+            return fail(SIMUSE_E_UNSUPPORTED,
+                        "receptor synapses need every synapse and source delay <= 1 step and at most %u "
+                        "populations (have max delay %u)",
+                        kMaxBinPops, max_delay);
+        if ((rc = alloc(&rec_code, n_syn))) return rc;
+        if ((rc = alloc(&g, static_cast<size_t>(kReceptors) * N))) return rc;
+        if (n_syn) HIP_TRY(hipMemcpy(rec_code, h_rec_code.data(), n_syn, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemsetAsync(g, 0, sizeof(float) * kReceptors * N, stream));
+        if (!h_g_ext.empty()) {
+            if ((rc = alloc(&g_ext, N))) return rc;
+            HIP_TRY(hipMemcpy(g_ext, h_g_ext.data(), 4 * static_cast<size_t>(N), hipMemcpyHostToDevice));
+        }
     }
     if ((rc = alloc(&tile_events, static_cast<size_t>(n_tiles) + n_src_tiles))) return rc;
     HIP_TRY(hipMemsetAsync(tile_events, 0, sizeof(uint64_t) * (n_tiles + n_src_tiles), stream));
@@ -300,6 +318,8 @@ int NetState::upload_pop_table() {
         D.drive_seed0 = P.drive_seed0;
         D.seed = P.seed;
         std::memcpy(D.puni, P.puni, sizeof(D.puni));
+        // DecayFactors::new(dt), P/src/kernels/synapse/kinetics.rs:24-38
+        for (int r = 0; r < kReceptors; ++r) D.gdec[r] = fast_expf(-P.puni[0] / kRecTau[r]);
     }
     HIP_TRY(hipMemcpy(d_pops, pd.data(), sizeof(PopDev) * pd.size(), hipMemcpyHostToDevice));
     return SIMUSE_OK;
@@ -318,6 +338,7 @@ int NetState::reset_dynamic(hipStream_t stream) {
     HIP_TRY(hipMemsetAsync(bits, 0, sizeof(uint64_t) * n_words, stream));
     HIP_TRY(hipMemsetAsync(tile_spikes, 0, sizeof(uint64_t) * n_tiles, stream));
     if (tile_events) HIP_TRY(hipMemsetAsync(tile_events, 0, sizeof(uint64_t) * (n_tiles + n_src_tiles), stream));
+    if (g) HIP_TRY(hipMemsetAsync(g, 0, sizeof(float) * kReceptors * N, stream));
     if (stdp) {
         HIP_TRY(hipMemsetAsync(tr_pre, 0, sizeof(float) * N, stream));
         HIP_TRY(hipMemsetAsync(tr_post, 0, sizeof(float) * N, stream));
@@ -392,6 +413,46 @@ int NetState::set_stp(const simuse_stp_profile* profiles, int32_t n_profiles, fl
                         static_cast<unsigned long long>(e), types[e]);
     h_tm_prof = std::move(prof);
     h_tm_type = std::move(types);
+    return SIMUSE_OK;
+}
+
+// Receptor types per synapse (ReceptorType::from_u8, P/src/kernels/synapse/
+// receptor.rs:64-76: 0..6, anything else AMPA), stored as 1 + type; 255
+// keeps a synapse current-based (0).  Jumps scale by `scale`
+// (csr_propagate_conductance_from_queue, CN/src/propagate.rs:225-290).
+int NetState::set_receptors(const uint8_t* types, float scale, const float* ext) {
+    if (finalized) return fail(SIMUSE_E_STATE, "receptor synapses must be set before the first step");
+    if (!types && n_syn) return fail(SIMUSE_E_INVALID, "receptor synapses: null receptor_types");
+    if (!std::isfinite(scale)) return fail(SIMUSE_E_INVALID, "receptor synapses: scale must be finite");
+    std::vector<uint8_t> code(n_syn);
+    for (uint64_t e = 0; e < n_syn; ++e) {
+        const uint8_t r = types[e];
+        code[e] = r == 255 ? 0 : r < kReceptors ? static_cast<uint8_t>(r + 1) : 1;
+    }
+    std::vector<float> x;
+    if (ext) {
+        for (uint32_t i = 0; i < N; ++i)
+            if (!std::isfinite(ext[i]))
+                return fail(SIMUSE_E_INVALID, "receptor synapses: external current %u must be finite", i);
+        x.assign(ext, ext + N);
+    }
+    h_rec_code = std::move(code);
+    if (h_rec_code.empty()) h_rec_code.assign(1, 0);  // no synapses: still on (decay, external current)
+    h_g_ext = std::move(x);
+    rec_scale = scale;
+    return SIMUSE_OK;
+}
+
+int NetState::download_conductances(float* out, uint64_t len, hipStream_t stream) {
+    if (!out || len != static_cast<uint64_t>(kReceptors) * N)
+        return fail(SIMUSE_E_INVALID, "conductances len must be 7*n_total");
+    if (!g) {
+        if (h_rec_code.empty()) return fail(SIMUSE_E_STATE, "receptor synapses are off");
+        std::memset(out, 0, sizeof(float) * len);  // not stepped yet
+        return SIMUSE_OK;
+    }
+    HIP_TRY(hipMemcpyAsync(out, g, sizeof(float) * len, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
     return SIMUSE_OK;
 }
 
@@ -771,6 +832,7 @@ int Engine::autotune(uint32_t trial_steps) {
             if ((rc = snap.add(s->tm_u, sizeof(float) * n_ev))) return rc;
             if ((rc = snap.add(s->tm_last, sizeof(uint32_t) * n_ev))) return rc;
         }
+        if (s->g && (rc = snap.add(s->g, sizeof(float) * kReceptors * N))) return rc;
         if (s->stdp) {
             if ((rc = snap.add(s->w, sizeof(float) * s->n_syn))) return rc;
             if ((rc = snap.add(s->w64, sizeof(double) * s->n_syn))) return rc;
@@ -1049,6 +1111,7 @@ int simuse_net_stdp_step(simuse_net* net, float trace_decay, float lr_ltp, float
         }
         launch_bin(e.d_nets, s.d_dense_writers, static_cast<uint32_t>(s.dense_writers.size()), s.n_bins, e.stream);
         launch_bin_apply(e.d_nets, e.d_apply, e.n_apply, 0u, e.stream);
+        launch_bin_apply_rec(e.d_nets, e.d_apply_rec, e.n_apply_rec, e.stream);
     } else {
         launch_insert(e.sched, e.d_nets, s.d_all_blocks, n_blk, e.stream);
         launch_fixup(e.d_nets, nullptr, s.n_fix, e.stream);
@@ -1106,6 +1169,18 @@ int simuse_net_reset(simuse_net* net, const simuse_pop_desc* pops, int32_t n_pop
     rc = net->st.reset(pops, n_pops, net->eng.stream);
     net->eng.mark_dirty();
     return rc;
+}
+
+int simuse_net_set_receptors(simuse_net* net, const uint8_t* receptor_types, float scale, const float* ext) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    return net->st.set_receptors(receptor_types, scale, ext);
+}
+
+int simuse_net_download_conductances(simuse_net* net, float* out, uint64_t len) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    return net->st.download_conductances(out, len, net->eng.stream);
 }
 
 int simuse_net_save_state(simuse_net* net, const char* path) {

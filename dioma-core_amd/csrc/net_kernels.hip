@@ -561,7 +561,7 @@ __device__ __forceinline__ uint32_t post_bin_cell(const BinMap& M, uint32_t n_po
 // positions [s_pref[k], s_pref[k+1]).  Counting adds 1 to s_bin[bin]; writing
 // puts the event at record s_bin[bin] of `ev` and advances it.  Four events
 // per thread are loaded before any is used.
-template <bool kWrite, uint32_t kThreads>
+template <bool kWrite, uint32_t kThreads, bool kRec = false>
 __device__ __forceinline__ void bin_events(const NetArgs& a, const BinMap& M, bool is_src, uint32_t cnt,
                                            uint32_t total, const uint32_t* s_rs, const uint32_t* s_pref,
                                            uint32_t* s_bin, uint4* ev, uint64_t step) {
@@ -596,8 +596,9 @@ __device__ __forceinline__ void bin_events(const NetArgs& a, const BinMap& M, bo
                 atomicAdd(&s_bin[bc >> 11], 1u);
             } else {
                 const float x = a.tm_type ? tm_deliver(a, eb + e[u], wt[u], step) : wt[u];
+                const uint32_t code = kRec && !is_src ? static_cast<uint32_t>(a.rec_code[e[u]]) : 0u;
                 const uint32_t pos = atomicAdd(&s_bin[bc >> 11], 1u);
-                ev[pos] = make_uint4(eb + e[u], __float_as_uint(x), bc & 2047u, 0u);
+                ev[pos] = make_uint4(eb + e[u], __float_as_uint(x), bc & 2047u, code);
             }
         }
     }
@@ -647,7 +648,7 @@ __device__ __forceinline__ void ev_cache_count(const EvCache& c, uint32_t total,
         if (threadIdx.x + u * kThreads < total) atomicAdd(&s_bin[c.bc[u] >> 11], 1u);
 }
 
-template <uint32_t kThreads>
+template <uint32_t kThreads, bool kRec = false>
 __device__ __forceinline__ void ev_cache_write(const NetArgs& a, bool is_src, const EvCache& c, uint32_t total,
                                                uint32_t* s_bin, uint4* ev, uint64_t step) {
     const uint32_t eb = is_src ? static_cast<uint32_t>(a.n_syn) : 0u;
@@ -656,8 +657,9 @@ __device__ __forceinline__ void ev_cache_write(const NetArgs& a, bool is_src, co
         if (threadIdx.x + u * kThreads >= total) continue;
         const uint32_t e = eb + c.e[u];
         const float x = a.tm_type ? tm_deliver(a, e, c.w[u], step) : c.w[u];
+        const uint32_t code = kRec && !is_src ? static_cast<uint32_t>(a.rec_code[e]) : 0u;
         const uint32_t pos = atomicAdd(&s_bin[c.bc[u] >> 11], 1u);
-        ev[pos] = make_uint4(e, __float_as_uint(x), c.bc[u] & 2047u, 0u);
+        ev[pos] = make_uint4(e, __float_as_uint(x), c.bc[u] & 2047u, code);
     }
 }
 
@@ -690,6 +692,47 @@ __device__ __forceinline__ void mega_done(const NetArgs& a, uint64_t step) {
         *a.bin_mat = 0u;
         *a.step = step + 1;
     }
+}
+
+// Conductance-based receptor synapses (SynapticKinetics, P/src/kernels/
+// synapse/kinetics.rs:112-205, in the step order of CN/src/sim/hh.rs:115-166):
+// every conductance decays by its population's factor, the constant external
+// current joins the input, then each receptor adds g_r (E_r - v) with v the
+// membrane potential before the update; NMDA's g is scaled by the Mg2+ block
+// B(v) = 1 / (1 + 0.33 fast_exp(clamp(-0.06 v, -10, 10))) (receptor.rs:
+// 126-150).  The reference adds neurons below n8 = N & ~7 in 8-wide lanes
+// straight onto the input (input + t_0 + ... + t_6) and the rest in a scalar
+// tail (input + (t_0 + ... + t_6)); both orders are kept.  Out of line: only
+// networks with receptor synapses call it.
+__device__ __noinline__ float receptor_input(const NetArgs* ap, const PopDev* pp, uint32_t li, float I) {
+    const NetArgs& a = *ap;
+    const PopDev& P = *pp;
+    const uint32_t gi = P.off + li;
+    const float v = P.st[li];  // state column 0
+    const float i0 = a.g_ext ? I + a.g_ext[gi] : I;
+    float x = -0.06f * v;
+    x = x < -10.0f ? -10.0f : x;
+    x = x > 10.0f ? 10.0f : x;
+    const float block = 1.0f / (1.0f + 0.33f * fast_expf(x));
+    float t[kReceptors];
+#pragma unroll
+    for (int r = 0; r < kReceptors; ++r) {
+        float* gp = a.g + static_cast<size_t>(r) * a.N + gi;
+        const float g = *gp * P.gdec[r];
+        *gp = g;
+        const float ge = r == kRecNmda ? g * block : g;
+        t[r] = ge * (kRecErev[r] - v);
+    }
+    if (gi < a.n8) {
+        float cur = i0;
+#pragma unroll
+        for (int r = 0; r < kReceptors; ++r) cur = cur + t[r];
+        return cur;
+    }
+    float acc = 0.0f;
+#pragma unroll
+    for (int r = 0; r < kReceptors; ++r) acc = acc + t[r];
+    return i0 + acc;
 }
 
 // One step of up to kMaxGroup consecutive tiles of one population: read ring
@@ -781,6 +824,7 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
                 const float sg = (P.flags & kPopNoiseScalar) ? P.sigma_uni : P.sigma[li];
                 I = I + sg * normal_draw(P.seed, li, step);
             }
+            if (a.rec_code) I = receptor_input(&a, &P, li, I);
             spike = nrn[g].step(P, li, I, dt);
             nrn[g].store(P, li);
             if (a.stdp) {
@@ -1039,7 +1083,11 @@ __global__ __launch_bounds__(kTile) void bin_kernel(const NetArgs* __restrict__ 
                     bin_events<false, kTile>(a, M, false, nr, total, s_rs, s_pref, s_bin, ev, step);
                 }
             } else if (cached) {
-                ev_cache_write<kTile>(a, false, ec, total, s_bin, ev, step);
+                // receptor networks tag each synapse event with its receptor
+                if (a.rec_code) ev_cache_write<kTile, true>(a, false, ec, total, s_bin, ev, step);
+                else ev_cache_write<kTile>(a, false, ec, total, s_bin, ev, step);
+            } else if (a.rec_code) {
+                bin_events<true, kTile, true>(a, M, false, nr, total, s_rs, s_pref, s_bin, ev, step);
             } else {
                 bin_events<true, kTile>(a, M, false, nr, total, s_rs, s_pref, s_bin, ev, step);
             }
@@ -1092,12 +1140,50 @@ constexpr uint32_t kListNil = 0xFFFFFFFFu, kListLong = 0xFFFFFFFEu;
 constexpr uint16_t kLinkNil = 0xFFFFu;
 constexpr int kShortList = 8;
 
-// Record q of the pass: cell, event id, weight.
-__device__ __forceinline__ void bin_link(BinApplyLds& L, uint32_t q, uint32_t cell, uint32_t eid, float w) {
+// Record q of the pass: cell, event id, weight; receptor networks keep the
+// record's receptor code (0 current, 1 + type) above the cell's 11 bits.
+__device__ __forceinline__ void bin_link(BinApplyLds& L, uint32_t q, uint32_t cell, uint32_t eid, float w,
+                                         uint32_t code = 0u) {
     L.r_eid()[q] = eid;
     L.r_w[q] = w;
-    L.r_post()[q] = static_cast<uint16_t>(cell);
+    L.r_post()[q] = static_cast<uint16_t>(cell | (code << kBinLog2));
     L.nxt()[q] = static_cast<uint16_t>(atomicExch(&L.head[cell], q));  // kListNil -> kLinkNil
+}
+
+// Receptor networks: the bin's consumer block adds each conductance jump to
+// g[r][post] (CN/src/propagate.rs:225-290: g += w * scale, in push order),
+// starting from the stored conductance; `touched` marks the loaded ones.
+struct RecCtx {
+    const NetArgs* a;
+    uint32_t k;  // bin
+};
+
+// Global neuron of cell c of bin k (a cell that received events).
+__device__ __forceinline__ uint32_t cell_post(const NetArgs& a, uint32_t k, uint32_t c) {
+    const uint32_t tile = k * kBinTiles + c / kTile;
+    const PopDev& P = a.pops[a.tile_pop[tile]];
+    return P.off + (tile - P.tile0) * kTile + c % kTile;
+}
+
+__device__ __forceinline__ void rec_jump(const NetArgs& a, uint32_t post, uint32_t code, float w,
+                                         float (&g)[kReceptors], uint32_t& touched) {
+    const float x = w * a.rec_scale;
+#pragma unroll
+    for (int r = 0; r < kReceptors; ++r)
+        if (code == static_cast<uint32_t>(r + 1)) {
+            if (!((touched >> r) & 1u)) {
+                g[r] = a.g[static_cast<size_t>(r) * a.N + post];
+                touched |= 1u << r;
+            }
+            g[r] = g[r] + x;
+        }
+}
+
+__device__ __forceinline__ void rec_store(const NetArgs& a, uint32_t post, const float (&g)[kReceptors],
+                                          uint32_t touched) {
+#pragma unroll
+    for (int r = 0; r < kReceptors; ++r)
+        if ((touched >> r) & 1u) a.g[static_cast<size_t>(r) * a.N + post] = g[r];
 }
 
 // Add the pass's m records (linked per cell from head[]) into the slice,
@@ -1105,7 +1191,8 @@ __device__ __forceinline__ void bin_link(BinApplyLds& L, uint32_t q, uint32_t ce
 // to kShortList records are insertion-sorted in registers by their owning
 // thread; longer ones (hub posts) are sorted together by (cell, event id)
 // with a block bitonic sort.
-__device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
+template <bool kRec>
+__device__ void bin_accumulate(BinApplyLds& L, uint32_t m, RecCtx rc) {
     const uint32_t t = threadIdx.x;
     const uint32_t* r_eid = L.r_eid();
     const uint16_t* nxt = L.nxt();
@@ -1115,12 +1202,13 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
         const uint32_t p = t + pr * kApply;
         const uint32_t h = L.head[p];
         if (h == kListNil) continue;
-        uint32_t ids[kShortList];
+        uint32_t ids[kShortList], cs[kShortList];
         float ws[kShortList];
 #pragma unroll
         for (int q = 0; q < kShortList; ++q) {
             ids[q] = 0xFFFFFFFFu;  // above every event id
             ws[q] = 0.0f;
+            cs[q] = 0u;
         }
         int n = 0;
         bool fits = true;
@@ -1132,6 +1220,7 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
             const uint32_t e = r_eid[i];
             const float w = L.r_w[i];
             const uint16_t nx = nxt[i];
+            const uint32_t cd = kRec ? static_cast<uint32_t>(L.r_post()[i]) >> kBinLog2 : 0u;
             // insert (e, w) into the ascending ids[]: every slot holding a larger
             // id takes its left neighbour's entry, or (e, w) if that one is smaller
 #pragma unroll
@@ -1141,6 +1230,7 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
                 if (ids[q] > e) {
                     ids[q] = prev_gt ? ids[qp] : e;
                     ws[q] = prev_gt ? ws[qp] : w;
+                    if (kRec) cs[q] = prev_gt ? cs[qp] : cd;
                 }
             }
             ++n;
@@ -1154,15 +1244,30 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
         }
         L.head[p] = kListNil;
         float acc = L.slice[p];
+        if (kRec) {
+            float g[kReceptors];
+            uint32_t touched = 0u;
+            const uint32_t post = cell_post(*rc.a, rc.k, p);
 #pragma unroll
-        for (int q = 0; q < kShortList; ++q)
-            if (q < n) acc = acc + ws[q];
+            for (int q = 0; q < kShortList; ++q) {
+                if (q >= n) continue;
+                if (cs[q] == 0u) acc = acc + ws[q];
+                else rec_jump(*rc.a, post, cs[q], ws[q], g, touched);
+            }
+            rec_store(*rc.a, post, g, touched);
+        } else {
+#pragma unroll
+            for (int q = 0; q < kShortList; ++q)
+                if (q < n) acc = acc + ws[q];
+        }
         L.slice[p] = acc;
     }
     if (!__syncthreads_or(longs)) return;
     // the records of long lists, compacted as keys (cell << 43 | eid << 11 |
-    // record), then sorted
+    // record; receptor networks: cell << 46 | eid << 14 | code << 11 |
+    // record, the keys overwrite the records' codes), then sorted
     constexpr uint32_t kPer = kBinCap / kApply;
+    constexpr uint32_t kCellSh = kRec ? 46u : 43u, kEidSh = kRec ? 14u : 11u;
     uint64_t kk[kPer];
     uint32_t nk = 0;
 #pragma unroll
@@ -1170,9 +1275,10 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
         const uint32_t i = t + r * kApply;
         kk[r] = ~0ull;
         if (i < m) {
-            const uint32_t cell = L.r_post()[i];
+            const uint32_t cell = static_cast<uint32_t>(L.r_post()[i]) & (kBinPosts - 1u);
             if (L.head[cell] == kListLong) {
-                kk[r] = (static_cast<uint64_t>(cell) << 43) | (static_cast<uint64_t>(r_eid[i]) << 11) | i;
+                kk[r] = (static_cast<uint64_t>(cell) << kCellSh) | (static_cast<uint64_t>(r_eid[i]) << kEidSh) | i;
+                if (kRec) kk[r] |= static_cast<uint64_t>(L.r_post()[i] >> kBinLog2) << 11;
                 ++nk;
             }
         }
@@ -1207,13 +1313,28 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
     // one thread per long list sums its sorted run
     for (uint32_t i = t; i < n_long; i += kApply) {
         const uint64_t k0 = L.u[i];
-        const uint32_t cell = static_cast<uint32_t>(k0 >> 43);
-        if (i > 0 && static_cast<uint32_t>(L.u[i - 1] >> 43) == cell) continue;
+        const uint32_t cell = static_cast<uint32_t>(k0 >> kCellSh);
+        if (i > 0 && static_cast<uint32_t>(L.u[i - 1] >> kCellSh) == cell) continue;
         float acc = L.slice[cell];
-        for (uint32_t q = i; q < n_long; ++q) {
-            const uint64_t kq = L.u[q];
-            if (static_cast<uint32_t>(kq >> 43) != cell) break;
-            acc = acc + L.r_w[kq & (kBinCap - 1)];
+        if (kRec) {
+            float g[kReceptors];
+            uint32_t touched = 0u;
+            const uint32_t post = cell_post(*rc.a, rc.k, cell);
+            for (uint32_t q = i; q < n_long; ++q) {
+                const uint64_t kq = L.u[q];
+                if (static_cast<uint32_t>(kq >> kCellSh) != cell) break;
+                const uint32_t r = static_cast<uint32_t>(kq & (kBinCap - 1));
+                const uint32_t cd = static_cast<uint32_t>(kq >> 11) & 7u;
+                if (cd == 0u) acc = acc + L.r_w[r];
+                else rec_jump(*rc.a, post, cd, L.r_w[r], g, touched);
+            }
+            rec_store(*rc.a, post, g, touched);
+        } else {
+            for (uint32_t q = i; q < n_long; ++q) {
+                const uint64_t kq = L.u[q];
+                if (static_cast<uint32_t>(kq >> kCellSh) != cell) break;
+                acc = acc + L.r_w[kq & (kBinCap - 1)];
+            }
         }
         L.slice[cell] = acc;
         L.head[cell] = kListNil;
@@ -1226,7 +1347,8 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m) {
 // thread; the entries read are zeroed), ev = the records.  Every thread of
 // the block calls it after zeroing L.slice and emptying L.head; it ends with
 // a barrier.
-__device__ __forceinline__ void bin_gather(BinApplyLds& L, uint2* hrow2, const uint4* ev, uint32_t nw) {
+template <bool kRec>
+__device__ __forceinline__ void bin_gather(BinApplyLds& L, uint2* hrow2, const uint4* ev, uint32_t nw, RecCtx rc) {
     static_assert(kMaxWriters == 2 * kApply, "two writers per thread");
     const uint32_t t = threadIdx.x;
     uint4* hrow = reinterpret_cast<uint4*>(hrow2);  // rows hold an even number of writers
@@ -1273,11 +1395,11 @@ __device__ __forceinline__ void bin_gather(BinApplyLds& L, uint2* hrow2, const u
                 for (uint32_t i = t; i < seg; i += kApply) {
                     const uint4 r = rec[i];
                     if (r.x < w0) continue;
-                    if (r.x - w0 < kBinCap) bin_link(L, atomicAdd(&L.m, 1u), r.z, r.x, __uint_as_float(r.y));
+                    if (r.x - w0 < kBinCap) bin_link(L, atomicAdd(&L.m, 1u), r.z, r.x, __uint_as_float(r.y), kRec ? r.w : 0u);
                     else atomicMin(&L.next, r.x);
                 }
                 __syncthreads();
-                bin_accumulate(L, L.m);
+                bin_accumulate<kRec>(L, L.m, rc);
                 if (L.next == kListNil) break;
                 w0 = L.next;
             }
@@ -1286,10 +1408,10 @@ __device__ __forceinline__ void bin_gather(BinApplyLds& L, uint2* hrow2, const u
             continue;
         }
         const uint32_t m = L.seg_pref[jb] - f0;
-        constexpr uint32_t kRec = kBinCap / kApply;
-        uint4 r[kRec];
+        constexpr uint32_t kRecs = kBinCap / kApply;
+        uint4 r[kRecs];
 #pragma unroll
-        for (uint32_t u = 0; u < kRec; ++u) {
+        for (uint32_t u = 0; u < kRecs; ++u) {
             const uint32_t f = t + u * kApply;
             if (f < m) {
                 const uint32_t sg = floor_index<kMaxWriters>(L.seg_pref, ja, jb, f0 + f);  // group of the record
@@ -1297,12 +1419,12 @@ __device__ __forceinline__ void bin_gather(BinApplyLds& L, uint2* hrow2, const u
             }
         }
 #pragma unroll
-        for (uint32_t u = 0; u < kRec; ++u) {
+        for (uint32_t u = 0; u < kRecs; ++u) {
             const uint32_t f = t + u * kApply;
-            if (f < m) bin_link(L, f, r[u].z, r[u].x, __uint_as_float(r[u].y));
+            if (f < m) bin_link(L, f, r[u].z, r[u].x, __uint_as_float(r[u].y), kRec ? r[u].w : 0u);
         }
         __syncthreads();
-        bin_accumulate(L, m);
+        bin_accumulate<kRec>(L, m, rc);
         ja = jb;
     }
     __syncthreads();
@@ -1312,6 +1434,7 @@ __device__ __forceinline__ void bin_gather(BinApplyLds& L, uint2* hrow2, const u
 // XCD-ordered list.  mode 0: the step's apply (slot from bin_slot, then the
 // step counter moves on); mode 1: a megastep network's pending events (the
 // previous step's buffer) into ring slot step % 32, unless already there.
+template <bool kRec>
 __global__ __launch_bounds__(kApply) void bin_apply_kernel(const NetArgs* __restrict__ nets,
                                                           const uint2* __restrict__ refs, uint32_t mode) {
     __shared__ BinApplyLds L;
@@ -1338,7 +1461,7 @@ __global__ __launch_bounds__(kApply) void bin_apply_kernel(const NetArgs* __rest
         L.slice[t + pr * kApply] = 0.0f;
         L.head[t + pr * kApply] = kListNil;
     }
-    bin_gather(L, hdr + static_cast<size_t>(k) * a.bin_rows, ev, a.n_bblk);
+    bin_gather<kRec>(L, hdr + static_cast<size_t>(k) * a.bin_rows, ev, a.n_bblk, RecCtx{&a, k});
     // cell c of the bin -> its post in the slot
     float* out = a.ring + static_cast<size_t>(slot) * a.N;
 #pragma unroll
@@ -1417,8 +1540,8 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
             L.head[t + pr * kApply] = kListNil;
         }
         const uint32_t rpar = wpar ^ 1u;
-        bin_gather(L, a.bin_hdr + rpar * a.hdr_stride + static_cast<size_t>(k) * a.bin_rows, a.bin_ev + rpar * a.n_ev,
-                   a.n_bblk);
+        bin_gather<false>(L, a.bin_hdr + rpar * a.hdr_stride + static_cast<size_t>(k) * a.bin_rows,
+                          a.bin_ev + rpar * a.n_ev, a.n_bblk, RecCtx{nullptr, 0u});
 #pragma unroll
         for (uint32_t q = 0; q < 2; ++q) cur[q] = L.slice[t + q * kApply];
     }
@@ -1947,7 +2070,11 @@ void launch_bin(const NetArgs* nets, const uint4* refs, uint32_t n, uint32_t bin
 }
 void launch_bin_apply(const NetArgs* nets, const uint2* refs, uint32_t n, uint32_t mode, hipStream_t s) {
     if (n == 0) return;
-    bin_apply_kernel<<<n, kApply, 0, s>>>(nets, refs, mode);
+    bin_apply_kernel<false><<<n, kApply, 0, s>>>(nets, refs, mode);
+}
+void launch_bin_apply_rec(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s) {
+    if (n == 0) return;
+    bin_apply_kernel<true><<<n, kApply, 0, s>>>(nets, refs, 0u);
 }
 void launch_megastep(int model, const NetArgs* nets, const uint2* items, uint32_t n, hipStream_t s) {
     if (n == 0) return;

@@ -11,6 +11,12 @@ constexpr uint32_t kWavesPerTile = kTile / 64;
 constexpr uint32_t kSlots = 32;          // P/src/api/net/graph.rs:33
 constexpr uint32_t kEmpty = 0xFFFFFFFFu; // end of a scatter chain
 constexpr int kMaxParams = 16;
+// Receptor types AMPA, NMDA, GABA_A, GABA_B, nAChR, GluCl, mAChR: decay time
+// constant (ms) and reversal potential (mV), P/src/kernels/synapse/receptor.rs:26-99.
+constexpr int kReceptors = 7;
+constexpr float kRecTau[kReceptors] = {2.0f, 50.0f, 10.0f, 200.0f, 3.0f, 5.0f, 150.0f};
+constexpr float kRecErev[kReceptors] = {0.0f, 0.0f, -65.0f, -90.0f, 0.0f, -70.0f, -90.0f};
+constexpr int kRecNmda = 1;  // the receptor with the Mg2+ block
 constexpr uint32_t kFixTiles = 8;        // scatter tiles per fix-up block
 constexpr uint32_t kMaxGroup = 2;        // neuron tiles per block of the neuron kernel
 constexpr uint32_t kMaxNeurons = 1u << 27;  // per network: slot*N + post fits 32 bits
@@ -58,6 +64,7 @@ struct PopDev {
     uint32_t drive_seed0;  // seed of neuron i: drive_seed0 + i * 2654435761 (mod 2^32)
     uint64_t seed;
     float puni[kMaxParams];
+    float gdec[kReceptors];  // receptor conductance decay per step, fast_exp(-dt / tau_r)
 };
 
 // Everything one step of one network touches.  An engine keeps an array of
@@ -141,6 +148,12 @@ struct NetArgs {
     uint32_t* tm_last;
     const float4* tm_prof;      // {u0, e^(-dt/tau_rec), e^(-dt/tau_fac), facilitating}
     float* ev_w;                // delivered weight of each event this step (for the fix-up)
+    // conductance-based receptor synapses (rec_code == nullptr: off)
+    const uint8_t* rec_code;    // per synapse: 0 current-based, 1 + receptor type
+    float* g;                   // [kReceptors][N] conductances
+    const float* g_ext;         // [N] constant external current, or nullptr
+    float rec_scale;            // a jump adds weight * rec_scale
+    uint32_t n8;                // N & ~7: neurons below it sum their receptor currents onto the input
 };
 
 }  // namespace simuse

@@ -26,6 +26,7 @@
 //   tm.r tm.u f32, tm.last i32        Tsodyks-Markram state (when enabled)
 //   stdp.w64 u64 (FP64 bit patterns), stdp.pre stdp.post f32  (when enabled)
 //   stdp.dense_trace f32              trace of the dense stdp_step rule
+//   receptors.g f32   [7][N]          receptor conductances (when enabled)
 //
 // The raster recorder is not part of the state: a network that records
 // restarts its raster at the loaded step.
@@ -36,7 +37,7 @@ namespace simuse {
 namespace {
 
 constexpr uint64_t kStateFormat = 1;
-constexpr size_t kFpHead = 16;  // fixed fingerprint entries before the per-population ones
+constexpr size_t kFpHead = 18;  // fixed fingerprint entries before the per-population ones
 constexpr size_t kFpPop = 8;
 
 const char* fp_name(size_t i) {
@@ -45,7 +46,8 @@ const char* fp_name(size_t i) {
                                         "plasticity",       "short-term plasticity",
                                         "CSR row_ptr",      "CSR col",          "CSR delays",
                                         "Poisson sources",  "STDP parameters",  "plastic mask",
-                                        "synapse types",    "synapse profiles"};
+                                        "synapse types",    "synapse profiles", "receptor types",
+                                        "receptor scale and external current"};
     static const char* pop[kFpPop] = {"model",       "size",        "state columns", "parameter columns",
                                       "host stride", "noise/drive flags", "noise/drive settings", "noise sigma"};
     return i < kFpHead ? head[i] : pop[(i - kFpHead) % kFpPop];
@@ -124,6 +126,11 @@ int NetState::fingerprint(std::vector<uint64_t>& fp) const {
     if (tm_type) {
         fp[14] = taas::crc32(h_tm_type.data(), h_tm_type.size());
         fp[15] = taas::crc32(h_tm_prof.data(), sizeof(float4) * h_tm_prof.size());
+    }
+    if (g) {
+        fp[16] = (uint64_t(1) << 32) | taas::crc32(h_rec_code.data(), h_rec_code.size());
+        const uint32_t sc = f_bits(rec_scale);
+        fp[17] = taas::crc32(h_g_ext.data(), 4 * h_g_ext.size(), taas::crc32(&sc, 4));
     }
     for (const HostPop& P : pops) {
         fp.push_back(static_cast<uint64_t>(P.model));
@@ -214,6 +221,10 @@ int NetState::save_state(const char* path, hipStream_t stream) {
     if (dtrace) {
         if ((rc = download(f, dtrace, N))) return rc;
         push(out, "stdp.dense_trace", "f32", f);
+    }
+    if (g) {
+        if ((rc = download(f, g, static_cast<size_t>(kReceptors) * N))) return rc;
+        push(out, "receptors.g", "f32", f);
     }
     if (out.to_file(path)) return fail(SIMUSE_E_INVALID, "save_state: cannot write %s", path);
     return SIMUSE_OK;
@@ -319,6 +330,11 @@ int NetState::load_state(const char* path, hipStream_t stream) {
         SNAP_GET(a, "stdp.dense_trace", "f32", N)
         s_dt = a;
     }
+    const uint8_t* s_g = nullptr;
+    if (g) {
+        SNAP_GET(a, "receptors.g", "f32", static_cast<uint64_t>(kReceptors) * N)
+        s_g = a;
+    }
 #undef SNAP_GET
 
     // Upload.  The work of earlier steps is finished first.
@@ -346,6 +362,7 @@ int NetState::load_state(const char* path, hipStream_t stream) {
         HIP_TRY(hipMemcpy(tr_pre, s_pre, 4 * static_cast<size_t>(N), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(tr_post, s_post, 4 * static_cast<size_t>(N), hipMemcpyHostToDevice));
     }
+    if (s_g) HIP_TRY(hipMemcpy(g, s_g, 4 * static_cast<size_t>(kReceptors) * N, hipMemcpyHostToDevice));
     if (s_dt) {
         if (!dtrace && (rc = alloc(&dtrace, N))) return rc;
         HIP_TRY(hipMemcpy(dtrace, s_dt, 4 * static_cast<size_t>(N), hipMemcpyHostToDevice));

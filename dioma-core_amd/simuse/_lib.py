@@ -104,6 +104,8 @@ _SIGS = {
     "simuse_net_download_weights": ([P, f32p, C.c_uint64], C.c_int),
     "simuse_net_inject_current": ([P, C.c_int32, u32p, f32p, C.c_uint32], C.c_int),
     "simuse_net_reset": ([P, C.POINTER(PopDesc), C.c_int32], C.c_int),
+    "simuse_net_set_receptors": ([P, u8p, C.c_float, f32p], C.c_int),
+    "simuse_net_download_conductances": ([P, f32p, C.c_uint64], C.c_int),
     "simuse_net_save_state": ([P, C.c_char_p], C.c_int),
     "simuse_net_load_state": ([P, C.c_char_p], C.c_int),
     "simuse_net_step_counter": ([P, u64p], C.c_int),

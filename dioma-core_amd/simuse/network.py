@@ -245,6 +245,28 @@ class CompiledNetwork:
         descs = _descs(self.pops)
         check(lib().simuse_net_reset(self._h, descs, len(self.pops)))
 
+    # -- receptor synapses ----------------------------------------------------------
+    def set_receptors(self, types, scale: float = 1.0, ext=None) -> None:
+        """Conductance-based synapses: receptor type per synapse (0 AMPA, 1 NMDA,
+        2 GABA_A, 3 GABA_B, 4 nAChR, 5 GluCl, 6 mAChR, 255 current-based), jump
+        scale, optional constant external current per neuron."""
+        t = np.ascontiguousarray(types, dtype=np.uint8)
+        if t.size != self.n_syn:
+            raise ValueError(f"receptor types: {t.size} values for {self.n_syn} synapses")
+        x = None
+        if ext is not None:
+            x = np.ascontiguousarray(ext, dtype=np.float32)
+            if x.size != self._n_total:
+                raise ValueError("external current: one value per neuron")
+        check(lib().simuse_net_set_receptors(self._h, _ptr(t, C.c_uint8), float(scale),
+                                             _ptr(x, C.c_float) if x is not None else None))
+        self._rec_keep = (t, x)
+
+    def download_conductances(self) -> np.ndarray:
+        out = np.zeros((7, self._n_total), dtype=np.float32)
+        check(lib().simuse_net_download_conductances(self._h, _ptr(out, C.c_float), out.size))
+        return out
+
     # -- checkpoint / resume --------------------------------------------------------
     def save_state(self, path) -> None:
         """Write the network's whole dynamic state to a ``.taas`` v2 file."""

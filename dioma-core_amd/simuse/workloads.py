@@ -42,6 +42,9 @@ class NetSpec:
     # Tsodyks-Markram short-term plasticity: {profiles: [(u0, tau_rec, tau_fac)], dt,
     # syn_types: uint8[n_syn] (0 = static), src_types: uint8[source edges] or absent}
     stp: Optional[dict] = None
+    # conductance-based receptor synapses: {types: uint8[n_syn] (0..6, 255 current-based),
+    # scale: float, ext: float32[n_total] or absent}
+    receptors: Optional[dict] = None
 
     @property
     def n_total(self) -> int:
@@ -220,6 +223,8 @@ def build_device(spec: NetSpec, graph_steps: int = 8) -> CompiledNetwork:
     if spec.stp:
         net.set_short_term_plasticity(spec.stp["profiles"], spec.stp["dt"], spec.stp["syn_types"],
                                       spec.stp.get("src_types"))
+    if spec.receptors:
+        net.set_receptors(spec.receptors["types"], spec.receptors.get("scale", 1.0), spec.receptors.get("ext"))
     net.set_graph_steps(graph_steps)
     return net
 

@@ -188,6 +188,27 @@ int simuse_net_inject_current(simuse_net* net, int32_t pop, const uint32_t* idx,
  * step counter and traces. */
 int simuse_net_reset(simuse_net* net, const simuse_pop_desc* pops, int32_t n_pops);
 
+/* Conductance-based receptor synapses (SynapticKinetics,
+ * P/src/kernels/synapse/kinetics.rs:67-233, receptor.rs:26-150; the step of
+ * the reference's conductance simulations, CN/src/sim/hh.rs:115-166 with
+ * CN = crates/SiMuse/crates/pillars-connectome/).  Synapse e gets receptor
+ * type receptor_types[e]: 0 AMPA, 1 NMDA, 2 GABA_A, 3 GABA_B, 4 nAChR,
+ * 5 GluCl, 6 mAChR (other values AMPA, as ReceptorType::from_u8), or 255 to
+ * stay current-based.  Every step, before the update: each neuron's seven
+ * conductances decay by fast_exp(-dt/tau_r) (dt of its population), the
+ * constant external current ext[i] (NULL: none) is added to the input, then
+ * sum_r g_r (E_r - v) with v the membrane potential before the update and
+ * the NMDA Mg2+ block 1/(1 + 0.33 fast_exp(clamp(-0.06 v, -10, 10))).  A
+ * spike jumps g[type][post] += w*scale in push order, seen from the next
+ * step on (propagate.rs:225-290).  Neurons below (n_total & ~7) add the seven
+ * currents onto the input one by one, the rest add their sum (the
+ * reference's 8-lane / scalar-tail split).  Poisson sources stay
+ * current-based.  Call before the first step; needs every delay <= 1
+ * (SIMUSE_E_UNSUPPORTED at the first step otherwise). */
+int simuse_net_set_receptors(simuse_net* net, const uint8_t* receptor_types, float scale, const float* ext);
+/* Conductances [7][n_total] (receptor-major). */
+int simuse_net_download_conductances(simuse_net* net, float* out, uint64_t len);
+
 /* Checkpoint / resume: the reference saves download_states /
  * download_weights (PC/src/network.rs:481,736) as named sections of a
  * `.taas` v2 file (S/crates/taas-format/src/writer.rs:73-99, lib.rs:8-24)

@@ -65,6 +65,11 @@ _SIGS = {
     "orc_vtrap_f32": ([C.c_float, C.c_float], C.c_float),
     "orc_vtrap_f64": ([C.c_double, C.c_double], C.c_double),
     "orc_compact": ([i32p, C.c_uint32, u32p], C.c_uint32),
+    "orc_net_set_receptors": ([P, C.c_void_p, C.c_float, C.c_void_p], C.c_int),
+    "orc_net_conductances": ([P, f64p], None),
+    "orc_rec_decay_factors": ([C.c_float, f32p], None),
+    "orc_nmda_block": ([C.c_float], C.c_float),
+    "orc_rec_currents": ([f32p, f32p, f32p, C.c_uint32], None),
     "orc_kura_create": ([C.c_int, C.c_uint32, u32p, u32p, f32p, f32p, f32p, C.c_float], P),
     "orc_kura_destroy": ([P], None),
     "orc_kura_step_rk4": ([P, C.c_uint32], None),
@@ -129,6 +134,25 @@ def fast_exp(x):
     return float(lib().orc_fast_exp(x))
 
 
+def rec_decay_factors(dt):
+    out = np.zeros(7, np.float32)
+    lib().orc_rec_decay_factors(dt, _p(out, C.c_float))
+    return out
+
+
+def nmda_block(v):
+    return float(lib().orc_nmda_block(v))
+
+
+def rec_currents(g, v, i_out):
+    """SynapticKinetics::compute_currents: i_out += currents of g[7][n] at v."""
+    g = np.ascontiguousarray(g, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    out = np.array(i_out, dtype=np.float32, copy=True)
+    lib().orc_rec_currents(_p(g, C.c_float), _p(v, C.c_float), _p(out, C.c_float), v.size)
+    return out
+
+
 def gpea_rates(v, cai):
     """m_inf, h_inf, n_inf, p_inf, q_inf, r_inf, tau_m, tau_h, tau_n, tau_p, tau_q."""
     out = np.zeros(11, np.float32)
@@ -183,6 +207,13 @@ class OracleNetwork:
             assert L.orc_net_set_stp(self.h, prof, n_prof, float(spec.stp["dt"]), syn.ctypes.data,
                                      src.ctypes.data if src is not None else None, n_src) == 0
             self._stp_keep = (prof, syn, src)
+        rec = getattr(spec, "receptors", None)
+        if rec:
+            types = np.ascontiguousarray(rec["types"], np.uint8)
+            ext = rec.get("ext")
+            ext = np.ascontiguousarray(ext, np.float32) if ext is not None else None
+            assert L.orc_net_set_receptors(self.h, types.ctypes.data, float(rec.get("scale", 1.0)),
+                                           ext.ctypes.data if ext is not None else None) == 0
         if threads > 1:
             L.orc_net_set_threads(self.h, threads)
 
@@ -235,6 +266,11 @@ class OracleNetwork:
     def weights(self):
         out = np.zeros(self.spec.n_syn, np.float64)
         lib().orc_net_weights(self.h, _p(out, C.c_double))
+        return out
+
+    def conductances(self):
+        out = np.zeros((7, self.N), np.float64)
+        lib().orc_net_conductances(self.h, _p(out, C.c_double))
         return out
 
     def counters(self):

@@ -256,6 +256,45 @@ template <> struct Math<double> {
 template <class R> static inline R maxr(R a, R b) { return a > b ? a : b; }
 
 // ---------------------------------------------------------------------------
+// Conductance-based receptor synapses
+// ---------------------------------------------------------------------------
+// ReceptorType (P/src/kernels/synapse/receptor.rs:26-99): AMPA, NMDA, GABA_A,
+// GABA_B, nAChR, GluCl, mAChR with decay time constants (ms) and reversal
+// potentials (mV).
+static const int kRec = 7;
+static const float kRecTauF[kRec] = {2.0f, 50.0f, 10.0f, 200.0f, 3.0f, 5.0f, 150.0f};
+static const float kRecErevF[kRec] = {0.0f, 0.0f, -65.0f, -90.0f, 0.0f, -70.0f, -90.0f};
+
+// DecayFactors::new (kinetics.rs:24-38): fast_exp(-dt / tau) per receptor.
+template <class R> static inline R rec_decay(R dt, int r) { return Math<R>::fast_exp(-dt / R(kRecTauF[r])); }
+
+// nmda_mg_block (receptor.rs:126-150): 1 / (1 + 0.33 fast_exp(clamp(-0.06 v, -10, 10))).
+template <class R> static inline R nmda_block(R v) {
+    R x = R(-0.06f) * v;
+    x = x < R(-10) ? R(-10) : x;
+    x = x > R(10) ? R(10) : x;
+    return R(1) / (R(1) + R(0.33f) * Math<R>::fast_exp(x));
+}
+
+// SynapticKinetics::compute_currents (kinetics.rs:157-205) for neuron i with
+// conductances g[r] (already decayed), potential v, input `base`: neurons
+// below n8 = n & ~7 run in 8-wide lanes, base + t_0 + ... + t_6 added in
+// turn; the scalar tail adds base + (t_0 + ... + t_6).
+template <class R> static inline R rec_current(const R* g, R v, R base, bool lane) {
+    const R b = nmda_block<R>(v);
+    R t[kRec];
+    for (int r = 0; r < kRec; ++r) t[r] = (r == 1 ? g[r] * b : g[r]) * (R(kRecErevF[r]) - v);
+    if (lane) {
+        R cur = base;
+        for (int r = 0; r < kRec; ++r) cur = cur + t[r];
+        return cur;
+    }
+    R acc = R(0);
+    for (int r = 0; r < kRec; ++r) acc = acc + t[r];
+    return base + acc;
+}
+
+// ---------------------------------------------------------------------------
 // Neuron models
 // ---------------------------------------------------------------------------
 
@@ -480,6 +519,8 @@ struct NetBase {
     virtual int set_stp(const orc_stp_profile* profiles, int n_profiles, float dt, const uint8_t* syn_types,
                         const uint8_t* src_types, uint64_t n_src_edges) = 0;
     virtual int inject(int pop, const uint32_t* idx, const float* val, uint32_t count) = 0;
+    virtual int set_receptors(const uint8_t* types, float scale, const float* ext) = 0;
+    virtual void conductances(double* out) const = 0;
     virtual void step(uint32_t n) = 0;
     virtual void stdp_step(float decay, float lr_ltp, float lr_ltd, float w_min, float w_max) = 0;
     virtual uint64_t step_counter() const = 0;
@@ -543,6 +584,13 @@ struct Net final : NetBase {
     // dense every-synapse STDP of CompiledNetwork::stdp_step (one shared trace)
     std::vector<R> dtrace;
     bool dense = false;
+    // receptor synapses: per synapse 0 current-based or 1 + receptor type;
+    // conductances [7][N]; constant external current
+    bool rec_on = false;
+    std::vector<uint8_t> rec_code;
+    std::vector<R> gcond;
+    std::vector<float> g_ext;
+    float rec_scale = 1.0f;
     float d_lr_ltp = 0, d_lr_ltd = 0, d_wmin = 0, d_wmax = 0;
 
     int init(const orc_pop_desc* pd, int np_, const uint32_t* rp, const uint32_t* c,
@@ -656,6 +704,45 @@ struct Net final : NetBase {
         if (tm_type.empty() || tm_type[e] == 0) return weight;
         return tm_spike_r(tm_prof[tm_type[e] - 1], tm_r[e], tm_u[e], tm_last[e], weight, stepc);
     }
+    int set_receptors(const uint8_t* types, float scale, const float* ext) override {
+        // ReceptorType::from_u8 (receptor.rs:64-76): 0..6, anything else AMPA;
+        // 255 keeps a synapse current-based
+        rec_code.assign(col.size(), 0);
+        for (size_t e = 0; e < col.size(); ++e) {
+            const uint8_t r = types[e];
+            rec_code[e] = r == 255 ? 0 : r < kRec ? static_cast<uint8_t>(r + 1) : 1;
+        }
+        rec_on = true;
+        rec_scale = scale;
+        gcond.assign(static_cast<size_t>(kRec) * N, R(0));
+        g_ext.clear();
+        if (ext) g_ext.assign(ext, ext + N);
+        return 0;
+    }
+    void conductances(double* out) const override {
+        for (size_t k = 0; k < gcond.size(); ++k) out[k] = static_cast<double>(gcond[k]);
+    }
+    // Before the update (CN/src/sim/hh.rs:115-166): decay every conductance,
+    // then the input becomes (input + external) plus the receptor currents
+    // of the pre-update potential.
+    void receptor_inputs() {
+        const uint32_t n8 = N & ~7u;
+        for (const Pop& P : pops) {
+            R f[kRec];
+            for (int r = 0; r < kRec; ++r) f[r] = rec_decay<R>(P.pr[0], r);
+            for (uint32_t i = 0; i < P.n; ++i) {
+                const uint32_t gi = P.off + i;
+                R g[kRec];
+                for (int r = 0; r < kRec; ++r) {
+                    R& x = gcond[static_cast<size_t>(r) * N + gi];
+                    x = x * f[r];
+                    g[r] = x;
+                }
+                const R base = g_ext.empty() ? i_in[gi] : i_in[gi] + R(g_ext[gi]);
+                i_in[gi] = rec_current<R>(g, P.st[i], base, gi < n8);  // state column 0 = v
+            }
+        }
+    }
     int inject(int p, const uint32_t* idx, const float* val, uint32_t count) override {
         // Network::inject_current — P/src/api/net/network/build.rs:243-275
         if (p < 0 || p >= static_cast<int>(pops.size())) return -3;
@@ -754,6 +841,7 @@ struct Net final : NetBase {
         // 1-2: read slot -> i_in, clear slot (delay.rs:68-77)
         std::memcpy(i_in.data(), cur, sizeof(R) * N);
         std::fill(cur, cur + N, R(0));
+        if (rec_on) receptor_inputs();
         // 3: populations (population.rs:284-420)
         for (auto& P : pops) step_pop(P);
         // 4: compaction, ascending (compact.rs:30-61)
@@ -771,6 +859,12 @@ struct Net final : NetBase {
         //    rayon variant (push.rs:184-237) applies triples in the same order.
         for (uint32_t pre : act) {
             for (uint32_t e = row_ptr[pre]; e < row_ptr[pre + 1]; ++e) {
+                if (rec_on && rec_code[e]) {
+                    // conductance jump, in push order (CN/src/propagate.rs:225-290)
+                    R& x = gcond[static_cast<size_t>(rec_code[e] - 1) * N + col[e]];
+                    x = x + delivered(e, w[e]) * R(rec_scale);
+                    continue;
+                }
                 uint32_t dd = d[e] < 1 ? 1u : d[e];
                 uint32_t ts = static_cast<uint32_t>((stepc + dd) % kSlots);
                 ringv[static_cast<size_t>(ts) * N + col[e]] += delivered(e, w[e]);
@@ -951,6 +1045,22 @@ int orc_net_inject_current(void* net, int32_t pop, const uint32_t* idx, const fl
     return static_cast<NetBase*>(net)->inject(pop, idx, val, count);
 }
 void orc_net_step(void* net, uint32_t n_steps) { static_cast<NetBase*>(net)->step(n_steps); }
+int orc_net_set_receptors(void* net, const uint8_t* types, float scale, const float* ext) {
+    return static_cast<NetBase*>(net)->set_receptors(types, scale, ext);
+}
+void orc_net_conductances(void* net, double* out) { static_cast<NetBase*>(net)->conductances(out); }
+void orc_rec_decay_factors(float dt, float* out) {
+    for (int r = 0; r < kRec; ++r) out[r] = rec_decay<float>(dt, r);
+}
+float orc_nmda_block(float v) { return nmda_block<float>(v); }
+void orc_rec_currents(const float* g, const float* v, float* i_out, uint32_t n) {
+    const uint32_t n8 = n & ~7u;
+    for (uint32_t i = 0; i < n; ++i) {
+        float gi[kRec];
+        for (int r = 0; r < kRec; ++r) gi[r] = g[static_cast<size_t>(r) * n + i];
+        i_out[i] = rec_current<float>(gi, v[i], i_out[i], i < n8);
+    }
+}
 int orc_net_set_stp(void* net, const orc_stp_profile* profiles, int32_t n_profiles, float dt,
                     const uint8_t* syn_types, const uint8_t* src_types, uint64_t n_src_edges) {
     return static_cast<NetBase*>(net)->set_stp(profiles, n_profiles, dt, syn_types, src_types, n_src_edges);

@@ -72,6 +72,14 @@ int orc_net_set_plasticity(void* net, const orc_stdp_params* p, const uint8_t* p
  * for the Poisson-source edges (src_types may be NULL); profiles are 1-based. */
 int orc_net_set_stp(void* net, const orc_stp_profile* profiles, int32_t n_profiles, float dt,
                     const uint8_t* syn_types, const uint8_t* src_types, uint64_t n_src_edges);
+/* Conductance-based receptor synapses (P/src/kernels/synapse/kinetics.rs,
+ * receptor.rs; step order of CN/src/sim/hh.rs:115-166). */
+int orc_net_set_receptors(void* net, const uint8_t* types, float scale, const float* ext);
+void orc_net_conductances(void* net, double* out); /* [7][n_total] */
+void orc_rec_decay_factors(float dt, float* out);  /* 7 values */
+float orc_nmda_block(float v);
+/* compute_currents: i_out[i] += receptor currents of g[7][n] at v[i] */
+void orc_rec_currents(const float* g, const float* v, float* i_out, uint32_t n);
 int orc_net_inject_current(void* net, int32_t pop, const uint32_t* idx,
                            const float* val, uint32_t count);
 void orc_net_step(void* net, uint32_t n_steps);

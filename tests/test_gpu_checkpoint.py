@@ -60,6 +60,16 @@ def _spec(kind):
         return spec
     if kind == "dense_stdp":
         return random_net(2500, 40, seed=31, model="lif", max_delay=3, sources=6)
+    if kind == "receptors":
+        spec = random_net(3001, 30, seed=61, model="lif", max_delay=1, sources=4)
+        rng = np.random.default_rng(61)
+        types = rng.integers(0, 7, spec.n_syn).astype(np.uint8)
+        types[rng.random(spec.n_syn) < 0.1] = 255
+        spec.w = np.where(types != 255, np.abs(spec.w) * 0.02, spec.w).astype(np.float32)
+        ext = np.zeros(3001, np.float32)
+        ext[:100] = 2.0
+        spec.receptors = dict(types=types, scale=1.0, ext=ext)
+        return spec
     raise ValueError(kind)
 
 
@@ -101,7 +111,8 @@ def _assert_same(a, b, spec):
     assert a.step_counter() == b.step_counter()
 
 
-@pytest.mark.parametrize("kind", ["binned_izh", "binned_mixed", "chained_lif", "hh", "stdp", "tm", "dense_stdp"])
+@pytest.mark.parametrize("kind", ["binned_izh", "binned_mixed", "chained_lif", "hh", "stdp", "tm", "dense_stdp",
+                                  "receptors"])
 def test_checkpoint_resume_bitexact(kind, tmp_path):
     say("checkpoint", kind)
     spec = _spec(kind)
@@ -144,6 +155,9 @@ def test_checkpoint_resume_bitexact(kind, tmp_path):
             assert np.array_equal(st[c, : pop.n].view(np.uint32), ora.state(p, c).astype(np.float32).view(np.uint32))
     assert np.array_equal(b.download_weights().view(np.uint32), ora.weights().astype(np.float32).view(np.uint32))
     assert np.array_equal(b.download_ring().view(np.uint32), ora.ring().astype(np.float32).view(np.uint32))
+    if spec.receptors:
+        assert np.array_equal(b.download_conductances().view(np.uint32),
+                              ora.conductances().astype(np.float32).view(np.uint32))
 
 
 def test_snapshot_file_contents(tmp_path):
