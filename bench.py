@@ -179,15 +179,18 @@ def bench_serving(args) -> dict:
 
 def bench_precision(args) -> dict:
     """Precision configuration: the Hodgkin-Huxley network with heavy-tailed
-    out-degree and pair STDP on its excitatory synapses (dt = 0.01 ms).  The
-    device state (FP32) is compared with the oracle's FP64 twin after every
-    window of 1000 steps on a reduced network (the oracle runs on the host
-    cores), and the device's step time is taken at the full 200k-neuron size."""
+    out-degree, lognormal conductance synapses (AMPA from excitatory, GABA_A
+    from inhibitory neurons) and pair STDP on its excitatory synapses
+    (dt = 0.01 ms).  The device state (FP32) is compared with the oracle's
+    FP64 twin after every window of 1000 steps on a reduced network (the
+    oracle runs on the host cores), and the device's step time is taken at
+    the full 200k-neuron size."""
     import pyoracle as po
     from simuse.workloads import build_device, hh_network
     n, windows = args.prec_neurons, args.prec_windows
-    spec = hh_network(n=n, seed=3, steps=windows * 1000, noise_sd=30.0)
-    log(f"precision: HH {n} neurons, {spec.n_syn} synapses, pair STDP, {windows} windows of 1000 steps vs FP64")
+    spec = hh_network(n=n, seed=3, steps=windows * 1000, noise_sd=30.0, receptors=True)
+    log(f"precision: HH {n} neurons, {spec.n_syn} conductance synapses, pair STDP, {windows} windows of 1000 steps "
+        "vs FP64")
     dev = build_device(spec, graph_steps=16)
     o64 = po.OracleNetwork(spec, 64, threads=min(16, os.cpu_count() or 1))
     plastic = spec.stdp["plastic"].astype(bool)
@@ -199,6 +202,9 @@ def bench_precision(args) -> dict:
         v_dev = dev.download_states(0)[0, :n].astype(np.float64)
         v64 = o64.state(0, 0)
         dv = np.abs(v_dev - v64)
+        g_dev = dev.download_conductances().astype(np.float64)
+        g64 = o64.conductances()
+        dg = np.abs(g_dev - g64)
         w_dev = dev.download_weights().astype(np.float64)[plastic]
         w64 = o64.weights()[plastic]
         dw = np.abs(w_dev - w64)
@@ -207,13 +213,15 @@ def bench_precision(args) -> dict:
         rate_ora = (s_ora - s_ora0) / n / (1000 * spec.dt * 1e-3)
         s_dev0, s_ora0 = s_dev, s_ora
         rows.append({"window": k, "v_mean_abs_err_mV": float(dv.mean()), "v_max_abs_err_mV": float(dv.max()),
+                     "g_mean_abs_err": float(dg.mean()), "g_max_abs_err": float(dg.max()),
                      "w_rms_err": float(np.sqrt(np.mean(dw ** 2))), "w_max_abs_err": float(dw.max()),
                      "rate_hz_fp32": float(rate_dev), "rate_hz_fp64": float(rate_ora),
                      "rate_rel_err": float(abs(rate_dev - rate_ora) / max(rate_ora, 1e-12))})
-        log(f"  window {k}: |dv| mean {dv.mean():.3e} max {dv.max():.3e} mV, w rms {rows[-1]['w_rms_err']:.3e}, "
-            f"rate {rate_dev:.2f} vs {rate_ora:.2f} Hz")
+        log(f"  window {k}: |dv| mean {dv.mean():.3e} max {dv.max():.3e} mV, |dg| max {dg.max():.3e}, "
+            f"w rms {rows[-1]['w_rms_err']:.3e}, rate {rate_dev:.2f} vs {rate_ora:.2f} Hz")
     dev.close()
-    big = hh_network(n=args.prec_full_neurons, seed=3, steps=args.prec_full_steps + 200, noise_sd=30.0)
+    big = hh_network(n=args.prec_full_neurons, seed=3, steps=args.prec_full_steps + 200, noise_sd=30.0,
+                     receptors=True)
     net = build_device(big, graph_steps=16)
     net.run(200)
     net.sync()
@@ -224,9 +232,9 @@ def bench_precision(args) -> dict:
     spikes, events = net.counters()
     net.close()
     log(f"  full size {args.prec_full_neurons} neurons / {big.n_syn} synapses: {us:.2f} us/step")
-    return {"workload": f"HH {n} neurons ({spec.n_syn} synapses), Pareto out-degree, lognormal weights, "
-                        "pair STDP (A+ 0.01, A- 0.012, tau 20 ms, [0,1]) on excitatory synapses, Gaussian input sd 30, "
-                        "dt 0.01 ms",
+    return {"workload": f"HH {n} neurons ({spec.n_syn} synapses), Pareto out-degree, lognormal conductances "
+                        "(AMPA excitatory, GABA_A inhibitory, 0.003 mS/cm2 units), pair STDP (A+ 0.01, A- 0.012, "
+                        "tau 20 ms, [0,1]) on excitatory synapses, Gaussian input sd 30, dt 0.01 ms",
             "reference": "oracle FP64 twin", "windows": rows,
             "full_size": {"neurons": args.prec_full_neurons, "synapses": big.n_syn,
                           "steps": args.prec_full_steps, "us_per_step": us}}

@@ -173,15 +173,19 @@ def brunel_lif_network(n: int = 10_000, p: float = 0.01, seed: int = 1, steps: i
 
 
 def hh_network(n: int = 200_000, seed: int = 1, steps: int = 50_000, mean_deg: float = 200.0,
-               w_scale: float = 1.0, noise_sd: float = 3.0, stdp: bool = True) -> NetSpec:
+               w_scale: float = 1.0, noise_sd: float = 3.0, stdp: bool = True, receptors: bool = False,
+               g_unit: float = 0.003) -> NetSpec:
     """Hodgkin–Huxley network with heavy-tailed out-degree and pair STDP.
 
     Out-degree ceil(x_m * U^(-1/2)) (Pareto alpha=2, x_m = mean/2, mean 200),
     capped at n-1; excitatory weights lognormal(mu=-1, sigma=1) scaled by
-    w_scale (current pulses delivered through the ring), 20% inhibitory
-    neurons with weight -lognormal; dt = 0.01 ms; Gaussian current noise
-    keeps the network active.  STDP (A+=0.01, A-=0.012,
-    tau=20 ms, clip [0,1]) on excitatory synapses.
+    w_scale, 20% inhibitory neurons with weight -lognormal; dt = 0.01 ms;
+    Gaussian current noise keeps the network active.  STDP (A+=0.01,
+    A-=0.012, tau=20 ms, clip [0,1]) on excitatory synapses.
+    Without `receptors` the weights are current pulses through the ring;
+    with it they are conductances: excitatory synapses AMPA, inhibitory ones
+    GABA_A (the lognormal magnitude), each spike jumping g by weight * g_unit
+    (mS/cm^2; 0.003 keeps the network asynchronous at about 10 Hz).
     """
     rng = np.random.default_rng(seed)
     dt = 0.01
@@ -203,6 +207,9 @@ def hh_network(n: int = 200_000, seed: int = 1, steps: int = 50_000, mean_deg: f
                    np.ones(ns, dtype=np.uint8), dt, steps)
     if noise_sd > 0:
         spec.noise.append((0, np.full(n, noise_sd, np.float32), seed * 104729 + 3))
+    if receptors:
+        spec.w = np.abs(w).astype(np.float32)
+        spec.receptors = dict(types=np.where(exc, 0, 2).astype(np.uint8), scale=float(g_unit))
     if stdp:
         spec.stdp = dict(a_plus=0.01, a_minus=0.012, tau_plus=20.0, tau_minus=20.0, w_min=0.0, w_max=1.0,
                          plastic=exc.astype(np.uint8))

@@ -29,12 +29,22 @@ def _drive(spec, w):
     return spec
 
 
+def _receptors(spec, g_scale):
+    # conductance synapses: excitatory neurons' AMPA, inhibitory ones' GABA_A
+    pre = np.repeat(np.arange(spec.n_total), np.diff(spec.row_ptr.astype(np.int64)))
+    spec.w = (np.abs(spec.w) * g_scale).astype(np.float32)
+    spec.receptors = dict(types=np.where(pre < spec.n_total * 4 // 5, 0, 2).astype(np.uint8), scale=1.0)
+    return spec
+
+
 CASES = {
     "izhikevich": (lambda: izhikevich_network(n=10000, k=100, seed=3), 2000, 1e-3),
     "lif": (lambda: random_net(5000, 50, seed=5, model="lif", max_delay=5, noise=3.0), 3000, 1e-4),
     "hh": (lambda: _drive(random_net(1000, 40, seed=13, model="hh", max_delay=4, noise=40.0), 40.0), 20000, 1e-5),
     "gpea": (lambda: _drive(random_net(1000, 40, seed=13, model="gpea", max_delay=4, noise=60.0), 40.0), 8000,
              2.5e-5),
+    "hh_receptors": (lambda: _receptors(_drive(random_net(1000, 40, seed=13, model="hh", max_delay=1, noise=40.0),
+                                               40.0), 0.005), 20000, 1e-5),
 }
 
 
