@@ -1554,11 +1554,7 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
         li[q] = (tile[q] - PP[q]->tile0) * kTile + tl;
         on[q] = tile[q] < a.n_tiles && li[q] < PP[q]->n;
         row0[q] = deg[q] = 0u;
-        if (on[q]) {
-            nrn[q].load(*PP[q], li[q]);
-            row0[q] = a.row_ptr[PP[q]->off + li[q]];
-            deg[q] = a.row_ptr[PP[q]->off + li[q] + 1] - row0[q];
-        }
+        if (on[q]) nrn[q].load(*PP[q], li[q]);
     }
 
     // advance every neuron; per tile ballot the spikes and scan the spiking
@@ -1587,6 +1583,10 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
             }
             spike = nrn[q].step(P, i, I, P.puni[0]);
             nrn[q].store(P, i);
+            if (spike) {  // only spiking rows need their CSR extent
+                row0[q] = a.row_ptr[P.off + i];
+                deg[q] = a.row_ptr[P.off + i + 1] - row0[q];
+            }
         }
         m[q] = __ballot(spike);
         const uint32_t drive_events = static_cast<uint32_t>(__popcll(__ballot(drive_now)));
