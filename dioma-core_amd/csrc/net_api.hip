@@ -915,6 +915,21 @@ struct simuse_net {
     NetState st;
 };
 
+// Host work that builds vectors proportional to the caller's data runs
+// through this, so that no C++ exception crosses the C ABI.
+template <class F>
+static int guarded(F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return fail(SIMUSE_E_NOMEM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(SIMUSE_E_INVALID, "host error: %s", e.what());
+    } catch (...) {
+        return fail(SIMUSE_E_INVALID, "unexpected host error");
+    }
+}
+
 static int check_net(simuse_net* net) {
     if (!net) return fail(SIMUSE_E_INVALID, "null network handle");
     return SIMUSE_OK;
@@ -954,14 +969,17 @@ int simuse_net_create(const simuse_pop_desc* pops, int32_t n_pops, const uint32_
         (void)hipGetLastError();
         return fail(SIMUSE_E_HIP, "no HIP device available");
     }
-    std::unique_ptr<simuse_net> net(new (std::nothrow) simuse_net());
-    if (!net) return fail(SIMUSE_E_NOMEM, "host allocation failed");
-    if ((rc = net->eng.init())) return rc;
-    net->st.eng = &net->eng;
-    if ((rc = net->st.init(pops, n_pops, row_ptr, col, w, delay, n_syn, net->eng.stream))) return rc;
-    net->eng.nets.push_back(&net->st);
-    *out = net.release();
-    return SIMUSE_OK;
+    return guarded([&]() {
+        std::unique_ptr<simuse_net> net(new (std::nothrow) simuse_net());
+        if (!net) return fail(SIMUSE_E_NOMEM, "host allocation failed");
+        int r;
+        if ((r = net->eng.init())) return r;
+        net->st.eng = &net->eng;
+        if ((r = net->st.init(pops, n_pops, row_ptr, col, w, delay, n_syn, net->eng.stream))) return r;
+        net->eng.nets.push_back(&net->st);
+        *out = net.release();
+        return SIMUSE_OK;
+    });
 }
 
 void simuse_net_destroy(simuse_net* net) {
@@ -974,7 +992,7 @@ int simuse_net_add_poisson_source(simuse_net* net, float rate_hz, float dt_ms, u
                                   const float* w, const uint8_t* delay, uint32_t count) {
     int rc = check_net(net);
     if (rc) return rc;
-    return net->st.add_source(rate_hz, dt_ms, seed, posts, w, delay, count);
+    return guarded([&]() { return net->st.add_source(rate_hz, dt_ms, seed, posts, w, delay, count); });
 }
 
 int simuse_net_set_noise(simuse_net* net, int32_t pop, const float* sigma, uint64_t seed) {
@@ -1067,7 +1085,7 @@ int simuse_net_set_short_term_plasticity(simuse_net* net, const simuse_stp_profi
                                          uint64_t n_src_edges) {
     int rc = check_net(net);
     if (rc) return rc;
-    return net->st.set_stp(profiles, n_profiles, dt, syn_types, src_types, n_src_edges);
+    return guarded([&]() { return net->st.set_stp(profiles, n_profiles, dt, syn_types, src_types, n_src_edges); });
 }
 
 int simuse_net_stdp_step(simuse_net* net, float trace_decay, float lr_ltp, float lr_ltd, float w_min,
@@ -1174,7 +1192,7 @@ int simuse_net_reset(simuse_net* net, const simuse_pop_desc* pops, int32_t n_pop
 int simuse_net_set_receptors(simuse_net* net, const uint8_t* receptor_types, float scale, const float* ext) {
     int rc = check_net(net);
     if (rc) return rc;
-    return net->st.set_receptors(receptor_types, scale, ext);
+    return guarded([&]() { return net->st.set_receptors(receptor_types, scale, ext); });
 }
 
 int simuse_net_download_conductances(simuse_net* net, float* out, uint64_t len) {
@@ -1186,17 +1204,22 @@ int simuse_net_download_conductances(simuse_net* net, float* out, uint64_t len) 
 int simuse_net_save_state(simuse_net* net, const char* path) {
     int rc = check_net(net);
     if (rc) return rc;
-    if ((rc = net->eng.prepare())) return rc;
-    return net->st.save_state(path, net->eng.stream);
+    return guarded([&]() {
+        int r = net->eng.prepare();
+        return r ? r : net->st.save_state(path, net->eng.stream);
+    });
 }
 
 int simuse_net_load_state(simuse_net* net, const char* path) {
     int rc = check_net(net);
     if (rc) return rc;
-    if ((rc = net->eng.prepare())) return rc;
-    rc = net->st.load_state(path, net->eng.stream);
-    net->eng.mark_dirty();
-    return rc;
+    return guarded([&]() {
+        int r = net->eng.prepare();
+        if (r) return r;
+        r = net->st.load_state(path, net->eng.stream);
+        net->eng.mark_dirty();
+        return r;
+    });
 }
 
 int simuse_net_step_counter(simuse_net* net, uint64_t* step) {

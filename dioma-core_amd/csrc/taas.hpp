@@ -17,6 +17,7 @@
 #include <cstring>
 #include <ctime>
 #include <map>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -257,19 +258,18 @@ private:
                 case 'r': out += '\r'; break;
                 case 't': out += '\t'; break;
                 case 'u': {
-                    if (i_ + 4 > s_.size()) return false;
-                    const unsigned v = static_cast<unsigned>(std::stoul(s_.substr(i_, 4), nullptr, 16));
-                    i_ += 4;
-                    if (v < 0x80) {
-                        out += static_cast<char>(v);
-                    } else if (v < 0x800) {
-                        out += static_cast<char>(0xC0 | (v >> 6));
-                        out += static_cast<char>(0x80 | (v & 0x3F));
-                    } else {
-                        out += static_cast<char>(0xE0 | (v >> 12));
-                        out += static_cast<char>(0x80 | ((v >> 6) & 0x3F));
-                        out += static_cast<char>(0x80 | (v & 0x3F));
+                    unsigned v;
+                    if (!hex4(v)) return false;
+                    if (v >= 0xD800 && v < 0xDC00) {  // high surrogate: a low one must follow
+                        unsigned lo;
+                        if (i_ + 2 > s_.size() || s_[i_] != '\\' || s_[i_ + 1] != 'u') return false;
+                        i_ += 2;
+                        if (!hex4(lo) || lo < 0xDC00 || lo >= 0xE000) return false;
+                        v = 0x10000 + ((v - 0xD800) << 10) + (lo - 0xDC00);
+                    } else if (v >= 0xDC00 && v < 0xE000) {
+                        return false;
                     }
+                    utf8(out, v);
                     break;
                 }
                 default: return false;
@@ -277,12 +277,48 @@ private:
         }
         return false;
     }
+    bool hex4(unsigned& v) {
+        if (i_ + 4 > s_.size()) return false;
+        v = 0;
+        for (int k = 0; k < 4; ++k) {
+            const char c = s_[i_ + k];
+            const int d = c >= '0' && c <= '9' ? c - '0' : c >= 'a' && c <= 'f' ? c - 'a' + 10
+                        : c >= 'A' && c <= 'F' ? c - 'A' + 10 : -1;
+            if (d < 0) return false;
+            v = v * 16 + static_cast<unsigned>(d);
+        }
+        i_ += 4;
+        return true;
+    }
+    static void utf8(std::string& out, unsigned v) {
+        if (v < 0x80) {
+            out += static_cast<char>(v);
+        } else if (v < 0x800) {
+            out += static_cast<char>(0xC0 | (v >> 6));
+            out += static_cast<char>(0x80 | (v & 0x3F));
+        } else if (v < 0x10000) {
+            out += static_cast<char>(0xE0 | (v >> 12));
+            out += static_cast<char>(0x80 | ((v >> 6) & 0x3F));
+            out += static_cast<char>(0x80 | (v & 0x3F));
+        } else {
+            out += static_cast<char>(0xF0 | (v >> 18));
+            out += static_cast<char>(0x80 | ((v >> 12) & 0x3F));
+            out += static_cast<char>(0x80 | ((v >> 6) & 0x3F));
+            out += static_cast<char>(0x80 | (v & 0x3F));
+        }
+    }
+    // A non-negative integer below 2^64.
     bool number(uint64_t& v) {
         ws();
         size_t j = i_;
-        while (j < s_.size() && s_[j] >= '0' && s_[j] <= '9') ++j;
+        v = 0;
+        while (j < s_.size() && s_[j] >= '0' && s_[j] <= '9') {
+            const uint64_t d = static_cast<uint64_t>(s_[j] - '0');
+            if (v > (~uint64_t(0) - d) / 10) return false;
+            v = v * 10 + d;
+            ++j;
+        }
         if (j == i_) return false;
-        v = std::stoull(s_.substr(i_, j - i_));
         i_ = j;
         return true;
     }
@@ -364,20 +400,26 @@ public:
             std::fclose(f);
             return err("file too short");
         }
-        bytes_.resize(static_cast<size_t>(len));
+        try {
+            bytes_.resize(static_cast<size_t>(len));
+        } catch (const std::bad_alloc&) {
+            std::fclose(f);
+            return err("file too large for host memory");
+        }
         const bool ok = std::fread(bytes_.data(), 1, bytes_.size(), f) == bytes_.size();
         std::fclose(f);
         if (!ok) return err("read failed");
         if (std::memcmp(bytes_.data(), "TAAS2", 5) != 0) return err("invalid magic (expected TAAS2)");
         uint64_t hl = 0;
         for (int i = 0; i < 8; ++i) hl |= static_cast<uint64_t>(bytes_[5 + i]) << (8 * i);
-        if (13 + hl > bytes_.size()) return err("manifest truncated");
+        if (hl > bytes_.size() - 13) return err("manifest truncated");
         const std::string man(reinterpret_cast<const char*>(bytes_.data() + 13), static_cast<size_t>(hl));
         data_ = 13 + static_cast<size_t>(hl);
         Json j(man);
         if (!j.manifest(sections, metadata, version, error)) return false;
+        const uint64_t avail = bytes_.size() - data_;
         for (const Section& s : sections)
-            if (data_ + s.offset + s.size > bytes_.size()) return err("a section exceeds the file");
+            if (s.offset > avail || s.size > avail - s.offset) return err("a section exceeds the file");
         return true;
     }
 
@@ -394,7 +436,7 @@ public:
         if (!s) return errp("missing section " + name);
         if (s->kind != kind) return errp("section " + name + " is " + s->kind + ", expected " + kind);
         if (s->compressed) return errp("section " + name + " is compressed (zstd is not supported)");
-        if (s->count != count || s->size != count * kind_size(kind))
+        if (s->count != count || s->size / kind_size(kind) != count || s->size % kind_size(kind))
             return errp("section " + name + " has " + std::to_string(s->count) + " elements, expected " +
                         std::to_string(count));
         const uint8_t* p = bytes_.data() + data_ + s->offset;

@@ -84,9 +84,19 @@ class SectionEntry:
 
     @staticmethod
     def from_json(d: dict) -> "SectionEntry":
+        def u64(x, bits=64):
+            # serde's usize / u64 / u32: a JSON integer in range, not a bool or a float
+            if isinstance(x, bool) or not isinstance(x, int) or not 0 <= x < (1 << bits):
+                raise ValueError(f"{x!r} is not an unsigned {bits}-bit integer")
+            return x
         try:
-            return SectionEntry(str(d["name"]), str(d["kind"]), int(d["count"]), int(d["offset"]), int(d["size"]),
-                                d.get("crc32"), d.get("compression"), d.get("uncompressed_size"))
+            crc = d.get("crc32")
+            unc = d.get("uncompressed_size")
+            comp = d.get("compression")
+            if comp is not None and not isinstance(comp, str):
+                raise ValueError("compression must be a string")
+            return SectionEntry(str(d["name"]), str(d["kind"]), u64(d["count"]), u64(d["offset"]), u64(d["size"]),
+                                None if crc is None else u64(crc, 32), comp, None if unc is None else u64(unc))
         except (KeyError, TypeError, ValueError) as e:
             raise TaasError(f"Invalid manifest JSON: bad section entry ({e})") from None
 
@@ -207,7 +217,9 @@ def _parse_header(b: bytes, magic: bytes):
         raise TaasError("Header truncated")
     try:
         man = json.loads(b[13:13 + hl].decode("utf-8"))
-    except (UnicodeDecodeError, json.JSONDecodeError) as e:
+        # a lone surrogate escape is not a Unicode string (serde_json refuses it)
+        json.dumps(man, ensure_ascii=False).encode("utf-8")
+    except (UnicodeDecodeError, UnicodeEncodeError, json.JSONDecodeError, RecursionError) as e:
         raise TaasError(f"Invalid manifest JSON: {e}") from None
     if not isinstance(man, dict):
         raise TaasError("Invalid manifest JSON: not an object")

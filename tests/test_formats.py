@@ -341,3 +341,32 @@ def test_connectome_csr_wiring():
     assert d.tolist() == [0, 0, 0]
     bad = ConnectomeData([0, 1, 1], [-5], [1.0], [1.0, 1.0])
     assert bad.csr()[1].tolist() == [0]
+
+
+@pytest.mark.parametrize("manifest,ok", [
+    (b'{"version":"2.0.0","created":"\\ud83d\\ude00","sections":[],"metadata":{"k":"\\u00e9"}}', True),
+    (b'{"version":"2.0.0","created":"\\uZZZZ","sections":[],"metadata":{}}', False),
+    (b'{"version":"2.0.0","created":"\\ud83d","sections":[],"metadata":{}}', False),
+    (b'{"version":"2.0.0","created":"x","sections":[{"name":"a","kind":"f32","count":99999999999999999999999,'
+     b'"offset":0,"size":4}],"metadata":{}}', False),
+    (b'{"version":"2.0.0","created":"x","sections":[{"name":"a","kind":"f32","count":1,'
+     b'"offset":18446744073709551615,"size":4}],"metadata":{}}', False),
+    (b'{"version":"2.0.0","created":"x","sections":[{"name":"a","kind":"f32","count":"one",'
+     b'"offset":0,"size":4}],"metadata":{}}', False),
+    (b'{"version":"2.0.0","created":"x","sections":[', False),
+    (b'[]', False),
+])
+def test_cpp_reader_rejects_malformed_manifests(taas_check, tmp_path, manifest, ok):
+    # the library's reader reports an error for every malformed manifest
+    # instead of throwing (nothing may cross the C ABI); the Python reader agrees
+    p = tmp_path / "m.taas"
+    p.write_bytes(TAAS_MAGIC + struct.pack("<Q", len(manifest)) + manifest + b"\x00\x00\x80\x3f")
+    res = _cpp(taas_check, "read", p)
+    assert res.returncode == (0 if ok else 1), (res.returncode, res.stdout, res.stderr)
+    if ok:
+        assert "meta \"k\" \"é\"" in res.stdout
+        TaasReader.from_file(p)
+    else:
+        assert res.stdout.startswith("error ")
+        with pytest.raises(TaasError):
+            TaasReader.from_file(p)
