@@ -480,7 +480,8 @@ struct Engine {
                 for (uint32_t t = 0; t < P.ntiles; t += group, ++j)
                     tl[P.model].push_back(make_uint4(k, P.tile0 + t, std::min(group, P.ntiles - t) | (j << 4), p));
             }
-            if (s->stdp)
+            // binned plastic networks run their STDP inside the writers (bin_kernel)
+            if (s->stdp && !s->binned)
                 for (uint32_t t = 0; t < s->n_tiles; ++t) sl.push_back(make_uint2(k, t));
             if (s->binned) {
                 std::vector<uint4>& wl = s->dense_writers;
@@ -502,6 +503,11 @@ struct Engine {
                     }
                 }
                 if (j > s->bin_rows) return fail(SIMUSE_E_STATE, "binned scatter: %u writers, %u rows", j, s->bin_rows);
+                // plastic networks: the LTP of the spiking posts' in-edges runs in
+                // blocks of its own beside the writers (which take the out-edges)
+                if (s->stdp)
+                    for (uint32_t b = 0; b < s->n_tiles; b += kBinBlockTiles)
+                        binl.push_back(make_uint4(k, b, std::min(kBinBlockTiles, s->n_tiles - b), kLtpItem));
                 xcd_order(wl, make_uint4(kEmpty, 0u, 0u, 0u));
                 s->dense_dirty = true;
                 A.n_bblk = j;

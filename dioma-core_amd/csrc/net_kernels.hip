@@ -6,7 +6,7 @@
 //        (LIF/Izh), advance the neuron, update STDP traces, wave-ballot the
 //        spike mask and write the tile's spiking ids in ascending order; for
 //        networks without plasticity, scatter the tile's spikes right away.
-//   2. ltp_kernel / ltd_kernel     (only with pair-based STDP)
+//   2. stdp_kernel                 (only with pair-based STDP)
 //   3. insert_kernel<SCHED>        scatter of plastic networks' spikes and of
 //        the Poisson sources that fire this step
 //   4. fixup_kernel                re-sums, in ascending event id, every ring
@@ -556,12 +556,90 @@ __device__ __forceinline__ uint32_t post_bin_cell(const BinMap& M, uint32_t n_po
     return ((tile / kBinTiles) << 11) | ((tile % kBinTiles) * kTile + r % kTile);
 }
 
+// Whether neuron gi spiked this step (its population's spike word).
+__device__ __forceinline__ bool spiked_now(const NetArgs& a, uint32_t gi) {
+    uint32_t p = 0;
+    if (a.n_pops > 1) {  // last population starting at or before gi
+        uint32_t lo = 0, hi = a.n_pops;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (a.pops[mid].off <= gi) lo = mid; else hi = mid;
+        }
+        p = lo;
+    }
+    const PopDev& P = a.pops[p];
+    const uint32_t li = gi - P.off;
+    return (a.bits[P.word0 + li / 64u] >> (li % 64u)) & 1ull;
+}
+
+// One STDP update of synapse e: FP64 master weight +- dw, clipped to
+// [w_min, w_max], FP32 mirror for the scatter; returns the mirror.
+__device__ __forceinline__ float stdp_apply(const NetArgs& a, uint32_t e, float dw, bool add) {
+    double nw = add ? a.w64[e] + static_cast<double>(dw) : a.w64[e] - static_cast<double>(dw);
+    nw = fmin(fmax(nw, static_cast<double>(a.w_min)), static_cast<double>(a.w_max));
+    a.w64[e] = nw;
+    const float f = static_cast<float>(nw);
+    const_cast<float*>(a.w)[e] = f;
+    return f;
+}
+
+// The STDP of out-edge e (post `post`) of a pre that spiked this step, then
+// its weight: LTP first when the post spiked too, then LTD (the order of the
+// reference's two passes, P/src/api/learn/brain.rs:204-286).
+__device__ __forceinline__ float stdp_out_edge(const NetArgs& a, uint32_t e, uint32_t post) {
+    if (a.plastic && !a.plastic[e]) return a.w[e];
+    if (spiked_now(a, post)) stdp_apply(a, e, a.a_plus * a.tr_pre[a.edge_pre[e]], true);
+    return stdp_apply(a, e, a.a_minus * a.tr_post[post], false);
+}
+
+// LTP on the in-edges of the spiking neurons of neuron tiles [t0, t1) whose
+// pre did not spike this step (a spiking pre's out-edge walk takes the
+// others).  Every thread of the block calls it; s_rs/s_pref hold kTile.
+__device__ void stdp_in_edges(const NetArgs& a, uint32_t t0, uint32_t t1, uint32_t* s_rs, uint32_t* s_pref,
+                              uint32_t* s_wave, uint32_t* s_cnt) {
+    const uint32_t t = threadIdx.x;
+    const float ap = a.a_plus;
+    __syncthreads();  // the caller's use of s_cnt is over
+    for (uint32_t b = t0 + t; b < t1; b += blockDim.x) s_cnt[b - t0] = a.tile_count[b];  // all tiles at once
+    __syncthreads();
+    for (uint32_t b = t0; b < t1; ++b) {
+        const uint32_t cnt = s_cnt[b - t0];
+        if (cnt == 0) continue;
+        __syncthreads();  // the previous tile's rows are read
+        uint32_t rs = 0, dg = 0;
+        if (t < cnt) {
+            const uint32_t id = a.active[static_cast<size_t>(b) * kTile + t];
+            rs = a.in_ptr[id];
+            dg = a.in_ptr[id + 1] - rs;
+        }
+        uint32_t excl;
+        const uint32_t total = block_exclusive_scan(dg, s_wave, excl);
+        if (t < cnt) {
+            s_rs[t] = rs;
+            s_pref[t] = excl;
+        }
+        __syncthreads();
+        for (uint32_t f = t; f < total; f += blockDim.x) {
+            uint32_t lo = 0, hi = cnt;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_pref[mid] <= f) lo = mid; else hi = mid;
+            }
+            const uint32_t e = a.in_edge[s_rs[lo] + (f - s_pref[lo])];
+            if (a.plastic && !a.plastic[e]) continue;
+            const uint32_t pre = a.edge_pre[e];
+            if (spiked_now(a, pre)) continue;
+            stdp_apply(a, e, ap * a.tr_pre[pre], true);
+        }
+    }
+}
+
 // Count (kWrite false) or write out (kWrite true) the events of `cnt` staged
 // rows (cnt <= 2048): row k starts at edge s_rs[k] and holds concatenation
 // positions [s_pref[k], s_pref[k+1]).  Counting adds 1 to s_bin[bin]; writing
 // puts the event at record s_bin[bin] of `ev` and advances it.  Four events
 // per thread are loaded before any is used.
-template <bool kWrite, uint32_t kThreads, bool kRec = false>
+template <bool kWrite, uint32_t kThreads, bool kRec = false, bool kStdp = false>
 __device__ __forceinline__ void bin_events(const NetArgs& a, const BinMap& M, bool is_src, uint32_t cnt,
                                            uint32_t total, const uint32_t* s_rs, const uint32_t* s_pref,
                                            uint32_t* s_bin, uint4* ev, uint64_t step) {
@@ -585,7 +663,7 @@ __device__ __forceinline__ void bin_events(const NetArgs& a, const BinMap& M, bo
         for (uint32_t u = 0; u < kU; ++u) {
             if (f0 + u * kThreads < total) {
                 post[u] = col[e[u]];
-                if (kWrite) wt[u] = w[e[u]];
+                if (kWrite) wt[u] = kStdp && !is_src ? stdp_out_edge(a, e[u], post[u]) : w[e[u]];
             }
         }
 #pragma unroll
@@ -613,7 +691,7 @@ struct EvCache {
     float w[kEvCache];
 };
 
-template <uint32_t kThreads>
+template <uint32_t kThreads, bool kStdp = false>
 __device__ __forceinline__ void ev_cache_load(const NetArgs& a, const BinMap& M, bool is_src, uint32_t cnt,
                                               uint32_t total, const uint32_t* s_rs, const uint32_t* s_pref,
                                               EvCache& c) {
@@ -633,7 +711,7 @@ __device__ __forceinline__ void ev_cache_load(const NetArgs& a, const BinMap& M,
     for (uint32_t u = 0; u < kEvCache; ++u) {
         if (threadIdx.x + u * kThreads < total) {
             post[u] = col[c.e[u]];
-            c.w[u] = w[c.e[u]];
+            c.w[u] = kStdp && !is_src ? stdp_out_edge(a, c.e[u], post[u]) : w[c.e[u]];
         }
     }
 #pragma unroll
@@ -1028,6 +1106,10 @@ __global__ __launch_bounds__(kTile) void bin_kernel(const NetArgs* __restrict__ 
     const uint64_t step = *a.step;
     if (step >= a.step_limit) return;
     const uint32_t n_nt = b0 < a.n_tiles ? min(b1, a.n_tiles) - b0 : 0u;  // neuron tiles (<= kTile)
+    if (j == kLtpItem) {  // plastic networks: the in-edge LTP of tiles [b0, b1), beside the writers
+        stdp_in_edges(a, b0, b0 + n_nt, s_rs, s_pref, s_wave, s_tp);
+        return;
+    }
     const uint32_t base = a.tile_ebase[b0];
     // megastep networks: this step's buffer of records and headers
     const uint32_t par = a.mega ? static_cast<uint32_t>(step & 1u) : 0u;
@@ -1077,7 +1159,10 @@ __global__ __launch_bounds__(kTile) void bin_kernel(const NetArgs* __restrict__ 
                 events += total;
                 cached = one && total <= kEvCache * kTile;
                 if (cached) {
-                    ev_cache_load<kTile>(a, M, false, nr, total, s_rs, s_pref, ec);
+                    // plastic networks: each spiking row's STDP happens as its
+                    // events are loaded, exactly once
+                    if (a.stdp) ev_cache_load<kTile, true>(a, M, false, nr, total, s_rs, s_pref, ec);
+                    else ev_cache_load<kTile>(a, M, false, nr, total, s_rs, s_pref, ec);
                     ev_cache_count<kTile>(ec, total, s_bin);
                 } else {
                     bin_events<false, kTile>(a, M, false, nr, total, s_rs, s_pref, s_bin, ev, step);
@@ -1086,6 +1171,9 @@ __global__ __launch_bounds__(kTile) void bin_kernel(const NetArgs* __restrict__ 
                 // receptor networks tag each synapse event with its receptor
                 if (a.rec_code) ev_cache_write<kTile, true>(a, false, ec, total, s_bin, ev, step);
                 else ev_cache_write<kTile>(a, false, ec, total, s_bin, ev, step);
+            } else if (a.stdp) {
+                if (a.rec_code) bin_events<true, kTile, true, true>(a, M, false, nr, total, s_rs, s_pref, s_bin, ev, step);
+                else bin_events<true, kTile, false, true>(a, M, false, nr, total, s_rs, s_pref, s_bin, ev, step);
             } else if (a.rec_code) {
                 bin_events<true, kTile, true>(a, M, false, nr, total, s_rs, s_pref, s_bin, ev, step);
             } else {
@@ -1813,79 +1901,65 @@ __global__ void head_clear_kernel(unsigned long long* head, const uint64_t* step
         head[i] = v;
 }
 
-// Pair-based STDP, LTP pass: in-edges of every spiking post gain
-// A+ * pre-trace.  Each edge has one post, so each edge is written by at most
-// one thread.  FP64 master weight, FP32 mirror for the scatter.
-__global__ __launch_bounds__(kTile) void ltp_kernel(const NetArgs* __restrict__ nets,
-                                                  const uint2* __restrict__ tiles) {
-    __shared__ uint32_t s_rs[kTile], s_pref[kTile], s_wave[kWavesPerTile];
+// Pair-based STDP of one step for the spiking neurons of tile b, in one pass
+// that equals the reference's LTP pass followed by its LTD pass
+// (P/src/api/learn/brain.rs:204-286): in-edges of every spiking post gain
+// A+ * pre-trace, out-edges of every spiking pre lose A- * post-trace, each
+// update clipped to [w_min, w_max] with the FP64 master weight.  A synapse
+// whose pre and post both spiked takes both, LTP first; it is handled only
+// by its pre's out-edge walk (the in-edge walk skips it), so no synapse is
+// touched by two threads and the two walks need no ordering.
+__global__ __launch_bounds__(kTile) void stdp_kernel(const NetArgs* __restrict__ nets,
+                                                   const uint2* __restrict__ tiles) {
+    __shared__ uint32_t s_rs[2][kTile], s_pref[2][kTile], s_id[kTile], s_wave[kWavesPerTile];
     const uint2 ref = tiles[blockIdx.x];
     const NetArgs& a = nets[ref.x];
     if (*a.step >= a.step_limit) return;
     const uint32_t b = ref.y, t = threadIdx.x;
     const uint32_t cnt = a.tile_count[b];
     if (cnt == 0) return;
-    uint32_t rs = 0, dg = 0;
+    uint32_t id = 0, ri = 0, di = 0, ro = 0, dout = 0;
     if (t < cnt) {
-        const uint32_t id = a.active[static_cast<size_t>(b) * kTile + t];
-        rs = a.in_ptr[id];
-        dg = a.in_ptr[id + 1] - rs;
+        id = a.active[static_cast<size_t>(b) * kTile + t];
+        ri = a.in_ptr[id];
+        di = a.in_ptr[id + 1] - ri;
+        ro = a.row_ptr[id];
+        dout = a.row_ptr[id + 1] - ro;
     }
-    uint32_t excl;
-    const uint32_t total = block_exclusive_scan(dg, s_wave, excl);
-    if (t < cnt) { s_rs[t] = rs; s_pref[t] = excl; }
+    uint32_t ei, eo;
+    const uint32_t tin = block_exclusive_scan(di, s_wave, ei);
     __syncthreads();
-    const float ap = a.a_plus;
-    for (uint32_t f = t; f < total; f += kTile) {
+    const uint32_t tout = block_exclusive_scan(dout, s_wave, eo);
+    if (t < cnt) {
+        s_rs[0][t] = ri;
+        s_pref[0][t] = ei;
+        s_rs[1][t] = ro;
+        s_pref[1][t] = eo;
+        s_id[t] = id;
+    }
+    __syncthreads();
+    const float ap = a.a_plus, am = a.a_minus;
+    for (uint32_t f = t; f < tin + tout; f += kTile) {
+        const bool out = f >= tin;
+        const uint32_t g = out ? f - tin : f, side = out ? 1u : 0u;
         uint32_t lo = 0, hi = cnt;
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (s_pref[mid] <= f) lo = mid; else hi = mid;
+            if (s_pref[side][mid] <= g) lo = mid; else hi = mid;
         }
-        const uint32_t e = a.in_edge[s_rs[lo] + (f - s_pref[lo])];
-        if (a.plastic && !a.plastic[e]) continue;
-        const float dw = ap * a.tr_pre[a.edge_pre[e]];
-        double nw = a.w64[e] + static_cast<double>(dw);
-        nw = fmin(fmax(nw, static_cast<double>(a.w_min)), static_cast<double>(a.w_max));
-        a.w64[e] = nw;
-        const_cast<float*>(a.w)[e] = static_cast<float>(nw);
-    }
-}
-
-// LTD pass: out-edges of every spiking pre lose A- * post-trace.
-__global__ __launch_bounds__(kTile) void ltd_kernel(const NetArgs* __restrict__ nets,
-                                                  const uint2* __restrict__ tiles) {
-    __shared__ uint32_t s_rs[kTile], s_pref[kTile], s_wave[kWavesPerTile];
-    const uint2 ref = tiles[blockIdx.x];
-    const NetArgs& a = nets[ref.x];
-    if (*a.step >= a.step_limit) return;
-    const uint32_t b = ref.y, t = threadIdx.x;
-    const uint32_t cnt = a.tile_count[b];
-    if (cnt == 0) return;
-    uint32_t rs = 0, dg = 0;
-    if (t < cnt) {
-        const uint32_t id = a.active[static_cast<size_t>(b) * kTile + t];
-        rs = a.row_ptr[id];
-        dg = a.row_ptr[id + 1] - rs;
-    }
-    uint32_t excl;
-    const uint32_t total = block_exclusive_scan(dg, s_wave, excl);
-    if (t < cnt) { s_rs[t] = rs; s_pref[t] = excl; }
-    __syncthreads();
-    const float am = a.a_minus;
-    for (uint32_t f = t; f < total; f += kTile) {
-        uint32_t lo = 0, hi = cnt;
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (s_pref[mid] <= f) lo = mid; else hi = mid;
+        if (!out) {  // LTP on an in-edge of spiking post s_id[lo]
+            const uint32_t e = a.in_edge[s_rs[0][lo] + (g - s_pref[0][lo])];
+            if (a.plastic && !a.plastic[e]) continue;
+            const uint32_t pre = a.edge_pre[e];
+            if (spiked_now(a, pre)) continue;  // its pre's out-edge walk takes it
+            stdp_apply(a, e, ap * a.tr_pre[pre], true);
+        } else {  // LTD (and LTP first if the post spiked too) on an out-edge of spiking pre s_id[lo]
+            const uint32_t e = s_rs[1][lo] + (g - s_pref[1][lo]);
+            if (a.plastic && !a.plastic[e]) continue;
+            const uint32_t post = a.col[e];
+            if (spiked_now(a, post)) stdp_apply(a, e, ap * a.tr_pre[s_id[lo]], true);
+            stdp_apply(a, e, am * a.tr_post[post], false);
         }
-        const uint32_t e = s_rs[lo] + (f - s_pref[lo]);
-        if (a.plastic && !a.plastic[e]) continue;
-        const float dw = am * a.tr_post[a.col[e]];
-        double nw = a.w64[e] - static_cast<double>(dw);
-        nw = fmin(fmax(nw, static_cast<double>(a.w_min)), static_cast<double>(a.w_max));
-        a.w64[e] = nw;
-        const_cast<float*>(a.w)[e] = static_cast<float>(nw);
     }
 }
 
@@ -2056,8 +2130,7 @@ void launch_dense_weights(const NetArgs* nets, uint32_t N, const float* trace, f
 }
 void launch_stdp(const NetArgs* nets, const uint2* tiles, uint32_t n, hipStream_t s) {
     if (n == 0) return;
-    ltp_kernel<<<n, kTile, 0, s>>>(nets, tiles);
-    ltd_kernel<<<n, kTile, 0, s>>>(nets, tiles);
+    stdp_kernel<<<n, kTile, 0, s>>>(nets, tiles);
 }
 void launch_insert(int sched, const NetArgs* nets, const uint2* blocks, uint32_t n, hipStream_t s) {
     if (n == 0) return;

@@ -33,6 +33,7 @@ constexpr uint32_t kMaxBinPops = 64;    // populations of a binned network
 constexpr uint32_t kBinCap = 2048;      // records bin_apply orders in one LDS pass
 constexpr uint32_t kMaxWriters = 2048;  // binned writers per network
 constexpr uint32_t kBinBlockTiles = 4;  // source tiles per bin_kernel writer
+constexpr uint32_t kLtpItem = 0xFFFFFFFEu;  // bin_kernel item that runs in-edge LTP instead of writing
 
 enum : uint32_t {
     kPopRefrElided = 1u,  // refractory timer provably stays 0: not loaded/stored

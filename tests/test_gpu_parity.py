@@ -320,6 +320,24 @@ def test_stdp_weights_bitexact():
     assert_same_state(dev, ora, spec)
 
 
+def test_stdp_coincident_pre_post_bitexact():
+    # groups of neurons with identical state fire in the same steps, so many
+    # synapses see their pre and post spike together (LTP then LTD on one
+    # synapse, P/src/api/learn/brain.rs:204-286)
+    say("stdp coincident")
+    spec = random_net(1200, 40, seed=17, model="lif", max_delay=2, noise=0.0)
+    v0 = np.repeat(np.linspace(-65.0, -52.0, 6), 200).astype(np.float32)
+    spec.pops[0].states[0, :1200] = v0
+    spec.w = (np.abs(spec.w) / 200.0).astype(np.float32)
+    spec.stdp = dict(a_plus=0.01, a_minus=0.012, tau_plus=20.0, tau_minus=20.0, w_min=0.0, w_max=1.0)
+    dev, ora = run_pair(spec, 300)
+    assert ora.counters()[0] > 300  # six synchronous groups: ~2300 coincident synapse events
+    wd, wo = dev.download_weights(), ora.weights().astype(np.float32)
+    assert np.array_equal(wd.view(np.uint32), wo.view(np.uint32))
+    assert not np.array_equal(wd, spec.w)
+    assert_same_state(dev, ora, spec)
+
+
 @pytest.mark.parametrize("sched", [SCATTER_TILE, SCATTER_WAVE])
 def test_dense_stdp_step_bitexact(sched):
     # CompiledNetwork::stdp_step (pillars-cuda/src/network.rs:495-718): dense rule on every
