@@ -26,6 +26,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 REPO = os.path.dirname(os.path.abspath(__file__))
@@ -181,10 +182,11 @@ def bench_precision(args) -> dict:
     """Precision configuration: the Hodgkin-Huxley network with heavy-tailed
     out-degree, lognormal conductance synapses (AMPA from excitatory, GABA_A
     from inhibitory neurons) and pair STDP on its excitatory synapses
-    (dt = 0.01 ms).  The device state (FP32) is compared with the oracle's
-    FP64 twin after every window of 1000 steps on a reduced network (the
-    oracle runs on the host cores), and the device's step time is taken at
-    the full 200k-neuron size."""
+    (dt = 0.01 ms).  Every window of 1000 steps starts the oracle's FP64
+    twin from the device's own state (its `.taas` snapshot: every FP32 value
+    is exact in FP64), so each window's error is that of 1000 FP32 steps
+    from a common start, on a reduced network (the oracle runs on the host
+    cores); the device's step time is taken at the full 200k-neuron size."""
     import pyoracle as po
     from simuse.workloads import build_device, hh_network
     n, windows = args.prec_neurons, args.prec_windows
@@ -195,10 +197,17 @@ def bench_precision(args) -> dict:
     o64 = po.OracleNetwork(spec, 64, threads=min(16, os.cpu_count() or 1))
     plastic = spec.stdp["plastic"].astype(bool)
     rows = []
-    s_dev0, s_ora0 = 0, 0
+    snap = os.path.join(tempfile.mkdtemp(), "window.taas")
     for k in range(windows):
-        dev.run(1000)
-        o64.step(1000)
+        if k:  # the FP64 twin restarts from the device's state
+            dev.save_state(snap)
+            o64.set_state_from_snapshot(snap)
+        s_dev0, s_ora0 = dev.counters()[0], o64.counters()[0]
+        dev.run(100)  # 1 ms in: the FP32 rounding error before spike-time flips spread
+        o64.step(100)
+        dv100 = np.abs(dev.download_states(0)[0, :n].astype(np.float64) - o64.state(0, 0))
+        dev.run(900)
+        o64.step(900)
         v_dev = dev.download_states(0)[0, :n].astype(np.float64)
         v64 = o64.state(0, 0)
         dv = np.abs(v_dev - v64)
@@ -211,14 +220,17 @@ def bench_precision(args) -> dict:
         s_dev, s_ora = dev.counters()[0], o64.counters()[0]
         rate_dev = (s_dev - s_dev0) / n / (1000 * spec.dt * 1e-3)
         rate_ora = (s_ora - s_ora0) / n / (1000 * spec.dt * 1e-3)
-        s_dev0, s_ora0 = s_dev, s_ora
-        rows.append({"window": k, "v_mean_abs_err_mV": float(dv.mean()), "v_max_abs_err_mV": float(dv.max()),
+        rows.append({"window": k, "v_mean_abs_err_mV_100": float(dv100.mean()),
+                     "v_max_abs_err_mV_100": float(dv100.max()),
+                     "v_mean_abs_err_mV": float(dv.mean()), "v_max_abs_err_mV": float(dv.max()),
+                     "frac_over_1mV": float(np.mean(dv > 1.0)),
                      "g_mean_abs_err": float(dg.mean()), "g_max_abs_err": float(dg.max()),
                      "w_rms_err": float(np.sqrt(np.mean(dw ** 2))), "w_max_abs_err": float(dw.max()),
                      "rate_hz_fp32": float(rate_dev), "rate_hz_fp64": float(rate_ora),
                      "rate_rel_err": float(abs(rate_dev - rate_ora) / max(rate_ora, 1e-12))})
-        log(f"  window {k}: |dv| mean {dv.mean():.3e} max {dv.max():.3e} mV, |dg| max {dg.max():.3e}, "
-            f"w rms {rows[-1]['w_rms_err']:.3e}, rate {rate_dev:.2f} vs {rate_ora:.2f} Hz")
+        log(f"  window {k}: 100 steps |dv| mean {dv100.mean():.2e} max {dv100.max():.2e} mV; 1000 steps |dv| mean "
+            f"{dv.mean():.2e} max {dv.max():.2e} mV, >1 mV {np.mean(dv > 1.0):.2%}, |dg| max {dg.max():.2e}, "
+            f"w rms {rows[-1]['w_rms_err']:.2e}, rate {rate_dev:.2f} vs {rate_ora:.2f} Hz")
     dev.close()
     big = hh_network(n=args.prec_full_neurons, seed=3, steps=args.prec_full_steps + 200, noise_sd=30.0,
                      receptors=True)
@@ -235,7 +247,8 @@ def bench_precision(args) -> dict:
     return {"workload": f"HH {n} neurons ({spec.n_syn} synapses), Pareto out-degree, lognormal conductances "
                         "(AMPA excitatory, GABA_A inhibitory, 0.003 mS/cm2 units), pair STDP (A+ 0.01, A- 0.012, "
                         "tau 20 ms, [0,1]) on excitatory synapses, Gaussian input sd 30, dt 0.01 ms",
-            "reference": "oracle FP64 twin", "windows": rows,
+            "reference": "oracle FP64 twin, restarted from the device state at every window",
+            "windows": rows,
             "full_size": {"neurons": args.prec_full_neurons, "synapses": big.n_syn,
                           "steps": args.prec_full_steps, "us_per_step": us}}
 
@@ -284,7 +297,7 @@ def main() -> int:
     ap.add_argument("--serve-timeout", type=float, default=300.0)
     ap.add_argument("--no-precision", action="store_true", help="skip the HH precision leg")
     ap.add_argument("--prec-neurons", type=int, default=10_000)
-    ap.add_argument("--prec-windows", type=int, default=3)
+    ap.add_argument("--prec-windows", type=int, default=5)
     ap.add_argument("--prec-full-neurons", type=int, default=200_000)
     ap.add_argument("--prec-full-steps", type=int, default=2000)
     args = ap.parse_args()

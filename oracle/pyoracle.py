@@ -67,6 +67,8 @@ _SIGS = {
     "orc_compact": ([i32p, C.c_uint32, u32p], C.c_uint32),
     "orc_net_set_receptors": ([P, C.c_void_p, C.c_float, C.c_void_p], C.c_int),
     "orc_net_conductances": ([P, f64p], None),
+    "orc_net_set_state": ([P, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                           C.c_void_p, C.c_void_p], C.c_int),
     "orc_rec_decay_factors": ([C.c_float, f32p], None),
     "orc_nmda_block": ([C.c_float], C.c_float),
     "orc_rec_currents": ([f32p, f32p, f32p, C.c_uint32], None),
@@ -267,6 +269,30 @@ class OracleNetwork:
         out = np.zeros(self.spec.n_syn, np.float64)
         lib().orc_net_weights(self.h, _p(out, C.c_double))
         return out
+
+    def set_state_from_snapshot(self, path):
+        """Load a device state written by CompiledNetwork.save_state (`.taas`
+        v2, every FP32 value exact in either twin)."""
+        from simuse.formats import TaasReader
+        r = TaasReader.from_file(path)
+        keep = []
+
+        def ptr(a):
+            if a is None:
+                return None
+            keep.append(a)
+            return a.ctypes.data
+
+        pops = [np.ascontiguousarray(r.get_f32(f"pop{p}.states")) for p in range(len(self.spec.pops))]
+        arr = (C.c_void_p * len(pops))(*[ptr(x) for x in pops])
+        w64 = r.get_u64("stdp.w64").view(np.float64) if r.has_section("stdp.w64") else None
+        pre = r.get_f32("stdp.pre") if r.has_section("stdp.pre") else None
+        post = r.get_f32("stdp.post") if r.has_section("stdp.post") else None
+        g = r.get_f32("receptors.g") if r.has_section("receptors.g") else None
+        rc = lib().orc_net_set_state(self.h, int(r.get_u64("step")[0]), C.cast(arr, C.c_void_p),
+                                     ptr(r.get_f32("ring")), ptr(r.get_f32("weights")), ptr(w64), ptr(pre),
+                                     ptr(post), ptr(g))
+        assert rc == 0
 
     def conductances(self):
         out = np.zeros((7, self.N), np.float64)

@@ -521,6 +521,10 @@ struct NetBase {
     virtual int inject(int pop, const uint32_t* idx, const float* val, uint32_t count) = 0;
     virtual int set_receptors(const uint8_t* types, float scale, const float* ext) = 0;
     virtual void conductances(double* out) const = 0;
+    // Overwrite the dynamic state with a (device) snapshot: FP32 values are
+    // exact in either twin.  Null pointers leave that part as it is.
+    virtual int set_state(uint64_t step, const float* const* pop_states, const float* ring, const float* w,
+                          const double* w64, const float* tr_pre, const float* tr_post, const float* g) = 0;
     virtual void step(uint32_t n) = 0;
     virtual void stdp_step(float decay, float lr_ltp, float lr_ltd, float w_min, float w_max) = 0;
     virtual uint64_t step_counter() const = 0;
@@ -721,6 +725,33 @@ struct Net final : NetBase {
     }
     void conductances(double* out) const override {
         for (size_t k = 0; k < gcond.size(); ++k) out[k] = static_cast<double>(gcond[k]);
+    }
+    int set_state(uint64_t step, const float* const* pop_states, const float* ring, const float* wf,
+                  const double* w64v, const float* pre, const float* post, const float* g) override {
+        stepc = step;
+        if (pop_states)
+            for (size_t p = 0; p < pops.size(); ++p) {
+                Pop& P = pops[p];
+                const int ns = n_state(P.model);
+                for (int c = 0; c < ns; ++c)
+                    for (uint32_t i = 0; i < P.n; ++i)
+                        P.st[static_cast<size_t>(c) * P.cap + i] = R(pop_states[p][static_cast<size_t>(c) * P.n + i]);
+            }
+        if (ring)
+            for (size_t k = 0; k < ringv.size(); ++k) ringv[k] = R(ring[k]);
+        if (wf)
+            for (size_t e = 0; e < w.size(); ++e) w[e] = R(wf[e]);
+        if (w64v && stdp) {
+            for (size_t e = 0; e < w64.size(); ++e) w64[e] = w64v[e];
+            for (size_t e = 0; e < w.size(); ++e) w[e] = R(w64v[e]);
+        }
+        if (pre && stdp)
+            for (uint32_t i = 0; i < N; ++i) tr_pre[i] = R(pre[i]);
+        if (post && stdp)
+            for (uint32_t i = 0; i < N; ++i) tr_post[i] = R(post[i]);
+        if (g && rec_on)
+            for (size_t k = 0; k < gcond.size(); ++k) gcond[k] = R(g[k]);
+        return 0;
     }
     // Before the update (CN/src/sim/hh.rs:115-166): decay every conductance,
     // then the input becomes (input + external) plus the receptor currents
@@ -1049,6 +1080,10 @@ int orc_net_set_receptors(void* net, const uint8_t* types, float scale, const fl
     return static_cast<NetBase*>(net)->set_receptors(types, scale, ext);
 }
 void orc_net_conductances(void* net, double* out) { static_cast<NetBase*>(net)->conductances(out); }
+int orc_net_set_state(void* net, uint64_t step, const float* const* pop_states, const float* ring, const float* w,
+                      const double* w64, const float* tr_pre, const float* tr_post, const float* g) {
+    return static_cast<NetBase*>(net)->set_state(step, pop_states, ring, w, w64, tr_pre, tr_post, g);
+}
 void orc_rec_decay_factors(float dt, float* out) {
     for (int r = 0; r < kRec; ++r) out[r] = rec_decay<float>(dt, r);
 }

@@ -76,6 +76,12 @@ int orc_net_set_stp(void* net, const orc_stp_profile* profiles, int32_t n_profil
  * receptor.rs; step order of CN/src/sim/hh.rs:115-166). */
 int orc_net_set_receptors(void* net, const uint8_t* types, float scale, const float* ext);
 void orc_net_conductances(void* net, double* out); /* [7][n_total] */
+/* Overwrite the dynamic state (step counter, population states [ns][n] per
+ * population, ring [32][N], weights, FP64 master weights, STDP traces,
+ * conductances); NULL leaves a part unchanged. */
+int orc_net_set_state(void* net, uint64_t step, const float* const* pop_states, const float* ring,
+                      const float* w, const double* w64, const float* tr_pre, const float* tr_post,
+                      const float* g);
 void orc_rec_decay_factors(float dt, float* out);  /* 7 values */
 float orc_nmda_block(float v);
 /* compute_currents: i_out[i] += receptor currents of g[7][n] at v[i] */

@@ -263,3 +263,29 @@ def test_connectome_baked_runs_like_oracle(tmp_path):
         assert np.array_equal(ids, ora.active()), t
         total += ids.size
     assert total > 200
+
+
+@pytest.mark.parametrize("kind", ["stdp", "receptors", "binned_izh"])
+def test_device_snapshot_resumes_in_oracle(kind, tmp_path):
+    # the snapshot holds everything the step depends on: the FP32 oracle twin
+    # loaded from the device's file continues exactly as the device does
+    say("snapshot into oracle", kind)
+    spec = _spec(kind)
+    dev = build_device(spec)
+    dev.run(90)
+    path = tmp_path / "d.taas"
+    dev.save_state(path)
+    ora = po.OracleNetwork(spec, 32)
+    ora.set_state_from_snapshot(path)
+    for t in range(100):
+        dev.step()
+        ora.step()
+        assert np.array_equal(dev.download_spike_ids(), ora.active()), t
+    for p, pop in enumerate(spec.pops):
+        st = dev.download_states(p)
+        for c in range(pop.ns):
+            assert np.array_equal(st[c, : pop.n].view(np.uint32), ora.state(p, c).astype(np.float32).view(np.uint32))
+    assert np.array_equal(dev.download_weights().view(np.uint32), ora.weights().astype(np.float32).view(np.uint32))
+    if spec.receptors:
+        assert np.array_equal(dev.download_conductances().view(np.uint32),
+                              ora.conductances().astype(np.float32).view(np.uint32))

@@ -180,3 +180,36 @@ def test_receptor_current_enters_next_update(w):
     v_exp = v1 + ((v_rest - v1) + i_syn) * inv_tau * dt
     assert abs(ora.state(0, 0)[1] - v_exp) < 1e-9
     assert abs(ora.conductances()[0, 1] - g) < 1e-15
+
+
+def test_oracle_set_state_resumes_exactly(tmp_path):
+    # a state written as a `.taas` snapshot (the layout the device's
+    # save_state writes) and loaded into a fresh FP32 twin continues exactly
+    # as the twin it came from
+    from nets import random_net
+    from simuse.formats import TaasWriter
+    spec = random_net(600, 20, seed=4, model="lif", max_delay=1, sources=2)
+    spec.receptors = dict(types=np.full(spec.n_syn, 0, np.uint8), scale=1.0)
+    spec.w = np.abs(spec.w) * np.float32(0.02)
+    a = po.OracleNetwork(spec, 32)
+    a.step(60)
+    w = TaasWriter()
+    w.u64("step", [a.step_counter()])
+    pop = spec.pops[0]
+    w.f32("pop0.states", np.concatenate([a.state(0, c).astype(np.float32) for c in range(pop.ns)]))
+    w.f32("ring", a.ring().astype(np.float32).ravel())
+    w.f32("weights", a.weights().astype(np.float32))
+    w.f32("receptors.g", a.conductances().astype(np.float32).ravel())
+    p = tmp_path / "s.taas"
+    w.to_file(p)
+    b = po.OracleNetwork(spec, 32)
+    b.step(7)  # some other state first
+    b.set_state_from_snapshot(p)
+    assert b.step_counter() == 60
+    for _ in range(80):
+        a.step()
+        b.step()
+        assert np.array_equal(a.active(), b.active())
+    for c in range(pop.ns):
+        assert np.array_equal(a.state(0, c), b.state(0, c))
+    assert np.array_equal(a.conductances(), b.conductances())
