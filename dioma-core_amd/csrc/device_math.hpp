@@ -99,10 +99,11 @@ __host__ __device__ __forceinline__ float fast_expf(float x) {
     return scale * poly;
 }
 
-// cos(2*pi*u) for u in [0, 1) on a 2^-24 grid: quadrant from 4u (exact),
-// then sin or cos of (pi/2)*g, g in [0, 1/2], by their Taylor series through
-// x^9 / x^10 (truncation < 2e-9 on [0, pi/4]), Horner in FP32.
-__device__ __forceinline__ float det_cos2pi(float u) {
+// cos and sin of 2 pi u for u in [0, 1) on a 2^-24 grid: quadrant from 4u
+// (exact), then sin and cos of (pi/2) g, g in [0, 1/2], by their Taylor
+// series through x^9 / x^10 (truncation < 2e-9 on [0, pi/4]), Horner in FP32;
+// sin(2 pi u) = cos(2 pi (u - 1/4)) is the quadrant before.
+__device__ __forceinline__ void det_sincos2pi(float u, float& cosv, float& sinv) {
     const float t = u * 4.0f;
     const float q = floorf(t);
     const float f = t - q;
@@ -117,19 +118,33 @@ __device__ __forceinline__ float det_cos2pi(float u) {
     const float c = hi ? sn : cs;  // cos((pi/2) f)
     const float s = hi ? cs : sn;  // sin((pi/2) f)
     const int qi = static_cast<int>(q);
-    return qi == 0 ? c : qi == 1 ? -s : qi == 2 ? -c : s;
+    cosv = qi == 0 ? c : qi == 1 ? -s : qi == 2 ? -c : s;
+    sinv = qi == 0 ? s : qi == 1 ? c : qi == 2 ? -s : -c;
 }
 
-// Standard normal draw keyed by (seed, neuron, step): Box & Muller (1958),
-// z = sqrt(-2 ln u1) cos(2 pi u2), on two 24-bit uniforms of one SplitMix64
-// hash of the counter (u1 in (0, 1], u2 in [0, 1)).  Counter-based and
-// branch-free, so every lane of a wavefront draws in lockstep.
-__device__ __forceinline__ float normal_draw(uint64_t seed, uint32_t i, uint64_t step) {
-    const uint64_t base = splitmix(seed ^ (static_cast<uint64_t>(i) * 0xD1B54A32D192ED03ull));
+// Standard normal draws keyed by (seed, neuron, step): Box & Muller (1958)
+// on two 24-bit uniforms of one SplitMix64 hash of (seed, pair, step), u1 in
+// (0, 1], u2 in [0, 1).  Neurons i and i ^ 1024 form a pair (key i & ~1024):
+// the one with bit 10 clear gets sqrt(-2 ln u1) cos(2 pi u2), its partner
+// the sine, two independent normals from one hash, one log and one sqrt.
+// Counter-based and branch-free, so every lane of a wavefront draws in
+// lockstep.
+__device__ __forceinline__ void normal_pair(uint64_t seed, uint32_t pair, uint64_t step, float& zc, float& zs) {
+    const uint64_t base = splitmix(seed ^ (static_cast<uint64_t>(pair) * 0xD1B54A32D192ED03ull));
     const uint64_t h = splitmix(base ^ (step * 0xA24BAED4963EE407ull));
     const float u1 = static_cast<float>((h >> 40) + 1u) * 5.9604644775390625e-08f;
     const float u2 = static_cast<float>((h >> 16) & 0xFFFFFFu) * 5.9604644775390625e-08f;
-    return __builtin_sqrtf(-2.0f * det_logf(u1)) * det_cos2pi(u2);
+    const float r = __builtin_sqrtf(-2.0f * det_logf(u1));
+    float c, s;
+    det_sincos2pi(u2, c, s);
+    zc = r * c;
+    zs = r * s;
+}
+
+__device__ __forceinline__ float normal_draw(uint64_t seed, uint32_t i, uint64_t step) {
+    float zc, zs;
+    normal_pair(seed, i & ~1024u, step, zc, zs);
+    return (i & 1024u) ? zs : zc;
 }
 
 // x / (e^{x/y} - 1) (NEURON hh.mod vtrap, PC/kernels/hh.cu:28-34).  For

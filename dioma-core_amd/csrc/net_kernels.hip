@@ -1645,6 +1645,29 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
         if (on[q]) nrn[q].load(*PP[q], li[q]);
     }
 
+    // Gaussian input: the thread's two neurons are li and li + 1024 of one
+    // population whenever its bin lies inside that population, and then
+    // share one Box-Muller draw (the pair of normal_pair)
+    float nz[2] = {0.0f, 0.0f};
+    if (on[0] && on[1] && PP[0] == PP[1] && (PP[0]->flags & kPopNoise) && (li[0] & 1024u) == 0u &&
+        li[1] == li[0] + 1024u) {
+        const PopDev& P = *PP[0];
+        float zc, zs;
+        normal_pair(P.seed, li[0], step, zc, zs);
+        const bool uni = P.flags & kPopNoiseScalar;
+        nz[0] = (uni ? P.sigma_uni : P.sigma[li[0]]) * zc;
+        nz[1] = (uni ? P.sigma_uni : P.sigma[li[1]]) * zs;
+    } else {
+#pragma unroll
+        for (uint32_t q = 0; q < 2; ++q) {
+            const PopDev& P = *PP[q];
+            if (on[q] && (P.flags & kPopNoise)) {
+                const float sg = (P.flags & kPopNoiseScalar) ? P.sigma_uni : P.sigma[li[q]];
+                nz[q] = sg * normal_draw(P.seed, li[q], step);
+            }
+        }
+    }
+
     // advance every neuron; per tile ballot the spikes and scan the spiking
     // rows' lengths within each wave
     uint64_t m[2];
@@ -1665,10 +1688,7 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
                 if (step > 0 && poisson_roll(sd, 0u, step - 1, P.drive_p)) I = I + P.drive_w;
                 drive_now = poisson_roll(sd, 0u, step, P.drive_p) ? 1 : 0;
             }
-            if (P.flags & kPopNoise) {
-                const float sg = (P.flags & kPopNoiseScalar) ? P.sigma_uni : P.sigma[i];
-                I = I + sg * normal_draw(P.seed, i, step);
-            }
+            if (P.flags & kPopNoise) I = I + nz[q];
             spike = nrn[q].step(P, i, I, P.puni[0]);
             nrn[q].store(P, i);
             if (spike) {  // only spiking rows need their CSR extent

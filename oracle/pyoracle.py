@@ -60,6 +60,7 @@ _SIGS = {
     "orc_gpea_rates": ([C.c_float, C.c_float, f32p], None),
     "orc_net_set_stp": ([P, C.c_void_p, C.c_int32, C.c_float, C.c_void_p, C.c_void_p, C.c_uint64], C.c_int),
     "orc_cos2pi_f32": ([C.c_float], C.c_float),
+    "orc_sin2pi_f32": ([C.c_float], C.c_float),
     "orc_exp_f32": ([C.c_float], C.c_float),
     "orc_ln_f32": ([C.c_float], C.c_float),
     "orc_vtrap_f32": ([C.c_float, C.c_float], C.c_float),
@@ -134,6 +135,10 @@ def powi_f32(a, n):
 
 def fast_exp(x):
     return float(lib().orc_fast_exp(x))
+
+
+def sin2pi_f32(u):
+    return float(lib().orc_sin2pi_f32(u))
 
 
 def rec_decay_factors(dt):

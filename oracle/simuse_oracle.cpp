@@ -151,10 +151,11 @@ static inline R tm_spike_r(const TmProfile& p, R& r, R& u, uint32_t& last, R wei
     return weight * scale;
 }
 
-// cos(2*pi*u), u in [0,1) on a 2^-24 grid: exact quadrant split of 4u, then
-// the Taylor series of sin/cos((pi/2) g), g in [0, 1/2], through x^9 / x^10,
-// Horner in FP32 (same operation sequence as the device).
-static inline float cos2pi_f32(float u) {
+// cos(2*pi*u) and sin(2*pi*u), u in [0,1) on a 2^-24 grid: exact quadrant
+// split of 4u, then the Taylor series of sin/cos((pi/2) g), g in [0, 1/2],
+// through x^9 / x^10, Horner in FP32 (same operation sequence as the device);
+// the sine is the cosine of the quadrant before.
+static inline void sincos2pi_f32(float u, float& cosv, float& sinv) {
     const float t = u * 4.0f;
     const float q = std::floor(t);
     const float f = t - q;
@@ -169,18 +170,29 @@ static inline float cos2pi_f32(float u) {
     const float c = hi ? sn : cs;
     const float s = hi ? cs : sn;
     const int qi = static_cast<int>(q);
-    return qi == 0 ? c : qi == 1 ? -s : qi == 2 ? -c : s;
+    cosv = qi == 0 ? c : qi == 1 ? -s : qi == 2 ? -c : s;
+    sinv = qi == 0 ? s : qi == 1 ? c : qi == 2 ? -s : -c;
+}
+static inline float cos2pi_f32(float u) {
+    float c, s;
+    sincos2pi_f32(u, c, s);
+    return c;
 }
 
 // Thalamic Gaussian input sigma * N(0,1) per neuron and step: Box-Muller on
-// one SplitMix64 hash of (seed, neuron, step), z = sqrt(-2 ln u1) cos(2 pi u2)
-// with 24-bit u1 in (0,1], u2 in [0,1).
+// one SplitMix64 hash of (seed, pair, step) with 24-bit u1 in (0,1], u2 in
+// [0,1); neurons i and i ^ 1024 share the hash (key i & ~1024): bit 10 clear
+// takes sqrt(-2 ln u1) cos(2 pi u2), set takes the sine.
 static inline float gauss(uint64_t seed, uint32_t i, uint64_t step) {
-    const uint64_t base = mix64(seed ^ (static_cast<uint64_t>(i) * 0xD1B54A32D192ED03ull));
+    const uint32_t pair = i & ~1024u;
+    const uint64_t base = mix64(seed ^ (static_cast<uint64_t>(pair) * 0xD1B54A32D192ED03ull));
     const uint64_t h = mix64(base ^ (step * 0xA24BAED4963EE407ull));
     const float u1 = static_cast<float>((h >> 40) + 1u) * 5.9604644775390625e-08f;
     const float u2 = static_cast<float>((h >> 16) & 0xFFFFFFu) * 5.9604644775390625e-08f;
-    return std::sqrt(-2.0f * ln_f32(u1)) * cos2pi_f32(u2);
+    const float r = std::sqrt(-2.0f * ln_f32(u1));
+    float c, s;
+    sincos2pi_f32(u2, c, s);
+    return (i & 1024u) ? r * s : r * c;
 }
 
 // vtrap(x, y) = x / (exp(x/y) - 1) — PC/kernels/hh.cu:28-34 (NEURON hh.mod).
@@ -1138,6 +1150,11 @@ int32_t orc_poisson_spike(uint32_t seed, uint32_t stream, uint64_t step, float p
 }
 float orc_gauss(uint64_t seed, uint32_t i, uint64_t step) { return gauss(seed, i, step); }
 float orc_cos2pi_f32(float u) { return cos2pi_f32(u); }
+float orc_sin2pi_f32(float u) {
+    float c, s;
+    sincos2pi_f32(u, c, s);
+    return s;
+}
 float orc_exp_f32(float x) { return exp_f32(x); }
 float orc_ln_f32(float x) { return ln_f32(x); }
 float orc_vtrap_f32(float x, float y) { return vtrap_f32(x, y); }

@@ -121,6 +121,7 @@ float orc_fast_exp(float x);
  * p_inf, q_inf, r_inf, tau_m, tau_h, tau_n, tau_p, tau_q. */
 void orc_gpea_rates(float v, float cai, float* out);
 float orc_cos2pi_f32(float u);
+float orc_sin2pi_f32(float u);
 float orc_exp_f32(float x);
 float orc_ln_f32(float x);
 float orc_vtrap_f32(float x, float y);

@@ -222,19 +222,37 @@ def test_cos2pi_accuracy():
 
 
 def test_gauss_is_box_muller_of_the_counter_hash():
-    # z = sqrt(-2 ln u1) cos(2 pi u2) with u1 = (h>>40 + 1)/2^24, u2 = ((h>>16)&0xFFFFFF)/2^24,
-    # h = SplitMix64(SplitMix64(seed ^ i*C1) ^ step*C2): recomputed in float64 here
-    for seed, i, t in [(42, 0, 0), (42, 7, 3), (1, 123456, 999), (2 ** 40 + 5, 3, 2 ** 33)]:
-        base = _splitmix((seed ^ (i * 0xD1B54A32D192ED03)) & M64)
+    # z = sqrt(-2 ln u1) cos(2 pi u2) (bit 10 of i clear) or sin(2 pi u2) (set) with
+    # u1 = (h>>40 + 1)/2^24, u2 = ((h>>16)&0xFFFFFF)/2^24,
+    # h = SplitMix64(SplitMix64(seed ^ p*C1) ^ step*C2), p = i & ~1024: recomputed in float64 here
+    for seed, i, t in [(42, 0, 0), (42, 7, 3), (42, 1031, 3), (1, 123456, 999), (1, 123456 ^ 1024, 999),
+                       (2 ** 40 + 5, 3, 2 ** 33), (2 ** 40 + 5, 3 + 1024, 2 ** 33)]:
+        pair = i & ~1024
+        base = _splitmix((seed ^ (pair * 0xD1B54A32D192ED03)) & M64)
         h = _splitmix(base ^ ((t * 0xA24BAED4963EE407) & M64))
         u1 = ((h >> 40) + 1) / 2.0 ** 24
         u2 = ((h >> 16) & 0xFFFFFF) / 2.0 ** 24
-        want = math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+        trig = math.sin if i & 1024 else math.cos
+        want = math.sqrt(-2.0 * math.log(u1)) * trig(2.0 * math.pi * u2)
         assert abs(po.gauss(seed, i, t) - want) < 2e-6 * max(1.0, abs(want)), (seed, i, t)
 
 
+def test_sin2pi_accuracy_and_pair_independence():
+    k = np.concatenate([np.arange(0, 1 << 24, 97), [0, 1 << 22, 1 << 23, 3 << 22, (1 << 24) - 1]])
+    u = (k.astype(np.float64) / (1 << 24)).astype(np.float32)
+    got = np.array([po.sin2pi_f32(float(x)) for x in u[::7]])
+    want = np.sin(2.0 * np.pi * u[::7].astype(np.float64))
+    assert np.max(np.abs(got - want)) < 3e-7
+    assert po.sin2pi_f32(0.25) == 1.0 and po.sin2pi_f32(0.75) == -1.0 and po.sin2pi_f32(0.0) == 0.0
+    # the two members of a pair are uncorrelated normals
+    a = np.array([po.gauss(9, i, t) for i in range(100) for t in range(400)])
+    b = np.array([po.gauss(9, i + 1024, t) for i in range(100) for t in range(400)])
+    assert abs(np.corrcoef(a, b)[0, 1]) < 0.02
+    assert abs(b.var() - 1.0) < 0.02
+
+
 def test_gauss_moments():
-    z = np.array([po.gauss(42, i, t) for i in range(200) for t in range(500)])
+    z = np.array([po.gauss(42, i, t) for i in list(range(100)) + list(range(1024, 1124)) for t in range(500)])
     assert abs(z.mean()) < 0.01
     assert abs(z.var() - 1.0) < 0.015
     assert abs(np.mean(np.abs(z) > 2.0) - 0.0455) < 0.004
