@@ -222,8 +222,8 @@ struct NetState {
 
     NetArgs args() const {
         NetArgs a{};
-        a.pops = d_pops;
-        a.tile_pop = d_tile_pop;
+        a.pops = dev_ptr(d_pops);
+        a.tile_pop = dev_ptr(d_tile_pop);
         a.N = N;
         a.n_tiles = n_tiles;
         a.n_src_tiles = n_src_tiles;
@@ -232,66 +232,66 @@ struct NetState {
         a.n_fix = n_fix;
         a.n_syn = n_syn;
         a.step_limit = step_limit;
-        a.ring = ring;
+        a.ring = dev_ptr(ring);
         a.ring_fresh = max_delay <= 1 ? 1 : 0;
         a.n_bins = binned ? n_bins : 0u;
-        a.bin_hdr = bin_hdr;
+        a.bin_hdr = dev_ptr(bin_hdr);
         a.bin_rows = bin_rows;
         a.n_pops = static_cast<uint32_t>(pops.size());
         a.n_ev = n_syn + n_src_edges;
         a.hdr_stride = static_cast<uint64_t>(n_bins) * bin_rows;
-        a.bin_done = bin_done;
-        a.bin_mat = bin_mat;
-        a.bin_ev = bin_ev;
-        a.bin_slot = bin_slot;
-        a.head = head;
-        a.rec = rec;
-        a.coll = coll;
-        a.tile_coll = tile_coll;
-        a.tile_ebase = tile_ebase;
-        a.row_ptr = row_ptr;
-        a.col = col;
-        a.w = w;
-        a.delay = delay;
-        a.src_row_ptr = src_row_ptr;
-        a.src_post = src_post;
-        a.src_w = src_w;
-        a.src_delay = src_delay;
-        a.src_p = src_p;
-        a.src_seed = src_seed;
-        a.tile_count = tile_count;
-        a.active = active;
-        a.bits = bits;
-        a.tile_spikes = tile_spikes;
-        a.tile_events = tile_events;
-        a.step = step;
-        a.raster = raster;
+        a.bin_done = dev_ptr(bin_done);
+        a.bin_mat = dev_ptr(bin_mat);
+        a.bin_ev = dev_ptr(bin_ev);
+        a.bin_slot = dev_ptr(bin_slot);
+        a.head = dev_ptr(head);
+        a.rec = dev_ptr(rec);
+        a.coll = dev_ptr(coll);
+        a.tile_coll = dev_ptr(tile_coll);
+        a.tile_ebase = dev_ptr(tile_ebase);
+        a.row_ptr = dev_ptr(row_ptr);
+        a.col = dev_ptr(col);
+        a.w = dev_ptr(w);
+        a.delay = dev_ptr(delay);
+        a.src_row_ptr = dev_ptr(src_row_ptr);
+        a.src_post = dev_ptr(src_post);
+        a.src_w = dev_ptr(src_w);
+        a.src_delay = dev_ptr(src_delay);
+        a.src_p = dev_ptr(src_p);
+        a.src_seed = dev_ptr(src_seed);
+        a.tile_count = dev_ptr(tile_count);
+        a.active = dev_ptr(active);
+        a.bits = dev_ptr(bits);
+        a.tile_spikes = dev_ptr(tile_spikes);
+        a.tile_events = dev_ptr(tile_events);
+        a.step = dev_ptr(step);
+        a.raster = dev_ptr(raster);
         a.raster_cap = raster_cap;
         a.n_words = n_words;
-        a.raster_base = step + 1;
+        a.raster_base = dev_ptr(step + 1);
         a.stdp = stdp ? 1 : 0;
-        a.w64 = w64;
-        a.tr_pre = tr_pre;
-        a.tr_post = tr_post;
-        a.in_ptr = in_ptr;
-        a.in_edge = in_edge;
-        a.edge_pre = edge_pre;
-        a.plastic = plastic;
+        a.w64 = dev_ptr(w64);
+        a.tr_pre = dev_ptr(tr_pre);
+        a.tr_post = dev_ptr(tr_post);
+        a.in_ptr = dev_ptr(in_ptr);
+        a.in_edge = dev_ptr(in_edge);
+        a.edge_pre = dev_ptr(edge_pre);
+        a.plastic = dev_ptr(plastic);
         a.stdp_dpre = stdp_dpre;
         a.stdp_dpost = stdp_dpost;
         a.a_plus = sp.a_plus;
         a.a_minus = sp.a_minus;
         a.w_min = sp.w_min;
         a.w_max = sp.w_max;
-        a.tm_type = tm_type;
-        a.tm_r = tm_r;
-        a.tm_u = tm_u;
-        a.tm_last = tm_last;
-        a.tm_prof = tm_prof;
-        a.ev_w = ev_w;
-        a.rec_code = rec_code;
-        a.g = g;
-        a.g_ext = g_ext;
+        a.tm_type = dev_ptr(tm_type);
+        a.tm_r = dev_ptr(tm_r);
+        a.tm_u = dev_ptr(tm_u);
+        a.tm_last = dev_ptr(tm_last);
+        a.tm_prof = dev_ptr(tm_prof);
+        a.ev_w = dev_ptr(ev_w);
+        a.rec_code = dev_ptr(rec_code);
+        a.g = dev_ptr(g);
+        a.g_ext = dev_ptr(g_ext);
         a.rec_scale = rec_scale;
         a.n8 = N & ~7u;
         return a;

@@ -309,9 +309,9 @@ int NetState::upload_pop_table() {
         D.dstride = P.dstride;
         D.uni_mask = P.uni_mask;
         D.flags = P.flags;
-        D.st = P.st;
-        D.pr = P.pr;
-        D.sigma = P.sigma;
+        D.st = dev_ptr(P.st);
+        D.pr = dev_ptr(P.pr);
+        D.sigma = dev_ptr(P.sigma);
         D.sigma_uni = P.sigma_uni;
         D.drive_p = P.drive_p;
         D.drive_w = P.drive_w;

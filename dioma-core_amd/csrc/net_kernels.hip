@@ -579,7 +579,7 @@ __device__ __forceinline__ float stdp_apply(const NetArgs& a, uint32_t e, float 
     nw = fmin(fmax(nw, static_cast<double>(a.w_min)), static_cast<double>(a.w_max));
     a.w64[e] = nw;
     const float f = static_cast<float>(nw);
-    const_cast<float*>(a.w)[e] = f;
+    const_cast<dptr<float>>(a.w)[e] = f;
     return f;
 }
 
@@ -595,8 +595,9 @@ __device__ __forceinline__ float stdp_out_edge(const NetArgs& a, uint32_t e, uin
 // LTP on the in-edges of the spiking neurons of neuron tiles [t0, t1) whose
 // pre did not spike this step (a spiking pre's out-edge walk takes the
 // others).  Every thread of the block calls it; s_rs/s_pref hold kTile.
-__device__ void stdp_in_edges(const NetArgs& a, uint32_t t0, uint32_t t1, uint32_t* s_rs, uint32_t* s_pref,
+__device__ void stdp_in_edges(dptr<const NetArgs> nap, uint32_t t0, uint32_t t1, uint32_t* s_rs, uint32_t* s_pref,
                               uint32_t* s_wave, uint32_t* s_cnt) {
+    const NetArgs& a = *nap;
     const uint32_t t = threadIdx.x;
     const float ap = a.a_plus;
     __syncthreads();  // the caller's use of s_cnt is over
@@ -782,7 +783,7 @@ __device__ __forceinline__ void mega_done(const NetArgs& a, uint64_t step) {
 // straight onto the input (input + t_0 + ... + t_6) and the rest in a scalar
 // tail (input + (t_0 + ... + t_6)); both orders are kept.  Out of line: only
 // networks with receptor synapses call it.
-__device__ __noinline__ float receptor_input(const NetArgs* ap, const PopDev* pp, uint32_t li, float I) {
+__device__ __noinline__ float receptor_input(dptr<const NetArgs> ap, dptr<const PopDev> pp, uint32_t li, float I) {
     const NetArgs& a = *ap;
     const PopDev& P = *pp;
     const uint32_t gi = P.off + li;
@@ -902,7 +903,7 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
                 const float sg = (P.flags & kPopNoiseScalar) ? P.sigma_uni : P.sigma[li];
                 I = I + sg * normal_draw(P.seed, li, step);
             }
-            if (a.rec_code) I = receptor_input(&a, &P, li, I);
+            if (a.rec_code) I = receptor_input((dptr<const NetArgs>)&a, (dptr<const PopDev>)&P, li, I);
             spike = nrn[g].step(P, li, I, dt);
             nrn[g].store(P, li);
             if (a.stdp) {
@@ -1107,7 +1108,7 @@ __global__ __launch_bounds__(kTile) void bin_kernel(const NetArgs* __restrict__ 
     if (step >= a.step_limit) return;
     const uint32_t n_nt = b0 < a.n_tiles ? min(b1, a.n_tiles) - b0 : 0u;  // neuron tiles (<= kTile)
     if (j == kLtpItem) {  // plastic networks: the in-edge LTP of tiles [b0, b1), beside the writers
-        stdp_in_edges(a, b0, b0 + n_nt, s_rs, s_pref, s_wave, s_tp);
+        stdp_in_edges((dptr<const NetArgs>)&a, b0, b0 + n_nt, s_rs, s_pref, s_wave, s_tp);
         return;
     }
     const uint32_t base = a.tile_ebase[b0];
@@ -1242,7 +1243,7 @@ __device__ __forceinline__ void bin_link(BinApplyLds& L, uint32_t q, uint32_t ce
 // g[r][post] (CN/src/propagate.rs:225-290: g += w * scale, in push order),
 // starting from the stored conductance; `touched` marks the loaded ones.
 struct RecCtx {
-    const NetArgs* a;
+    dptr<const NetArgs> a;
     uint32_t k;  // bin
 };
 
@@ -1549,7 +1550,7 @@ __global__ __launch_bounds__(kApply) void bin_apply_kernel(const NetArgs* __rest
         L.slice[t + pr * kApply] = 0.0f;
         L.head[t + pr * kApply] = kListNil;
     }
-    bin_gather<kRec>(L, hdr + static_cast<size_t>(k) * a.bin_rows, ev, a.n_bblk, RecCtx{&a, k});
+    bin_gather<kRec>(L, hdr + static_cast<size_t>(k) * a.bin_rows, ev, a.n_bblk, RecCtx{(dptr<const NetArgs>)&a, k});
     // cell c of the bin -> its post in the slot
     float* out = a.ring + static_cast<size_t>(slot) * a.N;
 #pragma unroll
@@ -2032,7 +2033,7 @@ __global__ __launch_bounds__(kTile) void dense_stdp_kernel(const NetArgs* __rest
     s_rs[t] = rs;
     s_pref[t] = excl;
     __syncthreads();
-    float* w = const_cast<float*>(a.w);
+    const dptr<float> w = const_cast<dptr<float>>(a.w);
     for (uint32_t f = t; f < total; f += kTile) {
         uint32_t lo = 0, hi = cnt;
         while (hi - lo > 1) {

@@ -6,6 +6,23 @@
 
 namespace simuse {
 
+// A pointer into device memory.  Device code sees a global (address space 1)
+// pointer, so its loads, stores and atomics compile to global_* instructions
+// (scalar base plus 32-bit offset; unlike flat_* they do not count against
+// the LDS/scalar-load counter, so LDS work does not wait for them); host code
+// sees a plain pointer.  Host code stores into these fields through dev_ptr.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using dptr = __attribute__((address_space(1))) T*;
+#else
+template <class T>
+using dptr = T*;
+#endif
+template <class T>
+__host__ __device__ inline dptr<T> dev_ptr(T* p) {
+    return (dptr<T>)p;
+}
+
 constexpr uint32_t kTile = 256;          // neurons per tile = threads per block
 constexpr uint32_t kWavesPerTile = kTile / 64;
 constexpr uint32_t kSlots = 32;          // P/src/api/net/graph.rs:33
@@ -56,9 +73,9 @@ struct PopDev {
     uint32_t dstride;   // device column stride (multiple of 64)
     uint32_t uni_mask;
     uint32_t flags;
-    float* st;          // [ns][dstride]
-    const float* pr;    // [np][dstride]
-    const float* sigma; // [n] noise amplitude
+    dptr<float> st;          // [ns][dstride]
+    dptr<const float> pr;    // [np][dstride]
+    dptr<const float> sigma; // [n] noise amplitude
     float sigma_uni;    // noise amplitude when kPopNoiseScalar
     float drive_p;      // Poisson drive: per-step probability
     float drive_w;      //                weight of one drive event
@@ -71,8 +88,8 @@ struct PopDev {
 // Everything one step of one network touches.  An engine keeps an array of
 // these in device memory; every kernel block is handed a (network, tile) pair.
 struct NetArgs {
-    const PopDev* pops;
-    const uint32_t* tile_pop;   // tile -> population
+    dptr<const PopDev> pops;
+    dptr<const uint32_t> tile_pop;   // tile -> population
     uint32_t N;                 // neurons
     uint32_t n_tiles;           // neuron tiles
     uint32_t n_src_tiles;       // Poisson-source tiles (after the neuron tiles)
@@ -81,7 +98,7 @@ struct NetArgs {
     uint32_t n_fix;             // fix-up blocks: ceil(n_blocks / kFixTiles)
     uint64_t n_syn;
     uint64_t step_limit;        // kernels skip the network once *step reaches it
-    float* ring;                // [32][N]
+    dptr<float> ring;                // [32][N]
     int32_t ring_fresh;         // every delay <= 1: this step's target slot holds zeros
     // Binned scatter (n_bins > 0): bins of kBinTiles tiles; writer j (of
     // n_bblk, ascending event ids) puts its events grouped by bin from its
@@ -96,63 +113,63 @@ struct NetArgs {
     uint32_t n_pops;
     uint64_t n_ev;              // synapses + source edges
     uint64_t hdr_stride;        // n_bins * bin_rows
-    uint2* bin_hdr;             // [1 or 2][n_bins][bin_rows]
-    uint4* bin_ev;              // [1 or 2][n_ev] {event id, weight bits, cell, 0}
-    uint32_t* bin_slot;         // ring slot bin_apply fills, kEmpty past the horizon
-    uint32_t* bin_done;         // megastep: blocks of the step that finished
-    uint32_t* bin_mat;          // megastep: the pending events are in ring slot step % 32         // ring slot this step's events target, kEmpty past the horizon
+    dptr<uint2> bin_hdr;             // [1 or 2][n_bins][bin_rows]
+    dptr<uint4> bin_ev;              // [1 or 2][n_ev] {event id, weight bits, cell, 0}
+    dptr<uint32_t> bin_slot;         // ring slot bin_apply fills, kEmpty past the horizon
+    dptr<uint32_t> bin_done;         // megastep: blocks of the step that finished
+    dptr<uint32_t> bin_mat;          // megastep: the pending events are in ring slot step % 32
     // Scatter bookkeeping.  head[key] = (step tag << 32) | event id of the
     // last event that reached key = slot*N + post in the step with that tag;
     // a head with another tag is idle, so heads are never cleared per step.
-    unsigned long long* head;   // [32][N]
-    uint2* rec;                 // [events] per event: {earlier event of its key or kEmpty,
+    dptr<unsigned long long> head;   // [32][N]
+    dptr<uint2> rec;                 // [events] per event: {earlier event of its key or kEmpty,
                                 //   ring value before the first add (first event only)}
-    uint2* coll;                // [events] {event, key} of events that found their key
+    dptr<uint2> coll;                // [events] {event, key} of events that found their key
                                 //   taken, per block from tile_ebase[first tile]
-    uint32_t* tile_coll;        // [n_blocks] how many, this step
-    const uint32_t* tile_ebase; // [n_blocks] first event id of the tile's rows
-    const uint32_t* row_ptr;    // [N+1]
-    const uint32_t* col;        // [n_syn]
-    const float* w;             // [n_syn]
-    const uint8_t* delay;       // [n_syn]
-    const uint32_t* src_row_ptr;
-    const uint32_t* src_post;
-    const float* src_w;
-    const uint8_t* src_delay;
-    const float* src_p;
-    const uint32_t* src_seed;
-    uint32_t* tile_count;       // [n_tiles] spikes of the last step per tile
-    uint32_t* active;           // [n_tiles*kTile] ascending ids per tile
-    uint64_t* bits;             // spike bit words of the last step
-    uint64_t* tile_spikes;      // [n_tiles] running totals
-    uint64_t* tile_events;      // [n_tiles + n_src_tiles] running totals
-    uint64_t* step;             // device step counter
-    uint64_t* raster;           // optional [raster_cap][n_words]
+    dptr<uint32_t> tile_coll;        // [n_blocks] how many, this step
+    dptr<const uint32_t> tile_ebase; // [n_blocks] first event id of the tile's rows
+    dptr<const uint32_t> row_ptr;    // [N+1]
+    dptr<const uint32_t> col;        // [n_syn]
+    dptr<const float> w;             // [n_syn]
+    dptr<const uint8_t> delay;       // [n_syn]
+    dptr<const uint32_t> src_row_ptr;
+    dptr<const uint32_t> src_post;
+    dptr<const float> src_w;
+    dptr<const uint8_t> src_delay;
+    dptr<const float> src_p;
+    dptr<const uint32_t> src_seed;
+    dptr<uint32_t> tile_count;       // [n_tiles] spikes of the last step per tile
+    dptr<uint32_t> active;           // [n_tiles*kTile] ascending ids per tile
+    dptr<uint64_t> bits;             // spike bit words of the last step
+    dptr<uint64_t> tile_spikes;      // [n_tiles] running totals
+    dptr<uint64_t> tile_events;      // [n_tiles + n_src_tiles] running totals
+    dptr<uint64_t> step;             // device step counter
+    dptr<uint64_t> raster;           // optional [raster_cap][n_words]
     uint32_t raster_cap;
     uint32_t n_words;
-    uint64_t* raster_base;      // step at which recording started
+    dptr<uint64_t> raster_base;      // step at which recording started
     // pair-based STDP
     int32_t stdp;
-    double* w64;                // FP64 master weights
-    float* tr_pre;
-    float* tr_post;
-    const uint32_t* in_ptr;     // [N+1]
-    const uint32_t* in_edge;    // [n_syn] in-edges per post, ascending edge id
-    const uint32_t* edge_pre;   // [n_syn]
-    const uint8_t* plastic;     // [n_syn] or null (all)
+    dptr<double> w64;                // FP64 master weights
+    dptr<float> tr_pre;
+    dptr<float> tr_post;
+    dptr<const uint32_t> in_ptr;     // [N+1]
+    dptr<const uint32_t> in_edge;    // [n_syn] in-edges per post, ascending edge id
+    dptr<const uint32_t> edge_pre;   // [n_syn]
+    dptr<const uint8_t> plastic;     // [n_syn] or null (all)
     float stdp_dpre, stdp_dpost, a_plus, a_minus, w_min, w_max;
     // Tsodyks-Markram short-term plasticity per event id (synapses, then
     // source edges); null when no edge carries a profile
-    const uint8_t* tm_type;     // profile id, 0 = static
-    float* tm_r;
-    float* tm_u;
-    uint32_t* tm_last;
-    const float4* tm_prof;      // {u0, e^(-dt/tau_rec), e^(-dt/tau_fac), facilitating}
-    float* ev_w;                // delivered weight of each event this step (for the fix-up)
+    dptr<const uint8_t> tm_type;     // profile id, 0 = static
+    dptr<float> tm_r;
+    dptr<float> tm_u;
+    dptr<uint32_t> tm_last;
+    dptr<const float4> tm_prof;      // {u0, e^(-dt/tau_rec), e^(-dt/tau_fac), facilitating}
+    dptr<float> ev_w;                // delivered weight of each event this step (for the fix-up)
     // conductance-based receptor synapses (rec_code == nullptr: off)
-    const uint8_t* rec_code;    // per synapse: 0 current-based, 1 + receptor type
-    float* g;                   // [kReceptors][N] conductances
-    const float* g_ext;         // [N] constant external current, or nullptr
+    dptr<const uint8_t> rec_code;    // per synapse: 0 current-based, 1 + receptor type
+    dptr<float> g;                   // [kReceptors][N] conductances
+    dptr<const float> g_ext;         // [N] constant external current, or nullptr
     float rec_scale;            // a jump adds weight * rec_scale
     uint32_t n8;                // N & ~7: neurons below it sum their receptor currents onto the input
 };
