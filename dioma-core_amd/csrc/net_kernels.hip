@@ -78,15 +78,14 @@ __device__ __forceinline__ void hh_gate(float& g, float alpha, float beta, float
 // Hodgkin–Huxley with Na/K(/CaT) channels, equations of
 // PC/kernels/hh.cu:107-216 (exponential-Euler membrane, rising edge through
 // v_thresh gated by the refractory timer, the ODE keeps running while
-// refractory).
-__device__ __forceinline__ int hh_update(const PopDev& P, uint32_t li, float I, float dt) {
-    float* s = P.st;
-    const size_t ds = P.dstride;
-    float v = s[li], m = s[ds + li], h = s[2 * ds + li], n = s[3 * ds + li], refr = s[4 * ds + li];
-    const float g_na = par(P, 1, li), g_k = par(P, 2, li), g_l = par(P, 3, li);
-    const float e_na = par(P, 4, li), e_k = par(P, 5, li), e_l = par(P, 6, li);
-    const float inv_cm = par(P, 7, li), q10 = par(P, 8, li), v_th = par(P, 9, li);
-    const float g_cat = par(P, 10, li), e_cat = par(P, 11, li), tau_ref = par(P, 12, li);
+// refractory).  State v, m, h, n, refr (and the CaT gate when g_cat != 0);
+// p[k] = parameter column k.
+__device__ __forceinline__ int hh_update(float& v, float& m, float& h, float& n, float& refr, float& cg,
+                                         const float* p, float I, float dt) {
+    const float g_na = p[1], g_k = p[2], g_l = p[3];
+    const float e_na = p[4], e_k = p[5], e_l = p[6];
+    const float inv_cm = p[7], q10 = p[8], v_th = p[9];
+    const float g_cat = p[10], e_cat = p[11], tau_ref = p[12];
     const float ref = refr;
     if (ref > 0.0f) refr = fmax_ref(0.0f, ref - dt);
     const float vi = v;
@@ -108,10 +107,8 @@ __device__ __forceinline__ int hh_update(const PopDev& P, uint32_t li, float I, 
     g_ion += gk;
     i_ion += gk * e_k;
     if (g_cat != 0.0f) {
-        float cg = s[5 * ds + li];
         const float minf = 1.0f / (1.0f + det_expf(-(vi + 56.0f) / 6.2f));
         hh_gate(cg, minf / 0.6f, (1.0f - minf) / 0.6f, dt, q10);
-        s[5 * ds + li] = cg;
         const float gc = g_cat * cg;
         g_ion += gc;
         i_ion += gc * e_cat;
@@ -125,7 +122,7 @@ __device__ __forceinline__ int hh_update(const PopDev& P, uint32_t li, float I, 
         spiked = 1;
         refr = tau_ref;
     }
-    s[li] = vn; s[ds + li] = m; s[2 * ds + li] = h; s[3 * ds + li] = n; s[4 * ds + li] = refr;
+    v = vn;
     return spiked;
 }
 
@@ -381,6 +378,7 @@ struct Neuron<0> {  // LIF: states v, refr; params dt, inv_tau, v_rest, v_th, v_
         load_state(P, li);
         load_params(P, li);
     }
+    __device__ __forceinline__ float vm(const PopDev&, uint32_t) const { return v; }
     __device__ __forceinline__ int step(const PopDev&, uint32_t, float I, float dt) {
         I = fminf(fmaxf(I, -100.0f), 100.0f);  // population.rs:79-88
         return lif_update(v, refr, I, dt, p1, p2, p3, p4, p5);
@@ -407,6 +405,7 @@ struct Neuron<1> {  // Izhikevich: states v, u, refr; params dt, a, b, c, d, tau
         load_state(P, li);
         load_params(P, li);
     }
+    __device__ __forceinline__ float vm(const PopDev&, uint32_t) const { return v; }
     __device__ __forceinline__ int step(const PopDev&, uint32_t, float I, float dt) {
         I = fminf(fmaxf(I, -100.0f), 100.0f);
         return izh_update(v, u, refr, I, dt, p1, p2, p3, p4, p5, p6);
@@ -423,6 +422,7 @@ struct Neuron<3> {  // GPEA: reads and writes its columns inside the update
     __device__ __forceinline__ void load_state(const PopDev&, uint32_t) {}
     __device__ __forceinline__ void load_params(const PopDev&, uint32_t) {}
     __device__ __forceinline__ void load(const PopDev&, uint32_t) {}
+    __device__ __forceinline__ float vm(const PopDev& P, uint32_t li) const { return P.st[li]; }
     __device__ __forceinline__ int step(const PopDev& P, uint32_t li, float I, float dt) {
         return gpea_update(P, li, I, dt);
     }
@@ -430,14 +430,40 @@ struct Neuron<3> {  // GPEA: reads and writes its columns inside the update
 };
 
 template <>
-struct Neuron<2> {  // Hodgkin-Huxley: reads and writes its columns inside the update
-    __device__ __forceinline__ void load_state(const PopDev&, uint32_t) {}
-    __device__ __forceinline__ void load_params(const PopDev&, uint32_t) {}
-    __device__ __forceinline__ void load(const PopDev&, uint32_t) {}
-    __device__ __forceinline__ int step(const PopDev& P, uint32_t li, float I, float dt) {
-        return hh_update(P, li, I, dt);  // #[no_clamp] (hh.rs:13)
+struct Neuron<2> {  // Hodgkin-Huxley: states v, m, h, n, refr (, CaT gate); params 1..12
+    float v, m, h, n, refr, cg;
+    float p[13];
+    __device__ __forceinline__ void load_state(const PopDev& P, uint32_t li) {
+        const size_t ds = P.dstride;
+        v = P.st[li];
+        m = P.st[ds + li];
+        h = P.st[2 * ds + li];
+        n = P.st[3 * ds + li];
+        refr = P.st[4 * ds + li];
     }
-    __device__ __forceinline__ void store(const PopDev&, uint32_t) const {}
+    __device__ __forceinline__ void load_params(const PopDev& P, uint32_t li) {
+        p[0] = 0.0f;
+#pragma unroll
+        for (int k = 1; k < 13; ++k) p[k] = par(P, k, li);
+        cg = p[10] != 0.0f ? P.st[5 * static_cast<size_t>(P.dstride) + li] : 0.0f;
+    }
+    __device__ __forceinline__ void load(const PopDev& P, uint32_t li) {
+        load_state(P, li);
+        load_params(P, li);
+    }
+    __device__ __forceinline__ float vm(const PopDev&, uint32_t) const { return v; }
+    __device__ __forceinline__ int step(const PopDev&, uint32_t, float I, float dt) {
+        return hh_update(v, m, h, n, refr, cg, p, I, dt);  // #[no_clamp] (hh.rs:13)
+    }
+    __device__ __forceinline__ void store(const PopDev& P, uint32_t li) const {
+        const size_t ds = P.dstride;
+        P.st[li] = v;
+        P.st[ds + li] = m;
+        P.st[2 * ds + li] = h;
+        P.st[3 * ds + li] = n;
+        P.st[4 * ds + li] = refr;
+        if (p[10] != 0.0f) P.st[5 * ds + li] = cg;
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -781,36 +807,43 @@ __device__ __forceinline__ void mega_done(const NetArgs& a, uint64_t step) {
 // B(v) = 1 / (1 + 0.33 fast_exp(clamp(-0.06 v, -10, 10))) (receptor.rs:
 // 126-150).  The reference adds neurons below n8 = N & ~7 in 8-wide lanes
 // straight onto the input (input + t_0 + ... + t_6) and the rest in a scalar
-// tail (input + (t_0 + ... + t_6)); both orders are kept.  Out of line: only
-// networks with receptor synapses call it.
-__device__ __noinline__ float receptor_input(dptr<const NetArgs> ap, dptr<const PopDev> pp, uint32_t li, float I) {
-    const NetArgs& a = *ap;
-    const PopDev& P = *pp;
-    const uint32_t gi = P.off + li;
-    const float v = P.st[li];  // state column 0
-    const float i0 = a.g_ext ? I + a.g_ext[gi] : I;
+// tail (input + (t_0 + ... + t_6)); both orders are kept.  The conductances
+// are loaded (rec_load) with the neuron's state, before any store of the step.
+struct RecIn {
+    float g[kReceptors];
+    float ext;
+};
+__device__ __forceinline__ void rec_load(const NetArgs& a, uint32_t gi, RecIn& r) {
+#pragma unroll
+    for (int k = 0; k < kReceptors; ++k) r.g[k] = a.g[static_cast<size_t>(k) * a.N + gi];
+    r.ext = a.g_ext ? a.g_ext[gi] : 0.0f;
+}
+// v: the membrane potential before the update; r: the neuron's conductances
+// and external current, loaded by rec_load
+__device__ __forceinline__ float receptor_input(const NetArgs& a, const PopDev& P, uint32_t gi, float v,
+                                                const RecIn& r, float I) {
+    const float i0 = a.g_ext ? I + r.ext : I;
     float x = -0.06f * v;
     x = x < -10.0f ? -10.0f : x;
     x = x > 10.0f ? 10.0f : x;
     const float block = 1.0f / (1.0f + 0.33f * fast_expf(x));
     float t[kReceptors];
 #pragma unroll
-    for (int r = 0; r < kReceptors; ++r) {
-        float* gp = a.g + static_cast<size_t>(r) * a.N + gi;
-        const float g = *gp * P.gdec[r];
-        *gp = g;
-        const float ge = r == kRecNmda ? g * block : g;
-        t[r] = ge * (kRecErev[r] - v);
+    for (int k = 0; k < kReceptors; ++k) {
+        const float g = r.g[k] * P.gdec[k];
+        a.g[static_cast<size_t>(k) * a.N + gi] = g;
+        const float ge = k == kRecNmda ? g * block : g;
+        t[k] = ge * (kRecErev[k] - v);
     }
     if (gi < a.n8) {
         float cur = i0;
 #pragma unroll
-        for (int r = 0; r < kReceptors; ++r) cur = cur + t[r];
+        for (int k = 0; k < kReceptors; ++k) cur = cur + t[k];
         return cur;
     }
     float acc = 0.0f;
 #pragma unroll
-    for (int r = 0; r < kReceptors; ++r) acc = acc + t[r];
+    for (int k = 0; k < kReceptors; ++k) acc = acc + t[k];
     return i0 + acc;
 }
 
@@ -851,6 +884,8 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
     const uint64_t step = *a.step;
     float* ring_slot = a.ring + static_cast<size_t>(step & (kSlots - 1)) * a.N + P.off;
     Neuron<MODEL> nrn[kMaxGroup];
+    RecIn rin[kMaxGroup];
+    float trp[kMaxGroup], trq[kMaxGroup];  // STDP traces
     uint32_t row0[kMaxGroup], deg[kMaxGroup];
     float cur[kMaxGroup];
 #pragma unroll
@@ -860,6 +895,11 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
         const uint32_t li = li0 + g * kTile;
         if (g < ntile && li < P.n) {
             nrn[g].load(P, li);
+            if (a.rec_code) rec_load(a, P.off + li, rin[g]);
+            if (a.stdp) {
+                trp[g] = a.tr_pre[P.off + li];
+                trq[g] = a.tr_post[P.off + li];
+            }
             if (inline_push) {
                 row0[g] = a.row_ptr[P.off + li];
                 deg[g] = a.row_ptr[P.off + li + 1] - row0[g];
@@ -903,14 +943,14 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
                 const float sg = (P.flags & kPopNoiseScalar) ? P.sigma_uni : P.sigma[li];
                 I = I + sg * normal_draw(P.seed, li, step);
             }
-            if (a.rec_code) I = receptor_input((dptr<const NetArgs>)&a, (dptr<const PopDev>)&P, li, I);
+            if (a.rec_code) I = receptor_input(a, P, P.off + li, nrn[g].vm(P, li), rin[g], I);
             spike = nrn[g].step(P, li, I, dt);
             nrn[g].store(P, li);
             if (a.stdp) {
                 const uint32_t gi = P.off + li;
                 const float add = spike ? 1.0f : 0.0f;
-                a.tr_pre[gi] = fmaf(a.tr_pre[gi], a.stdp_dpre, add);
-                a.tr_post[gi] = fmaf(a.tr_post[gi], a.stdp_dpost, add);
+                a.tr_pre[gi] = fmaf(trp[g], a.stdp_dpre, add);
+                a.tr_post[gi] = fmaf(trq[g], a.stdp_dpost, add);
             }
         }
         m[g] = __ballot(spike);
