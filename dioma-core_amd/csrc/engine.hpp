@@ -184,6 +184,7 @@ struct NetState {
     std::vector<uint8_t> h_rec_code;  // per synapse: 0 current-based, 1 + receptor type; empty = off
     std::vector<float> h_g_ext;       // constant external current per neuron (empty: none)
     float rec_scale = 1.0f;
+    uint32_t rec_live = 0;  // bit r: receptor type r can hold a nonzero conductance
     uint8_t* rec_code = nullptr;
     float* g = nullptr;               // [kReceptors][N]
     float* g_ext = nullptr;
@@ -293,6 +294,7 @@ struct NetState {
         a.g = dev_ptr(g);
         a.g_ext = dev_ptr(g_ext);
         a.rec_scale = rec_scale;
+        a.rec_live = rec_live;
         a.n8 = N & ~7u;
         return a;
     }

@@ -436,6 +436,9 @@ int NetState::set_receptors(const uint8_t* types, float scale, const float* ext)
                 return fail(SIMUSE_E_INVALID, "receptor synapses: external current %u must be finite", i);
         x.assign(ext, ext + N);
     }
+    rec_live = 0;
+    for (const uint8_t c : code)
+        if (c) rec_live |= 1u << (c - 1);
     h_rec_code = std::move(code);
     if (h_rec_code.empty()) h_rec_code.assign(1, 0);  // no synapses: still on (decay, external current)
     h_g_ext = std::move(x);

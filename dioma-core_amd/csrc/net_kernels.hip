@@ -815,7 +815,8 @@ struct RecIn {
 };
 __device__ __forceinline__ void rec_load(const NetArgs& a, uint32_t gi, RecIn& r) {
 #pragma unroll
-    for (int k = 0; k < kReceptors; ++k) r.g[k] = a.g[static_cast<size_t>(k) * a.N + gi];
+    for (int k = 0; k < kReceptors; ++k)
+        r.g[k] = ((a.rec_live >> k) & 1u) ? a.g[static_cast<size_t>(k) * a.N + gi] : 0.0f;
     r.ext = a.g_ext ? a.g_ext[gi] : 0.0f;
 }
 // v: the membrane potential before the update; r: the neuron's conductances
@@ -831,7 +832,7 @@ __device__ __forceinline__ float receptor_input(const NetArgs& a, const PopDev& 
 #pragma unroll
     for (int k = 0; k < kReceptors; ++k) {
         const float g = r.g[k] * P.gdec[k];
-        a.g[static_cast<size_t>(k) * a.N + gi] = g;
+        if ((a.rec_live >> k) & 1u) a.g[static_cast<size_t>(k) * a.N + gi] = g;
         const float ge = k == kRecNmda ? g * block : g;
         t[k] = ge * (kRecErev[k] - v);
     }

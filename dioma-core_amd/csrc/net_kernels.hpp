@@ -171,6 +171,7 @@ struct NetArgs {
     dptr<float> g;                   // [kReceptors][N] conductances
     dptr<const float> g_ext;         // [N] constant external current, or nullptr
     float rec_scale;            // a jump adds weight * rec_scale
+    uint32_t rec_live;          // bit r: some synapse has receptor type r (else g[r] stays 0)
     uint32_t n8;                // N & ~7: neurons below it sum their receptor currents onto the input
 };
 

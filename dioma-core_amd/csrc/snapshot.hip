@@ -362,7 +362,18 @@ int NetState::load_state(const char* path, hipStream_t stream) {
         HIP_TRY(hipMemcpy(tr_pre, s_pre, 4 * static_cast<size_t>(N), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(tr_post, s_post, 4 * static_cast<size_t>(N), hipMemcpyHostToDevice));
     }
-    if (s_g) HIP_TRY(hipMemcpy(g, s_g, 4 * static_cast<size_t>(kReceptors) * N, hipMemcpyHostToDevice));
+    if (s_g) {
+        HIP_TRY(hipMemcpy(g, s_g, 4 * static_cast<size_t>(kReceptors) * N, hipMemcpyHostToDevice));
+        // a receptor type without synapses keeps whatever conductance the file holds
+        for (int r = 0; r < kReceptors; ++r) {
+            const uint8_t* col = s_g + 4 * static_cast<size_t>(r) * N;
+            for (uint32_t i = 0; i < N && !((rec_live >> r) & 1u); ++i) {
+                uint32_t b;
+                std::memcpy(&b, col + 4 * static_cast<size_t>(i), 4);
+                if (b) rec_live |= 1u << r;  // the caller rebuilds the kernels' descriptors
+            }
+        }
+    }
     if (s_dt) {
         if (!dtrace && (rc = alloc(&dtrace, N))) return rc;
         HIP_TRY(hipMemcpy(dtrace, s_dt, 4 * static_cast<size_t>(N), hipMemcpyHostToDevice));

@@ -203,3 +203,50 @@ def test_receptor_reset_and_long_delay_refused():
         net2.set_receptors(np.zeros(3, np.uint8))
     with pytest.raises(SimuseError, match="scale must be finite"):
         net2.set_receptors(np.zeros(net2.n_syn, np.uint8), float("nan"))
+
+
+def test_receptor_types_without_synapses(tmp_path):
+    # only AMPA and GABA_A synapses: the other five conductances stay zero and
+    # the step skips their columns; a snapshot whose file holds a nonzero
+    # conductance of such a type (here GABA_B and NMDA) brings them back
+    say("dead receptor types")
+    from simuse.formats import TaasReader, TaasWriter
+    n = 2000
+    spec = random_net(n, 20, seed=31, model="lif", max_delay=1, sources=2)
+    ext = np.zeros(n, np.float32)
+    ext[: n // 10] = 3.0
+    spec = with_receptors(spec, 31, 0.02, ext=ext, codes=[0, 2])
+    dev, ora, total = run_check(spec, 120)
+    assert total > 0
+    g = dev.download_conductances()
+    assert np.count_nonzero(g[[0, 2]]) > 0 and not np.any(g[[1, 3, 4, 5, 6]])
+    path = tmp_path / "s.taas"
+    dev.save_state(path)
+    r = TaasReader.from_file(path)
+    w = TaasWriter()
+    for k, v in r.metadata().items():
+        w.add_metadata(k, v)
+    rng = np.random.default_rng(31)
+    for s in r.list_sections():
+        raw = r._read(s)
+        if s.name == "receptors.g":
+            gg = np.frombuffer(raw, np.float32).reshape(7, n).copy()
+            gg[3, rng.integers(0, n, 300)] = 0.5   # GABA_B
+            gg[1, rng.integers(0, n, 300)] = 0.25  # NMDA
+            raw = gg.tobytes()
+        w.push(s.name, s.kind, s.count, raw)
+    edited = tmp_path / "edited.taas"
+    w.to_file(edited)
+    dev2 = build_device(spec)
+    dev2.load_state(edited)
+    ora2 = po.OracleNetwork(spec, 32)
+    ora2.set_state_from_snapshot(edited)
+    for t in range(150):
+        dev2.step()
+        ora2.step()
+        assert np.array_equal(dev2.download_spike_ids(), ora2.active()), t
+    gd, go = dev2.download_conductances(), ora2.conductances().astype(np.float32)
+    assert np.count_nonzero(gd[3]) > 0 and np.count_nonzero(gd[1]) > 0
+    assert np.array_equal(gd.view(np.uint32), go.view(np.uint32))
+    st = dev2.download_states(0)
+    assert np.array_equal(st[0, :n].view(np.uint32), ora2.state(0, 0).astype(np.float32).view(np.uint32))
