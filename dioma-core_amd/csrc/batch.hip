@@ -200,6 +200,7 @@ int build_generated(NetState& s, const simuse_gen_desc& g, hipStream_t stream) {
     HIP_TRY(hipGetLastError());
     s.h_row_ptr.resize(static_cast<size_t>(n) + 1);
     for (uint32_t i = 0; i <= n; ++i) s.h_row_ptr[i] = i * g.k;
+    s.uni_deg = g.k;  // row i holds edges [i k, (i + 1) k)
     if ((rc = s.alloc(&s.ring, static_cast<size_t>(kSlots) * s.N))) return rc;
     if ((rc = s.alloc(&s.tile_count, s.n_tiles))) return rc;
     if ((rc = s.alloc(&s.active, static_cast<size_t>(s.n_tiles) * kTile))) return rc;

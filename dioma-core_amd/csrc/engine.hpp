@@ -196,6 +196,7 @@ struct NetState {
     float* inj_val = nullptr;
     uint32_t inj_cap = 0;
     std::vector<uint32_t> h_row_ptr;
+    uint32_t uni_deg = 0;  // every row has this out-degree (0: rows differ)
 
     NetState() = default;
     NetState(const NetState&) = delete;
@@ -251,6 +252,7 @@ struct NetState {
         a.tile_coll = dev_ptr(tile_coll);
         a.tile_ebase = dev_ptr(tile_ebase);
         a.row_ptr = dev_ptr(row_ptr);
+        a.uni_deg = uni_deg;
         a.col = dev_ptr(col);
         a.w = dev_ptr(w);
         a.delay = dev_ptr(delay);

@@ -122,6 +122,10 @@ int NetState::init(const simuse_pop_desc* pd, int32_t n_pops, const uint32_t* rp
         HIP_TRY(hipMemcpy(delay, dl, nsyn, hipMemcpyHostToDevice));
     }
     h_row_ptr.assign(rp, rp + N + 1);
+    // every row with the same out-degree: the kernels compute row extents
+    uni_deg = N && n_syn % N == 0 && n_syn / N > 0 && n_syn / N < (1ull << 20) ? static_cast<uint32_t>(n_syn / N) : 0u;
+    for (uint32_t i = 0; i <= N && uni_deg; ++i)
+        if (rp[i] != static_cast<uint64_t>(i) * uni_deg) uni_deg = 0u;
     max_delay = 0;
     for (uint64_t e = 0; e < nsyn; ++e) max_delay = std::max<uint32_t>(max_delay, dl[e]);
     if ((rc = alloc(&ring, static_cast<size_t>(kSlots) * N))) return rc;

@@ -556,6 +556,19 @@ __device__ __forceinline__ uint32_t floor_index(const uint32_t* v, uint32_t lo0,
     return lo;
 }
 
+// CSR extent of row i: first edge and out-degree.  A network whose rows all
+// have the same out-degree d (uni_deg = d) computes it instead of reading
+// row_ptr, which takes a dependent global load off the spike path.
+__device__ __forceinline__ void row_extent(const NetArgs& a, uint32_t i, uint32_t& r0, uint32_t& d) {
+    if (a.uni_deg) {
+        r0 = i * a.uni_deg;
+        d = a.uni_deg;
+    } else {
+        r0 = a.row_ptr[i];
+        d = a.row_ptr[i + 1] - r0;
+    }
+}
+
 // The network's population table in LDS: first neuron and first tile.
 struct BinMap {
     uint32_t off[kMaxBinPops];
@@ -902,8 +915,7 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
                 trq[g] = a.tr_post[P.off + li];
             }
             if (inline_push) {
-                row0[g] = a.row_ptr[P.off + li];
-                deg[g] = a.row_ptr[P.off + li + 1] - row0[g];
+                row_extent(a, P.off + li, row0[g], deg[g]);
             }
             cur[g] = ring_slot[li];
         }
@@ -1188,9 +1200,10 @@ __global__ __launch_bounds__(kTile) void bin_kernel(const NetArgs* __restrict__ 
                 for (uint32_t u = 0; u < kRowsPer; ++u) {
                     const uint32_t i = t + u * kTile;
                     if (i < nr) {
-                        const uint32_t rs = a.row_ptr[id[u]];
+                        uint32_t rs, dg;
+                        row_extent(a, id[u], rs, dg);
                         s_rs[i] = rs;
-                        s_pref[i] = a.row_ptr[id[u] + 1] - rs;
+                        s_pref[i] = dg;
                     }
                 }
                 __syncthreads();
@@ -1734,8 +1747,7 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
             spike = nrn[q].step(P, i, I, P.puni[0]);
             nrn[q].store(P, i);
             if (spike) {  // only spiking rows need their CSR extent
-                row0[q] = a.row_ptr[P.off + i];
-                deg[q] = a.row_ptr[P.off + i + 1] - row0[q];
+                row_extent(a, P.off + i, row0[q], deg[q]);
             }
         }
         m[q] = __ballot(spike);

@@ -129,6 +129,7 @@ struct NetArgs {
     dptr<uint32_t> tile_coll;        // [n_blocks] how many, this step
     dptr<const uint32_t> tile_ebase; // [n_blocks] first event id of the tile's rows
     dptr<const uint32_t> row_ptr;    // [N+1]
+    uint32_t uni_deg;                // every row's out-degree when they are all equal, else 0
     dptr<const uint32_t> col;        // [n_syn]
     dptr<const float> w;             // [n_syn]
     dptr<const uint8_t> delay;       // [n_syn]

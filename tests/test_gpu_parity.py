@@ -549,3 +549,26 @@ def test_gpea_network_bitexact():
     assert dev.counters() == ora.counters()
     assert_same_state(dev, ora, spec)
     assert_same_ring(dev, ora)
+
+
+@pytest.mark.parametrize("plastic", [False, True])
+def test_fixed_out_degree_network_bitexact(plastic):
+    # every row holds k edges: the kernels compute the CSR row extents (i k,
+    # k) instead of reading row_ptr; one-kernel form, and with pair STDP the
+    # two-kernel form whose writers take the extents the same way
+    say("fixed out-degree", plastic)
+    from simuse.workloads import izhikevich_network
+    spec = izhikevich_network(n=30000, k=40, seed=17, steps=300)
+    n = sum(p.n for p in spec.pops)
+    assert np.array_equal(spec.row_ptr.astype(np.uint64), np.arange(n + 1, dtype=np.uint64) * 40)
+    if plastic:
+        rng = np.random.default_rng(17)
+        spec.w = np.abs(spec.w)
+        spec.stdp = dict(a_plus=0.01, a_minus=0.012, tau_plus=20.0, tau_minus=20.0, w_min=0.0, w_max=1.0,
+                         plastic=(rng.random(spec.n_syn) < 0.5).astype(np.uint8))
+    dev, ora = run_pair(spec, 300, check_every=1, graph_steps=16)
+    assert ora.counters()[0] > 1000
+    assert_same_state(dev, ora, spec)
+    if plastic:
+        assert np.array_equal(dev.download_weights().view(np.uint32),
+                              ora.weights().astype(np.float32).view(np.uint32))
