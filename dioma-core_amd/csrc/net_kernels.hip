@@ -126,15 +126,26 @@ __device__ __forceinline__ int hh_update(float& v, float& m, float& h, float& n,
     return spiked;
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave, every lane active: four
+// DPP row shifts scan each row of 16 lanes, then row_bcast:15 and
+// row_bcast:31 carry the row totals forward (GFX9 data-parallel primitives,
+// full-rate VALU; a __shfl_up ladder goes through the LDS crossbar instead).
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    int v = static_cast<int>(x);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+    return static_cast<uint32_t>(v);
+}
+
 // Exclusive prefix sum over one block of kTile threads; returns the total.
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t* s_wave, uint32_t& excl) {
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     uint32_t inc = x;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(inc, o, 64);
-        if (lane >= static_cast<uint32_t>(o)) inc += y;
-    }
+    inc = wave_incl_scan(inc);
     if (lane == 63) s_wave[wave] = inc;
     __syncthreads();
     uint32_t base = 0, total = 0;
@@ -497,11 +508,7 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t x, uint32_t* s_wave, uin
     constexpr uint32_t kW = kThreads / 64;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     uint32_t inc = x;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(inc, o, 64);
-        if (lane >= static_cast<uint32_t>(o)) inc += y;
-    }
+    inc = wave_incl_scan(inc);
     if (lane == 63) s_wave[wave] = inc;
     __syncthreads();
     uint32_t base = 0, total = 0;
@@ -971,11 +978,7 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
         const uint32_t d = spike ? deg[g] : 0u;
         uint32_t inc = d;
         if (m[g]) {
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t y = __shfl_up(inc, o, 64);
-                if (lane >= static_cast<uint32_t>(o)) inc += y;
-            }
+            inc = wave_incl_scan(inc);
         }
         excl[g] = inc - d;
         if (g < ntile) {
@@ -1755,11 +1758,7 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
         const uint32_t d = spike ? deg[q] : 0u;
         uint32_t inc = d;
         if (m[q]) {
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t y = __shfl_up(inc, o, 64);
-                if (lane >= static_cast<uint32_t>(o)) inc += y;
-            }
+            inc = wave_incl_scan(inc);
         }
         excl[q] = inc - d;
         if (lane == 63) s_deg[ti][wq] = inc;
