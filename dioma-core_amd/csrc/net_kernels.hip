@@ -797,8 +797,18 @@ __device__ __forceinline__ void bin_offsets(const NetArgs& a, uint32_t* s_bin, u
                                             uint32_t base, uint32_t events, uint2* hdr) {
     __syncthreads();
     const uint32_t nb = a.n_bins, t = threadIdx.x;
-    if (nb <= kThreads) lds_scan<kThreads, 1>(s_bin, nb, s_wave);
-    else if (nb <= 4 * kThreads) lds_scan<kThreads, 4>(s_bin, nb, s_wave);
+    if (nb <= kThreads) {  // one bin per thread: its count, offset and header at once
+        const uint32_t c = t < nb ? s_bin[t] : 0u;
+        uint32_t excl;
+        (void)block_scan<kThreads>(c, s_wave, excl);
+        if (t < nb) {
+            s_bin[t] = base + excl;
+            if (c) hdr[static_cast<size_t>(t) * a.bin_rows + j] = make_uint2(base + excl, c);
+        }
+        __syncthreads();
+        return;
+    }
+    if (nb <= 4 * kThreads) lds_scan<kThreads, 4>(s_bin, nb, s_wave);
     else lds_scan<kThreads, (kMaxBins + kThreads - 1) / kThreads>(s_bin, nb, s_wave);
     for (uint32_t k = t; k < nb; k += kThreads) {
         const uint32_t o = s_bin[k], e = k + 1 < nb ? s_bin[k + 1] : events;
@@ -1787,17 +1797,19 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
             ed += s_deg[t][w];
             dv += s_drv[t][w];
         }
-        uint32_t isp = sp, ied = ed, idv = dv;
-#pragma unroll
-        for (uint32_t o = 1; o < kBinTiles; o <<= 1) {
-            const uint32_t a1 = __shfl_up(isp, o, kBinTiles), a2 = __shfl_up(ied, o, kBinTiles),
-                           a3 = __shfl_up(idv, o, kBinTiles);
-            if (t >= o) {
-                isp += a1;
-                ied += a2;
-                idv += a3;
-            }
-        }
+        // inclusive scan over lanes 0..7 (one DPP row; a lane shifted out of
+        // the row reads 0)
+        static_assert(kBinTiles == 8, "one DPP row of eight lanes");
+        int isp = static_cast<int>(sp), ied = static_cast<int>(ed), idv = static_cast<int>(dv);
+        isp += __builtin_amdgcn_update_dpp(0, isp, 0x111, 0xf, 0xf, false);
+        ied += __builtin_amdgcn_update_dpp(0, ied, 0x111, 0xf, 0xf, false);
+        idv += __builtin_amdgcn_update_dpp(0, idv, 0x111, 0xf, 0xf, false);
+        isp += __builtin_amdgcn_update_dpp(0, isp, 0x112, 0xf, 0xf, false);
+        ied += __builtin_amdgcn_update_dpp(0, ied, 0x112, 0xf, 0xf, false);
+        idv += __builtin_amdgcn_update_dpp(0, idv, 0x112, 0xf, 0xf, false);
+        isp += __builtin_amdgcn_update_dpp(0, isp, 0x114, 0xf, 0xf, false);
+        ied += __builtin_amdgcn_update_dpp(0, ied, 0x114, 0xf, 0xf, false);
+        idv += __builtin_amdgcn_update_dpp(0, idv, 0x114, 0xf, 0xf, false);
         s_tp[t][0] = isp - sp;
         s_tp[t][1] = ied - ed;
         if (t == kBinTiles - 1) {
