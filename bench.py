@@ -282,7 +282,8 @@ def main() -> int:
     ap.add_argument("--neurons", type=int, default=1_000_000)
     ap.add_argument("--synapses-per-neuron", type=int, default=100)
     ap.add_argument("--graph-steps", type=int, default=16)
-    ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--profile-steps", type=int, default=100,
+                    help="steps timed per kernel with HIP events for the roofline (at least 1)")
     ap.add_argument("--cpu-steps", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scatter", choices=["auto", "tile", "wave"], default="auto")
@@ -301,6 +302,8 @@ def main() -> int:
     ap.add_argument("--prec-full-neurons", type=int, default=200_000)
     ap.add_argument("--prec-full-steps", type=int, default=2000)
     args = ap.parse_args()
+    if args.profile_steps < 1:
+        ap.error("--profile-steps must be at least 1 (the roofline needs the step kernel's time)")
     if args.copy_calibration:
         return copy_calibration()
 
