@@ -1302,6 +1302,33 @@ int simuse_net_kernel_times(simuse_net* net, float* ms, int32_t n) {
     return net->eng.kernel_times(ms, n);
 }
 
+int simuse_net_time_steps(simuse_net* net, uint32_t n, float* ms) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    if (!ms) return fail(SIMUSE_E_INVALID, "null output");
+    return guarded([&]() {
+        int r = net->eng.prepare();
+        if (r) return r;
+        if (net->eng.profiling) return fail(SIMUSE_E_STATE, "time_steps: turn profiling off first");
+        hipEvent_t e[2] = {nullptr, nullptr};
+        HIP_TRY(hipEventCreate(&e[0]));
+        if (hipEventCreate(&e[1]) != hipSuccess) {
+            (void)hipEventDestroy(e[0]);
+            return fail(SIMUSE_E_HIP, "hipEventCreate failed");
+        }
+        hipError_t he = hipEventRecord(e[0], net->eng.stream);
+        if (he == hipSuccess) r = net->eng.step_async(n);
+        if (he == hipSuccess && r == SIMUSE_OK) he = hipEventRecord(e[1], net->eng.stream);
+        if (he == hipSuccess && r == SIMUSE_OK) he = hipEventSynchronize(e[1]);
+        if (he == hipSuccess && r == SIMUSE_OK) he = hipEventElapsedTime(ms, e[0], e[1]);
+        (void)hipEventDestroy(e[0]);
+        (void)hipEventDestroy(e[1]);
+        if (r) return r;
+        if (he != hipSuccess) return fail(SIMUSE_E_HIP, "time_steps: %s", hipGetErrorString(he));
+        return SIMUSE_OK;
+    });
+}
+
 int simuse_net_device_bytes(simuse_net* net, uint64_t* bytes) {
     int rc = check_net(net);
     if (rc) return rc;

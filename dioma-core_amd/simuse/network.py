@@ -339,6 +339,13 @@ class CompiledNetwork:
     def set_profiling(self, on: bool) -> None:
         check(lib().simuse_net_set_profiling(self._h, 1 if on else 0))
 
+    def time_steps(self, n: int) -> float:
+        """Run n steps (captured graphs) between two HIP events on the
+        network's stream; returns the elapsed milliseconds."""
+        out = C.c_float()
+        check(lib().simuse_net_time_steps(self._h, int(n), C.byref(out)))
+        return float(out.value)
+
     def kernel_times(self):
         """Mean ms per step of [neuron (+ fused scatter), stdp, insert, fix-up] over profiled steps."""
         out = (C.c_float * 4)()

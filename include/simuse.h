@@ -258,6 +258,11 @@ int simuse_net_device_bytes(simuse_net* net, uint64_t* bytes);
  * fix-up]. */
 int simuse_net_set_profiling(simuse_net* net, int32_t on);
 int simuse_net_kernel_times(simuse_net* net, float* ms, int32_t n);
+/* No counterpart in the reference (a measurement helper): run n steps as
+ * simuse_net_step_async does (captured graphs) between two HIP events on the
+ * handle's stream, wait for the second, and return the elapsed time in ms.
+ * Profiling must be off. */
+int simuse_net_time_steps(simuse_net* net, uint32_t n, float* ms);
 
 /* ---------------------------------------------------------------------------
  * Batches: many independent networks stepped by one launch sequence.
