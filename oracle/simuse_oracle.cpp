@@ -107,8 +107,6 @@ static inline float exp_f32(float x) {
     return p * bits_to_f(static_cast<uint32_t>(k + 127) << 23);
 }
 
-// Standard normal draw for (seed, neuron, step): Marsaglia polar method on
-// 24-bit uniforms from mix64.  sqrt and / are IEEE correctly rounded.
 // x^n for n >= 0 by right-to-left binary exponentiation, the expansion of
 // f32::powi (compiler-rt __powisf2): r *= a on set bits, a *= a between.
 template <class R>

@@ -201,6 +201,58 @@ def test_poisson_roll_rate_and_independence():
 
 
 M64 = (1 << 64) - 1
+PHI64 = 0x9E3779B97F4A7C15
+
+# Published outputs of the SplitMix64 generator (Steele, Lea & Flood 2014;
+# Vigna's splitmix64.c: x += phi; z = x; ...finaliser...): the first five
+# draws after seeding with 1234567, and the first three after seeding with 0.
+SPLITMIX_1234567 = [6457827717110365317, 3203168211198807973, 9817491932198370423,
+                    4593380528125082431, 16408922859458223821]
+SPLITMIX_0 = [0xE220A8397B1DCDAF, 7960286522194355700, 487617019471545679]
+
+
+def _fires_exactly_below(h, seed, stream, step):
+    """poisson_spike compares u = (h >> 40) / 2^24 with p: it fires for every p
+    above u and for none at or below it (the float grid of u is exact)."""
+    u = np.float32((h >> 40) / float(1 << 24))
+    above = np.nextafter(u, np.float32(2.0))
+    below = np.nextafter(u, np.float32(-1.0)) if u > 0 else None
+    assert po.poisson_spike(seed, stream, step, float(above)), (seed, stream, step)
+    assert not po.poisson_spike(seed, stream, step, float(u)), (seed, stream, step)
+    if below is not None:
+        assert not po.poisson_spike(seed, stream, step, float(below)), (seed, stream, step)
+
+
+def test_poisson_hash_is_splitmix64_on_published_vectors():
+    # P/src/api/net/network/sources.rs:5-14: key = (seed << 32) | stream,
+    # z = key ^ step*phi, then SplitMix64's step (z += phi, finaliser).  The
+    # generator's n-th draw from seed x is finaliser(x + n*phi), so a key
+    # k = (x + (n-1)*phi) ^ step*phi hashes to that published draw.  Seed and
+    # stream are the key's high and low halves, so a wrong shift, a missing
+    # OR, a wrong step multiplier or a dropped z += phi all land elsewhere.
+    cases = []
+    for n, h in enumerate(SPLITMIX_1234567, start=1):
+        for step in (0, 1, 2, 12345, 2 ** 40 + 3):
+            z = (1234567 + (n - 1) * PHI64) & M64
+            key = z ^ ((step * PHI64) & M64)
+            cases.append((h, key >> 32, key & 0xFFFFFFFF, step))
+    # seed 0, stream 0: step s hashes s*phi, i.e. the (s+1)-th draw from seed 0
+    for s, h in enumerate(SPLITMIX_0):
+        cases.append((h, 0, 0, s))
+    assert any(c[1] != 0 for c in cases) and any(c[2] != 0 for c in cases)
+    for h, seed, stream, step in cases:
+        _fires_exactly_below(h, seed, stream, step)
+
+
+def test_poisson_threshold_on_the_2_pow_minus_24_grid():
+    # u takes the top 24 bits of the hash: p exactly on the grid value k/2^24
+    # fires only when the draw is below k; p = 1 always fires, p = 0 never
+    h = SPLITMIX_0[0]  # seed 0, stream 0, step 0
+    k = h >> 40
+    assert k == 0xE220A8
+    assert not po.poisson_spike(0, 0, 0, k / float(1 << 24))
+    assert po.poisson_spike(0, 0, 0, (k + 1) / float(1 << 24))
+    assert po.poisson_spike(0, 0, 0, 1.0) and not po.poisson_spike(0, 0, 0, 0.0)
 
 
 def _splitmix(z):
@@ -555,6 +607,59 @@ def test_kuramoto_uncoupled_is_linear():
     want = th.astype(np.float64) + om.astype(np.float64) * (1000 * float(np.float32(0.01)))
     diff = np.angle(np.exp(1j * (k.phases() - want)))
     assert np.max(np.abs(diff)) < 1e-9
+
+
+def test_kuramoto_asymmetric_pair_locks_at_closed_form():
+    # Two oscillators, W_01 != W_10 (P/src/kernels/phase/sparse.rs:63-96:
+    # c_i = sum over row i of W_ij sin(theta_j - theta_i)).  With phi =
+    # theta_1 - theta_0: dphi/dt = (w1 - w0) - (W_01 + W_10) sin(phi), so the
+    # pair locks at sin(phi*) = (w1 - w0) / (W_01 + W_10) and turns together
+    # at Omega = (W_10 w0 + W_01 w1) / (W_01 + W_10).  Ignoring the weights,
+    # swapping them (row vs column), an extra 1/N or a sign error each move
+    # phi* or Omega.
+    w01, w10 = 0.9, 0.3
+    om = np.array([0.2, 0.7], np.float32)
+    rp = np.array([0, 1, 2], np.uint32)
+    col = np.array([1, 0], np.uint32)
+    w = np.array([w01, w10], np.float32)
+    dt = 0.01
+    k = po.OracleKuramoto(rp, col, w, om, np.array([0.0, 0.0], np.float32), dt)
+    k.rk4(4000)  # 40 time units: relaxation rate sqrt(1.2^2 - 0.5^2) ~ 1.09
+    p0 = k.phases()
+    k.rk4(1000)
+    p1 = k.phases()
+    w01f, w10f = float(np.float32(w01)), float(np.float32(w10))
+    om0, om1 = float(om[0]), float(om[1])
+    phi = math.atan2(math.sin(p1[1] - p1[0]), math.cos(p1[1] - p1[0]))
+    assert abs(math.sin(phi) - (om1 - om0) / (w01f + w10f)) < 1e-9
+    assert abs(math.cos(phi) - math.sqrt(1 - ((om1 - om0) / (w01f + w10f)) ** 2)) < 1e-9  # the stable root
+    omega = (w10f * om0 + w01f * om1) / (w01f + w10f)
+    adv = np.angle(np.exp(1j * (p1 - p0)))
+    want = (omega * 1000 * float(np.float32(dt)) + math.pi) % (2 * math.pi) - math.pi
+    assert np.all(np.abs(np.angle(np.exp(1j * (adv - want)))) < 1e-9)
+
+
+def test_kuramoto_euler_step_brute_force_weighted_graph():
+    # One forward-Euler step on a 4-node graph with distinct, signed weights
+    # and a multi-edge, from the definition theta_i + dt (w_i + sum_e W_e
+    # sin(theta_col[e] - theta_i)), wrapped to [-pi, pi) (kuramoto.rs:42-68,
+    # P/src/kernels/mod.rs:74-77).
+    rp = np.array([0, 2, 3, 6, 7], np.uint32)
+    col = np.array([1, 3, 2, 0, 1, 1, 0], np.uint32)
+    w = np.array([0.5, -1.25, 2.0, 0.75, 0.1, 0.3, -0.6], np.float32)
+    om = np.array([1.0, -0.5, 0.25, 3.0], np.float32)
+    th = np.array([0.3, -2.9, 1.7, 3.1], np.float32)
+    dt = 0.25
+    k = po.OracleKuramoto(rp, col, w, om, th, dt)
+    k.euler(1)
+    want = []
+    for i in range(4):
+        c = sum(float(w[e]) * math.sin(float(th[col[e]]) - float(th[i])) for e in range(rp[i], rp[i + 1]))
+        x = float(th[i]) + float(np.float32(dt)) * (float(om[i]) + c)
+        want.append(x - 2 * math.pi * round(x / (2 * math.pi)))
+    assert np.max(np.abs(k.phases() - np.array(want))) < 1e-12
+    # node 3's step crosses +pi and wraps
+    assert k.phases()[3] < 0 < float(th[3])
 
 
 def test_kuramoto_rk4_fourth_order():
