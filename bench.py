@@ -1,21 +1,28 @@
 #!/usr/bin/env python3
 """Benchmark of the SiMuse network step on one MI355X.
 
-Workload: the Izhikevich network of 1,000,000 neurons (800k regular spiking,
-200k fast spiking) with 100 random outgoing synapses each (100M synapses,
-CSR, int32 indices), Gaussian thalamic input (sd 5 RS / 2 FS), dt = 1 ms as
-two 0.5 ms half steps.  A "step" is one full network step: synaptic input,
-neuron update, spike compaction, CSR push of every spike's events.  Inputs
-are resident in HBM before the timed region.
+Headline workload: the Izhikevich network of 1,000,000 neurons (800k regular
+spiking, 200k fast spiking) with 100 random outgoing synapses each (100M
+synapses, CSR, int32 indices), Gaussian thalamic input (sd 5 RS / 2 FS),
+dt = 1 ms as two 0.5 ms half steps.  A "step" is one full network step:
+synaptic input, neuron update, spike compaction, CSR push of every spike's
+events.  Inputs are resident in HBM before the timed region, and the step
+graph is captured and instantiated before it too.
 
-The whole step is one kernel, megastep_kernel<Izhikevich> (every delay is
-0, so the network runs the binned scatter in its one-kernel form): each
-block gathers the previous step's events for its 2048 neurons, advances
-them and writes its own spikes' events.  Its algorithmic bytes per launch
-are the neuron-step bytes (v, u, ring slot, varying parameter columns) plus,
-per spiking neuron, its row bounds (8 B) and, per synaptic event, target,
-delay, weight and the ring cell's read and write (17 B) -- what the
-sequential CPU push moves.
+Every delay is 0, so the network steps with one kernel per step,
+megastep_kernel<Izhikevich> (the engine reports its step form): each block
+gathers the previous step's events for its 2048 neurons, advances them and
+writes its own spikes' events.  The roofline prices one launch at its
+algorithmic bytes: the neuron-step bytes (v, u, ring-slot equivalent,
+varying parameter columns) plus, per spiking neuron, its row bounds (8 B)
+and, per synaptic event, target, delay, weight and the ring cell's read and
+write (17 B) -- what the sequential CPU push moves.
+
+Further legs: the Brunel LIF configuration on the GPU and on the CPU, the
+batching configuration (512 networks, batched vs plain), the serving
+configuration (Poisson arrivals, p50/p99 latency) and the Hodgkin-Huxley
+precision configuration.  Each leg reports the FP32 result's error against
+the oracle's FP64 twin (the checker) beside its throughput.
 
 Prints one JSON line.  `value` is neuron-steps/s of the whole job;
 synaptic events/s is reported beside it.
@@ -36,20 +43,21 @@ import numpy as np  # noqa: E402
 
 METRIC = "synaptic events/s + neuron-steps/s (1 GPU); batched vs plain; p99 req latency"
 HBM_PEAK_GBS = 8000.0
+MEGA_THREADS, MEGA_NEURONS = 1024, 2048  # megastep block shape: threads, neurons (one bin)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def one_kernel_step(spec) -> bool:
-    """The engine steps a network with one kernel launch per step when every
-    delay is at most one step, all populations share one model, there is no
-    plasticity or receptor synapse, and its bins (2048 neurons each) number at
-    most 2048."""
-    import numpy as np
-    return (int(np.max(spec.delay, initial=0)) <= 1 and len({p.model for p in spec.pops}) == 1
-            and not spec.stdp and not getattr(spec, "receptors", None) and spec.n_total <= 2048 * 2048)
+def host_threads() -> int:
+    """Host threads of this process's CPU share: the affinity mask, capped by
+    OMP_NUM_THREADS when the environment sets it."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
 
 
 def izh_algorithmic_bytes_per_neuron(spec) -> float:
@@ -69,16 +77,253 @@ def izh_algorithmic_bytes_per_neuron(spec) -> float:
     return total / sum(p.n for p in spec.pops)
 
 
-def cpu_baseline(spec, steps: int, threads: int):
+def form_name(form: int) -> str:
+    import simuse
+    return {simuse.FORM_ONE_KERNEL: "one_kernel", simuse.FORM_BINNED: "binned_two_kernel",
+            simuse.FORM_CHAINED: "chained"}[form]
+
+
+# -- FP64 checks (the oracle's FP64 twin is the reference of every error) -------
+def fp64_window(net, spec, steps, threads) -> dict:
+    """Start the oracle's FP64 twin from the device's own snapshot (every FP32
+    value is exact in FP64), advance both `steps` steps and compare membrane
+    potentials and spike counts: the error of that many FP32 steps from a
+    common start."""
+    import pyoracle as po
+    snap = os.path.join(tempfile.mkdtemp(), "fp64_window.taas")
+    net.save_state(snap)
+    o64 = po.OracleNetwork(spec, 64, threads=threads)
+    o64.set_state_from_snapshot(snap)
+    os.remove(snap)
+    s_dev0, s_ora0 = net.counters()[0], o64.counters()[0]
+    out = {"steps": steps, "reference": "oracle FP64 twin started from the device snapshot"}
+    done = 0
+    for mark in sorted({min(10, steps), steps}):
+        net.run(mark - done)
+        o64.step(mark - done)
+        done = mark
+        dv = np.concatenate([np.abs(net.download_states(p)[0, : pp.n].astype(np.float64) - o64.state(p, 0))
+                             for p, pp in enumerate(spec.pops)])
+        out[f"v_mean_abs_err_mV_{mark}"] = float(dv.mean())
+        out[f"v_max_abs_err_mV_{mark}"] = float(dv.max())
+        out[f"frac_over_1mV_{mark}"] = float(np.mean(dv > 1.0))
+    s_dev, s_ora = net.counters()[0] - s_dev0, o64.counters()[0] - s_ora0
+    out["spikes_fp32"], out["spikes_fp64"] = int(s_dev), int(s_ora)
+    out["spike_count_rel_err"] = abs(s_dev - s_ora) / max(s_ora, 1)
+    o64.close()
+    return out
+
+
+def cpu_baseline(spec, steps: int, threads: int, warmup: int = 10):
+    """The oracle's FP32 path (the reference's Network::step restated: threaded
+    population loop and push_spikes_par) on the host; returns neuron-steps/s,
+    synaptic events/s and the wall time of the timed steps."""
     import pyoracle as po
     o = po.OracleNetwork(spec, 32, threads=threads)
-    o.step(3)  # warm-up
+    o.step(warmup)
+    e0 = o.counters()[1]
     t0 = time.perf_counter()
     o.step(steps)
     dt = time.perf_counter() - t0
-    return spec.n_total * steps / dt, dt
+    ev = o.counters()[1] - e0
+    o.close()
+    return spec.n_total * steps / dt, ev / dt, dt
 
 
+# -- C2: the headline ---------------------------------------------------------------
+def bench_izhikevich(args) -> dict:
+    import simuse
+    from simuse.workloads import build_device, izhikevich_network
+
+    log(f"building Izhikevich network N={args.neurons} k={args.synapses_per_neuron}")
+    t0 = time.perf_counter()
+    spec = izhikevich_network(n=args.neurons, k=args.synapses_per_neuron, seed=1,
+                              steps=args.steps + args.warmup)
+    log(f"  generated in {time.perf_counter() - t0:.1f}s")
+    net = build_device(spec, graph_steps=args.graph_steps)
+    form = net.step_form()
+    log(f"  device bytes {net.device_bytes() / 2**30:.2f} GiB, step form {form_name(form)}")
+    tuned = None
+    if form != simuse.FORM_ONE_KERNEL:
+        # tune the two-kernel forms' schedule on the steady state, not on the transient
+        net.run(max(1, args.warmup // 2))
+        tuned = {"scatter": "tile" if net.autotune(32) == simuse.SCATTER_TILE else "wave",
+                 "block_tiles": net.block_tiles()}
+    net.run(args.warmup)
+    # one untimed replay of the step graph, whatever --warmup is: the graph is
+    # captured, instantiated and launched once before the timed region
+    net.run(args.graph_steps)
+    s0, e0 = net.counters()
+    net.sync()
+    # The timed region: the K steps between two HIP events on the network's
+    # stream, bracketed by synchronisations.
+    t0 = time.perf_counter()
+    region_ms = net.time_steps(args.steps)
+    net.sync()
+    wall = time.perf_counter() - t0
+    s1, e1 = net.counters()
+    spikes, events = s1 - s0, e1 - e0
+    ms_per_step = wall * 1e3 / args.steps
+    log(f"  {ms_per_step * 1e3:.2f} us/step (events {region_ms * 1e3 / args.steps:.2f} us), "
+        f"{spikes / args.steps:.0f} spikes/step, {events / args.steps:.0f} events/step")
+
+    # per-phase HIP events around each kernel (no graph) over profile_steps more
+    net.set_profiling(True)
+    ps0, pe0 = net.counters()
+    net.run(args.profile_steps)
+    k_ms = net.kernel_times()
+    ps1, pe1 = net.counters()
+    net.set_profiling(False)
+    step_ms, stdp_ms, insert_ms, fixup_ms = k_ms
+    bpn = izh_algorithmic_bytes_per_neuron(spec)
+    neuron_bytes = bpn * spec.n_total
+    ev_step = events / args.steps
+    act_step = spikes / args.steps
+    kernel_bytes = neuron_bytes + ev_step * 17 + act_step * 8  # the step kernel also pushes its spikes' events
+    # the step kernel's average launch duration over the timed region: one
+    # launch per step inside the replayed graphs, so the region's event time
+    # per step bounds it from above (gaps between graph nodes included)
+    one_kernel = form == simuse.FORM_ONE_KERNEL
+    kernel_ms = region_ms / args.steps if one_kernel else step_ms
+    achieved = kernel_bytes / (kernel_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "traffic_izhikevich_1m.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get("kernel_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+
+    fp64 = None
+    if not args.no_fp64:
+        log(f"  FP64 window: {args.fp64_steps} steps from the device state")
+        fp64 = fp64_window(net, spec, args.fp64_steps, host_threads())
+        log(f"    |dv| after {args.fp64_steps} steps: mean {fp64[f'v_mean_abs_err_mV_{args.fp64_steps}']:.2e} mV, "
+            f">1 mV {fp64[f'frac_over_1mV_{args.fp64_steps}']:.2%}, spikes {fp64['spikes_fp32']} vs "
+            f"{fp64['spikes_fp64']}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = host_threads()
+        log(f"cpu baseline: oracle FP32 path, {args.cpu_steps} steps on {threads} threads, "
+            f"{max(4, args.cpu_steps // 8)} on 1")
+        v_all, ev_all, t_all = cpu_baseline(spec, args.cpu_steps, threads)
+        v_one, ev_one, t_one = cpu_baseline(spec, max(4, args.cpu_steps // 8), 1)
+        cpu = {"value": v_all, "unit": "neuron-steps/s", "cores": threads, "kind": "port",
+               "sample": f"same 1M/100M network from its initial state, {args.cpu_steps} steps after 10 warm-up "
+                         f"steps ({t_all:.1f}s) on {threads} threads (host CPU share; nproc "
+                         f"{os.cpu_count()}): threaded population loop, threaded noise and push_spikes_par; "
+                         f"1 thread: {v_one:.3e} neuron-steps/s over {max(4, args.cpu_steps // 8)} steps "
+                         f"({t_one:.1f}s)",
+               "synaptic_events_per_s": ev_all, "one_core_value": v_one, "one_core_synaptic_events_per_s": ev_one}
+        log(f"  {v_all:.3e} neuron-steps/s on {threads} threads, {v_one:.3e} on 1")
+    shape = ({"kernel": "megastep_kernel<Izhikevich>", "threads_per_block": MEGA_THREADS,
+              "neurons_per_block": MEGA_NEURONS, "blocks": -(-spec.n_total // MEGA_NEURONS)} if one_kernel
+             else tuned)
+    out = {
+        "metric": METRIC,
+        "value": spec.n_total * args.steps / wall,
+        "unit": "neuron-steps/s",
+        "synaptic_events_per_s": events / wall,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Izhikevich 2003 network, random-init parameters and weights)",
+        "config": {"workload": "izhikevich_1M_100M", "neurons": spec.n_total,
+                   "synapses": spec.n_syn, "dt_ms": 1.0, "graph_steps": args.graph_steps,
+                   "step_form": form_name(form), "shape": shape,
+                   "untimed_steps_before_region": args.warmup + args.graph_steps,
+                   "spikes_per_step": spikes / args.steps, "events_per_step": events / args.steps},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "megastep_kernel<Izhikevich> (input gather, update, compaction, push)",
+                     "algorithmic_bytes": kernel_bytes,
+                     "algorithmic_bytes_per_neuron": bpn, "algorithmic_bytes_per_event": 17,
+                     "algorithmic_bytes_per_spike": 8,
+                     "kernel_ms": kernel_ms,
+                     "kernel_ms_source": ("HIP events around the timed region of graph replays / steps "
+                                          "(one kernel per step)" if one_kernel
+                                          else "HIP events around the step kernels")},
+        # HIP-event phases without graphs: step kernels (megastep, or neuron
+        # update with its fused push), STDP, stand-alone push writers, apply /
+        # fix-up; an empty phase reads the event gap
+        "kernels_ms_per_step": {"step_kernel": step_ms, "stdp": stdp_ms, "push_writers": insert_ms,
+                                "apply_fixup": fixup_ms, "steps": args.profile_steps,
+                                "events_per_step": (pe1 - pe0) / args.profile_steps,
+                                "spikes_per_step": (ps1 - ps0) / args.profile_steps},
+        "step_frac_of_peak": kernel_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "fp64_error": fp64,
+        "cpu_baseline": cpu,
+    }
+    net.close()
+    return out
+
+
+# -- C1: Brunel LIF on the GPU and on the CPU ------------------------------------------
+def bench_brunel(args) -> dict:
+    """The Brunel LIF configuration (10k neurons, p = 0.01, delays 1..15,
+    2 kHz Poisson per neuron, dt 0.1 ms) over its full 10,000 steps on the
+    device and on the host (1 thread and the CPU share), with the FP64 twin's
+    firing statistics beside the device's."""
+    import pyoracle as po
+    from simuse import stats
+    from simuse.workloads import brunel_lif_network, build_device
+    T = args.c1_steps
+    spec = brunel_lif_network(n=10_000, p=0.01, seed=1, steps=T, rate_hz=2000.0)
+    log(f"brunel: LIF {spec.n_total} neurons, {spec.n_syn} synapses, {len(spec.sources)} Poisson sources, {T} steps")
+    net = build_device(spec, graph_steps=args.graph_steps)
+    form = net.step_form()
+    net.run(args.graph_steps)  # graph built and replayed once; counters below are differences
+    s0, e0 = net.counters()
+    net.sync()
+    t0 = time.perf_counter()
+    region_ms = net.time_steps(T)
+    net.sync()
+    wall = time.perf_counter() - t0
+    s1, e1 = net.counters()
+    net.close()
+    gpu = {"steps": T, "wall_s": wall, "us_per_step": wall * 1e6 / T, "region_us_per_step": region_ms * 1e3 / T,
+           "neuron_steps_per_s": spec.n_total * T / wall, "synaptic_events_per_s": (e1 - e0) / wall,
+           "spikes_per_step": (s1 - s0) / T, "events_per_step": (e1 - e0) / T, "step_form": form_name(form)}
+    log(f"  GPU {gpu['us_per_step']:.2f} us/step, {gpu['spikes_per_step']:.2f} spikes/step, "
+        f"{gpu['events_per_step']:.0f} events/step")
+    threads = host_threads()
+    v1, ev1, t1 = cpu_baseline(spec, T, 1)
+    va, eva, ta = cpu_baseline(spec, T, threads)
+    cpu = {"one_core": {"neuron_steps_per_s": v1, "synaptic_events_per_s": ev1, "wall_s": t1},
+           "all_cores": {"threads": threads, "neuron_steps_per_s": va, "synaptic_events_per_s": eva, "wall_s": ta},
+           "kind": "port", "nproc": os.cpu_count()}
+    log(f"  CPU {v1:.3e} neuron-steps/s on 1 thread, {va:.3e} on {threads}")
+    fp64 = None
+    if not args.no_fp64:
+        # statistics of the device run vs the FP64 twin over the whole horizon
+        dev = build_device(spec, graph_steps=args.graph_steps)
+        dev.record(T)
+        dev.run(T)
+        rd = dev.download_raster()
+        dev.close()
+        o64 = po.OracleNetwork(spec, 64, threads=threads)
+        o64.record(True)
+        o64.step(T)
+        ro = o64.raster().reshape(T, -1)
+        o64.close()
+        sd, so = stats.summary(rd, spec.dt * 1e-3), stats.summary(ro, spec.dt * 1e-3)
+        fp64 = {"reference": "oracle FP64 twin, same network and inputs, whole horizon", "fp32": sd, "fp64": so,
+                **stats.rel_errors(sd, so)}
+        log(f"  rate {sd['rate_hz']:.4f} vs FP64 {so['rate_hz']:.4f} Hz")
+    return {"workload": "Brunel LIF 10k neurons (80/20), p=0.01, J_E +0.1 mV / J_I -0.5 mV, delays 1-15 steps, "
+                        f"2 kHz Poisson per neuron, dt 0.1 ms, {T} steps",
+            "synapses": spec.n_syn, "gpu": gpu, "cpu": cpu, "fp64_error": fp64,
+            "note": "under the stated 2 kHz drive (about 4 mV mean depolarisation against a 15 mV gap to "
+                    "threshold) the network is nearly silent: the step is launch- and latency-bound"}
+
+
+# -- C4: batching -------------------------------------------------------------------------
 def bench_batching(args) -> dict:
     """Batching configuration: 512 generated networks (log-uniform 1k-200k
     neurons, LIF/Izhikevich, 100 synapses each, Poisson drive U(1,10) kHz)
@@ -86,9 +331,11 @@ def bench_batching(args) -> dict:
     per-network runs (one network per engine, same graphs) on a random sample.
     networks/s = networks simulated for the full horizon per second of stepping
     (network construction excluded on both sides)."""
+    import pyoracle as po
     import simuse
+    from simuse import stats
     from simuse.batch import Batch
-    from simuse.workloads import batching_mix
+    from simuse.workloads import batching_mix, spec_from_generated
     descs = batching_mix(args.batch_networks, seed=1)
     T = args.batch_steps
     log(f"batching: {len(descs)} networks, sum N {sum(d.n for d in descs)}, {T} steps")
@@ -121,7 +368,7 @@ def bench_batching(args) -> dict:
     for i in sample:
         d = descs[int(i)]
         pb = Batch()
-        nid = pb.add_generated(d)
+        pb.add_generated(d)
         pb.set_graph_steps(16)
         pb.run(17)
         t0 = time.perf_counter()
@@ -134,12 +381,49 @@ def bench_batching(args) -> dict:
              "networks_per_s": len(sample) / plain_wall, "neuron_steps_per_s": plain_nsteps / plain_wall}
     log(f"  plain: {len(sample)} networks one at a time in {plain_wall:.3f} s, "
         f"{len(sample) / plain_wall:.1f} networks/s")
+
+    fp64 = None
+    if not args.no_fp64:
+        # a sample of the mix (two LIF, two Izhikevich among the networks of at
+        # most 20k neurons) rerun alone with recording: spike counts equal to
+        # the batched run's, statistics against the FP64 twin
+        small = [i for i, d in enumerate(descs) if d.n <= 20_000]
+        pick = [i for i in small if descs[i].model == simuse.MODEL_LIF][:2] + \
+               [i for i in small if descs[i].model == simuse.MODEL_IZHIKEVICH][:2]
+        rows = []
+        for i in pick:
+            d = descs[i]
+            ab = Batch()
+            nid = ab.add_generated(d)
+            spec = spec_from_generated(ab, nid, d, T)  # the generator's initial state
+            ab.set_graph_steps(16)
+            ab.record(nid, T + 17)
+            ab.run(T + 17)
+            rd = ab.download_raster(nid)
+            ab.close()
+            # the batched run timed steps [17, 17 + T) of every network
+            alone = int(rd[17: 17 + T].sum())
+            o64 = po.OracleNetwork(spec, 64, threads=host_threads())
+            o64.record(True)
+            o64.step(T)
+            ro = o64.raster().reshape(T, -1)
+            o64.close()
+            dt_s = d.dt * 1e-3
+            sd, so = stats.summary(rd[:T], dt_s), stats.summary(ro, dt_s)
+            rows.append({"network": i, "model": "lif" if d.model == simuse.MODEL_LIF else "izhikevich", "n": d.n,
+                         "spikes_alone_equal_batched": alone == c1[i][0] - c0[i][0],
+                         "fp32": sd, "fp64": so, **stats.rel_errors(sd, so)})
+            log(f"  fp64 net {i} ({rows[-1]['model']}, n={d.n}): rate {sd['rate_hz']:.3f}/{so['rate_hz']:.3f} "
+                f"cv {sd['isi_cv']:.3f}/{so['isi_cv']:.3f} chi {sd['chi']:.4f}/{so['chi']:.4f}, alone == batched "
+                f"{rows[-1]['spikes_alone_equal_batched']}")
+        fp64 = {"reference": f"oracle FP64 twin, {T} steps from the generated state", "sample": rows}
     return {"workload": f"{len(descs)} generated networks, 1k-200k neurons log-uniform, LIF/Izhikevich 50/50, "
                         f"100 syn/neuron, Poisson drive U(1,10) kHz, {T} steps",
             "batched": batched, "plain": plain,
-            "batched_vs_plain": batched["networks_per_s"] / plain["networks_per_s"]}
+            "batched_vs_plain": batched["networks_per_s"] / plain["networks_per_s"], "fp64_error": fp64}
 
 
+# -- C5: serving ----------------------------------------------------------------------------
 def bench_serving(args) -> dict:
     """Serving configuration: Poisson arrivals (200/s) of LIF/Izhikevich/HH
     networks (5k-500k neurons) and Kuramoto graphs (10k-1M oscillators, RK4),
@@ -180,14 +464,70 @@ def bench_serving(args) -> dict:
                        "or Kuramoto 10k-1M (RK4), horizons 500-20000 steps",
            "completed": len(done), "rejected_or_failed": len(res) - len(done),
            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
-           "mean_admission_wait_s": float(wait.mean()), "networks_per_s": len(done) / span,
+           "mean_admission_wait_s": float(wait.mean()), "max_admission_wait_s": float(wait.max()),
+           "networks_per_s": len(done) / span,
            "wall_s": wall, "peak_live": int(st.peak_live), "peak_bytes": int(st.peak_bytes),
            "budget_bytes": int(args.serve_budget_gb * (1 << 30))}
     log(f"  {len(done)} done, p50 {out['p50_latency_s']:.3f} s, p99 {out['p99_latency_s']:.3f} s, "
         f"{out['networks_per_s']:.1f} networks/s, peak live {st.peak_live}")
+    if not args.no_fp64:
+        out["fp64_error"] = serving_fp64(reqs, res, args)
     return out
 
 
+def serving_fp64(reqs, res, args) -> dict:
+    """FP32 vs FP64 for a sample of served requests: the two network requests
+    with the least work (neurons x horizon) get their spike count over the
+    horizon compared with the FP64 twin's; the Kuramoto request with the least
+    work gets its order parameter R compared with the FP64 twin's at
+    min(horizon, kura_fp64_steps) steps, on the graph rebuilt alone."""
+    import pyoracle as po
+    import simuse
+    from simuse.batch import Batch
+    from simuse.kuramoto import KuramotoGraph
+    from simuse.server import DONE, REQ_KURAMOTO
+    from simuse.workloads import spec_from_generated
+    nets = [(r.net.n * r.horizon, k) for k, (_, r) in enumerate(reqs) if r.kind != REQ_KURAMOTO]
+    kuras = [(r.kura.n * r.horizon, k) for k, (_, r) in enumerate(reqs) if r.kind == REQ_KURAMOTO]
+    rows = []
+    for _, k in sorted(nets)[:2]:
+        r = reqs[k][1]
+        if res[k].status != DONE:
+            continue
+        b = Batch()
+        nid = b.add_generated(r.net)
+        spec = spec_from_generated(b, nid, r.net, int(r.horizon))
+        b.close()
+        o64 = po.OracleNetwork(spec, 64, threads=host_threads())
+        o64.step(int(r.horizon))
+        s64 = o64.counters()[0]
+        o64.close()
+        s32 = int(res[k].spikes)
+        model = {simuse.MODEL_LIF: "lif", simuse.MODEL_IZHIKEVICH: "izhikevich", simuse.MODEL_HH: "hh"}[r.net.model]
+        rows.append({"request": k, "model": model, "n": int(r.net.n), "horizon": int(r.horizon),
+                     "spikes_fp32": s32, "spikes_fp64": int(s64),
+                     "rate_rel_err": abs(s32 - s64) / max(s64, 1)})
+        log(f"  fp64 request {k} ({model}, n={r.net.n}, {r.horizon} steps): spikes {s32} vs {s64}")
+    if kuras:
+        _, k = sorted(kuras)[0]
+        r = reqs[k][1]
+        T = int(min(r.horizon, args.kura_fp64_steps))
+        g = KuramotoGraph.generated(r.kura, "rk4")
+        rp, col, w, om = g.download_graph()
+        th0 = g.phases()
+        g.run(T)
+        r32 = g.order_parameter()
+        g.close()
+        ok = po.OracleKuramoto(rp, col, w, om, th0.astype(np.float32), r.kura.dt)
+        ok.rk4(T)
+        r64 = ok.order()
+        rows.append({"request": k, "model": "kuramoto", "n": int(r.kura.n), "steps": T,
+                     "served_horizon": int(r.horizon), "R_fp32": r32, "R_fp64": r64, "R_abs_err": abs(r32 - r64)})
+        log(f"  fp64 request {k} (Kuramoto n={r.kura.n}, {T} steps): R {r32:.6f} vs {r64:.6f}")
+    return {"reference": "oracle FP64 twin, same generated networks and inputs", "sample": rows}
+
+
+# -- C3: precision ------------------------------------------------------------------------
 def bench_precision(args) -> dict:
     """Precision configuration: the Hodgkin-Huxley network with heavy-tailed
     out-degree, lognormal conductance synapses (AMPA from excitatory, GABA_A
@@ -204,7 +544,7 @@ def bench_precision(args) -> dict:
     log(f"precision: HH {n} neurons, {spec.n_syn} conductance synapses, pair STDP, {windows} windows of 1000 steps "
         "vs FP64")
     dev = build_device(spec, graph_steps=16)
-    o64 = po.OracleNetwork(spec, 64, threads=min(16, os.cpu_count() or 1))
+    o64 = po.OracleNetwork(spec, 64, threads=host_threads())
     plastic = spec.stdp["plastic"].astype(bool)
     rows = []
     snap = os.path.join(tempfile.mkdtemp(), "window.taas")
@@ -245,22 +585,23 @@ def bench_precision(args) -> dict:
     big = hh_network(n=args.prec_full_neurons, seed=3, steps=args.prec_full_steps + 200, noise_sd=30.0,
                      receptors=True)
     net = build_device(big, graph_steps=16)
-    net.run(200)
+    form = net.step_form()
+    net.run(208)
     net.sync()
     t0 = time.perf_counter()
-    net.step_graph(args.prec_full_steps)
+    ms = net.time_steps(args.prec_full_steps)
     net.sync()
     us = (time.perf_counter() - t0) * 1e6 / args.prec_full_steps
-    spikes, events = net.counters()
     net.close()
     log(f"  full size {args.prec_full_neurons} neurons / {big.n_syn} synapses: {us:.2f} us/step")
     return {"workload": f"HH {n} neurons ({spec.n_syn} synapses), Pareto out-degree, lognormal conductances "
                         "(AMPA excitatory, GABA_A inhibitory, 0.003 mS/cm2 units), pair STDP (A+ 0.01, A- 0.012, "
                         "tau 20 ms, [0,1]) on excitatory synapses, Gaussian input sd 30, dt 0.01 ms",
-            "reference": "oracle FP64 twin, restarted from the device state at every window",
-            "windows": rows,
-            "full_size": {"neurons": args.prec_full_neurons, "synapses": big.n_syn,
-                          "steps": args.prec_full_steps, "us_per_step": us}}
+            "fp64_error": {"reference": "oracle FP64 twin, restarted from the device state at every window",
+                           "windows": rows},
+            "full_size": {"neurons": args.prec_full_neurons, "synapses": big.n_syn, "step_form": form_name(form),
+                          "steps": args.prec_full_steps, "us_per_step": us,
+                          "region_us_per_step": ms * 1e3 / args.prec_full_steps}}
 
 
 def copy_calibration(mib: int = 1024) -> int:
@@ -293,10 +634,14 @@ def main() -> int:
     ap.add_argument("--synapses-per-neuron", type=int, default=100)
     ap.add_argument("--graph-steps", type=int, default=16)
     ap.add_argument("--profile-steps", type=int, default=100,
-                    help="steps timed per kernel with HIP events for the roofline (at least 1)")
-    ap.add_argument("--cpu-steps", type=int, default=1000)
+                    help="steps timed per kernel with HIP events for the phase breakdown (at least 1)")
+    ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--scatter", choices=["auto", "tile", "wave"], default="auto")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-twin error checks")
+    ap.add_argument("--fp64-steps", type=int, default=100)
+    ap.add_argument("--only-headline", action="store_true", help="run the Izhikevich leg only")
+    ap.add_argument("--no-brunel", action="store_true", help="skip the Brunel LIF leg")
+    ap.add_argument("--c1-steps", type=int, default=10_000)
     ap.add_argument("--no-batching", action="store_true", help="skip the batched-vs-plain leg")
     ap.add_argument("--batch-networks", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=2000)
@@ -306,6 +651,7 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-gb", type=float, default=160.0)
     ap.add_argument("--serve-timeout", type=float, default=300.0)
+    ap.add_argument("--kura-fp64-steps", type=int, default=2000)
     ap.add_argument("--no-precision", action="store_true", help="skip the HH precision leg")
     ap.add_argument("--prec-neurons", type=int, default=10_000)
     ap.add_argument("--prec-windows", type=int, default=5)
@@ -313,7 +659,7 @@ def main() -> int:
     ap.add_argument("--prec-full-steps", type=int, default=2000)
     args = ap.parse_args()
     if args.profile_steps < 1:
-        ap.error("--profile-steps must be at least 1 (the roofline needs the step kernel's time)")
+        ap.error("--profile-steps must be at least 1")
     if args.copy_calibration:
         return copy_calibration()
 
@@ -323,131 +669,19 @@ def main() -> int:
         return 2
 
     import simuse
-    from simuse.workloads import build_device, izhikevich_network
-
     if simuse.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         return 1
-
-    log(f"building Izhikevich network N={args.neurons} k={args.synapses_per_neuron}")
-    t0 = time.perf_counter()
-    spec = izhikevich_network(n=args.neurons, k=args.synapses_per_neuron, seed=1,
-                              steps=args.steps + args.warmup)
-    log(f"  generated in {time.perf_counter() - t0:.1f}s")
-    net = build_device(spec, graph_steps=args.graph_steps)
-    log(f"  device bytes {net.device_bytes() / 2**30:.2f} GiB")
-    if args.scatter == "auto":
-        # tune on the network's steady state, not on the initial transient
-        net.run(max(1, args.warmup // 2))
-        mode = net.autotune(32)
-    else:
-        net.set_scatter_mode(simuse.SCATTER_TILE if args.scatter == "tile" else simuse.SCATTER_WAVE)
-        mode = net.scatter_mode()
-    log(f"  scatter schedule {'tile' if mode == simuse.SCATTER_TILE else 'wave'}, "
-        f"{net.block_tiles()} tiles per block")
-
-    net.run(args.warmup)
-    s0, e0 = net.counters()
-    net.sync()
-    # The timed region: the K steps between two HIP events on the network's
-    # stream (one megastep launch per step), bracketed by synchronisations.
-    t0 = time.perf_counter()
-    region_ms = net.time_steps(args.steps)
-    net.sync()
-    wall = time.perf_counter() - t0
-    s1, e1 = net.counters()
-    spikes = s1 - s0
-    events = e1 - e0
-    ms_per_step = wall * 1e3 / args.steps
-    neuron_steps_per_s = spec.n_total * args.steps / wall
-    events_per_s = events / wall
-    log(f"  {ms_per_step * 1e3:.2f} us/step, {spikes / args.steps:.0f} spikes/step, "
-        f"{events / args.steps:.0f} events/step")
-
-    # Live per-kernel durations: HIP events around each kernel on the
-    # network's stream, over profile_steps further steps.
-    net.set_profiling(True)
-    ps0, pe0 = net.counters()
-    net.run(args.profile_steps)
-    k_ms = net.kernel_times()
-    ps1, pe1 = net.counters()
-    net.set_profiling(False)
-    neuron_ms, stdp_ms, insert_ms, fixup_ms = k_ms
-    bpn = izh_algorithmic_bytes_per_neuron(spec)
-    neuron_bytes = bpn * spec.n_total
-    ev_step = (pe1 - pe0) / max(1, args.profile_steps)
-    act_step = (ps1 - ps0) / max(1, args.profile_steps)
-    scatter_bytes = ev_step * 17 + act_step * 8
-    kernel_bytes = neuron_bytes + scatter_bytes  # the step kernel also pushes its spikes' events
-    # the step kernel's average launch duration over the timed region: the
-    # network runs one kernel per step, so the region's event time per step
-    # (launch gaps included) bounds it from above
-    kernel_ms = region_ms / args.steps if one_kernel_step(spec) else neuron_ms
-    achieved = kernel_bytes / (kernel_ms * 1e-3) / 1e9
-    step_bytes = kernel_bytes
-    traffic = None
-    tpath = os.path.join(REPO, "profiles", "traffic_izhikevich_1m.json")
-    if os.path.exists(tpath):
-        try:
-            traffic = json.load(open(tpath)).get("kernel_bytes_per_launch")
-        except Exception:
-            traffic = None
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: oracle FP32 path, {args.cpu_steps} steps on {threads} threads")
-        v_all, t_all = cpu_baseline(spec, args.cpu_steps, threads)
-        v_one, t_one = cpu_baseline(spec, max(4, args.cpu_steps // 8), 1)
-        cpu = {"value": v_all, "unit": "neuron-steps/s", "cores": threads, "kind": "port",
-               "sample": f"same 1M/100M network, {args.cpu_steps} steps after 3 warm-up steps "
-                         f"({t_all:.1f}s); 1 thread: {v_one:.3e} neuron-steps/s over "
-                         f"{max(4, args.cpu_steps // 8)} steps ({t_one:.1f}s)",
-               "one_core_value": v_one}
-
-    out = {
-        "metric": METRIC,
-        "value": neuron_steps_per_s,
-        "unit": "neuron-steps/s",
-        "synaptic_events_per_s": events_per_s,
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded Izhikevich 2003 network, random-init parameters and weights)",
-        "config": {"workload": "izhikevich_1M_100M", "neurons": spec.n_total,
-                   "synapses": spec.n_syn, "dt_ms": 1.0, "graph_steps": args.graph_steps,
-                   "scatter": "tile" if mode == simuse.SCATTER_TILE else "wave",
-                   "block_tiles": net.block_tiles(),
-                   "spikes_per_step": spikes / args.steps, "events_per_step": events / args.steps},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "megastep_kernel<Izhikevich> (input gather, update, compaction, push)",
-                     "algorithmic_bytes": kernel_bytes,
-                     "algorithmic_bytes_per_neuron": bpn, "algorithmic_bytes_per_event": 17,
-                     "algorithmic_bytes_per_spike": 8,
-                     "kernel_ms": kernel_ms,
-                     "kernel_ms_source": ("HIP events over the timed region / steps (one kernel per step)"
-                                          if one_kernel_step(spec) else "HIP events around the step kernel")},
-        # HIP-event phases: step kernels (megastep, or neuron update with its
-        # fused push), STDP, stand-alone push writers, apply / fix-up; an empty
-        # phase reads the ~4.5 us event gap
-        "kernels_ms_per_step": {"step_kernel": neuron_ms, "stdp": stdp_ms, "push_writers": insert_ms,
-                                "apply_fixup": fixup_ms},
-        "step_frac_of_peak": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
-        "cpu_baseline": cpu,
-    }
-    net.close()
-    if not args.no_batching:
-        out["batching"] = bench_batching(args)
-    if not args.no_serving:
-        out["serving"] = bench_serving(args)
-    if not args.no_precision:
-        out["precision"] = bench_precision(args)
+    out = bench_izhikevich(args)
+    if not args.only_headline:
+        if not args.no_brunel:
+            out["brunel"] = bench_brunel(args)
+        if not args.no_batching:
+            out["batching"] = bench_batching(args)
+        if not args.no_serving:
+            out["serving"] = bench_serving(args)
+        if not args.no_precision:
+            out["precision"] = bench_precision(args)
     print(json.dumps(out), flush=True)
     return 0
 

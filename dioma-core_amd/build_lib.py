@@ -42,15 +42,37 @@ def hipcc() -> str:
 
 
 def build_library(verbose: bool = False) -> str:
+    """Compile every translation unit in parallel (one hipcc each), then link."""
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    odir = os.path.join(HERE, "_build")
+    os.makedirs(odir, exist_ok=True)
+    cflags = [f for f in HIPCC_FLAGS if f not in ("-shared", "-Wl,--no-undefined")]
+    jobs = []
+    for src in srcs:
+        obj = os.path.join(odir, os.path.basename(src) + ".o")
+        cmd = [hipcc(), *cflags, "-c", "-o", obj, src]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        jobs.append((obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+    failed = False
+    for obj, p in jobs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            failed = True
+        elif verbose and out.strip():
+            sys.stderr.write(out)
+    if failed:
+        raise RuntimeError("hipcc failed building libsimuse.so")
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, *srcs]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-fPIC", "-shared", "-Wl,--no-undefined", "-o", tmp,
+           *[o for o, _ in jobs]]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libsimuse.so")
+        raise RuntimeError("hipcc failed linking libsimuse.so")
     os.replace(tmp, LIB)
     return LIB
 

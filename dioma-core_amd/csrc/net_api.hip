@@ -1310,6 +1310,9 @@ int simuse_net_time_steps(simuse_net* net, uint32_t n, float* ms) {
         int r = net->eng.prepare();
         if (r) return r;
         if (net->eng.profiling) return fail(SIMUSE_E_STATE, "time_steps: turn profiling off first");
+        // the graph's capture and instantiation are host work: done before the
+        // first event, so the region holds only replays (and the rest steps)
+        if (n >= net->eng.graph_steps && !net->eng.gexec && (r = net->eng.build_graph())) return r;
         hipEvent_t e[2] = {nullptr, nullptr};
         HIP_TRY(hipEventCreate(&e[0]));
         if (hipEventCreate(&e[1]) != hipSuccess) {
@@ -1325,6 +1328,19 @@ int simuse_net_time_steps(simuse_net* net, uint32_t n, float* ms) {
         (void)hipEventDestroy(e[1]);
         if (r) return r;
         if (he != hipSuccess) return fail(SIMUSE_E_HIP, "time_steps: %s", hipGetErrorString(he));
+        return SIMUSE_OK;
+    });
+}
+
+int simuse_net_step_form(simuse_net* net, int32_t* form) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    if (!form) return fail(SIMUSE_E_INVALID, "null output");
+    return guarded([&]() {
+        int r = net->eng.prepare();
+        if (r) return r;
+        const NetState& s = net->st;
+        *form = s.mega ? SIMUSE_FORM_ONE_KERNEL : s.binned ? SIMUSE_FORM_BINNED : SIMUSE_FORM_CHAINED;
         return SIMUSE_OK;
     });
 }

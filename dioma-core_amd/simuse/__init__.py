@@ -4,8 +4,8 @@ The compute path is the HIP library ``libsimuse.so`` (C ABI in
 include/simuse.h); this package is its host-side mirror of the reference's
 ``pillars_cuda::CompiledNetwork`` API plus workload builders.
 """
-from ._lib import (MODEL_GPEA, MODEL_HH, MODEL_IZHIKEVICH, MODEL_LIF, SCATTER_AUTO, SCATTER_TILE, SCATTER_WAVE,
-                   SimuseError, device_count, lib)
+from ._lib import (FORM_BINNED, FORM_CHAINED, FORM_ONE_KERNEL, MODEL_GPEA, MODEL_HH, MODEL_IZHIKEVICH, MODEL_LIF,
+                   SCATTER_AUTO, SCATTER_TILE, SCATTER_WAVE, SimuseError, device_count, lib)
 from .models import hh_pop, izh_pop, lif_pop, make_pop
 from .network import CompiledNetwork, NetworkPop, stride_for
 
@@ -13,4 +13,5 @@ __all__ = [
     "CompiledNetwork", "NetworkPop", "SimuseError", "device_count", "lib", "stride_for",
     "lif_pop", "izh_pop", "hh_pop", "make_pop",
     "MODEL_LIF", "MODEL_IZHIKEVICH", "MODEL_HH", "MODEL_GPEA", "SCATTER_AUTO", "SCATTER_TILE", "SCATTER_WAVE",
+    "FORM_ONE_KERNEL", "FORM_BINNED", "FORM_CHAINED",
 ]

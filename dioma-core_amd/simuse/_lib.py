@@ -26,6 +26,10 @@ MODEL_GPEA = 3
 KURA_EULER = 0
 KURA_RK4 = 1
 
+FORM_ONE_KERNEL = 0
+FORM_BINNED = 1
+FORM_CHAINED = 2
+
 SCATTER_AUTO = 0
 SCATTER_TILE = 1
 SCATTER_WAVE = 2
@@ -118,6 +122,7 @@ _SIGS = {
     "simuse_net_download_raster": ([P, u64p, C.c_uint64, u32p], C.c_int),
     "simuse_net_raster_stats": ([P, C.c_uint32, C.c_uint32, C.c_float, f32p, f32p], C.c_int),
     "simuse_net_device_bytes": ([P, u64p], C.c_int),
+    "simuse_net_step_form": ([P, i32p], C.c_int),
     "simuse_net_set_profiling": ([P, C.c_int32], C.c_int),
     "simuse_net_kernel_times": ([P, f32p, C.c_int32], C.c_int),
     "simuse_net_time_steps": ([P, C.c_uint32, f32p], C.c_int),

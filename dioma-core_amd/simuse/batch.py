@@ -267,6 +267,24 @@ class Batch:
                                                 C.byref(wps)))
         return buf[: steps.value * wps.value].reshape(steps.value, wps.value)
 
+    def download_raster(self, nid):
+        """Dense int32 raster [steps, n_total] of a recording network (the bit
+        raster holds per population ceil(n/256) tiles of four 64-bit words)."""
+        bits = self.download_raster_bits(nid)
+        T = bits.shape[0]
+        n, _, npops = self.info(nid)
+        out = np.zeros((T, n), np.int32)
+        word0 = off = 0
+        for p in range(npops):
+            _, pn = self.pop_info(nid, p)
+            nt = (pn + 255) // 256
+            wp = np.ascontiguousarray(bits[:, word0: word0 + nt * 4])
+            un = np.unpackbits(wp.view(np.uint8).reshape(T, -1), axis=1, bitorder="little")
+            out[:, off: off + pn] = un[:, :pn]
+            word0 += nt * 4
+            off += pn
+        return out
+
     def device_bytes(self) -> int:
         b = C.c_uint64()
         check(_L().simuse_batch_device_bytes(self._h, C.byref(b)))

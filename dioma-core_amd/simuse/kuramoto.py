@@ -34,6 +34,33 @@ class KuramotoGraph:
                                        _p(om, C.c_float), _p(th, C.c_float), self.n, float(dt), m, C.byref(h)))
         self._h = h
 
+    @classmethod
+    def generated(cls, desc, method: str = "rk4") -> "KuramotoGraph":
+        """A modular small-world graph built on the device from a KuraGenDesc
+        (simuse_kura_create_generated)."""
+        from .server import _S
+        L = _S()
+        self = cls.__new__(cls)
+        h = C.c_void_p()
+        m = _lib.KURA_RK4 if method == "rk4" else _lib.KURA_EULER
+        check(L.simuse_kura_create_generated(C.byref(desc), m, C.byref(h)))
+        self._h = h
+        n, ne = C.c_uint32(), C.c_uint64()
+        check(L.simuse_kura_info(h, C.byref(n), C.byref(ne)))
+        self.n, self.n_edges = int(n.value), int(ne.value)
+        return self
+
+    def download_graph(self):
+        """(row_ptr, col, w, omega) of the graph."""
+        from .server import _S
+        rp = np.zeros(self.n + 1, np.uint32)
+        col = np.zeros(max(1, self.n_edges), np.uint32)
+        w = np.zeros(max(1, self.n_edges), np.float32)
+        om = np.zeros(self.n, np.float32)
+        check(_S().simuse_kura_download_graph(self._h, _p(rp, C.c_uint32), _p(col, C.c_uint32), _p(w, C.c_float),
+                                              _p(om, C.c_float)))
+        return rp, col[: self.n_edges], w[: self.n_edges], om
+
     def close(self):
         if getattr(self, "_h", None) is not None:
             lib().simuse_kura_destroy(self._h)

@@ -302,6 +302,12 @@ class CompiledNetwork:
         check(lib().simuse_net_device_bytes(self._h, C.byref(b)))
         return int(b.value)
 
+    def step_form(self) -> int:
+        """FORM_ONE_KERNEL, FORM_BINNED or FORM_CHAINED (finalizes the network)."""
+        f = C.c_int32()
+        check(lib().simuse_net_step_form(self._h, C.byref(f)))
+        return int(f.value)
+
     # -- recording and statistics ----------------------------------------------
     def record(self, max_steps: int) -> None:
         check(lib().simuse_net_record(self._h, int(max_steps)))

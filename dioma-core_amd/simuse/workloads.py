@@ -236,6 +236,37 @@ def build_device(spec: NetSpec, graph_steps: int = 8) -> CompiledNetwork:
     return net
 
 
+def gen_noise_seed(seed: int, fs: int) -> int:
+    """Noise seed the device generator gives population `fs` (0 or 1)."""
+    return (seed * 0x9E3779B97F4A7C15 + 17 + fs) & ((1 << 64) - 1)
+
+
+def spec_from_generated(b, nid: int, g, steps: int) -> NetSpec:
+    """A device-generated network of batch `b` as a NetSpec: CSR and
+    populations downloaded, the per-neuron Poisson drive written as one
+    explicit source per neuron (bit-identical, simuse_net_set_poisson_drive),
+    the noise with the generator's seeds.  For checkers that rebuild the
+    network on the host."""
+    from . import _lib
+    rp, col, w, d = b.download_csr(nid)
+    pops = b.pops(nid)
+    spec = NetSpec("generated", pops, rp, col, w, d, g.dt, steps)
+    off = 0
+    for p, pop in enumerate(pops):
+        fs = 1 if (g.model == _lib.MODEL_IZHIKEVICH and p == 1) else 0
+        if g.noise_sigma > 0:
+            spec.noise.append((p, np.full(pop.n, g.noise_sigma, np.float32), gen_noise_seed(g.seed, fs)))
+        if g.drive_rate_hz > 0:
+            seed0 = (g.seed * 2654435761 + off) & 0xFFFFFFFF
+            post = np.array([0], np.uint32)
+            for i in range(pop.n):
+                spec.sources.append(Source(g.drive_rate_hz, g.dt, (seed0 + i * 2654435761) & 0xFFFFFFFF,
+                                           post + (off + i), np.array([g.drive_weight], np.float32),
+                                           np.array([0], np.uint8)))
+        off += pop.n
+    return spec
+
+
 def modular_small_world(n: int, modules: int = 64, k: int = 20, beta: float = 0.1, inter: float = 0.05,
                         coupling: float = 2.0, seed: int = 1):
     """Modular Watts–Strogatz graph for Kuramoto oscillators.

@@ -250,6 +250,13 @@ int simuse_net_raster_stats(simuse_net* net, uint32_t first, uint32_t count, flo
 
 /* Bytes of device memory the handle holds. */
 int simuse_net_device_bytes(simuse_net* net, uint64_t* bytes);
+/* Which kernel sequence steps the network (finalizes it if it has not stepped
+ * yet).  No counterpart in the reference, whose network always takes the
+ * same four launches (PC/src/network.rs:895-1035). */
+#define SIMUSE_FORM_ONE_KERNEL 0 /* delays <= 1, one model, no pair STDP or receptors: megastep_kernel */
+#define SIMUSE_FORM_BINNED 1     /* delays <= 1: neuron kernel / binned writers, then the per-bin apply */
+#define SIMUSE_FORM_CHAINED 2    /* some delay > 1: chained scatter and its fix-up */
+int simuse_net_step_form(simuse_net* net, int32_t* form);
 /* Mean duration (ms) of each phase of the step over the last
  * simuse_net_step_async window, measured with HIP events on the handle's
  * stream when profiling is on: [step kernels (the one-kernel step of networks

@@ -876,9 +876,15 @@ struct Net final : NetBase {
         // Gaussian current noise enters the current slot like inject_current
         // (build.rs:243-275) issued just before the step.
         for (auto& P : pops)
-            if (P.noise)
-                for (uint32_t i = 0; i < P.n; ++i)
-                    cur[P.off + i] += R(P.sigma[i] * gauss(P.seed, i, stepc));
+            if (P.noise) {
+                const int64_t pn = P.n;
+                R* c = cur + P.off;
+#pragma omp parallel for schedule(static) num_threads(n_threads) if (n_threads > 1 && pn > 4096)
+                for (int64_t ii = 0; ii < pn; ++ii) {
+                    const uint32_t i = static_cast<uint32_t>(ii);
+                    c[i] += R(P.sigma[i] * gauss(P.seed, i, stepc));
+                }
+            }
         // 1-2: read slot -> i_in, clear slot (delay.rs:68-77)
         std::memcpy(i_in.data(), cur, sizeof(R) * N);
         std::fill(cur, cur + N, R(0));
@@ -898,6 +904,9 @@ struct Net final : NetBase {
         if (dense) dense_update();
         // 5: push, pre ascending then CSR row order (push.rs:126-173); the
         //    rayon variant (push.rs:184-237) applies triples in the same order.
+        if (push_par()) {
+            n_events += push_spikes_par();
+        } else
         for (uint32_t pre : act) {
             for (uint32_t e = row_ptr[pre]; e < row_ptr[pre + 1]; ++e) {
                 if (rec_on && rec_code[e]) {
@@ -928,6 +937,48 @@ struct Net final : NetBase {
         ++stepc;  // 6
     }
     void step(uint32_t n) override { for (uint32_t k = 0; k < n; ++k) step_once(); }
+
+    // push_spikes_par, P/src/api/net/push.rs:184-237: without short-term
+    // plasticity (or conductance jumps), with more than one thread and more
+    // than 64 active neurons, the active list is cut into one chunk per thread
+    // (chunk = max(ceil(active / threads), 8)); each thread collects
+    // (post, slot, weight) triples of its chunk's rows in CSR order
+    // (collect_push_triples, :246-269), then one thread adds them into the
+    // ring in chunk order -- the sequential push's order, so the ring is
+    // bit-identical to push_spikes.
+    struct Triple {
+        uint32_t post, slot;
+        R w;
+    };
+    std::vector<std::vector<Triple>> trip;
+    bool push_par() const {
+        return n_threads > 1 && act.size() > 64 && tm_type.empty() && !rec_on;
+    }
+    uint64_t push_spikes_par() {
+        const size_t na = act.size(), nt = static_cast<size_t>(n_threads);
+        const size_t chunk = std::max<size_t>((na + nt - 1) / nt, 8);
+        if (trip.size() < nt) trip.resize(nt);
+        const uint64_t sc = stepc;
+#pragma omp parallel for schedule(static, 1) num_threads(n_threads)
+        for (int64_t tid = 0; tid < static_cast<int64_t>(nt); ++tid) {
+            std::vector<Triple>& v = trip[tid];
+            v.clear();
+            const size_t a0 = static_cast<size_t>(tid) * chunk, a1 = std::min(a0 + chunk, na);
+            for (size_t a = a0; a < a1; ++a) {
+                const uint32_t pre = act[a];
+                for (uint32_t e = row_ptr[pre]; e < row_ptr[pre + 1]; ++e) {
+                    const uint32_t dd = d[e] < 1 ? 1u : d[e];
+                    v.push_back(Triple{col[e], static_cast<uint32_t>((sc + dd) % kSlots), w[e]});
+                }
+            }
+        }
+        uint64_t events = 0;
+        for (size_t tid = 0; tid < nt; ++tid) {
+            for (const Triple& x : trip[tid]) ringv[static_cast<size_t>(x.slot) * N + x.post] += x.w;
+            events += trip[tid].size();
+        }
+        return events;
+    }
 
     // CompiledNetwork::stdp_step, PC/src/network.rs:495-718 with the kernels
     // of :53-83: trace = trace*decay + (spike of the previous step), then the

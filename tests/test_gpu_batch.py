@@ -7,38 +7,9 @@ import pyoracle as po
 from nets import random_net
 from simuse import MODEL_HH, MODEL_IZHIKEVICH, MODEL_LIF
 from simuse.batch import Batch, gen_desc
-from simuse.workloads import NetSpec, Source, build_device
+from simuse.workloads import NetSpec, Source, build_device, spec_from_generated
 
 pytestmark = pytest.mark.gpu
-
-M64 = (1 << 64) - 1
-
-
-def gen_noise_seed(seed, fs):
-    return (seed * 0x9E3779B97F4A7C15 + 17 + fs) & M64
-
-
-def spec_from_generated(b, nid, g, steps):
-    """Rebuild a device-generated network as a NetSpec for the oracle: CSR and
-    populations downloaded, the Poisson drive written as one explicit source
-    per neuron, the noise with the generator's seeds."""
-    rp, col, w, d = b.download_csr(nid)
-    pops = b.pops(nid)
-    spec = NetSpec("generated", pops, rp, col, w, d, g.dt, steps)
-    off = 0
-    for p, pop in enumerate(pops):
-        fs = 1 if (g.model == MODEL_IZHIKEVICH and p == 1) else 0
-        if g.noise_sigma > 0:
-            spec.noise.append((p, np.full(pop.n, g.noise_sigma, np.float32), gen_noise_seed(g.seed, fs)))
-        if g.drive_rate_hz > 0:
-            seed0 = (g.seed * 2654435761 + off) & 0xFFFFFFFF
-            for i in range(pop.n):
-                spec.sources.append(Source(g.drive_rate_hz, g.dt, (seed0 + i * 2654435761) & 0xFFFFFFFF,
-                                           np.array([off + i], np.uint32), np.array([g.drive_weight], np.float32),
-                                           np.array([0], np.uint8)))
-        off += pop.n
-    return spec
-
 
 @pytest.mark.parametrize("model", [MODEL_IZHIKEVICH, MODEL_LIF, MODEL_HH])
 def test_generated_network_matches_oracle(model):

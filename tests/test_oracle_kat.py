@@ -77,6 +77,35 @@ def test_push_accumulates():
     assert abs(o.ring()[1][2] - 7.0) < 1e-6
 
 
+def test_push_spikes_par_matches_sequential():
+    # push.rs:433-458: the threaded push (thread-chunked triples, in-order
+    # apply) leaves the ring byte-identical to the sequential one.  Here 300
+    # neurons fire at once (above the 64-active fallback), their rows collide
+    # on shared posts with mixed delays, and the noise is drawn threaded too.
+    rng = np.random.default_rng(5)
+    n = 400
+    pre = np.repeat(np.arange(n, dtype=np.uint32), 30)
+    post = rng.integers(0, 50, size=pre.size).astype(np.uint32)
+    w = rng.normal(size=pre.size).astype(np.float32)
+    d = rng.integers(0, 32, size=pre.size).astype(np.uint8)
+    spec = _push_net(n, pre, post, w, d)
+    spec.noise.append((0, np.full(n, 3.0, np.float32), 11))
+    runs = []
+    for threads in (1, 8):
+        o = po.OracleNetwork(spec, 32, threads=threads)
+        _force_fire(o, 0, np.arange(300))
+        o.step()
+        assert o.active().size >= 300
+        first = o.ring()
+        o.step(40)
+        runs.append((first, o.ring(), o.state(0, 0), o.counters()))
+    for k in range(3):
+        assert np.array_equal(runs[0][k].view(np.uint64), runs[1][k].view(np.uint64))
+    assert runs[0][3] == runs[1][3]
+    # the cells hold sums of several events whose order changes their bits
+    assert np.count_nonzero(runs[0][0]) > 1000
+
+
 def test_inject_same_step_arrival():
     # build.rs:243-275 — current injected before step s drives step s
     o = po.OracleNetwork(_push_net(2, [0], [1], [1.0], [0]))
