@@ -135,7 +135,8 @@ struct NetState {
     std::vector<uint4> dense_writers;  // every writer as a bin_kernel item (dense STDP step)
     uint4* d_dense_writers = nullptr;  // [bin_rows + kXcds]
     bool dense_dirty = true;
-    uint4* bin_ev = nullptr;
+    uint2* bin_ev = nullptr;   // [1 or 2][n_ev] records (two uint2 per record when ev_wide)
+    bool ev_wide = false;      // 16-byte records: receptor synapses or a writer spanning >= 2^21 edges
     uint32_t* bin_slot = nullptr;
     uint64_t since_clear = 0;  // steps enqueued since the heads were last rewritten
     uint32_t* row_ptr = nullptr;
@@ -245,6 +246,7 @@ struct NetState {
         a.bin_done = dev_ptr(bin_done);
         a.bin_mat = dev_ptr(bin_mat);
         a.bin_ev = dev_ptr(bin_ev);
+        a.ev_wide = ev_wide ? 1 : 0;
         a.bin_slot = dev_ptr(bin_slot);
         a.head = dev_ptr(head);
         a.rec = dev_ptr(rec);
@@ -518,7 +520,7 @@ struct Engine {
                 A.bin_fused = fz ? 1 : 0;
                 if (fz) bins_fused = std::max(bins_fused, s->n_bins);
                 bins_bin = std::max(bins_bin, s->n_bins);
-                for (uint32_t b = 0; b < s->n_bins; ++b) (s->rec_code ? aplr : apl).push_back(make_uint2(k, b));
+                for (uint32_t b = 0; b < s->n_bins; ++b) (s->ev_wide ? aplr : apl).push_back(make_uint2(k, b));
                 continue;
             }
             // chained scatter: plastic networks scatter after the STDP kernels;

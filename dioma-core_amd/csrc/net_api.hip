@@ -167,7 +167,21 @@ int NetState::finalize(hipStream_t stream) {
     bool one_model = true;
     for (const HostPop& P : pops) one_model = one_model && P.model == pops[0].model;
     binned = max_delay <= 1 && n_tiles > 0 && n_bins_t <= kMaxBins && pops.size() <= kMaxBinPops;
+    // First event id of every scatter tile's rows: a tile's events, and so its
+    // collisions, fit in [tile_ebase, next tile's tile_ebase).
+    std::vector<uint32_t> eb(n_blk + 1);
+    for (const HostPop& P : pops)
+        for (uint32_t t = 0; t < P.ntiles; ++t) eb[P.tile0 + t] = h_row_ptr[P.off + t * kTile];
+    for (uint32_t j = 0; j < n_src_tiles; ++j) eb[n_tiles + j] = static_cast<uint32_t>(n_syn + rp[j * kTile]);
+    eb[n_blk] = static_cast<uint32_t>(n_ev);
     if (binned) {
+        // 8-byte records need every writer's events below 2^kRecFBits: every
+        // writer the engine forms is a run of at most W consecutive scatter
+        // tiles (a bin, a group of neuron tiles, a bin_kernel writer)
+        const uint32_t W = std::max({kBinTiles, kBinBlockTiles, (n_blk + kMaxWriters - 1) / kMaxWriters});
+        uint64_t span = 0;
+        for (uint32_t b = 0; b < n_blk; ++b) span = std::max<uint64_t>(span, eb[std::min(b + W, n_blk)] - eb[b]);
+        ev_wide = !h_rec_code.empty() || span > (1ull << kRecFBits);
         n_bins = n_bins_t;
         const uint32_t ns = (n_src_tiles + kBinBlockTiles - 1) / kBinBlockTiles;
         // writers: at most one per neuron tile plus one per kBinBlockTiles source tiles
@@ -175,10 +189,10 @@ int NetState::finalize(hipStream_t stream) {
         bin_rows += bin_rows & 1u;  // even: consumers read two writers' headers at once
         // one kernel per step needs one model, no pair-based plasticity and a
         // writer per bin
-        mega_ok = one_model && !stdp && h_rec_code.empty() && n_bins + ns <= kMaxWriters;
+        mega_ok = one_model && !stdp && !ev_wide && n_bins + ns <= kMaxWriters;
         mega_model = pops[0].model;
         const size_t nbuf = mega_ok ? 2 : 1;
-        if ((rc = alloc(&bin_ev, nbuf * n_ev))) return rc;
+        if ((rc = alloc(&bin_ev, nbuf * n_ev * (ev_wide ? 2 : 1)))) return rc;
         if ((rc = alloc(&bin_hdr, nbuf * n_bins * bin_rows))) return rc;
         HIP_TRY(hipMemsetAsync(bin_hdr, 0, sizeof(uint2) * nbuf * n_bins * bin_rows, stream));
         if ((rc = alloc(&d_dense_writers, bin_rows + kXcds))) return rc;  // XCD-ordered, padded
@@ -199,15 +213,7 @@ int NetState::finalize(hipStream_t stream) {
     if ((rc = alloc(&tile_coll, n_blk))) return rc;
     if ((rc = alloc(&tile_ebase, n_blk))) return rc;
     HIP_TRY(hipMemsetAsync(tile_coll, 0, sizeof(uint32_t) * n_blk, stream));
-    // First event id of every scatter tile's rows: a tile's events, and so its
-    // collisions, fit in [tile_ebase, next tile's tile_ebase).
-    {
-        std::vector<uint32_t> eb(n_blk);
-        for (const HostPop& P : pops)
-            for (uint32_t t = 0; t < P.ntiles; ++t) eb[P.tile0 + t] = h_row_ptr[P.off + t * kTile];
-        for (uint32_t j = 0; j < n_src_tiles; ++j) eb[n_tiles + j] = static_cast<uint32_t>(n_syn + rp[j * kTile]);
-        if (n_blk) HIP_TRY(hipMemcpy(tile_ebase, eb.data(), 4 * eb.size(), hipMemcpyHostToDevice));
-    }
+    if (n_blk) HIP_TRY(hipMemcpy(tile_ebase, eb.data(), 4 * static_cast<size_t>(n_blk), hipMemcpyHostToDevice));
     n_fix = std::max<uint32_t>(1u, (n_blk + kFixTiles - 1) / kFixTiles);
     if (!h_tm_type.empty()) {
         h_tm_type.resize(n_ev, 0);  // source edges added after set_stp are static

@@ -681,15 +681,33 @@ __device__ void stdp_in_edges(dptr<const NetArgs> nap, uint32_t t0, uint32_t t1,
     }
 }
 
+// Records of buffer p (megastep networks keep two, by step parity).
+__device__ __forceinline__ dptr<uint2> ev_buf(const NetArgs& a, uint32_t p) {
+    return a.bin_ev + static_cast<size_t>(p) * a.n_ev * (a.ev_wide ? 2u : 1u);
+}
+
+// Record `pos` of a writer: event id eid (f-th of the writer's events this
+// step), weight x, (bin << 11 | cell) bc, receptor code.  kNarrow: the
+// network is known to use 8-byte records (megastep networks).
+template <bool kRec, bool kNarrow = false>
+__device__ __forceinline__ void put_record(const NetArgs& a, dptr<uint2> ev, uint32_t pos, uint32_t eid, uint32_t f,
+                                           float x, uint32_t bc, uint32_t code) {
+    if (kRec || (!kNarrow && a.ev_wide))
+        reinterpret_cast<dptr<uint4>>(ev)[pos] = make_uint4(eid, __float_as_uint(x), bc & (kBinPosts - 1u), code);
+    else
+        ev[pos] = make_uint2(__float_as_uint(x), (f << kBinLog2) | (bc & (kBinPosts - 1u)));
+}
+
 // Count (kWrite false) or write out (kWrite true) the events of `cnt` staged
 // rows (cnt <= 2048): row k starts at edge s_rs[k] and holds concatenation
 // positions [s_pref[k], s_pref[k+1]).  Counting adds 1 to s_bin[bin]; writing
-// puts the event at record s_bin[bin] of `ev` and advances it.  Four events
-// per thread are loaded before any is used.
-template <bool kWrite, uint32_t kThreads, bool kRec = false, bool kStdp = false>
+// puts the event at record s_bin[bin] of `ev` and advances it; the writer's
+// events before these rows number fb.  Four events per thread are loaded
+// before any is used.
+template <bool kWrite, uint32_t kThreads, bool kRec = false, bool kStdp = false, bool kNarrow = false>
 __device__ __forceinline__ void bin_events(const NetArgs& a, const BinMap& M, bool is_src, uint32_t cnt,
                                            uint32_t total, const uint32_t* s_rs, const uint32_t* s_pref,
-                                           uint32_t* s_bin, uint4* ev, uint64_t step) {
+                                           uint32_t* s_bin, dptr<uint2> ev, uint64_t step, uint32_t fb = 0u) {
     constexpr uint32_t kU = 4;
     const uint32_t* col = is_src ? a.src_post : a.col;
     const float* w = is_src ? a.src_w : a.w;
@@ -723,7 +741,7 @@ __device__ __forceinline__ void bin_events(const NetArgs& a, const BinMap& M, bo
                 const float x = a.tm_type ? tm_deliver(a, eb + e[u], wt[u], step) : wt[u];
                 const uint32_t code = kRec && !is_src ? static_cast<uint32_t>(a.rec_code[e[u]]) : 0u;
                 const uint32_t pos = atomicAdd(&s_bin[bc >> 11], 1u);
-                ev[pos] = make_uint4(eb + e[u], __float_as_uint(x), bc & 2047u, code);
+                put_record<kRec, kNarrow>(a, ev, pos, eb + e[u], fb + f0 + u * kThreads, x, bc, code);
             }
         }
     }
@@ -773,18 +791,19 @@ __device__ __forceinline__ void ev_cache_count(const EvCache& c, uint32_t total,
         if (threadIdx.x + u * kThreads < total) atomicAdd(&s_bin[c.bc[u] >> 11], 1u);
 }
 
-template <uint32_t kThreads, bool kRec = false>
+template <uint32_t kThreads, bool kRec = false, bool kNarrow = false>
 __device__ __forceinline__ void ev_cache_write(const NetArgs& a, bool is_src, const EvCache& c, uint32_t total,
-                                               uint32_t* s_bin, uint4* ev, uint64_t step) {
+                                               uint32_t* s_bin, dptr<uint2> ev, uint64_t step) {
     const uint32_t eb = is_src ? static_cast<uint32_t>(a.n_syn) : 0u;
 #pragma unroll
     for (uint32_t u = 0; u < kEvCache; ++u) {
-        if (threadIdx.x + u * kThreads >= total) continue;
+        const uint32_t f = threadIdx.x + u * kThreads;
+        if (f >= total) continue;
         const uint32_t e = eb + c.e[u];
         const float x = a.tm_type ? tm_deliver(a, e, c.w[u], step) : c.w[u];
         const uint32_t code = kRec && !is_src ? static_cast<uint32_t>(a.rec_code[e]) : 0u;
         const uint32_t pos = atomicAdd(&s_bin[c.bc[u] >> 11], 1u);
-        ev[pos] = make_uint4(e, __float_as_uint(x), c.bc[u] & 2047u, code);
+        put_record<kRec, kNarrow>(a, ev, pos, e, f, x, c.bc[u], code);
     }
 }
 
@@ -1180,7 +1199,7 @@ __global__ __launch_bounds__(kTile) void bin_kernel(const NetArgs* __restrict__ 
     const uint32_t base = a.tile_ebase[b0];
     // megastep networks: this step's buffer of records and headers
     const uint32_t par = a.mega ? static_cast<uint32_t>(step & 1u) : 0u;
-    uint4* ev = a.bin_ev + par * a.n_ev;
+    const dptr<uint2> ev = ev_buf(a, par);
     uint2* hdr = a.bin_hdr + par * a.hdr_stride;
     for (uint32_t k = t; k < a.n_bins; k += kTile) s_bin[k] = 0u;
     bin_map_load(a, M);
@@ -1194,6 +1213,7 @@ __global__ __launch_bounds__(kTile) void bin_kernel(const NetArgs* __restrict__ 
     bool cached = false;  // one chunk of at most kEvCache events per thread
     EvCache ec;
     for (int pass = 0; pass < 2; ++pass) {
+        uint32_t fb = 0;  // the writer's events before the current rows
         for (uint32_t r0 = 0; r0 < n_rows; r0 += kRows) {
             const uint32_t nr = min(kRows, n_rows - r0);
             uint32_t total = kept;
@@ -1240,13 +1260,15 @@ __global__ __launch_bounds__(kTile) void bin_kernel(const NetArgs* __restrict__ 
                 if (a.rec_code) ev_cache_write<kTile, true>(a, false, ec, total, s_bin, ev, step);
                 else ev_cache_write<kTile>(a, false, ec, total, s_bin, ev, step);
             } else if (a.stdp) {
-                if (a.rec_code) bin_events<true, kTile, true, true>(a, M, false, nr, total, s_rs, s_pref, s_bin, ev, step);
-                else bin_events<true, kTile, false, true>(a, M, false, nr, total, s_rs, s_pref, s_bin, ev, step);
+                if (a.rec_code)
+                    bin_events<true, kTile, true, true>(a, M, false, nr, total, s_rs, s_pref, s_bin, ev, step, fb);
+                else bin_events<true, kTile, false, true>(a, M, false, nr, total, s_rs, s_pref, s_bin, ev, step, fb);
             } else if (a.rec_code) {
-                bin_events<true, kTile, true>(a, M, false, nr, total, s_rs, s_pref, s_bin, ev, step);
+                bin_events<true, kTile, true>(a, M, false, nr, total, s_rs, s_pref, s_bin, ev, step, fb);
             } else {
-                bin_events<true, kTile>(a, M, false, nr, total, s_rs, s_pref, s_bin, ev, step);
+                bin_events<true, kTile>(a, M, false, nr, total, s_rs, s_pref, s_bin, ev, step, fb);
             }
+            fb += total;
             if (!one) __syncthreads();  // before the next chunk is staged
         }
         // source tiles (their rows are the sources that fire this step)
@@ -1257,8 +1279,9 @@ __global__ __launch_bounds__(kTile) void bin_kernel(const NetArgs* __restrict__ 
                 events += total;
                 bin_events<false, kTile>(a, M, true, cnt, total, s_srs, s_spref, s_bin, ev, step);
             } else {
-                bin_events<true, kTile>(a, M, true, cnt, total, s_srs, s_spref, s_bin, ev, step);
+                bin_events<true, kTile>(a, M, true, cnt, total, s_srs, s_spref, s_bin, ev, step, fb);
             }
+            fb += total;
             __syncthreads();
         }
         if (pass == 0) bin_offsets<kTile>(a, s_bin, s_wave, j, base, events, hdr);
@@ -1500,11 +1523,14 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m, RecCtx rc) {
 
 // Gather one bin's records of one step into L.slice (each cell from zero in
 // ascending event id): hrow = the bin's row of headers (nw writers, two per
-// thread; the entries read are zeroed), ev = the records.  Every thread of
-// the block calls it after zeroing L.slice and emptying L.head; it ends with
-// a barrier.
+// thread; the entries read are zeroed), ev = the records (16-byte records
+// when kRec, else 8-byte ones, whose per-cell order key is writer << 21 | f).
+// Every thread of the block calls it after zeroing L.slice and emptying
+// L.head; it ends with a barrier.
 template <bool kRec>
-__device__ __forceinline__ void bin_gather(BinApplyLds& L, uint2* hrow2, const uint4* ev, uint32_t nw, RecCtx rc) {
+__device__ __forceinline__ void bin_gather(BinApplyLds& L, uint2* hrow2, dptr<const uint2> ev, uint32_t nw,
+                                           RecCtx rc) {
+    const dptr<const uint4> ev4 = reinterpret_cast<dptr<const uint4>>(ev);
     static_assert(kMaxWriters == 2 * kApply, "two writers per thread");
     const uint32_t t = threadIdx.x;
     uint4* hrow = reinterpret_cast<uint4*>(hrow2);  // rows hold an even number of writers
@@ -1532,13 +1558,13 @@ __device__ __forceinline__ void bin_gather(BinApplyLds& L, uint2* hrow2, const u
         const uint32_t jb = lo;
         if (jb == ja) {
             // Group ja alone exceeds a pass; its records are unordered, so
-            // take them in windows of kBinCap consecutive event ids (at most
-            // kBinCap records each), lowest first.
-            const uint4* rec = ev + L.seg_base[ja];
-            const uint32_t seg = L.seg_pref[ja + 1] - f0;
+            // take them in windows of kBinCap consecutive event ids (or
+            // indices f) -- at most kBinCap records each -- lowest first.
+            const uint32_t seg = L.seg_pref[ja + 1] - f0, rb = L.seg_base[ja];
             if (t == 0) L.next = kListNil;
             __syncthreads();
-            for (uint32_t i = t; i < seg; i += kApply) atomicMin(&L.next, rec[i].x);
+            for (uint32_t i = t; i < seg; i += kApply)
+                atomicMin(&L.next, kRec ? ev4[rb + i].x : ev[rb + i].y >> kBinLog2);
             __syncthreads();
             uint32_t w0 = L.next;
             for (;;) {
@@ -1549,10 +1575,23 @@ __device__ __forceinline__ void bin_gather(BinApplyLds& L, uint2* hrow2, const u
                 }
                 __syncthreads();
                 for (uint32_t i = t; i < seg; i += kApply) {
-                    const uint4 r = rec[i];
-                    if (r.x < w0) continue;
-                    if (r.x - w0 < kBinCap) bin_link(L, atomicAdd(&L.m, 1u), r.z, r.x, __uint_as_float(r.y), kRec ? r.w : 0u);
-                    else atomicMin(&L.next, r.x);
+                    uint32_t key, cell, code = 0u;
+                    float w;
+                    if (kRec) {
+                        const uint4 r = ev4[rb + i];
+                        key = r.x;
+                        cell = r.z;
+                        code = r.w;
+                        w = __uint_as_float(r.y);
+                    } else {
+                        const uint2 r = ev[rb + i];
+                        key = r.y >> kBinLog2;
+                        cell = r.y & (kBinPosts - 1u);
+                        w = __uint_as_float(r.x);
+                    }
+                    if (key < w0) continue;
+                    if (key - w0 < kBinCap) bin_link(L, atomicAdd(&L.m, 1u), cell, key, w, code);
+                    else atomicMin(&L.next, key);
                 }
                 __syncthreads();
                 bin_accumulate<kRec>(L, L.m, rc);
@@ -1565,19 +1604,42 @@ __device__ __forceinline__ void bin_gather(BinApplyLds& L, uint2* hrow2, const u
         }
         const uint32_t m = L.seg_pref[jb] - f0;
         constexpr uint32_t kRecs = kBinCap / kApply;
-        uint4 r[kRecs];
+        if (kRec) {
+            uint4 r[kRecs];
 #pragma unroll
-        for (uint32_t u = 0; u < kRecs; ++u) {
-            const uint32_t f = t + u * kApply;
-            if (f < m) {
-                const uint32_t sg = floor_index<kMaxWriters>(L.seg_pref, ja, jb, f0 + f);  // group of the record
-                r[u] = ev[L.seg_base[sg] + (f0 + f - L.seg_pref[sg])];
+            for (uint32_t u = 0; u < kRecs; ++u) {
+                const uint32_t f = t + u * kApply;
+                if (f < m) {
+                    const uint32_t sg = floor_index<kMaxWriters>(L.seg_pref, ja, jb, f0 + f);  // group of the record
+                    r[u] = ev4[L.seg_base[sg] + (f0 + f - L.seg_pref[sg])];
+                }
             }
-        }
 #pragma unroll
-        for (uint32_t u = 0; u < kRecs; ++u) {
-            const uint32_t f = t + u * kApply;
-            if (f < m) bin_link(L, f, r[u].z, r[u].x, __uint_as_float(r[u].y), kRec ? r[u].w : 0u);
+            for (uint32_t u = 0; u < kRecs; ++u) {
+                const uint32_t f = t + u * kApply;
+                if (f < m) bin_link(L, f, r[u].z, r[u].x, __uint_as_float(r[u].y), r[u].w);
+            }
+        } else {
+            static_assert(kRecs == 2 && kMaxWriters <= (1u << 16), "two writer indices share a register");
+            uint2 r[kRecs];
+            uint32_t sgp = 0u;  // the records' groups (= writers), 16 bits each
+#pragma unroll
+            for (uint32_t u = 0; u < kRecs; ++u) {
+                const uint32_t f = t + u * kApply;
+                if (f < m) {
+                    const uint32_t sg = floor_index<kMaxWriters>(L.seg_pref, ja, jb, f0 + f);
+                    sgp |= sg << (16u * u);
+                    r[u] = ev[L.seg_base[sg] + (f0 + f - L.seg_pref[sg])];
+                }
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kRecs; ++u) {
+                const uint32_t f = t + u * kApply;
+                if (f < m)
+                    bin_link(L, f, r[u].y & (kBinPosts - 1u),
+                             (((sgp >> (16u * u)) & 0xFFFFu) << kRecFBits) | (r[u].y >> kBinLog2),
+                             __uint_as_float(r[u].x));
+            }
         }
         __syncthreads();
         bin_accumulate<kRec>(L, m, rc);
@@ -1600,7 +1662,7 @@ __global__ __launch_bounds__(kApply) void bin_apply_kernel(const NetArgs* __rest
     const uint32_t k = ref.y, t = threadIdx.x;
     uint32_t slot;
     uint2* hdr = a.bin_hdr;
-    const uint4* ev = a.bin_ev;
+    dptr<const uint2> ev = ev_buf(a, 0u);
     if (mode == 0) {
         slot = *a.bin_slot;
         if (slot == kEmpty) return;
@@ -1610,7 +1672,7 @@ __global__ __launch_bounds__(kApply) void bin_apply_kernel(const NetArgs* __rest
         slot = static_cast<uint32_t>(step & (kSlots - 1));
         const uint32_t par = static_cast<uint32_t>((step + 1) & 1u);
         hdr += par * a.hdr_stride;
-        ev += par * a.n_ev;
+        ev = ev_buf(a, par);
     }
 #pragma unroll
     for (uint32_t pr = 0; pr < kBinPosts / kApply; ++pr) {
@@ -1697,7 +1759,7 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
         }
         const uint32_t rpar = wpar ^ 1u;
         bin_gather<false>(L, a.bin_hdr + rpar * a.hdr_stride + static_cast<size_t>(k) * a.bin_rows,
-                          a.bin_ev + rpar * a.n_ev, a.n_bblk, RecCtx{nullptr, 0u});
+                          ev_buf(a, rpar), a.n_bblk, RecCtx{nullptr, 0u});
 #pragma unroll
         for (uint32_t q = 0; q < 2; ++q) cur[q] = L.slice[t + q * kApply];
     }
@@ -1850,7 +1912,7 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
     __syncthreads();
 
     // writer k of this step's events
-    uint4* ev = a.bin_ev + wpar * a.n_ev;
+    const dptr<uint2> ev = ev_buf(a, wpar);
     uint2* hdr = a.bin_hdr + wpar * a.hdr_stride;
     const bool cached = blk_ed <= kEvCache * kApply;
     EvCache ec;
@@ -1861,8 +1923,8 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
         bin_events<false, kApply>(a, M, false, blk_sp, blk_ed, s_rs, s_pref, s_bin, ev, step);
     }
     bin_offsets<kApply>(a, s_bin, L.wave, k, a.tile_ebase[k * kBinTiles], blk_ed, hdr);
-    if (cached) ev_cache_write<kApply>(a, false, ec, blk_ed, s_bin, ev, step);
-    else bin_events<true, kApply>(a, M, false, blk_sp, blk_ed, s_rs, s_pref, s_bin, ev, step);
+    if (cached) ev_cache_write<kApply, false, true>(a, false, ec, blk_ed, s_bin, ev, step);
+    else bin_events<true, kApply, false, false, true>(a, M, false, blk_sp, blk_ed, s_rs, s_pref, s_bin, ev, step);
     __syncthreads();
     if (t == 0) mega_done(a, step);
 }

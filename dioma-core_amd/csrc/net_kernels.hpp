@@ -51,6 +51,11 @@ constexpr uint32_t kBinCap = 2048;      // records bin_apply orders in one LDS p
 constexpr uint32_t kMaxWriters = 2048;  // binned writers per network
 constexpr uint32_t kBinBlockTiles = 4;  // source tiles per bin_kernel writer
 constexpr uint32_t kLtpItem = 0xFFFFFFFEu;  // bin_kernel item that runs in-edge LTP instead of writing
+// 8-byte records: the event's index within its writer's events of the step
+// takes the 21 bits above the cell; a writer spanning more edges than that
+// makes its network use 16-byte records.  Writer index (< kMaxWriters = 2^11)
+// and f together form a 32-bit key ascending with the event id.
+constexpr uint32_t kRecFBits = 21;
 
 enum : uint32_t {
     kPopRefrElided = 1u,  // refractory timer provably stays 0: not loaded/stored
@@ -104,7 +109,12 @@ struct NetArgs {
     // n_bblk, ascending event ids) puts its events grouped by bin from its
     // first edge id, and bin_hdr[k][j] = {first record, count} of its group
     // for bin k.  Megastep networks keep two buffers of each, by step parity:
-    // bin_ev + p*n_ev and bin_hdr + p*hdr_stride.
+    // records from ev_buf(a, p) and bin_hdr + p*hdr_stride.  A record is
+    // 8 bytes, {weight bits, f << 11 | cell} with f the event's index in its
+    // writer's concatenation of spiking rows (ascending event id within the
+    // writer, below 2^kRecFBits), or, in networks with receptor synapses or a
+    // writer spanning more edges (ev_wide), 16 bytes {event id, weight bits,
+    // cell, receptor code}.
     uint32_t n_bins;
     uint32_t n_bblk;            // writers
     int32_t bin_fused;          // the neuron kernel's blocks are the neuron-tile writers
@@ -114,7 +124,8 @@ struct NetArgs {
     uint64_t n_ev;              // synapses + source edges
     uint64_t hdr_stride;        // n_bins * bin_rows
     dptr<uint2> bin_hdr;             // [1 or 2][n_bins][bin_rows]
-    dptr<uint4> bin_ev;              // [1 or 2][n_ev] {event id, weight bits, cell, 0}
+    dptr<uint2> bin_ev;              // [1 or 2][n_ev] records, 8 B (or 16 B when ev_wide)
+    int32_t ev_wide;            // 16-byte records
     dptr<uint32_t> bin_slot;         // ring slot bin_apply fills, kEmpty past the horizon
     dptr<uint32_t> bin_done;         // megastep: blocks of the step that finished
     dptr<uint32_t> bin_mat;          // megastep: the pending events are in ring slot step % 32
