@@ -193,7 +193,7 @@ __device__ __forceinline__ float powi_f32(float a, int n) {
 // recovery over the steps since the edge last fired (capped at 1e6), deliver
 // weight * R*U with the pre-spike factor, then deplete R (facilitate U).  An
 // edge fires at most once per step, so its state has a single writer.
-__device__ __forceinline__ float tm_deliver(const NetArgs& a, uint32_t e, float wt, uint64_t step) {
+__device__ __noinline__ float tm_deliver(const NetArgs& a, uint32_t e, float wt, uint64_t step) {
     const uint32_t t = a.tm_type[e];
     if (t == 0) return wt;
     const float4 p = a.tm_prof[t - 1];
@@ -1722,31 +1722,34 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
     if (step >= a.step_limit) return;
     const uint32_t wpar = static_cast<uint32_t>(step & 1u);
 
-    // the block's neurons' populations; their state loads follow the gather
-    // (issued before it, they slow its latency-bound reads more than they gain)
-    uint32_t pop[2];
-#pragma unroll
-    for (uint32_t q = 0; q < 2; ++q) {
-        const uint32_t tile = k * kBinTiles + t / kTile + q * (kApply / kTile);
-        pop[q] = 0u;
-        if (tile < a.n_tiles && a.n_pops > 1) pop[q] = a.tile_pop[tile];
-    }
+    // Thread t's neurons are cell t and t + 1024 of the bin: tiles w/4 and
+    // w/4 + 4 of wave w.  Tiles and populations are wave-uniform, so the
+    // population table and descriptors are read through the constant address
+    // space (scalar loads into scalar registers, which neither wait behind
+    // nor hold up the vector loads) and looked up again wherever needed.
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    auto tile_of = [&](uint32_t q) -> uint32_t { return k * kBinTiles + wv / kWavesPerTile + q * (kApply / kTile); };
+    auto pop_of = [&](uint32_t q) -> uint32_t {
+        const uint32_t tile = tile_of(q);
+        return tile < a.n_tiles && a.n_pops > 1 ? ((cptr<uint32_t>)a.tile_pop)[tile] : 0u;
+    };
+    auto pop_dev = [&](uint32_t p) -> PopDev { return ((cptr<PopDev>)a.pops)[p]; };
     bin_map_load(a, M);
     float cur[2] = {0.0f, 0.0f};
-    const PopDev* PP[2];
-    uint32_t li[2], tile[2];
+    PopDev PV[2];
+    uint32_t li[2], tile[2], pidx[2];
     bool on[2];
     if (*a.bin_mat) {
         // a host call (current injection, ring download) has applied them
         float* ring_slot = a.ring + static_cast<size_t>(step & (kSlots - 1)) * a.N;
 #pragma unroll
         for (uint32_t q = 0; q < 2; ++q) {
-            tile[q] = k * kBinTiles + t / kTile + q * (kApply / kTile);
-            PP[q] = a.pops + pop[q];
-            li[q] = (tile[q] - PP[q]->tile0) * kTile + tl;
-            on[q] = tile[q] < a.n_tiles && li[q] < PP[q]->n;
+            tile[q] = tile_of(q);
+            PV[q] = pop_dev(pop_of(q));
+            li[q] = (tile[q] - PV[q].tile0) * kTile + tl;
+            on[q] = tile[q] < a.n_tiles && li[q] < PV[q].n;
             if (!on[q]) continue;
-            const uint32_t i = PP[q]->off + li[q];
+            const uint32_t i = PV[q].off + li[q];
             cur[q] = ring_slot[i];
             ring_slot[i] = 0.0f;
         }
@@ -1767,21 +1770,22 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
     Neuron<MODEL> nrn[2];
 #pragma unroll
     for (uint32_t q = 0; q < 2; ++q) {
-        tile[q] = k * kBinTiles + t / kTile + q * (kApply / kTile);
-        PP[q] = a.pops + pop[q];
-        li[q] = (tile[q] - PP[q]->tile0) * kTile + tl;
-        on[q] = tile[q] < a.n_tiles && li[q] < PP[q]->n;
+        tile[q] = tile_of(q);
+        pidx[q] = pop_of(q);
+        PV[q] = pop_dev(pidx[q]);
+        li[q] = (tile[q] - PV[q].tile0) * kTile + tl;
+        on[q] = tile[q] < a.n_tiles && li[q] < PV[q].n;
         row0[q] = deg[q] = 0u;
-        if (on[q]) nrn[q].load(*PP[q], li[q]);
+        if (on[q]) nrn[q].load(PV[q], li[q]);
     }
 
     // Gaussian input: the thread's two neurons are li and li + 1024 of one
     // population whenever its bin lies inside that population, and then
     // share one Box-Muller draw (the pair of normal_pair)
     float nz[2] = {0.0f, 0.0f};
-    if (on[0] && on[1] && PP[0] == PP[1] && (PP[0]->flags & kPopNoise) && (li[0] & 1024u) == 0u &&
+    if (on[0] && on[1] && pidx[0] == pidx[1] && (PV[0].flags & kPopNoise) && (li[0] & 1024u) == 0u &&
         li[1] == li[0] + 1024u) {
-        const PopDev& P = *PP[0];
+        const PopDev& P = PV[0];
         float zc, zs;
         normal_pair(P.seed, li[0], step, zc, zs);
         const bool uni = P.flags & kPopNoiseScalar;
@@ -1790,7 +1794,7 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
     } else {
 #pragma unroll
         for (uint32_t q = 0; q < 2; ++q) {
-            const PopDev& P = *PP[q];
+            const PopDev& P = PV[q];
             if (on[q] && (P.flags & kPopNoise)) {
                 const float sg = (P.flags & kPopNoiseScalar) ? P.sigma_uni : P.sigma[li[q]];
                 nz[q] = sg * normal_draw(P.seed, li[q], step);
@@ -1804,7 +1808,7 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
     uint32_t excl[2];
 #pragma unroll
     for (uint32_t q = 0; q < 2; ++q) {
-        const PopDev& P = *PP[q];
+        const PopDev& P = PV[q];
         const uint32_t ti = t / kTile + q * (kApply / kTile);
         int spike = 0, drive_now = 0;
         if (on[q]) {
@@ -1895,7 +1899,7 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
                 dbase += s_deg[ti][w];
             }
             const uint32_t rank = base + static_cast<uint32_t>(__popcll(m[q] & ((1ull << lane) - 1ull)));
-            a.active[static_cast<size_t>(tile[q]) * kTile + rank] = PP[q]->off + li[q];
+            a.active[static_cast<size_t>(tile[q]) * kTile + rank] = PV[q].off + li[q];
             s_rs[sp0 + rank] = row0[q];
             s_pref[sp0 + rank] = ed0 + dbase + excl[q];
         }

@@ -22,6 +22,15 @@ template <class T>
 __host__ __device__ inline dptr<T> dev_ptr(T* p) {
     return (dptr<T>)p;
 }
+// Read-only device data seen through the constant address space: uniform
+// loads through it compile to scalar (s_load) loads.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using cptr = __attribute__((address_space(4))) const T*;
+#else
+template <class T>
+using cptr = const T*;
+#endif
 
 constexpr uint32_t kTile = 256;          // neurons per tile = threads per block
 constexpr uint32_t kWavesPerTile = kTile / 64;
