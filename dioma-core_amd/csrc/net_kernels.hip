@@ -1539,23 +1539,41 @@ __device__ __forceinline__ void bin_gather(BinApplyLds& L, uint2* hrow2, dptr<co
     if (hw.y | hw.w) hrow[t] = make_uint4(0u, 0u, 0u, 0u);  // consumed: empty for the next use
     uint32_t excl;
     const uint32_t total = block_scan<kApply>(hw.y + hw.w, L.wave, excl);
-    L.seg_base[2 * t] = hw.x;
-    L.seg_base[2 * t + 1] = hw.z;
-    L.seg_pref[2 * t] = excl;
-    L.seg_pref[2 * t + 1] = excl + hw.y;
-    if (t == 0) L.seg_pref[kMaxWriters] = total;
+    // One pass takes every record when they fit (the common case): then
+    // each thread lists its two groups' records -- buffer index and writer --
+    // at their positions in the pass, so that a record's thread finds it
+    // with one LDS read instead of a search over the groups.
+    const bool one = total <= kBinCap;
+    uint32_t* raddr = L.seg_base;
+    uint32_t* rgrp = L.seg_pref;
+    if (one) {
+        for (uint32_t i = 0; i < hw.y; ++i) {
+            raddr[excl + i] = hw.x + i;
+            rgrp[excl + i] = 2 * t;
+        }
+        for (uint32_t i = 0; i < hw.w; ++i) {
+            raddr[excl + hw.y + i] = hw.z + i;
+            rgrp[excl + hw.y + i] = 2 * t + 1;
+        }
+    } else {
+        L.seg_base[2 * t] = hw.x;
+        L.seg_base[2 * t + 1] = hw.z;
+        L.seg_pref[2 * t] = excl;
+        L.seg_pref[2 * t + 1] = excl + hw.y;
+        if (t == 0) L.seg_pref[kMaxWriters] = total;
+    }
     __syncthreads();
     uint32_t ja = 0;
     while (ja < nw) {
-        const uint32_t f0 = L.seg_pref[ja];
+        const uint32_t f0 = one ? 0u : L.seg_pref[ja];
         if (f0 == total) break;  // nothing left
         // the longest run of whole groups [ja, jb) that fits one pass
         uint32_t lo = ja, hi = nw;  // largest jb with seg_pref[jb] <= f0 + kBinCap
-        while (lo < hi) {
+        while (!one && lo < hi) {
             const uint32_t mid = (lo + hi + 1) >> 1;
             if (L.seg_pref[mid] - f0 <= kBinCap) lo = mid; else hi = mid - 1;
         }
-        const uint32_t jb = lo;
+        const uint32_t jb = one ? nw : lo;
         if (jb == ja) {
             // Group ja alone exceeds a pass; its records are unordered, so
             // take them in windows of kBinCap consecutive event ids (or
@@ -1602,7 +1620,7 @@ __device__ __forceinline__ void bin_gather(BinApplyLds& L, uint2* hrow2, dptr<co
             ++ja;
             continue;
         }
-        const uint32_t m = L.seg_pref[jb] - f0;
+        const uint32_t m = one ? total : L.seg_pref[jb] - f0;
         constexpr uint32_t kRecs = kBinCap / kApply;
         if (kRec) {
             uint4 r[kRecs];
@@ -1610,8 +1628,12 @@ __device__ __forceinline__ void bin_gather(BinApplyLds& L, uint2* hrow2, dptr<co
             for (uint32_t u = 0; u < kRecs; ++u) {
                 const uint32_t f = t + u * kApply;
                 if (f < m) {
-                    const uint32_t sg = floor_index<kMaxWriters>(L.seg_pref, ja, jb, f0 + f);  // group of the record
-                    r[u] = ev4[L.seg_base[sg] + (f0 + f - L.seg_pref[sg])];
+                    if (one) {
+                        r[u] = ev4[raddr[f]];
+                    } else {
+                        const uint32_t sg = floor_index<kMaxWriters>(L.seg_pref, ja, jb, f0 + f);  // its group
+                        r[u] = ev4[L.seg_base[sg] + (f0 + f - L.seg_pref[sg])];
+                    }
                 }
             }
 #pragma unroll
@@ -1627,9 +1649,16 @@ __device__ __forceinline__ void bin_gather(BinApplyLds& L, uint2* hrow2, dptr<co
             for (uint32_t u = 0; u < kRecs; ++u) {
                 const uint32_t f = t + u * kApply;
                 if (f < m) {
-                    const uint32_t sg = floor_index<kMaxWriters>(L.seg_pref, ja, jb, f0 + f);
+                    uint32_t sg, at;
+                    if (one) {
+                        sg = rgrp[f];
+                        at = raddr[f];
+                    } else {
+                        sg = floor_index<kMaxWriters>(L.seg_pref, ja, jb, f0 + f);
+                        at = L.seg_base[sg] + (f0 + f - L.seg_pref[sg]);
+                    }
                     sgp |= sg << (16u * u);
-                    r[u] = ev[L.seg_base[sg] + (f0 + f - L.seg_pref[sg])];
+                    r[u] = ev[at];
                 }
             }
 #pragma unroll
