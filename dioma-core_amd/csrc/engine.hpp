@@ -130,10 +130,10 @@ struct NetState {
     uint32_t* bin_done = nullptr;
     uint32_t* bin_mat = nullptr;
     std::vector<uint2> mat_items;      // (engine index, bin) of every bin, XCD-ordered
-    uint2* d_mat_items = nullptr;      // [n_bins + kXcds]
+    uint2* d_mat_items = nullptr;      // [n_bins + kXcdPad]
     bool mat_dirty = true;
     std::vector<uint4> dense_writers;  // every writer as a bin_kernel item (dense STDP step)
-    uint4* d_dense_writers = nullptr;  // [bin_rows + kXcds]
+    uint4* d_dense_writers = nullptr;  // [bin_rows + kXcdPad]
     bool dense_dirty = true;
     uint2* bin_ev = nullptr;   // [1 or 2][n_ev] records (two uint2 per record when ev_wide)
     bool ev_wide = false;      // 16-byte records: receptor synapses or a writer spanning >= 2^21 edges
@@ -427,16 +427,23 @@ struct Engine {
     }
 
     // Workgroup b of a launch runs on XCD b % kXcds: reorder a work list so
-    // that XCD x takes the contiguous items [x*per, (x+1)*per) (padded with
-    // `pad`, which the kernels skip).  Neighbouring tiles and bins then share
-    // that XCD's L2.
+    // that runs of kXcdChunk consecutive items go to the XCDs in turn (run c
+    // to XCD c % kXcds; padded with `pad`, which the kernels skip).  A run's
+    // bins write their entries of every header row -- 16 adjacent 8-byte
+    // entries, one 128-byte line -- from one L2, and populations whose bins
+    // carry more work (more spikes to push) spread over the XCDs instead of
+    // loading a few of them.
     template <class T>
     static void xcd_order(std::vector<T>& v, const T& pad) {
-        const size_t n = v.size(), per = (n + kXcds - 1) / kXcds;
+        const size_t n = v.size();
+        std::vector<std::vector<T>> per_x(kXcds);
+        for (size_t i = 0; i < n; ++i) per_x[(i / kXcdChunk) % kXcds].push_back(v[i]);
+        size_t per = 0;
+        for (const auto& q : per_x) per = std::max(per, q.size());
         std::vector<T> o(per * kXcds, pad);
         for (size_t b = 0; b < o.size(); ++b) {
-            const size_t i = (b % kXcds) * per + b / kXcds;
-            if (i < n) o[b] = v[i];
+            const auto& q = per_x[b % kXcds];
+            if (b / kXcds < q.size()) o[b] = q[b / kXcds];
         }
         v.swap(o);
     }
@@ -637,6 +644,8 @@ struct Engine {
         if ((rc = ensure_tables())) return rc;
         if (!s->finalized || !s->mega) return SIMUSE_OK;
         if (s->mat_dirty) {
+            if (s->mat_items.size() > static_cast<size_t>(s->n_bins) + kXcdPad)
+                return fail(SIMUSE_E_STATE, "materialize: %zu work items exceed the list's room", s->mat_items.size());
             HIP_TRY(hipMemcpy(s->d_mat_items, s->mat_items.data(), sizeof(uint2) * s->mat_items.size(),
                               hipMemcpyHostToDevice));
             s->mat_dirty = false;

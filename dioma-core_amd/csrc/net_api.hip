@@ -195,8 +195,8 @@ int NetState::finalize(hipStream_t stream) {
         if ((rc = alloc(&bin_ev, nbuf * n_ev * (ev_wide ? 2 : 1)))) return rc;
         if ((rc = alloc(&bin_hdr, nbuf * n_bins * bin_rows))) return rc;
         HIP_TRY(hipMemsetAsync(bin_hdr, 0, sizeof(uint2) * nbuf * n_bins * bin_rows, stream));
-        if ((rc = alloc(&d_dense_writers, bin_rows + kXcds))) return rc;  // XCD-ordered, padded
-        if ((rc = alloc(&d_mat_items, n_bins + kXcds))) return rc;
+        if ((rc = alloc(&d_dense_writers, bin_rows + kXcdPad))) return rc;  // XCD-ordered, padded
+        if ((rc = alloc(&d_mat_items, n_bins + kXcdPad))) return rc;
         if ((rc = alloc(&bin_slot, 1))) return rc;
         if ((rc = alloc(&bin_done, 1))) return rc;
         if ((rc = alloc(&bin_mat, 1))) return rc;
@@ -1136,6 +1136,8 @@ int simuse_net_stdp_step(simuse_net* net, float trace_decay, float lr_ltp, float
     launch_dense_weights(e.d_nets, s.N, s.dtrace, lr_ltp, lr_ltd, w_min, w_max, e.stream);
     if (s.binned) {
         if (s.dense_dirty) {
+            if (s.dense_writers.size() > static_cast<size_t>(s.bin_rows) + kXcdPad)
+                return fail(SIMUSE_E_STATE, "dense step: %zu writers exceed the list's room", s.dense_writers.size());
             HIP_TRY(hipMemcpy(s.d_dense_writers, s.dense_writers.data(), sizeof(uint4) * s.dense_writers.size(),
                               hipMemcpyHostToDevice));
             s.dense_dirty = false;

@@ -54,6 +54,8 @@ constexpr uint32_t kBinLog2 = 11;
 constexpr uint32_t kBinPosts = 1u << kBinLog2;
 constexpr uint32_t kMaxBins = 4096;
 constexpr uint32_t kXcds = 8;          // MI355X: workgroups go round-robin over 8 XCDs
+constexpr uint32_t kXcdChunk = 16;     // consecutive work items per XCD turn (see Engine::xcd_order)
+constexpr uint32_t kXcdPad = kXcds * kXcdChunk;  // most pad items an XCD-ordered list gains
 constexpr uint32_t kBinTiles = 8;       // tiles per bin
 constexpr uint32_t kMaxBinPops = 64;    // populations of a binned network
 constexpr uint32_t kBinCap = 2048;      // records bin_apply orders in one LDS pass
