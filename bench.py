@@ -77,6 +77,14 @@ def izh_algorithmic_bytes_per_neuron(spec) -> float:
     return total / sum(p.n for p in spec.pops)
 
 
+def graph_steps_for(steps: int, most: int) -> int:
+    """Steps per captured graph: the largest divisor of the timed step count
+    up to `most` (at least 8 when one exists, else `most`), so that every timed
+    step is a graph replay rather than a stand-alone launch."""
+    best = max((d for d in range(1, min(64, max(most, 1)) + 1) if steps % d == 0), default=1)
+    return best if best >= 8 else most
+
+
 def form_name(form: int) -> str:
     import simuse
     return {simuse.FORM_ONE_KERNEL: "one_kernel", simuse.FORM_BINNED: "binned_two_kernel",
@@ -140,7 +148,8 @@ def bench_izhikevich(args) -> dict:
     spec = izhikevich_network(n=args.neurons, k=args.synapses_per_neuron, seed=1,
                               steps=args.steps + args.warmup)
     log(f"  generated in {time.perf_counter() - t0:.1f}s")
-    net = build_device(spec, graph_steps=args.graph_steps)
+    gsteps = graph_steps_for(args.steps, args.graph_steps)
+    net = build_device(spec, graph_steps=gsteps)
     form = net.step_form()
     log(f"  device bytes {net.device_bytes() / 2**30:.2f} GiB, step form {form_name(form)}")
     tuned = None
@@ -152,7 +161,7 @@ def bench_izhikevich(args) -> dict:
     net.run(args.warmup)
     # one untimed replay of the step graph, whatever --warmup is: the graph is
     # captured, instantiated and launched once before the timed region
-    net.run(args.graph_steps)
+    net.run(gsteps)
     s0, e0 = net.counters()
     net.sync()
     # The timed region: the K steps between two HIP events on the network's
@@ -235,9 +244,9 @@ def bench_izhikevich(args) -> dict:
         "dtype": "f32",
         "data": "synthetic (seeded Izhikevich 2003 network, random-init parameters and weights)",
         "config": {"workload": "izhikevich_1M_100M", "neurons": spec.n_total,
-                   "synapses": spec.n_syn, "dt_ms": 1.0, "graph_steps": args.graph_steps,
+                   "synapses": spec.n_syn, "dt_ms": 1.0, "graph_steps": gsteps,
                    "step_form": form_name(form), "shape": shape,
-                   "untimed_steps_before_region": args.warmup + args.graph_steps,
+                   "untimed_steps_before_region": args.warmup + gsteps,
                    "spikes_per_step": spikes / args.steps, "events_per_step": events / args.steps},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -276,9 +285,10 @@ def bench_brunel(args) -> dict:
     T = args.c1_steps
     spec = brunel_lif_network(n=10_000, p=0.01, seed=1, steps=T, rate_hz=2000.0)
     log(f"brunel: LIF {spec.n_total} neurons, {spec.n_syn} synapses, {len(spec.sources)} Poisson sources, {T} steps")
-    net = build_device(spec, graph_steps=args.graph_steps)
+    gsteps = graph_steps_for(T, args.graph_steps)
+    net = build_device(spec, graph_steps=gsteps)
     form = net.step_form()
-    net.run(args.graph_steps)  # graph built and replayed once; counters below are differences
+    net.run(gsteps)  # graph built and replayed once; counters below are differences
     s0, e0 = net.counters()
     net.sync()
     t0 = time.perf_counter()
@@ -302,7 +312,7 @@ def bench_brunel(args) -> dict:
     fp64 = None
     if not args.no_fp64:
         # statistics of the device run vs the FP64 twin over the whole horizon
-        dev = build_device(spec, graph_steps=args.graph_steps)
+        dev = build_device(spec, graph_steps=gsteps)
         dev.record(T)
         dev.run(T)
         rd = dev.download_raster()
@@ -632,7 +642,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--neurons", type=int, default=1_000_000)
     ap.add_argument("--synapses-per-neuron", type=int, default=100)
-    ap.add_argument("--graph-steps", type=int, default=16)
+    ap.add_argument("--graph-steps", type=int, default=64,
+                    help="most steps per captured graph (the bench uses the largest divisor of --steps up to it)")
     ap.add_argument("--profile-steps", type=int, default=100,
                     help="steps timed per kernel with HIP events for the phase breakdown (at least 1)")
     ap.add_argument("--cpu-steps", type=int, default=300)

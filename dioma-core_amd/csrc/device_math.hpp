@@ -167,4 +167,25 @@ __device__ __forceinline__ float vtrap(float x, float y) {
     return x / (det_expf(r) - 1.0f);
 }
 
+// AlphaLifModel::step, P/src/zoo/neurons/alpha_lif.rs:99-136: refractoriness
+// masks only the delayed input; v integrates with the previous g through an
+// explicit fused multiply-add, the threshold is strict (v > v_thresh), a
+// spike resets v and g, and g decays by syn_decay with a fused multiply-add.
+// (fmaf is the one contraction the library asks for: the reference's shader
+// and CPU step both fuse exactly these two nodes.)
+__device__ __forceinline__ int alpha_lif_update(float& v, float& g, float& refr_cnt, float I, float tf,
+                                                float v_rest, float v_th, float v_reset, float refr_steps,
+                                                float syn_decay) {
+    const float g_old = g;
+    const float delayed = refr_cnt >= refr_steps ? I : 0.0f;
+    const float integrand = g_old - (v - v_rest);
+    const float v_new = fmaf(tf, integrand, v);
+    const bool spike = v_new > v_th;
+    v = spike ? v_reset : v_new;
+    const float g_new = fmaf(syn_decay, g_old, delayed);
+    g = spike ? 0.0f : g_new;
+    refr_cnt = spike ? 0.0f : refr_cnt + 1.0f;
+    return spike ? 1 : 0;
+}
+
 }  // namespace simuse

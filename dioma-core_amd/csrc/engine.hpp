@@ -45,6 +45,7 @@ void launch_megastep(int model, const NetArgs* nets, const uint2* items, uint32_
 void launch_head_clear(unsigned long long* head, const uint64_t* step, size_t n, hipStream_t s);
 void launch_inject(float* ring, const uint64_t* step, uint32_t N, const uint32_t* idx, const float* val,
                    uint32_t count, int serial, hipStream_t s);
+void launch_vjump(float* v, const uint32_t* idx, const float* val, uint32_t count, int serial, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 void launch_tm_init(const uint8_t* type, const float4* prof, float* r, float* u, uint32_t* last, size_t n,
                     hipStream_t s);
@@ -61,10 +62,12 @@ void launch_raster_stats(const uint64_t* raster, uint32_t n_words, uint32_t step
     } while (0)
 
 inline int model_states(int m) {
-    return m == SIMUSE_MODEL_LIF ? 2 : m == SIMUSE_MODEL_IZHIKEVICH ? 3 : m == SIMUSE_MODEL_HH ? 6 : 9;
+    return m == SIMUSE_MODEL_LIF ? 2 : m == SIMUSE_MODEL_IZHIKEVICH ? 3 : m == SIMUSE_MODEL_HH ? 6
+         : m == SIMUSE_MODEL_ALPHA_LIF ? 3 : 9;
 }
 inline int model_params(int m) {
-    return m == SIMUSE_MODEL_LIF ? 6 : m == SIMUSE_MODEL_IZHIKEVICH ? 7 : m == SIMUSE_MODEL_HH ? 13 : 14;
+    return m == SIMUSE_MODEL_LIF ? 6 : m == SIMUSE_MODEL_IZHIKEVICH ? 7 : m == SIMUSE_MODEL_HH ? 13
+         : m == SIMUSE_MODEL_ALPHA_LIF ? 7 : 14;
 }
 // column of tau_ref and of the refractory state (LIF/Izhikevich only)
 inline int model_tau_ref_col(int m) { return m == SIMUSE_MODEL_LIF || m == SIMUSE_MODEL_IZHIKEVICH ? 5 : -1; }
@@ -330,6 +333,7 @@ struct NetState {
     int upload_weights(const float* wf, uint64_t n, hipStream_t stream);
     int download_weights(float* wf, uint64_t n, hipStream_t stream);
     int inject(int32_t pop, const uint32_t* idx, const float* val, uint32_t count, hipStream_t stream);
+    int inject_voltage(int32_t pop, const uint32_t* idx, const float* val, uint32_t count, hipStream_t stream);
     int reset(const simuse_pop_desc* pd, int32_t n_pops, hipStream_t stream);
     int counters(uint64_t* spikes, uint64_t* events, hipStream_t stream);
     int record(uint32_t max_steps, hipStream_t stream);
@@ -348,7 +352,7 @@ struct Engine {
     // device tables
     NetArgs* d_nets = nullptr;
     uint32_t nets_cap = 0;
-    static constexpr int kModels = 4;  // LIF, Izhikevich, HH, GPEA
+    static constexpr int kModels = 5;  // LIF, Izhikevich, HH, GPEA, alpha-LIF
     uint4* d_tiles[kModels] = {};      // neuron work items per model: {net, tile, tiles | writer << 4, pop}
     uint32_t n_tiles_m[kModels] = {}, cap_tiles_m[kModels] = {};
     uint2* d_mega[kModels] = {};       // megastep work items per model: {net, bin}

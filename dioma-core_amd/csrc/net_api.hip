@@ -8,6 +8,8 @@
 // need no host round trip (the reference patches a host-written slot scalar
 // before every replay, PC/src/network.rs:431-460).
 
+#include <cmath>
+
 #include "device_math.hpp"
 #include "engine.hpp"
 
@@ -30,7 +32,7 @@ int validate_pops(const simuse_pop_desc* pops, int32_t n_pops, uint64_t* n_total
     uint64_t tot = 0;
     for (int32_t p = 0; p < n_pops; ++p) {
         const simuse_pop_desc& d = pops[p];
-        if (d.model < SIMUSE_MODEL_LIF || d.model > SIMUSE_MODEL_GPEA)
+        if (d.model < SIMUSE_MODEL_LIF || d.model > SIMUSE_MODEL_ALPHA_LIF)
             return fail(SIMUSE_E_UNSUPPORTED, "population %d: unknown model %d", p, d.model);
         if (d.ns != static_cast<uint32_t>(model_states(d.model)) ||
             d.np != static_cast<uint32_t>(model_params(d.model)))
@@ -675,6 +677,44 @@ int NetState::inject(int32_t pop, const uint32_t* idx, const float* val, uint32_
     return SIMUSE_OK;
 }
 
+int NetState::inject_voltage(int32_t pop, const uint32_t* idx, const float* val, uint32_t count,
+                             hipStream_t stream) {
+    if (pop < 0 || pop >= static_cast<int32_t>(pops.size()))
+        return fail(SIMUSE_E_INVALID, "inject_voltage: population %d out of range (have %zu)", pop, pops.size());
+    if (count == 0) return SIMUSE_OK;
+    const HostPop& P = pops[pop];
+    if (!val) return fail(SIMUSE_E_INVALID, "inject_voltage: null values");
+    if (!idx && count != P.n)
+        return fail(SIMUSE_E_INVALID, "inject_voltage: %u values without indices for a population of %u", count, P.n);
+    bool dup = false;
+    if (idx) {
+        std::vector<uint32_t> sorted(idx, idx + count);
+        for (uint32_t i = 0; i < count; ++i)
+            if (idx[i] >= P.n)
+                return fail(SIMUSE_E_INVALID, "inject_voltage: neuron index %u out of bounds (population has %u)",
+                            idx[i], P.n);
+        std::sort(sorted.begin(), sorted.end());
+        dup = std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end();
+    }
+    for (uint32_t i = 0; i < count; ++i)
+        if (!std::isfinite(val[i])) return fail(SIMUSE_E_INVALID, "inject_voltage: value %u must be finite", i);
+    HIP_TRY(hipStreamSynchronize(stream));
+    int rc;
+    if (count > inj_cap) {
+        const uint32_t cap = std::max(count, 2 * inj_cap);
+        if ((rc = alloc(&inj_idx, cap))) return rc;
+        if ((rc = alloc(&inj_val, cap))) return rc;
+        inj_cap = cap;
+    }
+    if (idx) HIP_TRY(hipMemcpy(inj_idx, idx, 4 * static_cast<size_t>(count), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(inj_val, val, 4 * static_cast<size_t>(count), hipMemcpyHostToDevice));
+    // state column 0 of the population is the membrane potential of every model
+    launch_vjump(P.st, idx ? inj_idx : nullptr, inj_val, count, dup ? 1 : 0, stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(stream));
+    return SIMUSE_OK;
+}
+
 int NetState::reset(const simuse_pop_desc* pd, int32_t n_pops, hipStream_t stream) {
     uint64_t nt = 0;
     int rc;
@@ -1196,6 +1236,12 @@ int simuse_net_inject_current(simuse_net* net, int32_t pop, const uint32_t* idx,
     return net->st.inject(pop, idx, val, count, net->eng.stream);
 }
 
+int simuse_net_inject_voltage(simuse_net* net, int32_t pop, const uint32_t* idx, const float* val, uint32_t count) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    return net->st.inject_voltage(pop, idx, val, count, net->eng.stream);
+}
+
 int simuse_net_reset(simuse_net* net, const simuse_pop_desc* pops, int32_t n_pops) {
     int rc = check_net(net);
     if (rc) return rc;
@@ -1274,8 +1320,12 @@ int simuse_net_counters(simuse_net* net, uint64_t* spikes, uint64_t* events) {
 int simuse_net_record(simuse_net* net, uint32_t max_steps) {
     int rc = check_net(net);
     if (rc) return rc;
+    const uint64_t* before = net->st.raster;
+    const uint32_t cap = net->st.raster_cap;
     rc = net->st.record(max_steps, net->eng.stream);
-    net->eng.mark_dirty();
+    // the kernels see the raster through the network table: rebuild it (and
+    // drop the captured graph) only when the buffer changed
+    if (net->st.raster != before || net->st.raster_cap != cap) net->eng.mark_dirty();
     return rc;
 }
 

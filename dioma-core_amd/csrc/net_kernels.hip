@@ -429,6 +429,33 @@ struct Neuron<1> {  // Izhikevich: states v, u, refr; params dt, a, b, c, d, tau
 };
 
 template <>
+struct Neuron<4> {  // alpha-LIF: states v, g, refr_cnt; params dt, tf, v_rest, v_th, v_reset, refr_steps, syn_decay
+    float v, g, refr, p1, p2, p3, p4, p5, p6;
+    __device__ __forceinline__ void load_state(const PopDev& P, uint32_t li) {
+        v = P.st[li];
+        g = P.st[P.dstride + li];
+        refr = P.st[2 * P.dstride + li];
+    }
+    __device__ __forceinline__ void load_params(const PopDev& P, uint32_t li) {
+        p1 = par(P, 1, li); p2 = par(P, 2, li); p3 = par(P, 3, li);
+        p4 = par(P, 4, li); p5 = par(P, 5, li); p6 = par(P, 6, li);
+    }
+    __device__ __forceinline__ void load(const PopDev& P, uint32_t li) {
+        load_state(P, li);
+        load_params(P, li);
+    }
+    __device__ __forceinline__ float vm(const PopDev&, uint32_t) const { return v; }
+    __device__ __forceinline__ int step(const PopDev&, uint32_t, float I, float) {
+        return alpha_lif_update(v, g, refr, I, p1, p2, p3, p4, p5, p6);  // #[no_clamp]
+    }
+    __device__ __forceinline__ void store(const PopDev& P, uint32_t li) const {
+        P.st[li] = v;
+        P.st[P.dstride + li] = g;
+        P.st[2 * P.dstride + li] = refr;
+    }
+};
+
+template <>
 struct Neuron<3> {  // GPEA: reads and writes its columns inside the update
     __device__ __forceinline__ void load_state(const PopDev&, uint32_t) {}
     __device__ __forceinline__ void load_params(const PopDev&, uint32_t) {}
@@ -1733,7 +1760,7 @@ __global__ __launch_bounds__(kApply) void bin_apply_kernel(const NetArgs* __rest
 // LIF and Izhikevich fit two blocks per CU (eight waves per SIMD); HH and
 // GPEA keep one, their updates need the registers.
 template <int MODEL>
-constexpr int kMegaWaves = MODEL <= 1 ? 8 : 4;
+constexpr int kMegaWaves = MODEL <= 1 || MODEL == 4 ? 8 : 4;
 
 template <int MODEL>
 __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(const NetArgs* __restrict__ nets,
@@ -2219,6 +2246,18 @@ __global__ void inject_kernel(float* ring, const uint64_t* step, uint32_t N, con
     if (i < count) ring[slot * N + idx[i]] += val[i];
 }
 
+// inject_voltage: v += value on state column 0 (the v_jump kernel of
+// PW/shaders/csr_scatter.wgsl:117-124), between two steps.
+__global__ void vjump_kernel(float* v, const uint32_t* idx, const float* val, uint32_t count, int serial) {
+    if (serial) {
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            for (uint32_t i = 0; i < count; ++i) v[idx ? idx[i] : i] += val[i];
+        return;
+    }
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) v[idx ? idx[i] : i] += val[i];
+}
+
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, size_t n) {
     for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
          i += static_cast<size_t>(gridDim.x) * blockDim.x)
@@ -2291,12 +2330,14 @@ void launch_neuron(int model, int sched, const NetArgs* nets, const uint4* items
         if (model == 0) neuron_kernel<0, 2><<<n, kTile, lds, s>>>(nets, items, defer_scatter);
         else if (model == 1) neuron_kernel<1, 2><<<n, kTile, lds, s>>>(nets, items, defer_scatter);
         else if (model == 2) neuron_kernel<2, 2><<<n, kTile, lds, s>>>(nets, items, defer_scatter);
-        else neuron_kernel<3, 2><<<n, kTile, lds, s>>>(nets, items, defer_scatter);
+        else if (model == 3) neuron_kernel<3, 2><<<n, kTile, lds, s>>>(nets, items, defer_scatter);
+        else neuron_kernel<4, 2><<<n, kTile, lds, s>>>(nets, items, defer_scatter);
     } else {
         if (model == 0) neuron_kernel<0, 1><<<n, kTile, lds, s>>>(nets, items, defer_scatter);
         else if (model == 1) neuron_kernel<1, 1><<<n, kTile, lds, s>>>(nets, items, defer_scatter);
         else if (model == 2) neuron_kernel<2, 1><<<n, kTile, lds, s>>>(nets, items, defer_scatter);
-        else neuron_kernel<3, 1><<<n, kTile, lds, s>>>(nets, items, defer_scatter);
+        else if (model == 3) neuron_kernel<3, 1><<<n, kTile, lds, s>>>(nets, items, defer_scatter);
+        else neuron_kernel<4, 1><<<n, kTile, lds, s>>>(nets, items, defer_scatter);
     }
 }
 void launch_dense_trace(const NetArgs* nets, uint32_t n_tiles, float* trace, float decay, hipStream_t s) {
@@ -2334,7 +2375,8 @@ void launch_megastep(int model, const NetArgs* nets, const uint2* items, uint32_
     if (model == 0) megastep_kernel<0><<<n, kApply, 0, s>>>(nets, items);
     else if (model == 1) megastep_kernel<1><<<n, kApply, 0, s>>>(nets, items);
     else if (model == 2) megastep_kernel<2><<<n, kApply, 0, s>>>(nets, items);
-    else megastep_kernel<3><<<n, kApply, 0, s>>>(nets, items);
+    else if (model == 3) megastep_kernel<3><<<n, kApply, 0, s>>>(nets, items);
+    else megastep_kernel<4><<<n, kApply, 0, s>>>(nets, items);
 }
 void launch_fixup(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s) {
     if (n == 0) return;
@@ -2351,6 +2393,11 @@ void launch_inject(float* ring, const uint64_t* step, uint32_t N, const uint32_t
     if (count == 0) return;
     if (serial) inject_kernel<<<1, 1, 0, s>>>(ring, step, N, idx, val, count, 1);
     else inject_kernel<<<(count + 255) / 256, 256, 0, s>>>(ring, step, N, idx, val, count, 0);
+}
+void launch_vjump(float* v, const uint32_t* idx, const float* val, uint32_t count, int serial, hipStream_t s) {
+    if (count == 0) return;
+    if (serial) vjump_kernel<<<1, 1, 0, s>>>(v, idx, val, count, 1);
+    else vjump_kernel<<<(count + 255) / 256, 256, 0, s>>>(v, idx, val, count, 0);
 }
 void launch_tm_init(const uint8_t* type, const float4* prof, float* r, float* u, uint32_t* last, size_t n,
                     hipStream_t s) {

@@ -22,6 +22,9 @@ MODEL_LIF = 0
 MODEL_IZHIKEVICH = 1
 MODEL_HH = 2
 MODEL_GPEA = 3
+MODEL_ALPHA_LIF = 4
+
+POOL_MAX_STEPS_PER_SUBMIT = 64
 
 KURA_EULER = 0
 KURA_RK4 = 1
@@ -60,6 +63,17 @@ class StdpParams(C.Structure):
 
 class StpProfile(C.Structure):
     _fields_ = [("u0", C.c_float), ("tau_rec", C.c_float), ("tau_fac", C.c_float)]
+
+
+class PoolConfig(C.Structure):
+    """NeuronConfig::LifAlpha (crates/SiMuse/crates/pillars-connectome/src/runner.rs:114)."""
+    _fields_ = [(k, C.c_float) for k in ("dt_ms", "tau_ref", "tau_m", "v_rest", "v_thresh", "v_reset", "tau_syn",
+                                          "t_delay")]
+
+
+class PoolSlot(C.Structure):
+    """PoolSlot (crates/dioma-engine/src/brain/dynamics/gpu_pool.rs:5-11)."""
+    _fields_ = [("neuron_off", C.c_uint64), ("n", C.c_uint64), ("syn_off", C.c_uint64), ("n_syn", C.c_uint64)]
 
 
 class SimuseError(RuntimeError):
@@ -107,6 +121,17 @@ _SIGS = {
     "simuse_net_upload_weights": ([P, f32p, C.c_uint64], C.c_int),
     "simuse_net_download_weights": ([P, f32p, C.c_uint64], C.c_int),
     "simuse_net_inject_current": ([P, C.c_int32, u32p, f32p, C.c_uint32], C.c_int),
+    "simuse_net_inject_voltage": ([P, C.c_int32, u32p, f32p, C.c_uint32], C.c_int),
+    "simuse_pool_create": ([C.POINTER(PoolConfig), C.c_int32, C.POINTER(P)], C.c_int),
+    "simuse_pool_destroy": ([P], None),
+    "simuse_pool_register": ([P, C.c_uint32, i32p, i32p, f32p, f32p, C.POINTER(PoolSlot)], C.c_int),
+    "simuse_pool_set_delays": ([P, C.c_uint32, u8p, C.c_uint64], C.c_int),
+    "simuse_pool_n_creatures": ([P, u32p], C.c_int),
+    "simuse_pool_slots": ([P, C.POINTER(PoolSlot), C.c_uint32], C.c_int),
+    "simuse_pool_sync_weights": ([P, C.POINTER(f32p), u8p, C.c_uint32], C.c_int),
+    "simuse_pool_step": ([P, C.POINTER(f32p), C.c_uint32, C.c_uint32, u32p, C.c_uint64], C.c_int),
+    "simuse_pool_download_state": ([P, C.c_int32, f32p, C.c_uint64], C.c_int),
+    "simuse_pool_info": ([P, u64p, u64p, u64p, i32p], C.c_int),
     "simuse_net_reset": ([P, C.POINTER(PopDesc), C.c_int32], C.c_int),
     "simuse_net_set_receptors": ([P, u8p, C.c_float, f32p], C.c_int),
     "simuse_net_download_conductances": ([P, f32p, C.c_uint64], C.c_int),

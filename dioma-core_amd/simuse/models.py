@@ -9,15 +9,19 @@ Columns follow the ``#[state]``/``#[param]`` field order of each model:
              the equations of PC/kernels/hh.cu:107-216
              state [v, m, h, n, refr, c]; params [dt, g_na, g_k, g_leak, e_na, e_k,
              e_leak, inv_cm, q10, v_thresh, g_ca_t, e_ca_t, tau_ref]
+  alpha-LIF  P/src/zoo/neurons/alpha_lif.rs:60-91
+             state [v, g, refr_cnt]; params [dt, time_factor, v_rest, v_thresh,
+             v_reset, refr_steps, syn_decay]
 (crates/SiMuse/crates/ is the prefix of every path above.)
 """
 from __future__ import annotations
 
+import math
 from typing import Sequence, Union
 
 import numpy as np
 
-from ._lib import MODEL_GPEA, MODEL_HH, MODEL_IZHIKEVICH, MODEL_LIF
+from ._lib import MODEL_ALPHA_LIF, MODEL_GPEA, MODEL_HH, MODEL_IZHIKEVICH, MODEL_LIF
 from .network import N_PARAMS, N_STATE, NetworkPop, stride_for
 
 Num = Union[float, np.ndarray]
@@ -45,6 +49,27 @@ def lif_pop(n: int, dt: Num, inv_tau_m: Num, v_rest: Num, v_thresh: Num, v_reset
 def izh_pop(n: int, dt: Num, a: Num, b: Num, c: Num, d: Num, tau_ref: Num, v_thresh: Num, v0: Num,
             u0: Num, refr0: Num = 0.0) -> NetworkPop:
     return make_pop(MODEL_IZHIKEVICH, n, [dt, a, b, c, d, tau_ref, v_thresh], [v0, u0, refr0])
+
+
+def round_half_away(x: np.float32) -> np.float32:
+    """f32::round: halves away from zero (numpy rounds them to even).  An FP32
+    value plus 0.5 is exact in FP64."""
+    x64 = float(x)
+    r = math.floor(x64 + 0.5) if x64 >= 0 else -math.floor(-x64 + 0.5)
+    return np.float32(r)
+
+
+def alpha_lif_pop(n: int, dt: float = 0.5, tau_ref: float = 2.2, tau_m: float = 20.0, v_rest: float = -52.0,
+                  v_thresh: float = -45.0, v_reset: float = -52.0, tau_syn: float = 3.0) -> NetworkPop:
+    """Alpha-LIF population with the derived parameters and initial state of
+    ensure_built (crates/dioma-engine/src/brain/dynamics/gpu_pool.rs:205-240):
+    refr_steps = max(round(tau_ref/dt), 1), time_factor = dt/tau_m,
+    syn_decay = 1 - dt/tau_syn, all in FP32; v = v_rest, g = 0, refr_cnt = refr_steps."""
+    f = np.float32
+    refr = max(round_half_away(f(tau_ref) / f(dt)), f(1.0))
+    tf = f(dt) / f(tau_m)
+    syn_decay = f(1.0) - f(dt) / f(tau_syn)
+    return make_pop(MODEL_ALPHA_LIF, n, [dt, tf, v_rest, v_thresh, v_reset, refr, syn_decay], [v_rest, 0.0, refr])
 
 
 # NEURON reference geometry of the GPEA tests (soma L=20, diam=40 um): area pi*40*20 um^2

@@ -22,8 +22,9 @@ import numpy as np
 from . import _lib
 from ._lib import PopDesc, StdpParams, StpProfile, check, lib
 
-N_STATE = {_lib.MODEL_LIF: 2, _lib.MODEL_IZHIKEVICH: 3, _lib.MODEL_HH: 6, _lib.MODEL_GPEA: 9}
-N_PARAMS = {_lib.MODEL_LIF: 6, _lib.MODEL_IZHIKEVICH: 7, _lib.MODEL_HH: 13, _lib.MODEL_GPEA: 14}
+N_STATE = {_lib.MODEL_LIF: 2, _lib.MODEL_IZHIKEVICH: 3, _lib.MODEL_HH: 6, _lib.MODEL_GPEA: 9, _lib.MODEL_ALPHA_LIF: 3}
+N_PARAMS = {_lib.MODEL_LIF: 6, _lib.MODEL_IZHIKEVICH: 7, _lib.MODEL_HH: 13, _lib.MODEL_GPEA: 14,
+            _lib.MODEL_ALPHA_LIF: 7}
 
 
 def _ptr(a: np.ndarray, ctype):
@@ -239,6 +240,19 @@ class CompiledNetwork:
         if i.size != v.size:
             raise ValueError("inject_current: indices/values length mismatch")
         check(lib().simuse_net_inject_current(self._h, pop, _ptr(i, C.c_uint32), _ptr(v, C.c_float), i.size))
+
+    def inject_voltage(self, pop: int, indices, values) -> None:
+        """WgpuNetwork::inject_voltage (crates/SiMuse/crates/pillars-wgpu/src/network.rs:1632-1659):
+        v += value on local neurons of ``pop`` before the next step; ``indices`` None
+        means one value per neuron of the population."""
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        if indices is None:
+            check(lib().simuse_net_inject_voltage(self._h, pop, None, _ptr(v, C.c_float), v.size))
+            return
+        i = np.ascontiguousarray(indices, dtype=np.uint32)
+        if i.size != v.size:
+            raise ValueError("inject_voltage: indices/values length mismatch")
+        check(lib().simuse_net_inject_voltage(self._h, pop, _ptr(i, C.c_uint32), _ptr(v, C.c_float), i.size))
 
     def reset(self, pops: Sequence[NetworkPop]) -> None:
         self.pops = list(pops)

@@ -12,7 +12,7 @@ from typing import List, Optional, Tuple
 
 import numpy as np
 
-from .models import hh_pop, izh_pop, lif_pop
+from .models import alpha_lif_pop, hh_pop, izh_pop, lif_pop
 from .network import CompiledNetwork, NetworkPop
 
 
@@ -389,3 +389,78 @@ def serving_mix(count: int = 64, rate_hz: float = 200.0, seed: int = 1):
             out.append((t, network_request(_gen_network(model, n, s, float(rng.uniform(1000.0, 10000.0))),
                                            horizon)))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Creature pool (crates/dioma-engine/src/brain/dynamics/gpu_pool.rs)
+# ---------------------------------------------------------------------------
+@dataclass
+class Creature:
+    """A creature brain as ConnectomeData holds it
+    (crates/SiMuse/crates/pillars-connectome/src/data.rs:80-103): CSR with local
+    i32 indices, weights, and a transmitter sign per presynaptic neuron."""
+    row_ptr: np.ndarray
+    col: np.ndarray
+    values: np.ndarray
+    nt: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return int(self.row_ptr.size - 1)
+
+    @property
+    def n_syn(self) -> int:
+        return int(self.col.size)
+
+
+def creature(n: int, p: float = 0.08, seed: int = 1, w_lo: float = 10.0, w_hi: float = 60.0,
+             inhib: float = 0.2) -> Creature:
+    """Random recurrent creature: each ordered pair (i != j) connected with
+    probability p, weights uniform in [w_lo, w_hi), a fraction ``inhib`` of the
+    neurons inhibitory (nt_types -1)."""
+    rng = np.random.default_rng(seed)
+    adj = rng.random((n, n)) < p
+    np.fill_diagonal(adj, False)
+    pre, post = np.nonzero(adj)
+    deg = np.bincount(pre, minlength=n)
+    rp = np.zeros(n + 1, np.int32)
+    np.cumsum(deg, out=rp[1:])
+    w = rng.uniform(w_lo, w_hi, size=post.size).astype(np.float32)
+    nt = np.where(rng.random(n) < inhib, -1.0, 1.0).astype(np.float32)
+    return Creature(rp, post.astype(np.int32), w, nt)
+
+
+def pool_net_spec(creatures: List[Creature], delay: int, delays: Optional[List[np.ndarray]] = None,
+                  config: Optional[dict] = None, steps: int = 0) -> NetSpec:
+    """The pool's block-diagonal network in the engine's (push) form: one
+    alpha-LIF population, weights signed by the presynaptic transmitter
+    (fma(w, sign, acc) = acc + w*sign exactly), and delay 1 + max(d, 1) for a
+    pool delay d: the pull at step s reads step s-1's spikes into slot
+    s + max(d, 1), which the engine reads at step (s - 1) + 1 + max(d, 1)."""
+    cfg = dict(dt_ms=0.5, tau_ref=2.2, tau_m=20.0, v_rest=-52.0, v_thresh=-45.0, v_reset=-52.0, tau_syn=3.0)
+    if config:
+        cfg.update({k: v for k, v in config.items() if k != "t_delay"})
+    n = sum(c.n for c in creatures)
+    pre_l, post_l, w_l, d_l = [], [], [], []
+    off = 0
+    for k, c in enumerate(creatures):
+        deg = np.diff(c.row_ptr)
+        pre = np.repeat(np.arange(c.n), deg) + off
+        pre_l.append(pre)
+        post_l.append(c.col.astype(np.int64) + off)
+        sign = np.where(c.nt < 0, np.float32(-1.0), np.float32(1.0))
+        w_l.append(c.values * sign[pre - off])
+        dd = np.full(c.n_syn, delay, np.int64) if delays is None or delays[k] is None else delays[k].astype(np.int64)
+        d_l.append(1 + np.maximum(dd, 1))
+        off += c.n
+    pre = np.concatenate(pre_l) if pre_l else np.zeros(0, np.int64)
+    d = np.concatenate(d_l) if d_l else np.zeros(0, np.int64)
+    if d.size and d.max() > 31:
+        raise ValueError("pool_net_spec: a pool delay above 30 has no engine counterpart")
+    rp = np.zeros(n + 1, np.uint32)
+    np.cumsum(np.bincount(pre, minlength=n), out=rp[1:])
+    col = np.concatenate(post_l).astype(np.uint32) if post_l else np.zeros(0, np.uint32)
+    w = np.concatenate(w_l).astype(np.float32) if w_l else np.zeros(0, np.float32)
+    pop = alpha_lif_pop(n, dt=cfg["dt_ms"], tau_ref=cfg["tau_ref"], tau_m=cfg["tau_m"], v_rest=cfg["v_rest"],
+                        v_thresh=cfg["v_thresh"], v_reset=cfg["v_reset"], tau_syn=cfg["tau_syn"])
+    return NetSpec("pool", [pop], rp, col, w, d.astype(np.uint8), float(cfg["dt_ms"]), steps)

@@ -48,6 +48,9 @@ extern "C" {
 #define SIMUSE_MODEL_IZHIKEVICH 1 /* P/src/zoo/neurons/izhikevich.rs:30-125: 3 states, 7 params */
 #define SIMUSE_MODEL_HH 2         /* PC/kernels/hh.cu:107-216: 6 states, 13 params */
 #define SIMUSE_MODEL_GPEA 3       /* P/src/zoo/neurons/gpea.rs:266-360: 9 states, 14 params */
+#define SIMUSE_MODEL_ALPHA_LIF 4  /* P/src/zoo/neurons/alpha_lif.rs:38-137: 3 states (v, g, refr_cnt),
+                                     7 params (dt, time_factor, v_rest, v_thresh, v_reset, refr_steps,
+                                     syn_decay); input not clamped */
 
 /* Scatter schedules the autotuner chooses between. */
 #define SIMUSE_SCATTER_AUTO 0
@@ -184,6 +187,16 @@ int simuse_net_download_weights(simuse_net* net, float* w, uint64_t n_syn);
  * adds into the slot the next step reads. */
 int simuse_net_inject_current(simuse_net* net, int32_t pop, const uint32_t* idx,
                               const float* val, uint32_t count);
+/* inject_voltage (WgpuNetwork::inject_voltage, PW/src/network.rs:1632-1659;
+ * kernel v_jump_kernel, PW/shaders/csr_scatter.wgsl:117-124): adds val[k] to
+ * the membrane potential (state column 0) of neuron idx[k] of population pop,
+ * a local index; idx NULL means count == the population's size and neuron k
+ * gets val[k].  The reference defers the jump to the start of the next step,
+ * before that step reads its input; it is applied here at once, between two
+ * steps, which is the same state change (two calls before one step both
+ * apply here, where the reference keeps only the last).  Synchronous.
+ * (PW/ = crates/SiMuse/crates/pillars-wgpu/.) */
+int simuse_net_inject_voltage(simuse_net* net, int32_t pop, const uint32_t* idx, const float* val, uint32_t count);
 /* reset (PC/src/network.rs:827-842): reload state and params, zero the ring,
  * step counter and traces. */
 int simuse_net_reset(simuse_net* net, const simuse_pop_desc* pops, int32_t n_pops);
@@ -454,6 +467,71 @@ int simuse_kura_step_counter(simuse_kura* k, uint64_t* steps);
 /* HIP-event timing of whole steps (all stage kernels of a step). */
 int simuse_kura_set_profiling(simuse_kura* k, int32_t on);
 int simuse_kura_kernel_times(simuse_kura* k, float* ms, int32_t n);
+
+/* ---------------------------------------------------------------------------
+ * Creature pool: many small alpha-LIF brains stepped together as one
+ * block-diagonal network, with the reference's deterministic per-post pull
+ * scatter and delay buckets.  Drop-in for the game engine's BrainPool
+ * (DE/src/brain/dynamics/gpu_pool.rs:1-260) over WgpuNetwork::step_many
+ * (PW/src/network.rs:802-885) and PW/shaders/csr_scatter.wgsl:26-59,117-124.
+ * Per substep: v += ext at the first substep of every submit of up to
+ * SIMUSE_POOL_MAX_STEPS_PER_SUBMIT substeps, input = ring slot, alpha-LIF
+ * update (SIMUSE_MODEL_ALPHA_LIF), then the previous substep's spikes pulled
+ * into ring slot (cur + max(d, 1)) % 32 per post, fma(w, sign(pre), acc) in
+ * ascending source order.  Weights are read at the substep that pulls them,
+ * so a weight sync between two steps applies to the spikes of the last
+ * substep before it, as in the reference.
+ * (DE/ = crates/dioma-engine/, PW/ = crates/SiMuse/crates/pillars-wgpu/.)
+ * ------------------------------------------------------------------------- */
+#define SIMUSE_POOL_MAX_STEPS_PER_SUBMIT 64 /* GPU_MAX_STEPS_PER_SUBMIT, gpu_pool.rs:26 */
+
+typedef struct simuse_pool simuse_pool;
+
+/* NeuronConfig::LifAlpha (crates/SiMuse/crates/pillars-connectome/src/runner.rs:114) */
+typedef struct {
+    float dt_ms, tau_ref, tau_m, v_rest, v_thresh, v_reset, tau_syn, t_delay;
+} simuse_pool_config;
+
+/* PoolSlot, gpu_pool.rs:5-11: a creature's first neuron and synapse */
+typedef struct {
+    uint64_t neuron_off, n, syn_off, n_syn;
+} simuse_pool_slot;
+
+/* BrainPool::new (gpu_pool.rs:57-90).  cfg NULL = pool_config()
+ * (gpu_pool.rs:13-24: dt 0.5, tau_ref 2.2, tau_m 20, v_rest -52, v_thresh -45,
+ * v_reset -52, tau_syn 3, t_delay 1.8).  delay_steps < 0: every synapse
+ * delayed round(t_delay / dt) + 1 steps; >= 0: that value (the DIOMA_GPU_DELAY
+ * override); capped at 31 either way. */
+int simuse_pool_create(const simuse_pool_config* cfg, int32_t delay_steps, simuse_pool** out);
+void simuse_pool_destroy(simuse_pool* p);
+/* BrainPool::register (gpu_pool.rs:101-128): a creature's ConnectomeData CSR
+ * with local indices (row_ptr: n + 1 entries, col_indices and values:
+ * row_ptr[n]) and nt_types (n values; < 0 marks an inhibitory presynaptic
+ * neuron, sign -1, else +1).  As in the reference, the pool is rebuilt at the
+ * next step or weight sync, and every creature restarts from the initial
+ * state (v = v_rest, g = 0, refr_cnt = refr_steps).  At most 2^27 neurons and
+ * 2^32 synapses in all (SIMUSE_E_UNSUPPORTED beyond). */
+int simuse_pool_register(simuse_pool* p, uint32_t n, const int32_t* row_ptr, const int32_t* col_indices,
+                         const float* values, const float* nt_types, simuse_pool_slot* slot);
+/* Per-synapse delays of a creature (n_syn values; the pool's delay until
+ * set), as WgpuNetwork::new takes them (PW/src/network.rs:122-131).  BrainPool
+ * itself always uses the one delay.  Rebuilds at the next step like register. */
+int simuse_pool_set_delays(simuse_pool* p, uint32_t creature, const uint8_t* d, uint64_t n_syn);
+int simuse_pool_n_creatures(simuse_pool* p, uint32_t* n);
+int simuse_pool_slots(simuse_pool* p, simuse_pool_slot* out, uint32_t cap);
+/* BrainPool::sync_weights (gpu_pool.rs:130-148): mirrors[k] (slot k's n_syn
+ * weights) is taken for every creature with dirty[k] != 0. */
+int simuse_pool_sync_weights(simuse_pool* p, const float* const* mirrors, const uint8_t* dirty, uint32_t n_creatures);
+/* BrainPool::step (gpu_pool.rs:150-203): exts[k] is NULL or slot k's n voltage
+ * inputs; spt substeps in submits of up to 64; flat receives the spikes,
+ * spt rows of n_neurons u32 (1 spiked, 0 not), or is NULL to skip the
+ * readback.  Synchronous.  A pool with no creature returns at once. */
+int simuse_pool_step(simuse_pool* p, const float* const* exts, uint32_t n_creatures, uint32_t spt, uint32_t* flat,
+                     uint64_t flat_len);
+/* State column of every neuron: 0 v, 1 g, 2 refr_cnt
+ * (WgpuNetwork::download_states, PW/src/network.rs:1140). */
+int simuse_pool_download_state(simuse_pool* p, int32_t col, float* out, uint64_t n);
+int simuse_pool_info(simuse_pool* p, uint64_t* n_neurons, uint64_t* n_syn, uint64_t* steps, int32_t* delay);
 
 #ifdef __cplusplus
 }

@@ -37,6 +37,16 @@ _SIGS = {
     "orc_net_set_noise": ([P, C.c_int32, f32p, C.c_uint64], C.c_int),
     "orc_net_set_plasticity": ([P, C.POINTER(OrcStdp), u8p], C.c_int),
     "orc_net_inject_current": ([P, C.c_int32, u32p, f32p, C.c_uint32], C.c_int),
+    "orc_net_inject_voltage": ([P, C.c_int32, u32p, f32p, C.c_uint32], C.c_int),
+    "orc_pool_create": ([C.c_void_p, C.c_int32], P),
+    "orc_pool_destroy": ([P], None),
+    "orc_pool_register": ([P, C.c_uint32, i32p, i32p, f32p, f32p], C.c_int32),
+    "orc_pool_set_weights": ([P, C.c_int32, f32p, C.c_uint64], C.c_int),
+    "orc_pool_step": ([P, f32p, C.c_uint32, u32p], C.c_int),
+    "orc_pool_set_delays": ([P, C.c_int32, u8p, C.c_uint64], C.c_int),
+    "orc_pool_state": ([P, C.c_int32, f32p], C.c_int),
+    "orc_pool_n_total": ([P], C.c_uint32),
+    "orc_pool_delay": ([P], C.c_int32),
     "orc_net_step": ([P, C.c_uint32], None),
     "orc_net_stdp_step": ([P, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float], None),
     "orc_net_step_counter": ([P], C.c_uint64),
@@ -247,6 +257,15 @@ class OracleNetwork:
         v = np.ascontiguousarray(val, np.float32)
         assert lib().orc_net_inject_current(self.h, pop, _p(i, C.c_uint32), _p(v, C.c_float), i.size) == 0
 
+    def inject_voltage(self, pop, idx, val):
+        """v += val (idx None: one value per neuron of pop) before the next step."""
+        v = np.ascontiguousarray(val, np.float32)
+        if idx is None:
+            assert lib().orc_net_inject_voltage(self.h, pop, None, _p(v, C.c_float), v.size) == 0
+            return
+        i = np.ascontiguousarray(idx, np.uint32)
+        assert lib().orc_net_inject_voltage(self.h, pop, _p(i, C.c_uint32), _p(v, C.c_float), i.size) == 0
+
     def step_counter(self):
         return int(lib().orc_net_step_counter(self.h))
 
@@ -399,3 +418,80 @@ def isi_cv(raster, dt_s):
     out = np.zeros(n, np.float32)
     lib().orc_isi_cv(_p(r, C.c_int32), n, t, dt_s, _p(out, C.c_float))
     return out
+
+
+class _OrcPoolParams(C.Structure):
+    _fields_ = [(k, C.c_float) for k in ("dt_ms", "tau_ref", "tau_m", "v_rest", "v_thresh", "v_reset", "tau_syn",
+                                          "t_delay")]
+
+
+class OraclePool:
+    """The creature pool (gpu_pool.rs + csr_scatter.wgsl) restated on the CPU:
+    same register / set_weights / step / state calls as the HIP pool, with the
+    tick input given per creature."""
+
+    DEFAULT = dict(dt_ms=0.5, tau_ref=2.2, tau_m=20.0, v_rest=-52.0, v_thresh=-45.0, v_reset=-52.0, tau_syn=3.0,
+                   t_delay=1.8)
+
+    def __init__(self, config=None, delay_steps=-1):
+        cfg = dict(self.DEFAULT)
+        if config:
+            cfg.update(config)
+        p = _OrcPoolParams(**{k: float(v) for k, v in cfg.items()})
+        self.h = lib().orc_pool_create(C.byref(p), int(delay_steps))
+        self.slots = []
+
+    def close(self):
+        if self.h:
+            lib().orc_pool_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def register(self, row_ptr, col, values, nt):
+        rp = np.ascontiguousarray(row_ptr, np.int32)
+        c = np.ascontiguousarray(col, np.int32)
+        w = np.ascontiguousarray(values, np.float32)
+        t = np.ascontiguousarray(nt, np.float32)
+        n = rp.size - 1
+        k = lib().orc_pool_register(self.h, n, _p(rp, C.c_int32), _p(c, C.c_int32), _p(w, C.c_float),
+                                    _p(t, C.c_float))
+        off = sum(s[1] for s in self.slots)
+        soff = sum(s[3] for s in self.slots)
+        self.slots.append((off, n, soff, int(rp[-1])))
+        return k
+
+    @property
+    def n(self):
+        return int(lib().orc_pool_n_total(self.h))
+
+    @property
+    def delay(self):
+        return int(lib().orc_pool_delay(self.h))
+
+    def set_weights(self, k, w):
+        a = np.ascontiguousarray(w, np.float32)
+        assert lib().orc_pool_set_weights(self.h, int(k), _p(a, C.c_float), a.size) == 0
+
+    def set_delays(self, k, d):
+        a = np.ascontiguousarray(d, np.uint8)
+        assert lib().orc_pool_set_delays(self.h, int(k), _p(a, C.c_uint8), a.size) == 0
+
+    def step(self, exts, spt):
+        n = self.n
+        ext = np.zeros(max(n, 1), np.float32)
+        for (off, m, _, _), e in zip(self.slots, exts):
+            if e is not None:
+                ext[off: off + m] = np.asarray(e, np.float32)
+        out = np.zeros(max(1, spt * n), np.uint32)
+        assert lib().orc_pool_step(self.h, _p(ext, C.c_float), int(spt), _p(out, C.c_uint32)) == 0
+        return out[: spt * n]
+
+    def state(self, col):
+        out = np.zeros(max(self.n, 1), np.float32)
+        assert lib().orc_pool_state(self.h, int(col), _p(out, C.c_float)) == 0
+        return out[: self.n]

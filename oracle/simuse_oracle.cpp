@@ -330,6 +330,27 @@ static inline int lif_step(R& v, R& refr, R I, R dt, R inv_tau, R v_rest, R v_th
 
 // IzhikevichModel::step — P/src/zoo/neurons/izhikevich.rs:71-125.
 // params: [0]dt [1]a [2]b [3]c [4]d [5]tau_ref [6]v_thresh
+// AlphaLifModel::step — P/src/zoo/neurons/alpha_lif.rs:99-136 (the scalar
+// runner of pillars-connectome, runner.rs:1511-1600, in SIMD form):
+// refractoriness masks only the delayed input, v integrates with the
+// previous g through mul_add, strict threshold, a spike resets v and g.
+// state: [0]v [1]g [2]refr_cnt; params: [1]time_factor [2]v_rest [3]v_thresh
+// [4]v_reset [5]refr_steps [6]syn_decay.  #[no_clamp].
+template <class R>
+static inline int alpha_lif_step(R& v, R& g, R& refr_cnt, R I, R tf, R v_rest, R v_th, R v_reset, R refr_steps,
+                                 R syn_decay) {
+    const R g_old = g;
+    const R delayed = refr_cnt >= refr_steps ? I : R(0);
+    const R integrand = g_old - (v - v_rest);
+    const R v_new = std::fma(tf, integrand, v);
+    const bool spike = v_new > v_th;
+    v = spike ? v_reset : v_new;
+    const R g_new = std::fma(syn_decay, g_old, delayed);
+    g = spike ? R(0) : g_new;
+    refr_cnt = spike ? R(0) : refr_cnt + R(1);
+    return spike ? 1 : 0;
+}
+
 // state:  [0]v [1]u [2]refr
 template <class R>
 static inline int izh_step(R& v, R& u, R& refr, R I, R dt, R a, R b, R c, R d,
@@ -507,10 +528,12 @@ struct Gpea {
 };
 
 static inline int n_state(int model) {
-    return model == ORC_MODEL_LIF ? 2 : model == ORC_MODEL_IZHIKEVICH ? 3 : model == ORC_MODEL_HH ? 6 : 9;
+    return model == ORC_MODEL_LIF ? 2 : model == ORC_MODEL_IZHIKEVICH ? 3 : model == ORC_MODEL_HH ? 6
+         : model == ORC_MODEL_ALPHA_LIF ? 3 : 9;
 }
 static inline int n_param(int model) {
-    return model == ORC_MODEL_LIF ? 6 : model == ORC_MODEL_IZHIKEVICH ? 7 : model == ORC_MODEL_HH ? 13 : 14;
+    return model == ORC_MODEL_LIF ? 6 : model == ORC_MODEL_IZHIKEVICH ? 7 : model == ORC_MODEL_HH ? 13
+         : model == ORC_MODEL_ALPHA_LIF ? 7 : 14;
 }
 static inline bool clamps(int model) {  // #[no_clamp] on HH (hh.rs:13) and GPEA (gpea.rs:213)
     return model == ORC_MODEL_LIF || model == ORC_MODEL_IZHIKEVICH;
@@ -529,6 +552,7 @@ struct NetBase {
     virtual int set_stp(const orc_stp_profile* profiles, int n_profiles, float dt, const uint8_t* syn_types,
                         const uint8_t* src_types, uint64_t n_src_edges) = 0;
     virtual int inject(int pop, const uint32_t* idx, const float* val, uint32_t count) = 0;
+    virtual int inject_voltage(int pop, const uint32_t* idx, const float* val, uint32_t count) = 0;
     virtual int set_receptors(const uint8_t* types, float scale, const float* ext) = 0;
     virtual void conductances(double* out) const = 0;
     // Overwrite the dynamic state with a (device) snapshot: FP32 values are
@@ -795,6 +819,20 @@ struct Net final : NetBase {
         return 0;
     }
 
+    int inject_voltage(int p, const uint32_t* idx, const float* val, uint32_t count) override {
+        // WgpuNetwork::inject_voltage + v_jump_kernel, PW/src/network.rs:1632-1659,
+        // PW/shaders/csr_scatter.wgsl:117-124: v += value before the next step
+        if (p < 0 || p >= static_cast<int>(pops.size())) return -3;
+        Pop& P = pops[p];
+        if (!idx && count != P.n) return -3;
+        for (uint32_t k = 0; k < count; ++k) {
+            const uint32_t i = idx ? idx[k] : k;
+            if (i >= P.n) return -3;
+            P.st[i] = P.st[i] + R(val[k]);
+        }
+        return 0;
+    }
+
     void step_pop(Pop& P) {
         const uint32_t cap = P.cap;
         // dt broadcast from param column 0 of neuron 0 (population.rs:307,361)
@@ -810,7 +848,10 @@ struct Net final : NetBase {
             R* s = P.st.data();
             const R* pr = P.pr.data();
             int spike;
-            if (P.model == ORC_MODEL_LIF) {
+            if (P.model == ORC_MODEL_ALPHA_LIF) {
+                spike = alpha_lif_step<R>(s[i], s[cap + i], s[2 * cap + i], I, pr[cap + i], pr[2 * cap + i],
+                                          pr[3 * cap + i], pr[4 * cap + i], pr[5 * cap + i], pr[6 * cap + i]);
+            } else if (P.model == ORC_MODEL_LIF) {
                 spike = lif_step<R>(s[i], s[cap + i], I, dt, pr[cap + i], pr[2 * cap + i],
                                     pr[3 * cap + i], pr[4 * cap + i], pr[5 * cap + i]);
             } else if (P.model == ORC_MODEL_IZHIKEVICH) {
@@ -1104,6 +1145,90 @@ struct Kura final : KuraBase {
 
 using namespace orc;
 
+// ---------------------------------------------------------------------------
+// Creature pool: BrainPool (DE/src/brain/dynamics/gpu_pool.rs) over the
+// WgpuNetwork step (PW/src/network.rs:802-885) and its shaders
+// (PW/shaders/csr_scatter.wgsl), restated loop for loop in FP32.
+// ---------------------------------------------------------------------------
+struct OraclePool {
+    orc_pool_params c;
+    uint8_t delay;
+    // BrainPool's concatenated CSR (gpu_pool.rs:101-128)
+    std::vector<uint32_t> row_ptr{0}, col;
+    std::vector<float> values, nt;
+    std::vector<uint8_t> delays;
+    std::vector<uint32_t> slot_off, slot_n, slot_syn;
+    uint32_t n = 0;
+    bool built = false;
+    // WgpuNetwork state
+    std::vector<uint32_t> rev_ptr, rev_src, rev_edge, rev_d;
+    std::vector<float> ring, i_in, v, g, refr;
+    std::vector<uint32_t> spikes, spikes_prev;
+    uint32_t slot = 0;
+    float refr_steps = 0, tf = 0, syn_decay = 0;
+
+    void build() {  // ensure_built, gpu_pool.rs:205-260, and WgpuNetwork::new
+        if (built) return;
+        refr_steps = std::max(std::round(c.tau_ref / c.dt_ms), 1.0f);
+        tf = c.dt_ms / c.tau_m;
+        syn_decay = 1.0f - c.dt_ms / c.tau_syn;
+        v.assign(n, c.v_rest);
+        g.assign(n, 0.0f);
+        refr.assign(n, refr_steps);
+        const size_t ns = col.size();
+        rev_ptr.assign(n + 1, 0);
+        for (uint32_t x : col) rev_ptr[x + 1] += 1;
+        for (uint32_t i = 0; i < n; ++i) rev_ptr[i + 1] += rev_ptr[i];
+        std::vector<uint32_t> cursor(rev_ptr);
+        rev_src.assign(ns, 0);
+        rev_edge.assign(ns, 0);
+        rev_d.assign(ns, 0);
+        for (uint32_t s = 0; s < n; ++s)
+            for (uint32_t e = row_ptr[s]; e < row_ptr[s + 1]; ++e) {
+                const uint32_t k = cursor[col[e]]++;
+                rev_src[k] = s;
+                rev_edge[k] = e;
+                rev_d[k] = delays[e];
+            }
+        ring.assign(static_cast<size_t>(kSlots) * n, 0.0f);
+        i_in.assign(n, 0.0f);
+        spikes.assign(n, 0);
+        spikes_prev.assign(n, 0);
+        slot = 0;
+        built = true;
+    }
+
+    void substep(const std::vector<float>* jump) {
+        if (jump)  // v_jump_kernel: v += value
+            for (uint32_t i = 0; i < n; ++i) v[i] += (*jump)[i];
+        for (uint32_t i = 0; i < n; ++i) i_in[i] = ring[static_cast<size_t>(slot) * n + i];  // copy_slot_kernel
+        for (uint32_t i = 0; i < n; ++i) {
+            float vv = v[i], gg = g[i], rr = refr[i];
+            spikes[i] = static_cast<uint32_t>(
+                alpha_lif_step<float>(vv, gg, rr, i_in[i], tf, c.v_rest, c.v_thresh, c.v_reset, refr_steps, syn_decay));
+            v[i] = vv;
+            g[i] = gg;
+            refr[i] = rr;
+        }
+        if (!col.empty()) {  // scatter_kernel over the previous step's spikes
+            for (uint32_t post = 0; post < n; ++post) {
+                ring[static_cast<size_t>(slot) * n + post] = 0.0f;
+                float acc[kSlots] = {};
+                for (uint32_t k = rev_ptr[post]; k < rev_ptr[post + 1]; ++k) {
+                    const uint32_t src = rev_src[k];
+                    if (spikes_prev[src] == 0) continue;
+                    const uint32_t d = std::max(rev_d[k], 1u);
+                    acc[d % kSlots] = std::fma(values[rev_edge[k]], nt[src], acc[d % kSlots]);
+                }
+                for (uint32_t d = 0; d < kSlots; ++d)
+                    if (acc[d] != 0.0f) ring[static_cast<size_t>((slot + d) % kSlots) * n + post] += acc[d];
+            }
+        }
+        spikes_prev = spikes;
+        slot = (slot + 1) % kSlots;
+    }
+};
+
 extern "C" {
 
 void* orc_net_create(int kind, const orc_pop_desc* pops, int32_t n_pops, const uint32_t* row_ptr,
@@ -1135,6 +1260,9 @@ int orc_net_set_plasticity(void* net, const orc_stdp_params* p, const uint8_t* p
 }
 int orc_net_inject_current(void* net, int32_t pop, const uint32_t* idx, const float* val, uint32_t count) {
     return static_cast<NetBase*>(net)->inject(pop, idx, val, count);
+}
+int orc_net_inject_voltage(void* net, int32_t pop, const uint32_t* idx, const float* val, uint32_t count) {
+    return static_cast<NetBase*>(net)->inject_voltage(pop, idx, val, count);
 }
 void orc_net_step(void* net, uint32_t n_steps) { static_cast<NetBase*>(net)->step(n_steps); }
 int orc_net_set_receptors(void* net, const uint8_t* types, float scale, const float* ext) {
@@ -1282,5 +1410,78 @@ void orc_isi_cv(const int32_t* raster, uint32_t n, uint32_t steps, float dt_s, f
         }
     }
 }
+
+void* orc_pool_create(const orc_pool_params* p, int32_t delay) {
+    auto* P = new OraclePool();
+    P->c = *p;
+    // BrainPool::new, gpu_pool.rs:63-90: steps_delay + 1, capped at 31
+    const float sd = std::max(std::round(p->t_delay / p->dt_ms), 0.0f);
+    const uint32_t d = delay >= 0 ? static_cast<uint32_t>(delay) : static_cast<uint32_t>(sd) + 1u;
+    P->delay = static_cast<uint8_t>(std::min(d, 31u));
+    return P;
+}
+void orc_pool_destroy(void* pool) { delete static_cast<OraclePool*>(pool); }
+int32_t orc_pool_register(void* pool, uint32_t n, const int32_t* row_ptr, const int32_t* col, const float* values,
+                          const float* nt_types) {
+    auto* P = static_cast<OraclePool*>(pool);
+    const uint32_t off = P->n, syn_off = static_cast<uint32_t>(P->values.size());
+    const uint32_t ns = static_cast<uint32_t>(row_ptr[n]);
+    P->row_ptr.pop_back();
+    for (uint32_t i = 0; i <= n; ++i) P->row_ptr.push_back(static_cast<uint32_t>(row_ptr[i]) + syn_off);
+    for (uint32_t e = 0; e < ns; ++e) P->col.push_back(static_cast<uint32_t>(col[e]) + off);
+    P->values.insert(P->values.end(), values, values + ns);
+    P->delays.insert(P->delays.end(), ns, P->delay);
+    for (uint32_t i = 0; i < n; ++i) P->nt.push_back(nt_types[i] < 0.0f ? -1.0f : 1.0f);
+    P->slot_off.push_back(off);
+    P->slot_n.push_back(n);
+    P->slot_syn.push_back(syn_off);
+    P->n += n;
+    P->built = false;  // needs_rebuild: every creature starts over
+    return static_cast<int32_t>(P->slot_off.size() - 1);
+}
+int orc_pool_set_weights(void* pool, int32_t k, const float* w, uint64_t n_syn) {
+    auto* P = static_cast<OraclePool*>(pool);
+    if (k < 0 || k >= static_cast<int32_t>(P->slot_off.size())) return -3;
+    const uint32_t s0 = P->slot_syn[k];
+    const uint32_t s1 = k + 1 < static_cast<int32_t>(P->slot_syn.size()) ? P->slot_syn[k + 1]
+                                                                          : static_cast<uint32_t>(P->values.size());
+    if (n_syn != s1 - s0) return -3;
+    if (P->n) P->build();
+    std::copy(w, w + n_syn, P->values.begin() + s0);
+    return 0;
+}
+int orc_pool_set_delays(void* pool, int32_t k, const uint8_t* d, uint64_t n_syn) {
+    auto* P = static_cast<OraclePool*>(pool);
+    if (k < 0 || k >= static_cast<int32_t>(P->slot_off.size())) return -3;
+    const uint32_t s0 = P->slot_syn[k];
+    const uint32_t s1 = k + 1 < static_cast<int32_t>(P->slot_syn.size()) ? P->slot_syn[k + 1]
+                                                                          : static_cast<uint32_t>(P->values.size());
+    if (n_syn != s1 - s0) return -3;
+    std::copy(d, d + n_syn, P->delays.begin() + s0);
+    P->built = false;  // WgpuNetwork::new takes the delays: a new network
+    return 0;
+}
+int orc_pool_step(void* pool, const float* ext, uint32_t spt, uint32_t* out) {
+    auto* P = static_cast<OraclePool*>(pool);
+    if (P->slot_off.empty()) return 0;
+    P->build();
+    std::vector<float> jump(P->n, 0.0f);
+    if (ext) std::copy(ext, ext + P->n, jump.begin());
+    for (uint32_t k = 0; k < spt; ++k) {
+        P->substep(k % 64 == 0 ? &jump : nullptr);  // one v_jump per submit of <= 64 steps
+        std::copy(P->spikes.begin(), P->spikes.end(), out + static_cast<size_t>(k) * P->n);
+    }
+    return 0;
+}
+int orc_pool_state(void* pool, int32_t col, float* out) {
+    auto* P = static_cast<OraclePool*>(pool);
+    if (P->n) P->build();
+    const std::vector<float>* s = col == 0 ? &P->v : col == 1 ? &P->g : col == 2 ? &P->refr : nullptr;
+    if (!s) return -3;
+    std::copy(s->begin(), s->end(), out);
+    return 0;
+}
+uint32_t orc_pool_n_total(void* pool) { return static_cast<OraclePool*>(pool)->n; }
+int32_t orc_pool_delay(void* pool) { return static_cast<OraclePool*>(pool)->delay; }
 
 }  // extern "C"

@@ -29,6 +29,7 @@ extern "C" {
 #define ORC_MODEL_IZHIKEVICH 1
 #define ORC_MODEL_HH 2
 #define ORC_MODEL_GPEA 3
+#define ORC_MODEL_ALPHA_LIF 4
 
 /* One population: column-major SoA with host stride ceil8(n)
  * (P/src/core/exec/population.rs:106-131). */
@@ -86,6 +87,9 @@ void orc_rec_decay_factors(float dt, float* out);  /* 7 values */
 float orc_nmda_block(float v);
 /* compute_currents: i_out[i] += receptor currents of g[7][n] at v[i] */
 void orc_rec_currents(const float* g, const float* v, float* i_out, uint32_t n);
+/* v += val[k] on state column 0 of neuron idx[k] of pop (idx NULL: count ==
+ * the population's size, neuron k gets val[k]), before the next step. */
+int orc_net_inject_voltage(void* net, int32_t pop, const uint32_t* idx, const float* val, uint32_t count);
 int orc_net_inject_current(void* net, int32_t pop, const uint32_t* idx,
                            const float* val, uint32_t count);
 void orc_net_step(void* net, uint32_t n_steps);
@@ -136,6 +140,33 @@ void orc_kura_step_rk4(void* k, uint32_t n_steps);
 void orc_kura_step_euler(void* k, uint32_t n_steps);
 void orc_kura_phases(void* k, double* out);
 double orc_kura_order(void* k);
+
+/* Creature pool of the game engine (crates/dioma-engine/src/brain/dynamics/
+ * gpu_pool.rs) restated with the reference's own per-post pull scatter
+ * (crates/SiMuse/crates/pillars-wgpu/shaders/csr_scatter.wgsl:26-59 over the
+ * in-edge CSR of crates/SiMuse/crates/pillars-wgpu/src/network.rs:231-257):
+ * creatures' CSR blocks concatenated block-diagonally, alpha-LIF neurons,
+ * weights signed by the presynaptic neuron's transmitter, one axonal delay
+ * (delay < 0: min(round(t_delay/dt) + 1, 31)). */
+typedef struct {
+    float dt_ms, tau_ref, tau_m, v_rest, v_thresh, v_reset, tau_syn, t_delay;
+} orc_pool_params;
+void* orc_pool_create(const orc_pool_params* p, int32_t delay);
+void orc_pool_destroy(void* pool);
+/* Appends a creature (its CSR with local indices); returns its index. */
+int32_t orc_pool_register(void* pool, uint32_t n, const int32_t* row_ptr, const int32_t* col,
+                          const float* values, const float* nt_types);
+int orc_pool_set_weights(void* pool, int32_t creature, const float* w, uint64_t n_syn);
+/* per-synapse delays of a creature (a new network, as WgpuNetwork::new takes them) */
+int orc_pool_set_delays(void* pool, int32_t creature, const uint8_t* d, uint64_t n_syn);
+/* spt substeps; ext (n_total values, NULL = zeros) is added to v once before
+ * the first substep of every run of up to 64 substeps; spikes of each substep
+ * into out[spt][n_total] (0/1). */
+int orc_pool_step(void* pool, const float* ext, uint32_t spt, uint32_t* out);
+/* state column (0 v, 1 g, 2 refr_cnt) of every neuron */
+int orc_pool_state(void* pool, int32_t col, float* out);
+uint32_t orc_pool_n_total(void* pool);
+int32_t orc_pool_delay(void* pool);
 
 /* Statistics (P/src/kernels/stats/rates.rs:6-35, isi.rs:30-67). */
 void orc_firing_rates(const int32_t* raster, uint32_t n, uint32_t steps, float dt_s, float* out);
