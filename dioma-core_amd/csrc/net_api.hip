@@ -9,6 +9,7 @@
 // before every replay, PC/src/network.rs:431-460).
 
 #include <cmath>
+#include <mutex>
 
 #include "device_math.hpp"
 #include "engine.hpp"
@@ -1346,6 +1347,94 @@ int simuse_net_raster_stats(simuse_net* net, uint32_t first, uint32_t count, flo
     int rc = check_net(net);
     if (rc) return rc;
     return net->st.raster_stats(first, count, dt_s, rates, cvs, net->eng.stream);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Raster statistics without allocation.  simuse_net_raster_stats stages the
+// raster word and bit of each neuron and the two outputs in four buffers it
+// allocates and frees per call; this variant keeps them in one file-scope
+// device array, scratch = [word_of | bit_of | rates | cvs] of
+// SIMUSE_RASTER_STATS_NOALLOC_MAX entries each (1 MiB, every GPU the library
+// runs on holds its own copy until the process ends), and derives word and
+// bit on the device from the population table instead of uploading them.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr uint32_t kStatsMax = SIMUSE_RASTER_STATS_NOALLOC_MAX;
+static __device__ __align__(256) uint32_t scratch[4 * kStatsMax];
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;                       // serialises every use of scratch in the process
+uint32_t* g_scratch[kScratchDevices] = {};     // device address of scratch, per device
+
+// Device address of the current device's scratch (the name `scratch` is a
+// host-side stand-in, not a device address).  Caller holds g_scratch_mu.
+int scratch_of_current_device(uint32_t** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchDevices)
+        return fail(SIMUSE_E_UNSUPPORTED, "raster_stats_noalloc: device %d beyond the %d this library tracks", dev,
+                    kScratchDevices);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return fail(SIMUSE_E_HIP, "raster_stats_noalloc: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        }
+        g_scratch[dev] = static_cast<uint32_t*>(p);
+    }
+    *out = g_scratch[dev];
+    return SIMUSE_OK;
+}
+
+// Raster word and bit of neurons first .. first+count-1: the population
+// holding global neuron gi is the last one with off <= gi.
+__global__ void raster_index_kernel(const PopDev* pops, int32_t n_pops, uint32_t first, uint32_t count,
+                                    uint32_t* word_of, uint32_t* bit_of) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const uint32_t gi = first + k;
+    int32_t p = 0;
+    while (p + 1 < n_pops && gi >= pops[p + 1].off) ++p;
+    const uint32_t li = gi - pops[p].off;
+    word_of[k] = pops[p].word0 + li / 64;
+    bit_of[k] = li % 64;
+}
+}  // namespace
+
+extern "C" {
+
+int simuse_net_raster_stats_noalloc(simuse_net* net, uint32_t first, uint32_t count, float dt_s, float* rates,
+                                    float* cvs) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    NetState& s = net->st;
+    if (!s.raster) return fail(SIMUSE_E_STATE, "raster_stats_noalloc: recording is off");
+    if (static_cast<uint64_t>(first) + count > s.N)
+        return fail(SIMUSE_E_INVALID, "raster_stats_noalloc: neurons [%u, %llu) outside the network's %u", first,
+                    static_cast<unsigned long long>(first) + count, s.N);
+    if (count > kStatsMax)
+        return fail(SIMUSE_E_INVALID, "raster_stats_noalloc: %u neurons in one call, at most %u", count, kStatsMax);
+    if (count == 0) return SIMUSE_OK;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    uint32_t* base = nullptr;
+    if ((rc = scratch_of_current_device(&base))) return rc;
+    uint32_t* word_of = base;
+    uint32_t* bit_of = base + kStatsMax;
+    float* r = reinterpret_cast<float*>(base + 2 * kStatsMax);
+    float* c = reinterpret_cast<float*>(base + 3 * kStatsMax);
+    const uint32_t steps = static_cast<uint32_t>(std::min<uint64_t>(s.host_step - s.raster_start, s.raster_cap));
+    hipStream_t st = net->eng.stream;
+    raster_index_kernel<<<(count + 255) / 256, 256, 0, st>>>(s.d_pops, static_cast<int32_t>(s.pops.size()), first,
+                                                             count, word_of, bit_of);
+    HIP_TRY(hipGetLastError());
+    launch_raster_stats(s.raster, s.n_words, steps, word_of, bit_of, count, dt_s, r, c, st);
+    HIP_TRY(hipGetLastError());
+    if (rates) HIP_TRY(hipMemcpyAsync(rates, r, 4 * static_cast<size_t>(count), hipMemcpyDeviceToHost, st));
+    if (cvs) HIP_TRY(hipMemcpyAsync(cvs, c, 4 * static_cast<size_t>(count), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return SIMUSE_OK;
 }
 
 int simuse_net_set_profiling(simuse_net* net, int32_t on) {

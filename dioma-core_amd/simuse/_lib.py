@@ -25,6 +25,7 @@ MODEL_GPEA = 3
 MODEL_ALPHA_LIF = 4
 
 POOL_MAX_STEPS_PER_SUBMIT = 64
+RASTER_STATS_NOALLOC_MAX = 65536
 
 KURA_EULER = 0
 KURA_RK4 = 1
@@ -146,6 +147,7 @@ _SIGS = {
     "simuse_net_raster_words": ([P, u64p], C.c_int),
     "simuse_net_download_raster": ([P, u64p, C.c_uint64, u32p], C.c_int),
     "simuse_net_raster_stats": ([P, C.c_uint32, C.c_uint32, C.c_float, f32p, f32p], C.c_int),
+    "simuse_net_raster_stats_noalloc": ([P, C.c_uint32, C.c_uint32, C.c_float, f32p, f32p], C.c_int),
     "simuse_net_device_bytes": ([P, u64p], C.c_int),
     "simuse_net_step_form": ([P, i32p], C.c_int),
     "simuse_net_set_profiling": ([P, C.c_int32], C.c_int),

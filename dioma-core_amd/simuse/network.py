@@ -355,6 +355,15 @@ class CompiledNetwork:
                                             _ptr(cvs, C.c_float)))
         return rates, cvs
 
+    def raster_stats_noalloc(self, first: int, count: int, dt_s: float):
+        """raster_stats through the entry point that allocates no device memory
+        (at most RASTER_STATS_NOALLOC_MAX neurons per call)."""
+        rates = np.zeros(count, dtype=np.float32)
+        cvs = np.zeros(count, dtype=np.float32)
+        check(lib().simuse_net_raster_stats_noalloc(self._h, first, count, float(dt_s), _ptr(rates, C.c_float),
+                                                    _ptr(cvs, C.c_float)))
+        return rates, cvs
+
     # -- profiling -------------------------------------------------------------------
     def set_profiling(self, on: bool) -> None:
         check(lib().simuse_net_set_profiling(self._h, 1 if on else 0))

@@ -260,6 +260,20 @@ int simuse_net_download_raster(simuse_net* net, uint64_t* out, uint64_t n_words,
  * (P/src/kernels/stats/isi.rs:30-67), computed on the device. */
 int simuse_net_raster_stats(simuse_net* net, uint32_t first, uint32_t count, float dt_s,
                             float* rates, float* cvs);
+/* No counterpart in the reference.  simuse_net_raster_stats without any
+ * device allocation, for callers that cannot allocate (tight serving loops,
+ * harnesses that forbid hipMalloc): the staging buffers that call allocates
+ * and frees live instead in one file-scope device array holding
+ * SIMUSE_RASTER_STATS_NOALLOC_MAX neurons (1 MiB; each GPU the library runs
+ * on keeps its copy until the process ends).  count above that maximum
+ * returns SIMUSE_E_INVALID.  Same results as simuse_net_raster_stats.  Every
+ * call on a GPU shares that GPU's array and leaves its contents there; the
+ * library serialises these calls within a process (calls that overlapped on
+ * one GPU would race), and each call finishes its work before returning.
+ * Call with the network's device current. */
+#define SIMUSE_RASTER_STATS_NOALLOC_MAX 65536
+int simuse_net_raster_stats_noalloc(simuse_net* net, uint32_t first, uint32_t count, float dt_s,
+                                    float* rates, float* cvs);
 
 /* Bytes of device memory the handle holds. */
 int simuse_net_device_bytes(simuse_net* net, uint64_t* bytes);
