@@ -45,6 +45,7 @@ void launch_megastep(int model, const NetArgs* nets, const uint2* items, uint32_
 void launch_head_clear(unsigned long long* head, const uint64_t* step, size_t n, hipStream_t s);
 void launch_inject(float* ring, const uint64_t* step, uint32_t N, const uint32_t* idx, const float* val,
                    uint32_t count, int serial, hipStream_t s);
+void launch_spacer(uint64_t ticks, hipStream_t s);
 void launch_vjump(float* v, const uint32_t* idx, const float* val, uint32_t count, int serial, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 void launch_tm_init(const uint8_t* type, const float4* prof, float* r, float* u, uint32_t* last, size_t n,

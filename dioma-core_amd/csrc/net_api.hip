@@ -1449,7 +1449,7 @@ int simuse_net_kernel_times(simuse_net* net, float* ms, int32_t n) {
     return net->eng.kernel_times(ms, n);
 }
 
-int simuse_net_time_steps(simuse_net* net, uint32_t n, float* ms) {
+static int time_steps_impl(simuse_net* net, uint32_t n, float* ms, bool queued) {
     int rc = check_net(net);
     if (rc) return rc;
     if (!ms) return fail(SIMUSE_E_INVALID, "null output");
@@ -1466,7 +1466,17 @@ int simuse_net_time_steps(simuse_net* net, uint32_t n, float* ms) {
             (void)hipEventDestroy(e[0]);
             return fail(SIMUSE_E_HIP, "hipEventCreate failed");
         }
-        hipError_t he = hipEventRecord(e[0], net->eng.stream);
+        hipError_t he = hipSuccess;
+        if (queued) {
+            // 200 us of the constant-rate wall clock ahead of the first event
+            int khz = 0;
+            he = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0);
+            if (he == hipSuccess) {
+                launch_spacer(static_cast<uint64_t>(khz > 0 ? khz : 100000) / 5, net->eng.stream);
+                he = hipGetLastError();
+            }
+        }
+        if (he == hipSuccess) he = hipEventRecord(e[0], net->eng.stream);
         if (he == hipSuccess) r = net->eng.step_async(n);
         if (he == hipSuccess && r == SIMUSE_OK) he = hipEventRecord(e[1], net->eng.stream);
         if (he == hipSuccess && r == SIMUSE_OK) he = hipEventSynchronize(e[1]);
@@ -1478,6 +1488,9 @@ int simuse_net_time_steps(simuse_net* net, uint32_t n, float* ms) {
         return SIMUSE_OK;
     });
 }
+
+int simuse_net_time_steps(simuse_net* net, uint32_t n, float* ms) { return time_steps_impl(net, n, ms, false); }
+int simuse_net_time_steps_queued(simuse_net* net, uint32_t n, float* ms) { return time_steps_impl(net, n, ms, true); }
 
 int simuse_net_step_form(simuse_net* net, int32_t* form) {
     int rc = check_net(net);

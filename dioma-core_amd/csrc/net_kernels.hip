@@ -1774,7 +1774,9 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
     if (it.x == kEmpty) return;  // pad of the XCD-ordered list
     const NetArgs& a = nets[it.x];
     const uint32_t k = it.y, t = threadIdx.x, lane = t & 63u, wq = (t / 64u) % kWavesPerTile, tl = t % kTile;
+    // the step counter and the materialised flag in one batch of scalar loads
     const uint64_t step = *a.step;
+    const uint32_t mat = *a.bin_mat;
     if (step >= a.step_limit) return;
     const uint32_t wpar = static_cast<uint32_t>(step & 1u);
 
@@ -1790,12 +1792,11 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
         return tile < a.n_tiles && a.n_pops > 1 ? ((cptr<uint32_t>)a.tile_pop)[tile] : 0u;
     };
     auto pop_dev = [&](uint32_t p) -> PopDev { return ((cptr<PopDev>)a.pops)[p]; };
-    bin_map_load(a, M);
     float cur[2] = {0.0f, 0.0f};
     PopDev PV[2];
     uint32_t li[2], tile[2], pidx[2];
     bool on[2];
-    if (*a.bin_mat) {
+    if (mat) {
         // a host call (current injection, ring download) has applied them
         float* ring_slot = a.ring + static_cast<size_t>(step & (kSlots - 1)) * a.N;
 #pragma unroll
@@ -1834,6 +1835,9 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
         row0[q] = deg[q] = 0u;
         if (on[q]) nrn[q].load(PV[q], li[q]);
     }
+    // the bin map (read by the writer phase) loads behind the state, not in
+    // front of wave 0's header load; valid after the barrier below
+    bin_map_load(a, M);
 
     // Gaussian input: the thread's two neurons are li and li + 1024 of one
     // population whenever its bin lies inside that population, and then
@@ -2258,6 +2262,13 @@ __global__ void vjump_kernel(float* v, const uint32_t* idx, const float* val, ui
     if (i < count) v[idx ? idx[i] : i] += val[i];
 }
 
+// Holds the stream for `ticks` of the constant-rate wall clock: work queued
+// behind it starts without waiting for the host's submission.
+__global__ void spacer_kernel(uint64_t ticks) {
+    const uint64_t t0 = wall_clock64();
+    while (wall_clock64() - t0 < ticks) __builtin_amdgcn_s_sleep(8);
+}
+
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, size_t n) {
     for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
          i += static_cast<size_t>(gridDim.x) * blockDim.x)
@@ -2394,6 +2405,7 @@ void launch_inject(float* ring, const uint64_t* step, uint32_t N, const uint32_t
     if (serial) inject_kernel<<<1, 1, 0, s>>>(ring, step, N, idx, val, count, 1);
     else inject_kernel<<<(count + 255) / 256, 256, 0, s>>>(ring, step, N, idx, val, count, 0);
 }
+void launch_spacer(uint64_t ticks, hipStream_t s) { spacer_kernel<<<1, 64, 0, s>>>(ticks); }
 void launch_vjump(float* v, const uint32_t* idx, const float* val, uint32_t count, int serial, hipStream_t s) {
     if (count == 0) return;
     if (serial) vjump_kernel<<<1, 1, 0, s>>>(v, idx, val, count, 1);

@@ -153,6 +153,7 @@ _SIGS = {
     "simuse_net_set_profiling": ([P, C.c_int32], C.c_int),
     "simuse_net_kernel_times": ([P, f32p, C.c_int32], C.c_int),
     "simuse_net_time_steps": ([P, C.c_uint32, f32p], C.c_int),
+    "simuse_net_time_steps_queued": ([P, C.c_uint32, f32p], C.c_int),
 }
 
 # Optional families (present when their sources are built in).

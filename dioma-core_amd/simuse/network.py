@@ -375,6 +375,13 @@ class CompiledNetwork:
         check(lib().simuse_net_time_steps(self._h, int(n), C.byref(out)))
         return float(out.value)
 
+    def time_steps_queued(self, n: int) -> float:
+        """time_steps with the steps queued behind a device-side spacer: the
+        events bracket the step kernels only (no host submission gap)."""
+        out = C.c_float()
+        check(lib().simuse_net_time_steps_queued(self._h, int(n), C.byref(out)))
+        return float(out.value)
+
     def kernel_times(self):
         """Mean ms per step of [neuron (+ fused scatter), stdp, insert, fix-up] over profiled steps."""
         out = (C.c_float * 4)()

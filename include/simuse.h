@@ -297,6 +297,10 @@ int simuse_net_kernel_times(simuse_net* net, float* ms, int32_t n);
  * handle's stream, wait for the second, and return the elapsed time in ms.
  * Profiling must be off. */
 int simuse_net_time_steps(simuse_net* net, uint32_t n, float* ms);
+/* time_steps with the steps queued behind a 200 us device-side spacer, so
+ * that the two events bracket the step kernels only (the host's submission of
+ * the graph replays happens while the spacer runs): the steps' device time. */
+int simuse_net_time_steps_queued(simuse_net* net, uint32_t n, float* ms);
 
 /* ---------------------------------------------------------------------------
  * Batches: many independent networks stepped by one launch sequence.
