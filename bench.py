@@ -77,6 +77,24 @@ def izh_algorithmic_bytes_per_neuron(spec) -> float:
     return total / sum(p.n for p in spec.pops)
 
 
+def hh_algorithmic_bytes_per_neuron(spec) -> float:
+    """Bytes one Hodgkin-Huxley neuron-step must move: v, m, h, n and the
+    refractory timer read and written (40 B; the CaT gate too, 8 B, where
+    g_CaT != 0), the ring slot read and zeroed (8 B), each live receptor's
+    conductance read and written (8 B each), both STDP traces read and
+    written (16 B) when plasticity is on, and 4 B per varying parameter."""
+    live = 0
+    if spec.receptors is not None:
+        types = np.asarray(spec.receptors["types"])
+        live = len(set(int(t) for t in np.unique(types) if t < 7))
+    total = 0.0
+    for p in spec.pops:
+        varying = sum(1 for k in range(1, 13) if not np.all(p.params[k, : p.n] == p.params[k, 0]))
+        cat = 8 if np.any(p.params[10, : p.n] != 0) else 0
+        total += p.n * (40 + cat + 8 + 8 * live + (16 if spec.stdp else 0) + 4 * varying)
+    return total / sum(p.n for p in spec.pops)
+
+
 def graph_steps_for(steps: int, most: int) -> int:
     """Steps per captured graph: the largest divisor of the timed step count
     up to `most` (at least 8 when one exists, else `most`), so that every timed
@@ -613,12 +631,21 @@ def bench_precision(args) -> dict:
     form = net.step_form()
     net.run(208)
     net.sync()
+    s0, e0 = net.counters()
     t0 = time.perf_counter()
     ms = net.time_steps(args.prec_full_steps)
     net.sync()
     us = (time.perf_counter() - t0) * 1e6 / args.prec_full_steps
+    s1, e1 = net.counters()
     net.close()
-    log(f"  full size {args.prec_full_neurons} neurons / {big.n_syn} synapses: {us:.2f} us/step")
+    q_ms = ms / args.prec_full_steps  # device time per step: the region's events over many graph replays
+    sp, ev = (s1 - s0) / args.prec_full_steps, (e1 - e0) / args.prec_full_steps
+    bpn = hh_algorithmic_bytes_per_neuron(big)
+    step_bytes = bpn * big.n_total + ev * 17 + sp * 8
+    achieved = step_bytes / (q_ms * 1e-3) / 1e9
+    log(f"  full size {args.prec_full_neurons} neurons / {big.n_syn} synapses: {us:.2f} us/step wall, "
+        f"{q_ms * 1e3:.2f} us/step device, {sp:.0f} spikes and {ev:.0f} events per step, "
+        f"{achieved:.0f} GB/s algorithmic")
     return {"workload": f"HH {n} neurons ({spec.n_syn} synapses), Pareto out-degree, lognormal conductances "
                         "(AMPA excitatory, GABA_A inhibitory, 0.003 mS/cm2 units), pair STDP (A+ 0.01, A- 0.012, "
                         "tau 20 ms, [0,1]) on excitatory synapses, Gaussian input sd 30, dt 0.01 ms",
@@ -626,7 +653,75 @@ def bench_precision(args) -> dict:
                            "windows": rows},
             "full_size": {"neurons": args.prec_full_neurons, "synapses": big.n_syn, "step_form": form_name(form),
                           "steps": args.prec_full_steps, "us_per_step": us,
-                          "region_us_per_step": ms * 1e3 / args.prec_full_steps}}
+                          "region_us_per_step": ms * 1e3 / args.prec_full_steps,
+                          "spikes_per_step": sp, "events_per_step": ev,
+                          "roofline": {"bound": "hbm", "kernel": "one step: neuron_kernel<HH> (update, receptor "
+                                                                   "currents, STDP traces, compaction), bin_kernel "
+                                                                   "(STDP, event records), bin_apply_kernel<receptors>",
+                                       "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                       "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                                       "algorithmic_bytes_per_step": step_bytes,
+                                       "algorithmic_bytes_per_neuron": bpn, "algorithmic_bytes_per_event": 17,
+                                       "algorithmic_bytes_per_spike": 8,
+                                       "note": "STDP weight traffic (FP64 masters of the spiking rows' edges) "
+                                               "left out: the bytes are a lower bound",
+                                       "step_ms": q_ms,
+                                       "step_ms_source": "HIP events around the timed region (graph replays) "
+                                                         "/ steps"}}}
+
+
+# -- Kuramoto: the phase-coupling kernel at 1M oscillators / 20M edges ---------------
+def kura_algorithmic_bytes(n: int, n_edges: int) -> dict:
+    """Bytes an RK4 step must move at least once per stage: per edge its
+    column (4 B), weight (4 B) and the neighbour's (sin, cos) (8 B); per node
+    the row pointer (4), own (sin, cos) (8), omega (4), FP64 phase (8), the
+    FP64 RK4 accumulator (written in stage 1, read and written in 2 and 3,
+    read in 4), the new (sin, cos) (8) and the phase written back in stage 4."""
+    per_edge = 16
+    node = {1: 4 + 8 + 4 + 8 + 8 + 8, 2: 4 + 8 + 4 + 8 + 16 + 8, 3: 4 + 8 + 4 + 8 + 16 + 8,
+            4: 4 + 8 + 4 + 8 + 8 + 8 + 8}
+    return {"per_edge_per_stage": per_edge, "per_node_per_step": sum(node.values()),
+            "per_step": 4 * per_edge * n_edges + sum(node.values()) * n}
+
+
+def bench_kuramoto(args) -> dict:
+    """RK4 Kuramoto on a modular small-world graph built on the device:
+    1M oscillators in 64 modules, 20 in-edges each (20M edges), W = 2/20,
+    dt 0.01.  Timed with HIP events per step (four stage kernels) after a
+    warm-up; R is reported with the FP64-phase order parameter."""
+    from simuse.kuramoto import KuramotoGraph
+    from simuse.server import KuraGenDesc
+    desc = KuraGenDesc(args.kura_nodes, 64, args.kura_k, 0.1, 0.05, 2.0, 0.01, 7)
+    g = KuramotoGraph.generated(desc, "rk4")
+    log(f"kuramoto: {g.n} oscillators, {g.n_edges} edges, RK4")
+    g.run(20)
+    g.set_profiling(True)
+    g.run(args.kura_steps)
+    step_ms = g.step_ms()
+    g.set_profiling(False)
+    g.sync()
+    t0 = time.perf_counter()
+    g.run(args.kura_steps)
+    wall_us = (time.perf_counter() - t0) * 1e6 / args.kura_steps
+    r = g.order_parameter()
+    g.close()
+    ab = kura_algorithmic_bytes(g.n, g.n_edges)
+    achieved = ab["per_step"] / (step_ms * 1e-3) / 1e9
+    log(f"  {step_ms * 1e3:.1f} us/step (HIP events, 4 stages), {wall_us:.1f} us/step wall, "
+        f"{achieved:.0f} GB/s algorithmic, R {r:.4f}")
+    return {"workload": f"Kuramoto RK4, {g.n} oscillators, {g.n_edges} edges (64 modules, k {args.kura_k}, "
+                        "beta 0.1, inter 0.05), W = 2/k, dt 0.01",
+            "steps": args.kura_steps, "us_per_step": wall_us, "order_parameter": r,
+            "oscillator_steps_per_s": g.n * 1e6 / wall_us,
+            "roofline": {"bound": "hbm", "kernel": "kura_stage_kernel (4 launches per RK4 step)",
+                         "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "algorithmic_bytes_per_step": ab["per_step"],
+                         "algorithmic_bytes_per_edge_per_stage": ab["per_edge_per_stage"],
+                         "algorithmic_bytes_per_node_per_step": ab["per_node_per_step"],
+                         "kernel_ms_per_step": step_ms,
+                         "kernel_ms_source": f"HIP events around the 4 stage kernels of each of {args.kura_steps} "
+                                             "steps (no graph)"}}
 
 
 def copy_calibration(mib: int = 1024) -> int:
@@ -678,7 +773,13 @@ def main() -> int:
     ap.add_argument("--serve-budget-gb", type=float, default=160.0)
     ap.add_argument("--serve-timeout", type=float, default=300.0)
     ap.add_argument("--kura-fp64-steps", type=int, default=2000)
+    ap.add_argument("--no-kuramoto", action="store_true", help="skip the Kuramoto leg")
+    ap.add_argument("--only-kuramoto", action="store_true", help="run the Kuramoto leg only (profiling)")
+    ap.add_argument("--kura-nodes", type=int, default=1_000_000)
+    ap.add_argument("--kura-k", type=int, default=20)
+    ap.add_argument("--kura-steps", type=int, default=100)
     ap.add_argument("--no-precision", action="store_true", help="skip the HH precision leg")
+    ap.add_argument("--only-precision", action="store_true", help="run the HH precision leg only (profiling)")
     ap.add_argument("--prec-neurons", type=int, default=10_000)
     ap.add_argument("--prec-windows", type=int, default=5)
     ap.add_argument("--prec-full-neurons", type=int, default=200_000)
@@ -698,6 +799,12 @@ def main() -> int:
     if simuse.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         return 1
+    if args.only_kuramoto:
+        print(json.dumps({"kuramoto": bench_kuramoto(args)}), flush=True)
+        return 0
+    if args.only_precision:
+        print(json.dumps({"precision": bench_precision(args)}), flush=True)
+        return 0
     out = bench_izhikevich(args)
     if not args.only_headline:
         if not args.no_brunel:
@@ -706,6 +813,8 @@ def main() -> int:
             out["batching"] = bench_batching(args)
         if not args.no_serving:
             out["serving"] = bench_serving(args)
+        if not args.no_kuramoto:
+            out["kuramoto"] = bench_kuramoto(args)
         if not args.no_precision:
             out["precision"] = bench_precision(args)
     print(json.dumps(out), flush=True)

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -46,6 +47,7 @@ void launch_head_clear(unsigned long long* head, const uint64_t* step, size_t n,
 void launch_inject(float* ring, const uint64_t* step, uint32_t N, const uint32_t* idx, const float* val,
                    uint32_t count, int serial, hipStream_t s);
 void launch_spacer(uint64_t ticks, hipStream_t s);
+void launch_coupling(const NetArgs* nets, const uint2* items, uint32_t n, hipStream_t s);
 void launch_vjump(float* v, const uint32_t* idx, const float* val, uint32_t count, int serial, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 void launch_tm_init(const uint8_t* type, const float4* prof, float* r, float* u, uint32_t* last, size_t n,
@@ -193,6 +195,24 @@ struct NetState {
     uint8_t* rec_code = nullptr;
     float* g = nullptr;               // [kReceptors][N]
     float* g_ext = nullptr;
+    // electrical coupling (host lists in call order until finalize)
+    struct CplHost {
+        uint32_t a, b;
+        float g;
+    };
+    std::vector<CplHost> h_gap, h_graded;
+    std::vector<std::array<float, kSigProf>> h_sig_prof;  // vth, 1/delta, k, erev, conductance, dt
+    struct SigConnHost {
+        uint32_t prof, pre, post;
+        float w;
+    };
+    std::vector<SigConnHost> h_sig_conn;  // call order
+    uint32_t* cp_ptr = nullptr;           // [N+1]
+    uint4* cp = nullptr;
+    float* sig_s = nullptr;               // [connections], sigmoid activation
+    float* sig_prof = nullptr;
+    uint32_t n_sig = 0;
+    bool coupled() const { return !h_gap.empty() || !h_graded.empty() || !h_sig_conn.empty(); }
     uint2* d_all_blocks = nullptr;  // (0, b) for every scatter tile: dense-step insert list
     bool finalized = false;
     uint64_t host_step = 0;
@@ -304,6 +324,10 @@ struct NetState {
         a.rec_scale = rec_scale;
         a.rec_live = rec_live;
         a.n8 = N & ~7u;
+        a.cp_ptr = dev_ptr(static_cast<const uint32_t*>(cp_ptr));
+        a.cp = dev_ptr(static_cast<const uint4*>(cp));
+        a.sig_s = dev_ptr(sig_s);
+        a.sig_prof = dev_ptr(static_cast<const float*>(sig_prof));
         return a;
     }
 
@@ -311,6 +335,7 @@ struct NetState {
     int init(const simuse_pop_desc* pd, int32_t n_pops, const uint32_t* rp, const uint32_t* c, const float* wf,
              const uint8_t* dl, uint64_t nsyn, hipStream_t stream);
     int finalize(hipStream_t stream);
+    int finalize_coupling(hipStream_t stream);
     int upload_pops(const simuse_pop_desc* pd);
     int upload_pop_table();
     int reset_dynamic(hipStream_t stream);
@@ -326,6 +351,9 @@ struct NetState {
     int set_stp(const simuse_stp_profile* profiles, int32_t n_profiles, float dt, const uint8_t* syn_types,
                 const uint8_t* src_types, uint64_t n_src_edges_given);
     int set_receptors(const uint8_t* types, float scale, const float* ext);
+    int add_coupling(int kind, const uint32_t* a, const uint32_t* b, const float* g, uint32_t count);
+    int add_sigmoid_profile(float vth, float delta, float k, float erev, float cond, float dt_ms, int32_t* id);
+    int add_sigmoid_conns(int32_t prof, const uint32_t* pre, const uint32_t* post, const float* w, uint32_t count);
     int download_conductances(float* out, uint64_t len, hipStream_t stream);
     int download_spikes(int32_t* out, uint64_t len, hipStream_t stream);
     int download_spike_ids(uint32_t* ids, uint32_t cap, uint32_t* count, hipStream_t stream);
@@ -371,6 +399,8 @@ struct Engine {
     uint32_t n_apply = 0, cap_apply = 0;
     uint2* d_apply_rec = nullptr;  // the same for networks with receptor synapses
     uint32_t n_apply_rec = 0, cap_apply_rec = 0;
+    uint2* d_cpl = nullptr;     // coupling_kernel work: {net, tile} of coupled networks
+    uint32_t n_cpl = 0, cap_cpl = 0;
     bool direct_fix = false;
     // execution
     int sched = SIMUSE_SCATTER_TILE;
@@ -396,7 +426,7 @@ struct Engine {
             if (t) (void)hipFree(t);
         for (void* p : {static_cast<void*>(d_nets), static_cast<void*>(d_blocks), static_cast<void*>(d_stdp),
                         static_cast<void*>(d_fix), static_cast<void*>(d_bin), static_cast<void*>(d_apply),
-                        static_cast<void*>(d_apply_rec)})
+                        static_cast<void*>(d_apply_rec), static_cast<void*>(d_cpl)})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -459,13 +489,15 @@ struct Engine {
         HIP_TRY(hipStreamSynchronize(stream));
         std::vector<NetArgs> na;
         std::vector<uint4> tl[kModels], binl;
-        std::vector<uint2> bl, sl, fl, apl, aplr, ml[kModels];
+        std::vector<uint2> bl, sl, fl, apl, aplr, ml[kModels], cpl;
         bins_fused = bins_bin = 0;
         for (uint32_t k = 0; k < nets.size(); ++k) {
             NetState* s = nets[k];
             na.push_back(s->args());
             NetArgs& A = na.back();
             const uint32_t n_blk = s->n_tiles + s->n_src_tiles;
+            if (s->cp_ptr)
+                for (uint32_t t = 0; t < s->n_tiles; ++t) cpl.push_back(make_uint2(k, t));
             uint32_t ng = 0;  // neuron blocks
             for (const HostPop& P : s->pops) ng += (P.ntiles + group - 1) / group;
             const uint32_t ns = (s->n_src_tiles + kBinBlockTiles - 1) / kBinBlockTiles;
@@ -581,6 +613,11 @@ struct Engine {
             if ((rc = grow(&d_bin, &cap_bin, n_bin))) return rc;
             HIP_TRY(hipMemcpy(d_bin, binl.data(), sizeof(uint4) * binl.size(), hipMemcpyHostToDevice));
         }
+        n_cpl = static_cast<uint32_t>(cpl.size());
+        if (n_cpl) {
+            if ((rc = grow(&d_cpl, &cap_cpl, n_cpl))) return rc;
+            HIP_TRY(hipMemcpy(d_cpl, cpl.data(), sizeof(uint2) * cpl.size(), hipMemcpyHostToDevice));
+        }
         xcd_order(apl, make_uint2(kEmpty, 0u));
         n_apply = static_cast<uint32_t>(apl.size());
         if (n_apply) {
@@ -600,6 +637,7 @@ struct Engine {
 
     void launch_step(hipEvent_t* e) {
         if (e) (void)hipEventRecord(e[0], stream);
+        launch_coupling(d_nets, d_cpl, n_cpl, stream);
         for (int m = 0; m < kModels; ++m) launch_megastep(m, d_nets, d_mega[m], n_mega_m[m], stream);
         for (int m = 0; m < kModels; ++m)
             launch_neuron(m, sched, d_nets, d_tiles[m], n_tiles_m[m], stream, 0u, bins_fused);

@@ -192,7 +192,7 @@ int NetState::finalize(hipStream_t stream) {
         bin_rows += bin_rows & 1u;  // even: consumers read two writers' headers at once
         // one kernel per step needs one model, no pair-based plasticity and a
         // writer per bin
-        mega_ok = one_model && !stdp && !ev_wide && n_bins + ns <= kMaxWriters;
+        mega_ok = one_model && !stdp && !ev_wide && !coupled() && n_bins + ns <= kMaxWriters;
         mega_model = pops[0].model;
         const size_t nbuf = mega_ok ? 2 : 1;
         if ((rc = alloc(&bin_ev, nbuf * n_ev * (ev_wide ? 2 : 1)))) return rc;
@@ -248,6 +248,7 @@ int NetState::finalize(hipStream_t stream) {
             HIP_TRY(hipMemcpy(g_ext, h_g_ext.data(), 4 * static_cast<size_t>(N), hipMemcpyHostToDevice));
         }
     }
+    if (coupled() && (rc = finalize_coupling(stream))) return rc;
     if ((rc = alloc(&tile_events, static_cast<size_t>(n_tiles) + n_src_tiles))) return rc;
     HIP_TRY(hipMemsetAsync(tile_events, 0, sizeof(uint64_t) * (n_tiles + n_src_tiles), stream));
     if (n_src) {
@@ -268,6 +269,110 @@ int NetState::finalize(hipStream_t stream) {
     }
     HIP_TRY(hipStreamSynchronize(stream));
     finalized = true;
+    return SIMUSE_OK;
+}
+
+// Electrical coupling: per neuron, its contributions in the order the
+// reference adds them (step.rs:55-104): every gap junction in call order
+// (both ends), every graded synapse in call order, then the sigmoid graded
+// synapses profile by profile, each profile's connections in call order.
+int NetState::finalize_coupling(hipStream_t stream) {
+    if (!h_rec_code.empty())
+        return fail(SIMUSE_E_UNSUPPORTED, "electrical coupling and receptor synapses in one network");
+    for (const HostPop& P : pops)
+        if (P.flags & kPopDrive)
+            return fail(SIMUSE_E_UNSUPPORTED,
+                        "electrical coupling with the fused Poisson drive (use add_poisson_source instead)");
+    if (pops.size() > kCplPopMask)
+        return fail(SIMUSE_E_UNSUPPORTED, "electrical coupling: more than %u populations", kCplPopMask);
+    auto pop_of = [&](uint32_t gi) -> uint32_t {
+        uint32_t p = 0;
+        while (p + 1 < pops.size() && gi >= pops[p + 1].off) ++p;
+        return p;
+    };
+    auto entry = [&](uint32_t other, uint32_t kind, uint32_t prof, float gw, uint32_t conn) -> uint4 {
+        const uint32_t p = pop_of(other);
+        uint32_t gb;
+        std::memcpy(&gb, &gw, 4);
+        return make_uint4(other - pops[p].off, p | (kind << 16) | (prof << 18), gb, conn);
+    };
+    std::vector<std::vector<uint4>> per(N);
+    for (const CplHost& c : h_gap) {
+        per[c.a].push_back(entry(c.b, 0u, 0u, c.g, 0u));
+        per[c.b].push_back(entry(c.a, 1u, 0u, c.g, 0u));
+    }
+    for (const CplHost& c : h_graded) per[c.b].push_back(entry(c.a, 2u, 0u, c.g, 0u));
+    n_sig = 0;
+    for (uint32_t q = 0; q < h_sig_prof.size(); ++q)
+        for (const SigConnHost& c : h_sig_conn)
+            if (c.prof == q) per[c.post].push_back(entry(c.pre, 3u, q, c.w, n_sig++));
+    std::vector<uint32_t> ptr(static_cast<size_t>(N) + 1, 0);
+    std::vector<uint4> flat;
+    for (uint32_t i = 0; i < N; ++i) {
+        ptr[i + 1] = ptr[i] + static_cast<uint32_t>(per[i].size());
+        flat.insert(flat.end(), per[i].begin(), per[i].end());
+    }
+    int rc;
+    if ((rc = alloc(&cp_ptr, static_cast<size_t>(N) + 1))) return rc;
+    if ((rc = alloc(&cp, flat.size()))) return rc;
+    if ((rc = alloc(&sig_s, n_sig))) return rc;
+    if ((rc = alloc(&sig_prof, static_cast<size_t>(kSigProf) * h_sig_prof.size()))) return rc;
+    HIP_TRY(hipMemcpy(cp_ptr, ptr.data(), 4 * ptr.size(), hipMemcpyHostToDevice));
+    if (!flat.empty()) HIP_TRY(hipMemcpy(cp, flat.data(), sizeof(uint4) * flat.size(), hipMemcpyHostToDevice));
+    if (!h_sig_prof.empty())
+        HIP_TRY(hipMemcpy(sig_prof, h_sig_prof.data(), sizeof(float) * kSigProf * h_sig_prof.size(),
+                          hipMemcpyHostToDevice));
+    HIP_TRY(hipMemsetAsync(sig_s, 0, sizeof(float) * std::max<uint32_t>(n_sig, 1u), stream));
+    return SIMUSE_OK;
+}
+
+int NetState::add_coupling(int kind, const uint32_t* a, const uint32_t* b, const float* gv, uint32_t count) {
+    const char* what = kind == 0 ? "connect_gap_junction" : "connect_graded_synapse";
+    if (finalized) return fail(SIMUSE_E_STATE, "%s must be called before the first step", what);
+    if (count && (!a || !b || !gv)) return fail(SIMUSE_E_INVALID, "%s: null arrays", what);
+    for (uint32_t i = 0; i < count; ++i) {
+        if (a[i] >= N || b[i] >= N)
+            return fail(SIMUSE_E_INVALID, "%s: neuron index out of bounds (%u, %u of %u)", what, a[i], b[i], N);
+        if (!(std::isfinite(gv[i]) && gv[i] >= 0.0f))
+            return fail(SIMUSE_E_INVALID, "%s: conductance must be finite and >= 0, got %g", what, gv[i]);
+    }
+    std::vector<CplHost>& L = kind == 0 ? h_gap : h_graded;
+    for (uint32_t i = 0; i < count; ++i) L.push_back(CplHost{a[i], b[i], gv[i]});
+    return SIMUSE_OK;
+}
+
+int NetState::add_sigmoid_profile(float vth, float delta, float k, float erev, float cond, float dt_ms, int32_t* id) {
+    if (finalized) return fail(SIMUSE_E_STATE, "register_sigmoid_graded must be called before the first step");
+    if (!(std::isfinite(dt_ms) && dt_ms > 0.0f))
+        return fail(SIMUSE_E_INVALID, "register_sigmoid_graded: dt_ms must be finite and > 0, got %g", dt_ms);
+    // SigmoidGradedSynapse::validate, P/src/api/net/network/sources.rs:71-86
+    const float p[5] = {vth, delta, k, erev, cond};
+    for (float x : p)
+        if (!std::isfinite(x)) return fail(SIMUSE_E_INVALID, "SigmoidGradedSynapse: parameters must be finite");
+    if (delta == 0.0f) return fail(SIMUSE_E_INVALID, "SigmoidGradedSynapse: delta must be non-zero");
+    if (!(k > 0.0f)) return fail(SIMUSE_E_INVALID, "SigmoidGradedSynapse: k must be > 0");
+    if (cond < 0.0f) return fail(SIMUSE_E_INVALID, "SigmoidGradedSynapse: conductance must be >= 0");
+    if (h_sig_prof.size() >= (1u << 14)) return fail(SIMUSE_E_UNSUPPORTED, "register_sigmoid_graded: 16384 profiles");
+    h_sig_prof.push_back({vth, 1.0f / delta, k, erev, cond, dt_ms});
+    if (id) *id = static_cast<int32_t>(h_sig_prof.size() - 1);
+    return SIMUSE_OK;
+}
+
+int NetState::add_sigmoid_conns(int32_t prof, const uint32_t* pre, const uint32_t* post, const float* wv,
+                                uint32_t count) {
+    if (finalized) return fail(SIMUSE_E_STATE, "connect_sigmoid_graded must be called before the first step");
+    if (prof < 0 || static_cast<size_t>(prof) >= h_sig_prof.size())
+        return fail(SIMUSE_E_INVALID, "sigmoid graded profile %d out of range (have %zu)", prof, h_sig_prof.size());
+    if (count && (!pre || !post || !wv)) return fail(SIMUSE_E_INVALID, "connect_sigmoid_graded: null arrays");
+    for (uint32_t i = 0; i < count; ++i) {
+        if (pre[i] >= N || post[i] >= N)
+            return fail(SIMUSE_E_INVALID, "connect_sigmoid_graded: neuron index out of bounds (%u, %u of %u)", pre[i],
+                        post[i], N);
+        if (!std::isfinite(wv[i]))
+            return fail(SIMUSE_E_INVALID, "connect_sigmoid_graded: weight must be finite, got %g", wv[i]);
+    }
+    for (uint32_t i = 0; i < count; ++i)
+        h_sig_conn.push_back(SigConnHost{static_cast<uint32_t>(prof), pre[i], post[i], wv[i]});
     return SIMUSE_OK;
 }
 
@@ -352,6 +457,7 @@ int NetState::reset_dynamic(hipStream_t stream) {
     HIP_TRY(hipMemsetAsync(tile_spikes, 0, sizeof(uint64_t) * n_tiles, stream));
     if (tile_events) HIP_TRY(hipMemsetAsync(tile_events, 0, sizeof(uint64_t) * (n_tiles + n_src_tiles), stream));
     if (g) HIP_TRY(hipMemsetAsync(g, 0, sizeof(float) * kReceptors * N, stream));
+    if (n_sig) HIP_TRY(hipMemsetAsync(sig_s, 0, sizeof(float) * n_sig, stream));
     if (stdp) {
         HIP_TRY(hipMemsetAsync(tr_pre, 0, sizeof(float) * N, stream));
         HIP_TRY(hipMemsetAsync(tr_post, 0, sizeof(float) * N, stream));
@@ -887,6 +993,7 @@ int Engine::autotune(uint32_t trial_steps) {
             if ((rc = snap.add(s->tm_last, sizeof(uint32_t) * n_ev))) return rc;
         }
         if (s->g && (rc = snap.add(s->g, sizeof(float) * kReceptors * N))) return rc;
+        if (s->n_sig && (rc = snap.add(s->sig_s, sizeof(float) * s->n_sig))) return rc;
         if (s->stdp) {
             if ((rc = snap.add(s->w, sizeof(float) * s->n_syn))) return rc;
             if ((rc = snap.add(s->w64, sizeof(double) * s->n_syn))) return rc;
@@ -1257,6 +1364,86 @@ int simuse_net_set_receptors(simuse_net* net, const uint8_t* receptor_types, flo
     return guarded([&]() { return net->st.set_receptors(receptor_types, scale, ext); });
 }
 
+static int pop_index(const NetState& s, int32_t pop, uint32_t idx, const char* what, const char* end, uint32_t* gi) {
+    if (pop < 0 || pop >= static_cast<int32_t>(s.pops.size()))
+        return fail(SIMUSE_E_INVALID, "%s: %s population %d out of range (have %zu)", what, end, pop, s.pops.size());
+    if (idx >= s.pops[pop].n) return fail(SIMUSE_E_INVALID, "%s: %s index %u out of bounds", what, end, idx);
+    *gi = s.pops[pop].off + idx;
+    return SIMUSE_OK;
+}
+
+int simuse_net_add_gap_junctions(simuse_net* net, const uint32_t* a, const uint32_t* b, const float* g,
+                                 uint32_t count) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    return guarded([&]() { return net->st.add_coupling(0, a, b, g, count); });
+}
+
+int simuse_net_connect_gap_junction(simuse_net* net, int32_t from, uint32_t from_idx, int32_t to, uint32_t to_idx,
+                                    float conductance) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    uint32_t a, b;
+    if ((rc = pop_index(net->st, from, from_idx, "connect_gap_junction", "from", &a))) return rc;
+    if ((rc = pop_index(net->st, to, to_idx, "connect_gap_junction", "to", &b))) return rc;
+    return simuse_net_add_gap_junctions(net, &a, &b, &conductance, 1);
+}
+
+int simuse_net_add_graded_synapses(simuse_net* net, const uint32_t* pre, const uint32_t* post, const float* g,
+                                   uint32_t count) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    return guarded([&]() { return net->st.add_coupling(1, pre, post, g, count); });
+}
+
+int simuse_net_connect_graded_synapse(simuse_net* net, int32_t from, uint32_t from_idx, int32_t to, uint32_t to_idx,
+                                      float conductance) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    uint32_t a, b;
+    if ((rc = pop_index(net->st, from, from_idx, "connect_graded_synapse", "from", &a))) return rc;
+    if ((rc = pop_index(net->st, to, to_idx, "connect_graded_synapse", "to", &b))) return rc;
+    return simuse_net_add_graded_synapses(net, &a, &b, &conductance, 1);
+}
+
+int simuse_net_register_sigmoid_graded(simuse_net* net, float vth, float delta, float k, float erev,
+                                       float conductance, float dt_ms, int32_t* profile) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    return guarded([&]() { return net->st.add_sigmoid_profile(vth, delta, k, erev, conductance, dt_ms, profile); });
+}
+
+int simuse_net_add_sigmoid_graded(simuse_net* net, int32_t profile, const uint32_t* pre, const uint32_t* post,
+                                  const float* weight, uint32_t count) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    return guarded([&]() { return net->st.add_sigmoid_conns(profile, pre, post, weight, count); });
+}
+
+int simuse_net_connect_sigmoid_graded(simuse_net* net, int32_t profile, int32_t from, uint32_t from_idx, int32_t to,
+                                      uint32_t to_idx, float weight) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    uint32_t a, b;
+    if ((rc = pop_index(net->st, from, from_idx, "connect_sigmoid_graded", "from", &a))) return rc;
+    if ((rc = pop_index(net->st, to, to_idx, "connect_sigmoid_graded", "to", &b))) return rc;
+    return simuse_net_add_sigmoid_graded(net, profile, &a, &b, &weight, 1);
+}
+
+int simuse_net_register_kinetics(simuse_net* net, int32_t kind, const float* params, int32_t n_params, float dt,
+                                 int32_t* id) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    (void)kind;
+    (void)params;
+    (void)n_params;
+    (void)dt;
+    if (id) *id = 0;
+    return fail(SIMUSE_E_UNSUPPORTED,
+                "register_kinetics: kinetic synapses (expOne/expTwo/alpha/expThree waveforms with their own delay "
+                "rings) are not implemented by this engine");
+}
+
 int simuse_net_download_conductances(simuse_net* net, float* out, uint64_t len) {
     int rc = check_net(net);
     if (rc) return rc;
@@ -1468,11 +1655,12 @@ static int time_steps_impl(simuse_net* net, uint32_t n, float* ms, bool queued) 
         }
         hipError_t he = hipSuccess;
         if (queued) {
-            // 200 us of the constant-rate wall clock ahead of the first event
-            int khz = 0;
-            he = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0);
+            // 1 ms of the constant-rate wall clock ahead of the first event
+            int khz = 0, dev = 0;
+            he = hipGetDevice(&dev);
+            if (he == hipSuccess) he = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
             if (he == hipSuccess) {
-                launch_spacer(static_cast<uint64_t>(khz > 0 ? khz : 100000) / 5, net->eng.stream);
+                launch_spacer(static_cast<uint64_t>(khz > 0 ? khz : 100000), net->eng.stream);
                 he = hipGetLastError();
             }
         }

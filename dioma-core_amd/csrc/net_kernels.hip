@@ -1015,7 +1015,8 @@ __global__ __launch_bounds__(kTile) void neuron_kernel(const NetArgs* __restrict
                 if (step > 0 && poisson_roll(sd, 0u, step - 1, P.drive_p)) I = I + P.drive_w;
                 drive_now = poisson_roll(sd, 0u, step, P.drive_p) ? 1 : 0;
             }
-            if (P.flags & kPopNoise) {
+            // (coupled networks took their noise into the slot, before the coupling)
+            if ((P.flags & kPopNoise) && !a.cp_ptr) {
                 const float sg = (P.flags & kPopNoiseScalar) ? P.sigma_uni : P.sigma[li];
                 I = I + sg * normal_draw(P.seed, li, step);
             }
@@ -2262,6 +2263,68 @@ __global__ void vjump_kernel(float* v, const uint32_t* idx, const float* val, ui
     if (i < count) v[idx ? idx[i] : i] += val[i];
 }
 
+// Electrical coupling, P/src/api/net/network/step.rs:36-104: with V at the
+// start of the step (no neuron of this step is advanced before this kernel
+// ends), every neuron adds its contributions to its input slot in the
+// reference's order -- gap junctions g (V_other - V_self) at the first end
+// and - g (V_self - V_other) at the second, graded synapses g (V_pre -
+// V_post), then sigmoid graded synapses (SigmoidGradedRuntime::step,
+// P/src/api/net/network/sources.rs:117-129) whose activation s is advanced
+// here by its post neuron's thread, the only writer.  Gaussian noise (the
+// reference's inject_current before the step) enters the slot first.  Work
+// item {network, tile}.
+__global__ __launch_bounds__(kTile) void coupling_kernel(const NetArgs* __restrict__ nets,
+                                                         const uint2* __restrict__ items) {
+    const uint2 it = items[blockIdx.x];
+    if (it.x == kEmpty) return;
+    const NetArgs& a = nets[it.x];
+    const uint64_t step = *a.step;
+    if (step >= a.step_limit) return;
+    const uint32_t tile = it.y;
+    const PopDev& P = a.pops[a.tile_pop[tile]];
+    const uint32_t li = (tile - P.tile0) * kTile + threadIdx.x;
+    if (li >= P.n) return;
+    const uint32_t gi = P.off + li;
+    const uint32_t e0 = a.cp_ptr[gi], e1 = a.cp_ptr[gi + 1];
+    const bool noise = P.flags & kPopNoise;
+    if (e0 == e1 && !noise) return;
+    float* cell = a.ring + static_cast<size_t>(step & (kSlots - 1)) * a.N + gi;
+    float I = *cell;
+    if (noise) {
+        const float sg = (P.flags & kPopNoiseScalar) ? P.sigma_uni : P.sigma[li];
+        I = I + sg * normal_draw(P.seed, li, step);
+    }
+    const float vme = P.st[li];
+    for (uint32_t e = e0; e < e1; ++e) {
+        const uint4 c = a.cp[e];
+        const float vo = a.pops[c.y & kCplPopMask].st[c.x];
+        const uint32_t kind = (c.y >> 16) & 3u;
+        const float g = __uint_as_float(c.z);
+        if (kind == 0u) {
+            const float dv = vo - vme;
+            I = I + g * dv;
+        } else if (kind == 1u) {
+            const float dv = vme - vo;
+            I = I - g * dv;
+        } else if (kind == 2u) {
+            const float dv = vo - vme;
+            I = I + g * dv;
+        } else {
+            const dptr<const float> pf = a.sig_prof + static_cast<size_t>(kSigProf) * (c.y >> 18);
+            const float arg = (vo - pf[0]) * pf[1];
+            const float inf = 1.0f / (1.0f + det_expf(-arg));
+            const float tau = 1.0f / (pf[2] * (1.0f + det_expf(arg)));
+            const float decay = det_expf(-pf[5] / tau);
+            float s = a.sig_s[c.w];
+            s = s + (inf - s) * (1.0f - decay);
+            a.sig_s[c.w] = s;
+            const float current = g * pf[4] * s * (pf[3] - vme);
+            I = I + current;
+        }
+    }
+    *cell = I;
+}
+
 // Holds the stream for `ticks` of the constant-rate wall clock: work queued
 // behind it starts without waiting for the host's submission.
 __global__ void spacer_kernel(uint64_t ticks) {
@@ -2406,6 +2469,9 @@ void launch_inject(float* ring, const uint64_t* step, uint32_t N, const uint32_t
     else inject_kernel<<<(count + 255) / 256, 256, 0, s>>>(ring, step, N, idx, val, count, 0);
 }
 void launch_spacer(uint64_t ticks, hipStream_t s) { spacer_kernel<<<1, 64, 0, s>>>(ticks); }
+void launch_coupling(const NetArgs* nets, const uint2* items, uint32_t n, hipStream_t s) {
+    if (n) coupling_kernel<<<n, kTile, 0, s>>>(nets, items);
+}
 void launch_vjump(float* v, const uint32_t* idx, const float* val, uint32_t count, int serial, hipStream_t s) {
     if (count == 0) return;
     if (serial) vjump_kernel<<<1, 1, 0, s>>>(v, idx, val, count, 1);

@@ -67,6 +67,8 @@ constexpr uint32_t kLtpItem = 0xFFFFFFFEu;  // bin_kernel item that runs in-edge
 // makes its network use 16-byte records.  Writer index (< kMaxWriters = 2^11)
 // and f together form a 32-bit key ascending with the event id.
 constexpr uint32_t kRecFBits = 21;
+constexpr uint32_t kCplPopMask = 0xFFFFu;  // coupling entry: population bits of .y
+constexpr int kSigProf = 6;                // floats per sigmoid graded profile
 
 enum : uint32_t {
     kPopRefrElided = 1u,  // refractory timer provably stays 0: not loaded/stored
@@ -196,6 +198,16 @@ struct NetArgs {
     float rec_scale;            // a jump adds weight * rec_scale
     uint32_t rec_live;          // bit r: some synapse has receptor type r (else g[r] stays 0)
     uint32_t n8;                // N & ~7: neurons below it sum their receptor currents onto the input
+    // Electrical coupling (gap junctions, graded and sigmoid graded synapses;
+    // cp_ptr == nullptr: none).  Neuron i's contributions cp[cp_ptr[i] ..
+    // cp_ptr[i+1]) in the reference's order: {other's local index, other's
+    // population | kind << 16 | sigmoid profile << 18, conductance or weight
+    // bits, sigmoid connection}; kind 0 gap junction (i is its first end),
+    // 1 gap junction (second end), 2 graded synapse, 3 sigmoid graded synapse.
+    dptr<const uint32_t> cp_ptr;
+    dptr<const uint4> cp;
+    dptr<float> sig_s;               // sigmoid activation per connection
+    dptr<const float> sig_prof;      // per profile: vth, 1/delta, k, erev, conductance, dt
 };
 
 }  // namespace simuse

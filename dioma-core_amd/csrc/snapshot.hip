@@ -27,6 +27,7 @@
 //   stdp.w64 u64 (FP64 bit patterns), stdp.pre stdp.post f32  (when enabled)
 //   stdp.dense_trace f32              trace of the dense stdp_step rule
 //   receptors.g f32   [7][N]          receptor conductances (when enabled)
+//   coupling.s f32                    sigmoid graded activations (when any)
 //
 // The raster recorder is not part of the state: a network that records
 // restarts its raster at the loaded step.
@@ -37,7 +38,7 @@ namespace simuse {
 namespace {
 
 constexpr uint64_t kStateFormat = 1;
-constexpr size_t kFpHead = 18;  // fixed fingerprint entries before the per-population ones
+constexpr size_t kFpHead = 19;  // fixed fingerprint entries before the per-population ones
 constexpr size_t kFpPop = 8;
 
 const char* fp_name(size_t i) {
@@ -47,7 +48,7 @@ const char* fp_name(size_t i) {
                                         "CSR row_ptr",      "CSR col",          "CSR delays",
                                         "Poisson sources",  "STDP parameters",  "plastic mask",
                                         "synapse types",    "synapse profiles", "receptor types",
-                                        "receptor scale and external current"};
+                                        "receptor scale and external current", "electrical coupling"};
     static const char* pop[kFpPop] = {"model",       "size",        "state columns", "parameter columns",
                                       "host stride", "noise/drive flags", "noise/drive settings", "noise sigma"};
     return i < kFpHead ? head[i] : pop[(i - kFpHead) % kFpPop];
@@ -131,6 +132,13 @@ int NetState::fingerprint(std::vector<uint64_t>& fp) const {
         fp[16] = (uint64_t(1) << 32) | taas::crc32(h_rec_code.data(), h_rec_code.size());
         const uint32_t sc = f_bits(rec_scale);
         fp[17] = taas::crc32(h_g_ext.data(), 4 * h_g_ext.size(), taas::crc32(&sc, 4));
+    }
+    if (coupled()) {
+        c = taas::crc32(h_gap.data(), sizeof(CplHost) * h_gap.size());
+        c = taas::crc32(h_graded.data(), sizeof(CplHost) * h_graded.size(), c);
+        c = taas::crc32(h_sig_prof.data(), sizeof(float) * kSigProf * h_sig_prof.size(), c);
+        c = taas::crc32(h_sig_conn.data(), sizeof(SigConnHost) * h_sig_conn.size(), c);
+        fp[18] = (uint64_t(1) << 32) | c;
     }
     for (const HostPop& P : pops) {
         fp.push_back(static_cast<uint64_t>(P.model));
@@ -225,6 +233,10 @@ int NetState::save_state(const char* path, hipStream_t stream) {
     if (g) {
         if ((rc = download(f, g, static_cast<size_t>(kReceptors) * N))) return rc;
         push(out, "receptors.g", "f32", f);
+    }
+    if (n_sig) {
+        if ((rc = download(f, sig_s, n_sig))) return rc;
+        push(out, "coupling.s", "f32", f);
     }
     if (out.to_file(path)) return fail(SIMUSE_E_INVALID, "save_state: cannot write %s", path);
     return SIMUSE_OK;
@@ -335,6 +347,11 @@ int NetState::load_state(const char* path, hipStream_t stream) {
         SNAP_GET(a, "receptors.g", "f32", static_cast<uint64_t>(kReceptors) * N)
         s_g = a;
     }
+    const uint8_t* s_sig = nullptr;
+    if (n_sig) {
+        SNAP_GET(a, "coupling.s", "f32", n_sig)
+        s_sig = a;
+    }
 #undef SNAP_GET
 
     // Upload.  The work of earlier steps is finished first.
@@ -374,6 +391,7 @@ int NetState::load_state(const char* path, hipStream_t stream) {
             }
         }
     }
+    if (s_sig) HIP_TRY(hipMemcpy(sig_s, s_sig, 4 * static_cast<size_t>(n_sig), hipMemcpyHostToDevice));
     if (s_dt) {
         if (!dtrace && (rc = alloc(&dtrace, N))) return rc;
         HIP_TRY(hipMemcpy(dtrace, s_dt, 4 * static_cast<size_t>(N), hipMemcpyHostToDevice));

@@ -123,6 +123,16 @@ _SIGS = {
     "simuse_net_download_weights": ([P, f32p, C.c_uint64], C.c_int),
     "simuse_net_inject_current": ([P, C.c_int32, u32p, f32p, C.c_uint32], C.c_int),
     "simuse_net_inject_voltage": ([P, C.c_int32, u32p, f32p, C.c_uint32], C.c_int),
+    "simuse_net_connect_gap_junction": ([P, C.c_int32, C.c_uint32, C.c_int32, C.c_uint32, C.c_float], C.c_int),
+    "simuse_net_add_gap_junctions": ([P, u32p, u32p, f32p, C.c_uint32], C.c_int),
+    "simuse_net_connect_graded_synapse": ([P, C.c_int32, C.c_uint32, C.c_int32, C.c_uint32, C.c_float], C.c_int),
+    "simuse_net_add_graded_synapses": ([P, u32p, u32p, f32p, C.c_uint32], C.c_int),
+    "simuse_net_register_sigmoid_graded": ([P, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                            i32p], C.c_int),
+    "simuse_net_connect_sigmoid_graded": ([P, C.c_int32, C.c_int32, C.c_uint32, C.c_int32, C.c_uint32, C.c_float],
+                                          C.c_int),
+    "simuse_net_add_sigmoid_graded": ([P, C.c_int32, u32p, u32p, f32p, C.c_uint32], C.c_int),
+    "simuse_net_register_kinetics": ([P, C.c_int32, f32p, C.c_int32, C.c_float, i32p], C.c_int),
     "simuse_pool_create": ([C.POINTER(PoolConfig), C.c_int32, C.POINTER(P)], C.c_int),
     "simuse_pool_destroy": ([P], None),
     "simuse_pool_register": ([P, C.c_uint32, i32p, i32p, f32p, f32p, C.POINTER(PoolSlot)], C.c_int),

@@ -241,6 +241,50 @@ class CompiledNetwork:
             raise ValueError("inject_current: indices/values length mismatch")
         check(lib().simuse_net_inject_current(self._h, pop, _ptr(i, C.c_uint32), _ptr(v, C.c_float), i.size))
 
+    # -- electrical coupling (before the first step) ------------------------------
+    def connect_gap_junction(self, from_pop: int, from_idx: int, to_pop: int, to_idx: int,
+                             conductance: float) -> None:
+        """Network::connect_gap_junction (crates/SiMuse/crates/pillars/src/api/net/network/core.rs:391)."""
+        check(lib().simuse_net_connect_gap_junction(self._h, from_pop, from_idx, to_pop, to_idx, float(conductance)))
+
+    def connect_graded_synapse(self, from_pop: int, from_idx: int, to_pop: int, to_idx: int,
+                               conductance: float) -> None:
+        """Network::connect_graded_synapse (core.rs:438)."""
+        check(lib().simuse_net_connect_graded_synapse(self._h, from_pop, from_idx, to_pop, to_idx,
+                                                      float(conductance)))
+
+    def add_gap_junctions(self, a, b, g) -> None:
+        """Gap junctions between global neurons a[k], b[k] with conductance g[k], in order."""
+        a, b, g = (np.ascontiguousarray(a, np.uint32), np.ascontiguousarray(b, np.uint32),
+                   np.ascontiguousarray(g, np.float32))
+        check(lib().simuse_net_add_gap_junctions(self._h, _ptr(a, C.c_uint32), _ptr(b, C.c_uint32),
+                                                 _ptr(g, C.c_float), a.size))
+
+    def add_graded_synapses(self, pre, post, g) -> None:
+        pre, post, g = (np.ascontiguousarray(pre, np.uint32), np.ascontiguousarray(post, np.uint32),
+                        np.ascontiguousarray(g, np.float32))
+        check(lib().simuse_net_add_graded_synapses(self._h, _ptr(pre, C.c_uint32), _ptr(post, C.c_uint32),
+                                                   _ptr(g, C.c_float), pre.size))
+
+    def register_sigmoid_graded(self, vth: float, delta: float, k: float, erev: float, conductance: float,
+                                dt_ms: float) -> int:
+        """Network::register_sigmoid_graded (core.rs:483); returns the profile id."""
+        out = C.c_int32()
+        check(lib().simuse_net_register_sigmoid_graded(self._h, vth, delta, k, erev, conductance, dt_ms,
+                                                       C.byref(out)))
+        return int(out.value)
+
+    def connect_sigmoid_graded(self, profile: int, from_pop: int, from_idx: int, to_pop: int, to_idx: int,
+                               weight: float) -> None:
+        check(lib().simuse_net_connect_sigmoid_graded(self._h, profile, from_pop, from_idx, to_pop, to_idx,
+                                                      float(weight)))
+
+    def add_sigmoid_graded(self, profile: int, pre, post, weight) -> None:
+        pre, post, w = (np.ascontiguousarray(pre, np.uint32), np.ascontiguousarray(post, np.uint32),
+                        np.ascontiguousarray(weight, np.float32))
+        check(lib().simuse_net_add_sigmoid_graded(self._h, profile, _ptr(pre, C.c_uint32), _ptr(post, C.c_uint32),
+                                                  _ptr(w, C.c_float), pre.size))
+
     def inject_voltage(self, pop: int, indices, values) -> None:
         """WgpuNetwork::inject_voltage (crates/SiMuse/crates/pillars-wgpu/src/network.rs:1632-1659):
         v += value on local neurons of ``pop`` before the next step; ``indices`` None

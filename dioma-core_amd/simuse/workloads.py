@@ -45,6 +45,9 @@ class NetSpec:
     # conductance-based receptor synapses: {types: uint8[n_syn] (0..6, 255 current-based),
     # scale: float, ext: float32[n_total] or absent}
     receptors: Optional[dict] = None
+    # electrical coupling, global indices in call order: {gap: (a, b, g), graded: (pre, post, g),
+    # sigmoid: [((vth, delta, k, erev, conductance, dt), (pre, post, weight)), ...]}
+    coupling: Optional[dict] = None
 
     @property
     def n_total(self) -> int:
@@ -232,6 +235,15 @@ def build_device(spec: NetSpec, graph_steps: int = 8) -> CompiledNetwork:
                                       spec.stp.get("src_types"))
     if spec.receptors:
         net.set_receptors(spec.receptors["types"], spec.receptors.get("scale", 1.0), spec.receptors.get("ext"))
+    if spec.coupling:
+        cp = spec.coupling
+        if cp.get("gap") is not None:
+            net.add_gap_junctions(*cp["gap"])
+        if cp.get("graded") is not None:
+            net.add_graded_synapses(*cp["graded"])
+        for prof, conns in cp.get("sigmoid", []):
+            pid = net.register_sigmoid_graded(*prof)
+            net.add_sigmoid_graded(pid, *conns)
     net.set_graph_steps(graph_steps)
     return net
 

@@ -222,6 +222,47 @@ int simuse_net_set_receptors(simuse_net* net, const uint8_t* receptor_types, flo
 /* Conductances [7][n_total] (receptor-major). */
 int simuse_net_download_conductances(simuse_net* net, float* out, uint64_t len);
 
+/* Electrical coupling inside the step (P/src/api/net/network/step.rs:36-104):
+ * with every membrane potential V taken at the start of the step, after the
+ * ring slot is read, each gap junction (a, b, g) adds g (V_b - V_a) to a's
+ * input and subtracts it from b's, each graded synapse (pre, post, g) adds
+ * g (V_pre - V_post) to post's, then each sigmoid graded synapse (profile
+ * order, then connection order) advances its activation s (exponential
+ * Euler) and adds weight * conductance * s * (erev - V_post) to post's; all
+ * in call order.  Gaussian noise (set_noise) enters the input before them,
+ * as the reference's inject_current before the step would.  Call before the
+ * first step.  Networks with coupling run the two-kernel step; coupling is
+ * refused (SIMUSE_E_UNSUPPORTED at the first step) with receptor synapses or
+ * the fused Poisson drive.  The sigmoid's exponentials are this engine's
+ * deterministic exp (within 2 ulp of the reference's f32::exp).
+ * connect_gap_junction: Network::connect_gap_junction, core.rs:391-426.
+ * connect_graded_synapse: Network::connect_graded_synapse, core.rs:438-473.
+ * register_sigmoid_graded: Network::register_sigmoid_graded, core.rs:483-500,
+ *   with SigmoidGradedSynapse::validate (sources.rs:71-86); returns the
+ *   profile id in *profile.
+ * connect_sigmoid_graded: Network::connect_sigmoid_graded, core.rs:510-553.
+ * The add_* forms take global neuron indices and arrays (bulk wiring). */
+int simuse_net_connect_gap_junction(simuse_net* net, int32_t from, uint32_t from_idx, int32_t to, uint32_t to_idx,
+                                    float conductance);
+int simuse_net_add_gap_junctions(simuse_net* net, const uint32_t* a, const uint32_t* b, const float* g,
+                                 uint32_t count);
+int simuse_net_connect_graded_synapse(simuse_net* net, int32_t from, uint32_t from_idx, int32_t to, uint32_t to_idx,
+                                      float conductance);
+int simuse_net_add_graded_synapses(simuse_net* net, const uint32_t* pre, const uint32_t* post, const float* g,
+                                   uint32_t count);
+int simuse_net_register_sigmoid_graded(simuse_net* net, float vth, float delta, float k, float erev,
+                                       float conductance, float dt_ms, int32_t* profile);
+int simuse_net_connect_sigmoid_graded(simuse_net* net, int32_t profile, int32_t from, uint32_t from_idx, int32_t to,
+                                      uint32_t to_idx, float weight);
+int simuse_net_add_sigmoid_graded(simuse_net* net, int32_t profile, const uint32_t* pre, const uint32_t* post,
+                                  const float* weight, uint32_t count);
+/* Network::register_kinetics (core.rs:564-586: SynapseKinetics waveforms
+ * expOne/expTwo/alpha/expThree, P/src/api/net/synapse.rs:46-70, with their
+ * own delay rings) is not implemented: always SIMUSE_E_UNSUPPORTED, so that a
+ * network that needs it is refused rather than run without it. */
+int simuse_net_register_kinetics(simuse_net* net, int32_t kind, const float* params, int32_t n_params, float dt,
+                                 int32_t* id);
+
 /* Checkpoint / resume: the reference saves download_states /
  * download_weights (PC/src/network.rs:481,736) as named sections of a
  * `.taas` v2 file (S/crates/taas-format/src/writer.rs:73-99, lib.rs:8-24)
@@ -297,7 +338,7 @@ int simuse_net_kernel_times(simuse_net* net, float* ms, int32_t n);
  * handle's stream, wait for the second, and return the elapsed time in ms.
  * Profiling must be off. */
 int simuse_net_time_steps(simuse_net* net, uint32_t n, float* ms);
-/* time_steps with the steps queued behind a 200 us device-side spacer, so
+/* time_steps with the steps queued behind a 1 ms device-side spacer, so
  * that the two events bracket the step kernels only (the host's submission of
  * the graph replays happens while the spacer runs): the steps' device time. */
 int simuse_net_time_steps_queued(simuse_net* net, uint32_t n, float* ms);

@@ -38,6 +38,10 @@ _SIGS = {
     "orc_net_set_plasticity": ([P, C.POINTER(OrcStdp), u8p], C.c_int),
     "orc_net_inject_current": ([P, C.c_int32, u32p, f32p, C.c_uint32], C.c_int),
     "orc_net_inject_voltage": ([P, C.c_int32, u32p, f32p, C.c_uint32], C.c_int),
+    "orc_net_add_gap_junction": ([P, C.c_uint32, C.c_uint32, C.c_float], None),
+    "orc_net_add_graded_synapse": ([P, C.c_uint32, C.c_uint32, C.c_float], None),
+    "orc_net_add_sigmoid_profile": ([P, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float], C.c_int),
+    "orc_net_add_sigmoid_conn": ([P, C.c_int32, C.c_uint32, C.c_uint32, C.c_float], C.c_int),
     "orc_pool_create": ([C.c_void_p, C.c_int32], P),
     "orc_pool_destroy": ([P], None),
     "orc_pool_register": ([P, C.c_uint32, i32p, i32p, f32p, f32p], C.c_int32),
@@ -231,6 +235,18 @@ class OracleNetwork:
             ext = np.ascontiguousarray(ext, np.float32) if ext is not None else None
             assert L.orc_net_set_receptors(self.h, types.ctypes.data, float(rec.get("scale", 1.0)),
                                            ext.ctypes.data if ext is not None else None) == 0
+        cp = getattr(spec, "coupling", None)
+        if cp:
+            if cp.get("gap") is not None:
+                for a, b, g in zip(*cp["gap"]):
+                    L.orc_net_add_gap_junction(self.h, int(a), int(b), float(g))
+            if cp.get("graded") is not None:
+                for a, b, g in zip(*cp["graded"]):
+                    L.orc_net_add_graded_synapse(self.h, int(a), int(b), float(g))
+            for prof, (pre, post, w) in cp.get("sigmoid", []):
+                pid = L.orc_net_add_sigmoid_profile(self.h, *[float(x) for x in prof])
+                for a, b, x in zip(pre, post, w):
+                    assert L.orc_net_add_sigmoid_conn(self.h, pid, int(a), int(b), float(x)) == 0
         if threads > 1:
             L.orc_net_set_threads(self.h, threads)
 

@@ -553,6 +553,10 @@ struct NetBase {
                         const uint8_t* src_types, uint64_t n_src_edges) = 0;
     virtual int inject(int pop, const uint32_t* idx, const float* val, uint32_t count) = 0;
     virtual int inject_voltage(int pop, const uint32_t* idx, const float* val, uint32_t count) = 0;
+    // electrical coupling, global neuron indices: kind 0 gap junction, 1 graded synapse
+    virtual void add_coupling(int kind, uint32_t a, uint32_t b, float g) = 0;
+    virtual int add_sigmoid_profile(float vth, float delta, float k, float erev, float cond, float dt) = 0;
+    virtual int add_sigmoid_conn(int prof, uint32_t pre, uint32_t post, float weight) = 0;
     virtual int set_receptors(const uint8_t* types, float scale, const float* ext) = 0;
     virtual void conductances(double* out) const = 0;
     // Overwrite the dynamic state with a (device) snapshot: FP32 values are
@@ -625,6 +629,21 @@ struct Net final : NetBase {
     // receptor synapses: per synapse 0 current-based or 1 + receptor type;
     // conductances [7][N]; constant external current
     bool rec_on = false;
+    // Electrical coupling in the reference's list orders (step.rs:36-104,
+    // core.rs:391-553): gap junctions, one-way graded synapses, and sigmoid
+    // graded synapses per registered profile (sources.rs:91-130).
+    struct Cpl {
+        uint32_t a, b;
+        R g;
+    };
+    std::vector<Cpl> gapj, graded;
+    struct SigRt {
+        R vth, inv_delta, k, erev, cond, dt;
+        std::vector<uint32_t> pre, post;
+        std::vector<R> w, s;
+    };
+    std::vector<SigRt> sig;
+    std::vector<R> vscr;
     std::vector<uint8_t> rec_code;
     std::vector<R> gcond;
     std::vector<float> g_ext;
@@ -819,6 +838,58 @@ struct Net final : NetBase {
         return 0;
     }
 
+    void add_coupling(int kind, uint32_t a, uint32_t b, float g) override {
+        (kind == 0 ? gapj : graded).push_back(Cpl{a, b, R(g)});
+    }
+    int add_sigmoid_profile(float vth, float delta, float k, float erev, float cond, float dt) override {
+        SigRt r;
+        r.vth = R(vth);
+        r.inv_delta = R(1) / R(delta);  // SigmoidGradedRuntime::new, sources.rs:103-113
+        r.k = R(k);
+        r.erev = R(erev);
+        r.cond = R(cond);
+        r.dt = R(dt);
+        sig.push_back(r);
+        return static_cast<int>(sig.size() - 1);
+    }
+    int add_sigmoid_conn(int prof, uint32_t pre, uint32_t post, float weight) override {
+        if (prof < 0 || prof >= static_cast<int>(sig.size())) return -3;
+        SigRt& r = sig[prof];
+        r.pre.push_back(pre);
+        r.post.push_back(post);
+        r.w.push_back(R(weight));
+        r.s.push_back(R(0));
+        return 0;
+    }
+    bool coupled() const { return !gapj.empty() || !graded.empty() || !sig.empty(); }
+    // step.rs:36-104: V at the start of the step, then the three couplings
+    // add into i_in in list order
+    void coupling() {
+        vscr.resize(N);
+        for (auto& P : pops)
+            for (uint32_t i = 0; i < P.n; ++i) vscr[P.off + i] = P.st[i];
+        for (const Cpl& c : gapj) {
+            const R dv = vscr[c.b] - vscr[c.a];
+            i_in[c.a] = i_in[c.a] + c.g * dv;
+            i_in[c.b] = i_in[c.b] - c.g * dv;
+        }
+        for (const Cpl& c : graded) {
+            const R dv = vscr[c.a] - vscr[c.b];
+            i_in[c.b] = i_in[c.b] + c.g * dv;
+        }
+        for (SigRt& r : sig)
+            for (size_t q = 0; q < r.pre.size(); ++q) {  // SigmoidGradedRuntime::step, sources.rs:117-129
+                const R vpre = vscr[r.pre[q]], vpost = vscr[r.post[q]];
+                const R arg = (vpre - r.vth) * r.inv_delta;
+                const R inf = R(1) / (R(1) + Math<R>::exp(-arg));
+                const R tau = R(1) / (r.k * (R(1) + Math<R>::exp(arg)));
+                const R decay = Math<R>::exp(-r.dt / tau);
+                r.s[q] = r.s[q] + (inf - r.s[q]) * (R(1) - decay);
+                const R current = r.w[q] * r.cond * r.s[q] * (r.erev - vpost);
+                i_in[r.post[q]] = i_in[r.post[q]] + current;
+            }
+    }
+
     int inject_voltage(int p, const uint32_t* idx, const float* val, uint32_t count) override {
         // WgpuNetwork::inject_voltage + v_jump_kernel, PW/src/network.rs:1632-1659,
         // PW/shaders/csr_scatter.wgsl:117-124: v += value before the next step
@@ -929,6 +1000,7 @@ struct Net final : NetBase {
         // 1-2: read slot -> i_in, clear slot (delay.rs:68-77)
         std::memcpy(i_in.data(), cur, sizeof(R) * N);
         std::fill(cur, cur + N, R(0));
+        if (coupled()) coupling();
         if (rec_on) receptor_inputs();
         // 3: populations (population.rs:284-420)
         for (auto& P : pops) step_pop(P);
@@ -1263,6 +1335,18 @@ int orc_net_inject_current(void* net, int32_t pop, const uint32_t* idx, const fl
 }
 int orc_net_inject_voltage(void* net, int32_t pop, const uint32_t* idx, const float* val, uint32_t count) {
     return static_cast<NetBase*>(net)->inject_voltage(pop, idx, val, count);
+}
+void orc_net_add_gap_junction(void* net, uint32_t a, uint32_t b, float g) {
+    static_cast<NetBase*>(net)->add_coupling(0, a, b, g);
+}
+void orc_net_add_graded_synapse(void* net, uint32_t pre, uint32_t post, float g) {
+    static_cast<NetBase*>(net)->add_coupling(1, pre, post, g);
+}
+int orc_net_add_sigmoid_profile(void* net, float vth, float delta, float k, float erev, float cond, float dt) {
+    return static_cast<NetBase*>(net)->add_sigmoid_profile(vth, delta, k, erev, cond, dt);
+}
+int orc_net_add_sigmoid_conn(void* net, int32_t prof, uint32_t pre, uint32_t post, float weight) {
+    return static_cast<NetBase*>(net)->add_sigmoid_conn(prof, pre, post, weight);
 }
 void orc_net_step(void* net, uint32_t n_steps) { static_cast<NetBase*>(net)->step(n_steps); }
 int orc_net_set_receptors(void* net, const uint8_t* types, float scale, const float* ext) {

@@ -90,6 +90,12 @@ void orc_rec_currents(const float* g, const float* v, float* i_out, uint32_t n);
 /* v += val[k] on state column 0 of neuron idx[k] of pop (idx NULL: count ==
  * the population's size, neuron k gets val[k]), before the next step. */
 int orc_net_inject_voltage(void* net, int32_t pop, const uint32_t* idx, const float* val, uint32_t count);
+/* Electrical coupling, global neuron indices, in call order
+ * (P/src/api/net/network/core.rs:391-553, step.rs:36-104). */
+void orc_net_add_gap_junction(void* net, uint32_t a, uint32_t b, float g);
+void orc_net_add_graded_synapse(void* net, uint32_t pre, uint32_t post, float g);
+int orc_net_add_sigmoid_profile(void* net, float vth, float delta, float k, float erev, float cond, float dt);
+int orc_net_add_sigmoid_conn(void* net, int32_t prof, uint32_t pre, uint32_t post, float weight);
 int orc_net_inject_current(void* net, int32_t pop, const uint32_t* idx,
                            const float* val, uint32_t count);
 void orc_net_step(void* net, uint32_t n_steps);
