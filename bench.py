@@ -467,7 +467,7 @@ def bench_batching(args) -> dict:
 
 
 # -- C5: serving ----------------------------------------------------------------------------
-def bench_serving(args) -> dict:
+def bench_serving(args, budget_gb: float, fp64: bool = True) -> dict:
     """Serving configuration: Poisson arrivals (200/s) of LIF/Izhikevich/HH
     networks (5k-500k neurons) and Kuramoto graphs (10k-1M oscillators, RK4),
     horizons 500-20,000 steps, admitted by an HBM budget and stepped together
@@ -475,8 +475,8 @@ def bench_serving(args) -> dict:
     from simuse.server import DONE, Server
     from simuse.workloads import serving_mix
     reqs = serving_mix(args.serve_requests, rate_hz=args.serve_rate, seed=3)
-    log(f"serving: {len(reqs)} requests at {args.serve_rate}/s, budget {args.serve_budget_gb} GB")
-    srv = Server(int(args.serve_budget_gb * (1 << 30)), chunk_steps=16)
+    log(f"serving: {len(reqs)} requests at {args.serve_rate}/s, budget {budget_gb} GB")
+    srv = Server(int(budget_gb * (1 << 30)), chunk_steps=16)
     ids = []
     t0 = time.perf_counter()
     for t, r in reqs:
@@ -510,10 +510,12 @@ def bench_serving(args) -> dict:
            "mean_admission_wait_s": float(wait.mean()), "max_admission_wait_s": float(wait.max()),
            "networks_per_s": len(done) / span,
            "wall_s": wall, "peak_live": int(st.peak_live), "peak_bytes": int(st.peak_bytes),
-           "budget_bytes": int(args.serve_budget_gb * (1 << 30))}
+           "budget_bytes": int(budget_gb * (1 << 30)),
+           "admitted_after_a_completion": int(sum(1 for x in done if x.t_admit > min(y.t_done for y in done)))}
     log(f"  {len(done)} done, p50 {out['p50_latency_s']:.3f} s, p99 {out['p99_latency_s']:.3f} s, "
-        f"{out['networks_per_s']:.1f} networks/s, peak live {st.peak_live}")
-    if not args.no_fp64:
+        f"{out['networks_per_s']:.1f} networks/s, peak live {st.peak_live}, admission wait max "
+        f"{out['max_admission_wait_s']:.3f} s")
+    if fp64 and not args.no_fp64:
         out["fp64_error"] = serving_fp64(reqs, res, args)
     return out
 
@@ -771,6 +773,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-gb", type=float, default=160.0)
+    ap.add_argument("--serve-tight-gb", type=float, default=6.0,
+                    help="budget of the second serving run, small enough that requests wait for memory")
     ap.add_argument("--serve-timeout", type=float, default=300.0)
     ap.add_argument("--kura-fp64-steps", type=int, default=2000)
     ap.add_argument("--no-kuramoto", action="store_true", help="skip the Kuramoto leg")
@@ -812,7 +816,8 @@ def main() -> int:
         if not args.no_batching:
             out["batching"] = bench_batching(args)
         if not args.no_serving:
-            out["serving"] = bench_serving(args)
+            out["serving"] = bench_serving(args, args.serve_budget_gb)
+            out["serving_tight_budget"] = bench_serving(args, args.serve_tight_gb, fp64=False)
         if not args.no_kuramoto:
             out["kuramoto"] = bench_kuramoto(args)
         if not args.no_precision:
