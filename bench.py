@@ -726,6 +726,87 @@ def bench_kuramoto(args) -> dict:
                                              "steps (no graph)"}}
 
 
+# -- creature pool: many small alpha-LIF brains, per-post pull scatter ---------------
+def bench_pool(args) -> dict:
+    """The game engine's creature pool (BrainPool): `pool_creatures` random
+    brains of `pool_neurons` alpha-LIF neurons (connection probability
+    0.02, 20% inhibitory), a tenth of each brain's neurons receiving a voltage
+    input each tick, 100 substeps per tick (two submits, 64 + 36).  Timed:
+    whole ticks with the spike-history readback the reference does (wall),
+    and the substeps' device time without readback (HIP events).  The first
+    tick's history is checked against the oracle pool; the CPU baseline is
+    the oracle pool (single thread) on one tick."""
+    import pyoracle as po
+    from simuse.pool import BrainPool
+    from simuse.workloads import creature
+    nc, nn, spt = args.pool_creatures, args.pool_neurons, 100
+    rng = np.random.default_rng(5)
+    cs = [creature(nn, p=0.02, seed=1000 + k) for k in range(nc)]
+    pool = BrainPool()
+    for c in cs:
+        pool.register(c.row_ptr, c.col, c.values, c.nt)
+    n_total, n_syn, _, delay = pool.info()
+    log(f"pool: {nc} creatures x {nn} alpha-LIF neurons = {n_total}, {n_syn} synapses, {spt} substeps per tick")
+    exts_all = []
+    for t in range(args.pool_ticks + 1):
+        ex = []
+        for c in cs:
+            e = np.zeros(c.n, np.float32)
+            m = rng.random(c.n) < 0.1
+            e[m] = rng.uniform(0.0, 12.0, int(m.sum())).astype(np.float32)
+            ex.append(e)
+        exts_all.append(ex)
+    # parity of the first tick against the oracle pool (also the CPU baseline)
+    ora = po.OraclePool()
+    for c in cs:
+        ora.register(c.row_ptr, c.col, c.values, c.nt)
+    flat0, _ = pool.step(exts_all[0], spt)
+    t0 = time.perf_counter()
+    ref0 = ora.step(exts_all[0], spt)
+    cpu_s = time.perf_counter() - t0
+    ora.close()
+    same = bool(np.array_equal(flat0, ref0))
+    outdeg = np.concatenate([np.diff(c.row_ptr) for c in cs]).astype(np.int64)
+    pool.sync_weights([(c.values, True) for c in cs])  # the reference's per-tick weight upload
+    # whole ticks with readback
+    spikes = act_edges = 0
+    t0 = time.perf_counter()
+    for t in range(1, args.pool_ticks + 1):
+        flat, _ = pool.step(exts_all[t], spt)
+        f = flat.reshape(spt, n_total)
+        spikes += int(f.sum())
+        act_edges += int((f.astype(np.int64) @ outdeg).sum())
+    wall = time.perf_counter() - t0
+    # device time of the substeps alone (same input, no readback)
+    dev_ms = pool.time_substeps(args.pool_ticks * spt)
+    pool.close()
+    subs = args.pool_ticks * spt
+    us_sub = dev_ms * 1e3 / subs
+    per_sub = n_total * 40 + n_syn * 4 + (act_edges / subs) * 4
+    achieved = per_sub / (us_sub * 1e-6) / 1e9
+    log(f"  {wall * 1e3 / args.pool_ticks:.2f} ms per tick with readback, {us_sub:.2f} us per substep on the device, "
+        f"{spikes / subs:.0f} spikes per substep, {achieved:.0f} GB/s algorithmic; first tick == oracle: {same}; "
+        f"oracle tick {cpu_s:.2f} s")
+    return {"workload": f"creature pool: {nc} alpha-LIF brains x {nn} neurons ({n_total}), {n_syn} synapses "
+                        f"(p 0.02, 20% inhibitory), delay {delay}, 100 substeps per tick, voltage input on a tenth",
+            "ticks": args.pool_ticks, "ms_per_tick_with_readback": wall * 1e3 / args.pool_ticks,
+            "ticks_per_s": args.pool_ticks / wall, "readback_bytes_per_tick": spt * n_total * 4,
+            "device_us_per_substep": us_sub, "neuron_substeps_per_s_device": n_total / (us_sub * 1e-6),
+            "spikes_per_substep": spikes / subs, "first_tick_equals_oracle": same,
+            "roofline": {"bound": "hbm", "kernel": "pool_step_kernel (update + pull scatter, one per substep)",
+                         "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "algorithmic_bytes_per_substep": per_sub,
+                         "algorithmic_bytes": "40 B per neuron (v, g, refr_cnt read and written, slot read and "
+                                              "zeroed, history, row pointer), 4 B per in-edge (source and bucket), "
+                                              "4 B per in-edge whose source spiked (signed weight); the ring adds "
+                                              "left out (lower bound)",
+                         "kernel_ms_per_substep": dev_ms / subs},
+            "cpu_baseline": {"value": n_total * spt / cpu_s, "unit": "neuron-substeps/s", "cores": 1,
+                             "kind": "port", "sample": f"oracle pool (restated per-post pull), first tick, {spt} "
+                                                       "substeps, single thread"}}
+
+
 def copy_calibration(mib: int = 1024) -> int:
     """Device-to-device copy of a known size (the HIP runtime's copy kernel),
     used to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes."""
@@ -782,6 +863,11 @@ def main() -> int:
     ap.add_argument("--kura-nodes", type=int, default=1_000_000)
     ap.add_argument("--kura-k", type=int, default=20)
     ap.add_argument("--kura-steps", type=int, default=100)
+    ap.add_argument("--no-pool", action="store_true", help="skip the creature-pool leg")
+    ap.add_argument("--only-pool", action="store_true", help="run the creature-pool leg only (profiling)")
+    ap.add_argument("--pool-creatures", type=int, default=64)
+    ap.add_argument("--pool-neurons", type=int, default=1000)
+    ap.add_argument("--pool-ticks", type=int, default=20)
     ap.add_argument("--no-precision", action="store_true", help="skip the HH precision leg")
     ap.add_argument("--only-precision", action="store_true", help="run the HH precision leg only (profiling)")
     ap.add_argument("--prec-neurons", type=int, default=10_000)
@@ -809,6 +895,9 @@ def main() -> int:
     if args.only_precision:
         print(json.dumps({"precision": bench_precision(args)}), flush=True)
         return 0
+    if args.only_pool:
+        print(json.dumps({"pool": bench_pool(args)}), flush=True)
+        return 0
     out = bench_izhikevich(args)
     if not args.only_headline:
         if not args.no_brunel:
@@ -820,6 +909,8 @@ def main() -> int:
             out["serving_tight_budget"] = bench_serving(args, args.serve_tight_gb, fp64=False)
         if not args.no_kuramoto:
             out["kuramoto"] = bench_kuramoto(args)
+        if not args.no_pool:
+            out["pool"] = bench_pool(args)
         if not args.no_precision:
             out["precision"] = bench_precision(args)
     print(json.dumps(out), flush=True)

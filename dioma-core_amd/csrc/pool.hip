@@ -464,6 +464,54 @@ int simuse_pool_download_state(simuse_pool* p, int32_t col, float* out, uint64_t
     return SIMUSE_OK;
 }
 
+int simuse_pool_time_substeps(simuse_pool* p, uint32_t n, float* ms) {
+    if (!p || !ms) return fail(SIMUSE_E_INVALID, "pool_time_substeps: null argument");
+    if (p->slots.empty()) return fail(SIMUSE_E_STATE, "pool_time_substeps: no creature registered");
+    int rc = p->build();
+    if (rc) return rc;
+    for (uint32_t done = 0; done < n;) {  // graphs of every chunk size first: no capture inside the region
+        const uint32_t chunk = std::min(n - done, kMaxSubmit);
+        bool have = false;
+        for (const auto& gr : p->graphs) have = have || gr.chunk == chunk;
+        if (!have) {
+            PHIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+            p->enqueue(chunk);
+            hipGraph_t gg = nullptr;
+            hipError_t e = hipStreamEndCapture(p->stream, &gg);
+            if (e != hipSuccess) return fail(SIMUSE_E_HIP, "pool capture: %s", hipGetErrorString(e));
+            hipGraphExec_t x = nullptr;
+            e = hipGraphInstantiate(&x, gg, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(gg);
+                return fail(SIMUSE_E_HIP, "pool instantiate: %s", hipGetErrorString(e));
+            }
+            p->graphs.push_back({chunk, gg, x});
+        }
+        done += chunk;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    PHIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(SIMUSE_E_HIP, "pool_time_substeps: hipEventCreate");
+    }
+    hipError_t he = hipEventRecord(e0, p->stream);
+    for (uint32_t done = 0; he == hipSuccess && done < n;) {
+        const uint32_t chunk = std::min(n - done, kMaxSubmit);
+        if ((rc = p->run_chunk(chunk))) break;
+        done += chunk;
+    }
+    if (he == hipSuccess && !rc) he = hipEventRecord(e1, p->stream);
+    if (he == hipSuccess && !rc) he = hipEventSynchronize(e1);
+    if (he == hipSuccess && !rc) he = hipEventElapsedTime(ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (he != hipSuccess) return fail(SIMUSE_E_HIP, "pool_time_substeps: %s", hipGetErrorString(he));
+    p->steps += n;
+    return SIMUSE_OK;
+}
+
 int simuse_pool_info(simuse_pool* p, uint64_t* n_neurons, uint64_t* n_syn, uint64_t* steps, int32_t* delay) {
     if (!p) return fail(SIMUSE_E_INVALID, "pool_info: null pool");
     if (n_neurons) *n_neurons = p->n;

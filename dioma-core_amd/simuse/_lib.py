@@ -143,6 +143,7 @@ _SIGS = {
     "simuse_pool_step": ([P, C.POINTER(f32p), C.c_uint32, C.c_uint32, u32p, C.c_uint64], C.c_int),
     "simuse_pool_download_state": ([P, C.c_int32, f32p, C.c_uint64], C.c_int),
     "simuse_pool_info": ([P, u64p, u64p, u64p, i32p], C.c_int),
+    "simuse_pool_time_substeps": ([P, C.c_uint32, f32p], C.c_int),
     "simuse_net_reset": ([P, C.POINTER(PopDesc), C.c_int32], C.c_int),
     "simuse_net_set_receptors": ([P, u8p, C.c_float, f32p], C.c_int),
     "simuse_net_download_conductances": ([P, f32p, C.c_uint64], C.c_int),

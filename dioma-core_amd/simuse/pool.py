@@ -147,6 +147,13 @@ class BrainPool:
         check(lib().simuse_pool_step(self._h, ptrs, len(exts), int(spt), fp, flat.size))
         return flat, self.slots()
 
+    def time_substeps(self, n: int) -> float:
+        """n more substeps (submits of <= 64, the last tick's input at each
+        submit's start, no readback) between two HIP events; milliseconds."""
+        out = C.c_float()
+        check(lib().simuse_pool_time_substeps(self._h, int(n), C.byref(out)))
+        return float(out.value)
+
     def download_state(self, col: int) -> np.ndarray:
         """State column of every neuron: 0 v, 1 g, 2 refr_cnt."""
         out = np.zeros(self.n_neurons, dtype=np.float32)

@@ -591,6 +591,11 @@ int simuse_pool_step(simuse_pool* p, const float* const* exts, uint32_t n_creatu
  * (WgpuNetwork::download_states, PW/src/network.rs:1140). */
 int simuse_pool_download_state(simuse_pool* p, int32_t col, float* out, uint64_t n);
 int simuse_pool_info(simuse_pool* p, uint64_t* n_neurons, uint64_t* n_syn, uint64_t* steps, int32_t* delay);
+/* No counterpart in the reference (a measurement helper): n more substeps,
+ * in submits of up to 64 with the last tick's voltage input applied at each
+ * submit's first substep, between two HIP events on the pool's stream; no
+ * readback.  The elapsed time in ms. */
+int simuse_pool_time_substeps(simuse_pool* p, uint32_t n, float* ms);
 
 #ifdef __cplusplus
 }
