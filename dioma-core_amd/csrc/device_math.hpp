@@ -19,7 +19,8 @@ __device__ __forceinline__ bool poisson_roll(uint32_t seed, uint32_t stream, uin
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     z ^= z >> 31;
-    const float u = static_cast<float>(z >> 40) / 16777216.0f;
+    // 24-bit integers convert exactly from 32 bits (one instruction, not the 64-bit sequence)
+    const float u = static_cast<float>(static_cast<uint32_t>(z >> 40)) / 16777216.0f;
     return u < p;
 }
 
@@ -132,8 +133,8 @@ __device__ __forceinline__ void det_sincos2pi(float u, float& cosv, float& sinv)
 __device__ __forceinline__ void normal_pair(uint64_t seed, uint32_t pair, uint64_t step, float& zc, float& zs) {
     const uint64_t base = splitmix(seed ^ (static_cast<uint64_t>(pair) * 0xD1B54A32D192ED03ull));
     const uint64_t h = splitmix(base ^ (step * 0xA24BAED4963EE407ull));
-    const float u1 = static_cast<float>((h >> 40) + 1u) * 5.9604644775390625e-08f;
-    const float u2 = static_cast<float>((h >> 16) & 0xFFFFFFu) * 5.9604644775390625e-08f;
+    const float u1 = static_cast<float>(static_cast<uint32_t>(h >> 40) + 1u) * 5.9604644775390625e-08f;
+    const float u2 = static_cast<float>(static_cast<uint32_t>(h >> 16) & 0xFFFFFFu) * 5.9604644775390625e-08f;
     const float r = __builtin_sqrtf(-2.0f * det_logf(u1));
     float c, s;
     det_sincos2pi(u2, c, s);

@@ -590,6 +590,14 @@ __device__ __forceinline__ uint32_t floor_index(const uint32_t* v, uint32_t lo0,
     return lo;
 }
 
+// First step of a search over n >= 1 entries: the largest power of two
+// below n (0 when n == 1), so that the steps top, top/2, ..., 1 reach every
+// index of [0, n).
+__device__ __forceinline__ uint32_t search_top(uint32_t n) {
+    return n > 1u ? 1u << (31 - __builtin_clz(n - 1u)) : 0u;
+}
+
+
 // CSR extent of row i: first edge and out-degree.  A network whose rows all
 // have the same out-degree d (uni_deg = d) computes it instead of reading
 // row_ptr, which takes a dependent global load off the spike path.
@@ -621,8 +629,7 @@ __device__ __forceinline__ void bin_map_load(const NetArgs& a, BinMap& M) {
 // (bin << 11) | cell of global post `post`.
 __device__ __forceinline__ uint32_t post_bin_cell(const BinMap& M, uint32_t n_pops, uint32_t post) {
     uint32_t p = 0;  // last population starting at or before post
-#pragma unroll
-    for (uint32_t s = kMaxBinPops / 2; s > 0; s >>= 1)
+    for (uint32_t s = search_top(n_pops); s > 0; s >>= 1)
         if (p + s < n_pops && M.off[p + s] <= post) p += s;
     const uint32_t r = post - M.off[p];
     const uint32_t tile = M.tile0[p] + r / kTile;
@@ -1393,6 +1400,77 @@ __device__ __forceinline__ void rec_store(const NetArgs& a, uint32_t post, const
         if ((touched >> r) & 1u) a.g[static_cast<size_t>(r) * a.N + post] = g[r];
 }
 
+// Insert (e, w) into the ascending ids[0..W) holding fewer than W entries
+// (the free slots hold 0xFFFFFFFF): every slot holding a larger id takes its
+// left neighbour's entry, or (e, w) if that one is smaller.
+template <int W, int N>
+__device__ __forceinline__ void sorted_insert(uint32_t (&ids)[N], float (&ws)[N], uint32_t e, float w) {
+#pragma unroll
+    for (int q = W - 1; q >= 0; --q) {
+        const int qp = q > 0 ? q - 1 : 0;
+        const bool prev_gt = q > 0 && ids[qp] > e;
+        if (ids[q] > e) {
+            ids[q] = prev_gt ? ids[qp] : e;
+            ws[q] = prev_gt ? ws[qp] : w;
+        }
+    }
+}
+
+// The current-only form of the short lists: the first record needs no
+// compare, a list of one (most cells that receive anything) is one add, and
+// lists of two to four use a four-slot insertion, so that a wavefront runs
+// the eight-slot network only when one of its cells has five or more.
+// Returns whether the list was longer than kShortList (left for the sort).
+__device__ __forceinline__ bool short_list_sum(BinApplyLds& L, uint32_t p, uint32_t h) {
+    const uint32_t* r_eid = L.r_eid();
+    const uint16_t* nxt = L.nxt();
+    uint16_t nx = nxt[h];
+    float w0 = L.r_w[h];
+    if (nx == kLinkNil) {
+        L.head[p] = kListNil;
+        L.slice[p] = L.slice[p] + w0;
+        return false;
+    }
+    uint32_t ids[kShortList];
+    float ws[kShortList];
+    ids[0] = r_eid[h];
+    ws[0] = w0;
+#pragma unroll
+    for (int q = 1; q < kShortList; ++q) {
+        ids[q] = 0xFFFFFFFFu;
+        ws[q] = 0.0f;
+    }
+    int n = 1;
+    while (nx != kLinkNil && n < 4) {
+        const uint32_t i = nx;
+        nx = nxt[i];
+        sorted_insert<4>(ids, ws, r_eid[i], L.r_w[i]);
+        ++n;
+    }
+    while (nx != kLinkNil) {
+        if (n == kShortList) {
+            L.head[p] = kListLong;
+            return true;
+        }
+        const uint32_t i = nx;
+        nx = nxt[i];
+        sorted_insert<kShortList>(ids, ws, r_eid[i], L.r_w[i]);
+        ++n;
+    }
+    L.head[p] = kListNil;
+    float acc = L.slice[p];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (q < n) acc = acc + ws[q];
+    if (n > 4) {
+#pragma unroll
+        for (int q = 4; q < kShortList; ++q)
+            if (q < n) acc = acc + ws[q];
+    }
+    L.slice[p] = acc;
+    return false;
+}
+
 // Add the pass's m records (linked per cell from head[]) into the slice,
 // each cell in ascending event id, and leave every head empty.  Lists of up
 // to kShortList records are insertion-sorted in registers by their owning
@@ -1409,6 +1487,10 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m, RecCtx rc) {
         const uint32_t p = t + pr * kApply;
         const uint32_t h = L.head[p];
         if (h == kListNil) continue;
+        if (!kRec) {
+            if (short_list_sum(L, p, h)) longs = 1;
+            continue;
+        }
         uint32_t ids[kShortList], cs[kShortList];
         float ws[kShortList];
 #pragma unroll
@@ -1555,34 +1637,59 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m, RecCtx rc) {
 // when kRec, else 8-byte ones, whose per-cell order key is writer << 21 | f).
 // Every thread of the block calls it after zeroing L.slice and emptying
 // L.head; it ends with a barrier.
+// Thread t's header entries of a row of nw writers: writer t's {first
+// record, count} in x, y when nw <= kApply (z = w = 0), else writers 2t
+// and 2t + 1.  One writer per thread spreads the listing of the records
+// over all the block's waves.
+__device__ __forceinline__ uint4 bin_header_load(const uint2* hrow2, uint32_t nw) {
+    static_assert(kMaxWriters == 2 * kApply, "at most two writers per thread");
+    const uint32_t t = threadIdx.x;
+    uint4 hw = make_uint4(0u, 0u, 0u, 0u);
+    if (nw <= kApply) {
+        if (t < nw) {
+            const uint2 h = hrow2[t];
+            hw.x = h.x;
+            hw.y = h.y;
+        }
+    } else if (2u * t < nw) {
+        hw = reinterpret_cast<const uint4*>(hrow2)[t];  // rows hold an even count
+    }
+    return hw;
+}
+
 template <bool kRec>
-__device__ __forceinline__ void bin_gather(BinApplyLds& L, uint2* hrow2, dptr<const uint2> ev, uint32_t nw,
+__device__ __forceinline__ void bin_gather(BinApplyLds& L, uint2* hrow2, uint4 hw, dptr<const uint2> ev, uint32_t nw,
                                            RecCtx rc) {
     const dptr<const uint4> ev4 = reinterpret_cast<dptr<const uint4>>(ev);
-    static_assert(kMaxWriters == 2 * kApply, "two writers per thread");
     const uint32_t t = threadIdx.x;
-    uint4* hrow = reinterpret_cast<uint4*>(hrow2);  // rows hold an even number of writers
-    uint4 hw = make_uint4(0u, 0u, 0u, 0u);
-    if (2 * t < nw) hw = hrow[t];
-    if (hw.y | hw.w) hrow[t] = make_uint4(0u, 0u, 0u, 0u);  // consumed: empty for the next use
+    const bool one_w = nw <= kApply;  // thread t holds writer t only
+    const uint32_t w0 = one_w ? t : 2 * t;
+    if (hw.y | hw.w) {  // consumed: empty for the next use
+        if (one_w) hrow2[t] = make_uint2(0u, 0u);
+        else reinterpret_cast<uint4*>(hrow2)[t] = make_uint4(0u, 0u, 0u, 0u);
+    }
     uint32_t excl;
     const uint32_t total = block_scan<kApply>(hw.y + hw.w, L.wave, excl);
     // One pass takes every record when they fit (the common case): then
-    // each thread lists its two groups' records -- buffer index and writer --
-    // at their positions in the pass, so that a record's thread finds it
-    // with one LDS read instead of a search over the groups.
+    // each thread lists its groups' records -- buffer index and writer -- at
+    // their positions in the pass, so that a record's thread finds it with
+    // one LDS read instead of a search over the groups.
     const bool one = total <= kBinCap;
     uint32_t* raddr = L.seg_base;
     uint32_t* rgrp = L.seg_pref;
     if (one) {
         for (uint32_t i = 0; i < hw.y; ++i) {
             raddr[excl + i] = hw.x + i;
-            rgrp[excl + i] = 2 * t;
+            rgrp[excl + i] = w0;
         }
-        for (uint32_t i = 0; i < hw.w; ++i) {
+        for (uint32_t i = 0; i < hw.w; ++i) {  // hw.w == 0 with one writer per thread
             raddr[excl + hw.y + i] = hw.z + i;
             rgrp[excl + hw.y + i] = 2 * t + 1;
         }
+    } else if (one_w) {  // groups [0, nw] of the searches below, nw <= kApply
+        L.seg_base[t] = hw.x;
+        L.seg_pref[t] = excl;
+        if (t == kApply - 1) L.seg_pref[kApply] = total;
     } else {
         L.seg_base[2 * t] = hw.x;
         L.seg_base[2 * t + 1] = hw.z;
@@ -1736,7 +1843,8 @@ __global__ __launch_bounds__(kApply) void bin_apply_kernel(const NetArgs* __rest
         L.slice[t + pr * kApply] = 0.0f;
         L.head[t + pr * kApply] = kListNil;
     }
-    bin_gather<kRec>(L, hdr + static_cast<size_t>(k) * a.bin_rows, ev, a.n_bblk, RecCtx{(dptr<const NetArgs>)&a, k});
+    uint2* hrow = hdr + static_cast<size_t>(k) * a.bin_rows;
+    bin_gather<kRec>(L, hrow, bin_header_load(hrow, a.n_bblk), ev, a.n_bblk, RecCtx{(dptr<const NetArgs>)&a, k});
     // cell c of the bin -> its post in the slot
     float* out = a.ring + static_cast<size_t>(slot) * a.N;
 #pragma unroll
@@ -1797,35 +1905,6 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
     PopDev PV[2];
     uint32_t li[2], tile[2], pidx[2];
     bool on[2];
-    if (mat) {
-        // a host call (current injection, ring download) has applied them
-        float* ring_slot = a.ring + static_cast<size_t>(step & (kSlots - 1)) * a.N;
-#pragma unroll
-        for (uint32_t q = 0; q < 2; ++q) {
-            tile[q] = tile_of(q);
-            PV[q] = pop_dev(pop_of(q));
-            li[q] = (tile[q] - PV[q].tile0) * kTile + tl;
-            on[q] = tile[q] < a.n_tiles && li[q] < PV[q].n;
-            if (!on[q]) continue;
-            const uint32_t i = PV[q].off + li[q];
-            cur[q] = ring_slot[i];
-            ring_slot[i] = 0.0f;
-        }
-        __syncthreads();
-    } else {
-#pragma unroll
-        for (uint32_t pr = 0; pr < kBinPosts / kApply; ++pr) {
-            L.slice[t + pr * kApply] = 0.0f;
-            L.head[t + pr * kApply] = kListNil;
-        }
-        const uint32_t rpar = wpar ^ 1u;
-        bin_gather<false>(L, a.bin_hdr + rpar * a.hdr_stride + static_cast<size_t>(k) * a.bin_rows,
-                          ev_buf(a, rpar), a.n_bblk, RecCtx{nullptr, 0u});
-#pragma unroll
-        for (uint32_t q = 0; q < 2; ++q) cur[q] = L.slice[t + q * kApply];
-    }
-    uint32_t row0[2], deg[2];
-    Neuron<MODEL> nrn[2];
 #pragma unroll
     for (uint32_t q = 0; q < 2; ++q) {
         tile[q] = tile_of(q);
@@ -1833,16 +1912,12 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
         PV[q] = pop_dev(pidx[q]);
         li[q] = (tile[q] - PV[q].tile0) * kTile + tl;
         on[q] = tile[q] < a.n_tiles && li[q] < PV[q].n;
-        row0[q] = deg[q] = 0u;
-        if (on[q]) nrn[q].load(PV[q], li[q]);
     }
-    // the bin map (read by the writer phase) loads behind the state, not in
-    // front of wave 0's header load; valid after the barrier below
-    bin_map_load(a, M);
-
-    // Gaussian input: the thread's two neurons are li and li + 1024 of one
+    // Gaussian input, drawn before the gather (it depends on (seed, neuron,
+    // step) only): measured, the draw costs less here than after the gather,
+    // where the state loads wait on it.  The thread's two neurons are li and li + 1024 of one
     // population whenever its bin lies inside that population, and then
-    // share one Box-Muller draw (the pair of normal_pair)
+    // share one Box-Muller draw (the pair of normal_pair).
     float nz[2] = {0.0f, 0.0f};
     if (on[0] && on[1] && pidx[0] == pidx[1] && (PV[0].flags & kPopNoise) && (li[0] & 1024u) == 0u &&
         li[1] == li[0] + 1024u) {
@@ -1862,6 +1937,44 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
             }
         }
     }
+    if (mat) {
+        // a host call (current injection, ring download) has applied them
+        float* ring_slot = a.ring + static_cast<size_t>(step & (kSlots - 1)) * a.N;
+#pragma unroll
+        for (uint32_t q = 0; q < 2; ++q) {
+            if (!on[q]) continue;
+            const uint32_t i = PV[q].off + li[q];
+            cur[q] = ring_slot[i];
+            ring_slot[i] = 0.0f;
+        }
+        __syncthreads();
+    } else {
+#pragma unroll
+        for (uint32_t pr = 0; pr < kBinPosts / kApply; ++pr) {
+            L.slice[t + pr * kApply] = 0.0f;
+            L.head[t + pr * kApply] = kListNil;
+        }
+        const uint32_t rpar = wpar ^ 1u;
+        uint2* hrow = a.bin_hdr + rpar * a.hdr_stride + static_cast<size_t>(k) * a.bin_rows;
+        bin_gather<false>(L, hrow, bin_header_load(hrow, a.n_bblk), ev_buf(a, rpar), a.n_bblk, RecCtx{nullptr, 0u});
+#pragma unroll
+        for (uint32_t q = 0; q < 2; ++q) cur[q] = L.slice[t + q * kApply];
+    }
+    uint32_t row0[2], deg[2];
+    Neuron<MODEL> nrn[2];
+#pragma unroll
+    for (uint32_t q = 0; q < 2; ++q) {
+        tile[q] = tile_of(q);
+        pidx[q] = pop_of(q);
+        PV[q] = pop_dev(pidx[q]);
+        li[q] = (tile[q] - PV[q].tile0) * kTile + tl;
+        on[q] = tile[q] < a.n_tiles && li[q] < PV[q].n;
+        row0[q] = deg[q] = 0u;
+        if (on[q]) nrn[q].load(PV[q], li[q]);
+    }
+    // the bin map (read by the writer phase) loads behind the state, not in
+    // front of wave 0's header load; valid after the barrier below
+    bin_map_load(a, M);
 
     // advance every neuron; per tile ballot the spikes and scan the spiking
     // rows' lengths within each wave
