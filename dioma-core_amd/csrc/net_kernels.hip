@@ -797,13 +797,32 @@ __device__ __forceinline__ void ev_cache_load(const NetArgs& a, const BinMap& M,
     const uint32_t* col = is_src ? a.src_post : a.col;
     const float* w = is_src ? a.src_w : a.w;
     uint32_t post[kEvCache];
+    if (a.uni_deg && !is_src) {
+        // every row holds d edges, so event f is edge f mod d of row f / d:
+        // the quotient from one float multiply (error below one for f <
+        // 2^22 d) corrected by one, in place of the search over s_pref
+        const uint32_t d = a.uni_deg;
+        const float inv = 1.0f / static_cast<float>(d);
 #pragma unroll
-    for (uint32_t u = 0; u < kEvCache; ++u) {
-        const uint32_t f = threadIdx.x + u * kThreads;
-        c.e[u] = 0u;
-        if (f < total) {
-            const uint32_t k = floor_index<2048>(s_pref, 0, cnt, f);
-            c.e[u] = s_rs[k] + (f - s_pref[k]);
+        for (uint32_t u = 0; u < kEvCache; ++u) {
+            const uint32_t f = threadIdx.x + u * kThreads;
+            c.e[u] = 0u;
+            if (f < total) {
+                uint32_t k = static_cast<uint32_t>(static_cast<float>(f) * inv);
+                if (k * d > f) --k;
+                else if ((k + 1u) * d <= f) ++k;
+                c.e[u] = s_rs[k] + (f - k * d);
+            }
+        }
+    } else {
+#pragma unroll
+        for (uint32_t u = 0; u < kEvCache; ++u) {
+            const uint32_t f = threadIdx.x + u * kThreads;
+            c.e[u] = 0u;
+            if (f < total) {
+                const uint32_t k = floor_index<2048>(s_pref, 0, cnt, f);
+                c.e[u] = s_rs[k] + (f - s_pref[k]);
+            }
         }
     }
 #pragma unroll
