@@ -105,8 +105,8 @@ def graph_steps_for(steps: int, most: int) -> int:
 
 def form_name(form: int) -> str:
     import simuse
-    return {simuse.FORM_ONE_KERNEL: "one_kernel", simuse.FORM_BINNED: "binned_two_kernel",
-            simuse.FORM_CHAINED: "chained"}[form]
+    return {simuse.FORM_ONE_KERNEL: "one_kernel", simuse.FORM_SPLIT: "split_two_kernel",
+            simuse.FORM_BINNED: "binned_three_kernel", simuse.FORM_CHAINED: "chained"}[form]
 
 
 # -- FP64 checks (the oracle's FP64 twin is the reference of every error) -------
@@ -657,9 +657,7 @@ def bench_precision(args) -> dict:
                           "steps": args.prec_full_steps, "us_per_step": us,
                           "region_us_per_step": ms * 1e3 / args.prec_full_steps,
                           "spikes_per_step": sp, "events_per_step": ev,
-                          "roofline": {"bound": "hbm", "kernel": "one step: neuron_kernel<HH> (update, receptor "
-                                                                   "currents, STDP traces, compaction), bin_kernel "
-                                                                   "(STDP, event records), bin_apply_kernel<receptors>",
+                          "roofline": {"bound": "hbm", "kernel": step_kernels(form),
                                        "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                                        "algorithmic_bytes_per_step": step_bytes,
@@ -670,6 +668,15 @@ def bench_precision(args) -> dict:
                                        "step_ms": q_ms,
                                        "step_ms_source": "HIP events around the timed region (graph replays) "
                                                          "/ steps"}}}
+
+
+def step_kernels(form: int) -> str:
+    """The launches of one step of a plastic receptor network in `form`."""
+    if form == simuse.FORM_SPLIT:
+        return ("one step: front_kernel<HH, receptors> (the bin's conductance jumps and currents in event order, "
+                "update, receptor currents, STDP traces, compaction), bin_kernel (STDP, event records, in-edge LTP)")
+    return ("one step: neuron_kernel<HH> (update, receptor currents, STDP traces, compaction), bin_kernel (STDP, "
+            "event records), bin_apply_kernel<receptors>")
 
 
 # -- Kuramoto: the phase-coupling kernel at 1M oscillators / 20M edges ---------------

@@ -42,6 +42,8 @@ void launch_bin(const NetArgs* nets, const uint4* refs, uint32_t n, uint32_t bin
 void launch_bin_apply(const NetArgs* nets, const uint2* refs, uint32_t n, uint32_t mode, hipStream_t s);
 void launch_bin_apply_rec(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s);
 void launch_megastep(int model, const NetArgs* nets, const uint2* items, uint32_t n, hipStream_t s);
+void launch_front(int model, bool rec, const NetArgs* nets, const uint2* items, uint32_t n, hipStream_t s);
+void launch_bin_apply_rec_mode(const NetArgs* nets, const uint2* refs, uint32_t n, uint32_t mode, hipStream_t s);
 
 void launch_head_clear(unsigned long long* head, const uint64_t* step, size_t n, hipStream_t s);
 void launch_inject(float* ring, const uint64_t* step, uint32_t N, const uint32_t* idx, const float* val,
@@ -133,6 +135,12 @@ struct NetState {
     bool no_mega = false;   // left for the two-kernel path (dense STDP step, plasticity)
     bool mega = false;      // running it (set with the work lists)
     int mega_model = 0;
+    // split form (binned, one model, pair-based plasticity or receptor
+    // synapses): front_kernel + bin_kernel, the same parity buffers
+    bool split_ok = false;
+    bool split = false;     // running it (set with the work lists)
+    bool no_split = false;  // left for the three-kernel path (dense STDP step)
+    uint32_t nbuf() const { return mega_ok || split_ok ? 2u : 1u; }  // header and record buffers
     uint32_t* bin_done = nullptr;
     uint32_t* bin_mat = nullptr;
     std::vector<uint2> mat_items;      // (engine index, bin) of every bin, XCD-ordered
@@ -386,6 +394,8 @@ struct Engine {
     uint32_t n_tiles_m[kModels] = {}, cap_tiles_m[kModels] = {};
     uint2* d_mega[kModels] = {};       // megastep work items per model: {net, bin}
     uint32_t n_mega_m[kModels] = {}, cap_mega_m[kModels] = {};
+    uint2* d_front[2 * kModels] = {};  // front_kernel items per (model, receptors): {net, bin * parts + part}
+    uint32_t n_front_m[2 * kModels] = {}, cap_front_m[2 * kModels] = {};
     uint2* d_blocks = nullptr;  // insert_kernel work: plastic networks' tiles, source tiles
     uint32_t n_blocks = 0, cap_blocks = 0;
     uint2* d_stdp = nullptr;
@@ -423,6 +433,8 @@ struct Engine {
         for (uint4* t : d_tiles)
             if (t) (void)hipFree(t);
         for (uint2* t : d_mega)
+            if (t) (void)hipFree(t);
+        for (uint2* t : d_front)
             if (t) (void)hipFree(t);
         for (void* p : {static_cast<void*>(d_nets), static_cast<void*>(d_blocks), static_cast<void*>(d_stdp),
                         static_cast<void*>(d_fix), static_cast<void*>(d_bin), static_cast<void*>(d_apply),
@@ -489,7 +501,7 @@ struct Engine {
         HIP_TRY(hipStreamSynchronize(stream));
         std::vector<NetArgs> na;
         std::vector<uint4> tl[kModels], binl;
-        std::vector<uint2> bl, sl, fl, apl, aplr, ml[kModels], cpl;
+        std::vector<uint2> bl, sl, fl, apl, aplr, ml[kModels], fr[2 * kModels], cpl;
         bins_fused = bins_bin = 0;
         for (uint32_t k = 0; k < nets.size(); ++k) {
             NetState* s = nets[k];
@@ -519,9 +531,39 @@ struct Engine {
                     binl.push_back(make_uint4(k, b, std::min(kBinBlockTiles, n_blk - b), jw));
                 if (jw > s->bin_rows) return fail(SIMUSE_E_STATE, "megastep: %u writers, %u rows", jw, s->bin_rows);
                 A.n_bblk = jw;
+                A.n_done = jw;
                 A.mega = 1;
                 A.bin_fused = 0;
                 if (s->n_src_tiles) bins_bin = std::max(bins_bin, s->n_bins);
+                continue;
+            }
+            // split form: front blocks (kFrontParts per bin), then bin_kernel's
+            // writers over every scatter tile and the in-edge LTP blocks, all
+            // counted for the step's advance
+            s->split = s->binned && s->split_ok && !s->no_split && !s->mega;
+            if (s->split) {
+                s->mat_items.clear();
+                const int fi = 2 * s->mega_model + (s->rec_code ? 1 : 0);
+                for (uint32_t b = 0; b < s->n_bins; ++b) {
+                    for (uint32_t h = 0; h < kFrontParts; ++h) fr[fi].push_back(make_uint2(k, b * kFrontParts + h));
+                    s->mat_items.push_back(make_uint2(k, b));
+                }
+                xcd_order(s->mat_items, make_uint2(kEmpty, 0u));
+                s->mat_dirty = true;
+                const uint32_t T = std::max(kBinBlockTiles, (n_blk + kMaxWriters - 1) / kMaxWriters);
+                uint32_t jw = 0, nd = 0;
+                for (uint32_t b = 0; b < n_blk; b += T, ++jw, ++nd)
+                    binl.push_back(make_uint4(k, b, std::min(T, n_blk - b), jw));
+                if (jw > s->bin_rows) return fail(SIMUSE_E_STATE, "split form: %u writers, %u rows", jw, s->bin_rows);
+                if (s->stdp)
+                    for (uint32_t b = 0; b < s->n_tiles; b += kBinBlockTiles, ++nd)
+                        binl.push_back(make_uint4(k, b, std::min(kBinBlockTiles, s->n_tiles - b), kLtpItem));
+                A.n_bblk = jw;
+                A.n_done = nd;
+                A.mega = 1;
+                A.split = 1;
+                A.bin_fused = 0;
+                bins_bin = std::max(bins_bin, s->n_bins);
                 continue;
             }
             uint32_t j = 0;
@@ -586,6 +628,14 @@ struct Engine {
                 if ((rc = grow(&d_mega[m], &cap_mega_m[m], n_mega_m[m]))) return rc;
                 HIP_TRY(hipMemcpy(d_mega[m], ml[m].data(), sizeof(uint2) * ml[m].size(), hipMemcpyHostToDevice));
             }
+            for (int r = 0; r < 2; ++r) {
+                std::vector<uint2>& f = fr[2 * m + r];
+                xcd_order(f, make_uint2(kEmpty, 0u));
+                n_front_m[2 * m + r] = static_cast<uint32_t>(f.size());
+                if (f.empty()) continue;
+                if ((rc = grow(&d_front[2 * m + r], &cap_front_m[2 * m + r], n_front_m[2 * m + r]))) return rc;
+                HIP_TRY(hipMemcpy(d_front[2 * m + r], f.data(), sizeof(uint2) * f.size(), hipMemcpyHostToDevice));
+            }
             xcd_order(tl[m], make_uint4(kEmpty, 0u, 0u, 0u));
             n_tiles_m[m] = static_cast<uint32_t>(tl[m].size());
             if (tl[m].empty()) continue;
@@ -640,6 +690,8 @@ struct Engine {
         launch_coupling(d_nets, d_cpl, n_cpl, stream);
         for (int m = 0; m < kModels; ++m) launch_megastep(m, d_nets, d_mega[m], n_mega_m[m], stream);
         for (int m = 0; m < kModels; ++m)
+            for (int r = 0; r < 2; ++r) launch_front(m, r != 0, d_nets, d_front[2 * m + r], n_front_m[2 * m + r], stream);
+        for (int m = 0; m < kModels; ++m)
             launch_neuron(m, sched, d_nets, d_tiles[m], n_tiles_m[m], stream, 0u, bins_fused);
         if (e) (void)hipEventRecord(e[1], stream);
         launch_stdp(d_nets, d_stdp, n_stdp, stream);
@@ -679,13 +731,14 @@ struct Engine {
         return SIMUSE_OK;
     }
 
-    // A megastep network keeps its pending events (the last step's records)
-    // out of the ring; apply them into ring slot step % 32 before anything
-    // reads or adds to that slot from the host side.  Idempotent.
+    // A megastep or split-form network keeps its pending events (the last
+    // step's records) out of the ring; apply them into ring slot step % 32
+    // (and receptor jumps into the conductances) before anything reads or
+    // adds to that slot from the host side.  Idempotent.
     int materialize(NetState* s) {
         int rc;
         if ((rc = ensure_tables())) return rc;
-        if (!s->finalized || !s->mega) return SIMUSE_OK;
+        if (!s->finalized || !(s->mega || s->split)) return SIMUSE_OK;
         if (s->mat_dirty) {
             if (s->mat_items.size() > static_cast<size_t>(s->n_bins) + kXcdPad)
                 return fail(SIMUSE_E_STATE, "materialize: %zu work items exceed the list's room", s->mat_items.size());
@@ -693,17 +746,27 @@ struct Engine {
                               hipMemcpyHostToDevice));
             s->mat_dirty = false;
         }
-        launch_bin_apply(d_nets, s->d_mat_items, static_cast<uint32_t>(s->mat_items.size()), 1u, stream);
+        if (s->ev_wide) launch_bin_apply_rec_mode(d_nets, s->d_mat_items, static_cast<uint32_t>(s->mat_items.size()), 1u, stream);
+        else launch_bin_apply(d_nets, s->d_mat_items, static_cast<uint32_t>(s->mat_items.size()), 1u, stream);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(s->bin_mat), 1, 1, stream));
         return SIMUSE_OK;
     }
-    // Move a megastep network to the two-kernel path for good.
-    int leave_megastep(NetState* s) {
+    // Move a megastep network to the two-kernel path for good (and, with
+    // split, a split-form network to the three-kernel path).  The split
+    // form's writers fill whole header columns, which that path would read
+    // as events: both parities are cleared.
+    int leave_megastep(NetState* s, bool split = false) {
         int rc;
         if ((rc = materialize(s))) return rc;
         if (s->mega_ok && !s->no_mega) {
             s->no_mega = true;
+            mark_dirty();
+        }
+        if (split && s->split_ok && !s->no_split) {
+            s->no_split = true;
+            if (s->bin_hdr)
+                HIP_TRY(hipMemsetAsync(s->bin_hdr, 0, sizeof(uint2) * s->nbuf() * s->n_bins * s->bin_rows, stream));
             mark_dirty();
         }
         return SIMUSE_OK;

@@ -194,10 +194,13 @@ int NetState::finalize(hipStream_t stream) {
         // writer per bin
         mega_ok = one_model && !stdp && !ev_wide && !coupled() && n_bins + ns <= kMaxWriters;
         mega_model = pops[0].model;
-        const size_t nbuf = mega_ok ? 2 : 1;
-        if ((rc = alloc(&bin_ev, nbuf * n_ev * (ev_wide ? 2 : 1)))) return rc;
-        if ((rc = alloc(&bin_hdr, nbuf * n_bins * bin_rows))) return rc;
-        HIP_TRY(hipMemsetAsync(bin_hdr, 0, sizeof(uint2) * nbuf * n_bins * bin_rows, stream));
+        // the split form takes the rest of the one-model networks (pair-based
+        // plasticity, receptor synapses, wide records)
+        split_ok = one_model && !mega_ok && !coupled();
+        const size_t nb = nbuf();
+        if ((rc = alloc(&bin_ev, nb * n_ev * (ev_wide ? 2 : 1)))) return rc;
+        if ((rc = alloc(&bin_hdr, nb * n_bins * bin_rows))) return rc;
+        HIP_TRY(hipMemsetAsync(bin_hdr, 0, sizeof(uint2) * nb * n_bins * bin_rows, stream));
         if ((rc = alloc(&d_dense_writers, bin_rows + kXcdPad))) return rc;  // XCD-ordered, padded
         if ((rc = alloc(&d_mat_items, n_bins + kXcdPad))) return rc;
         if ((rc = alloc(&bin_slot, 1))) return rc;
@@ -449,7 +452,7 @@ int NetState::reset_dynamic(hipStream_t stream) {
     clear_heads(stream);
     if (tile_coll) HIP_TRY(hipMemsetAsync(tile_coll, 0, sizeof(uint32_t) * (n_tiles + n_src_tiles), stream));
     if (bin_hdr)
-        HIP_TRY(hipMemsetAsync(bin_hdr, 0, sizeof(uint2) * (mega_ok ? 2 : 1) * n_bins * bin_rows, stream));
+        HIP_TRY(hipMemsetAsync(bin_hdr, 0, sizeof(uint2) * nbuf() * n_bins * bin_rows, stream));
     if (bin_done) HIP_TRY(hipMemsetAsync(bin_done, 0, sizeof(uint32_t), stream));
     if (bin_mat) HIP_TRY(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(bin_mat), 1, 1, stream));  // zeroed ring
     HIP_TRY(hipMemsetAsync(tile_count, 0, sizeof(uint32_t) * n_tiles, stream));
@@ -573,6 +576,8 @@ int NetState::download_conductances(float* out, uint64_t len, hipStream_t stream
         std::memset(out, 0, sizeof(float) * len);  // not stepped yet
         return SIMUSE_OK;
     }
+    int rc;
+    if (eng && (rc = eng->materialize(this))) return rc;  // the split form's pending jumps into g
     HIP_TRY(hipMemcpyAsync(out, g, sizeof(float) * len, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     return SIMUSE_OK;
@@ -1010,8 +1015,7 @@ int Engine::autotune(uint32_t trial_steps) {
         for (NetState* s : nets) {
             s->clear_heads(stream);
             if (s->bin_hdr)  // the trial steps' pending events
-                HIP_TRY(hipMemsetAsync(s->bin_hdr, 0, sizeof(uint2) * (s->mega_ok ? 2 : 1) * s->n_bins * s->bin_rows,
-                                       stream));
+                HIP_TRY(hipMemsetAsync(s->bin_hdr, 0, sizeof(uint2) * s->nbuf() * s->n_bins * s->bin_rows, stream));
             if (s->bin_done) HIP_TRY(hipMemsetAsync(s->bin_done, 0, sizeof(uint32_t), stream));
         }
         HIP_TRY(hipGetLastError());
@@ -1260,7 +1264,7 @@ int simuse_net_stdp_step(simuse_net* net, float trace_decay, float lr_ltp, float
     Engine& e = net->eng;
     if (s.stdp) return fail(SIMUSE_E_STATE, "stdp_step: pair-based plasticity is enabled on this network");
     if ((rc = e.prepare())) return rc;
-    if ((rc = e.leave_megastep(&s))) return rc;  // this step's update, weights and push are separate kernels
+    if ((rc = e.leave_megastep(&s, true))) return rc;  // this step's update, weights and push are separate kernels
     if ((rc = e.prepare())) return rc;
     const uint32_t n_blk = s.n_tiles + s.n_src_tiles;
     if (!s.dtrace) {
@@ -1688,7 +1692,10 @@ int simuse_net_step_form(simuse_net* net, int32_t* form) {
         int r = net->eng.prepare();
         if (r) return r;
         const NetState& s = net->st;
-        *form = s.mega ? SIMUSE_FORM_ONE_KERNEL : s.binned ? SIMUSE_FORM_BINNED : SIMUSE_FORM_CHAINED;
+        *form = s.mega    ? SIMUSE_FORM_ONE_KERNEL
+                : s.split ? SIMUSE_FORM_SPLIT
+                : s.binned ? SIMUSE_FORM_BINNED
+                           : SIMUSE_FORM_CHAINED;
         return SIMUSE_OK;
     });
 }

@@ -863,7 +863,9 @@ __device__ __forceinline__ void ev_cache_write(const NetArgs& a, bool is_src, co
 // Writer j's per-bin counts (s_bin) become the absolute first record of each
 // bin's group, from `base`; column j of `hdr` gets {start, count} for each
 // bin it sends events to (consumers zero the entries they read, so the
-// others read as empty).  Synchronises before and after.
+// others read as empty), or, in the split form, for every bin (its several
+// consumer blocks per bin leave the row as read).  Synchronises before and
+// after.
 template <uint32_t kThreads>
 __device__ __forceinline__ void bin_offsets(const NetArgs& a, uint32_t* s_bin, uint32_t* s_wave, uint32_t j,
                                             uint32_t base, uint32_t events, uint2* hdr) {
@@ -875,7 +877,7 @@ __device__ __forceinline__ void bin_offsets(const NetArgs& a, uint32_t* s_bin, u
         (void)block_scan<kThreads>(c, s_wave, excl);
         if (t < nb) {
             s_bin[t] = base + excl;
-            if (c) hdr[static_cast<size_t>(t) * a.bin_rows + j] = make_uint2(base + excl, c);
+            if (c || a.split) hdr[static_cast<size_t>(t) * a.bin_rows + j] = make_uint2(base + excl, c);
         }
         __syncthreads();
         return;
@@ -884,17 +886,17 @@ __device__ __forceinline__ void bin_offsets(const NetArgs& a, uint32_t* s_bin, u
     else lds_scan<kThreads, (kMaxBins + kThreads - 1) / kThreads>(s_bin, nb, s_wave);
     for (uint32_t k = t; k < nb; k += kThreads) {
         const uint32_t o = s_bin[k], e = k + 1 < nb ? s_bin[k + 1] : events;
-        if (e != o) hdr[static_cast<size_t>(k) * a.bin_rows + j] = make_uint2(base + o, e - o);
+        if (e != o || a.split) hdr[static_cast<size_t>(k) * a.bin_rows + j] = make_uint2(base + o, e - o);
     }
     __syncthreads();
     for (uint32_t k = t; k < nb; k += kThreads) s_bin[k] += base;
     __syncthreads();
 }
 
-// Megastep: the network's last block of the step (of n_bblk writers) moves
-// the step counter on and clears the block count and the ring flag.
+// Megastep and split form: the network's last block of the step (of n_done)
+// moves the step counter on and clears the block count and the ring flag.
 __device__ __forceinline__ void mega_done(const NetArgs& a, uint64_t step) {
-    if (atomicAdd(a.bin_done, 1u) + 1u == a.n_bblk) {
+    if (atomicAdd(a.bin_done, 1u) + 1u == a.n_done) {
         *a.bin_done = 0u;
         *a.bin_mat = 0u;
         *a.step = step + 1;
@@ -1248,6 +1250,10 @@ __global__ __launch_bounds__(kTile) void bin_kernel(const NetArgs* __restrict__ 
     const uint32_t n_nt = b0 < a.n_tiles ? min(b1, a.n_tiles) - b0 : 0u;  // neuron tiles (<= kTile)
     if (j == kLtpItem) {  // plastic networks: the in-edge LTP of tiles [b0, b1), beside the writers
         stdp_in_edges((dptr<const NetArgs>)&a, b0, b0 + n_nt, s_rs, s_pref, s_wave, s_tp);
+        if (a.mega) {  // split form: counted with the writers, so the step moves on after both
+            __syncthreads();
+            if (t == 0) mega_done(a, step);
+        }
         return;
     }
     const uint32_t base = a.tile_ebase[b0];
@@ -1388,7 +1394,8 @@ __device__ __forceinline__ void bin_link(BinApplyLds& L, uint32_t q, uint32_t ce
 // starting from the stored conductance; `touched` marks the loaded ones.
 struct RecCtx {
     dptr<const NetArgs> a;
-    uint32_t k;  // bin
+    uint32_t k;                 // bin
+    uint32_t c0 = 0u, c1 = kBinPosts;  // the cells this block applies (split form: part of the bin)
 };
 
 // Global neuron of cell c of bin k (a cell that received events).
@@ -1508,6 +1515,10 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m, RecCtx rc) {
         if (h == kListNil) continue;
         if (!kRec) {
             if (short_list_sum(L, p, h)) longs = 1;
+            continue;
+        }
+        if (p < rc.c0 || p >= rc.c1) {  // another block's cell: its conductances are not touched here
+            L.head[p] = kListNil;
             continue;
         }
         uint32_t ids[kShortList], cs[kShortList];
@@ -1678,12 +1689,12 @@ __device__ __forceinline__ uint4 bin_header_load(const uint2* hrow2, uint32_t nw
 
 template <bool kRec>
 __device__ __forceinline__ void bin_gather(BinApplyLds& L, uint2* hrow2, uint4 hw, dptr<const uint2> ev, uint32_t nw,
-                                           RecCtx rc) {
+                                           RecCtx rc, bool zero_row = true) {
     const dptr<const uint4> ev4 = reinterpret_cast<dptr<const uint4>>(ev);
     const uint32_t t = threadIdx.x;
     const bool one_w = nw <= kApply;  // thread t holds writer t only
     const uint32_t w0 = one_w ? t : 2 * t;
-    if (hw.y | hw.w) {  // consumed: empty for the next use
+    if ((hw.y | hw.w) && zero_row) {  // consumed: empty for the next use
         if (one_w) hrow2[t] = make_uint2(0u, 0u);
         else reinterpret_cast<uint4*>(hrow2)[t] = make_uint4(0u, 0u, 0u, 0u);
     }
@@ -2124,6 +2135,160 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
     else bin_events<true, kApply, false, false, true>(a, M, false, blk_sp, blk_ed, s_rs, s_pref, s_bin, ev, step);
     __syncthreads();
     if (t == 0) mega_done(a, step);
+}
+
+// Split form, for binned networks the megastep cannot take (pair-based
+// plasticity, receptor synapses): per step front_kernel, then bin_kernel's
+// writers (with the out-edge STDP) and in-edge LTP blocks, the last of which
+// moves the step counter on -- two launches instead of neuron_kernel,
+// bin_kernel and bin_apply_kernel.  Work item {network, bin * kFrontParts +
+// part}: the block gathers the previous step's records of the whole bin (from
+// the parity buffers, leaving the header row as read, since the bin's other
+// part reads it too) and applies those of its own cells, part * kApply + t for
+// thread t: the current into the cell's input and, in receptor networks, each
+// conductance jump into g in event order (bin_apply_kernel's arithmetic).
+// Then it advances its neurons as neuron_kernel does (receptor currents, STDP
+// traces) and leaves each tile's ascending spiking ids and count for the
+// writers.  One neuron per thread keeps twice as many blocks at work as a
+// block per bin would.
+static_assert(kFrontParts * kApply == kBinPosts, "a front block's cells are one per thread");
+constexpr uint32_t kFrontTiles = kApply / kTile;  // tiles of one part
+
+template <int MODEL, bool kRec>
+constexpr int kFrontWaves = !kRec && (MODEL <= 1 || MODEL == 4) ? 8 : 4;
+
+template <int MODEL, bool kRec>
+__global__ __launch_bounds__(kApply, (kFrontWaves<MODEL, kRec>)) void front_kernel(const NetArgs* __restrict__ nets,
+                                                                          const uint2* __restrict__ items) {
+    __shared__ BinApplyLds L;
+    __shared__ uint32_t s_cnt[kFrontTiles][kWavesPerTile], s_drv[kFrontTiles][kWavesPerTile];
+    const uint2 it = items[blockIdx.x];
+    if (it.x == kEmpty) return;  // pad of the XCD-ordered list
+    const NetArgs& a = nets[it.x];
+    const uint32_t k = it.y / kFrontParts, part = it.y % kFrontParts;
+    const uint32_t t = threadIdx.x, lane = t & 63u, wq = (t / 64u) % kWavesPerTile, tl = t % kTile;
+    const uint64_t step = *a.step;
+    const uint32_t mat = *a.bin_mat;
+    if (step >= a.step_limit) return;
+
+    // thread t: cell part * kApply + t of bin k, in the wave-uniform tile
+    // tile_t; population descriptors through scalar loads, as in the megastep
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    const uint32_t ti = wv / kWavesPerTile;
+    const uint32_t tile = k * kBinTiles + part * kFrontTiles + ti;
+    const uint32_t pidx = tile < a.n_tiles && a.n_pops > 1 ? ((cptr<uint32_t>)a.tile_pop)[tile] : 0u;
+    const PopDev P = ((cptr<PopDev>)a.pops)[pidx];
+    const uint32_t li = (tile - P.tile0) * kTile + tl;
+    const bool on = tile < a.n_tiles && li < P.n;
+    const uint32_t gi = P.off + li;
+    // The neuron's state and traces: with a block per CU (four waves per
+    // SIMD) there are registers to load them before the gather, which
+    // touches neither; with two, after it.
+    constexpr bool kEarly = kFrontWaves<MODEL, kRec> == 4;
+    Neuron<MODEL> nrn;
+    float trp = 0.0f, trq = 0.0f;
+    auto load_state = [&]() {
+        if (on) {
+            nrn.load(P, li);
+            if (a.stdp) {
+                trp = a.tr_pre[gi];
+                trq = a.tr_post[gi];
+            }
+        }
+    };
+    if (kEarly) load_state();
+    float nz = 0.0f;
+    if (on && (P.flags & kPopNoise)) {
+        const float sg = (P.flags & kPopNoiseScalar) ? P.sigma_uni : P.sigma[li];
+        nz = sg * normal_draw(P.seed, li, step);
+    }
+    float cur = 0.0f;
+    if (mat) {
+        // a host call (current injection, ring download) has applied them
+        float* ring_slot = a.ring + static_cast<size_t>(step & (kSlots - 1)) * a.N;
+        if (on) {
+            cur = ring_slot[gi];
+            ring_slot[gi] = 0.0f;
+        }
+    } else {
+#pragma unroll
+        for (uint32_t pr = 0; pr < kBinPosts / kApply; ++pr) {
+            L.slice[t + pr * kApply] = 0.0f;
+            L.head[t + pr * kApply] = kListNil;
+        }
+        const uint32_t rpar = static_cast<uint32_t>(step & 1u) ^ 1u;
+        uint2* hrow = a.bin_hdr + rpar * a.hdr_stride + static_cast<size_t>(k) * a.bin_rows;
+        bin_gather<kRec>(L, hrow, bin_header_load(hrow, a.n_bblk), ev_buf(a, rpar), a.n_bblk,
+                         RecCtx{(dptr<const NetArgs>)&a, k, part * kApply, (part + 1u) * kApply}, false);
+        cur = L.slice[part * kApply + t];
+    }
+
+    // the update of neuron_kernel: conductances (after this step's jumps),
+    // drive, noise, receptor currents, STDP traces
+    if (!kEarly) load_state();
+    RecIn rin;
+    int spike = 0, drive_now = 0;
+    if (on) {
+        if (kRec) rec_load(a, gi, rin);
+        float I = cur;
+        if ((P.flags & kPopDrive) && P.drive_p > 0.0f) {
+            const uint32_t sd = P.drive_seed0 + li * 2654435761u;
+            if (step > 0 && poisson_roll(sd, 0u, step - 1, P.drive_p)) I = I + P.drive_w;
+            drive_now = poisson_roll(sd, 0u, step, P.drive_p) ? 1 : 0;
+        }
+        if (P.flags & kPopNoise) I = I + nz;
+        if (kRec) I = receptor_input(a, P, gi, nrn.vm(P, li), rin, I);
+        spike = nrn.step(P, li, I, P.puni[0]);
+        nrn.store(P, li);
+        if (a.stdp) {
+            const float add = spike ? 1.0f : 0.0f;
+            a.tr_pre[gi] = fmaf(trp, a.stdp_dpre, add);
+            a.tr_post[gi] = fmaf(trq, a.stdp_dpost, add);
+        }
+    }
+    const uint64_t m = __ballot(spike);
+    const uint32_t drive_events = static_cast<uint32_t>(__popcll(__ballot(drive_now)));
+    if (lane == 0) {
+        s_cnt[ti][wq] = static_cast<uint32_t>(__popcll(m));
+        s_drv[ti][wq] = drive_events;
+        if (tile < a.n_tiles) {
+            const uint32_t word = P.word0 + (tile - P.tile0) * kWavesPerTile + wq;
+            a.bits[word] = m;
+            if (a.raster) {
+                const uint64_t rs = step - *a.raster_base;
+                if (rs < a.raster_cap) a.raster[rs * a.n_words + word] = m;
+            }
+        }
+    }
+    __syncthreads();
+
+    // each tile's spiking ids in ascending order, and its count
+    if ((m >> lane) & 1ull) {
+        uint32_t base = 0;
+        for (uint32_t w = 0; w < wq; ++w) base += s_cnt[ti][w];
+        const uint32_t rank = base + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+        a.active[static_cast<size_t>(tile) * kTile + rank] = gi;
+    }
+    if (tl == 0 && tile < a.n_tiles) {
+        uint32_t c = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < kWavesPerTile; ++w) c += s_cnt[ti][w];
+        a.tile_count[tile] = c;
+    }
+    if (t == 0) {
+        uint32_t sp = 0, dv = 0;
+#pragma unroll
+        for (uint32_t x = 0; x < kFrontTiles; ++x)
+#pragma unroll
+            for (uint32_t w = 0; w < kWavesPerTile; ++w) {
+                sp += s_cnt[x][w];
+                dv += s_drv[x][w];
+            }
+        const uint32_t t0 = k * kBinTiles + part * kFrontTiles;
+        if (sp) atomicAdd(reinterpret_cast<unsigned long long*>(a.tile_spikes + t0), sp);
+        // each firing drive source is one synaptic event, counted at its roll
+        if (dv) atomicAdd(reinterpret_cast<unsigned long long*>(a.tile_events + t0), dv);
+    }
 }
 
 __device__ __forceinline__ float event_weight(const NetArgs& a, uint32_t eid) {
@@ -2583,6 +2748,23 @@ void launch_megastep(int model, const NetArgs* nets, const uint2* items, uint32_
     else if (model == 2) megastep_kernel<2><<<n, kApply, 0, s>>>(nets, items);
     else if (model == 3) megastep_kernel<3><<<n, kApply, 0, s>>>(nets, items);
     else megastep_kernel<4><<<n, kApply, 0, s>>>(nets, items);
+}
+template <bool kRec>
+static void launch_front_r(int model, const NetArgs* nets, const uint2* items, uint32_t n, hipStream_t s) {
+    if (model == 0) front_kernel<0, kRec><<<n, kApply, 0, s>>>(nets, items);
+    else if (model == 1) front_kernel<1, kRec><<<n, kApply, 0, s>>>(nets, items);
+    else if (model == 2) front_kernel<2, kRec><<<n, kApply, 0, s>>>(nets, items);
+    else if (model == 3) front_kernel<3, kRec><<<n, kApply, 0, s>>>(nets, items);
+    else front_kernel<4, kRec><<<n, kApply, 0, s>>>(nets, items);
+}
+void launch_front(int model, bool rec, const NetArgs* nets, const uint2* items, uint32_t n, hipStream_t s) {
+    if (n == 0) return;
+    if (rec) launch_front_r<true>(model, nets, items, n, s);
+    else launch_front_r<false>(model, nets, items, n, s);
+}
+void launch_bin_apply_rec_mode(const NetArgs* nets, const uint2* refs, uint32_t n, uint32_t mode, hipStream_t s) {
+    if (n == 0) return;
+    bin_apply_kernel<true><<<n, kApply, 0, s>>>(nets, refs, mode);
 }
 void launch_fixup(const NetArgs* nets, const uint2* refs, uint32_t n, hipStream_t s) {
     if (n == 0) return;

@@ -62,6 +62,7 @@ constexpr uint32_t kBinCap = 2048;      // records bin_apply orders in one LDS p
 constexpr uint32_t kMaxWriters = 2048;  // binned writers per network
 constexpr uint32_t kBinBlockTiles = 4;  // source tiles per bin_kernel writer
 constexpr uint32_t kLtpItem = 0xFFFFFFFEu;  // bin_kernel item that runs in-edge LTP instead of writing
+constexpr uint32_t kFrontParts = 2;  // split form: front_kernel blocks per bin, kBinPosts / kApply cells each
 // 8-byte records: the event's index within its writer's events of the step
 // takes the 21 bits above the cell; a writer spanning more edges than that
 // makes its network use 16-byte records.  Writer index (< kMaxWriters = 2^11)
@@ -131,7 +132,10 @@ struct NetArgs {
     uint32_t n_bins;
     uint32_t n_bblk;            // writers
     int32_t bin_fused;          // the neuron kernel's blocks are the neuron-tile writers
-    int32_t mega;               // megastep network: one kernel per step
+    int32_t mega;               // parity buffers; the last of n_done blocks moves the step on
+    int32_t split;              // split form (front_kernel + bin_kernel): writers write every header
+                                // entry of their column, consumers leave the rows as read
+    uint32_t n_done;            // blocks of a step counted by mega_done
     uint32_t bin_rows;          // row length of bin_hdr (even, >= n_bblk)
     uint32_t n_pops;
     uint64_t n_ev;              // synapses + source edges

@@ -403,7 +403,7 @@ int NetState::load_state(const char* path, hipStream_t stream) {
     clear_heads(stream);
     if (tile_coll) HIP_TRY(hipMemsetAsync(tile_coll, 0, sizeof(uint32_t) * (n_tiles + n_src_tiles), stream));
     if (bin_hdr)
-        HIP_TRY(hipMemsetAsync(bin_hdr, 0, sizeof(uint2) * (mega_ok ? 2 : 1) * n_bins * bin_rows, stream));
+        HIP_TRY(hipMemsetAsync(bin_hdr, 0, sizeof(uint2) * nbuf() * n_bins * bin_rows, stream));
     if (bin_done) HIP_TRY(hipMemsetAsync(bin_done, 0, sizeof(uint32_t), stream));
     if (bin_mat) HIP_TRY(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(bin_mat), 1, 1, stream));
     host_step = st[0];

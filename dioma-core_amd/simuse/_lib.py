@@ -33,6 +33,7 @@ KURA_RK4 = 1
 FORM_ONE_KERNEL = 0
 FORM_BINNED = 1
 FORM_CHAINED = 2
+FORM_SPLIT = 3
 
 SCATTER_AUTO = 0
 SCATTER_TILE = 1

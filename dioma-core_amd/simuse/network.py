@@ -361,7 +361,7 @@ class CompiledNetwork:
         return int(b.value)
 
     def step_form(self) -> int:
-        """FORM_ONE_KERNEL, FORM_BINNED or FORM_CHAINED (finalizes the network)."""
+        """FORM_ONE_KERNEL, FORM_SPLIT, FORM_BINNED or FORM_CHAINED (finalizes the network)."""
         f = C.c_int32()
         check(lib().simuse_net_step_form(self._h, C.byref(f)))
         return int(f.value)

@@ -324,6 +324,7 @@ int simuse_net_device_bytes(simuse_net* net, uint64_t* bytes);
 #define SIMUSE_FORM_ONE_KERNEL 0 /* delays <= 1, one model, no pair STDP or receptors: megastep_kernel */
 #define SIMUSE_FORM_BINNED 1     /* delays <= 1: neuron kernel / binned writers, then the per-bin apply */
 #define SIMUSE_FORM_CHAINED 2    /* some delay > 1: chained scatter and its fix-up */
+#define SIMUSE_FORM_SPLIT 3      /* delays <= 1, one model, pair STDP or receptors: front kernel, then writers */
 int simuse_net_step_form(simuse_net* net, int32_t* form);
 /* Mean duration (ms) of each phase of the step over the last
  * simuse_net_step_async window, measured with HIP events on the handle's
