@@ -672,6 +672,7 @@ def bench_precision(args) -> dict:
 
 def step_kernels(form: int) -> str:
     """The launches of one step of a plastic receptor network in `form`."""
+    import simuse
     if form == simuse.FORM_SPLIT:
         return ("one step: front_kernel<HH, receptors> (the bin's conductance jumps and currents in event order, "
                 "update, receptor currents, STDP traces, compaction), bin_kernel (STDP, event records, in-edge LTP)")

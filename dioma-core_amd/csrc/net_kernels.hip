@@ -663,13 +663,49 @@ __device__ __forceinline__ float stdp_apply(const NetArgs& a, uint32_t e, float 
     return f;
 }
 
-// The STDP of out-edge e (post `post`) of a pre that spiked this step, then
-// its weight: LTP first when the post spiked too, then LTD (the order of the
-// reference's two passes, P/src/api/learn/brain.rs:204-286).
-__device__ __forceinline__ float stdp_out_edge(const NetArgs& a, uint32_t e, uint32_t post) {
-    if (a.plastic && !a.plastic[e]) return a.w[e];
-    if (spiked_now(a, post)) stdp_apply(a, e, a.a_plus * a.tr_pre[a.edge_pre[e]], true);
-    return stdp_apply(a, e, a.a_minus * a.tr_post[post], false);
+// The STDP of K out-edges e[u] (distinct edges of rows whose pre spiked this
+// step; u with on[u]), then their weights: LTP first when the post spiked
+// too, then LTD (the order of the reference's two passes,
+// P/src/api/learn/brain.rs:204-286), each an FP64 master update clipped to
+// [w_min, w_max] with the FP32 mirror stored for the scatter; non-plastic
+// edges keep their weight.  Every load of a level is issued before any is
+// used -- the edge's flag and master, then the post's spike bit and trace --
+// and the stores come last, so the K chains overlap instead of queueing
+// behind each other's stores; the two updates of an edge are applied in
+// registers (the arithmetic of two stdp_apply calls) and stored once.
+template <uint32_t K>
+__device__ __forceinline__ void stdp_out_edges(const NetArgs& a, const uint32_t (&e)[K], const uint32_t (&post)[K],
+                                               const bool (&on)[K], float (&wout)[K]) {
+    double m[K];
+    bool pl[K], sp[K];
+    float trq[K];
+#pragma unroll
+    for (uint32_t u = 0; u < K; ++u) {
+        pl[u] = on[u] && (!a.plastic || a.plastic[e[u]]);
+        m[u] = pl[u] ? a.w64[e[u]] : 0.0;
+        if (on[u] && !pl[u]) wout[u] = a.w[e[u]];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < K; ++u) {
+        sp[u] = pl[u] && spiked_now(a, post[u]);
+        trq[u] = pl[u] ? a.tr_post[post[u]] : 0.0f;
+    }
+    const double lo = static_cast<double>(a.w_min), hi = static_cast<double>(a.w_max);
+#pragma unroll
+    for (uint32_t u = 0; u < K; ++u) {
+        if (!pl[u]) continue;
+        double nw = m[u];
+        if (sp[u]) nw = fmin(fmax(nw + static_cast<double>(a.a_plus * a.tr_pre[a.edge_pre[e[u]]]), lo), hi);
+        nw = fmin(fmax(nw - static_cast<double>(a.a_minus * trq[u]), lo), hi);
+        m[u] = nw;
+        wout[u] = static_cast<float>(nw);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < K; ++u) {
+        if (!pl[u]) continue;
+        a.w64[e[u]] = m[u];
+        const_cast<dptr<float>>(a.w)[e[u]] = wout[u];
+    }
 }
 
 // LTP on the in-edges of the spiking neurons of neuron tiles [t0, t1) whose
@@ -707,10 +743,19 @@ __device__ void stdp_in_edges(dptr<const NetArgs> nap, uint32_t t0, uint32_t t1,
                 if (s_pref[mid] <= f) lo = mid; else hi = mid;
             }
             const uint32_t e = a.in_edge[s_rs[lo] + (f - s_pref[lo])];
-            if (a.plastic && !a.plastic[e]) continue;
+            // the edge's flag, pre and master together, then the pre's spike
+            // bit and trace together (stdp_apply's arithmetic, one store)
+            const bool pl = !a.plastic || a.plastic[e];
             const uint32_t pre = a.edge_pre[e];
-            if (spiked_now(a, pre)) continue;
-            stdp_apply(a, e, ap * a.tr_pre[pre], true);
+            const double m = a.w64[e];
+            if (!pl) continue;
+            const bool pre_spiked = spiked_now(a, pre);
+            const float trp = a.tr_pre[pre];
+            if (pre_spiked) continue;
+            double nw = m + static_cast<double>(ap * trp);
+            nw = fmin(fmax(nw, static_cast<double>(a.w_min)), static_cast<double>(a.w_max));
+            a.w64[e] = nw;
+            const_cast<dptr<float>>(a.w)[e] = static_cast<float>(nw);
         }
     }
 }
@@ -758,11 +803,21 @@ __device__ __forceinline__ void bin_events(const NetArgs& a, const BinMap& M, bo
                 e[u] = s_rs[k] + (f - s_pref[k]);
             }
         }
+        if (kWrite && kStdp && !is_src) {
+            bool on[kU];
 #pragma unroll
-        for (uint32_t u = 0; u < kU; ++u) {
-            if (f0 + u * kThreads < total) {
-                post[u] = col[e[u]];
-                if (kWrite) wt[u] = kStdp && !is_src ? stdp_out_edge(a, e[u], post[u]) : w[e[u]];
+            for (uint32_t u = 0; u < kU; ++u) {
+                on[u] = f0 + u * kThreads < total;
+                if (on[u]) post[u] = col[e[u]];
+            }
+            stdp_out_edges<kU>(a, e, post, on, wt);
+        } else {
+#pragma unroll
+            for (uint32_t u = 0; u < kU; ++u) {
+                if (f0 + u * kThreads < total) {
+                    post[u] = col[e[u]];
+                    if (kWrite) wt[u] = w[e[u]];
+                }
             }
         }
 #pragma unroll
@@ -825,11 +880,21 @@ __device__ __forceinline__ void ev_cache_load(const NetArgs& a, const BinMap& M,
             }
         }
     }
+    if (kStdp && !is_src) {
+        bool on[kEvCache];
 #pragma unroll
-    for (uint32_t u = 0; u < kEvCache; ++u) {
-        if (threadIdx.x + u * kThreads < total) {
-            post[u] = col[c.e[u]];
-            c.w[u] = kStdp && !is_src ? stdp_out_edge(a, c.e[u], post[u]) : w[c.e[u]];
+        for (uint32_t u = 0; u < kEvCache; ++u) {
+            on[u] = threadIdx.x + u * kThreads < total;
+            if (on[u]) post[u] = col[c.e[u]];
+        }
+        stdp_out_edges<kEvCache>(a, c.e, post, on, c.w);
+    } else {
+#pragma unroll
+        for (uint32_t u = 0; u < kEvCache; ++u) {
+            if (threadIdx.x + u * kThreads < total) {
+                post[u] = col[c.e[u]];
+                c.w[u] = w[c.e[u]];
+            }
         }
     }
 #pragma unroll
