@@ -627,8 +627,8 @@ def bench_precision(args) -> dict:
             f"{dv.mean():.2e} max {dv.max():.2e} mV, >1 mV {np.mean(dv > 1.0):.2%}, |dg| max {dg.max():.2e}, "
             f"w rms {rows[-1]['w_rms_err']:.2e}, rate {rate_dev:.2f} vs {rate_ora:.2f} Hz")
     dev.close()
-    big = hh_network(n=args.prec_full_neurons, seed=3, steps=args.prec_full_steps + 200, noise_sd=30.0,
-                     receptors=True)
+    big = hh_network(n=args.prec_full_neurons, seed=3, steps=max(args.prec_horizon, args.prec_full_steps + 208),
+                     noise_sd=30.0, receptors=True)
     net = build_device(big, graph_steps=16)
     form = net.step_form()
     net.run(208)
@@ -639,6 +639,29 @@ def bench_precision(args) -> dict:
     net.sync()
     us = (time.perf_counter() - t0) * 1e6 / args.prec_full_steps
     s1, e1 = net.counters()
+    # the rest of the stated 50,000-step horizon on the device (its FP64
+    # comparison over the whole horizon runs on a 10k-neuron network of the
+    # same construction in the configuration tests)
+    horizon = {}
+    done = 208 + args.prec_full_steps
+    if args.prec_horizon > done:
+        plastic = big.stdp["plastic"].astype(bool)
+        w0 = float(big.w.astype(np.float64)[plastic].mean())
+        t1 = time.perf_counter()
+        net.run(args.prec_horizon - done)
+        net.sync()
+        secs = time.perf_counter() - t1
+        s2, e2 = net.counters()
+        w = net.download_weights().astype(np.float64)[plastic]
+        horizon = {"steps": args.prec_horizon, "simulated_ms": args.prec_horizon * big.dt,
+                   "device_s_for_remaining_steps": secs, "remaining_steps": args.prec_horizon - done,
+                   "us_per_step": secs * 1e6 / (args.prec_horizon - done),
+                   "spikes": s2, "events": e2,
+                   "rate_hz": s2 / big.n_total / (args.prec_horizon * big.dt * 1e-3),
+                   "mean_plastic_weight": float(w.mean()), "mean_plastic_weight_at_start": w0,
+                   "plastic_weights_at_bounds": float(np.mean((w <= 0.0) | (w >= 1.0)))}
+        log(f"  full horizon {args.prec_horizon} steps: {secs:.2f} s for the last {args.prec_horizon - done}, "
+            f"rate {horizon['rate_hz']:.2f} Hz, mean plastic weight {w0:.5f} -> {w.mean():.5f}")
     net.close()
     q_ms = ms / args.prec_full_steps  # device time per step: the region's events over many graph replays
     sp, ev = (s1 - s0) / args.prec_full_steps, (e1 - e0) / args.prec_full_steps
@@ -656,7 +679,7 @@ def bench_precision(args) -> dict:
             "full_size": {"neurons": args.prec_full_neurons, "synapses": big.n_syn, "step_form": form_name(form),
                           "steps": args.prec_full_steps, "us_per_step": us,
                           "region_us_per_step": ms * 1e3 / args.prec_full_steps,
-                          "spikes_per_step": sp, "events_per_step": ev,
+                          "spikes_per_step": sp, "events_per_step": ev, "full_horizon": horizon,
                           "roofline": {"bound": "hbm", "kernel": step_kernels(form),
                                        "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -882,6 +905,8 @@ def main() -> int:
     ap.add_argument("--prec-windows", type=int, default=5)
     ap.add_argument("--prec-full-neurons", type=int, default=200_000)
     ap.add_argument("--prec-full-steps", type=int, default=2000)
+    ap.add_argument("--prec-horizon", type=int, default=50_000,
+                    help="steps of the full-size precision network on the device (its stated horizon)")
     args = ap.parse_args()
     if args.profile_steps < 1:
         ap.error("--profile-steps must be at least 1")

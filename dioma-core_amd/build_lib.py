@@ -15,7 +15,8 @@ REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsimuse.so")
 
-SOURCES = ["net_kernels.hip", "net_api.hip", "snapshot.hip", "kuramoto.hip", "batch.hip", "server.hip", "pool.hip"]
+SOURCES = ["net_kernels.hip", "net_api.hip", "snapshot.hip", "kuramoto.hip", "batch.hip", "server.hip", "pool.hip",
+           "model.hip"]
 
 # -ffp-contract=off: every float multiply and add rounds on its own, which is
 # what makes the LIF/Izhikevich/ring arithmetic bit-identical to the

@@ -2687,6 +2687,53 @@ __global__ __launch_bounds__(kTile) void coupling_kernel(const NetArgs* __restri
     *cell = I;
 }
 
+// Single population (the CompiledModel path, PC/src/engine.rs:98-423): one
+// thread per neuron advances it with the device-held input current i_in
+// (the same every step until the host uploads another) and the call's dt,
+// with the network's update (Neuron<MODEL>, including the +-100 input clamp
+// of LIF and Izhikevich, P/src/core/exec/population.rs:69-88), and writes
+// the 0/1 spike flag and, by wave ballot, the step's spike bit word.
+template <int MODEL>
+__global__ __launch_bounds__(kTile) void pop_step_kernel(PopDev P, const float* __restrict__ i_in,
+                                                         int32_t* __restrict__ spikes, uint64_t* __restrict__ bits,
+                                                         float dt) {
+    const uint32_t li = blockIdx.x * kTile + threadIdx.x;
+    int spike = 0;
+    if (li < P.n) {
+        Neuron<MODEL> nrn;
+        nrn.load(P, li);
+        spike = nrn.step(P, li, i_in[li], dt);
+        nrn.store(P, li);
+        spikes[li] = spike;
+    }
+    const uint64_t m = __ballot(spike);
+    if ((threadIdx.x & 63u) == 0u && li < P.n) bits[li >> 6] = m;
+}
+
+// The last step's spiking neurons in ascending order from its bit words
+// (one block): thread t counts words [t * per, (t + 1) * per), a block scan
+// gives each thread its first output slot.  The reference's compaction is an
+// atomic counter (PC/src/engine.rs:40-52), so its order is unspecified; the
+// set is the same.
+__global__ __launch_bounds__(1024) void pop_compact_kernel(const uint64_t* __restrict__ bits, uint32_t n_words,
+                                                           uint32_t* __restrict__ ids, uint32_t* __restrict__ count) {
+    __shared__ uint32_t s_wave[1024 / 64];
+    const uint32_t t = threadIdx.x, per = (n_words + 1023u) / 1024u;
+    const uint32_t w0 = min(t * per, n_words), w1 = min(w0 + per, n_words);
+    uint32_t c = 0;
+    for (uint32_t w = w0; w < w1; ++w) c += static_cast<uint32_t>(__popcll(bits[w]));
+    uint32_t excl;
+    const uint32_t total = block_scan<1024>(c, s_wave, excl);
+    for (uint32_t w = w0; w < w1; ++w) {
+        uint64_t m = bits[w];
+        while (m) {
+            ids[excl++] = w * 64u + static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(m)) - 1);
+            m &= m - 1ull;
+        }
+    }
+    if (t == 0) *count = total;
+}
+
 // Holds the stream for `ticks` of the constant-rate wall clock: work queued
 // behind it starts without waiting for the host's submission.
 __global__ void spacer_kernel(uint64_t ticks) {
@@ -2848,6 +2895,19 @@ void launch_inject(float* ring, const uint64_t* step, uint32_t N, const uint32_t
     else inject_kernel<<<(count + 255) / 256, 256, 0, s>>>(ring, step, N, idx, val, count, 0);
 }
 void launch_spacer(uint64_t ticks, hipStream_t s) { spacer_kernel<<<1, 64, 0, s>>>(ticks); }
+void launch_pop_step(int model, const PopDev& P, const float* i_in, int32_t* spikes, uint64_t* bits, float dt,
+                     hipStream_t s) {
+    const uint32_t g = (P.n + kTile - 1) / kTile;
+    if (g == 0) return;
+    if (model == 0) pop_step_kernel<0><<<g, kTile, 0, s>>>(P, i_in, spikes, bits, dt);
+    else if (model == 1) pop_step_kernel<1><<<g, kTile, 0, s>>>(P, i_in, spikes, bits, dt);
+    else if (model == 2) pop_step_kernel<2><<<g, kTile, 0, s>>>(P, i_in, spikes, bits, dt);
+    else if (model == 3) pop_step_kernel<3><<<g, kTile, 0, s>>>(P, i_in, spikes, bits, dt);
+    else pop_step_kernel<4><<<g, kTile, 0, s>>>(P, i_in, spikes, bits, dt);
+}
+void launch_pop_compact(const uint64_t* bits, uint32_t n_words, uint32_t* ids, uint32_t* count, hipStream_t s) {
+    pop_compact_kernel<<<1, 1024, 0, s>>>(bits, n_words, ids, count);
+}
 void launch_coupling(const NetArgs* nets, const uint2* items, uint32_t n, hipStream_t s) {
     if (n) coupling_kernel<<<n, kTile, 0, s>>>(nets, items);
 }

@@ -345,6 +345,51 @@ int simuse_net_time_steps(simuse_net* net, uint32_t n, float* ms);
 int simuse_net_time_steps_queued(simuse_net* net, uint32_t n, float* ms);
 
 /* ---------------------------------------------------------------------------
+ * Single population without synapses: pillars_cuda::CompiledModel
+ * (PC/src/engine.rs:98-423).  States, params and the input current i_in
+ * stay in device memory; every step advances each neuron with the same
+ * i_in (until the host uploads another) and the call's dt, with the network
+ * engine's update for the model (LIF and Izhikevich clamp the input to
+ * +-100 as the CPU population does, P/src/core/exec/population.rs:69-88).
+ * The model is the enum of simuse_pop_desc instead of an NVRTC source
+ * string.  Host arrays use the column-major SoA with column stride `stride`.
+ * ------------------------------------------------------------------------- */
+typedef struct simuse_pop simuse_pop;
+/* CompiledModel::new (engine.rs:98-160): ns and np must be the model's state
+ * and parameter counts, stride >= n.  Everything starts at zero. */
+int simuse_pop_create(int32_t model, uint32_t ns, uint32_t np, uint32_t n, uint32_t stride, simuse_pop** out);
+/* Drop (engine.rs:426-455): frees everything; NULL is accepted. */
+void simuse_pop_destroy(simuse_pop* pop);
+/* upload (engine.rs:191-220): states [ns*stride], params [np*stride], i_in [n]. */
+int simuse_pop_upload(simuse_pop* pop, const float* states, const float* params, const float* i_in);
+/* upload_i_in (engine.rs:222-236). */
+int simuse_pop_upload_i_in(simuse_pop* pop, const float* i_in);
+/* step_graph (engine.rs:253-286): n_steps steps queued, asynchronous. */
+int simuse_pop_step_async(simuse_pop* pop, float dt, uint32_t n_steps);
+/* sync (engine.rs:288-296). */
+int simuse_pop_sync(simuse_pop* pop);
+/* step_device (engine.rs:238-251): one step, then sync. */
+int simuse_pop_step_device(simuse_pop* pop, float dt);
+/* compact_spikes (engine.rs:298-352): the last step's spiking neurons listed
+ * on the device in ascending order (the reference's atomic list has the same
+ * set in an unspecified order); *count receives how many. */
+int simuse_pop_compact_spikes(simuse_pop* pop, uint32_t* count);
+/* download_compact (engine.rs:354-373): the first `count` listed indices
+ * (count at most that of the last compaction). */
+int simuse_pop_download_compact(simuse_pop* pop, int32_t* indices, uint32_t count);
+/* download_spikes (engine.rs:375-390): the last step's 0/1 flags, len = n. */
+int simuse_pop_download_spikes(simuse_pop* pop, int32_t* spikes, uint64_t len);
+/* download_states (engine.rs:392-407): len = ns*stride (columns padded with 0). */
+int simuse_pop_download_states(simuse_pop* pop, float* states, uint64_t len);
+/* step (engine.rs:163-189): upload, one step, states and spikes back. */
+int simuse_pop_step(simuse_pop* pop, float* states, const float* params, const float* i_in, int32_t* spikes,
+                    float dt);
+/* run (engine.rs:409-423): n_steps steps with the same params and input;
+ * states and the last step's spikes back (n_steps = 0 touches nothing). */
+int simuse_pop_run(simuse_pop* pop, float* states, const float* params, const float* i_in, int32_t* spikes,
+                   float dt, uint32_t n_steps);
+
+/* ---------------------------------------------------------------------------
  * Batches: many independent networks stepped by one launch sequence.
  * No reference counterpart (the reference steps one CompiledNetwork per call);
  * each network keeps the simuse_net step semantics, its own ring, step

@@ -1141,7 +1141,14 @@ struct Net final : NetBase {
     }
     uint64_t total_spikes() const override { return n_spikes; }
     uint64_t total_events() const override { return n_events; }
-    void record(bool on) override { rec = on; }
+    // Turning recording on starts a fresh raster (the previous one is dropped).
+    void record(bool on) override {
+        rec = on;
+        if (on) {
+            rast.clear();
+            rast_steps = 0;
+        }
+    }
     uint64_t raster_steps() const override { return rast_steps; }
     void raster(int32_t* out) const override { std::memcpy(out, rast.data(), 4 * rast.size()); }
 };

@@ -117,3 +117,21 @@ def test_null_handle_is_an_error_not_a_crash():
     L = _lib.lib()
     assert L.simuse_net_step(None) == _lib.E_INVALID
     L.simuse_net_destroy(None)
+
+
+def test_population_api_rejects_bad_arguments_on_the_host():
+    # simuse_pop_create validates before touching the device
+    from simuse.model import _L
+    L = _L()
+    h = C.c_void_p()
+    assert L.simuse_pop_create(7, 2, 6, 10, 16, C.byref(h)) == _lib.E_INVALID  # unknown model
+    assert L.simuse_pop_create(simuse.MODEL_LIF, 3, 6, 10, 16, C.byref(h)) == _lib.E_INVALID  # ns
+    assert L.simuse_pop_create(simuse.MODEL_LIF, 2, 7, 10, 16, C.byref(h)) == _lib.E_INVALID  # np
+    assert L.simuse_pop_create(simuse.MODEL_LIF, 2, 6, 10, 8, C.byref(h)) == _lib.E_INVALID  # stride < n
+    assert L.simuse_pop_create(simuse.MODEL_LIF, 2, 6, 0, 8, C.byref(h)) == _lib.E_INVALID  # empty
+    assert L.simuse_pop_create(simuse.MODEL_LIF, 2, 6, 10, 16, None) == _lib.E_INVALID
+    assert not h.value
+    assert L.simuse_pop_upload(None, None, None, None) == _lib.E_INVALID
+    assert L.simuse_pop_step_async(None, 0.1, 1) == _lib.E_INVALID
+    assert L.simuse_pop_compact_spikes(None, None) == _lib.E_INVALID
+    L.simuse_pop_destroy(None)

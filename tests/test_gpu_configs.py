@@ -5,6 +5,10 @@
 * Hodgkin-Huxley: 200k neurons, Pareto out-degree (~40M synapses), lognormal
   conductance synapses and pair STDP, bit for bit against the oracle's FP32
   twin, and the FP32-vs-FP64 tolerance from a common start.
+* Hodgkin-Huxley over the stated 50,000-step horizon (a 10k-neuron network of
+  the same construction, so that the oracle's FP64 twin finishes in seconds):
+  firing rate, ISI CV, synchrony and the mean plastic weight within 1% of
+  FP64.
 * Batching: the full 512-network mix; a sample of networks equal alone and in
   the mix, generated networks equal to the oracle.
 * Serving: the 64-request Poisson stream at 200 requests/s under an HBM budget
@@ -99,6 +103,99 @@ def test_hh_200k_conductances_stdp_stated_size():
     dg = np.abs(dev.download_conductances().astype(np.float64) - o64.conductances())
     print(f"  FP64 window: |dv| mean {dv.mean():.2e} max {dv.max():.2e} mV, |dg| max {dg.max():.2e}", flush=True)
     assert dv.mean() < 2e-4 and dv.max() < 5e-3 and dg.max() < 1e-3
+    o64.close()
+    dev.close()
+
+
+class _SpikeStats:
+    """Rate, ISI CV (P/src/kernels/stats/rates.rs:6-35, isi.rs:30-67: n-1
+    variance, at least two intervals) and the synchrony chi of a raster fed
+    in consecutive chunks, without holding the whole raster."""
+
+    def __init__(self, n):
+        self.n = n
+        self.count = np.zeros(n, np.int64)
+        self.last = np.full(n, -1, np.int64)
+        self.s1 = np.zeros(n)
+        self.s2 = np.zeros(n)
+        self.k = np.zeros(n, np.int64)
+        self.pop = []
+        self.steps = 0
+
+    def add(self, r):
+        r = np.asarray(r)[:, : self.n]
+        t0 = self.steps
+        self.steps += r.shape[0]
+        self.pop.append(r.sum(1, dtype=np.int64))
+        self.count += r.sum(0, dtype=np.int64)
+        t, i = np.nonzero(r)
+        order = np.lexsort((t, i))  # by neuron, then time
+        t, i = t[order].astype(np.int64) + t0, i[order]
+        if t.size == 0:
+            return
+        first = np.ones(t.size, bool)
+        first[1:] = i[1:] != i[:-1]
+        prev = np.empty_like(t)
+        prev[1:] = t[:-1]
+        prev[first] = self.last[i[first]]
+        ok = prev >= 0
+        isi = (t - prev)[ok].astype(np.float64)
+        np.add.at(self.s1, i[ok], isi)
+        np.add.at(self.s2, i[ok], isi * isi)
+        np.add.at(self.k, i[ok], 1)
+        lastm = np.ones(t.size, bool)
+        lastm[:-1] = i[:-1] != i[1:]
+        self.last[i[lastm]] = t[lastm]
+
+    def result(self, dt_s):
+        T = self.steps
+        rate = float(self.count.mean() / (T * dt_s))
+        m = self.k >= 2
+        mean = self.s1[m] / self.k[m]
+        var = (self.s2[m] - self.k[m] * mean * mean) / (self.k[m] - 1)
+        cv = float(np.mean(np.sqrt(np.maximum(var, 0.0)) / mean))
+        p = self.count / T
+        pop = np.concatenate(self.pop) / self.n
+        chi = float(np.sqrt(pop.var() / max(np.mean(p * (1 - p)), 1e-30)))
+        return rate, cv, chi
+
+
+def test_hh_conductances_stdp_full_horizon_statistics():
+    # The C3 construction (Pareto out-degree, lognormal AMPA/GABA_A
+    # conductances, pair STDP with FP64 weight masters) over its whole
+    # 50,000-step horizon: the FP32 device against the oracle's FP64 twin
+    # from the same start.  The trajectories decorrelate within a few
+    # thousand steps (spike-time flips), so the criterion is statistical.
+    print("config hh full horizon statistics", flush=True)
+    T, chunk, n = 50_000, 1000, 10_000
+    spec = hh_network(n=n, seed=3, steps=T, noise_sd=30.0, receptors=True)
+    dev = build_device(spec, graph_steps=16)
+    o64 = po.OracleNetwork(spec, 64, threads=THREADS)
+    sd, so = _SpikeStats(n), _SpikeStats(n)
+    t0 = time.perf_counter()
+    for c in range(T // chunk):
+        dev.record(chunk)
+        dev.run(chunk)
+        sd.add(dev.download_raster())
+        o64.record(True)
+        o64.step(chunk)
+        so.add(o64.raster().reshape(chunk, -1))
+        if c % 10 == 9:
+            print(f"  {c + 1} chunks, {time.perf_counter() - t0:.1f} s", flush=True)
+    dt_s = spec.dt * 1e-3
+    (rd, cd, xd), (ro, co, xo) = sd.result(dt_s), so.result(dt_s)
+    plastic = spec.stdp["plastic"].astype(bool)
+    wd = float(dev.download_weights().astype(np.float64)[plastic].mean())
+    wo = float(o64.weights()[plastic].mean())
+    w0 = float(spec.w.astype(np.float64)[plastic].mean())
+    print(f"  rate {rd:.4f}/{ro:.4f} Hz, cv {cd:.4f}/{co:.4f}, chi {xd:.5f}/{xo:.5f}, "
+          f"mean plastic weight {wd:.6f}/{wo:.6f} (start {w0:.6f})", flush=True)
+    assert ro > 1.0 and sd.steps == T
+    assert abs(rd - ro) <= 0.01 * ro
+    assert abs(cd - co) <= 0.01 * co
+    assert abs(xd - xo) <= 0.01 * xo
+    assert abs(wo - w0) > 1e-4  # STDP moved the weights
+    assert abs(wd - wo) <= 0.01 * abs(wo)
     o64.close()
     dev.close()
 
