@@ -405,6 +405,18 @@ int simuse_batch_set_horizon(simuse_batch* b, int32_t id, uint64_t steps) {
     return SIMUSE_OK;
 }
 
+int simuse_batch_set_one_kernel(simuse_batch* b, int32_t on) {
+    if (!b) return fail(SIMUSE_E_INVALID, "null batch handle");
+    if (on != 0 && on != 1) return fail(SIMUSE_E_INVALID, "one-kernel switch must be 0 or 1");
+    return b->eng.set_one_kernel(on != 0);
+}
+
+int simuse_batch_get_one_kernel(simuse_batch* b, int32_t* on) {
+    if (!b || !on) return fail(SIMUSE_E_INVALID, "null argument");
+    *on = b->eng.one_kernel ? 1 : 0;
+    return SIMUSE_OK;
+}
+
 int simuse_batch_set_graph_steps(simuse_batch* b, uint32_t steps) {
     if (!b) return fail(SIMUSE_E_INVALID, "null batch handle");
     return b->eng.set_graph_steps(steps);

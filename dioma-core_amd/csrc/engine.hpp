@@ -413,6 +413,7 @@ struct Engine {
     uint32_t n_cpl = 0, cap_cpl = 0;
     bool direct_fix = false;
     // execution
+    bool one_kernel = true;  // networks that can take the one-kernel step do (the autotuner may turn it off)
     int sched = SIMUSE_SCATTER_TILE;
     uint32_t group = 1;  // neuron tiles per block (1..kMaxGroup; the autotuner may change it)
     uint32_t graph_steps = 8;
@@ -422,6 +423,7 @@ struct Engine {
     bool profiling = false;
     std::vector<hipEvent_t> ev;
     uint32_t prof_steps = 0;
+    hipEvent_t tev[2] = {nullptr, nullptr};  // simuse_net_time_steps' pair, kept for the engine's life
 
     Engine() = default;
     Engine(const Engine&) = delete;
@@ -430,6 +432,8 @@ struct Engine {
         if (stream) (void)hipStreamSynchronize(stream);
         drop_graph();
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        for (hipEvent_t e : tev)
+            if (e) (void)hipEventDestroy(e);
         for (uint4* t : d_tiles)
             if (t) (void)hipFree(t);
         for (uint2* t : d_mega)
@@ -446,6 +450,11 @@ struct Engine {
     int init() {
         // a blocking stream is ordered after the synchronous null-stream uploads
         HIP_TRY(hipStreamCreate(&stream));
+        return SIMUSE_OK;
+    }
+    int timing_events() {
+        for (hipEvent_t& e : tev)
+            if (!e) HIP_TRY(hipEventCreate(&e));
         return SIMUSE_OK;
     }
     void drop_graph() {
@@ -517,7 +526,7 @@ struct Engine {
             // (receptor networks: bin_kernel writers, which tag each event with its receptor)
             const bool fz = s->binned && !s->stdp && !s->rec_code && ng + ns <= kMaxWriters;
             // megastep: bin b's block is writer b, the source blocks follow
-            s->mega = s->binned && s->mega_ok && !s->no_mega && !s->stdp;
+            s->mega = s->binned && s->mega_ok && !s->no_mega && !s->stdp && one_kernel;
             if (s->mega) {
                 s->mat_items.clear();
                 for (uint32_t b = 0; b < s->n_bins; ++b) {
@@ -866,6 +875,26 @@ struct Engine {
             group = g;
             mark_dirty();
         }
+        return SIMUSE_OK;
+    }
+
+    // Networks that can take the one-kernel step take it (on) or the binned
+    // form (off).  Either way a network's next input ends up in ring slot
+    // step % 32 with the materialised flag set: a one-kernel network's
+    // pending events are applied there, and a binned network's apply has
+    // already written them there.
+    int set_one_kernel(bool on) {
+        int rc = prepare();
+        if (rc) return rc;
+        if (one_kernel == on) return SIMUSE_OK;
+        for (NetState* s : nets) {
+            if ((rc = materialize(s))) return rc;
+            if (s->bin_mat && s->binned && s->mega_ok && !s->no_mega && !s->stdp && !s->mega)
+                HIP_TRY(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(s->bin_mat), 1, 1, stream));
+        }
+        HIP_TRY(hipStreamSynchronize(stream));
+        one_kernel = on;
+        mark_dirty();
         return SIMUSE_OK;
     }
 

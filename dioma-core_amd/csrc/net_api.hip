@@ -971,9 +971,16 @@ int Engine::autotune(uint32_t trial_steps) {
     int rc = prepare();
     if (rc) return rc;
     if (trial_steps == 0) trial_steps = 16;
-    // megastep networks: the pending events into the ring, which the snapshot keeps
-    for (NetState* s : nets)
+    // megastep networks: the pending events into the ring, which the snapshot
+    // keeps, with the flag that makes a one-kernel step read them there.  A
+    // network that could take the one-kernel step but runs the binned form
+    // has its next input in the ring already: the flag is set for it too, so
+    // that a one-kernel trial (and the form chosen at the end) reads it.
+    for (NetState* s : nets) {
         if ((rc = materialize(s))) return rc;
+        if (s->bin_mat && s->binned && s->mega_ok && !s->no_mega && !s->stdp && !s->mega)
+            HIP_TRY(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(s->bin_mat), 1, 1, stream));
+    }
     HIP_TRY(hipStreamSynchronize(stream));
     Snapshot snap;
     for (NetState* s : nets) {
@@ -1021,22 +1028,39 @@ int Engine::autotune(uint32_t trial_steps) {
         HIP_TRY(hipGetLastError());
         return SIMUSE_OK;
     };
-    // Candidates: scatter schedule x neuron tiles per block.
+    // Candidates: step form (networks that can take the one-kernel step may
+    // instead run the binned form: neuron-kernel writers, then the per-bin
+    // apply) x scatter schedule x neuron tiles per block.  The schedule and
+    // the block size only shape the binned and chained forms, so with every
+    // network in the one-kernel step a single trial stands for all six.
+    bool can_mega = false, all_mega = true;
+    for (NetState* s : nets) {
+        const bool m = s->binned && s->mega_ok && !s->no_mega && !s->stdp;
+        can_mega = can_mega || m;
+        all_mega = all_mega && m;
+    }
     const int cand_s[2] = {SIMUSE_SCATTER_TILE, SIMUSE_SCATTER_WAVE};
     const uint32_t cand_g[3] = {4, 2, 1};
     const int saved = sched;
     const uint32_t saved_g = group;
+    const bool saved_one = one_kernel;
     float best_ms = 0;
     bool have = false;
     int best = sched;
     uint32_t best_g = group;
+    bool best_one = one_kernel;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    for (int c = 0; c < 6 && rc == SIMUSE_OK; ++c) {
-        if (cand_g[c / 2] > kMaxGroup) continue;
+    for (int c = 0; c < 12 && rc == SIMUSE_OK; ++c) {
+        const bool one = c < 6;
+        if (!one && !can_mega) continue;         // no network can tell the forms apart
+        if (one && all_mega && c > 0) continue;  // the one-kernel step has neither knob
+        if (cand_g[(c % 6) / 2] > kMaxGroup) continue;
+        one_kernel = can_mega ? one : saved_one;
+        mark_dirty();
         sched = cand_s[c % 2];
-        if ((rc = set_group(cand_g[c / 2]))) break;
+        if ((rc = set_group(cand_g[(c % 6) / 2]))) break;
         if ((rc = ensure_tables())) break;
         if ((rc = restore())) break;
         if ((rc = enqueue_steps(2))) break;  // warm-up
@@ -1056,6 +1080,7 @@ int Engine::autotune(uint32_t trial_steps) {
             best_ms = ms;
             best = sched;
             best_g = group;
+            best_one = one_kernel;
         }
     }
     (void)hipEventDestroy(e0);
@@ -1063,10 +1088,16 @@ int Engine::autotune(uint32_t trial_steps) {
     const int rc2 = restore();
     sched = saved;
     if (rc) {
+        one_kernel = saved_one;
+        mark_dirty();
         (void)set_group(saved_g);
         return rc;
     }
     if (rc2) return rc2;
+    // the snapshot holds the pending events in the ring with the flag set,
+    // which both forms read as the next step's input
+    one_kernel = best_one;
+    mark_dirty();
     if ((rc = set_group(best_g))) return rc;
     return set_sched(best);
 }
@@ -1219,6 +1250,13 @@ int simuse_net_autotune(simuse_net* net, uint32_t trial_steps) {
     int rc = check_net(net);
     if (rc) return rc;
     return net->eng.autotune(trial_steps);
+}
+
+int simuse_net_set_one_kernel(simuse_net* net, int32_t on) {
+    int rc = check_net(net);
+    if (rc) return rc;
+    if (on != 0 && on != 1) return fail(SIMUSE_E_INVALID, "one-kernel switch must be 0 or 1");
+    return guarded([&]() { return net->eng.set_one_kernel(on != 0); });
 }
 
 int simuse_net_set_graph_steps(simuse_net* net, uint32_t steps) {
@@ -1651,12 +1689,9 @@ static int time_steps_impl(simuse_net* net, uint32_t n, float* ms, bool queued) 
         // the graph's capture and instantiation are host work: done before the
         // first event, so the region holds only replays (and the rest steps)
         if (n >= net->eng.graph_steps && !net->eng.gexec && (r = net->eng.build_graph())) return r;
-        hipEvent_t e[2] = {nullptr, nullptr};
-        HIP_TRY(hipEventCreate(&e[0]));
-        if (hipEventCreate(&e[1]) != hipSuccess) {
-            (void)hipEventDestroy(e[0]);
-            return fail(SIMUSE_E_HIP, "hipEventCreate failed");
-        }
+        // the pair of events is the engine's, created on the first call
+        if ((r = net->eng.timing_events())) return r;
+        hipEvent_t* e = net->eng.tev;
         hipError_t he = hipSuccess;
         if (queued) {
             // 1 ms of the constant-rate wall clock ahead of the first event
@@ -1673,8 +1708,6 @@ static int time_steps_impl(simuse_net* net, uint32_t n, float* ms, bool queued) 
         if (he == hipSuccess && r == SIMUSE_OK) he = hipEventRecord(e[1], net->eng.stream);
         if (he == hipSuccess && r == SIMUSE_OK) he = hipEventSynchronize(e[1]);
         if (he == hipSuccess && r == SIMUSE_OK) he = hipEventElapsedTime(ms, e[0], e[1]);
-        (void)hipEventDestroy(e[0]);
-        (void)hipEventDestroy(e[1]);
         if (r) return r;
         if (he != hipSuccess) return fail(SIMUSE_E_HIP, "time_steps: %s", hipGetErrorString(he));
         return SIMUSE_OK;

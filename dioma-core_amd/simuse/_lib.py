@@ -113,6 +113,7 @@ _SIGS = {
     "simuse_net_set_block_tiles": ([P, C.c_uint32], C.c_int),
     "simuse_net_get_block_tiles": ([P, u32p], C.c_int),
     "simuse_net_set_graph_steps": ([P, C.c_uint32], C.c_int),
+    "simuse_net_set_one_kernel": ([P, C.c_int32], C.c_int),
     "simuse_net_step": ([P], C.c_int),
     "simuse_net_step_async": ([P, C.c_uint32], C.c_int),
     "simuse_net_sync": ([P], C.c_int),

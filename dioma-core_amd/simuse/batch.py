@@ -41,6 +41,8 @@ _SIGS = {
                                        C.c_int),
     "simuse_batch_set_horizon": ([P, C.c_int32, C.c_uint64], C.c_int),
     "simuse_batch_set_graph_steps": ([P, C.c_uint32], C.c_int),
+    "simuse_batch_set_one_kernel": ([P, C.c_int32], C.c_int),
+    "simuse_batch_get_one_kernel": ([P, i32p], C.c_int),
     "simuse_batch_set_scatter_mode": ([P, C.c_int32], C.c_int),
     "simuse_batch_autotune": ([P, C.c_uint32, i32p], C.c_int),
     "simuse_batch_set_block_tiles": ([P, C.c_uint32], C.c_int),
@@ -180,6 +182,14 @@ class Batch:
 
     def set_block_tiles(self, tiles):
         check(_L().simuse_batch_set_block_tiles(self._h, int(tiles)))
+
+    def set_one_kernel(self, on):
+        check(_L().simuse_batch_set_one_kernel(self._h, 1 if on else 0))
+
+    def one_kernel(self) -> bool:
+        v = C.c_int32()
+        check(_L().simuse_batch_get_one_kernel(self._h, C.byref(v)))
+        return bool(v.value)
 
     def autotune(self, trial_steps=16) -> int:
         m = C.c_int32()

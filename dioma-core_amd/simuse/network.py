@@ -179,6 +179,10 @@ class CompiledNetwork:
     def set_graph_steps(self, steps: int) -> None:
         check(lib().simuse_net_set_graph_steps(self._h, int(steps)))
 
+    def set_one_kernel(self, on: bool) -> None:
+        """One-kernel step (True, the default) or the binned form for networks that can take either."""
+        check(lib().simuse_net_set_one_kernel(self._h, 1 if on else 0))
+
     # -- stepping ---------------------------------------------------------------
     def step(self) -> None:
         """One step, synchronous (CompiledNetwork::step, network.rs:411)."""

@@ -127,9 +127,19 @@ int simuse_net_get_scatter_mode(simuse_net* net, int32_t* mode);
  * No effect on results. */
 int simuse_net_set_block_tiles(simuse_net* net, uint32_t tiles);
 int simuse_net_get_block_tiles(simuse_net* net, uint32_t* tiles);
-/* Times each scatter schedule and block size on a snapshot of the live
- * state for `trial_steps` steps, restores the snapshot, keeps the fastest. */
+/* Times each candidate on a snapshot of the live state for `trial_steps`
+ * steps, restores the snapshot, keeps the fastest.  Candidates: the scatter
+ * schedule x the block size, and, when the network can take the one-kernel
+ * step, that step against the binned form (simuse_net_set_one_kernel).  No
+ * effect on results. */
 int simuse_net_autotune(simuse_net* net, uint32_t trial_steps);
+/* on = 1 (the default): a network of one model without pair-based
+ * plasticity, receptor synapses or delays above one step takes the
+ * one-kernel step (megastep_kernel); on = 0: it takes the binned form
+ * (neuron-kernel writers, then the per-bin apply) instead.  No effect on
+ * results; the autotuner sets it.  No reference counterpart (the reference
+ * has one GPU step form). */
+int simuse_net_set_one_kernel(simuse_net* net, int32_t on);
 /* Steps captured per hipGraph replay (1..64). */
 int simuse_net_set_graph_steps(simuse_net* net, uint32_t steps);
 
@@ -448,6 +458,10 @@ int simuse_batch_set_poisson_drive(simuse_batch* b, int32_t net_id, int32_t pop,
 /* The network stops advancing once its step counter reaches `steps`. */
 int simuse_batch_set_horizon(simuse_batch* b, int32_t net_id, uint64_t steps);
 int simuse_batch_set_graph_steps(simuse_batch* b, uint32_t steps);
+/* simuse_net_set_one_kernel for every network of the batch, and the current
+ * setting (the autotuner's choice after simuse_batch_autotune). */
+int simuse_batch_set_one_kernel(simuse_batch* b, int32_t on);
+int simuse_batch_get_one_kernel(simuse_batch* b, int32_t* on);
 int simuse_batch_set_scatter_mode(simuse_batch* b, int32_t mode);
 int simuse_batch_set_block_tiles(simuse_batch* b, uint32_t tiles);
 int simuse_batch_autotune(simuse_batch* b, uint32_t trial_steps, int32_t* mode);

@@ -487,6 +487,37 @@ def test_autotune_keeps_state():
     assert_same_ring(dev, ora)
 
 
+def test_step_form_switches_and_autotune_bitexact():
+    # A network that can take the one-kernel step, switched to the binned form
+    # and back mid-run (the pending events move through the ring), then
+    # autotuned with both forms among the candidates: spike ids every step,
+    # final state, ring and counters equal the uninterrupted oracle.
+    from simuse import FORM_BINNED, FORM_ONE_KERNEL
+    say("step form switches")
+    spec = random_net(20_000, 100, seed=9, model="izh", sources=300)
+    dev = build_device(spec)
+    ora = po.OracleNetwork(spec, 32)
+    forms = []
+    for k, on in enumerate([True, False, True, False, None]):
+        if on is None:
+            dev.autotune(8)
+        else:
+            dev.set_one_kernel(on)
+        forms.append(dev.step_form())
+        for t in range(30):
+            dev.step() if t % 2 else dev.run(1)
+            ora.step()
+            assert np.array_equal(dev.download_spike_ids(), ora.active()), (k, t)
+        if k == 1:  # a multi-step graph in the binned form too
+            dev.run(16)
+            ora.step(16)
+    assert forms[:4] == [FORM_ONE_KERNEL, FORM_BINNED, FORM_ONE_KERNEL, FORM_BINNED], forms
+    assert forms[4] in (FORM_ONE_KERNEL, FORM_BINNED)
+    assert dev.counters() == ora.counters() and ora.counters()[0] > 0
+    assert_same_state(dev, ora, spec)
+    assert_same_ring(dev, ora)
+
+
 def test_raster_statistics_match_oracle():
     say("raster_stats")
     spec = random_net(3000, 60, seed=4, model="izh")
