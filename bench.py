@@ -109,7 +109,7 @@ def form_name(form: int) -> str:
             simuse.FORM_BINNED: "binned_three_kernel", simuse.FORM_CHAINED: "chained"}[form]
 
 
-# -- FP64 checks (the oracle's FP64 twin is the reference of every error) -------
+# -- FP64 checks (the oracle's FP64 twin is the reference of every error) --------------
 def fp64_window(net, spec, steps, threads) -> dict:
     """Start the oracle's FP64 twin from the device's own snapshot (every FP32
     value is exact in FP64), advance both `steps` steps and compare membrane
@@ -156,7 +156,7 @@ def cpu_baseline(spec, steps: int, threads: int, warmup: int = 10):
     return spec.n_total * steps / dt, ev / dt, dt
 
 
-# -- C2: the headline ---------------------------------------------------------------
+# -- Izhikevich 1M / 100M: the headline ------------------------------------------------
 def bench_izhikevich(args) -> dict:
     import simuse
     from simuse.workloads import build_device, izhikevich_network
@@ -308,7 +308,7 @@ def bench_izhikevich(args) -> dict:
     return out
 
 
-# -- C1: Brunel LIF on the GPU and on the CPU ------------------------------------------
+# -- Brunel LIF 10k on the GPU and on the CPU ------------------------------------------
 def bench_brunel(args) -> dict:
     """The Brunel LIF configuration (10k neurons, p = 0.01, delays 1..15,
     2 kHz Poisson per neuron, dt 0.1 ms) over its full 10,000 steps on the
@@ -368,7 +368,7 @@ def bench_brunel(args) -> dict:
                     "threshold) the network is nearly silent: the step is launch- and latency-bound"}
 
 
-# -- C4: batching -------------------------------------------------------------------------
+# -- 512 uneven networks batched -------------------------------------------------------
 def bench_batching(args) -> dict:
     """Batching configuration: 512 generated networks (log-uniform 1k-200k
     neurons, LIF/Izhikevich, 100 synapses each, Poisson drive U(1,10) kHz)
@@ -470,7 +470,7 @@ def bench_batching(args) -> dict:
             "batched_vs_plain": batched["networks_per_s"] / plain["networks_per_s"], "fp64_error": fp64}
 
 
-# -- C5: serving ----------------------------------------------------------------------------
+# -- serving a Poisson request stream --------------------------------------------------
 def bench_serving(args, budget_gb: float, fp64: bool = True) -> dict:
     """Serving configuration: Poisson arrivals (200/s) of LIF/Izhikevich/HH
     networks (5k-500k neurons) and Kuramoto graphs (10k-1M oscillators, RK4),
@@ -576,7 +576,7 @@ def serving_fp64(reqs, res, args) -> dict:
     return {"reference": "oracle FP64 twin, same generated networks and inputs", "sample": rows}
 
 
-# -- C3: precision ------------------------------------------------------------------------
+# -- HH 200k precision network ---------------------------------------------------------
 def bench_precision(args) -> dict:
     """Precision configuration: the Hodgkin-Huxley network with heavy-tailed
     out-degree, lognormal conductance synapses (AMPA from excitatory, GABA_A
@@ -707,7 +707,7 @@ def step_kernels(form: int) -> str:
             "event records), bin_apply_kernel<receptors>")
 
 
-# -- Kuramoto: the phase-coupling kernel at 1M oscillators / 20M edges ---------------
+# -- Kuramoto: the phase-coupling kernel at 1M oscillators / 20M edges -----------------
 def kura_algorithmic_bytes(n: int, n_edges: int) -> dict:
     """Bytes an RK4 step must move at least once per stage: per edge its
     column (4 B), weight (4 B) and the neighbour's (sin, cos) (8 B); per node
@@ -761,7 +761,7 @@ def bench_kuramoto(args) -> dict:
                                              "steps (no graph)"}}
 
 
-# -- creature pool: many small alpha-LIF brains, per-post pull scatter ---------------
+# -- creature pool: many small alpha-LIF brains, per-post pull scatter -----------------
 def bench_pool(args) -> dict:
     """The game engine's creature pool (BrainPool): `pool_creatures` random
     brains of `pool_neurons` alpha-LIF neurons (connection probability

@@ -161,7 +161,7 @@ class _SpikeStats:
 
 
 def test_hh_conductances_stdp_full_horizon_statistics():
-    # The C3 construction (Pareto out-degree, lognormal AMPA/GABA_A
+    # The precision network's construction (Pareto out-degree, lognormal AMPA/GABA_A
     # conductances, pair STDP with FP64 weight masters) over its whole
     # 50,000-step horizon: the FP32 device against the oracle's FP64 twin
     # from the same start.  The trajectories decorrelate within a few
