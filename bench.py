@@ -762,6 +762,86 @@ def bench_kuramoto(args) -> dict:
 
 
 # -- creature pool: many small alpha-LIF brains, per-post pull scatter -----------------
+# -- every neuron model as a 32M-neuron population (the single-population path) ------
+# State columns each model's update reads and writes, and parameter columns it
+# reads, in the single-population layout (every parameter per neuron, dt from
+# the call; HH's CaT gate only where g_CaT != 0, here 0)
+MODEL_COLUMNS = {"lif": (2, 5), "izhikevich": (3, 6), "hh": (5, 12), "gpea": (9, 13), "alpha_lif": (3, 6)}
+
+
+def bench_models(args) -> dict:
+    """Each neuron model on its own through the single-population API (the
+    CompiledModel path): --model-neurons neurons with a resident random input
+    current, one launch of the per-neuron update per step.  Device time
+    (HIP events behind a spacer) against the bytes the update must move:
+    state columns read and written, parameter columns read, the input read
+    and the spike flag written; the oracle's FP32 path on the host cores
+    beside it."""
+    import pyoracle as po
+    from simuse.model import CompiledModel
+    from simuse.models import GPEA_PULSE, alpha_lif_pop, gpea_pop, hh_pop, izh_pop, lif_pop
+    from simuse.workloads import NetSpec
+    n, steps = args.model_neurons, args.model_steps
+    rng = np.random.default_rng(5)
+    cases = [
+        ("lif", lif_pop(n, 0.1, 0.05, -65.0, -50.0, -65.0, 2.0, rng.uniform(-65, -50, n).astype(np.float32)),
+         0.1, (0.0, 30.0)),
+        ("izhikevich", izh_pop(n, 1.0, 0.02, 0.2, -65.0, 8.0, 0.0, 30.0, -65.0, -13.0), 1.0, (0.0, 15.0)),
+        ("hh", hh_pop(n, 0.01, v0=rng.uniform(-70, -60, n).astype(np.float32)), 0.01, (0.0, 20.0)),
+        ("gpea", gpea_pop(n), 0.025, (0.0, 3.0 * GPEA_PULSE)),
+        ("alpha_lif", alpha_lif_pop(n), 0.5, (0.0, 200.0)),
+    ]
+    threads = host_threads()
+    out = {"neurons": n, "steps": steps, "path": "simuse_pop_* (pop_step_kernel, one thread per neuron)",
+           "note": "at the default 32M neurons the working sets (1.4-4.2 GB) are far above the MI355X's 256 MB "
+                   "Infinity Cache; at 8M it still serves part of the light models' bytes (LIF 0.92 of the HBM "
+                   "peak), at 1M nearly all (LIF 1.005)",
+           "legs": {}}
+    for name, pop, dt, (lo, hi) in cases:
+        cur = rng.uniform(lo, hi, n).astype(np.float32)
+        m = CompiledModel(pop.model, n, pop.stride)
+        m.upload(pop.states, pop.params, cur)
+        m.step_graph(dt, 20)
+        m.sync()
+        ms = m.time_steps(dt, steps)
+        spikes = int(m.download_spikes().sum())
+        m.close()
+        ns_rw, np_r = MODEL_COLUMNS[name]
+        bpn = 8 * ns_rw + 4 * np_r + 4 + 4
+        step_ms = ms / steps
+        achieved = bpn * n / (step_ms * 1e-3) / 1e9
+        # the oracle: the same population as a network without synapses with
+        # the current injected every step, about a second of host time
+        rp = np.zeros(n + 1, np.uint32)
+        spec = NetSpec(f"pop_{name}", [pop], rp, np.zeros(0, np.uint32), np.zeros(0, np.float32),
+                       np.zeros(0, np.uint8), dt, 0)
+        o = po.OracleNetwork(spec, 32, threads=threads)
+        idx = np.arange(n, dtype=np.uint32)
+        o.inject_current(0, idx, cur)
+        t1 = time.perf_counter()
+        o.step()
+        per = max(time.perf_counter() - t1, 1e-6)
+        cs = int(min(100, max(3, 1.0 / per)))
+        t1 = time.perf_counter()
+        for _ in range(cs):
+            o.inject_current(0, idx, cur)
+            o.step()
+        cpu = n * cs / (time.perf_counter() - t1)
+        o.close()
+        out["legs"][name] = {
+            "us_per_step": step_ms * 1e3, "neuron_steps_per_s": n / (step_ms * 1e-3), "spikes_last_step": spikes,
+            "roofline": {"bound": "hbm", "kernel": f"pop_step_kernel<{name}>", "achieved": achieved,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "algorithmic_bytes_per_neuron": bpn, "kernel_ms": step_ms,
+                         "kernel_ms_source": "HIP events around the timed launches, queued behind a spacer / steps"},
+            "cpu_baseline": {"value": cpu, "unit": "neuron-steps/s", "cores": threads, "kind": "port",
+                             "sample": f"{cs} steps of the same population (current injected each step) on the "
+                                       "oracle's FP32 path"}}
+        log(f"  model {name}: {step_ms * 1e3:.2f} us/step on the device, {bpn} B/neuron -> {achieved:.0f} GB/s "
+            f"({achieved / HBM_PEAK_GBS:.3f}); CPU {cpu:.3e} neuron-steps/s on {threads} threads")
+    return out
+
+
 def bench_pool(args) -> dict:
     """The game engine's creature pool (BrainPool): `pool_creatures` random
     brains of `pool_neurons` alpha-LIF neurons (connection probability
@@ -899,6 +979,12 @@ def main() -> int:
     ap.add_argument("--kura-k", type=int, default=20)
     ap.add_argument("--kura-steps", type=int, default=100)
     ap.add_argument("--no-pool", action="store_true", help="skip the creature-pool leg")
+    ap.add_argument("--no-models", action="store_true", help="skip the per-model population leg")
+    ap.add_argument("--only-models", action="store_true", help="run the per-model population leg only")
+    ap.add_argument("--model-neurons", type=int, default=32_000_000,
+                    help="population size of the per-model leg (32M: 1.4-4.2 GB working sets, far above the "
+                         "256 MB Infinity Cache, so the bytes come from HBM)")
+    ap.add_argument("--model-steps", type=int, default=50)
     ap.add_argument("--only-pool", action="store_true", help="run the creature-pool leg only (profiling)")
     ap.add_argument("--pool-creatures", type=int, default=64)
     ap.add_argument("--pool-neurons", type=int, default=1000)
@@ -932,6 +1018,9 @@ def main() -> int:
     if args.only_precision:
         print(json.dumps({"precision": bench_precision(args)}), flush=True)
         return 0
+    if args.only_models:
+        print(json.dumps({"models": bench_models(args)}), flush=True)
+        return 0
     if args.only_pool:
         print(json.dumps({"pool": bench_pool(args)}), flush=True)
         return 0
@@ -948,6 +1037,8 @@ def main() -> int:
             out["kuramoto"] = bench_kuramoto(args)
         if not args.no_pool:
             out["pool"] = bench_pool(args)
+        if not args.no_models:
+            out["models"] = bench_models(args)
         if not args.no_precision:
             out["precision"] = bench_precision(args)
     print(json.dumps(out), flush=True)

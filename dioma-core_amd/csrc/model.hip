@@ -43,9 +43,12 @@ struct simuse_pop {
     uint32_t* count = nullptr;
     uint32_t compacted = 0;  // count of the last compact_spikes
     PopDev P{};
+    hipEvent_t tev[2] = {nullptr, nullptr};  // simuse_pop_time_steps' pair
 
     ~simuse_pop() {
         if (stream) (void)hipStreamSynchronize(stream);
+        for (hipEvent_t e : tev)
+            if (e) (void)hipEventDestroy(e);
         for (void* p : {static_cast<void*>(st), static_cast<void*>(pr), static_cast<void*>(i_in),
                         static_cast<void*>(spikes), static_cast<void*>(bits), static_cast<void*>(ids),
                         static_cast<void*>(count)})
@@ -171,6 +174,28 @@ int simuse_pop_upload_i_in(simuse_pop* p, const float* i_in) {
 int simuse_pop_step_async(simuse_pop* p, float dt, uint32_t n_steps) {
     POP_CHECK(p);
     return p->steps(dt, n_steps);
+}
+
+int simuse_pop_time_steps(simuse_pop* p, float dt, uint32_t n_steps, float* ms) {
+    POP_CHECK(p);
+    if (!ms) return fail(SIMUSE_E_INVALID, "null output");
+    if (!p->tev[0]) {
+        HIP_TRY(hipEventCreate(&p->tev[0]));
+        HIP_TRY(hipEventCreate(&p->tev[1]));
+    }
+    // 1 ms of the constant-rate wall clock ahead of the first event: the host
+    // enqueues the steps while it runs, so the events bracket the kernels
+    int khz = 0, dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    launch_spacer(static_cast<uint64_t>(khz > 0 ? khz : 100000), p->stream);
+    HIP_TRY(hipEventRecord(p->tev[0], p->stream));
+    int rc = p->steps(dt, n_steps);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(p->tev[1], p->stream));
+    HIP_TRY(hipEventSynchronize(p->tev[1]));
+    HIP_TRY(hipEventElapsedTime(ms, p->tev[0], p->tev[1]));
+    return SIMUSE_OK;
 }
 
 int simuse_pop_sync(simuse_pop* p) {

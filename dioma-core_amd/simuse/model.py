@@ -22,6 +22,7 @@ _SIGS = {
     "simuse_pop_upload_i_in": ([P, f32p], C.c_int),
     "simuse_pop_step_async": ([P, C.c_float, C.c_uint32], C.c_int),
     "simuse_pop_sync": ([P], C.c_int),
+    "simuse_pop_time_steps": ([P, C.c_float, C.c_uint32, C.POINTER(C.c_float)], C.c_int),
     "simuse_pop_step_device": ([P, C.c_float], C.c_int),
     "simuse_pop_compact_spikes": ([P, u32p], C.c_int),
     "simuse_pop_download_compact": ([P, i32p, C.c_uint32], C.c_int),
@@ -91,6 +92,12 @@ class CompiledModel:
 
     def sync(self) -> None:
         check(_L().simuse_pop_sync(self._h))
+
+    def time_steps(self, dt: float, n_steps: int) -> float:
+        """Device milliseconds of n_steps steps (queued behind a spacer)."""
+        ms = C.c_float()
+        check(_L().simuse_pop_time_steps(self._h, float(dt), int(n_steps), C.byref(ms)))
+        return float(ms.value)
 
     def step_device(self, dt: float) -> None:
         check(_L().simuse_pop_step_device(self._h, float(dt)))

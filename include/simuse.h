@@ -378,6 +378,10 @@ int simuse_pop_upload_i_in(simuse_pop* pop, const float* i_in);
 int simuse_pop_step_async(simuse_pop* pop, float dt, uint32_t n_steps);
 /* sync (engine.rs:288-296). */
 int simuse_pop_sync(simuse_pop* pop);
+/* n_steps steps between two HIP events, queued behind a 1 ms device-side
+ * spacer so that the events bracket the step kernels only; *ms = elapsed.
+ * No reference counterpart (measurement helper). */
+int simuse_pop_time_steps(simuse_pop* pop, float dt, uint32_t n_steps, float* ms);
 /* step_device (engine.rs:238-251): one step, then sync. */
 int simuse_pop_step_device(simuse_pop* pop, float dt);
 /* compact_spikes (engine.rs:298-352): the last step's spiking neurons listed

@@ -534,20 +534,28 @@ def test_raster_statistics_match_oracle():
 
 
 def test_izhikevich_1m_full_size_bitexact():
-    # The benchmark network itself (1M neurons, 100M synapses): 12 steps on
-    # both sides, bit for bit.
+    # The benchmark network itself (1M neurons, 100M synapses): 12 single
+    # steps with the spike ids of every step, then 88 steps in captured
+    # graphs (22 per call), bit for bit on both sides: spike ids and counters
+    # after each call, then state and ring.
     say("izhikevich_1m: building")
-    spec = izhikevich_network(seed=1, steps=12)
+    spec = izhikevich_network(seed=1, steps=100)
     say("izhikevich_1m: device")
-    dev = build_device(spec)
+    dev = build_device(spec, graph_steps=11)
     ora = po.OracleNetwork(spec, 32, threads=8)
     for t in range(12):
         dev.step()
         ora.step()
         assert np.array_equal(dev.download_spike_ids(), ora.active()), t
+    for c in range(4):
+        dev.run(22)
+        ora.step(22)
+        assert np.array_equal(dev.download_spike_ids(), ora.active()), c
+        assert dev.counters() == ora.counters(), c
+    assert dev.step_counter() == 100
     assert ora.counters()[0] > 0
-    assert dev.counters() == ora.counters()
     assert_same_state(dev, ora, spec)
+    assert_same_ring(dev, ora)
     say("izhikevich_1m: ok", ora.counters())
 
 
