@@ -47,6 +47,7 @@ constexpr uint32_t kSlotsP = 32;                 // MAX_SLOTS, PW/src/network.rs
 constexpr uint32_t kMaxSubmit = SIMUSE_POOL_MAX_STEPS_PER_SUBMIT;  // MAX_HIST / GPU_MAX_STEPS_PER_SUBMIT
 constexpr uint32_t kSrcBits = 27;
 constexpr uint32_t kSrcMask = (1u << kSrcBits) - 1u;
+constexpr uint32_t kPullBatch = 8;  // in-edges whose loads a pull issues together
 
 struct PoolArgs {
     uint32_t n, words;
@@ -88,15 +89,37 @@ __global__ __launch_bounds__(kPoolTile) void pool_step_kernel(PoolArgs a, uint32
     const uint32_t e1 = a.rev_ptr[i + 1];
     float acc = 0.0f;
     uint32_t cb = kSlotsP;  // no bucket yet
-    for (uint32_t e = a.rev_ptr[i]; e < e1; ++e) {
-        const uint32_t x = a.rev[e];
-        const uint32_t src = x & kSrcMask, bk = x >> kSrcBits;
-        if (bk != cb) {
-            if (acc != 0.0f) a.ring[static_cast<size_t>((cur + cb) & (kSlotsP - 1)) * a.n + i] += acc;
-            acc = 0.0f;
-            cb = bk;
+    // kPullBatch in-edges at a time: their packed sources and signed weights
+    // load together, then the sources' spike words, then the adds in edge
+    // order -- two dependent loads per batch instead of up to three per edge
+    // (measured: 14.1 -> 11.0 us per substep on the 64 x 1000 pool; 16 per
+    // batch costs occupancy, 13.3 us).
+    for (uint32_t e0 = a.rev_ptr[i]; e0 < e1; e0 += kPullBatch) {
+        uint32_t x[kPullBatch];
+        float w[kPullBatch];
+        uint64_t word[kPullBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kPullBatch; ++j) {
+            x[j] = 0u;
+            w[j] = 0.0f;
+            if (e0 + j < e1) {
+                x[j] = a.rev[e0 + j];
+                w[j] = a.ws[e0 + j];
+            }
         }
-        if ((prev[src >> 6] >> (src & 63)) & 1ull) acc = acc + a.ws[e];
+#pragma unroll
+        for (uint32_t j = 0; j < kPullBatch; ++j) word[j] = e0 + j < e1 ? prev[(x[j] & kSrcMask) >> 6] : 0ull;
+#pragma unroll
+        for (uint32_t j = 0; j < kPullBatch; ++j) {
+            if (e0 + j >= e1) break;
+            const uint32_t src = x[j] & kSrcMask, bk = x[j] >> kSrcBits;
+            if (bk != cb) {
+                if (acc != 0.0f) a.ring[static_cast<size_t>((cur + cb) & (kSlotsP - 1)) * a.n + i] += acc;
+                acc = 0.0f;
+                cb = bk;
+            }
+            if ((word[j] >> (src & 63)) & 1ull) acc = acc + w[j];
+        }
     }
     if (acc != 0.0f) a.ring[static_cast<size_t>((cur + cb) & (kSlotsP - 1)) * a.n + i] += acc;
 }
