@@ -38,9 +38,11 @@ constexpr double kTwoPi = 6.283185307179586476925286766559;
 
 struct KuraArgs {
     uint32_t n;
+    uint32_t fixed_k;     // every row has this many edges (row i starts at i * fixed_k), or 0: read row_ptr
     const uint32_t* row_ptr;
     const uint32_t* col;
     const float* w;
+    const float* w_row;   // every edge of row i has weight w_row[i] (bit-equal), or null: read w per edge
     const float* omega;
     double* theta;        // FP64 master phase, wrapped to [-pi, pi)
     double* acc;          // RK4 accumulator k1 + 2k2 + 2k3 (FP64)
@@ -54,13 +56,20 @@ __device__ __forceinline__ double wrap_phase(double p) {
     return p - kTwoPi * round(p / kTwoPi);
 }
 
-// Gather sum_j w_ij (sin_j, cos_j) for row i with G lanes per row.
+// Gather sum_j w_ij (sin_j, cos_j) for row i with G lanes per row.  A graph
+// whose rows each carry one weight (the modular small world's K / k) reads
+// it once per row instead of once per edge, and one whose rows all have the
+// same degree computes the row's extent: the same products and sums in the
+// same order, 12 instead of 16 bytes per edge.
 template <int G>
 __device__ __forceinline__ float2 row_gather(const KuraArgs& a, uint32_t i, uint32_t sub) {
     float s = 0.0f, c = 0.0f;
-    const uint32_t r0 = a.row_ptr[i], r1 = a.row_ptr[i + 1];
+    const uint32_t r0 = a.fixed_k ? i * a.fixed_k : a.row_ptr[i];
+    const uint32_t r1 = a.fixed_k ? r0 + a.fixed_k : a.row_ptr[i + 1];
+    const bool row_w = a.w_row != nullptr;
+    const float w_i = row_w ? a.w_row[i] : 0.0f;
     for (uint32_t e = r0 + sub; e < r1; e += G) {
-        const float wv = a.w[e];
+        const float wv = row_w ? w_i : a.w[e];
         const float2 sc = a.sc_in[a.col[e]];
         s += wv * sc.x;
         c += wv * sc.y;
@@ -237,6 +246,8 @@ struct simuse_kura {
     std::vector<void*> allocs;
     uint32_t *row_ptr = nullptr, *col = nullptr;
     float *w = nullptr, *omega = nullptr;
+    float* w_row = nullptr;  // per-row weight when every row's weights are bit-equal
+    uint32_t fixed_k = 0;    // common out-degree, or 0
     double* acc = nullptr;
     double* theta = nullptr;
     float2* sc[2] = {nullptr, nullptr};
@@ -277,7 +288,8 @@ struct simuse_kura {
     }
     KuraArgs args(int in) const {
         KuraArgs a;
-        a.n = n; a.row_ptr = row_ptr; a.col = col; a.w = w; a.omega = omega; a.theta = theta; a.acc = acc;
+        a.n = n; a.fixed_k = fixed_k; a.row_ptr = row_ptr; a.col = col; a.w = w; a.w_row = w_row;
+        a.omega = omega; a.theta = theta; a.acc = acc;
         a.sc_in = sc[in]; a.sc_out = sc[in ^ 1]; a.dt = dt;
         return a;
     }
@@ -356,6 +368,23 @@ int simuse_kura_create(const uint32_t* row_ptr, const uint32_t* col, const float
         KHIP(hipMemcpy(k->col, col, 4 * ne, hipMemcpyHostToDevice));
         KHIP(hipMemcpy(k->w, w, 4 * ne, hipMemcpyHostToDevice));
     }
+    // rows of one weight each (bit-equal) and one common degree
+    bool uni_w = true, uni_k = true;
+    std::vector<float> wr(n, 0.0f);
+    const uint32_t k0 = row_ptr[1] - row_ptr[0];
+    for (uint32_t i = 0; i < n && (uni_w || uni_k); ++i) {
+        const uint32_t a = row_ptr[i], b = row_ptr[i + 1];
+        uni_k = uni_k && b - a == k0;
+        if (a < b) {
+            wr[i] = w[a];
+            for (uint32_t e = a + 1; e < b && uni_w; ++e) uni_w = std::memcmp(&w[e], &w[a], 4) == 0;
+        }
+    }
+    if (uni_k) k->fixed_k = k0;
+    if (uni_w && ne) {
+        if ((rc = k->alloc(&k->w_row, n))) return rc;
+        KHIP(hipMemcpy(k->w_row, wr.data(), 4ull * n, hipMemcpyHostToDevice));
+    }
     KHIP(hipMemcpy(k->omega, omega, 4ull * n, hipMemcpyHostToDevice));
     std::vector<double> th(n);
     for (uint32_t i = 0; i < n; ++i) th[i] = static_cast<double>(theta0[i]);
@@ -410,6 +439,15 @@ int simuse_kura_create_generated(const simuse_kura_gen_desc* g, int32_t method, 
     kura_gen_kernel<<<(n + 256) / 256, 256, 0, k->stream>>>(k->row_ptr, k->col, k->w, k->omega, th0, n, g->modules,
                                                              g->k, g->beta, g->inter, g->coupling, g->seed);
     KHIP(hipGetLastError());
+    // the generator gives every row k edges of weight coupling / k (the same
+    // float division as the kernel's)
+    k->fixed_k = g->k;
+    if ((rc = k->alloc(&k->w_row, n))) return rc;
+    {
+        const std::vector<float> wr(n, g->coupling / static_cast<float>(g->k));
+        KHIP(hipMemcpyAsync(k->w_row, wr.data(), 4ull * n, hipMemcpyHostToDevice, k->stream));
+        KHIP(hipStreamSynchronize(k->stream));
+    }
     std::vector<float> th(n);
     KHIP(hipMemcpyAsync(th.data(), th0, 4ull * n, hipMemcpyDeviceToHost, k->stream));
     KHIP(hipStreamSynchronize(k->stream));

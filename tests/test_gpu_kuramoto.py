@@ -70,3 +70,28 @@ def test_phase_coherence_weighted():
     assert abs(dev.phase_coherence(a) - want) < 1e-9
     assert abs(dev.phase_coherence(np.ones(n, np.float32)) - dev.order_parameter()) < 1e-12
     assert dev.phase_coherence(np.zeros(n, np.float32)) == 0.0
+
+
+@pytest.mark.parametrize("method", ["rk4", "euler"])
+def test_kuramoto_row_weight_path_bitexact(method):
+    # The modular small world's rows each carry one weight (K / k_i), so the
+    # device reads it once per row.  The same graph with one more oscillator
+    # that no one listens to, whose two edges have different weights, makes
+    # every row read its weights per edge (and its extent from row_ptr):
+    # the first n phases must stay bit for bit the same.
+    print("kuramoto row-weight path", method, flush=True)
+    n = 8192
+    rp, col, w, om, th = modular_small_world(n, modules=8, seed=6)
+    a = KuramotoGraph(rp, col, w, om, th, 0.01, method)
+    rp2 = np.concatenate([rp, [rp[-1] + 2]]).astype(np.uint32)
+    col2 = np.concatenate([col, [0, 1]]).astype(np.uint32)
+    w2 = np.concatenate([w, [0.25, 0.5]]).astype(np.float32)
+    om2 = np.concatenate([om, [0.3]]).astype(np.float32)
+    th2 = np.concatenate([th, [0.1]]).astype(np.float32)
+    b = KuramotoGraph(rp2, col2, w2, om2, th2, 0.01, method)
+    a.set_group(8)  # the same lanes per row, so the same summation order
+    b.set_group(8)
+    a.run(300)
+    b.run(300)
+    pa, pb = a.phases(), b.phases()[:n]
+    assert np.array_equal(pa.view(np.uint64), pb.view(np.uint64))
