@@ -530,21 +530,27 @@ struct Neuron<2> {  // Hodgkin-Huxley: states v, m, h, n, refr (, CaT gate); par
 // ---------------------------------------------------------------------------
 
 // Exclusive prefix sum over a block of kThreads threads; returns the total.
+// The wave totals (at most 16) are scanned once per wave in lanes 0..kW-1,
+// one DPP row, and the wave's base and the block total come back through
+// two readlanes: a handful of VALU instructions per wave instead of a
+// compare, select and two adds per wave total in every lane.
 template <uint32_t kThreads>
 __device__ __forceinline__ uint32_t block_scan(uint32_t x, uint32_t* s_wave, uint32_t& excl) {
     constexpr uint32_t kW = kThreads / 64;
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    uint32_t inc = x;
-    inc = wave_incl_scan(inc);
+    static_assert(kW >= 1 && kW <= 16, "the wave totals fit one DPP row");
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t inc = wave_incl_scan(x);
     if (lane == 63) s_wave[wave] = inc;
     __syncthreads();
-    uint32_t base = 0, total = 0;
-#pragma unroll
-    for (uint32_t w = 0; w < kW; ++w) {
-        const uint32_t v = s_wave[w];
-        if (w < wave) base += v;
-        total += v;
-    }
+    const int v0 = lane < kW ? static_cast<int>(s_wave[lane]) : 0;
+    int v = v0;
+    if (kW > 1) v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    if (kW > 2) v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    if (kW > 4) v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    if (kW > 8) v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    const uint32_t base = static_cast<uint32_t>(__builtin_amdgcn_readlane(v - v0, static_cast<int>(wave)));
+    const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(v, static_cast<int>(kW - 1)));
     excl = base + inc - x;
     __syncthreads();
     return total;
