@@ -1441,7 +1441,8 @@ struct BinApplyLds {
     uint32_t seg_base[kMaxWriters];
     uint32_t seg_pref[kMaxWriters + 1];
     uint32_t wave[kApplyWaves];
-    uint32_t m, next;
+    uint16_t mq[kBinCap / 2];  // cells of the pass with two or more records (each holds two of <= kBinCap)
+    uint32_t m, next, nm;
     __device__ uint32_t* r_eid() { return reinterpret_cast<uint32_t*>(u); }
     __device__ uint16_t* nxt() { return reinterpret_cast<uint16_t*>(u + kBinCap / 2); }
     __device__ uint16_t* r_post() { return reinterpret_cast<uint16_t*>(u + 3 * kBinCap / 4); }
@@ -1579,15 +1580,48 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m, RecCtx rc) {
     const uint32_t* r_eid = L.r_eid();
     const uint16_t* nxt = L.nxt();
     int longs = 0;
+    if (!kRec) {
+        // A list of one record (most cells that receive anything) is added
+        // by its cell's thread; cells with more are queued (one LDS atomic
+        // per wave) and summed by the first threads of the block, so that
+        // the insertion runs in a few waves instead of in every one.
+        const uint32_t lane = t & 63u;
 #pragma unroll
-    for (uint32_t pr = 0; pr < kBinPosts / kApply; ++pr) {
+        for (uint32_t pr = 0; pr < kBinPosts / kApply; ++pr) {
+            const uint32_t p = t + pr * kApply;
+            const uint32_t h = L.head[p];
+            bool multi = false;
+            if (h != kListNil) {
+                if (nxt[h] == kLinkNil) {
+                    L.head[p] = kListNil;
+                    L.slice[p] = L.slice[p] + L.r_w[h];
+                } else {
+                    multi = true;
+                }
+            }
+            const uint64_t mb = __ballot(multi);
+            if (mb) {
+                const uint32_t lead = static_cast<uint32_t>(__builtin_ctzll(mb));
+                uint32_t base = 0u;
+                if (lane == lead) base = atomicAdd(&L.nm, static_cast<uint32_t>(__popcll(mb)));
+                base = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(base), static_cast<int>(lead)));
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mb >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mb), 0u));
+                if (multi) L.mq[base + rank] = static_cast<uint16_t>(p);
+            }
+        }
+        __syncthreads();
+        const uint32_t nm = L.nm;
+        for (uint32_t i = t; i < nm; i += kApply) {
+            const uint32_t p = L.mq[i];
+            if (short_list_sum(L, p, L.head[p])) longs = 1;
+        }
+    }
+#pragma unroll
+    for (uint32_t pr = 0; kRec && pr < kBinPosts / kApply; ++pr) {
         const uint32_t p = t + pr * kApply;
         const uint32_t h = L.head[p];
         if (h == kListNil) continue;
-        if (!kRec) {
-            if (short_list_sum(L, p, h)) longs = 1;
-            continue;
-        }
         if (p < rc.c0 || p >= rc.c1) {  // another block's cell: its conductances are not touched here
             L.head[p] = kListNil;
             continue;
@@ -1652,7 +1686,9 @@ __device__ void bin_accumulate(BinApplyLds& L, uint32_t m, RecCtx rc) {
         }
         L.slice[p] = acc;
     }
-    if (!__syncthreads_or(longs)) return;
+    const int any_long = __syncthreads_or(longs);
+    if (!kRec && t == 0) L.nm = 0u;  // every thread has read it
+    if (!any_long) return;
     // the records of long lists, compacted as keys (cell << 43 | eid << 11 |
     // record; receptor networks: cell << 46 | eid << 14 | code << 11 |
     // record, the keys overwrite the records' codes), then sorted
@@ -1765,6 +1801,7 @@ __device__ __forceinline__ void bin_gather(BinApplyLds& L, uint2* hrow2, uint4 h
     const uint32_t t = threadIdx.x;
     const bool one_w = nw <= kApply;  // thread t holds writer t only
     const uint32_t w0 = one_w ? t : 2 * t;
+    if (t == 0) L.nm = 0u;  // the accumulate's queue (read after the scan's barriers)
     if ((hw.y | hw.w) && zero_row) {  // consumed: empty for the next use
         if (one_w) hrow2[t] = make_uint2(0u, 0u);
         else reinterpret_cast<uint4*>(hrow2)[t] = make_uint4(0u, 0u, 0u, 0u);
