@@ -140,6 +140,7 @@ struct NetState {
     bool split_ok = false;
     bool split = false;     // running it (set with the work lists)
     bool no_split = false;  // left for the three-kernel path (dense STDP step)
+    bool hdr_dense = false;  // the header matrices hold whole columns (one-kernel or split form last)
     uint32_t nbuf() const { return mega_ok || split_ok ? 2u : 1u; }  // header and record buffers
     uint32_t* bin_done = nullptr;
     uint32_t* bin_mat = nullptr;
@@ -527,6 +528,16 @@ struct Engine {
             const bool fz = s->binned && !s->stdp && !s->rec_code && ng + ns <= kMaxWriters;
             // megastep: bin b's block is writer b, the source blocks follow
             s->mega = s->binned && s->mega_ok && !s->no_mega && !s->stdp && one_kernel;
+            s->split = s->binned && s->split_ok && !s->no_split && !s->mega;
+            // The one-kernel and split forms fill whole header columns and
+            // leave them as read; the binned form writes only the entries of
+            // bins with events and reads every other entry as empty, so a
+            // network that moves to it starts from cleared headers (its
+            // pending events were applied to the ring before the move).
+            const bool dense = s->mega || s->split;
+            if (s->hdr_dense && !dense && s->bin_hdr)
+                HIP_TRY(hipMemsetAsync(s->bin_hdr, 0, sizeof(uint2) * s->nbuf() * s->n_bins * s->bin_rows, stream));
+            s->hdr_dense = dense;
             if (s->mega) {
                 s->mat_items.clear();
                 for (uint32_t b = 0; b < s->n_bins; ++b) {
@@ -549,7 +560,6 @@ struct Engine {
             // split form: front blocks (kFrontParts per bin), then bin_kernel's
             // writers over every scatter tile and the in-edge LTP blocks, all
             // counted for the step's advance
-            s->split = s->binned && s->split_ok && !s->no_split && !s->mega;
             if (s->split) {
                 s->mat_items.clear();
                 const int fi = 2 * s->mega_model + (s->rec_code ? 1 : 0);

@@ -934,9 +934,10 @@ __device__ __forceinline__ void ev_cache_write(const NetArgs& a, bool is_src, co
 // Writer j's per-bin counts (s_bin) become the absolute first record of each
 // bin's group, from `base`; column j of `hdr` gets {start, count} for each
 // bin it sends events to (consumers zero the entries they read, so the
-// others read as empty), or, in the split form, for every bin (its several
-// consumer blocks per bin leave the row as read).  Synchronises before and
-// after.
+// others read as empty), or, in the one-kernel and split forms, for every
+// bin (their consumers leave the row as read: one write per entry and step
+// instead of a write and a clearing write for most).  Synchronises before
+// and after.
 template <uint32_t kThreads>
 __device__ __forceinline__ void bin_offsets(const NetArgs& a, uint32_t* s_bin, uint32_t* s_wave, uint32_t j,
                                             uint32_t base, uint32_t events, uint2* hdr) {
@@ -948,7 +949,7 @@ __device__ __forceinline__ void bin_offsets(const NetArgs& a, uint32_t* s_bin, u
         (void)block_scan<kThreads>(c, s_wave, excl);
         if (t < nb) {
             s_bin[t] = base + excl;
-            if (c || a.split) hdr[static_cast<size_t>(t) * a.bin_rows + j] = make_uint2(base + excl, c);
+            if (c || a.mega) hdr[static_cast<size_t>(t) * a.bin_rows + j] = make_uint2(base + excl, c);
         }
         __syncthreads();
         return;
@@ -957,7 +958,7 @@ __device__ __forceinline__ void bin_offsets(const NetArgs& a, uint32_t* s_bin, u
     else lds_scan<kThreads, (kMaxBins + kThreads - 1) / kThreads>(s_bin, nb, s_wave);
     for (uint32_t k = t; k < nb; k += kThreads) {
         const uint32_t o = s_bin[k], e = k + 1 < nb ? s_bin[k + 1] : events;
-        if (e != o || a.split) hdr[static_cast<size_t>(k) * a.bin_rows + j] = make_uint2(base + o, e - o);
+        if (e != o || a.mega) hdr[static_cast<size_t>(k) * a.bin_rows + j] = make_uint2(base + o, e - o);
     }
     __syncthreads();
     for (uint32_t k = t; k < nb; k += kThreads) s_bin[k] += base;
@@ -2094,7 +2095,9 @@ __global__ __launch_bounds__(kApply, kMegaWaves<MODEL>) void megastep_kernel(con
         }
         const uint32_t rpar = wpar ^ 1u;
         uint2* hrow = a.bin_hdr + rpar * a.hdr_stride + static_cast<size_t>(k) * a.bin_rows;
-        bin_gather<false>(L, hrow, bin_header_load(hrow, a.n_bblk), ev_buf(a, rpar), a.n_bblk, RecCtx{nullptr, 0u});
+        // every writer fills its whole column each step: the row is left as read
+        bin_gather<false>(L, hrow, bin_header_load(hrow, a.n_bblk), ev_buf(a, rpar), a.n_bblk, RecCtx{nullptr, 0u},
+                          false);
 #pragma unroll
         for (uint32_t q = 0; q < 2; ++q) cur[q] = L.slice[t + q * kApply];
     }

@@ -132,9 +132,10 @@ struct NetArgs {
     uint32_t n_bins;
     uint32_t n_bblk;            // writers
     int32_t bin_fused;          // the neuron kernel's blocks are the neuron-tile writers
-    int32_t mega;               // parity buffers; the last of n_done blocks moves the step on
-    int32_t split;              // split form (front_kernel + bin_kernel): writers write every header
-                                // entry of their column, consumers leave the rows as read
+    int32_t mega;               // one-kernel or split form: parity buffers; writers write every
+                                // header entry of their column, consumers leave the rows as read;
+                                // the last of n_done blocks moves the step on
+    int32_t split;              // split form (front_kernel + bin_kernel)
     uint32_t n_done;            // blocks of a step counted by mega_done
     uint32_t bin_rows;          // row length of bin_hdr (even, >= n_bblk)
     uint32_t n_pops;
