@@ -6,7 +6,7 @@
 // and its shaders (PW/shaders/csr_scatter.wgsl).  DE/ = crates/dioma-engine/,
 // PW/ = crates/SiMuse/crates/pillars-wgpu/.
 //
-// One substep is ONE kernel, one thread per neuron (post):
+// Every substep does, one thread per neuron (post):
 //   1. at the first substep of a submit, v += ext (v_jump_kernel);
 //   2. I = ring[cur][post] (copy_slot_kernel), alpha-LIF update, spike flag
 //      written to the history row and, by wave ballot, to this step's bit
@@ -15,16 +15,22 @@
 //      PREVIOUS step's spike bits: per delay bucket b, acc = fma(w, sign, acc)
 //      in ascending source order, ring[(cur + b) % 32][post] += acc when
 //      acc != 0 (scatter_kernel).
-// Step 3 reads only the previous kernel's bit plane, and every ring cell is
+// Step 3 reads only the previous substep's bit plane, and every ring cell is
 // touched by its own post's thread, so the result is independent of the
-// schedule and bit-identical to the reference's sequence of dispatches.  The
+// schedule and bit-identical to the reference's sequence of dispatches.
+// When every creature fits one workgroup (at most 1024 neurons), a submit
+// chunk of up to 64 substeps is ONE launch, one block per creature, with the
+// substeps separated by block barriers (pool_tick_kernel); otherwise each
+// substep is one launch over the whole pool (pool_step_kernel).  The
 // signed weight w*sign is formed once per weight upload (exact: sign = +-1),
 // so fma(w, sign, acc) = acc + w*sign is a single rounding either way.
 //
 // HBM layout: in-edges per post sorted by (delay bucket, source), packed as
 // source | bucket << 27 (4 B), the signed weight in the same order (4 B);
 // ring [32][n] fp32; state v, g, refr_cnt [n] each; spike bits [2][n/64]
-// (by step parity); history [64][n] u32 (one row per substep of a submit).
+// (by step parity; one-launch chunks keep each creature's last plane in its
+// own bits instead, [creatures][16] u64); history [64][n] u32 (one row per
+// substep of a submit).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -124,6 +130,151 @@ __global__ __launch_bounds__(kPoolTile) void pool_step_kernel(PoolArgs a, uint32
     if (acc != 0.0f) a.ring[static_cast<size_t>((cur + cb) & (kSlotsP - 1)) * a.n + i] += acc;
 }
 
+// One submit chunk of a pool whose creatures all fit one workgroup: block c
+// runs creature c's `chunk` substeps in one launch.  The creatures are
+// block-diagonal (every in-edge of a creature's neuron comes from the same
+// creature), so a substep needs only the block's own previous spike plane,
+// and substeps are separated by a block barrier instead of a kernel
+// boundary.  Thread t is the creature's neuron t: v, g, refr_cnt in
+// registers for the whole chunk, its column of the 32-slot ring in LDS
+// (each ring cell is touched by its own post only), its first kEdgeRegs
+// in-edges (local source | bucket, signed weight) in registers; the spike
+// planes (two, by step parity) of the creature's local bits in LDS.  Every
+// neuron does pool_step_kernel's arithmetic in pool_step_kernel's order, so
+// the results are the same bits.  The ring, the state and the last plane go
+// back to memory at the end; `lbits` carries the last plane to the next
+// launch.
+constexpr uint32_t kCreatureMax = 1024;           // neurons of a creature the one-launch chunk takes
+constexpr uint32_t kCreatureWords = kCreatureMax / 64;
+constexpr uint32_t kEdgeRegs = 32;                // in-edges per neuron held in registers
+constexpr uint32_t kPlaneWords = 64;              // u32 per plane: 32 of spike bits, word 32 stays zero
+
+// A register-held in-edge: bits 0-4 the source's bit in its 32-bit plane
+// word, 5-9 the delay bucket, 16-31 the byte offset of that word in a plane.
+// Unused slots point at the plane's zero word with the last edge's bucket.
+__device__ __forceinline__ uint32_t tick_edge(uint32_t src, uint32_t bk) {
+    return (src & 31u) | (bk << 5) | (((src >> 5) * 4u) << 16);
+}
+
+struct TickLds {
+    uint64_t plane[2][kPlaneWords / 2];  // by step parity, the creature's local bits
+    float ring[kSlotsP * kCreatureMax];  // [slot][neuron]
+};
+
+template <bool kOneBucket>
+__global__ __launch_bounds__(kCreatureMax, 1) void pool_tick_kernel(PoolArgs a, const uint32_t* __restrict__ cre_off,
+                                                                    uint64_t* __restrict__ lbits, uint32_t chunk,
+                                                                    uint32_t bucket0) {
+    __shared__ TickLds L;
+    const uint32_t c = blockIdx.x, t = threadIdx.x, lane = t & 63u;
+    const uint32_t off = cre_off[c], n = cre_off[c + 1] - off;
+    const uint32_t nw = (n + 63) / 64;
+    const bool live = t < n;
+    const uint32_t i = off + t;
+    const uint64_t s0 = *a.step;
+#pragma unroll 4
+    for (uint32_t q = 0; q < kSlotsP; ++q)
+        L.ring[q * kCreatureMax + t] = live ? a.ring[static_cast<size_t>(q) * a.n + i] : 0.0f;
+    if (t < kPlaneWords / 2) {
+        // the plane of substep s0 - 1 (read by substep s0's pull) from the last launch; the rest zero
+        const uint32_t pp = static_cast<uint32_t>((s0 - 1) & 1u);
+        L.plane[pp][t] = t < nw ? lbits[static_cast<size_t>(c) * kCreatureWords + t] : 0ull;
+        L.plane[pp ^ 1u][t] = 0ull;
+    }
+    float v = 0.0f, g = 0.0f, r = 0.0f;
+    uint32_t e0 = 0u, e1 = 0u;
+    if (live) {
+        v = a.v[i];
+        g = a.g[i];
+        r = a.refr[i];
+        e0 = a.rev_ptr[i];
+        e1 = a.rev_ptr[i + 1];
+    }
+    const uint32_t nreg = min(e1 - e0, kEdgeRegs);
+    uint32_t xs[kEdgeRegs];
+    float wv[kEdgeRegs];
+    uint32_t lastb = bucket0;
+#pragma unroll
+    for (uint32_t j = 0; j < kEdgeRegs; ++j) {
+        wv[j] = 0.0f;
+        if (j < nreg) {
+            const uint32_t x = a.rev[e0 + j];
+            lastb = x >> kSrcBits;
+            xs[j] = tick_edge((x & kSrcMask) - off, lastb);
+            wv[j] = a.ws[e0 + j];
+        } else {
+            xs[j] = tick_edge(32u * 32u, lastb);  // the zero word: never a spike
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = 0; k < chunk; ++k) {
+        const uint64_t s = s0 + k;
+        const uint32_t cur = static_cast<uint32_t>(s & (kSlotsP - 1));
+        const uint32_t par = static_cast<uint32_t>(s & 1);
+        float* ring_t = L.ring + t;  // slot q of this neuron: ring_t[q * kCreatureMax]
+        // the packed in-edges are opaque to the loop, so that the compiler
+        // does not hoist their unpacked fields (several registers per edge)
+        // out of it and spill them
+#pragma unroll
+        for (uint32_t j = 0; j < kEdgeRegs; ++j) asm volatile("" : "+v"(xs[j]));
+        int spike = 0;
+        if (live) {
+            if (k == 0) v = v + a.ext[i];
+            const float I = ring_t[cur * kCreatureMax];
+            spike = simuse::alpha_lif_update(v, g, r, I, a.tf, a.v_rest, a.v_th, a.v_reset, a.refr_steps,
+                                             a.syn_decay);
+            a.hist[static_cast<size_t>(k) * a.n + i] = static_cast<uint32_t>(spike);
+        }
+        const uint64_t b = __ballot(spike);
+        if (lane == 0) L.plane[par][t >> 6] = b;
+        if (live) {
+            ring_t[cur * kCreatureMax] = 0.0f;
+            const char* prev = reinterpret_cast<const char*>(L.plane[par ^ 1u]);
+            float acc = 0.0f;
+            uint32_t cb = kOneBucket ? bucket0 : kSlotsP;  // kSlotsP: no bucket yet
+#pragma unroll
+            for (uint32_t j = 0; j < kEdgeRegs; ++j) {
+                const uint32_t word = *reinterpret_cast<const uint32_t*>(prev + (xs[j] >> 16));
+                if (!kOneBucket) {
+                    const uint32_t bk = (xs[j] >> 5) & 31u;
+                    if (bk != cb) {
+                        if (acc != 0.0f) ring_t[((cur + cb) & (kSlotsP - 1)) * kCreatureMax] += acc;
+                        acc = 0.0f;
+                        cb = bk;
+                    }
+                }
+                // bit field extract at offset xs[j] (the instruction reads the offset's low 5 bits)
+                if (__builtin_amdgcn_ubfe(word, xs[j], 1u)) acc = acc + wv[j];
+            }
+            // in-edges beyond the registers' share, from memory in the same order
+            const uint32_t* prev32 = reinterpret_cast<const uint32_t*>(prev);
+            for (uint32_t e = e0 + kEdgeRegs; e < e1; ++e) {
+                const uint32_t x = a.rev[e];
+                const float w = a.ws[e];
+                const uint32_t src = (x & kSrcMask) - off, bk = x >> kSrcBits;
+                if (!kOneBucket && bk != cb) {
+                    if (acc != 0.0f) ring_t[((cur + cb) & (kSlotsP - 1)) * kCreatureMax] += acc;
+                    acc = 0.0f;
+                    cb = bk;
+                }
+                if ((prev32[src >> 5] >> (src & 31u)) & 1u) acc = acc + w;
+            }
+            if (acc != 0.0f) ring_t[((cur + cb) & (kSlotsP - 1)) * kCreatureMax] += acc;
+        }
+        __syncthreads();  // the plane just read is the next substep's to write
+    }
+    if (live) {
+        a.v[i] = v;
+        a.g[i] = g;
+        a.refr[i] = r;
+    }
+#pragma unroll 4
+    for (uint32_t q = 0; q < kSlotsP; ++q)
+        if (live) a.ring[static_cast<size_t>(q) * a.n + i] = L.ring[q * kCreatureMax + t];
+    if (chunk && t < nw)
+        lbits[static_cast<size_t>(c) * kCreatureWords + t] = L.plane[static_cast<uint32_t>((s0 + chunk - 1) & 1u)][t];
+}
+
 __global__ void pool_advance_kernel(uint64_t* step, uint32_t n) { *step += n; }
 
 // ws[e] = w[rev_edge[e]] * sign[source]: exact, sign is +-1.
@@ -168,6 +319,14 @@ struct simuse_pool {
     float *v = nullptr, *g = nullptr, *refr = nullptr;
     uint64_t *bits = nullptr, *d_step = nullptr;
     uint32_t words = 0;
+    // every creature within one workgroup: a submit chunk is one launch
+    // (pool_tick_kernel) over the creatures' offsets, with each creature's
+    // last spike plane in lbits; otherwise one launch per substep
+    bool one_launch = false;
+    bool one_bucket = false;  // every in-edge in delay bucket bucket0 (a pool-wide delay)
+    uint32_t bucket0 = 0;
+    uint32_t *cre_off = nullptr;
+    uint64_t* lbits = nullptr;
     uint64_t steps = 0;
     float refr_steps = 0, tf = 0, syn_decay = 0;
     std::vector<float> ext_full;
@@ -275,13 +434,25 @@ struct simuse_pool {
             }
         }
         words = (n + 63) / 64;
+        one_launch = true;
+        for (const simuse_pool_slot& sl : slots) one_launch = one_launch && sl.n <= kCreatureMax;
+        one_bucket = true;
+        bucket0 = ns ? bucket(0) : 0u;
+        for (uint32_t e = 1; e < ns && one_bucket; ++e) one_bucket = bucket(e) == bucket0;
+        const uint32_t nc = static_cast<uint32_t>(slots.size());
+        std::vector<uint32_t> h_off(static_cast<size_t>(nc) + 1, 0);
+        for (uint32_t k = 0; k < nc; ++k) h_off[k + 1] = h_off[k] + slots[k].n;
         int rc;
         if ((rc = alloc(&rev_ptr, static_cast<size_t>(n) + 1)) || (rc = alloc(&rev, ns)) ||
             (rc = alloc(&rev_edge, ns)) || (rc = alloc(&w, ns)) || (rc = alloc(&ws, ns)) || (rc = alloc(&sign, n)) ||
             (rc = alloc(&ring, static_cast<size_t>(kSlotsP) * n)) || (rc = alloc(&ext, n)) || (rc = alloc(&v, n)) ||
             (rc = alloc(&g, n)) || (rc = alloc(&refr, n)) || (rc = alloc(&bits, 2 * static_cast<size_t>(words))) ||
-            (rc = alloc(&hist, static_cast<size_t>(kMaxSubmit) * n)) || (rc = alloc(&d_step, 1)))
+            (rc = alloc(&hist, static_cast<size_t>(kMaxSubmit) * n)) || (rc = alloc(&d_step, 1)) ||
+            (rc = alloc(&cre_off, static_cast<size_t>(nc) + 1)) ||
+            (rc = alloc(&lbits, static_cast<size_t>(nc) * kCreatureWords)))
             return rc;
+        PHIP(hipMemcpyAsync(cre_off, h_off.data(), 4 * h_off.size(), hipMemcpyHostToDevice, stream));
+        PHIP(hipMemsetAsync(lbits, 0, 8 * static_cast<size_t>(nc) * kCreatureWords, stream));
         PHIP(hipMemcpyAsync(rev_ptr, h_ptr.data(), 4 * h_ptr.size(), hipMemcpyHostToDevice, stream));
         if (ns) {
             PHIP(hipMemcpyAsync(rev, h_rev.data(), 4 * static_cast<size_t>(ns), hipMemcpyHostToDevice, stream));
@@ -301,8 +472,17 @@ struct simuse_pool {
 
     void enqueue(uint32_t chunk) {
         const PoolArgs a = args();
-        const uint32_t blocks = (n + kPoolTile - 1) / kPoolTile;
-        for (uint32_t k = 0; k < chunk; ++k) pool_step_kernel<<<blocks, kPoolTile, 0, stream>>>(a, k);
+        if (one_launch) {
+            if (one_bucket)
+                pool_tick_kernel<true><<<static_cast<uint32_t>(slots.size()), kCreatureMax, 0, stream>>>(
+                    a, cre_off, lbits, chunk, bucket0);
+            else
+                pool_tick_kernel<false><<<static_cast<uint32_t>(slots.size()), kCreatureMax, 0, stream>>>(
+                    a, cre_off, lbits, chunk, bucket0);
+        } else {
+            const uint32_t blocks = (n + kPoolTile - 1) / kPoolTile;
+            for (uint32_t k = 0; k < chunk; ++k) pool_step_kernel<<<blocks, kPoolTile, 0, stream>>>(a, k);
+        }
         pool_advance_kernel<<<1, 1, 0, stream>>>(d_step, chunk);
     }
     int run_chunk(uint32_t chunk) {

@@ -98,6 +98,30 @@ def test_pool_per_synapse_delays_and_overrides_match_oracle():
         dev.close()
 
 
+def test_pool_creature_sizes_at_and_above_one_workgroup_match_oracle():
+    """Creatures of at most 1024 neurons run a submit chunk as one launch (one
+    block per creature, substeps separated by block barriers); a creature
+    above 1024 sends the pool to one launch per substep.  Both against the
+    oracle, bit for bit, with in-degrees above the 32 in-edges a neuron holds
+    in registers."""
+    rng = np.random.default_rng(5)
+    for sizes in ((1024, 64, 3), (1500, 230, 700)):
+        print(f"pool vs oracle: creatures {sizes}", flush=True)
+        cs = [creature(n, p=0.04, seed=400 + k) for k, n in enumerate(sizes)]
+        dev, ora = BrainPool(), po.OraclePool()
+        for c in cs:
+            _register((dev, ora), c)
+        total = 0
+        for t in range(6):
+            exts = [rng.uniform(0.0, 9.0, c.n).astype(np.float32) for c in cs]
+            flat, _ = dev.step(exts, 70)
+            assert np.array_equal(flat, ora.step(exts, 70)), (sizes, t)
+            total += int(flat.sum())
+        _same_state(dev, ora)
+        assert total > 1000, sizes
+        dev.close()
+
+
 def test_pool_equals_engine_push_scatter():
     """The engine's deterministic push scatter (chained form: delay 6) and the
     pool's pull scatter give the same spikes and state on the same pool."""
