@@ -911,14 +911,18 @@ def bench_pool(args) -> dict:
             "ticks_per_s": args.pool_ticks / wall, "readback_bytes_per_tick": spt * n_total * 4,
             "device_us_per_substep": us_sub, "neuron_substeps_per_s_device": n_total / (us_sub * 1e-6),
             "spikes_per_substep": spikes / subs, "first_tick_equals_oracle": same,
-            "roofline": {"bound": "hbm", "kernel": "pool_step_kernel (update + pull scatter, one per substep)",
+            "roofline": {"bound": "hbm", "kernel": "pool_tick_kernel (one launch per submit chunk of up to 64 "
+                                                   "substeps, one workgroup per creature: update + pull scatter "
+                                                   "per substep, substeps separated by block barriers)",
                          "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                          "algorithmic_bytes_per_substep": per_sub,
                          "algorithmic_bytes": "40 B per neuron (v, g, refr_cnt read and written, slot read and "
                                               "zeroed, history, row pointer), 4 B per in-edge (source and bucket), "
                                               "4 B per in-edge whose source spiked (signed weight); the ring adds "
-                                              "left out (lower bound)",
+                                              "left out (lower bound). The one-launch chunk keeps state, ring and "
+                                              "in-edges on chip between substeps, so these are the bytes a substep's "
+                                              "work reads and writes, not HBM traffic",
                          "kernel_ms_per_substep": dev_ms / subs},
             "cpu_baseline": {"value": n_total * spt / cpu_s, "unit": "neuron-substeps/s", "cores": 1,
                              "kind": "port", "sample": f"oracle pool (restated per-post pull), first tick, {spt} "
