@@ -747,6 +747,15 @@ def bench_kuramoto(args) -> dict:
     g.close()
     ab = kura_algorithmic_bytes(g.n, g.n_edges, row_weights=True)  # K / k on every edge of a row, k per row
     achieved = ab["per_step"] / (step_ms * 1e-3) / 1e9
+    # L2-miss bytes of the same workload's stage launches (FETCH_SIZE and
+    # WRITE_SIZE passes, copy-calibrated), per RK4 step
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "traffic_kuramoto_1m_20m.json")
+    if g.n == 1_000_000 and args.kura_k == 20 and os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get("bytes_per_step")
+        except (OSError, ValueError):
+            traffic = None
     log(f"  {step_ms * 1e3:.1f} us/step (HIP events, 4 stages), {wall_us:.1f} us/step wall, "
         f"{achieved:.0f} GB/s algorithmic, R {r:.4f}")
     return {"workload": f"Kuramoto RK4, {g.n} oscillators, {g.n_edges} edges (64 modules, k {args.kura_k}, "
@@ -755,7 +764,7 @@ def bench_kuramoto(args) -> dict:
             "oscillator_steps_per_s": g.n * 1e6 / wall_us,
             "roofline": {"bound": "hbm", "kernel": "kura_stage_kernel (4 launches per RK4 step)",
                          "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_per": "RK4 step (4 launches)",
                          "algorithmic_bytes_per_step": ab["per_step"],
                          "algorithmic_bytes_per_edge_per_stage": ab["per_edge_per_stage"],
                          "algorithmic_bytes_per_node_per_step": ab["per_node_per_step"],
