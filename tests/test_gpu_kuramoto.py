@@ -95,3 +95,31 @@ def test_kuramoto_row_weight_path_bitexact(method):
     b.run(300)
     pa, pb = a.phases(), b.phases()[:n]
     assert np.array_equal(pa.view(np.uint64), pb.view(np.uint64))
+
+
+@pytest.mark.parametrize("method,lanes", [("rk4", None), ("rk4", 1), ("rk4", 2), ("euler", 4)])
+def test_kuramoto_ragged_rows_match_fp64_oracle(method, lanes):
+    """Rows of degree 0 to 70 with a weight per edge: empty rows, rows shorter
+    than one batch of a lane's edges and rows of several batches."""
+    print("kuramoto ragged", method, lanes, flush=True)
+    rng = np.random.default_rng(11)
+    n = 3000
+    deg = rng.integers(0, 71, size=n)
+    deg[:50] = 0
+    rp = np.zeros(n + 1, np.uint32)
+    rp[1:] = np.cumsum(deg)
+    col = rng.integers(0, n, size=int(rp[-1])).astype(np.uint32)
+    w = (rng.random(col.size) * 0.2 - 0.05).astype(np.float32)
+    om = rng.normal(0.0, 1.0, n).astype(np.float32)
+    th = rng.uniform(-np.pi, np.pi, n).astype(np.float32)
+    dt, steps = 0.001, 400
+    dev = KuramotoGraph(rp, col, w, om, th, dt, method)
+    if lanes:
+        dev.set_group(lanes)
+    ora = po.OracleKuramoto(rp, col, w, om, th, dt, kind=64)
+    dev.run(steps)
+    (ora.rk4 if method == "rk4" else ora.euler)(steps)
+    err = wrapped_err(dev.phases(), ora.phases())
+    print("max phase error", err, flush=True)
+    assert err < 1e-4
+    assert dev.step_counter() == steps
